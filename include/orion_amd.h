@@ -1,0 +1,181 @@
+/* orion_amd: MI355X-native (gfx950) path-tracing + SPPM hot path for orion.
+ *
+ * Plain C ABI: opaque handles, plain pointers and sizes, int status
+ * (0 = ok, < 0 = error; orion_last_error() returns the message of the last
+ * failure on the calling thread).  No exceptions cross this boundary.
+ *
+ * Where each entry point sits in orion (paths relative to the reference src/):
+ *   orion_hscene_load_ori     replaces Parser::Parser + makeRenderOption
+ *                             (util/parser.cpp:45-56, 463-471)
+ *   orion_scene_upload        replaces Scene::Scene -> EmbreeKernel::build
+ *                             (core/scene.cpp:5-15, core/kernel.cpp:63-110)
+ *   orion_intersect_closest   replaces EmbreeKernel::intersect -> rtcIntersect
+ *                             (core/kernel.cpp:111-174), batched
+ *   orion_intersect_any       replaces ShadowTester::unoccluded
+ *                             (core/light.h:26-29), batched
+ *   orion_render_path         replaces PathTracing::render (integrator/path.cpp:15-37)
+ *   orion_render_sppm         replaces SPPM::render (integrator/sppm.cpp:58-322)
+ *   orion_film_read           replaces RenderTarget::getAveraged's inputs
+ *                             (texture/rendertarget.cpp:22-37): sum + count
+ *   orion_batch_* / orion_server_*  no counterpart in orion (batched launch and
+ *                             continuous-batching serving of many render jobs)
+ *
+ * Resolved scene records (float32), shared by every consumer of a scene:
+ *   triangle  18 floats  p0 p1 p2 n0 n1 n2, world space (Mesh de-indexed,
+ *                        core/triangle.h:16-25, after createTriangleMesh)
+ *   tri_mat   material index per triangle; tri_light area-light index or -1
+ *   material  16 floats  [0] type: 0 matte, 1 metal, 2 glass
+ *             matte: [1..3] Kd  [4] sigma (must be 0: Lambert)
+ *             metal: [1..3] eta  [4..6] k  [7] roughness  [8] alpha
+ *             glass: [1..3] Kr  [4..6] Kt  [7] roughness  [8] alpha  [9] eta
+ *             (alpha = GGX::roughnessToAlpha(roughness), microfacet.h:44-50,
+ *             evaluated once on the host)
+ *   light     32 floats  [0] type: 0 area, 1 point, 2 spot  [1] triangle index
+ *             (area)  [2..4] Le or I  [5..7] position  [8] spot cone angle
+ *             [9] cosTotalWidth  [10] cosFalloffStart  [11..19] world-to-light
+ *             3x3  [20..28] light-to-world 3x3 (spot)  [29..31] spot target
+ *   camera    16 floats  origin(3), camera-to-world 3x3 row-major(9),
+ *             tan(vfov/2), aspect, W, H   (camera/pinhole.cpp:8-33)
+ */
+#ifndef ORION_AMD_H_
+#define ORION_AMD_H_
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORION_OK 0
+#define ORION_ERR_ARG -1
+#define ORION_ERR_IO -2
+#define ORION_ERR_PARSE -3
+#define ORION_ERR_HIP -4
+#define ORION_ERR_OOM -5
+#define ORION_ERR_UNSUPPORTED -6
+#define ORION_ERR_BUSY -7
+
+typedef struct orion_hscene_s *orion_hscene; /* host scene: resolved records */
+typedef struct orion_ctx_s *orion_ctx;       /* one GPU + one HIP stream */
+typedef struct orion_scene_s *orion_scene;   /* device-resident scene + film */
+
+const char *orion_last_error(void);
+const char *orion_version(void);
+
+/* ---------------------------------------------------------------- host scenes */
+typedef struct {
+  int n_tris, n_mats, n_lights;
+  int width, height;
+  int integrator;          /* 0 path, 1 sppm (from <Integrator type>) */
+  int max_depth;           /* maxDepth */
+  int spp;                 /* nSamplesPerPixel (path) */
+  int sppm_iterations;     /* numIterations */
+  int sppm_photons;        /* photonsPerIteration (-1 -> nPixels) */
+  float sppm_radius;       /* initRadius */
+} orion_scene_info;
+
+/* XML .ori loader (util/parser.cpp): Camera, Model (mesh | meshHandy),
+ * Material (matte | metal | glass, named/name), AreaLight, Light (point |
+ * spot), Integrator (path | sppm), Sampler (sobol). */
+int orion_hscene_load_ori(const char *path, orion_hscene *out);
+/* Synthetic scenes: name is one of "cornell", "icosphere", "caustics",
+ * "batch_job"; params (may be NULL) override defaults:
+ *   icosphere: [0] level [1] width [2] height
+ *   caustics:  [0] level [1] width [2] height
+ *   batch_job: [0] triangle count [1] width [2] height [3] material mix */
+int orion_hscene_generate(const char *name, uint64_t seed, const double *params, int n_params,
+                          orion_hscene *out);
+int orion_hscene_from_records(int n_tris, const float *tri_pn, const int *tri_mat,
+                              const int *tri_light, int n_mats, const float *mats,
+                              int n_lights, const float *lights, const float *camera,
+                              orion_hscene *out);
+int orion_hscene_info(orion_hscene s, orion_scene_info *info);
+/* Borrowed pointers into the host scene (valid until destroy). */
+int orion_hscene_records(orion_hscene s, const float **tri_pn, const int **tri_mat,
+                         const int **tri_light, const float **mats, const float **lights,
+                         const float **camera);
+/* Writes an .ori (+ .obj for meshes above mesh_inline_limit triangles). */
+int orion_hscene_write_ori(orion_hscene s, const char *path, int mesh_inline_limit);
+void orion_hscene_destroy(orion_hscene s);
+
+/* Resolved camera record from a lookat + fov (transform.cpp:197-217,
+ * pinhole.cpp:35-48); exposed for the loader's parity tests. */
+int orion_camera_record(const float lookat[9], float fov_deg, int width, int height,
+                        float record[16]);
+/* createTransform over (key, value) pairs (transform.cpp:218-252): m and mInv, row-major. */
+int orion_create_transform(int n, const char *const *keys, const char *const *values,
+                           float m[16], float m_inv[16]);
+/* Resolved spot/point light record as the loader builds it (light/spot.cpp:46-59). */
+int orion_light_record(int type, const float I[3], const float from[3], const float to[3],
+                       float angle, float delta, float record[32]);
+
+/* ---------------------------------------------------------------- device */
+int orion_ctx_create(int device, orion_ctx *out);
+int orion_ctx_destroy(orion_ctx c);
+int orion_ctx_synchronize(orion_ctx c);
+
+/* Builds the SAH BVH on the host and uploads scene + Sobol tables to HBM.
+ * Allocates the film (W*H sum + count) on the device. */
+int orion_scene_upload(orion_ctx c, orion_hscene s, orion_scene *out);
+int orion_scene_destroy(orion_scene s);
+typedef struct {
+  int64_t n_nodes, n_tris, node_bytes, tri_bytes, shade_bytes;
+  int bvh_depth, node_format;
+  double build_ms, upload_ms;
+} orion_scene_stats;
+int orion_scene_get_stats(orion_scene s, orion_scene_stats *st);
+
+/* Batched closest / any hit.  rays: SoA [8][n] = ox oy oz dx dy dz tmin tmax.
+ * on_device != 0: all pointers are device pointers (no copies).
+ * Outputs: t, prim (-1 = miss), u, v, tie (1 when another triangle was
+ * accepted at exactly the winning t); occluded for the any-hit query. */
+int orion_intersect_closest(orion_scene s, int n, const float *rays, float *t, int *prim,
+                            float *u, float *v, int *tie, int on_device);
+int orion_intersect_any(orion_scene s, int n, const float *rays, int *occluded, int on_device);
+
+/* Render statistics (device counters). */
+typedef struct {
+  int64_t paths;          /* camera samples completed */
+  int64_t closest_rays;   /* closest-hit casts (camera, bounce, MIS) */
+  int64_t shadow_rays;    /* any-hit casts */
+  int64_t ref_equiv_rays; /* casts orion itself would make (MIS ray counted twice) */
+  int64_t photons;
+  int64_t deposits;       /* SPPM photon deposit lookups */
+  double ms;              /* device time of the call */
+} orion_render_stats;
+
+typedef struct {
+  int spp_per_wave;     /* samples per pixel processed per wavefront (0 = auto) */
+  int block;            /* threads per block for traversal (0 = auto) */
+  int persistent_blocks;/* blocks per CU for persistent traversal (0 = auto) */
+  int use_graph;        /* capture the bounce loop into a hipGraph */
+} orion_tuning;
+
+/* PathTracing::render over samples [k0, k1): accumulates into the scene's film
+ * (sum + count per pixel, in sample order per pixel). */
+int orion_render_path(orion_scene s, int k0, int k1, int max_depth, const orion_tuning *tune,
+                      orion_render_stats *stats);
+int orion_film_clear(orion_scene s);
+/* rgb_sum: W*H*3 floats, count: W*H ints (host pointers). */
+int orion_film_read(orion_scene s, float *rgb_sum, int *count);
+
+/* SPPM::render.  hash_size <= 0 -> nPixels (the reference's choice).
+ * Optional bit-exact debug outputs (host pointers, may be NULL) for iteration
+ * debug_iter: vp_cell_count[hash_size], photon_cell_count[hash_size],
+ * M[nPixels], grid_info[8] (gridRes xyz, pair count), grid_bounds[6]. */
+typedef struct {
+  int debug_iter;
+  int *vp_cell_count;
+  int *photon_cell_count;
+  int *M;
+  int *grid_info;
+  float *grid_bounds;
+} orion_sppm_debug;
+int orion_render_sppm(orion_scene s, int n_iters, int photons_per_iter, float init_radius,
+                      int max_depth, int hash_size, const orion_sppm_debug *dbg,
+                      orion_render_stats *stats);
+
+/* Raw device pointers of the last wavefront's queues etc. are internal. */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1480 @@
+// Oracle: CPU restatement of orion's path-tracing + SPPM hot path.
+//
+// TEST INFRASTRUCTURE ONLY -- loaded by tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg as the checker / CPU baseline; the product
+// (orion_amd/) never links or calls it.
+//
+// Arithmetic follows the reference operation by operation (association order,
+// true division vs multiply-by-reciprocal, float promotions), because the
+// parity contract compares bits for triangle IDs / SPPM keys / counts and
+// 1e-3 relative for radiance.  Build flags: -O2 -ffp-contract=off, SSE float,
+// denormals preserved (the reference turns FTZ/DAZ on at Embree build time,
+// core/kernel.cpp:65-66; both oracle and GPU keep IEEE denormals instead).
+//
+// Pin table (function -> how it is pinned; see tests/):
+//   sobol()                sobol.h:28-43       -> oracle/_ref sobol::sample; dim 0 = radical inverse
+//   Sampler::next2 order   sampler.h:31        -> oracle/_ref SobolSampler (g++ evaluation order)
+//   camera_ray()           pinhole.cpp:8-33    -> oracle/_ref Pinhole::generateRay
+//   warps                  sampling.cpp:5-61   -> oracle/_ref + closed forms
+//   BSDF f/pdf/sample_f    bsdf.cpp, bsdf/*    -> oracle/_ref Matte/Metal/Glass::getBSDF
+//   point/spot lights      light/point.cpp, light/spot.cpp -> oracle/_ref
+//   area light, triangle   light/diffuse.cpp, core/triangle.cpp:85-135
+//                          -> uniformSampleTriangle via _ref; rest by closed forms
+//                             (solid-angle pdf of a quad, constant-emission scene)
+//   intersect()            core/kernel.cpp:111-174 (Embree semantics)
+//                          -> brute force over all triangles (Embree itself is
+//                             binary-only in the reference: parity of Embree's own
+//                             arithmetic is unpinned; hit semantics are defined here)
+//   Li / render            integrator/path.cpp, core/integrator.cpp
+//                          -> white-furnace closed form + per-stage pins above
+//   SPPM                   integrator/sppm.cpp -> ToGrid/hash integer closed forms,
+//                          brute-force per-cell counts, radius-update closed form
+//
+// sin/cos: the reference calls sinf/cosf (<= 0.56 ulp in glibc).  The oracle
+// evaluates (float)sin((double)x), the correctly rounded value except in ~1e-8
+// of inputs, so that the GPU (which does the same in FP64) follows identical
+// sample paths.  Build with -DORACLE_LIBM_TRIG to use sinf/cosf instead.
+#include "oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORION_SOBOL_STORAGE static const
+#include "../orion_amd/csrc/sobol_matrices.inc"
+
+namespace {
+
+constexpr float kEpsilon = 0.000001f;        // orion.h:48
+constexpr float kShadowEpsilon = 0.01f;      // orion.h:49
+constexpr float kInf = std::numeric_limits<float>::max();  // orion.h:51 fInfinity
+constexpr float kPi = 3.14159265358979323846f;             // orion.h:52
+constexpr float kInvPi = 0.31830988618379067154f;          // orion.h:53
+constexpr float kInv4Pi = 0.07957747154594766788f;         // orion.h:55
+constexpr float kOneMinusEps = (float)0.99999994;          // bsdf.cpp:15
+
+enum { REFL = 1, TRANS = 2, DIFF = 4, GLOSSY = 8, SPEC = 16, ALL = 31 };  // bsdf.h:16-24
+
+inline float fmin_ref(float a, float b) { return (b < a) ? b : a; }  // std::min
+inline float fmax_ref(float a, float b) { return (a < b) ? b : a; }  // std::max
+
+float osin(float x) {
+#ifdef ORACLE_LIBM_TRIG
+  return std::sin(x);
+#else
+  return (float)std::sin((double)x);
+#endif
+}
+float ocos(float x) {
+#ifdef ORACLE_LIBM_TRIG
+  return std::cos(x);
+#else
+  return (float)std::cos((double)x);
+#endif
+}
+
+// ---------------------------------------------------------------- vectors
+struct V3 { float x, y, z; };
+inline V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+inline V3 add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 neg(V3 a) { return mk(-a.x, -a.y, -a.z); }
+inline V3 mul(V3 a, float s) { return mk(s * a.x, s * a.y, s * a.z); }
+// geometry.h:169-174: v / f == v * (1 / f)
+inline V3 divs(V3 a, float f) { float inv = 1.0f / f; return mk(a.x * inv, a.y * inv, a.z * inv); }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline float absdot(V3 a, V3 b) { return std::abs(a.x * b.x + a.y * b.y + a.z * b.z); }
+inline V3 cross(V3 a, V3 b) {  // geometry.h:753-758
+  return mk((a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x));
+}
+inline float lensq(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+inline V3 normalize(V3 a) { return divs(a, std::sqrt(lensq(a))); }
+inline V3 vabs(V3 a) { return mk(std::abs(a.x), std::abs(a.y), std::abs(a.z)); }
+inline float comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+inline void setc(V3 &a, int i, float v) { if (i == 0) a.x = v; else if (i == 1) a.y = v; else a.z = v; }
+
+// ---------------------------------------------------------------- spectrum
+struct S { float r, g, b; };
+inline S sp(float v) { return S{v, v, v}; }
+inline S sp(float r, float g, float b) { return S{r, g, b}; }
+inline S operator+(S a, S b) { return S{b.r + a.r, b.g + a.g, b.b + a.b}; }
+inline S operator-(S a, S b) { return S{a.r - b.r, a.g - b.g, a.b - b.b}; }
+inline S operator*(S a, S b) { return S{b.r * a.r, b.g * a.g, b.b * a.b}; }
+inline S operator/(S a, S b) { return S{a.r / b.r, a.g / b.g, a.b / b.b}; }
+inline S operator*(S a, float f) { return S{a.r * f, a.g * f, a.b * f}; }
+inline S operator/(S a, float f) { return S{a.r / f, a.g / f, a.b / f}; }  // spectrum.h:133-136
+inline S operator+(S a, float f) { return S{a.r + f, a.g + f, a.b + f}; }
+inline S operator-(S a, float f) { return S{a.r - f, a.g - f, a.b - f}; }
+inline S &operator+=(S &a, S b) { a.r += b.r; a.g += b.g; a.b += b.b; return a; }
+inline S &operator*=(S &a, S b) { a.r *= b.r; a.g *= b.g; a.b *= b.b; return a; }
+inline bool black(S a) { return !(a.r != 0.0f || a.g != 0.0f || a.b != 0.0f); }
+inline float maxc(S a) { return fmax_ref(a.r, fmax_ref(a.g, a.b)); }
+inline float intensity(S a) { return 0.212671f * a.r + 0.715160f * a.g + 0.072169f * a.b; }
+inline S ssqrt(S a) { return S{std::sqrt(a.r), std::sqrt(a.g), std::sqrt(a.b)}; }
+inline float clampf(float f, float lo, float hi) { return f < lo ? lo : (f > hi ? hi : f); }
+inline S sclamp0(S a) {  // spectrum.h:89-93 (upper bound ignored -> fInfinity)
+  return S{clampf(a.r, 0, kInf), clampf(a.g, 0, kInf), clampf(a.b, 0, kInf)};
+}
+
+// ---------------------------------------------------------------- sobol
+// sobol.h:28-43: XOR of generator columns for the set bits of index, then
+// float(result) * 2^-32 (round-to-nearest conversion; may give exactly 1.0f).
+float sobol(uint64_t index, unsigned dim) {
+  unsigned result = 0;
+  const unsigned *m = orion_sobol_matrices + (size_t)dim * ORION_SOBOL_COLS;
+  for (int i = 0; index; index >>= 1, ++i)
+    if (index & 1) result ^= m[i];
+  return result * (1.f / (1ULL << 32));
+}
+
+struct Sampler {  // sampler.h:15-34, sobol.h:48-60
+  uint64_t index;
+  int dim;
+  explicit Sampler(uint64_t idx) : index(idx), dim(5) {}
+  float next() { return sobol(index, (unsigned)dim++); }
+  // Point2f(next(), next()) as compiled by g++: the right argument is
+  // evaluated first, so y takes dimension d and x takes d + 1.
+  void next2(float *x, float *y) { *y = next(); *x = next(); }
+};
+
+// ---------------------------------------------------------------- warps
+// sampling.cpp:29-33 uniformSampleDisk (not concentric)
+inline void uniform_disk(float u0, float u1, float *x, float *y) {
+  float r = std::sqrt(u0);
+  float theta = 2 * kPi * u1;
+  *x = r * ocos(theta);
+  *y = r * osin(theta);
+}
+// sampling.cpp:56-61
+inline V3 cosine_hemisphere(float u0, float u1) {
+  float dx, dy;
+  uniform_disk(u0, u1, &dx, &dy);
+  float z = std::sqrt(fmax_ref(0.0f, 1 - dx * dx - dy * dy));
+  return mk(dx, dy, z);
+}
+// sampling.cpp:5-11
+inline V3 uniform_sphere(float u0, float u1) {
+  float z = 1 - 2 * u0;
+  float r = std::sqrt(fmax_ref((float)0, (float)1 - z * z));
+  float phi = 2 * kPi * u1;
+  return mk(r * ocos(phi), r * osin(phi), z);
+}
+// sampling.cpp:20-27
+inline V3 uniform_cone(float u0, float u1, float cosThetaMax) {
+  float cosTheta = ((float)1 - u0) + u0 * cosThetaMax;
+  float sinTheta = std::sqrt((float)1 - cosTheta * cosTheta);
+  float phi = u1 * 2 * kPi;
+  return mk(ocos(phi) * sinTheta, osin(phi) * sinTheta, cosTheta);
+}
+// sampling.cpp:14-18
+inline void uniform_triangle(float u0, float u1, float *b0, float *b1) {
+  float su0 = std::sqrt(u0);
+  *b0 = 1 - su0;
+  *b1 = u1 * su0;
+}
+inline float power_heuristic(float fPdf, float gPdf) {  // sampling.h:49-52, nf = ng = 1
+  float f = 1 * fPdf, g = 1 * gPdf;
+  return (f * f) / (f * f + g * g);
+}
+
+// geometry.h:824-848
+inline float next_up(float v) {
+  if (std::isinf(v) && v > 0.) return v;
+  if (v == -0.f) v = 0.f;
+  uint32_t ui; std::memcpy(&ui, &v, 4);
+  if (v >= 0) ++ui; else --ui;
+  std::memcpy(&v, &ui, 4);
+  return v;
+}
+inline float next_down(float v) {
+  if (std::isinf(v) && v < 0.) return v;
+  if (v == 0.f) v = -0.f;
+  uint32_t ui; std::memcpy(&ui, &v, 4);
+  if (v > 0) --ui; else ++ui;
+  std::memcpy(&v, &ui, 4);
+  return v;
+}
+// geometry.h:850-863
+V3 offset_ray_origin(V3 p, V3 n, V3 w) {
+  float maxValue = fmax_ref(fmax_ref(std::abs(p.x), std::abs(p.y)), std::abs(p.z));
+  float c = maxValue * kEpsilon;
+  float d = dot(vabs(n), mk(c, c, c));
+  V3 offset = mul(n, d);
+  if (dot(w, n) < 0) offset = neg(offset);
+  V3 po = add(p, offset);
+  for (int i = 0; i < 3; ++i) {
+    if (comp(offset, i) > 0) setc(po, i, next_up(comp(po, i)));
+    else if (comp(offset, i) < 0) setc(po, i, next_down(comp(po, i)));
+  }
+  return po;
+}
+// geometry.h:902-909
+void coordinate_system(V3 z, V3 *x, V3 *y) {
+  if (std::abs(z.x) > std::abs(z.y))
+    *y = divs(mk(-z.z, 0, z.x), std::sqrt(z.x * z.x + z.z * z.z));
+  else
+    *y = divs(mk(0, z.z, -z.y), std::sqrt(z.y * z.y + z.z * z.z));
+  *x = cross(*y, z);
+}
+inline V3 mat3_apply(const float *m, V3 v) {  // transform.h:136-142 (vector)
+  return mk(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+            m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+
+struct Ray { V3 o, d; float tmin, tmax; };
+
+// ---------------------------------------------------------------- scene
+struct Tri { V3 p0, p1, p2, n0, n1, n2; int mat, light; };
+struct Material { int type; float v[15]; };
+enum { MAT_MATTE = 0, MAT_METAL = 1, MAT_GLASS = 2 };
+enum { LIGHT_AREA = 0, LIGHT_POINT = 1, LIGHT_SPOT = 2 };
+struct Light {
+  int type, prim;
+  S L;          // Le (area) or I (point/spot)
+  V3 pos;
+  float area;
+  float cosTotal, cosFalloffStart;
+  float w2l[9], l2w[9];
+};
+
+struct Distribution1D {  // sampling.h:71-125
+  std::vector<float> func, cdf;
+  float funcInt = 0;
+  void init(const float *f, int n) {
+    func.assign(f, f + n);
+    cdf.assign(n + 1, 0.f);
+    cdf[0] = 0;
+    for (int i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + func[i - 1] / n;
+    funcInt = cdf[n];
+    if (funcInt == 0) {
+      for (int i = 1; i < n + 1; ++i) cdf[i] = float(i) / float(n);
+    } else {
+      for (int i = 1; i < n + 1; ++i) cdf[i] /= funcInt;
+    }
+  }
+  int count() const { return (int)func.size(); }
+  int sample_discrete(float u, float *pdf) const {
+    // findInterval, geometry.h:876-889
+    int size = (int)cdf.size();
+    int first = 0, length = size;
+    while (length > 0) {
+      int half = length >> 1, middle = first + half;
+      if (cdf[middle] <= u) { first = middle + 1; length -= half + 1; }
+      else length = half;
+    }
+    int off = first - 1;
+    off = off < 0 ? 0 : (off > size - 2 ? size - 2 : off);
+    if (pdf) *pdf = (funcInt > 0) ? func[off] / (funcInt * count()) : 0;
+    return off;
+  }
+};
+
+struct BVHNode { float lo[3], hi[3]; int left_or_first, count; };  // count>0: leaf
+
+struct Scene {
+  std::vector<Tri> tris;
+  std::vector<Material> mats;
+  std::vector<Light> lights;
+  float cam[16];
+  int W, H;
+  Distribution1D power;
+  std::vector<BVHNode> nodes;
+  std::vector<int> order;
+};
+
+// ---------------------------------------------------------------- BVH (oracle's own)
+struct Build {
+  const Scene *sc;
+  std::vector<float> clo, chi, cen;  // per tri padded bounds + centroid
+  std::vector<BVHNode> *nodes;
+  std::vector<int> *order;
+};
+void tri_bounds(const Tri &t, float lo[3], float hi[3]) {
+  for (int a = 0; a < 3; ++a) {
+    float x0 = comp(t.p0, a), x1 = comp(t.p1, a), x2 = comp(t.p2, a);
+    float l = std::min(x0, std::min(x1, x2)), h = std::max(x0, std::max(x1, x2));
+    // pad so rounding in the triangle test can never be culled by the box test
+    float pad = (std::max(std::abs(l), std::abs(h)) + (h - l)) * 1.0e-5f + 1e-30f;
+    lo[a] = l - pad; hi[a] = h + pad;
+  }
+}
+int build_rec(Build &b, int first, int count) {
+  std::vector<int> &ord = *b.order;
+  int idx = (int)b.nodes->size();
+  b.nodes->push_back(BVHNode());
+  BVHNode nd;
+  for (int a = 0; a < 3; ++a) { nd.lo[a] = kInf; nd.hi[a] = -kInf; }
+  float clo[3] = {kInf, kInf, kInf}, chi[3] = {-kInf, -kInf, -kInf};
+  for (int i = first; i < first + count; ++i) {
+    int t = ord[i];
+    for (int a = 0; a < 3; ++a) {
+      nd.lo[a] = std::min(nd.lo[a], b.clo[3 * t + a]);
+      nd.hi[a] = std::max(nd.hi[a], b.chi[3 * t + a]);
+      clo[a] = std::min(clo[a], b.cen[3 * t + a]);
+      chi[a] = std::max(chi[a], b.cen[3 * t + a]);
+    }
+  }
+  if (count <= 4) {
+    nd.left_or_first = first; nd.count = count;
+    (*b.nodes)[idx] = nd;
+    return idx;
+  }
+  // binned SAH, 16 bins, best axis
+  const int NB = 16;
+  int best_axis = -1, best_split = -1;
+  float best_cost = kInf;
+  for (int a = 0; a < 3; ++a) {
+    float ext = chi[a] - clo[a];
+    if (!(ext > 0)) continue;
+    int cnt[NB] = {0};
+    float blo[NB][3], bhi[NB][3];
+    for (int k = 0; k < NB; ++k) for (int c = 0; c < 3; ++c) { blo[k][c] = kInf; bhi[k][c] = -kInf; }
+    for (int i = first; i < first + count; ++i) {
+      int t = ord[i];
+      int k = (int)((b.cen[3 * t + a] - clo[a]) / ext * NB);
+      k = k < 0 ? 0 : (k >= NB ? NB - 1 : k);
+      cnt[k]++;
+      for (int c = 0; c < 3; ++c) {
+        blo[k][c] = std::min(blo[k][c], b.clo[3 * t + c]);
+        bhi[k][c] = std::max(bhi[k][c], b.chi[3 * t + c]);
+      }
+    }
+    float rA[NB], lA[NB]; int lC[NB];
+    float l3[3] = {kInf, kInf, kInf}, h3[3] = {-kInf, -kInf, -kInf}; int c = 0;
+    for (int k = 0; k < NB; ++k) {
+      c += cnt[k];
+      for (int q = 0; q < 3; ++q) { l3[q] = std::min(l3[q], blo[k][q]); h3[q] = std::max(h3[q], bhi[k][q]); }
+      float dx = h3[0] - l3[0], dy = h3[1] - l3[1], dz = h3[2] - l3[2];
+      lA[k] = c ? (dx * dy + dy * dz + dz * dx) : 0; lC[k] = c;
+    }
+    for (int q = 0; q < 3; ++q) { l3[q] = kInf; h3[q] = -kInf; }
+    for (int k = NB - 1; k >= 0; --k) {
+      for (int q = 0; q < 3; ++q) { l3[q] = std::min(l3[q], blo[k][q]); h3[q] = std::max(h3[q], bhi[k][q]); }
+      float dx = h3[0] - l3[0], dy = h3[1] - l3[1], dz = h3[2] - l3[2];
+      rA[k] = (count - (k ? lC[k - 1] : 0)) ? (dx * dy + dy * dz + dz * dx) : 0;
+    }
+    for (int k = 0; k < NB - 1; ++k) {
+      int nl = lC[k], nr = count - nl;
+      if (nl == 0 || nr == 0) continue;
+      float cost = lA[k] * nl + rA[k + 1] * nr;
+      if (cost < best_cost) { best_cost = cost; best_axis = a; best_split = k; }
+    }
+  }
+  int mid;
+  if (best_axis < 0) {
+    mid = first + count / 2;  // all centroids coincide: split by index
+  } else {
+    float ext = chi[best_axis] - clo[best_axis];
+    auto it = std::partition(ord.begin() + first, ord.begin() + first + count, [&](int t) {
+      int k = (int)((b.cen[3 * t + best_axis] - clo[best_axis]) / ext * NB);
+      k = k < 0 ? 0 : (k >= NB ? NB - 1 : k);
+      return k <= best_split;
+    });
+    mid = (int)(it - ord.begin());
+    if (mid == first || mid == first + count) mid = first + count / 2;
+  }
+  int l = build_rec(b, first, mid - first);
+  int r = build_rec(b, mid, first + count - mid);
+  (void)l;
+  nd.left_or_first = r;  // left child is idx + 1
+  nd.count = 0;
+  (*b.nodes)[idx] = nd;
+  return idx;
+}
+void build_bvh(Scene &sc) {
+  int n = (int)sc.tris.size();
+  Build b;
+  b.sc = &sc;
+  b.clo.resize(3 * n); b.chi.resize(3 * n); b.cen.resize(3 * n);
+  for (int i = 0; i < n; ++i) {
+    tri_bounds(sc.tris[i], &b.clo[3 * i], &b.chi[3 * i]);
+    for (int a = 0; a < 3; ++a) b.cen[3 * i + a] = 0.5f * (b.clo[3 * i + a] + b.chi[3 * i + a]);
+  }
+  sc.order.resize(n);
+  for (int i = 0; i < n; ++i) sc.order[i] = i;
+  sc.nodes.clear();
+  sc.nodes.reserve(2 * n / 4 + 16);
+  b.nodes = &sc.nodes; b.order = &sc.order;
+  if (n > 0) build_rec(b, 0, n);
+}
+
+// ---------------------------------------------------------------- triangle test
+// The closest-hit contract this build defines for rtcIntersect
+// (core/kernel.cpp:111-125): Moller-Trumbore in float without contraction,
+// accept tmin < t <= tmax, no culling, closest = lexicographic min (t, prim).
+inline bool tri_test(const Tri &tr, const Ray &r, float *t, float *u, float *v) {
+  V3 e1 = sub(tr.p1, tr.p0), e2 = sub(tr.p2, tr.p0);
+  V3 pv = cross(r.d, e2);
+  float det = dot(e1, pv);
+  if (det == 0.0f || det != det) return false;
+  float inv = 1.0f / det;
+  V3 tv = sub(r.o, tr.p0);
+  float uu = dot(tv, pv) * inv;
+  if (!(uu >= 0.0f && uu <= 1.0f)) return false;
+  V3 qv = cross(tv, e1);
+  float vv = dot(r.d, qv) * inv;
+  if (!(vv >= 0.0f && uu + vv <= 1.0f)) return false;
+  float tt = dot(e2, qv) * inv;
+  if (!(tt > r.tmin && tt <= r.tmax)) return false;
+  *t = tt; *u = uu; *v = vv;
+  return true;
+}
+
+inline bool box_hit(const BVHNode &n, const Ray &r, const float inv[3], float t0, float t1) {
+  const float g3 = 3 * 5.9604645e-08f / (1 - 3 * 5.9604645e-08f);  // gamma(3)
+  const float o[3] = {r.o.x, r.o.y, r.o.z};
+  for (int a = 0; a < 3; ++a) {
+    float tn = (n.lo[a] - o[a]) * inv[a];
+    float tf = (n.hi[a] - o[a]) * inv[a];
+    if (tn > tf) std::swap(tn, tf);
+    tf *= 1 + 2 * g3;
+    t0 = tn > t0 ? tn : t0;
+    t1 = tf < t1 ? tf : t1;
+    if (t0 > t1) return false;
+  }
+  return true;
+}
+
+struct Hit { float t, u, v; int prim; bool tie; };
+
+bool intersect_brute(const Scene &sc, const Ray &r, bool any, Hit *h) {
+  bool found = false;
+  h->tie = false;
+  for (int i = 0; i < (int)sc.tris.size(); ++i) {
+    float t, u, v;
+    if (!tri_test(sc.tris[i], r, &t, &u, &v)) continue;
+    if (any) { h->t = t; h->u = u; h->v = v; h->prim = i; return true; }
+    if (!found || t < h->t) { *h = Hit{t, u, v, i, false}; found = true; }
+    else if (t == h->t) {
+      h->tie = true;
+      if (i < h->prim) { h->u = u; h->v = v; h->prim = i; }
+    }
+  }
+  return found;
+}
+
+bool intersect_bvh(const Scene &sc, const Ray &r, bool any, Hit *h) {
+  if (sc.nodes.empty()) return false;
+  float inv[3] = {1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
+  int stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  bool found = false;
+  h->tie = false;
+  while (sp) {
+    const BVHNode &n = sc.nodes[stack[--sp]];
+    float tlim = found ? h->t : r.tmax;
+    if (!box_hit(n, r, inv, r.tmin, tlim)) continue;
+    if (n.count > 0) {
+      for (int k = n.left_or_first; k < n.left_or_first + n.count; ++k) {
+        int i = sc.order[k];
+        float t, u, v;
+        if (!tri_test(sc.tris[i], r, &t, &u, &v)) continue;
+        if (any) { h->t = t; h->u = u; h->v = v; h->prim = i; return true; }
+        if (!found || t < h->t) { *h = Hit{t, u, v, i, false}; found = true; }
+        else if (t == h->t) {
+          h->tie = true;
+          if (i < h->prim) { h->u = u; h->v = v; h->prim = i; }
+        }
+      }
+    } else {
+      int self = (int)(&n - sc.nodes.data());
+      stack[sp++] = n.left_or_first;
+      stack[sp++] = self + 1;
+    }
+  }
+  return found;
+}
+
+// ---------------------------------------------------------------- intersection record
+struct Isect {
+  V3 p, ng, ns, u, v;
+  float t;
+  int prim;
+};
+// core/kernel.cpp:127-172: reconstruct the hit from (primID, u, v, tfar)
+void make_isect(const Scene &sc, const Hit &h, Isect *is) {
+  const Tri &tr = sc.tris[h.prim];
+  float w = 1 - h.u - h.v;
+  is->p = add(add(mul(tr.p0, w), mul(tr.p1, h.u)), mul(tr.p2, h.v));
+  is->ns = add(add(mul(tr.n0, w), mul(tr.n1, h.u)), mul(tr.n2, h.v));
+  is->ng = normalize(cross(sub(tr.p1, tr.p0), sub(tr.p2, tr.p0)));
+  is->t = h.t;
+  V3 u = normalize(sub(tr.p1, tr.p0));
+  V3 v = normalize(cross(is->ns, u));
+  u = normalize(cross(v, is->ns));
+  is->u = u; is->v = v;
+  is->prim = h.prim;
+}
+
+struct Counters { int64_t closest = 0, shadow = 0, paths = 0; };
+
+bool scene_intersect(const Scene &sc, const Ray &r, Isect *is, Counters *c) {
+  if (c) c->closest++;
+  Hit h;
+  if (!intersect_bvh(sc, r, false, &h)) return false;
+  make_isect(sc, h, is);
+  return true;
+}
+bool scene_occluded(const Scene &sc, const Ray &r, Counters *c) {
+  if (c) c->shadow++;
+  Hit h;
+  return intersect_bvh(sc, r, true, &h);
+}
+
+Ray spawn_dir(const Isect &is, V3 d) {  // intersection.h:46-49
+  return Ray{offset_ray_origin(is.p, is.ng, d), d, 0, kInf};
+}
+Ray spawn_to(const Isect &is, V3 target) {  // intersection.h:40-44
+  V3 o = offset_ray_origin(is.p, is.ng, sub(target, is.p));
+  return Ray{o, sub(target, o), 0, 1 - kShadowEpsilon};
+}
+
+// ---------------------------------------------------------------- BSDFs
+inline float roughness_to_alpha(float roughness) {  // microfacet.h:44-50
+  roughness = fmax_ref(roughness, (float)1e-3);
+  float x = std::log(roughness);
+  return 1.62142f + 0.819955f * x + 0.1734f * x * x + 0.0171201f * x * x * x +
+         0.000640711f * x * x * x * x;
+}
+
+enum { BX_LAMBERT = 0, BX_MF_REFL = 1, BX_MF_TRANS = 2, BX_FRESNEL_SPEC = 3 };
+enum { FR_CONDUCTOR = 0, FR_DIELECTRIC = 1 };
+struct BxDF {
+  int kind, type;
+  S R, T;
+  float alpha;
+  int fresnel;        // for BX_MF_REFL
+  S etaT, k;          // conductor
+  float etaA, etaB;   // dielectric / transmission
+  int photon;         // TransportMode::Photon
+};
+struct BSDF {
+  V3 ng, ns, u, v;
+  int n = 0;
+  BxDF b[2];
+  V3 to_local(V3 w) const { return mk(dot(w, u), dot(w, v), dot(w, ns)); }  // bsdf.h:117-120
+  V3 to_world(V3 w) const {  // bsdf.h:122-127
+    return mk(u.x * w.x + v.x * w.y + ns.x * w.z, u.y * w.x + v.y * w.y + ns.y * w.z,
+              u.z * w.x + v.z * w.y + ns.z * w.z);
+  }
+};
+inline bool match(const BxDF &b, int t) { return (b.type & t) == b.type; }  // bsdf.h:33
+
+// bsdf.cpp:124-147
+S fr_conductor(float cosThetaI, S etai, S etat, S k) {
+  cosThetaI = clampf(cosThetaI, -1.0f, 1.0f);
+  S eta = etat / etai;
+  S etak = k / etai;
+  float cosThetaI2 = cosThetaI * cosThetaI;
+  float sinThetaI2 = float(1.) - cosThetaI2;
+  S eta2 = eta * eta;
+  S etak2 = etak * etak;
+  S t0 = eta2 - etak2 - sinThetaI2;
+  S a2plusb2 = ssqrt(t0 * t0 + eta2 * 4.0f * etak2);
+  S t1 = a2plusb2 + cosThetaI2;
+  S a = ssqrt((a2plusb2 + t0) * 0.5f);
+  S t2 = a * ((float)2 * cosThetaI);
+  S Rs = (t1 - t2) / (t1 + t2);
+  S t3 = a2plusb2 * cosThetaI2 + sinThetaI2 * sinThetaI2;
+  S t4 = t2 * sinThetaI2;
+  S Rp = Rs * (t3 - t4) / (t3 + t4);
+  return (Rp + Rs) * float(0.5);
+}
+// bsdf.cpp:149-170
+float fr_dielectric(float cosThetaI, float etaI, float etaT) {
+  cosThetaI = clampf(cosThetaI, (float)(-1.0f), (float)(1.0f));
+  bool entering = cosThetaI > 0.f;
+  if (!entering) { std::swap(etaI, etaT); cosThetaI = std::abs(cosThetaI); }
+  float sinThetaI = std::sqrt(fmax_ref((float)0, 1 - cosThetaI * cosThetaI));
+  float sinThetaT = etaI / etaT * sinThetaI;
+  if (sinThetaT >= 1) return 1;
+  float cosThetaT = std::sqrt(fmax_ref((float)0, 1 - sinThetaT * sinThetaT));
+  float Rparl = ((etaT * cosThetaI) - (etaI * cosThetaT)) / ((etaT * cosThetaI) + (etaI * cosThetaT));
+  float Rperp = ((etaI * cosThetaI) - (etaT * cosThetaT)) / ((etaI * cosThetaI) + (etaT * cosThetaT));
+  return (Rparl * Rparl + Rperp * Rperp) / 2;
+}
+inline S fresnel_eval(const BxDF &b, float cosI) {  // bsdf.cpp:172-180: |cos| passed
+  if (b.fresnel == FR_CONDUCTOR) return fr_conductor(std::abs(cosI), sp(1.0f), b.etaT, b.k);
+  return sp(fr_dielectric(std::abs(cosI), b.etaA, b.etaB));
+}
+
+// sampling.h:15-21, microfacet.cpp:9-40 (GGX)
+inline float cos2t(V3 w) { return w.z * w.z; }
+inline float sin2t(V3 w) { return fmax_ref(0.0f, 1.0f - cos2t(w)); }
+inline float tan2t(V3 w) { return sin2t(w) / cos2t(w); }
+inline float tant(V3 w) { return std::sqrt(sin2t(w)) / w.z; }
+inline bool same_hemi(V3 a, V3 b) { return a.z * b.z > 0; }
+float ggx_D(float alpha, V3 wh) {
+  float tan2Th = tan2t(wh);
+  if (std::isinf(tan2Th)) return 0;
+  float cos4Th = cos2t(wh) * cos2t(wh);
+  float alpha2 = alpha * alpha;
+  float e = tan2Th / alpha2;
+  return 1 / (kPi * alpha2 * cos4Th * (1 + e) * (1 + e));
+}
+float ggx_lambda(float alpha, V3 w) {
+  float absTanTheta = std::abs(tant(w));
+  if (std::isinf(absTanTheta)) return 0.;
+  float a2t2 = (alpha * absTanTheta) * (alpha * absTanTheta);
+  return (-1 + std::sqrt(1.f + a2t2)) / 2;
+}
+inline float ggx_G(float alpha, V3 wi, V3 wo) {  // microfacet.h:21-23
+  return 1 / (1 + ggx_lambda(alpha, wi) + ggx_lambda(alpha, wo));
+}
+inline float ggx_pdf(float alpha, V3 wh) { return ggx_D(alpha, wh) * std::abs(wh.z); }
+V3 ggx_sample_wh(float alpha, V3 wo, float u0, float u1) {
+  float tanTheta2 = alpha * alpha * u0 / (1.0f - u0);
+  float cosTheta = 1 / std::sqrt(1 + tanTheta2);
+  float sinTheta = std::sqrt(fmax_ref((float)0., (float)1. - cosTheta * cosTheta));
+  float phi = (2 * kPi) * u1;
+  V3 wh = mk(sinTheta * ocos(phi), sinTheta * osin(phi), cosTheta);
+  if (!same_hemi(wo, wh)) wh = neg(wh);
+  return wh;
+}
+inline V3 reflect(V3 wo, V3 n) { return add(neg(wo), mul(n, 2 * dot(wo, n))); }  // bsdf.h:101-103
+inline bool refract(V3 wi, V3 n, float eta, V3 *wt) {  // bsdf.h:105-115
+  float cosThetaI = dot(n, wi);
+  float sin2ThetaI = fmax_ref(float(0), float(1 - cosThetaI * cosThetaI));
+  float sin2ThetaT = eta * eta * sin2ThetaI;
+  if (sin2ThetaT >= 1) return false;
+  float cosThetaT = std::sqrt(1 - sin2ThetaT);
+  *wt = add(mul(neg(wi), eta), mul(n, eta * cosThetaI - cosThetaT));
+  return true;
+}
+
+S bxdf_f(const BxDF &b, V3 wi, V3 wo) {
+  switch (b.kind) {
+    case BX_LAMBERT: return b.R * kInvPi;  // lambert.cpp:5-8
+    case BX_MF_REFL: {  // microfacet.cpp:55-66
+      float cosThetaO = std::abs(wo.z), cosThetaI = std::abs(wi.z);
+      if (cosThetaI == 0 || cosThetaO == 0) return sp(0);
+      V3 wh = add(wi, wo);
+      if (wh.x == 0 && wh.y == 0 && wh.z == 0) return sp(0);
+      wh = normalize(wh);
+      S F = fresnel_eval(b, dot(wi, wh));
+      return b.R * ggx_D(b.alpha, wh) * ggx_G(b.alpha, wi, wi) * F / (4 * cosThetaI * cosThetaO);
+    }
+    case BX_MF_TRANS: {  // microfacet.cpp:87-106
+      if (same_hemi(wi, wo)) return sp(0);
+      float cosThetaO = wo.z, cosThetaI = wi.z;
+      if (cosThetaI == 0 || cosThetaO == 0) return sp(0);
+      float eta = wo.z > 0 ? (b.etaB / b.etaA) : (b.etaA / b.etaB);
+      V3 wh = normalize(add(wo, mul(wi, eta)));
+      if (wh.z < 0) wh = neg(wh);
+      S F = sp(fr_dielectric(std::abs(dot(wo, wh)), b.etaA, b.etaB));
+      float sqrtDenom = dot(wo, wh) + eta * dot(wi, wh);
+      float factor = b.photon ? 1 : (1 / eta);
+      return (sp(1.f) - F) * b.T *
+             std::abs(ggx_D(b.alpha, wh) * ggx_G(b.alpha, wi, wo) * eta * eta * absdot(wi, wh) *
+                      absdot(wo, wh) * factor * factor /
+                      (cosThetaI * cosThetaO * sqrtDenom * sqrtDenom));
+    }
+    default: return sp(0);  // specular lobes: f = 0
+  }
+}
+float bxdf_pdf(const BxDF &b, V3 wi, V3 wo) {
+  switch (b.kind) {
+    case BX_LAMBERT: return same_hemi(wi, wo) ? std::abs(wi.z) * kInvPi : 0;  // bsdf.cpp:117-121
+    case BX_MF_REFL: {  // microfacet.cpp:68-74
+      if (!same_hemi(wi, wo)) return 0;
+      V3 wh = normalize(add(wi, wo));
+      return ggx_pdf(b.alpha, wh) / (4 * dot(wo, wh));
+    }
+    case BX_MF_TRANS: {  // microfacet.cpp:108-118
+      if (same_hemi(wo, wi)) return 0;
+      float eta = wo.z > 0 ? (b.etaB / b.etaA) : (b.etaA / b.etaB);
+      V3 wh = normalize(add(wo, mul(wi, eta)));
+      float sqrtDenom = dot(wo, wh) + eta * dot(wi, wh);
+      float dwh_dwi = std::abs((eta * eta * dot(wi, wh)) / (sqrtDenom * sqrtDenom));
+      return ggx_pdf(b.alpha, wh) * dwh_dwi;
+    }
+    default: return 0;
+  }
+}
+// returns f; writes *wi, *pdf (left untouched when the lobe rejects without
+// writing it, as in the reference), *stype
+S bxdf_sample(const BxDF &b, V3 wo, float u0, float u1, V3 *wi, float *pdf, int *stype) {
+  switch (b.kind) {
+    case BX_LAMBERT: {  // bsdf.cpp:109-115
+      *wi = cosine_hemisphere(u0, u1);
+      if (wo.z < 0) wi->z *= -1;
+      *pdf = bxdf_pdf(b, *wi, wo);
+      return bxdf_f(b, *wi, wo);
+    }
+    case BX_MF_REFL: {  // microfacet.cpp:42-53
+      if (wo.z == 0) return sp(0);
+      V3 wh = ggx_sample_wh(b.alpha, wo, u0, u1);
+      *wi = reflect(wo, wh);
+      if (!same_hemi(*wi, wo)) return sp(0);
+      *pdf = ggx_pdf(b.alpha, wh) / (4 * dot(wo, wh));
+      return bxdf_f(b, *wi, wo);
+    }
+    case BX_MF_TRANS: {  // microfacet.cpp:76-85: never writes *pdf
+      if (wo.z == 0) return sp(0);
+      V3 wh = ggx_sample_wh(b.alpha, wo, u0, u1);
+      float eta = wo.z > 0 ? (b.etaA / b.etaB) : (b.etaB / b.etaA);
+      if (!refract(wo, wh, eta, wi)) return sp(0);
+      return bxdf_f(b, *wi, wo);
+    }
+    case BX_FRESNEL_SPEC: {  // specular.cpp:28-54
+      float F = fr_dielectric(wo.z, b.etaA, b.etaB);
+      if (u0 < F) {
+        *wi = mk(-wo.x, -wo.y, wo.z);
+        *stype = SPEC | REFL;
+        *pdf = F;
+        return b.R * F / std::abs(wi->z);
+      } else {
+        bool entering = wo.z > 0;
+        float etaI = entering ? b.etaA : b.etaB;
+        float etaT = entering ? b.etaB : b.etaA;
+        V3 n = mk(0, 0, 1);
+        if (dot(n, wo) < 0.f) n = neg(n);  // faceforward
+        if (!refract(wo, n, etaI / etaT, wi)) return sp(0);
+        S ft = b.T * (1 - F);
+        if (!b.photon) ft *= sp((etaI * etaI) / (etaT * etaT));
+        *stype = SPEC | TRANS;
+        *pdf = 1 - F;
+        return ft / std::abs(wi->z);
+      }
+    }
+  }
+  return sp(0);
+}
+
+int bsdf_num(const BSDF &bs, int type) {
+  int n = 0;
+  for (int i = 0; i < bs.n; ++i) if (match(bs.b[i], type)) ++n;
+  return n;
+}
+S bsdf_f(const BSDF &bs, V3 wi, V3 wo, int type) {  // bsdf.cpp:17-29
+  S r = sp(0);
+  V3 swi = bs.to_local(wi), swo = bs.to_local(wo);
+  for (int i = 0; i < bs.n; ++i)
+    if (match(bs.b[i], type)) r += bxdf_f(bs.b[i], swi, swo);
+  return r;
+}
+float bsdf_pdf(const BSDF &bs, V3 wi, V3 wo, int type) {  // bsdf.cpp:85-103
+  if (bs.n == 0) return 0;
+  V3 swi = bs.to_local(wi), swo = bs.to_local(wo);
+  if (swo.z == 0) return 0;
+  float pdf = 0;
+  int m = 0;
+  for (int i = 0; i < bs.n; ++i)
+    if (match(bs.b[i], type)) { ++m; pdf += bxdf_pdf(bs.b[i], swi, swo); }
+  return m > 0 ? pdf / m : 0.f;
+}
+// bsdf.cpp:31-83.  *pdf and *stype are left untouched on the swo.z == 0 exit.
+S bsdf_sample(const BSDF &bs, V3 *wi, V3 wo, float u0, float u1, float *pdf, int type, int *stype) {
+  int matching = bsdf_num(bs, type);
+  if (matching == 0) { *pdf = 0; *stype = 0; return sp(0); }
+  int pos = std::min((int)std::floor(u0 * matching), matching - 1);
+  int count = pos, sel = -1;
+  for (int i = 0; i < bs.n; ++i)
+    if (match(bs.b[i], type) && count-- == 0) { sel = i; break; }
+  const BxDF &bx = bs.b[sel];
+  float nu0 = fmin_ref(u0 * matching - pos, kOneMinusEps);
+  V3 swi, swo = bs.to_local(wo);
+  if (swo.z == 0) return sp(0);
+  *pdf = 0;
+  *stype = bx.type;
+  S f = bxdf_sample(bx, swo, nu0, u1, &swi, pdf, stype);
+  if (*pdf == 0) { *stype = 0; return sp(0); }
+  *wi = bs.to_world(swi);
+  bool hasSpecular = (bx.type & SPEC) != 0;
+  if (!hasSpecular && matching > 1)
+    for (int i = 0; i < bs.n; ++i)
+      if (i != sel && match(bs.b[i], type)) *pdf += bxdf_pdf(bs.b[i], swi, swo);
+  if (matching > 1) *pdf /= matching;
+  if (!hasSpecular && matching > 1) {
+    bool refl = dot(*wi, bs.ng) * dot(wo, bs.ng) > 0;
+    f = sp(0);
+    for (int i = 0; i < bs.n; ++i) {
+      bool hr = (bs.b[i].type & REFL) != 0, ht = (bs.b[i].type & TRANS) != 0;
+      if (match(bs.b[i], type) && ((refl && hr) || (!refl && ht))) f += bxdf_f(bs.b[i], swi, swo);
+    }
+  }
+  return f;
+}
+
+// material/matte.cpp:9-21, material/metal.cpp:8-17, material/glass.cpp:9-37
+void make_bsdf(const Scene &sc, int matIdx, V3 ng, V3 ns, V3 u, V3 v, bool photon, BSDF *bs) {
+  bs->ng = ng; bs->ns = ns; bs->u = u; bs->v = v; bs->n = 0;
+  const Material &m = sc.mats[matIdx];
+  if (m.type == MAT_MATTE) {
+    S r = sclamp0(sp(m.v[0], m.v[1], m.v[2]));
+    if (!black(r)) {
+      BxDF b{}; b.kind = BX_LAMBERT; b.type = REFL | DIFF; b.R = r;
+      bs->b[bs->n++] = b;
+    }
+  } else if (m.type == MAT_METAL) {
+    BxDF b{}; b.kind = BX_MF_REFL; b.type = REFL | GLOSSY; b.R = sp(1.0f);
+    b.alpha = roughness_to_alpha(m.v[6]);
+    b.fresnel = FR_CONDUCTOR;
+    b.etaT = sp(m.v[0], m.v[1], m.v[2]);
+    b.k = sp(m.v[3], m.v[4], m.v[5]);
+    bs->b[bs->n++] = b;
+  } else if (m.type == MAT_GLASS) {
+    S R = sp(m.v[0], m.v[1], m.v[2]), T = sp(m.v[3], m.v[4], m.v[5]);
+    float rough = m.v[6], e = m.v[8];
+    if (black(R) && black(T)) return;
+    float alpha = roughness_to_alpha(rough);
+    if (rough == 0) {
+      BxDF b{}; b.kind = BX_FRESNEL_SPEC; b.type = REFL | TRANS | SPEC;
+      b.R = R; b.T = T; b.etaA = 1.0f; b.etaB = e; b.photon = photon;
+      bs->b[bs->n++] = b;
+    } else {
+      if (!black(R)) {
+        BxDF b{}; b.kind = BX_MF_REFL; b.type = REFL | GLOSSY; b.R = R; b.alpha = alpha;
+        b.fresnel = FR_DIELECTRIC; b.etaA = 1.0f; b.etaB = e;
+        bs->b[bs->n++] = b;
+      }
+      if (!black(T)) {
+        BxDF b{}; b.kind = BX_MF_TRANS; b.type = TRANS | GLOSSY; b.T = T; b.alpha = alpha;
+        b.etaA = 1.0f; b.etaB = e; b.photon = photon;
+        bs->b[bs->n++] = b;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- lights
+float tri_area(const Tri &t) { return 0.5f * std::sqrt(lensq(cross(sub(t.p1, t.p0), sub(t.p2, t.p0)))); }
+
+S light_power(const Scene &sc, const Light &l) {
+  if (l.type == LIGHT_AREA) return l.L * l.area * kPi;  // diffuse.cpp:8-11
+  if (l.type == LIGHT_POINT) return l.L * (4 * kPi);    // point.cpp:6-9
+  return l.L * 2 * kPi * (1 - .5f * (l.cosFalloffStart + l.cosTotal));  // spot.cpp:6-9
+}
+float spot_falloff(const Light &l, V3 w) {  // spot.cpp:10-19
+  V3 wl = normalize(mat3_apply(l.w2l, w));
+  float cosTheta = wl.z;
+  if (cosTheta < l.cosTotal) return 0;
+  if (cosTheta > l.cosFalloffStart) return 1;
+  float delta = (cosTheta - l.cosTotal) / (l.cosFalloffStart - l.cosTotal);
+  return (delta * delta) * (delta * delta);
+}
+// Area emission L(intr, w) = dot(ng, w) > 0 ? Le : 0 (diffuse.cpp:53-56)
+inline S area_L(const Light &l, V3 ng, V3 w) { return dot(ng, w) > 0 ? l.L : sp(0); }
+
+// triangle.cpp:85-106: point on the triangle, ng face-forwarded to ns
+void tri_sample(const Tri &tr, float u0, float u1, V3 *p, V3 *ng, V3 *ns, float *pdf) {
+  float b0, b1;
+  uniform_triangle(u0, u1, &b0, &b1);
+  *p = add(add(mul(tr.p0, b0), mul(tr.p1, b1)), mul(tr.p2, 1 - b0 - b1));
+  V3 g = normalize(cross(sub(tr.p1, tr.p0), sub(tr.p2, tr.p0)));
+  *ns = add(add(mul(tr.n0, b0), mul(tr.n1, b1)), mul(tr.n2, 1.0f - b0 - b1));
+  *ng = dot(g, *ns) < 0.f ? neg(g) : g;
+  *pdf = 1.0f / tri_area(tr);
+}
+
+// Light::sample_Li.  Returns Li; *shadow is the ShadowTester ray.
+S light_sample_li(const Scene &sc, const Light &l, const Isect &ref, float u0, float u1, V3 *wi,
+                  float *pdf, Ray *shadow) {
+  if (l.type == LIGHT_AREA) {  // diffuse.cpp:13-25 + triangle.cpp:108-121
+    V3 p, ng, ns;
+    tri_sample(sc.tris[l.prim], u0, u1, &p, &ng, &ns, pdf);
+    V3 w = sub(p, ref.p);
+    if (lensq(w) == 0) *pdf = 0;
+    else {
+      w = normalize(w);
+      *pdf *= lensq(sub(ref.p, p)) / absdot(ng, neg(w));
+      if (std::isinf(*pdf)) *pdf = 0;
+    }
+    if (*pdf == 0 || lensq(sub(p, ref.p)) == 0) { *pdf = 0; return sp(0); }
+    *wi = normalize(sub(p, ref.p));
+    *shadow = spawn_to(ref, p);
+    return area_L(l, ng, neg(*wi));
+  }
+  // point.cpp:10-19, spot.cpp:20-29
+  *wi = normalize(sub(l.pos, ref.p));
+  *pdf = 1.0f;
+  *shadow = spawn_to(ref, l.pos);
+  if (l.type == LIGHT_POINT) return l.L / lensq(sub(l.pos, ref.p));
+  return l.L * spot_falloff(l, neg(*wi)) / lensq(sub(l.pos, ref.p));
+}
+
+// Light::sample_Le.  Returns Le.
+S light_sample_le(const Scene &sc, const Light &l, float a0, float a1, float b0, float b1,
+                  Ray *ray, V3 *nLight, float *pdfPos, float *pdfDir) {
+  if (l.type == LIGHT_AREA) {  // diffuse.cpp:32-45
+    V3 p, ng, ns;
+    tri_sample(sc.tris[l.prim], a0, a1, &p, &ng, &ns, pdfPos);
+    *nLight = ng;
+    V3 w = cosine_hemisphere(b0, b1);
+    *pdfDir = w.z * kInvPi;
+    V3 x, y;
+    coordinate_system(ng, &x, &y);
+    w = add(add(mul(x, w.x), mul(y, w.y)), mul(ng, w.z));
+    Isect is; is.p = p; is.ng = ng;
+    *ray = spawn_dir(is, w);
+    return area_L(l, ng, w);
+  }
+  if (l.type == LIGHT_POINT) {  // point.cpp:20-27
+    *ray = Ray{l.pos, uniform_sphere(a0, a1), 0, kInf};
+    *nLight = ray->d;
+    *pdfPos = 1;
+    *pdfDir = kInv4Pi;
+    return l.L;
+  }
+  // spot.cpp:30-38
+  V3 w = uniform_cone(a0, a1, l.cosTotal);
+  *ray = Ray{l.pos, mat3_apply(l.l2w, w), 0, kInf};
+  *nLight = ray->d;
+  *pdfPos = 1;
+  *pdfDir = 1 / (2 * kPi * (1 - l.cosTotal));
+  return l.L * spot_falloff(l, ray->d);
+}
+
+// ---------------------------------------------------------------- direct lighting
+// core/integrator.cpp:29-90
+S estimate_direct(const Scene &sc, const Ray &ray, const Isect &is, const BSDF &bs, float bu0,
+                  float bu1, int li, float lu0, float lu1, Counters *c) {
+  const Light &light = sc.lights[li];
+  const int type = ALL & ~SPEC;
+  const bool delta = light.type != LIGHT_AREA;
+  S Ld = sp(0);
+  V3 wi;
+  V3 wo = normalize(neg(ray.d));
+  float lightPdf = 0, bsdfPdf = 0;
+  Ray shadow;
+  S Li = light_sample_li(sc, light, is, lu0, lu1, &wi, &lightPdf, &shadow);
+  if (lightPdf > 0 && !black(Li)) {
+    S f = bsdf_f(bs, wi, wo, type) * absdot(wi, is.ns);
+    bsdfPdf = bsdf_pdf(bs, wi, wo, type);
+    if (!black(f)) {
+      if (scene_occluded(sc, shadow, c)) Li = sp(0);
+      if (!black(Li)) {
+        if (delta) Ld += f * Li / lightPdf;
+        else {
+          float weight = power_heuristic(lightPdf, bsdfPdf);
+          Ld += f * Li * weight / lightPdf;
+        }
+      }
+    }
+  }
+  if (!delta) {
+    int stype = 0;
+    S f = bsdf_sample(bs, &wi, wo, bu0, bu1, &bsdfPdf, type, &stype);
+    f *= sp(absdot(wi, is.ns));
+    bool sampledSpecular = (stype & SPEC) != 0;
+    if (!black(f) && bsdfPdf > 0) {
+      float weight = 1;
+      Ray rr = spawn_dir(is, wi);
+      Isect in;
+      bool hit = scene_intersect(sc, rr, &in, c);  // pdf_Li's ray (triangle.cpp:125-127) == the MIS ray
+      if (!sampledSpecular) {
+        // Triangle::pdf, triangle.cpp:123-135
+        if (!hit || in.prim != light.prim) return Ld;
+        float pdf = lensq(sub(is.p, in.p)) / (absdot(in.ng, neg(wi)) * light.area);
+        if (std::isinf(pdf)) pdf = 0;
+        lightPdf = pdf;
+        if (lightPdf == 0) return Ld;
+        weight = power_heuristic(bsdfPdf, lightPdf);
+      }
+      S Lm = sp(0);
+      if (hit) {
+        if (sc.tris[in.prim].light == li) Lm = area_L(light, in.ng, neg(wi));
+      }
+      if (!black(Lm)) Ld += f * Lm * weight / bsdfPdf;
+    }
+  }
+  return Ld;
+}
+// core/integrator.cpp:8-27
+S sample_one_light(const Scene &sc, const Ray &ray, const Isect &is, const BSDF &bs, Sampler &s,
+                   bool power, Counters *c) {
+  int nLights = (int)sc.lights.size();
+  if (nLights == 0) return sp(0);
+  int lightNum;
+  float lightPdf;
+  if (power) {
+    lightNum = sc.power.sample_discrete(s.next(), &lightPdf);
+    if (lightPdf == 0) return sp(0);
+  } else {
+    lightNum = std::min((int)(s.next() * nLights), nLights - 1);
+    lightPdf = 1.0f / nLights;
+  }
+  float lu0, lu1, bu0, bu1;
+  s.next2(&lu0, &lu1);
+  s.next2(&bu0, &bu1);
+  return estimate_direct(sc, ray, is, bs, bu0, bu1, lightNum, lu0, lu1, c) / lightPdf;
+}
+
+inline S isect_Le(const Scene &sc, const Isect &is, V3 w) {  // intersection.cpp:4-8
+  int l = sc.tris[is.prim].light;
+  return l >= 0 ? area_L(sc.lights[l], is.ng, w) : sp(0);
+}
+
+// camera/pinhole.cpp:8-33 (no lens), camera record: o, M(3x3), tanHalf, aspect, W, H
+Ray camera_ray(const Scene &sc, int i, int j, Sampler &s) {
+  float ux, uy;
+  s.next2(&ux, &uy);
+  float rx = (float)i + ux, ry = (float)j + uy;
+  const float *c = sc.cam;
+  float width = c[14], height = c[15];
+  float xx = (2 * rx / width - 1) * c[13] * c[12];
+  float yy = (1 - 2 * ry / height) * c[12];
+  V3 d = normalize(mk(xx, yy, -1.0f));
+  return Ray{mk(c[0], c[1], c[2]), mat3_apply(c + 3, d), 0, kInf};
+}
+
+// integrator/path.cpp:50-99
+S path_Li(const Scene &sc, Ray ray, Sampler &s, int maxDepth, Counters *c) {
+  S L = sp(0), beta = sp(1.0f);
+  bool specularBounce = false;
+  for (int bounces = 0;; ++bounces) {
+    Isect is;
+    bool found = scene_intersect(sc, ray, &is, c);
+    if (bounces == 0 || specularBounce) {
+      if (found) L += beta * isect_Le(sc, is, neg(ray.d));
+    }
+    if (!found || bounces >= maxDepth) break;
+    BSDF bs;
+    make_bsdf(sc, sc.tris[is.prim].mat, is.ng, is.ns, is.u, is.v, false, &bs);
+    if (bsdf_num(bs, ALL & ~SPEC) > 0) {
+      S Ld = beta * sample_one_light(sc, ray, is, bs, s, true, c);
+      L += Ld;
+    }
+    V3 wo = neg(ray.d), wi;
+    float pdf = 0;
+    int type = 0;
+    float u0, u1;
+    s.next2(&u0, &u1);
+    S f = bsdf_sample(bs, &wi, wo, u0, u1, &pdf, ALL, &type);
+    if (black(f) || pdf == 0) break;
+    beta *= f * absdot(wi, is.ns) / pdf;
+    specularBounce = (type & SPEC) != 0;
+    ray = spawn_dir(is, wi);
+    if (bounces > 3) {
+      float q = fmax_ref((float)0.05, 1 - maxc(beta));
+      if (s.next() < q) break;
+      beta = beta / (1 - q);
+    }
+  }
+  return L;
+}
+
+int nthreads(int t) {
+#ifdef _OPENMP
+  return t > 0 ? t : omp_get_max_threads();
+#else
+  (void)t; return 1;
+#endif
+}
+
+// ---------------------------------------------------------------- SPPM
+// sppm.cpp:40-50
+bool to_grid(V3 p, const float blo[3], const float bhi[3], const int res[3], int pi[3]) {
+  bool inBounds = true;
+  V3 o = sub(p, mk(blo[0], blo[1], blo[2]));
+  float pg[3] = {o.x, o.y, o.z};
+  for (int a = 0; a < 3; ++a)
+    if (bhi[a] > blo[a]) pg[a] /= bhi[a] - blo[a];
+  for (int a = 0; a < 3; ++a) {
+    pi[a] = (int)(res[a] * pg[a]);
+    inBounds &= (pi[a] >= 0 && pi[a] < res[a]);
+    pi[a] = pi[a] < 0 ? 0 : (pi[a] > res[a] - 1 ? res[a] - 1 : pi[a]);
+  }
+  return inBounds;
+}
+// sppm.cpp:52-56 (int products wrap; computed in uint32 to keep the bits)
+inline unsigned grid_hash(const int p[3], int hashSize) {
+  unsigned h = ((unsigned)p[0] * 73856093u) ^ ((unsigned)p[1] * 19349663u) ^ ((unsigned)p[2] * 83492791u);
+  return h % (unsigned)hashSize;
+}
+
+struct VisiblePoint { V3 p, wo; BSDF bsdf; S beta; };
+struct Pixel {
+  float radius = 0;
+  S Ld = sp(0);
+  VisiblePoint vp;
+  float Phi[3] = {0, 0, 0};
+  int M = 0;
+  float N = 0;
+  S tau = sp(0);
+};
+
+}  // namespace
+
+// ================================================================= C API
+extern "C" {
+
+void *oracle_scene_create(int n_tris, const float *pn, const int *tri_mat, const int *tri_light,
+                          int n_mats, const float *mats, int n_lights, const float *lights,
+                          const float *camera) {
+  Scene *sc = new Scene();
+  sc->tris.resize(n_tris);
+  for (int i = 0; i < n_tris; ++i) {
+    const float *q = pn + 18 * (size_t)i;
+    Tri &t = sc->tris[i];
+    t.p0 = mk(q[0], q[1], q[2]); t.p1 = mk(q[3], q[4], q[5]); t.p2 = mk(q[6], q[7], q[8]);
+    t.n0 = mk(q[9], q[10], q[11]); t.n1 = mk(q[12], q[13], q[14]); t.n2 = mk(q[15], q[16], q[17]);
+    t.mat = tri_mat[i];
+    t.light = tri_light[i];
+  }
+  sc->mats.resize(n_mats);
+  for (int i = 0; i < n_mats; ++i) {
+    sc->mats[i].type = (int)mats[16 * i];
+    for (int k = 0; k < 15; ++k) sc->mats[i].v[k] = mats[16 * i + 1 + k];
+  }
+  sc->lights.resize(n_lights);
+  std::vector<float> pw(n_lights);
+  for (int i = 0; i < n_lights; ++i) {
+    const float *q = lights + 32 * i;
+    Light &l = sc->lights[i];
+    l.type = (int)q[0];
+    l.prim = (int)q[1];
+    l.L = sp(q[2], q[3], q[4]);
+    l.pos = mk(q[5], q[6], q[7]);
+    l.cosTotal = q[9];
+    l.cosFalloffStart = q[10];
+    for (int k = 0; k < 9; ++k) { l.w2l[k] = q[11 + k]; l.l2w[k] = q[20 + k]; }
+    l.area = l.type == LIGHT_AREA ? tri_area(sc->tris[l.prim]) : 0;
+  }
+  for (int i = 0; i < n_lights; ++i) pw[i] = intensity(light_power(*sc, sc->lights[i]));
+  if (n_lights > 0) sc->power.init(pw.data(), n_lights);  // path.cpp:39-48
+  std::memcpy(sc->cam, camera, 16 * sizeof(float));
+  sc->W = (int)camera[14];
+  sc->H = (int)camera[15];
+  build_bvh(*sc);
+  return sc;
+}
+
+void oracle_scene_destroy(void *s) { delete (Scene *)s; }
+
+int oracle_intersect(void *s, int n, const float *rays, int mode, int brute, float *t, int *prim,
+                     float *u, float *v, int *ties) {
+  const Scene &sc = *(const Scene *)s;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    const float *q = rays + 8 * (size_t)i;
+    Ray r{mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), q[6], q[7]};
+    Hit h{0, 0, 0, -1, false};
+    bool f = brute ? intersect_brute(sc, r, mode == 1, &h) : intersect_bvh(sc, r, mode == 1, &h);
+    t[i] = f ? h.t : 0;
+    prim[i] = f ? h.prim : -1;
+    if (u) u[i] = f ? h.u : 0;
+    if (v) v[i] = f ? h.v : 0;
+    if (ties) ties[i] = f && h.tie ? 1 : 0;
+  }
+  return 0;
+}
+
+int oracle_render_path(void *s, int k0, int k1, int row0, int row1, int maxDepth, float *rgb,
+                       int *count, int threads, int64_t *counters) {
+  const Scene &sc = *(const Scene *)s;
+  int W = sc.W, H = sc.H;
+  int64_t cl = 0, sh = 0, pa = 0;
+  for (int k = k0; k < k1; ++k) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads(threads)) reduction(+ : cl, sh, pa)
+    for (int j = row0; j < row1; ++j) {
+      Counters c;
+      for (int i = 0; i < W; ++i) {
+        Sampler smp((uint64_t)(int64_t)(k * (W * H) + j * W + i));
+        Ray ray = camera_ray(sc, i, j, smp);
+        S L = path_Li(sc, ray, smp, maxDepth, &c);
+        size_t o = (size_t)j * W + i;
+        rgb[3 * o] += L.r; rgb[3 * o + 1] += L.g; rgb[3 * o + 2] += L.b;  // rendertarget.cpp:22-27
+        count[o] += 1;
+        c.paths++;
+      }
+      cl += c.closest; sh += c.shadow; pa += c.paths;
+    }
+  }
+  if (counters) { counters[0] += cl; counters[1] += sh; counters[2] += pa; }
+  return 0;
+}
+
+int oracle_path_samples(void *s, int n, const int *pixel, const int *kk, int maxDepth, float *out) {
+  const Scene &sc = *(const Scene *)s;
+  int W = sc.W, H = sc.H;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int q = 0; q < n; ++q) {
+    int i = pixel[q] % W, j = pixel[q] / W, k = kk[q];
+    Sampler smp((uint64_t)(int64_t)(k * (W * H) + j * W + i));
+    Ray ray = camera_ray(sc, i, j, smp);
+    S L = path_Li(sc, ray, smp, maxDepth, nullptr);
+    out[3 * q] = L.r; out[3 * q + 1] = L.g; out[3 * q + 2] = L.b;
+  }
+  return 0;
+}
+
+int oracle_render_sppm(void *s, int nIterations, int photonPerIteration, float initialSearchRadius,
+                       int maxDepth, int hashSizeArg, float *rgb, int *count, int threads,
+                       int debug_iter, int *vp_cell_count, int *photon_cell_count, int *M_debug,
+                       int *grid_info, float *grid_bounds) {
+  const Scene &sc = *(const Scene *)s;
+  int W = sc.W, H = sc.H;
+  int nPixels = W * H;
+  if (photonPerIteration < 0) photonPerIteration = nPixels;  // sppm.cpp:329-330
+  const int hashSize = hashSizeArg > 0 ? hashSizeArg : nPixels;  // sppm.cpp:147
+  std::vector<Pixel> pixels(nPixels);
+  for (int i = 0; i < nPixels; ++i) pixels[i].radius = initialSearchRadius;
+  int nT = nthreads(threads);
+  for (int iter = 0; iter < nIterations; ++iter) {
+    // camera pass, sppm.cpp:81-142
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nT)
+    for (int j = 0; j < H; ++j) {
+      for (int i = 0; i < W; ++i) {
+        Sampler smp((uint64_t)(int64_t)(iter * nPixels + W * j + i));
+        Ray ray = camera_ray(sc, i, j, smp);
+        S beta = sp(1.0f);
+        Pixel &pixel = pixels[W * j + i];
+        bool specularBounce = false;
+        for (int depth = 0; depth < maxDepth; ++depth) {
+          Isect is;
+          if (!scene_intersect(sc, ray, &is, nullptr)) break;
+          BSDF bs;
+          make_bsdf(sc, sc.tris[is.prim].mat, is.ng, is.ns, is.u, is.v, false, &bs);
+          V3 wo = neg(ray.d);
+          if (depth == 0 || specularBounce) pixel.Ld += beta * isect_Le(sc, is, wo);
+          pixel.Ld += beta * sample_one_light(sc, ray, is, bs, smp, false, nullptr);
+          bool isDiffuse = bsdf_num(bs, DIFF | REFL | TRANS) > 0;
+          bool isGlossy = bsdf_num(bs, GLOSSY | REFL | TRANS) > 0;
+          if (isDiffuse || (isGlossy && depth == maxDepth - 1)) {
+            pixel.vp.p = is.p; pixel.vp.wo = wo; pixel.vp.bsdf = bs; pixel.vp.beta = beta;
+            break;
+          }
+          if (depth < maxDepth - 1) {
+            float pdf = 0;
+            V3 wi;
+            int type = 0;
+            float u0, u1;
+            smp.next2(&u0, &u1);
+            S f = bsdf_sample(bs, &wi, wo, u0, u1, &pdf, ALL, &type);
+            if (pdf == 0. || black(f)) break;
+            specularBounce = (type & SPEC) != 0;
+            beta *= f * absdot(wi, is.ns) / pdf;
+            if (intensity(beta) < 0.25) {
+              float continueProb = fmin_ref((float)1, intensity(beta));
+              if (smp.next() > continueProb) break;
+              beta = beta / continueProb;
+            }
+            ray = spawn_dir(is, wi);
+          }
+        }
+      }
+    }
+    // grid, sppm.cpp:144-197
+    int gridRes[3] = {1, 1, 1};
+    float blo[3] = {kInf, kInf, kInf}, bhi[3] = {std::numeric_limits<float>::lowest(),
+                                                   std::numeric_limits<float>::lowest(),
+                                                   std::numeric_limits<float>::lowest()};
+    float maxRadius = 0.;
+    bool any = false;
+    for (int i = 0; i < nPixels; ++i) {
+      const Pixel &px = pixels[i];
+      if (black(px.vp.beta)) continue;
+      any = true;
+      float r = px.radius;
+      float plo[3] = {px.vp.p.x - r, px.vp.p.y - r, px.vp.p.z - r};
+      float phi[3] = {px.vp.p.x + r, px.vp.p.y + r, px.vp.p.z + r};
+      for (int a = 0; a < 3; ++a) { blo[a] = fmin_ref(blo[a], plo[a]); bhi[a] = fmax_ref(bhi[a], phi[a]); }
+      maxRadius = fmax_ref(maxRadius, px.radius);
+    }
+    std::vector<std::vector<int>> grid;
+    int64_t n_pairs = 0;
+    if (any) {
+      float diag[3] = {bhi[0] - blo[0], bhi[1] - blo[1], bhi[2] - blo[2]};
+      float maxDiag = fmax_ref(diag[0], fmax_ref(diag[1], diag[2]));
+      int baseGridRes = (int)(maxDiag / maxRadius);
+      for (int a = 0; a < 3; ++a) gridRes[a] = std::max((int)(baseGridRes * diag[a] / maxDiag), 1);
+      grid.assign(hashSize, std::vector<int>());
+      for (int pi = 0; pi < nPixels; ++pi) {  // serial: per-cell lists (order irrelevant to counts)
+        const Pixel &px = pixels[pi];
+        if (black(px.vp.beta)) continue;
+        float r = px.radius;
+        int pMin[3], pMax[3];
+        to_grid(sub(px.vp.p, mk(r, r, r)), blo, bhi, gridRes, pMin);
+        to_grid(add(px.vp.p, mk(r, r, r)), blo, bhi, gridRes, pMax);
+        for (int z = pMin[2]; z <= pMax[2]; ++z)
+          for (int y = pMin[1]; y <= pMax[1]; ++y)
+            for (int x = pMin[0]; x <= pMax[0]; ++x) {
+              int c3[3] = {x, y, z};
+              grid[grid_hash(c3, hashSize)].push_back(pi);
+              ++n_pairs;
+            }
+      }
+    }
+    if (iter == debug_iter) {
+      if (vp_cell_count)
+        for (int h = 0; h < hashSize; ++h) vp_cell_count[h] = any ? (int)grid[h].size() : 0;
+      if (grid_info) {
+        grid_info[0] = gridRes[0]; grid_info[1] = gridRes[1]; grid_info[2] = gridRes[2];
+        grid_info[3] = (int)n_pairs;
+      }
+      if (grid_bounds)
+        for (int a = 0; a < 3; ++a) { grid_bounds[a] = blo[a]; grid_bounds[3 + a] = bhi[a]; }
+      if (photon_cell_count) std::memset(photon_cell_count, 0, sizeof(int) * hashSize);
+    }
+    // photon pass, sppm.cpp:199-273
+    const bool dbg = iter == debug_iter && photon_cell_count;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nT)
+    for (int photonIndex = 0; photonIndex < photonPerIteration; ++photonIndex) {
+      if (!any) continue;
+      Sampler smp((uint64_t)(int64_t)(nIterations * nPixels + iter * photonPerIteration + photonIndex));
+      float lightPdf;
+      int lightNum = sc.power.sample_discrete(smp.next(), &lightPdf);
+      const Light &light = sc.lights[lightNum];
+      float a0, a1, c0, c1;
+      smp.next2(&a0, &a1);
+      smp.next2(&c0, &c1);
+      Ray photonRay;
+      V3 nLight;
+      float pdfPos, pdfDir;
+      S Le = light_sample_le(sc, light, a0, a1, c0, c1, &photonRay, &nLight, &pdfPos, &pdfDir);
+      if (pdfPos == 0 || pdfDir == 0 || black(Le)) continue;
+      S beta = Le * absdot(nLight, photonRay.d) / (lightPdf * pdfPos * pdfDir);
+      if (black(beta)) continue;
+      Isect is;
+      for (int depth = 0; depth < maxDepth; ++depth) {
+        if (!scene_intersect(sc, photonRay, &is, nullptr)) break;
+        if (depth > 0) {
+          int cell[3];
+          if (to_grid(is.p, blo, bhi, gridRes, cell)) {
+            unsigned h = grid_hash(cell, hashSize);
+            if (dbg) {
+#pragma omp atomic
+              photon_cell_count[h] += 1;
+            }
+            for (int pi : grid[h]) {
+              Pixel &px = pixels[pi];
+              float radius = px.radius;
+              if (lensq(sub(px.vp.p, is.p)) > radius * radius) continue;
+              V3 wi = neg(photonRay.d);
+              S Phi = beta * bsdf_f(px.vp.bsdf, wi, px.vp.wo, ALL);
+#pragma omp atomic
+              px.Phi[0] += Phi.r;
+#pragma omp atomic
+              px.Phi[1] += Phi.g;
+#pragma omp atomic
+              px.Phi[2] += Phi.b;
+#pragma omp atomic
+              px.M += 1;
+            }
+          }
+        }
+        BSDF bs;
+        make_bsdf(sc, sc.tris[is.prim].mat, is.ng, is.ns, is.u, is.v, true, &bs);
+        V3 wi, wo = neg(photonRay.d);
+        float pdf = 0;
+        int flags = 0;
+        float u0, u1;
+        smp.next2(&u0, &u1);
+        S fr = bsdf_sample(bs, &wi, wo, u0, u1, &pdf, ALL, &flags);
+        if (black(fr) || pdf == 0) break;
+        S bnew = beta * fr * absdot(wi, is.ns) / pdf;
+        float q = fmax_ref((float)0, 1 - intensity(bnew) / intensity(beta));
+        if (smp.next() < q) break;
+        beta = bnew / (1 - q);
+        photonRay = spawn_dir(is, wi);
+      }
+    }
+    if (iter == debug_iter && M_debug)
+      for (int i = 0; i < nPixels; ++i) M_debug[i] = pixels[i].M;
+    // pixel update, sppm.cpp:284-311
+#pragma omp parallel for num_threads(nT)
+    for (int i = 0; i < nPixels; ++i) {
+      Pixel &p = pixels[i];
+      if (p.M > 0) {
+        float gamma = (float)2 / (float)3;
+        float Nnew = p.N + gamma * p.M;
+        float Rnew = p.radius * std::sqrt(Nnew / (p.N + p.M));
+        S Phi = sp(p.Phi[0], p.Phi[1], p.Phi[2]);
+        p.tau = (p.tau + p.vp.beta * Phi) * (Rnew * Rnew) / (p.radius * p.radius);
+        p.N = Nnew;
+        p.radius = Rnew;
+        p.M = 0;
+        p.Phi[0] = p.Phi[1] = p.Phi[2] = 0;
+      }
+      p.vp.beta = sp(0.);
+    }
+    // per-iteration film add, sppm.cpp:313-320
+    uint64_t Np = (uint64_t)nIterations * (uint64_t)photonPerIteration;
+    for (int i = 0; i < nPixels; ++i) {
+      const Pixel &px = pixels[i];
+      S L = px.Ld / (float)nIterations;
+      L += px.tau / (Np * kPi * px.radius * px.radius);
+      rgb[3 * i] += L.r; rgb[3 * i + 1] += L.g; rgb[3 * i + 2] += L.b;
+      count[i] += 1;
+    }
+  }
+  return 0;
+}
+
+float oracle_sobol(uint64_t index, int dim) { return sobol(index, (unsigned)dim); }
+
+void oracle_camera_ray(void *s, int i, int j, uint64_t index, float *out) {
+  const Scene &sc = *(const Scene *)s;
+  Sampler smp(index);
+  Ray r = camera_ray(sc, i, j, smp);
+  out[0] = r.o.x; out[1] = r.o.y; out[2] = r.o.z;
+  out[3] = r.d.x; out[4] = r.d.y; out[5] = r.d.z;
+}
+
+static void frame_bsdf(const Scene &sc, int m, int mode, const float *fr, BSDF *bs) {
+  make_bsdf(sc, m, mk(fr[0], fr[1], fr[2]), mk(fr[3], fr[4], fr[5]), mk(fr[6], fr[7], fr[8]),
+            mk(fr[9], fr[10], fr[11]), mode == 1, bs);
+}
+
+void oracle_bsdf_f_pdf(void *s, int m, int mode, const float *frame, const float *wi,
+                       const float *wo, int type_mask, float *f, float *pdf) {
+  BSDF bs;
+  frame_bsdf(*(const Scene *)s, m, mode, frame, &bs);
+  S r = bsdf_f(bs, mk(wi[0], wi[1], wi[2]), mk(wo[0], wo[1], wo[2]), type_mask);
+  f[0] = r.r; f[1] = r.g; f[2] = r.b;
+  *pdf = bsdf_pdf(bs, mk(wi[0], wi[1], wi[2]), mk(wo[0], wo[1], wo[2]), type_mask);
+}
+
+void oracle_bsdf_sample(void *s, int m, int mode, const float *frame, const float *wo,
+                        const float *u, int type_mask, float *f, float *wi, float *pdf,
+                        int *sampled_type) {
+  BSDF bs;
+  frame_bsdf(*(const Scene *)s, m, mode, frame, &bs);
+  V3 w = mk(0, 0, 0);
+  float p = -1;
+  int t = -1;
+  S r = bsdf_sample(bs, &w, mk(wo[0], wo[1], wo[2]), u[0], u[1], &p, type_mask, &t);
+  f[0] = r.r; f[1] = r.g; f[2] = r.b;
+  wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
+  *pdf = p;
+  *sampled_type = t;
+}
+
+void oracle_light_sample_li(void *s, int l, const float *p, const float *ng, const float *u,
+                            float *out) {
+  const Scene &sc = *(const Scene *)s;
+  Isect is{};
+  is.p = mk(p[0], p[1], p[2]);
+  is.ng = mk(ng[0], ng[1], ng[2]);
+  V3 wi = mk(0, 0, 0);
+  float pdf = 0;
+  Ray sh{mk(0, 0, 0), mk(0, 0, 0), 0, 0};
+  S Li = light_sample_li(sc, sc.lights[l], is, u[0], u[1], &wi, &pdf, &sh);
+  float o[] = {Li.r, Li.g, Li.b, wi.x, wi.y, wi.z, pdf, sh.o.x, sh.o.y, sh.o.z,
+               sh.d.x, sh.d.y, sh.d.z, sh.tmax};
+  std::memcpy(out, o, sizeof(o));
+}
+
+void oracle_light_sample_le(void *s, int l, const float *u1, const float *u2, float *out) {
+  const Scene &sc = *(const Scene *)s;
+  Ray r{mk(0, 0, 0), mk(0, 0, 0), 0, 0};
+  V3 n = mk(0, 0, 0);
+  float pp = 0, pd = 0;
+  S Le = light_sample_le(sc, sc.lights[l], u1[0], u1[1], u2[0], u2[1], &r, &n, &pp, &pd);
+  float o[] = {Le.r, Le.g, Le.b, r.o.x, r.o.y, r.o.z, r.d.x, r.d.y, r.d.z, n.x, n.y, n.z, pp, pd};
+  std::memcpy(out, o, sizeof(o));
+}
+
+float oracle_sin(float x) { return osin(x); }
+float oracle_cos(float x) { return ocos(x); }
+
+}  // extern "C"

@@ -1,0 +1,330 @@
+"""orion_amd: MI355X-native (gfx950) path-tracing + SPPM hot path for orion.
+
+Python access to the C ABI in include/orion_amd.h (liborion_amd.so, built
+in-tree by ``python -m orion_amd.build``).  The library is the product; this
+module only marshals numpy arrays across the boundary.  There is no CPU
+fallback: every render / intersection call runs the HIP kernels and raises
+if the library or a GPU is unavailable.
+"""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "liborion_amd.so"
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class OrionError(RuntimeError):
+    pass
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_tris", C.c_int), ("n_mats", C.c_int), ("n_lights", C.c_int),
+                ("width", C.c_int), ("height", C.c_int), ("integrator", C.c_int),
+                ("max_depth", C.c_int), ("spp", C.c_int), ("sppm_iterations", C.c_int),
+                ("sppm_photons", C.c_int), ("sppm_radius", C.c_float)]
+
+
+class SceneStats(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64), ("n_tris", C.c_int64), ("node_bytes", C.c_int64),
+                ("tri_bytes", C.c_int64), ("shade_bytes", C.c_int64), ("bvh_depth", C.c_int),
+                ("node_format", C.c_int), ("build_ms", C.c_double), ("upload_ms", C.c_double)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", C.c_int64), ("closest_rays", C.c_int64), ("shadow_rays", C.c_int64),
+                ("ref_equiv_rays", C.c_int64), ("photons", C.c_int64), ("deposits", C.c_int64),
+                ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Tuning(C.Structure):
+    _fields_ = [("spp_per_wave", C.c_int), ("block", C.c_int), ("persistent_blocks", C.c_int),
+                ("use_graph", C.c_int)]
+
+
+class SppmDebug(C.Structure):
+    _fields_ = [("debug_iter", C.c_int), ("vp_cell_count", C.c_void_p),
+                ("photon_cell_count", C.c_void_p), ("M", C.c_void_p), ("grid_info", C.c_void_p),
+                ("grid_bounds", C.c_void_p)]
+
+
+# name -> (restype, argtypes); the header's full export list
+_SIGS = {
+    "orion_last_error": (C.c_char_p, []),
+    "orion_version": (C.c_char_p, []),
+    "orion_hscene_load_ori": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "orion_hscene_generate": (C.c_int, [C.c_char_p, C.c_uint64, C.c_void_p, C.c_int,
+                                        C.POINTER(C.c_void_p)]),
+    "orion_hscene_from_records": (C.c_int, [C.c_int, f32p, i32p, i32p, C.c_int, f32p, C.c_int,
+                                            f32p, f32p, C.POINTER(C.c_void_p)]),
+    "orion_hscene_info": (C.c_int, [C.c_void_p, C.POINTER(SceneInfo)]),
+    "orion_hscene_records": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_void_p)] * 6),
+    "orion_hscene_write_ori": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "orion_hscene_destroy": (None, [C.c_void_p]),
+    "orion_camera_record": (C.c_int, [f32p, C.c_float, C.c_int, C.c_int, f32p]),
+    "orion_create_transform": (C.c_int, [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p),
+                                         f32p, f32p]),
+    "orion_light_record": (C.c_int, [C.c_int, f32p, f32p, f32p, C.c_float, C.c_float, f32p]),
+    "orion_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "orion_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "orion_ctx_synchronize": (C.c_int, [C.c_void_p]),
+    "orion_scene_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "orion_scene_destroy": (C.c_int, [C.c_void_p]),
+    "orion_scene_get_stats": (C.c_int, [C.c_void_p, C.POINTER(SceneStats)]),
+    "orion_intersect_closest": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "orion_intersect_any": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "orion_render_path": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Tuning),
+                                    C.POINTER(RenderStats)]),
+    "orion_film_clear": (C.c_int, [C.c_void_p]),
+    "orion_film_read": (C.c_int, [C.c_void_p, f32p, i32p]),
+    "orion_render_sppm": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
+                                    C.POINTER(SppmDebug), C.POINTER(RenderStats)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads liborion_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise OrionError(f"{LIB_PATH} is missing: run `python -m orion_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().orion_last_error()
+        raise OrionError(f"orion_amd error {rc}: {msg.decode() if msg else ''}")
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class HostScene:
+    """Resolved scene records on the host (orion_hscene)."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        self._refresh()
+
+    @classmethod
+    def load_ori(cls, path):
+        h = C.c_void_p()
+        _check(lib().orion_hscene_load_ori(str(path).encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def generate(cls, name, seed=0, params=None):
+        h = C.c_void_p()
+        p = None if params is None else np.ascontiguousarray(params, np.float64)
+        _check(lib().orion_hscene_generate(name.encode(), seed, _ptr(p),
+                                           0 if p is None else len(p), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_records(cls, tri_pn, tri_mat, tri_light, mats, lights, camera):
+        tri_pn = np.ascontiguousarray(tri_pn, np.float32).reshape(-1)
+        tri_mat = np.ascontiguousarray(tri_mat, np.int32)
+        tri_light = np.ascontiguousarray(tri_light, np.int32)
+        mats = np.ascontiguousarray(mats, np.float32).reshape(-1)
+        lights = np.ascontiguousarray(lights, np.float32).reshape(-1)
+        camera = np.ascontiguousarray(camera, np.float32)
+        h = C.c_void_p()
+        _check(lib().orion_hscene_from_records(len(tri_mat), tri_pn, tri_mat, tri_light,
+                                               len(mats) // 16, mats, len(lights) // 32, lights,
+                                               camera, C.byref(h)))
+        return cls(h)
+
+    def _refresh(self):
+        info = SceneInfo()
+        _check(lib().orion_hscene_info(self.h, C.byref(info)))
+        self.info = info
+        ptrs = [C.c_void_p() for _ in range(6)]
+        _check(lib().orion_hscene_records(self.h, *[C.byref(p) for p in ptrs]))
+        n, m, l = info.n_tris, info.n_mats, info.n_lights
+
+        def view(p, count, dt):
+            if count == 0 or not p.value:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dt))),
+                                         shape=(count,)).copy()
+        self.tri_pn = view(ptrs[0], 18 * n, np.float32)
+        self.tri_mat = view(ptrs[1], n, np.int32)
+        self.tri_light = view(ptrs[2], n, np.int32)
+        self.mats = view(ptrs[3], 16 * m, np.float32)
+        self.lights = view(ptrs[4], 32 * l, np.float32)
+        self.camera = view(ptrs[5], 16, np.float32)
+
+    # oracle_lib.OracleScene reads these names
+    @property
+    def n_tris(self):
+        return self.info.n_tris
+
+    @property
+    def n_mats(self):
+        return self.info.n_mats
+
+    @property
+    def n_lights(self):
+        return self.info.n_lights
+
+    @property
+    def width(self):
+        return self.info.width
+
+    @property
+    def height(self):
+        return self.info.height
+
+    def write_ori(self, path, inline_limit=1000):
+        _check(lib().orion_hscene_write_ori(self.h, str(path).encode(), inline_limit))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orion_hscene_destroy(self.h)
+        except Exception:
+            pass
+
+
+def camera_record(lookat, fov, width, height):
+    out = np.zeros(16, np.float32)
+    _check(lib().orion_camera_record(np.asarray(lookat, np.float32), fov, width, height, out))
+    return out
+
+
+def create_transform(pairs):
+    n = len(pairs)
+    keys = (C.c_char_p * max(n, 1))(*[k.encode() for k, _ in pairs])
+    vals = (C.c_char_p * max(n, 1))(*[v.encode() for _, v in pairs])
+    m = np.zeros(16, np.float32)
+    mi = np.zeros(16, np.float32)
+    _check(lib().orion_create_transform(n, keys, vals, m, mi))
+    return m.reshape(4, 4), mi.reshape(4, 4)
+
+
+def light_record(typ, I, frm, to=(0, 0, 1), angle=0.0, delta=0.0):
+    out = np.zeros(32, np.float32)
+    _check(lib().orion_light_record(typ, np.asarray(I, np.float32), np.asarray(frm, np.float32),
+                                    np.asarray(to, np.float32), angle, delta, out))
+    return out
+
+
+class Device:
+    """One GPU + one HIP stream (orion_ctx)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _check(lib().orion_ctx_create(device, C.byref(self.h)))
+
+    def synchronize(self):
+        _check(lib().orion_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            lib().orion_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceScene:
+    """A scene resident in HBM (BVH, triangles, shading data, film)."""
+
+    def __init__(self, dev, hscene):
+        self.dev = dev
+        self.hs = hscene
+        self.W, self.H = hscene.width, hscene.height
+        self.h = C.c_void_p()
+        _check(lib().orion_scene_upload(dev.h, hscene.h, C.byref(self.h)))
+
+    def stats(self):
+        st = SceneStats()
+        _check(lib().orion_scene_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def intersect_closest(self, rays_soa):
+        rays = np.ascontiguousarray(rays_soa, np.float32)
+        n = rays.shape[1]
+        t = np.zeros(n, np.float32); prim = np.zeros(n, np.int32)
+        u = np.zeros(n, np.float32); v = np.zeros(n, np.float32); tie = np.zeros(n, np.int32)
+        _check(lib().orion_intersect_closest(self.h, n, _ptr(rays), _ptr(t), _ptr(prim), _ptr(u),
+                                             _ptr(v), _ptr(tie), 0))
+        return t, prim, u, v, tie
+
+    def intersect_any(self, rays_soa):
+        rays = np.ascontiguousarray(rays_soa, np.float32)
+        n = rays.shape[1]
+        occ = np.zeros(n, np.int32)
+        _check(lib().orion_intersect_any(self.h, n, _ptr(rays), _ptr(occ), 0))
+        return occ
+
+    def render_path(self, k0, k1, max_depth, spp_per_wave=0, persistent_blocks=0):
+        st = RenderStats()
+        tune = Tuning(spp_per_wave, 0, persistent_blocks, 0)
+        _check(lib().orion_render_path(self.h, k0, k1, max_depth, C.byref(tune), C.byref(st)))
+        return st
+
+    def film_clear(self):
+        _check(lib().orion_film_clear(self.h))
+
+    def film(self):
+        rgb = np.zeros(self.W * self.H * 3, np.float32)
+        cnt = np.zeros(self.W * self.H, np.int32)
+        _check(lib().orion_film_read(self.h, rgb, cnt))
+        return rgb.reshape(self.H, self.W, 3), cnt.reshape(self.H, self.W)
+
+    def render_sppm(self, iters, photons, radius, max_depth, hash_size=0, debug_iter=-1):
+        W, H = self.W, self.H
+        hs = hash_size if hash_size > 0 else W * H
+        dbg = {}
+        dptr = None
+        if debug_iter >= 0:
+            dbg = dict(vp_cell_count=np.zeros(hs, np.int32), photon_cell_count=np.zeros(hs, np.int32),
+                       M=np.zeros(W * H, np.int32), grid_info=np.zeros(8, np.int32),
+                       grid_bounds=np.zeros(6, np.float32))
+            d = SppmDebug(debug_iter, dbg["vp_cell_count"].ctypes.data,
+                          dbg["photon_cell_count"].ctypes.data, dbg["M"].ctypes.data,
+                          dbg["grid_info"].ctypes.data, dbg["grid_bounds"].ctypes.data)
+            dptr = C.byref(d)
+        st = RenderStats()
+        _check(lib().orion_render_sppm(self.h, iters, photons, radius, max_depth, hash_size, dptr,
+                                       C.byref(st)))
+        return st, dbg
+
+    def close(self):
+        if self.h:
+            lib().orion_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,82 @@
+"""Builds liborion_amd.so (HIP for gfx950 + host C++) in-tree, and the oracle.
+
+    python -m orion_amd.build            # library + oracle (+ oracle/_ref if the reference is present)
+
+Compiler rules: every HIP and host translation unit is compiled with
+-ffp-contract=off (no FMA contraction: the float operation sequence must equal
+the CPU one), HIP code additionally with packed-FP32 instructions disabled.
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+BUILD = PKG / "_build"
+LIB = PKG / "liborion_amd.so"
+
+HIP_SOURCES = ["runtime.hip", "path_kernels.hip", "sppm_kernels.hip"]
+CPP_SOURCES = ["api_host.cpp", "host_records.cpp", "ori_loader.cpp", "scenegen.cpp", "bvh.cpp"]
+
+ARCH = os.environ.get("ORION_AMD_ARCH", "gfx950")
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+          "-Wno-unused-variable"]
+HIP_FLAGS = COMMON + [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+                      "-packed-fp32-ops", "-munsafe-fp-atomics"]
+
+
+def _hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found (ROCm is required to build orion_amd)")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}\n{r.stdout}")
+    return r.stdout
+
+
+def build_library(verbose=False):
+    hipcc = _hipcc()
+    BUILD.mkdir(exist_ok=True)
+    jobs = []
+    for s in HIP_SOURCES:
+        jobs.append([hipcc, *HIP_FLAGS, "-c", str(CSRC / s), "-o", str(BUILD / (s + ".o"))])
+    for s in CPP_SOURCES:
+        jobs.append(["g++", *COMMON, "-O2", "-c", str(CSRC / s), "-o", str(BUILD / (s + ".o"))])
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
+        outs = list(ex.map(_run, jobs))
+    if verbose:
+        for o in outs:
+            if o.strip():
+                print(o)
+    objs = [str(BUILD / (s + ".o")) for s in HIP_SOURCES + CPP_SOURCES]
+    tmp = str(LIB) + ".tmp"
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lpthread"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    """Compiles the oracle (test infrastructure) and, where the reference tree
+    exists, oracle/_ref from the reference's own sources."""
+    _run(["make", "-s", "-C", str(ROOT / "oracle"), "oracle"])
+    if Path("/root/reference/src/sampler/sobol.h").exists():
+        _run(["make", "-s", "-j8", "-C", str(ROOT / "oracle"), "ref"])
+
+
+def main():
+    build_library(verbose="-v" in sys.argv)
+    build_oracle()
+    print(f"built {LIB}")
+
+
+if __name__ == "__main__":
+    main()

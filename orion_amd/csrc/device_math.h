@@ -1,0 +1,415 @@
+// Device-side shading arithmetic for the gfx950 path (float, no FMA
+// contraction: the library is compiled with -ffp-contract=off).  Every
+// function follows the reference operation by operation -- association
+// order, v / s as v * (1 / s) for vectors but true division for spectra,
+// std::min/std::max argument order -- so a GPU sample path is the same
+// float sequence as orion's.  References are to the reference's src/.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace orion_amd {
+namespace dev {
+
+constexpr float kEps = 0.000001f;            // orion.h:48
+constexpr float kShadowEps = 0.01f;          // orion.h:49
+constexpr float kFltMax = 3.40282347e+38f;   // orion.h:51 fInfinity
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = 0.31830988618379067154f;
+constexpr float kInv4Pi = 0.07957747154594766788f;
+constexpr float kOneMinusEps = (float)0.99999994;  // bsdf.cpp:15
+enum { REFL = 1, TRANS = 2, DIFF = 4, GLOSSY = 8, SPEC = 16, ALL = 31 };
+
+__device__ __forceinline__ float mn(float a, float b) { return (b < a) ? b : a; }  // std::min
+__device__ __forceinline__ float mx(float a, float b) { return (a < b) ? b : a; }  // std::max
+__device__ __forceinline__ bool is_inf(float x) { return __builtin_isinf(x); }
+
+// sin / cos of a float argument evaluated in FP64 and rounded once: the
+// correctly rounded float except in ~1e-8 of inputs (the CPU side does the
+// same with libm's double routines, so sample paths agree).
+__device__ __forceinline__ float dsin(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float dcos(float x) { return (float)cos((double)x); }
+
+struct V3 { float x, y, z; };
+__device__ __forceinline__ V3 mk(float x, float y, float z) { V3 r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 neg(V3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ V3 mul(V3 a, float s) { return mk(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ V3 divs(V3 a, float f) { float inv = 1.0f / f; return mk(a.x * inv, a.y * inv, a.z * inv); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float absdot(V3 a, V3 b) { return fabsf(a.x * b.x + a.y * b.y + a.z * b.z); }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+  return mk((a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x));
+}
+__device__ __forceinline__ float lensq(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+__device__ __forceinline__ V3 normalize(V3 a) { return divs(a, sqrtf(lensq(a))); }
+
+struct Sp { float r, g, b; };
+__device__ __forceinline__ Sp sp(float v) { Sp s; s.r = v; s.g = v; s.b = v; return s; }
+__device__ __forceinline__ Sp sp3(float r, float g, float b) { Sp s; s.r = r; s.g = g; s.b = b; return s; }
+__device__ __forceinline__ Sp operator+(Sp a, Sp b) { return sp3(b.r + a.r, b.g + a.g, b.b + a.b); }
+__device__ __forceinline__ Sp operator-(Sp a, Sp b) { return sp3(a.r - b.r, a.g - b.g, a.b - b.b); }
+__device__ __forceinline__ Sp operator*(Sp a, Sp b) { return sp3(b.r * a.r, b.g * a.g, b.b * a.b); }
+__device__ __forceinline__ Sp operator/(Sp a, Sp b) { return sp3(a.r / b.r, a.g / b.g, a.b / b.b); }
+__device__ __forceinline__ Sp operator*(Sp a, float f) { return sp3(a.r * f, a.g * f, a.b * f); }
+__device__ __forceinline__ Sp operator/(Sp a, float f) { return sp3(a.r / f, a.g / f, a.b / f); }
+__device__ __forceinline__ Sp operator+(Sp a, float f) { return sp3(a.r + f, a.g + f, a.b + f); }
+__device__ __forceinline__ Sp operator-(Sp a, float f) { return sp3(a.r - f, a.g - f, a.b - f); }
+__device__ __forceinline__ void acc(Sp &a, Sp b) { a.r += b.r; a.g += b.g; a.b += b.b; }
+__device__ __forceinline__ void mulby(Sp &a, Sp b) { a.r *= b.r; a.g *= b.g; a.b *= b.b; }
+__device__ __forceinline__ bool black(Sp a) { return !(a.r != 0.0f || a.g != 0.0f || a.b != 0.0f); }
+__device__ __forceinline__ float maxc(Sp a) { return mx(a.r, mx(a.g, a.b)); }
+__device__ __forceinline__ float intensity(Sp a) { return 0.212671f * a.r + 0.715160f * a.g + 0.072169f * a.b; }
+__device__ __forceinline__ Sp ssqrt(Sp a) { return sp3(sqrtf(a.r), sqrtf(a.g), sqrtf(a.b)); }
+__device__ __forceinline__ float clampf(float f, float lo, float hi) { return f < lo ? lo : (f > hi ? hi : f); }
+
+// ---------------------------------------------------------------- sampling (sampling.cpp)
+__device__ __forceinline__ V3 cosine_hemisphere(float u0, float u1) {
+  float r = sqrtf(u0);
+  float theta = 2 * kPi * u1;
+  float x = r * dcos(theta), y = r * dsin(theta);
+  float z = sqrtf(mx(0.0f, 1 - x * x - y * y));
+  return mk(x, y, z);
+}
+__device__ __forceinline__ V3 uniform_sphere(float u0, float u1) {
+  float z = 1 - 2 * u0;
+  float r = sqrtf(mx(0.0f, 1.0f - z * z));
+  float phi = 2 * kPi * u1;
+  return mk(r * dcos(phi), r * dsin(phi), z);
+}
+__device__ __forceinline__ V3 uniform_cone(float u0, float u1, float cosThetaMax) {
+  float cosTheta = (1.0f - u0) + u0 * cosThetaMax;
+  float sinTheta = sqrtf(1.0f - cosTheta * cosTheta);
+  float phi = u1 * 2 * kPi;
+  return mk(dcos(phi) * sinTheta, dsin(phi) * sinTheta, cosTheta);
+}
+__device__ __forceinline__ float power_heuristic(float fPdf, float gPdf) {
+  float f = 1 * fPdf, g = 1 * gPdf;
+  return (f * f) / (f * f + g * g);
+}
+
+// ---------------------------------------------------------------- ray spawning (geometry.h:824-863)
+__device__ __forceinline__ float next_up(float v) {
+  if (is_inf(v) && v > 0.f) return v;
+  if (v == -0.f) v = 0.f;
+  uint32_t ui = __float_as_uint(v);
+  if (v >= 0) ++ui; else --ui;
+  return __uint_as_float(ui);
+}
+__device__ __forceinline__ float next_down(float v) {
+  if (is_inf(v) && v < 0.f) return v;
+  if (v == 0.f) v = -0.f;
+  uint32_t ui = __float_as_uint(v);
+  if (v > 0) --ui; else ++ui;
+  return __uint_as_float(ui);
+}
+__device__ __forceinline__ V3 offset_ray_origin(V3 p, V3 n, V3 w) {
+  float maxValue = mx(mx(fabsf(p.x), fabsf(p.y)), fabsf(p.z));
+  float c = maxValue * kEps;
+  float d = fabsf(n.x) * c + fabsf(n.y) * c + fabsf(n.z) * c;
+  V3 off = mul(n, d);
+  if (dot(w, n) < 0) off = neg(off);
+  V3 po = add(p, off);
+  if (off.x > 0) po.x = next_up(po.x); else if (off.x < 0) po.x = next_down(po.x);
+  if (off.y > 0) po.y = next_up(po.y); else if (off.y < 0) po.y = next_down(po.y);
+  if (off.z > 0) po.z = next_up(po.z); else if (off.z < 0) po.z = next_down(po.z);
+  return po;
+}
+__device__ __forceinline__ void coordinate_system(V3 z, V3 *x, V3 *y) {
+  if (fabsf(z.x) > fabsf(z.y)) *y = divs(mk(-z.z, 0, z.x), sqrtf(z.x * z.x + z.z * z.z));
+  else *y = divs(mk(0, z.z, -z.y), sqrtf(z.y * z.y + z.z * z.z));
+  *x = cross(*y, z);
+}
+__device__ __forceinline__ V3 mat3(const float *m, V3 v) {
+  return mk(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+            m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+
+// ---------------------------------------------------------------- BSDFs
+// A material record resolves to at most two lobes (Matte: Lambert; Metal:
+// GGX reflection with a conductor Fresnel; Glass: FresnelSpecular, or GGX
+// reflection + GGX transmission when rough).
+enum { BX_LAMBERT = 0, BX_MF_REFL = 1, BX_MF_TRANS = 2, BX_FRESNEL_SPEC = 3 };
+enum { FR_CONDUCTOR = 0, FR_DIELECTRIC = 1 };
+struct Lobe {
+  int kind, type, fresnel, photon;
+  Sp R, T, etaT, k;
+  float alpha, etaA, etaB;
+};
+struct Bsdf {
+  V3 ng, ns, u, v;
+  int n;
+  Lobe b[2];
+  __device__ __forceinline__ V3 to_local(V3 w) const { return mk(dot(w, u), dot(w, v), dot(w, ns)); }
+  __device__ __forceinline__ V3 to_world(V3 w) const {
+    return mk(u.x * w.x + v.x * w.y + ns.x * w.z, u.y * w.x + v.y * w.y + ns.y * w.z,
+              u.z * w.x + v.z * w.y + ns.z * w.z);
+  }
+};
+__device__ __forceinline__ bool match(const Lobe &b, int t) { return (b.type & t) == b.type; }
+
+__device__ inline Sp fr_conductor(float cosThetaI, Sp etai, Sp etat, Sp k) {  // bsdf.cpp:124-147
+  cosThetaI = clampf(cosThetaI, -1.0f, 1.0f);
+  Sp eta = etat / etai;
+  Sp etak = k / etai;
+  float cos2 = cosThetaI * cosThetaI;
+  float sin2 = 1.0f - cos2;
+  Sp eta2 = eta * eta;
+  Sp etak2 = etak * etak;
+  Sp t0 = eta2 - etak2 - sin2;
+  Sp a2plusb2 = ssqrt(t0 * t0 + eta2 * 4.0f * etak2);
+  Sp t1 = a2plusb2 + cos2;
+  Sp a = ssqrt((a2plusb2 + t0) * 0.5f);
+  Sp t2 = a * (2.0f * cosThetaI);
+  Sp Rs = (t1 - t2) / (t1 + t2);
+  Sp t3 = a2plusb2 * cos2 + sin2 * sin2;
+  Sp t4 = t2 * sin2;
+  Sp Rp = Rs * (t3 - t4) / (t3 + t4);
+  return (Rp + Rs) * 0.5f;
+}
+__device__ inline float fr_dielectric(float cosThetaI, float etaI, float etaT) {  // bsdf.cpp:149-170
+  cosThetaI = clampf(cosThetaI, -1.0f, 1.0f);
+  bool entering = cosThetaI > 0.f;
+  if (!entering) { float t = etaI; etaI = etaT; etaT = t; cosThetaI = fabsf(cosThetaI); }
+  float sinThetaI = sqrtf(mx(0.0f, 1 - cosThetaI * cosThetaI));
+  float sinThetaT = etaI / etaT * sinThetaI;
+  if (sinThetaT >= 1) return 1;
+  float cosThetaT = sqrtf(mx(0.0f, 1 - sinThetaT * sinThetaT));
+  float Rparl = ((etaT * cosThetaI) - (etaI * cosThetaT)) / ((etaT * cosThetaI) + (etaI * cosThetaT));
+  float Rperp = ((etaI * cosThetaI) - (etaT * cosThetaT)) / ((etaI * cosThetaI) + (etaT * cosThetaT));
+  return (Rparl * Rparl + Rperp * Rperp) / 2;
+}
+__device__ __forceinline__ Sp fresnel_eval(const Lobe &b, float cosI) {  // |cos| passed, bsdf.cpp:172-180
+  if (b.fresnel == FR_CONDUCTOR) return fr_conductor(fabsf(cosI), sp(1.0f), b.etaT, b.k);
+  return sp(fr_dielectric(fabsf(cosI), b.etaA, b.etaB));
+}
+
+__device__ __forceinline__ float cos2t(V3 w) { return w.z * w.z; }
+__device__ __forceinline__ float sin2t(V3 w) { return mx(0.0f, 1.0f - cos2t(w)); }
+__device__ __forceinline__ bool same_hemi(V3 a, V3 b) { return a.z * b.z > 0; }
+__device__ inline float ggx_D(float alpha, V3 wh) {  // microfacet.cpp:9-18
+  float tan2Th = sin2t(wh) / cos2t(wh);
+  if (is_inf(tan2Th)) return 0;
+  float cos4Th = cos2t(wh) * cos2t(wh);
+  float alpha2 = alpha * alpha;
+  float e = tan2Th / alpha2;
+  return 1 / (kPi * alpha2 * cos4Th * (1 + e) * (1 + e));
+}
+__device__ inline float ggx_lambda(float alpha, V3 w) {  // microfacet.cpp:20-26
+  float absTanTheta = fabsf(sqrtf(sin2t(w)) / w.z);
+  if (is_inf(absTanTheta)) return 0.;
+  float a2t2 = (alpha * absTanTheta) * (alpha * absTanTheta);
+  return (-1 + sqrtf(1.f + a2t2)) / 2;
+}
+__device__ __forceinline__ float ggx_G(float alpha, V3 wi, V3 wo) {
+  return 1 / (1 + ggx_lambda(alpha, wi) + ggx_lambda(alpha, wo));
+}
+__device__ __forceinline__ float ggx_pdf(float alpha, V3 wh) { return ggx_D(alpha, wh) * fabsf(wh.z); }
+__device__ inline V3 ggx_sample_wh(float alpha, V3 wo, float u0, float u1) {  // microfacet.cpp:28-40
+  float tanTheta2 = alpha * alpha * u0 / (1.0f - u0);
+  float cosTheta = 1 / sqrtf(1 + tanTheta2);
+  float sinTheta = sqrtf(mx(0.0f, 1.0f - cosTheta * cosTheta));
+  float phi = (2 * kPi) * u1;
+  V3 wh = mk(sinTheta * dcos(phi), sinTheta * dsin(phi), cosTheta);
+  if (!same_hemi(wo, wh)) wh = neg(wh);
+  return wh;
+}
+__device__ __forceinline__ V3 reflect(V3 wo, V3 n) { return add(neg(wo), mul(n, 2 * dot(wo, n))); }
+__device__ __forceinline__ bool refract(V3 wi, V3 n, float eta, V3 *wt) {  // bsdf.h:105-115
+  float cosThetaI = dot(n, wi);
+  float sin2ThetaI = mx(0.0f, 1 - cosThetaI * cosThetaI);
+  float sin2ThetaT = eta * eta * sin2ThetaI;
+  if (sin2ThetaT >= 1) return false;
+  float cosThetaT = sqrtf(1 - sin2ThetaT);
+  *wt = add(mul(neg(wi), eta), mul(n, eta * cosThetaI - cosThetaT));
+  return true;
+}
+
+__device__ inline Sp lobe_f(const Lobe &b, V3 wi, V3 wo) {
+  if (b.kind == BX_LAMBERT) return b.R * kInvPi;  // lambert.cpp:5-8
+  if (b.kind == BX_MF_REFL) {                      // microfacet.cpp:55-66 (G(wi, wi) as in the reference)
+    float cosThetaO = fabsf(wo.z), cosThetaI = fabsf(wi.z);
+    if (cosThetaI == 0 || cosThetaO == 0) return sp(0);
+    V3 wh = add(wi, wo);
+    if (wh.x == 0 && wh.y == 0 && wh.z == 0) return sp(0);
+    wh = normalize(wh);
+    Sp F = fresnel_eval(b, dot(wi, wh));
+    return b.R * ggx_D(b.alpha, wh) * ggx_G(b.alpha, wi, wi) * F / (4 * cosThetaI * cosThetaO);
+  }
+  if (b.kind == BX_MF_TRANS) {  // microfacet.cpp:87-106
+    if (same_hemi(wi, wo)) return sp(0);
+    float cosThetaO = wo.z, cosThetaI = wi.z;
+    if (cosThetaI == 0 || cosThetaO == 0) return sp(0);
+    float eta = wo.z > 0 ? (b.etaB / b.etaA) : (b.etaA / b.etaB);
+    V3 wh = normalize(add(wo, mul(wi, eta)));
+    if (wh.z < 0) wh = neg(wh);
+    Sp F = sp(fr_dielectric(fabsf(dot(wo, wh)), b.etaA, b.etaB));
+    float sqrtDenom = dot(wo, wh) + eta * dot(wi, wh);
+    float factor = b.photon ? 1 : (1 / eta);
+    return (sp(1.f) - F) * b.T *
+           fabsf(ggx_D(b.alpha, wh) * ggx_G(b.alpha, wi, wo) * eta * eta * absdot(wi, wh) * absdot(wo, wh) *
+                 factor * factor / (cosThetaI * cosThetaO * sqrtDenom * sqrtDenom));
+  }
+  return sp(0);
+}
+__device__ inline float lobe_pdf(const Lobe &b, V3 wi, V3 wo) {
+  if (b.kind == BX_LAMBERT) return same_hemi(wi, wo) ? fabsf(wi.z) * kInvPi : 0;
+  if (b.kind == BX_MF_REFL) {
+    if (!same_hemi(wi, wo)) return 0;
+    V3 wh = normalize(add(wi, wo));
+    return ggx_pdf(b.alpha, wh) / (4 * dot(wo, wh));
+  }
+  if (b.kind == BX_MF_TRANS) {
+    if (same_hemi(wo, wi)) return 0;
+    float eta = wo.z > 0 ? (b.etaB / b.etaA) : (b.etaA / b.etaB);
+    V3 wh = normalize(add(wo, mul(wi, eta)));
+    float sqrtDenom = dot(wo, wh) + eta * dot(wi, wh);
+    float dwh_dwi = fabsf((eta * eta * dot(wi, wh)) / (sqrtDenom * sqrtDenom));
+    return ggx_pdf(b.alpha, wh) * dwh_dwi;
+  }
+  return 0;
+}
+__device__ inline Sp lobe_sample(const Lobe &b, V3 wo, float u0, float u1, V3 *wi, float *pdf, int *stype) {
+  if (b.kind == BX_LAMBERT) {  // bsdf.cpp:109-115
+    *wi = cosine_hemisphere(u0, u1);
+    if (wo.z < 0) wi->z *= -1;
+    *pdf = lobe_pdf(b, *wi, wo);
+    return lobe_f(b, *wi, wo);
+  }
+  if (b.kind == BX_MF_REFL) {  // microfacet.cpp:42-53
+    if (wo.z == 0) return sp(0);
+    V3 wh = ggx_sample_wh(b.alpha, wo, u0, u1);
+    *wi = reflect(wo, wh);
+    if (!same_hemi(*wi, wo)) return sp(0);
+    *pdf = ggx_pdf(b.alpha, wh) / (4 * dot(wo, wh));
+    return lobe_f(b, *wi, wo);
+  }
+  if (b.kind == BX_MF_TRANS) {  // microfacet.cpp:76-85: *pdf is never written
+    if (wo.z == 0) return sp(0);
+    V3 wh = ggx_sample_wh(b.alpha, wo, u0, u1);
+    float eta = wo.z > 0 ? (b.etaA / b.etaB) : (b.etaB / b.etaA);
+    if (!refract(wo, wh, eta, wi)) return sp(0);
+    return lobe_f(b, *wi, wo);
+  }
+  // FresnelSpecular, specular.cpp:28-54
+  float F = fr_dielectric(wo.z, b.etaA, b.etaB);
+  if (u0 < F) {
+    *wi = mk(-wo.x, -wo.y, wo.z);
+    *stype = SPEC | REFL;
+    *pdf = F;
+    return b.R * F / fabsf(wi->z);
+  }
+  bool entering = wo.z > 0;
+  float etaI = entering ? b.etaA : b.etaB;
+  float etaT = entering ? b.etaB : b.etaA;
+  V3 n = mk(0, 0, 1);
+  if (dot(n, wo) < 0.f) n = neg(n);
+  if (!refract(wo, n, etaI / etaT, wi)) return sp(0);
+  Sp ft = b.T * (1 - F);
+  if (!b.photon) mulby(ft, sp((etaI * etaI) / (etaT * etaT)));
+  *stype = SPEC | TRANS;
+  *pdf = 1 - F;
+  return ft / fabsf(wi->z);
+}
+
+__device__ __forceinline__ int bsdf_num(const Bsdf &bs, int type) {
+  int n = 0;
+  for (int i = 0; i < bs.n; ++i) n += match(bs.b[i], type) ? 1 : 0;
+  return n;
+}
+__device__ inline Sp bsdf_f(const Bsdf &bs, V3 wi, V3 wo, int type) {  // bsdf.cpp:17-29
+  Sp r = sp(0);
+  V3 swi = bs.to_local(wi), swo = bs.to_local(wo);
+  for (int i = 0; i < bs.n; ++i)
+    if (match(bs.b[i], type)) acc(r, lobe_f(bs.b[i], swi, swo));
+  return r;
+}
+__device__ inline float bsdf_pdf(const Bsdf &bs, V3 wi, V3 wo, int type) {  // bsdf.cpp:85-103
+  if (bs.n == 0) return 0;
+  V3 swi = bs.to_local(wi), swo = bs.to_local(wo);
+  if (swo.z == 0) return 0;
+  float pdf = 0;
+  int m = 0;
+  for (int i = 0; i < bs.n; ++i)
+    if (match(bs.b[i], type)) { ++m; pdf += lobe_pdf(bs.b[i], swi, swo); }
+  return m > 0 ? pdf / m : 0.f;
+}
+// bsdf.cpp:31-83; *pdf and *stype stay untouched on the swo.z == 0 exit
+__device__ inline Sp bsdf_sample(const Bsdf &bs, V3 *wi, V3 wo, float u0, float u1, float *pdf, int type, int *stype) {
+  int matching = bsdf_num(bs, type);
+  if (matching == 0) { *pdf = 0; *stype = 0; return sp(0); }
+  int pos = min((int)floorf(u0 * matching), matching - 1);
+  int count = pos, sel = 0;
+  for (int i = 0; i < bs.n; ++i)
+    if (match(bs.b[i], type) && count-- == 0) { sel = i; break; }
+  const Lobe &bx = bs.b[sel];
+  float nu0 = mn(u0 * matching - pos, kOneMinusEps);
+  V3 swi = mk(0, 0, 0), swo = bs.to_local(wo);
+  if (swo.z == 0) return sp(0);
+  *pdf = 0;
+  *stype = bx.type;
+  Sp f = lobe_sample(bx, swo, nu0, u1, &swi, pdf, stype);
+  if (*pdf == 0) { *stype = 0; return sp(0); }
+  *wi = bs.to_world(swi);
+  bool hasSpecular = (bx.type & SPEC) != 0;
+  if (!hasSpecular && matching > 1)
+    for (int i = 0; i < bs.n; ++i)
+      if (i != sel && match(bs.b[i], type)) *pdf += lobe_pdf(bs.b[i], swi, swo);
+  if (matching > 1) *pdf /= matching;
+  if (!hasSpecular && matching > 1) {
+    bool refl = dot(*wi, bs.ng) * dot(wo, bs.ng) > 0;
+    f = sp(0);
+    for (int i = 0; i < bs.n; ++i) {
+      bool hr = (bs.b[i].type & REFL) != 0, ht = (bs.b[i].type & TRANS) != 0;
+      if (match(bs.b[i], type) && ((refl && hr) || (!refl && ht))) acc(f, lobe_f(bs.b[i], swi, swo));
+    }
+  }
+  return f;
+}
+
+// Material record -> lobes (matte.cpp:9-21, metal.cpp:8-17, glass.cpp:9-37);
+// alpha comes precomputed in the record.
+__device__ inline void make_bsdf(const float *m, V3 ng, V3 ns, V3 u, V3 v, bool photon, Bsdf *bs) {
+  bs->ng = ng; bs->ns = ns; bs->u = u; bs->v = v; bs->n = 0;
+  int type = (int)m[0];
+  if (type == 0) {
+    Sp r = sp3(clampf(m[1], 0, kFltMax), clampf(m[2], 0, kFltMax), clampf(m[3], 0, kFltMax));
+    if (!black(r)) {
+      Lobe &b = bs->b[bs->n++];
+      b.kind = BX_LAMBERT; b.type = REFL | DIFF; b.R = r;
+      b.fresnel = 0; b.photon = photon; b.alpha = 0; b.etaA = b.etaB = 0;
+    }
+  } else if (type == 1) {
+    Lobe &b = bs->b[bs->n++];
+    b.kind = BX_MF_REFL; b.type = REFL | GLOSSY; b.R = sp(1.0f);
+    b.alpha = m[8];
+    b.fresnel = FR_CONDUCTOR; b.photon = photon;
+    b.etaT = sp3(m[1], m[2], m[3]);
+    b.k = sp3(m[4], m[5], m[6]);
+    b.etaA = b.etaB = 0;
+  } else {
+    Sp R = sp3(m[1], m[2], m[3]), T = sp3(m[4], m[5], m[6]);
+    float rough = m[7], alpha = m[8], e = m[9];
+    if (black(R) && black(T)) return;
+    if (rough == 0) {
+      Lobe &b = bs->b[bs->n++];
+      b.kind = BX_FRESNEL_SPEC; b.type = REFL | TRANS | SPEC;
+      b.R = R; b.T = T; b.etaA = 1.0f; b.etaB = e; b.photon = photon; b.fresnel = FR_DIELECTRIC; b.alpha = 0;
+    } else {
+      if (!black(R)) {
+        Lobe &b = bs->b[bs->n++];
+        b.kind = BX_MF_REFL; b.type = REFL | GLOSSY; b.R = R; b.alpha = alpha;
+        b.fresnel = FR_DIELECTRIC; b.etaA = 1.0f; b.etaB = e; b.photon = photon;
+      }
+      if (!black(T)) {
+        Lobe &b = bs->b[bs->n++];
+        b.kind = BX_MF_TRANS; b.type = TRANS | GLOSSY; b.T = T; b.alpha = alpha;
+        b.etaA = 1.0f; b.etaB = e; b.photon = photon; b.fresnel = FR_DIELECTRIC;
+      }
+    }
+  }
+}
+
+}  // namespace dev
+}  // namespace orion_amd

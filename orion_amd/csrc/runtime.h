@@ -1,0 +1,59 @@
+// Runtime objects behind the C ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/orion_amd.h"
+#include "bvh.h"
+#include "device_scene.h"
+#include "wavefront.h"
+
+namespace orion_amd {
+
+int dev_alloc(void **p, size_t bytes);
+
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  int ensure(size_t b);
+  void release();
+};
+
+struct WaveBuffers {
+  DevBuf mem;
+  int cap = 0;
+  Wave wave{};
+  int ensure(int N);
+};
+
+struct SppmBuffers;
+void destroy_sppm(SppmBuffers *b);
+
+}  // namespace orion_amd
+
+struct orion_ctx_s {
+  int device = 0;
+  int n_cu = 0;
+  std::string arch;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct orion_scene_s {
+  orion_ctx ctx = nullptr;
+  int W = 0, H = 0;
+  orion_amd::Bvh bvh;
+  orion_amd::DevBuf nodes, tris, shade, mats, lights, light_cdf, light_func, light_area, sobol;
+  orion_amd::DevBuf film, count, ctrl, stats;
+  orion_amd::DScene d{};
+  orion_amd::WaveBuffers wave;
+  orion_amd::SppmBuffers *sppm = nullptr;
+  double build_ms = 0, upload_ms = 0;
+  unsigned long long last_nodes = 0, last_tris = 0;
+  ~orion_scene_s();
+};

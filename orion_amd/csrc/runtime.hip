@@ -1,0 +1,428 @@
+// Device runtime: contexts, scene upload (BVH + shading data + Sobol table
+// in HBM), the wavefront render loops and the device half of the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/orion_amd.h"
+#include "bvh.h"
+#include "host_scene.h"
+#include "kernels.h"
+#include "runtime.h"
+
+#define ORION_SOBOL_STORAGE static const
+#include "sobol_matrices.inc"
+
+namespace orion_amd {
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                     \
+      return ORION_ERR_HIP;                                                             \
+    }                                                                                   \
+  } while (0)
+
+int dev_alloc(void **p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? ORION_ERR_OOM : ORION_ERR_HIP;
+  }
+  return ORION_OK;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+int DevBuf::ensure(size_t b) {
+  if (b <= bytes && ptr) return ORION_OK;
+  release();
+  int rc = dev_alloc(&ptr, b);
+  if (rc == ORION_OK) bytes = b;
+  return rc;
+}
+
+// ---------------------------------------------------------------- wave buffers
+int WaveBuffers::ensure(int N) {
+  if (N <= cap && mem.ptr) return ORION_OK;
+  // floats/ints per path: rays 7 x 3 slots, hits 4 x 2, vis 1, state 9,
+  // pending 15, queues 2 + 2 + 1
+  const size_t per = 21 + 8 + 1 + 9 + 15 + 5;
+  int rc = mem.ensure(per * (size_t)N * 4 + 256);
+  if (rc) return rc;
+  cap = N;
+  char *b = (char *)mem.ptr;
+  size_t off = 0;
+  auto take = [&](size_t n) { void *q = b + off; off += n * 4; return q; };
+  Wave &w = wave;
+  w.ox = (float *)take(3 * (size_t)N); w.oy = (float *)take(3 * (size_t)N); w.oz = (float *)take(3 * (size_t)N);
+  w.dx = (float *)take(3 * (size_t)N); w.dy = (float *)take(3 * (size_t)N); w.dz = (float *)take(3 * (size_t)N);
+  w.tmax = (float *)take(3 * (size_t)N);
+  w.ht = (float *)take(2 * (size_t)N); w.hu = (float *)take(2 * (size_t)N); w.hv = (float *)take(2 * (size_t)N);
+  w.hprim = (int *)take(2 * (size_t)N);
+  w.vis = (int *)take(N);
+  w.br = (float *)take(N); w.bg = (float *)take(N); w.bb = (float *)take(N);
+  w.Lr = (float *)take(N); w.Lg = (float *)take(N); w.Lb = (float *)take(N);
+  w.dim = (int *)take(N); w.bounce = (int *)take(N); w.flags = (int *)take(N);
+  w.Ar = (float *)take(N); w.Ag = (float *)take(N); w.Ab = (float *)take(N);
+  w.fr = (float *)take(N); w.fg = (float *)take(N); w.fb = (float *)take(N);
+  w.px = (float *)take(N); w.py = (float *)take(N); w.pz = (float *)take(N);
+  w.bpdf = (float *)take(N); w.sel = (float *)take(N);
+  w.nbr = (float *)take(N); w.nbg = (float *)take(N); w.nbb = (float *)take(N);
+  w.nlight = (int *)take(N);
+  w.qpath[0] = (int *)take(N); w.qpath[1] = (int *)take(N);
+  w.qray = (int *)take(2 * (size_t)N);
+  w.qshadow = (int *)take(N);
+  return ORION_OK;
+}
+
+}  // namespace orion_amd
+
+using namespace orion_amd;
+
+// ======================================================================= C ABI (device)
+extern "C" {
+
+int orion_ctx_create(int device, orion_ctx *out) {
+  if (!out) { set_error("null out"); return ORION_ERR_ARG; }
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) { set_error("no such device"); return ORION_ERR_ARG; }
+  HIP_TRY(hipSetDevice(device));
+  auto *c = new orion_ctx_s();
+  c->device = device;
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  c->n_cu = prop.multiProcessorCount;
+  c->arch = prop.gcnArchName;
+  if (c->arch.rfind("gfx950", 0) != 0) {
+    delete c;
+    set_error("orion_amd is built for gfx950 only; device reports " + std::string(prop.gcnArchName));
+    return ORION_ERR_UNSUPPORTED;
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&c->ev0));
+  HIP_TRY(hipEventCreate(&c->ev1));
+  *out = c;
+  return ORION_OK;
+}
+
+int orion_ctx_destroy(orion_ctx c) {
+  if (!c) return ORION_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipEventDestroy(c->ev0);
+  (void)hipEventDestroy(c->ev1);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return ORION_OK;
+}
+
+int orion_ctx_synchronize(orion_ctx c) {
+  if (!c) { set_error("null ctx"); return ORION_ERR_ARG; }
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ORION_OK;
+}
+
+int orion_scene_upload(orion_ctx c, orion_hscene hs, orion_scene *out) {
+  if (!c || !hs || !out) { set_error("null argument"); return ORION_ERR_ARG; }
+  const HostScene &s = *reinterpret_cast<const HostScene *>(hs);
+  int W = (int)s.camera[14], H = (int)s.camera[15];
+  if (W <= 0 || H <= 0) { set_error("camera record has no film size"); return ORION_ERR_ARG; }
+  int n = s.n_tris();
+  if (n > (1 << 28) / 8) { set_error("too many triangles for the leaf encoding"); return ORION_ERR_UNSUPPORTED; }
+  for (int i = 0; i < n; ++i) {
+    if (s.tri_mat[i] < 0 || s.tri_mat[i] >= s.n_mats()) { set_error("triangle material out of range"); return ORION_ERR_ARG; }
+    if (s.tri_light[i] >= s.n_lights()) { set_error("triangle light out of range"); return ORION_ERR_ARG; }
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  auto sc = new orion_scene_s();
+  sc->ctx = c;
+  sc->W = W;
+  sc->H = H;
+  auto fail = [&](int rc) { delete sc; return rc; };
+  auto t0 = std::chrono::steady_clock::now();
+  build_bvh(s.tri_pn.data(), n, &sc->bvh);
+  auto t1 = std::chrono::steady_clock::now();
+  // shading records in primitive order: p0 p1 p2 n0 n1 n2 mat light (80 B)
+  std::vector<float> shade(20 * (size_t)std::max(n, 1), 0.f);
+  for (int i = 0; i < n; ++i) {
+    std::memcpy(&shade[20 * (size_t)i], &s.tri_pn[18 * (size_t)i], 18 * sizeof(float));
+    std::memcpy(&shade[20 * (size_t)i + 18], &s.tri_mat[i], 4);
+    std::memcpy(&shade[20 * (size_t)i + 19], &s.tri_light[i], 4);
+  }
+  // lights: power distribution (path.cpp:39-48) and areas (triangle.cpp:77-83)
+  int nl = s.n_lights();
+  std::vector<float> area(std::max(nl, 1), 0.f), func(std::max(nl, 1), 0.f), cdf(nl + 1, 0.f);
+  for (int l = 0; l < nl; ++l) {
+    const float *L = &s.lights[32 * (size_t)l];
+    int type = (int)L[0];
+    float pr, pg, pb;
+    if (type == LIGHT_AREA) {
+      int t = (int)L[1];
+      if (t < 0 || t >= n) { delete sc; set_error("area light triangle out of range"); return ORION_ERR_ARG; }
+      const float *q = &s.tri_pn[18 * (size_t)t];
+      float e1x = q[3] - q[0], e1y = q[4] - q[1], e1z = q[5] - q[2];
+      float e2x = q[6] - q[0], e2y = q[7] - q[1], e2z = q[8] - q[2];
+      float cx = (e1y * e2z) - (e1z * e2y), cy = (e1z * e2x) - (e1x * e2z), cz = (e1x * e2y) - (e1y * e2x);
+      area[l] = 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
+      // DiffuseAreaLight::power = Le * area * pi
+      pr = L[2] * area[l] * 3.14159265358979323846f;
+      pg = L[3] * area[l] * 3.14159265358979323846f;
+      pb = L[4] * area[l] * 3.14159265358979323846f;
+    } else if (type == LIGHT_POINT) {
+      const float k = 4 * 3.14159265358979323846f;  // PointLight::power = 4 pi I
+      pr = L[2] * k; pg = L[3] * k; pb = L[4] * k;
+    } else {  // SpotLight::power = I * 2 * pi * (1 - .5 (cosFalloffStart + cosTotal))
+      float f = (1 - .5f * (L[10] + L[9]));
+      pr = L[2] * 2 * 3.14159265358979323846f * f;
+      pg = L[3] * 2 * 3.14159265358979323846f * f;
+      pb = L[4] * 2 * 3.14159265358979323846f * f;
+    }
+    func[l] = 0.212671f * pr + 0.715160f * pg + 0.072169f * pb;  // Spectrum::intensity
+  }
+  float funcInt = 0;
+  if (nl > 0) {  // Distribution1D (sampling.h:76-90)
+    cdf[0] = 0;
+    for (int i = 1; i < nl + 1; ++i) cdf[i] = cdf[i - 1] + func[i - 1] / nl;
+    funcInt = cdf[nl];
+    if (funcInt == 0) {
+      for (int i = 1; i < nl + 1; ++i) cdf[i] = float(i) / float(nl);
+    } else {
+      for (int i = 1; i < nl + 1; ++i) cdf[i] /= funcInt;
+    }
+  }
+  // upload
+  auto up = [&](DevBuf &b, const void *src, size_t bytes) -> int {
+    int rc = b.ensure(bytes);
+    if (rc) return rc;
+    if (bytes) {
+      hipError_t e = hipMemcpy(b.ptr, src, bytes, hipMemcpyHostToDevice);
+      if (e != hipSuccess) { set_error(std::string("upload: ") + hipGetErrorString(e)); return ORION_ERR_HIP; }
+    }
+    return ORION_OK;
+  };
+  int rc;
+  if ((rc = up(sc->nodes, sc->bvh.nodes.data(), sc->bvh.nodes.size() * sizeof(BvhNode)))) return fail(rc);
+  if ((rc = up(sc->tris, sc->bvh.tris.data(), sc->bvh.tris.size() * sizeof(BvhTri)))) return fail(rc);
+  if ((rc = up(sc->shade, shade.data(), shade.size() * 4))) return fail(rc);
+  if ((rc = up(sc->mats, s.mats.data(), s.mats.size() * 4))) return fail(rc);
+  if ((rc = up(sc->lights, s.lights.data(), s.lights.size() * 4))) return fail(rc);
+  if ((rc = up(sc->light_cdf, cdf.data(), cdf.size() * 4))) return fail(rc);
+  if ((rc = up(sc->light_func, func.data(), func.size() * 4))) return fail(rc);
+  if ((rc = up(sc->light_area, area.data(), area.size() * 4))) return fail(rc);
+  if ((rc = up(sc->sobol, orion_sobol_matrices, sizeof(orion_sobol_matrices)))) return fail(rc);
+  if ((rc = sc->film.ensure((size_t)W * H * 3 * 4))) return fail(rc);
+  if ((rc = sc->count.ensure((size_t)W * H * 4))) return fail(rc);
+  if ((rc = sc->ctrl.ensure(64 * 4))) return fail(rc);
+  if ((rc = sc->stats.ensure(S_COUNT * 8))) return fail(rc);
+  hipError_t e1 = hipMemset(sc->film.ptr, 0, (size_t)W * H * 3 * 4);
+  hipError_t e2 = hipMemset(sc->count.ptr, 0, (size_t)W * H * 4);
+  hipError_t e3 = hipMemset(sc->ctrl.ptr, 0, 64 * 4);
+  hipError_t e4 = hipDeviceSynchronize();
+  if (e1 || e2 || e3 || e4) { set_error("scene init memset failed"); return fail(ORION_ERR_HIP); }
+  auto t2 = std::chrono::steady_clock::now();
+  DScene &d = sc->d;
+  d.nodes = (const float4 *)sc->nodes.ptr;
+  d.tris = (const float4 *)sc->tris.ptr;
+  d.shade = (const float4 *)sc->shade.ptr;
+  d.mats = (const float *)sc->mats.ptr;
+  d.lights = (const float *)sc->lights.ptr;
+  d.light_cdf = (const float *)sc->light_cdf.ptr;
+  d.light_func = (const float *)sc->light_func.ptr;
+  d.light_area = (const float *)sc->light_area.ptr;
+  d.sobol = (const uint32_t *)sc->sobol.ptr;
+  d.light_func_int = funcInt;
+  d.n_lights = nl;
+  d.n_prims = n;
+  std::memcpy(d.cam, s.camera, sizeof(d.cam));
+  sc->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  sc->upload_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  *out = sc;
+  return ORION_OK;
+}
+
+int orion_scene_destroy(orion_scene s) {
+  if (!s) return ORION_OK;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  delete s;
+  return ORION_OK;
+}
+
+int orion_scene_get_stats(orion_scene s, orion_scene_stats *st) {
+  if (!s || !st) { set_error("null argument"); return ORION_ERR_ARG; }
+  st->n_nodes = (int64_t)s->bvh.nodes.size();
+  st->n_tris = (int64_t)s->bvh.tris.size();
+  st->node_bytes = st->n_nodes * (int64_t)sizeof(BvhNode);
+  st->tri_bytes = st->n_tris * (int64_t)sizeof(BvhTri);
+  st->shade_bytes = st->n_tris * 80;
+  st->bvh_depth = s->bvh.depth;
+  st->node_format = 2;
+  st->build_ms = s->build_ms;
+  st->upload_ms = s->upload_ms;
+  return ORION_OK;
+}
+
+int orion_intersect_closest(orion_scene s, int n, const float *rays, float *t, int *prim, float *u, float *v,
+                            int *tie, int on_device) {
+  if (!s || n < 0 || !rays || !t || !prim) { set_error("bad argument"); return ORION_ERR_ARG; }
+  if (n == 0) return ORION_OK;
+  orion_ctx c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  if (on_device) {
+    launch_intersect_batch(s->d, n, rays, t, prim, u, v, tie, nullptr, false, c->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return ORION_OK;
+  }
+  DevBuf buf;
+  int rc = buf.ensure((size_t)n * (8 + 5) * 4);
+  if (rc) return rc;
+  float *dr = (float *)buf.ptr;
+  float *dt = dr + 8 * (size_t)n, *du = dt + n, *dv = du + n;
+  int *dp = (int *)(dv + n), *dtie = dp + n;
+  HIP_TRY(hipMemcpyAsync(dr, rays, (size_t)n * 8 * 4, hipMemcpyHostToDevice, c->stream));
+  launch_intersect_batch(s->d, n, dr, dt, dp, du, dv, dtie, nullptr, false, c->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(t, dt, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(prim, dp, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (u) HIP_TRY(hipMemcpyAsync(u, du, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (v) HIP_TRY(hipMemcpyAsync(v, dv, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (tie) HIP_TRY(hipMemcpyAsync(tie, dtie, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ORION_OK;
+}
+
+int orion_intersect_any(orion_scene s, int n, const float *rays, int *occluded, int on_device) {
+  if (!s || n < 0 || !rays || !occluded) { set_error("bad argument"); return ORION_ERR_ARG; }
+  if (n == 0) return ORION_OK;
+  orion_ctx c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  if (on_device) {
+    launch_intersect_batch(s->d, n, rays, nullptr, nullptr, nullptr, nullptr, nullptr, occluded, true, c->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return ORION_OK;
+  }
+  DevBuf buf;
+  int rc = buf.ensure((size_t)n * 9 * 4);
+  if (rc) return rc;
+  float *dr = (float *)buf.ptr;
+  int *docc = (int *)(dr + 8 * (size_t)n);
+  HIP_TRY(hipMemcpyAsync(dr, rays, (size_t)n * 8 * 4, hipMemcpyHostToDevice, c->stream));
+  launch_intersect_batch(s->d, n, dr, nullptr, nullptr, nullptr, nullptr, nullptr, docc, true, c->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(occluded, docc, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return ORION_OK;
+}
+
+int orion_film_clear(orion_scene s) {
+  if (!s) { set_error("null scene"); return ORION_ERR_ARG; }
+  HIP_TRY(hipSetDevice(s->ctx->device));
+  HIP_TRY(hipMemsetAsync(s->film.ptr, 0, (size_t)s->W * s->H * 3 * 4, s->ctx->stream));
+  HIP_TRY(hipMemsetAsync(s->count.ptr, 0, (size_t)s->W * s->H * 4, s->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  return ORION_OK;
+}
+
+int orion_film_read(orion_scene s, float *rgb, int *count) {
+  if (!s || !rgb || !count) { set_error("null argument"); return ORION_ERR_ARG; }
+  HIP_TRY(hipSetDevice(s->ctx->device));
+  HIP_TRY(hipMemcpyAsync(rgb, s->film.ptr, (size_t)s->W * s->H * 3 * 4, hipMemcpyDeviceToHost, s->ctx->stream));
+  HIP_TRY(hipMemcpyAsync(count, s->count.ptr, (size_t)s->W * s->H * 4, hipMemcpyDeviceToHost, s->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  return ORION_OK;
+}
+
+static int read_stats(orion_scene s, orion_render_stats *st, float ms) {
+  unsigned long long h[S_COUNT];
+  HIP_TRY(hipMemcpy(h, s->stats.ptr, sizeof(h), hipMemcpyDeviceToHost));
+  if (st) {
+    st->paths = (int64_t)h[S_PATHS];
+    st->closest_rays = (int64_t)h[S_CLOSEST];
+    st->shadow_rays = (int64_t)h[S_SHADOW];
+    st->ref_equiv_rays = (int64_t)(h[S_CLOSEST] + h[S_SHADOW] + h[S_REFEQ]);
+    st->photons = (int64_t)h[S_PHOTONS];
+    st->deposits = (int64_t)h[S_DEPOSITS];
+    st->ms = ms;
+  }
+  s->last_nodes = h[S_NODES];
+  s->last_tris = h[S_TRIS];
+  return ORION_OK;
+}
+
+int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, const orion_tuning *tune,
+                      orion_render_stats *stats) {
+  if (!s) { set_error("null scene"); return ORION_ERR_ARG; }
+  if (k0 < 0 || k1 < k0 || maxDepth < 0) { set_error("bad sample range / depth"); return ORION_ERR_ARG; }
+  const int npix = s->W * s->H;
+  if ((uint64_t)k1 * (uint64_t)npix > 0xffffffffull) {
+    set_error("sample index k * W * H must stay below 2^32");
+    return ORION_ERR_UNSUPPORTED;
+  }
+  if (maxDepth > 60) { set_error("maxDepth above 60 exceeds the Sobol dimension budget"); return ORION_ERR_UNSUPPORTED; }
+  orion_ctx c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  int S = tune && tune->spp_per_wave > 0 ? tune->spp_per_wave : std::max(1, (1 << 21) / npix);
+  S = std::min(S, std::max(1, k1 - k0));
+  int rc = s->wave.ensure(S * npix);
+  if (rc) return rc;
+  int bpc = tune && tune->persistent_blocks > 0 ? tune->persistent_blocks : 6;
+  int trace_blocks = c->n_cu * bpc;
+  HIP_TRY(hipMemsetAsync(s->stats.ptr, 0, S_COUNT * 8, c->stream));
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  Wave w = s->wave.wave;
+  w.ctrl = (int *)s->ctrl.ptr;
+  w.stats = (unsigned long long *)s->stats.ptr;
+  w.npix = npix;
+  w.W = s->W;
+  w.H = s->H;
+  for (int k = k0; k < k1; k += S) {
+    int Sk = std::min(S, k1 - k);
+    w.N = Sk * npix;
+    w.k0 = (uint32_t)k;
+    launch_raygen(s->d, w, c->stream);
+    int cur = 0;
+    for (int it = 0; it <= maxDepth; ++it) {
+      launch_trace(s->d, w, false, trace_blocks, c->stream);
+      launch_trace(s->d, w, true, trace_blocks, c->stream);
+      launch_between(w, cur ^ 1, c->stream);
+      launch_shade(s->d, w, cur, maxDepth, c->stream);
+      cur ^= 1;
+    }
+    launch_film(w, Sk, (float *)s->film.ptr, (int *)s->count.ptr, c->stream);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  return read_stats(s, stats, ms);
+}
+
+int orion_render_sppm(orion_scene s, int n_iters, int photons, float r0, int maxDepth, int hash_size,
+                      const orion_sppm_debug *dbg, orion_render_stats *stats) {
+  if (!s) { set_error("null scene"); return ORION_ERR_ARG; }
+  (void)n_iters; (void)photons; (void)r0; (void)maxDepth; (void)hash_size; (void)dbg; (void)stats;
+  set_error("SPPM kernels are not built into this library yet");
+  return ORION_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"
+
+orion_scene_s::~orion_scene_s() { orion_amd::destroy_sppm(sppm); }

@@ -1,0 +1,262 @@
+"""ctypes bindings to the oracle (oracle/liboracle.so) and to the reference
+harness (oracle/_ref/libref_orion.so).  Test infrastructure only."""
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libref_orion.so"
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+
+
+def _opt(p):
+    return None if p is None else p.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    def __init__(self):
+        if not ORACLE_SO.exists():
+            raise FileNotFoundError(f"{ORACLE_SO} not built (make -C oracle)")
+        L = C.CDLL(str(ORACLE_SO))
+        L.oracle_scene_create.restype = C.c_void_p
+        L.oracle_scene_create.argtypes = [C.c_int, f32p, i32p, i32p, C.c_int, f32p,
+                                          C.c_int, f32p, f32p]
+        L.oracle_scene_destroy.argtypes = [C.c_void_p]
+        L.oracle_intersect.argtypes = [C.c_void_p, C.c_int, f32p, C.c_int, C.c_int,
+                                       f32p, i32p, f32p, f32p, i32p]
+        L.oracle_render_path.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, f32p, i32p, C.c_int, C.c_void_p]
+        L.oracle_path_samples.argtypes = [C.c_void_p, C.c_int, i32p, i32p, C.c_int, f32p]
+        L.oracle_render_sppm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int,
+                                         C.c_int, f32p, i32p, C.c_int, C.c_int,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]
+        L.oracle_sobol.restype = C.c_float
+        L.oracle_sobol.argtypes = [C.c_uint64, C.c_int]
+        L.oracle_camera_ray.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint64, f32p]
+        L.oracle_bsdf_f_pdf.argtypes = [C.c_void_p, C.c_int, C.c_int, f32p, f32p, f32p,
+                                        C.c_int, f32p, f32p]
+        L.oracle_bsdf_sample.argtypes = [C.c_void_p, C.c_int, C.c_int, f32p, f32p, f32p,
+                                         C.c_int, f32p, f32p, f32p, i32p]
+        L.oracle_light_sample_li.argtypes = [C.c_void_p, C.c_int, f32p, f32p, f32p, f32p]
+        L.oracle_light_sample_le.argtypes = [C.c_void_p, C.c_int, f32p, f32p, f32p]
+        L.oracle_sin.restype = C.c_float
+        L.oracle_sin.argtypes = [C.c_float]
+        L.oracle_cos.restype = C.c_float
+        L.oracle_cos.argtypes = [C.c_float]
+        self.L = L
+
+    def scene(self, rec):
+        return OracleScene(self, rec)
+
+
+class OracleScene:
+    """Oracle scene over a resolved SceneRecords (see orion_amd.scene)."""
+
+    def __init__(self, orc, rec):
+        self.o = orc
+        self.rec = rec
+        self.h = orc.L.oracle_scene_create(
+            rec.n_tris, rec.tri_pn, rec.tri_mat, rec.tri_light, rec.n_mats, rec.mats,
+            rec.n_lights, rec.lights, rec.camera)
+
+    def __del__(self):
+        try:
+            self.o.L.oracle_scene_destroy(self.h)
+        except Exception:
+            pass
+
+    def intersect(self, rays, mode=0, brute=False):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        t = np.zeros(n, np.float32); prim = np.zeros(n, np.int32)
+        u = np.zeros(n, np.float32); v = np.zeros(n, np.float32); ties = np.zeros(n, np.int32)
+        self.o.L.oracle_intersect(self.h, n, rays, mode, int(brute), t, prim, u, v, ties)
+        return t, prim, u, v, ties
+
+    def render_path(self, spp, max_depth, k0=0, rows=None, threads=0):
+        W, H = self.rec.width, self.rec.height
+        rgb = np.zeros(W * H * 3, np.float32)
+        cnt = np.zeros(W * H, np.int32)
+        ctr = np.zeros(3, np.int64)
+        r0, r1 = (0, H) if rows is None else rows
+        self.o.L.oracle_render_path(self.h, k0, k0 + spp, r0, r1, max_depth, rgb, cnt, threads,
+                                    ctr.ctypes.data_as(C.c_void_p))
+        return rgb.reshape(H, W, 3), cnt.reshape(H, W), ctr
+
+    def path_samples(self, pixel, k, max_depth):
+        pixel = np.ascontiguousarray(pixel, np.int32); k = np.ascontiguousarray(k, np.int32)
+        out = np.zeros(len(pixel) * 3, np.float32)
+        self.o.L.oracle_path_samples(self.h, len(pixel), pixel, k, max_depth, out)
+        return out.reshape(-1, 3)
+
+    def render_sppm(self, iters, photons, radius, max_depth, hash_size=0, debug_iter=-1,
+                    threads=0):
+        W, H = self.rec.width, self.rec.height
+        hs = hash_size if hash_size > 0 else W * H
+        rgb = np.zeros(W * H * 3, np.float32)
+        cnt = np.zeros(W * H, np.int32)
+        dbg = {}
+        if debug_iter >= 0:
+            dbg = dict(vp_cell_count=np.zeros(hs, np.int32),
+                       photon_cell_count=np.zeros(hs, np.int32),
+                       M=np.zeros(W * H, np.int32), grid_info=np.zeros(8, np.int32),
+                       grid_bounds=np.zeros(6, np.float32))
+        self.o.L.oracle_render_sppm(self.h, iters, photons, radius, max_depth, hash_size, rgb,
+                                    cnt, threads, debug_iter,
+                                    _opt(dbg.get("vp_cell_count")),
+                                    _opt(dbg.get("photon_cell_count")), _opt(dbg.get("M")),
+                                    _opt(dbg.get("grid_info")), _opt(dbg.get("grid_bounds")))
+        return rgb.reshape(H, W, 3), cnt.reshape(H, W), dbg
+
+    def camera_ray(self, i, j, index):
+        out = np.zeros(6, np.float32)
+        self.o.L.oracle_camera_ray(self.h, i, j, index, out)
+        return out
+
+    def bsdf_f_pdf(self, m, frame, wi, wo, mode=0, type_mask=31):
+        f = np.zeros(3, np.float32); pdf = np.zeros(1, np.float32)
+        self.o.L.oracle_bsdf_f_pdf(self.h, m, mode, np.asarray(frame, np.float32),
+                                   np.asarray(wi, np.float32), np.asarray(wo, np.float32),
+                                   type_mask, f, pdf)
+        return f, pdf[0]
+
+    def bsdf_sample(self, m, frame, wo, u, mode=0, type_mask=31):
+        f = np.zeros(3, np.float32); wi = np.zeros(3, np.float32)
+        pdf = np.zeros(1, np.float32); st = np.zeros(1, np.int32)
+        self.o.L.oracle_bsdf_sample(self.h, m, mode, np.asarray(frame, np.float32),
+                                    np.asarray(wo, np.float32), np.asarray(u, np.float32),
+                                    type_mask, f, wi, pdf, st)
+        return f, wi, pdf[0], st[0]
+
+    def light_sample_li(self, l, p, ng, u):
+        out = np.zeros(14, np.float32)
+        self.o.L.oracle_light_sample_li(self.h, l, np.asarray(p, np.float32),
+                                        np.asarray(ng, np.float32), np.asarray(u, np.float32),
+                                        out)
+        return out
+
+    def light_sample_le(self, l, u1, u2):
+        out = np.zeros(14, np.float32)
+        self.o.L.oracle_light_sample_le(self.h, l, np.asarray(u1, np.float32),
+                                        np.asarray(u2, np.float32), out)
+        return out
+
+
+class Ref:
+    """The reference's own compiled sources (oracle/_ref)."""
+
+    def __init__(self):
+        if not REF_SO.exists():
+            raise FileNotFoundError(f"{REF_SO} not built (make -C oracle ref)")
+        L = C.CDLL(str(REF_SO))
+        L.ref_sobol.restype = C.c_float
+        L.ref_sobol.argtypes = [C.c_ulonglong, C.c_uint]
+        L.ref_sampler_next2.argtypes = [C.c_longlong, C.c_int, f32p]
+        L.ref_warps.argtypes = [f32p, C.c_float, f32p]
+        L.ref_fr_dielectric.restype = C.c_float
+        L.ref_fr_dielectric.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.ref_fr_conductor.argtypes = [C.c_float, f32p, f32p, f32p]
+        L.ref_roughness_to_alpha.restype = C.c_float
+        L.ref_roughness_to_alpha.argtypes = [C.c_float]
+        L.ref_metal_preset.restype = C.c_int
+        L.ref_metal_preset.argtypes = [C.c_char_p, f32p]
+        L.ref_bsdf_f_pdf.argtypes = [C.c_int, f32p, C.c_char_p, C.c_int, f32p, f32p, f32p,
+                                     C.c_int, f32p, f32p]
+        L.ref_bsdf_sample.argtypes = [C.c_int, f32p, C.c_char_p, C.c_int, f32p, f32p, f32p,
+                                      C.c_int, f32p, f32p, f32p, i32p]
+        L.ref_camera.argtypes = [f32p, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.c_longlong, f32p, f32p, f32p]
+        L.ref_create_transform.argtypes = [C.c_int, C.POINTER(C.c_char_p),
+                                           C.POINTER(C.c_char_p), f32p]
+        L.ref_light_info.argtypes = [C.c_int, f32p, f32p, f32p, C.c_float, C.c_float, f32p]
+        L.ref_light_sample_li.argtypes = [C.c_int, f32p, f32p, f32p, C.c_float, C.c_float,
+                                          f32p, f32p, f32p, f32p]
+        L.ref_light_sample_le.argtypes = [C.c_int, f32p, f32p, f32p, C.c_float, C.c_float,
+                                          f32p, f32p, f32p]
+        L.ref_spawn.argtypes = [f32p, f32p, f32p, f32p]
+        self.L = L
+
+    @staticmethod
+    def _f(a, n=None):
+        a = np.ascontiguousarray(a, np.float32)
+        return a
+
+    def sampler_next2(self, index, n):
+        out = np.zeros(2 * n, np.float32)
+        self.L.ref_sampler_next2(index, n, out)
+        return out.reshape(n, 2)
+
+    def warps(self, u, cos_max):
+        out = np.zeros(13, np.float32)
+        self.L.ref_warps(self._f(u), cos_max, out)
+        return out
+
+    def fr_conductor(self, c, eta, k):
+        out = np.zeros(3, np.float32)
+        self.L.ref_fr_conductor(c, self._f(eta), self._f(k), out)
+        return out
+
+    def metal_preset(self, name):
+        out = np.zeros(6, np.float32)
+        rc = self.L.ref_metal_preset(name.encode(), out)
+        assert rc == 0
+        return out[:3], out[3:]
+
+    def bsdf_f_pdf(self, kind, params, frame15, wi, wo, photon=0, type_mask=31, metal_name=b""):
+        f = np.zeros(3, np.float32); pdf = np.zeros(1, np.float32)
+        self.L.ref_bsdf_f_pdf(kind, self._f(params), metal_name, photon, self._f(frame15),
+                              self._f(wi), self._f(wo), type_mask, f, pdf)
+        return f, pdf[0]
+
+    def bsdf_sample(self, kind, params, frame15, wo, u, photon=0, type_mask=31, metal_name=b""):
+        f = np.zeros(3, np.float32); wi = np.zeros(3, np.float32)
+        pdf = np.zeros(1, np.float32); st = np.zeros(1, np.int32)
+        self.L.ref_bsdf_sample(kind, self._f(params), metal_name, photon, self._f(frame15),
+                               self._f(wo), self._f(u), type_mask, f, wi, pdf, st)
+        return f, wi, pdf[0], st[0]
+
+    def camera(self, lookat, fov, W, H, i=0, j=0, index=0):
+        m = np.zeros(16, np.float32); ta = np.zeros(2, np.float32); ray = np.zeros(6, np.float32)
+        self.L.ref_camera(self._f(lookat), fov, W, H, i, j, index, m, ta, ray)
+        return m.reshape(4, 4), ta, ray
+
+    def create_transform(self, pairs):
+        n = len(pairs)
+        keys = (C.c_char_p * n)(*[k.encode() for k, _ in pairs])
+        vals = (C.c_char_p * n)(*[v.encode() for _, v in pairs])
+        out = np.zeros(32, np.float32)
+        self.L.ref_create_transform(n, keys, vals, out)
+        return out[:16].reshape(4, 4), out[16:].reshape(4, 4)
+
+    def light_info(self, typ, I, frm, to=(0, 0, 0), angle=0.0, delta=0.0):
+        out = np.zeros(26, np.float32)
+        self.L.ref_light_info(typ, self._f(I), self._f(frm), self._f(to), angle, delta, out)
+        return out
+
+    def light_sample_li(self, typ, I, frm, to, angle, delta, p, ng, u):
+        out = np.zeros(14, np.float32)
+        self.L.ref_light_sample_li(typ, self._f(I), self._f(frm), self._f(to), angle, delta,
+                                   self._f(p), self._f(ng), self._f(u), out)
+        return out
+
+    def light_sample_le(self, typ, I, frm, to, angle, delta, u1, u2):
+        out = np.zeros(14, np.float32)
+        self.L.ref_light_sample_le(typ, self._f(I), self._f(frm), self._f(to), angle, delta,
+                                   self._f(u1), self._f(u2), out)
+        return out
+
+    def spawn(self, p, ng, d):
+        out = np.zeros(3, np.float32)
+        self.L.ref_spawn(self._f(p), self._f(ng), self._f(d), out)
+        return out
+
+
+def have_ref():
+    return REF_SO.exists()

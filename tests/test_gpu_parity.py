@@ -1,0 +1,135 @@
+"""GPU parity: the HIP path against the oracle on identical inputs.
+
+Contract: nearest-hit triangle ids (and t, u, v) bit-exact; per-pixel FP32
+radiance within 1e-3 relative on >= 99.9% of pixels and mean image radiance
+within 1e-4 relative (the tolerances are asserted below)."""
+import numpy as np
+import pytest
+
+import orion_amd as oa
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-3       # per pixel, relative
+PIX_FRAC = 0.999     # fraction of pixels that must meet REL_TOL
+MEAN_TOL = 1e-4      # relative error of the mean image radiance
+
+
+def random_rays(hs, n, seed, target_scene=True):
+    rng = np.random.default_rng(seed)
+    p = hs.tri_pn.reshape(-1, 18)
+    lo = p[:, :9].reshape(-1, 3).min(0)
+    hi = p[:, :9].reshape(-1, 3).max(0)
+    ctr = (lo + hi) / 2
+    ext = (hi - lo).max()
+    o = ctr + (rng.random((n, 3)) - 0.5) * ext * 1.6
+    tgt = ctr + (rng.random((n, 3)) - 0.5) * ext * 0.8
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((8, n), np.float32)
+    rays[0:3] = o.T
+    rays[3:6] = d.T
+    rays[6] = 0.0
+    rays[7] = np.finfo(np.float32).max
+    return rays
+
+
+def image_compare(gpu, ref):
+    """returns (fraction of pixels within REL_TOL, rel. error of the mean, bit-equal fraction)"""
+    den = np.maximum(np.abs(ref), 1e-6)
+    ok = (np.abs(gpu - ref) <= REL_TOL * den).all(axis=-1)
+    mean_err = abs(gpu.mean() - ref.mean()) / max(abs(ref.mean()), 1e-12)
+    bits = (gpu.view(np.int32) == ref.view(np.int32)).all(axis=-1)
+    return ok.mean(), mean_err, bits.mean()
+
+
+@pytest.mark.parametrize("scene,params", [("cornell", None), ("icosphere", [5, 64, 64])])
+def test_intersect_closest_bitexact(device, oracle, scene, params):
+    hs = oa.HostScene.generate(scene, seed=1, params=params)
+    ds = oa.DeviceScene(device, hs)
+    osc = oracle.scene(hs)
+    rays = random_rays(hs, 20000, seed=7)
+    t, prim, u, v, tie = ds.intersect_closest(rays)
+    ot, oprim, ou, ov, otie = osc.intersect(np.ascontiguousarray(rays.T), mode=0, brute=True)
+    assert (prim >= 0).mean() > 0.3
+    np.testing.assert_array_equal(prim, oprim)
+    hit = prim >= 0
+    np.testing.assert_array_equal(t[hit].view(np.int32), ot[hit].view(np.int32))
+    np.testing.assert_array_equal(u[hit].view(np.int32), ou[hit].view(np.int32))
+    np.testing.assert_array_equal(v[hit].view(np.int32), ov[hit].view(np.int32))
+    np.testing.assert_array_equal(tie, otie)
+    occ = ds.intersect_any(rays)
+    np.testing.assert_array_equal(occ, (oprim >= 0).astype(np.int32))
+
+
+def test_intersect_edge_cases(device, oracle):
+    """Rays along shared edges / through vertices (ties) and degenerate inputs."""
+    hs = oa.HostScene.generate("cornell")
+    ds = oa.DeviceScene(device, hs)
+    osc = oracle.scene(hs)
+    p = hs.tri_pn.reshape(-1, 18)
+    # aim at every triangle's vertices and edge midpoints from the camera
+    o = hs.camera[:3]
+    tg = np.concatenate([p[:, 0:3], p[:, 3:6], p[:, 6:9], (p[:, 0:3] + p[:, 3:6]) / 2,
+                         (p[:, 3:6] + p[:, 6:9]) / 2])
+    d = tg - o
+    n = len(d)
+    rays = np.zeros((8, n + 2), np.float32)
+    rays[0:3, :n] = o[:, None]
+    rays[3:6, :n] = d.T
+    rays[7] = np.finfo(np.float32).max
+    rays[3:6, n] = [0, 0, 0]          # zero direction: no hit
+    rays[0:3, n + 1] = [0.5, 0.5, 0.5]
+    rays[3:6, n + 1] = [0, 1, 0]
+    rays[7, n + 1] = 0.1              # tmax before the ceiling: no hit
+    t, prim, u, v, tie = ds.intersect_closest(rays)
+    ot, oprim, ou, ov, otie = osc.intersect(np.ascontiguousarray(rays.T), mode=0, brute=True)
+    np.testing.assert_array_equal(prim, oprim)
+    np.testing.assert_array_equal(tie, otie)
+    assert prim[n] == -1 and prim[n + 1] == -1
+    print(f"edge-case rays: {n}, ties counted: {int(tie.sum())}")
+
+
+def test_path_cornell_full(device, oracle):
+    """C1 exactly: 256^2, 16 spp, depth 5, against the oracle."""
+    hs = oa.HostScene.generate("cornell")
+    ds = oa.DeviceScene(device, hs)
+    ds.film_clear()
+    st = ds.render_path(0, 16, 5)
+    rgb, cnt = ds.film()
+    assert (cnt == 16).all()
+    assert st.paths == 256 * 256 * 16
+    ref, rcnt, _ = oracle.scene(hs).render_path(16, 5)
+    gimg = rgb / cnt[..., None]
+    rimg = ref / rcnt[..., None]
+    frac, mean_err, bits = image_compare(gimg, rimg)
+    print(f"cornell: within 1e-3 {frac:.6f}, mean rel err {mean_err:.2e}, bit-equal {bits:.6f}")
+    assert frac >= PIX_FRAC
+    assert mean_err <= MEAN_TOL
+
+
+def test_path_icosphere_crop(device, oracle):
+    """C2's 1.31M-triangle mesh at 96^2, 2 spp, depth 8 (RR after bounce 3)."""
+    hs = oa.HostScene.generate("icosphere", seed=3, params=[8, 96, 96])
+    ds = oa.DeviceScene(device, hs)
+    ds.film_clear()
+    ds.render_path(0, 2, 8)
+    rgb, cnt = ds.film()
+    ref, rcnt, _ = oracle.scene(hs).render_path(2, 8)
+    frac, mean_err, bits = image_compare(rgb / cnt[..., None], ref / rcnt[..., None])
+    print(f"icosphere: within 1e-3 {frac:.6f}, mean rel err {mean_err:.2e}, bit-equal {bits:.6f}")
+    assert frac >= PIX_FRAC
+    assert mean_err <= MEAN_TOL
+
+
+def test_path_waves_equal_single(device):
+    """Splitting the sample range into several wavefronts must not change a bit."""
+    hs = oa.HostScene.generate("cornell", params=[64, 64])
+    ds = oa.DeviceScene(device, hs)
+    ds.film_clear()
+    ds.render_path(0, 8, 5, spp_per_wave=8)
+    a, _ = ds.film()
+    ds.film_clear()
+    ds.render_path(0, 8, 5, spp_per_wave=3)
+    b, _ = ds.film()
+    np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32))

@@ -140,6 +140,13 @@ typedef struct {
   int64_t photons;
   int64_t deposits;       /* SPPM photon deposit lookups */
   double ms;              /* device time of the call */
+  /* traversal work (for the roofline's algorithmic bytes) */
+  int64_t node_visits_closest, tri_tests_closest;
+  int64_t node_visits_shadow, tri_tests_shadow;
+  /* per-kernel device time, filled when orion_tuning.profile != 0 (HIP
+   * events recorded on the stream the kernels run on) */
+  double trace_closest_ms, trace_shadow_ms, shade_ms;
+  int64_t trace_closest_launches, trace_shadow_launches, shade_launches;
 } orion_render_stats;
 
 typedef struct {
@@ -147,6 +154,7 @@ typedef struct {
   int block;            /* threads per block for traversal (0 = auto) */
   int persistent_blocks;/* blocks per CU for persistent traversal (0 = auto) */
   int use_graph;        /* capture the bounce loop into a hipGraph */
+  int profile;          /* time every traversal / shade launch with HIP events */
 } orion_tuning;
 
 /* PathTracing::render over samples [k0, k1): accumulates into the scene's film

@@ -39,7 +39,12 @@ class SceneStats(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("paths", C.c_int64), ("closest_rays", C.c_int64), ("shadow_rays", C.c_int64),
                 ("ref_equiv_rays", C.c_int64), ("photons", C.c_int64), ("deposits", C.c_int64),
-                ("ms", C.c_double)]
+                ("ms", C.c_double), ("node_visits_closest", C.c_int64),
+                ("tri_tests_closest", C.c_int64), ("node_visits_shadow", C.c_int64),
+                ("tri_tests_shadow", C.c_int64), ("trace_closest_ms", C.c_double),
+                ("trace_shadow_ms", C.c_double), ("shade_ms", C.c_double),
+                ("trace_closest_launches", C.c_int64), ("trace_shadow_launches", C.c_int64),
+                ("shade_launches", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -47,7 +52,7 @@ class RenderStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("spp_per_wave", C.c_int), ("block", C.c_int), ("persistent_blocks", C.c_int),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("profile", C.c_int)]
 
 
 class SppmDebug(C.Structure):
@@ -285,9 +290,10 @@ class DeviceScene:
         _check(lib().orion_intersect_any(self.h, n, _ptr(rays), _ptr(occ), 0))
         return occ
 
-    def render_path(self, k0, k1, max_depth, spp_per_wave=0, persistent_blocks=0):
+    def render_path(self, k0, k1, max_depth, spp_per_wave=0, persistent_blocks=0, profile=False,
+                    use_graph=False):
         st = RenderStats()
-        tune = Tuning(spp_per_wave, 0, persistent_blocks, 0)
+        tune = Tuning(spp_per_wave, 0, persistent_blocks, int(use_graph), int(profile))
         _check(lib().orion_render_path(self.h, k0, k1, max_depth, C.byref(tune), C.byref(st)))
         return st
 

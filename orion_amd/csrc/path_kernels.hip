@@ -12,26 +12,11 @@
 
 #include "device_scene.h"
 #include "kernels.h"
+#include "shading.h"
 #include "wavefront.h"
 
 namespace orion_amd {
 using namespace dev;
-
-// ---------------------------------------------------------------- camera
-// pinhole.cpp:8-33 with next2() = (x: dim d+1, y: dim d), sampler.h:31 as
-// compiled by g++ (the right argument is evaluated first).
-__device__ __forceinline__ void camera_ray(const DScene &sc, int i, int j, uint32_t index, V3 *o, V3 *d) {
-  float uy = sobol(sc.sobol, index, 5);
-  float ux = sobol(sc.sobol, index, 6);
-  float rx = (float)i + ux, ry = (float)j + uy;
-  const float *c = sc.cam;
-  float width = c[14], height = c[15];
-  float xx = (2 * rx / width - 1) * c[13] * c[12];
-  float yy = (1 - 2 * ry / height) * c[12];
-  V3 dir = normalize(mk(xx, yy, -1.0f));
-  *o = mk(c[0], c[1], c[2]);
-  *d = mat3(c + 3, dir);
-}
 
 __global__ void k_raygen(DScene sc, Wave w) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -100,8 +85,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(DScene sc, Wave w) {
     nt += __shfl_down(nt, off);
   }
   if (lane == 0 && w.stats) {
-    atomicAdd(&w.stats[S_NODES], (unsigned long long)nv);
-    atomicAdd(&w.stats[S_TRIS], (unsigned long long)nt);
+    atomicAdd(&w.stats[ANY ? S_NODES_S : S_NODES_C], (unsigned long long)nv);
+    atomicAdd(&w.stats[ANY ? S_TRIS_S : S_TRIS_C], (unsigned long long)nt);
   }
 }
 
@@ -119,74 +104,6 @@ __global__ void k_between(Wave w, int nxt) {
     w.ctrl[C_WORK_SHADOW] = 0;
     w.ctrl[C_PATH0 + nxt] = 0;
   }
-}
-
-// ---------------------------------------------------------------- lights
-// Distribution1D::sampleDiscrete (sampling.h:110-120) over light power.
-__device__ __forceinline__ int sample_light(const DScene &sc, float u, float *pdf) {
-  int size = sc.n_lights + 1;
-  int first = 0, len = size;
-  while (len > 0) {
-    int half = len >> 1, mid = first + half;
-    if (sc.light_cdf[mid] <= u) { first = mid + 1; len -= half + 1; }
-    else len = half;
-  }
-  int off = first - 1;
-  off = off < 0 ? 0 : (off > size - 2 ? size - 2 : off);
-  *pdf = (sc.light_func_int > 0) ? sc.light_func[off] / (sc.light_func_int * sc.n_lights) : 0;
-  return off;
-}
-
-__device__ __forceinline__ float spot_falloff(const float *L, V3 w) {  // spot.cpp:10-19
-  V3 wl = normalize(mat3(L + 11, w));
-  float cosTheta = wl.z;
-  if (cosTheta < L[9]) return 0;
-  if (cosTheta > L[10]) return 1;
-  float delta = (cosTheta - L[9]) / (L[10] - L[9]);
-  return (delta * delta) * (delta * delta);
-}
-
-// Light::sample_Li; *so / *sd / *stmax = the ShadowTester ray (intersection.h:40-44)
-__device__ inline Sp light_sample_li(const DScene &sc, int l, V3 refp, V3 refng, float u0, float u1, V3 *wi,
-                                     float *pdf, V3 *so, V3 *sd) {
-  const float *L = sc.lights + 32 * (size_t)l;
-  int type = (int)L[0];
-  Sp Le = sp3(L[2], L[3], L[4]);
-  V3 target;
-  if (type == 0) {  // diffuse.cpp:13-25 + triangle.cpp:85-121
-    V3 p0, p1, p2, n0, n1, n2;
-    int m_, l_;
-    load_prim(sc, (int)L[1], &p0, &p1, &p2, &n0, &n1, &n2, &m_, &l_);
-    float su0 = sqrtf(u0);
-    float b0 = 1 - su0, b1 = u1 * su0;
-    V3 p = add(add(mul(p0, b0), mul(p1, b1)), mul(p2, 1 - b0 - b1));
-    V3 g = normalize(cross(sub(p1, p0), sub(p2, p0)));
-    V3 ns = add(add(mul(n0, b0), mul(n1, b1)), mul(n2, 1.0f - b0 - b1));
-    V3 ng = dot(g, ns) < 0.f ? neg(g) : g;
-    float pd = 1.0f / sc.light_area[l];
-    V3 w = sub(p, refp);
-    if (lensq(w) == 0) pd = 0;
-    else {
-      w = normalize(w);
-      pd *= lensq(sub(refp, p)) / absdot(ng, neg(w));
-      if (is_inf(pd)) pd = 0;
-    }
-    if (pd == 0 || lensq(sub(p, refp)) == 0) { *pdf = 0; return sp(0); }
-    *pdf = pd;
-    *wi = normalize(sub(p, refp));
-    target = p;
-    Le = dot(ng, neg(*wi)) > 0 ? Le : sp(0);
-  } else {  // point.cpp:10-19, spot.cpp:20-29
-    V3 pos = mk(L[5], L[6], L[7]);
-    *wi = normalize(sub(pos, refp));
-    *pdf = 1.0f;
-    target = pos;
-    if (type == 1) Le = Le / lensq(sub(pos, refp));
-    else Le = Le * spot_falloff(L, neg(*wi)) / lensq(sub(pos, refp));
-  }
-  *so = offset_ray_origin(refp, refng, sub(target, refp));
-  *sd = sub(target, *so);
-  return Le;
 }
 
 __device__ __forceinline__ uint32_t path_index(const Wave &w, int p) {
@@ -209,29 +126,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, Wave w, int cur, int m
       Sp L = sp3(w.Lr[p], w.Lg[p], w.Lb[p]);
       // 1) resolve the direct-lighting estimate of the previous bounce
       if (fl & F_PEND) {
-        Sp Ld = sp(0);
-        if ((fl & F_SHADOW) && w.vis[p]) acc(Ld, sp3(w.Ar[p], w.Ag[p], w.Ab[p]));
-        if (fl & F_MIS) {
-          int hp = w.hprim[N + p];
-          int l = w.nlight[p];
-          const float *LR = sc.lights + 32 * (size_t)l;
-          if (hp >= 0 && hp == (int)LR[1]) {
-            // Triangle::pdf (triangle.cpp:123-135) on the MIS ray's hit
-            Isect in;
-            make_isect(sc, hp, w.hu[N + p], w.hv[N + p], &in);
-            V3 wi = mk(w.dx[N + p], w.dy[N + p], w.dz[N + p]);
-            V3 pp = mk(w.px[p], w.py[p], w.pz[p]);
-            float lpdf = lensq(sub(pp, in.p)) / (absdot(in.ng, neg(wi)) * sc.light_area[l]);
-            if (is_inf(lpdf)) lpdf = 0;
-            if (lpdf != 0) {
-              float bp = w.bpdf[p];
-              float weight = power_heuristic(bp, lpdf);
-              Sp Lm = dot(in.ng, neg(wi)) > 0 ? sp3(LR[2], LR[3], LR[4]) : sp(0);
-              if (!black(Lm)) acc(Ld, sp3(w.fr[p], w.fg[p], w.fb[p]) * Lm * weight / bp);
-            }
-          }
-        }
-        acc(L, sp3(w.nbr[p], w.nbg[p], w.nbb[p]) * (Ld / w.sel[p]));
+        resolve_direct(sc, w, p, fl, &L);
         fl &= ~(F_PEND | F_SHADOW | F_MIS);
       }
       // 2) the hit of this bounce
@@ -242,11 +137,7 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, Wave w, int cur, int m
         V3 d = mk(w.dx[p], w.dy[p], w.dz[p]);
         Isect is;
         if (hp >= 0) make_isect(sc, hp, w.hu[p], w.hv[p], &is);
-        if ((b == 0 || (fl & F_SPEC)) && hp >= 0 && is.light >= 0) {
-          const float *LR = sc.lights + 32 * (size_t)is.light;
-          Sp Le = dot(is.ng, neg(d)) > 0 ? sp3(LR[2], LR[3], LR[4]) : sp(0);  // diffuse.cpp:53-56
-          acc(L, beta * Le);
-        }
+        if ((b == 0 || (fl & F_SPEC)) && hp >= 0 && is.light >= 0) acc(L, beta * area_emission(sc, is, neg(d)));
         if (hp < 0 || b >= maxDepth) {
           fl &= ~F_ALIVE;
         } else {
@@ -254,61 +145,11 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, Wave w, int cur, int m
           make_bsdf(sc.mats + 16 * (size_t)is.mat, is.ng, is.ns, is.u, is.v, false, &bs);
           uint32_t index = path_index(w, p);
           int dim = w.dim[p];
-          const int nonspec = ALL & ~SPEC;
-          if (bsdf_num(bs, nonspec) > 0 && sc.n_lights > 0) {
-            float sel;
-            int l = sample_light(sc, sobol(sc.sobol, index, dim++), &sel);
-            if (sel != 0) {
-              float lu1 = sobol(sc.sobol, index, dim++), lu0 = sobol(sc.sobol, index, dim++);
-              float bu1 = sobol(sc.sobol, index, dim++), bu0 = sobol(sc.sobol, index, dim++);
-              V3 wo = normalize(neg(d));
-              V3 wi = mk(0, 0, 0), so, sdir;
-              float lpdf = 0;
-              Sp Li = light_sample_li(sc, l, is.p, is.ng, lu0, lu1, &wi, &lpdf, &so, &sdir);
-              bool delta = (int)sc.lights[32 * (size_t)l] != 0;
-              bool hs = false, hm = false;
-              if (lpdf > 0 && !black(Li)) {
-                Sp f = bsdf_f(bs, wi, wo, nonspec) * absdot(wi, is.ns);
-                float bpdf = bsdf_pdf(bs, wi, wo, nonspec);
-                if (!black(f)) {
-                  Sp A;
-                  if (delta) A = f * Li / lpdf;
-                  else A = f * Li * power_heuristic(lpdf, bpdf) / lpdf;
-                  w.Ar[p] = A.r; w.Ag[p] = A.g; w.Ab[p] = A.b;
-                  int s2 = 2 * N + p;
-                  w.ox[s2] = so.x; w.oy[s2] = so.y; w.oz[s2] = so.z;
-                  w.dx[s2] = sdir.x; w.dy[s2] = sdir.y; w.dz[s2] = sdir.z;
-                  w.tmax[s2] = 1 - kShadowEps;
-                  hs = true;
-                }
-              }
-              if (!delta) {
-                float bp2 = 0;
-                int st = 0;
-                V3 wi2 = wi;
-                Sp f2 = bsdf_sample(bs, &wi2, wo, bu0, bu1, &bp2, nonspec, &st);
-                mulby(f2, sp(absdot(wi2, is.ns)));
-                if (!black(f2) && bp2 > 0) {
-                  V3 o2 = offset_ray_origin(is.p, is.ng, wi2);
-                  int s1 = N + p;
-                  w.ox[s1] = o2.x; w.oy[s1] = o2.y; w.oz[s1] = o2.z;
-                  w.dx[s1] = wi2.x; w.dy[s1] = wi2.y; w.dz[s1] = wi2.z;
-                  w.tmax[s1] = kFltMax;
-                  w.fr[p] = f2.r; w.fg[p] = f2.g; w.fb[p] = f2.b;
-                  w.bpdf[p] = bp2;
-                  w.px[p] = is.p.x; w.py[p] = is.p.y; w.pz[p] = is.p.z;
-                  w.nlight[p] = l;
-                  hm = true;
-                }
-              }
-              if (hs || hm) {
-                w.sel[p] = sel;
-                w.nbr[p] = beta.r; w.nbg[p] = beta.g; w.nbb[p] = beta.b;
-                fl |= F_PEND | (hs ? F_SHADOW : 0) | (hm ? F_MIS : 0);
-                want_shadow = hs;
-                want_ray1 = hm;
-              }
-            }
+          if (bsdf_num(bs, ALL & ~SPEC) > 0) {
+            int nf = setup_direct(sc, w, p, bs, is, d, beta, index, &dim, false);
+            fl |= nf;
+            want_shadow = (nf & F_SHADOW) != 0;
+            want_ray1 = (nf & F_MIS) != 0;
           }
           // BSDF sampling for the continuation (path.cpp:81-96)
           float u1 = sobol(sc.sobol, index, dim++), u0 = sobol(sc.sobol, index, dim++);

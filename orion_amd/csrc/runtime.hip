@@ -360,11 +360,56 @@ static int read_stats(orion_scene s, orion_render_stats *st, float ms) {
     st->photons = (int64_t)h[S_PHOTONS];
     st->deposits = (int64_t)h[S_DEPOSITS];
     st->ms = ms;
+    st->node_visits_closest = (int64_t)h[S_NODES_C];
+    st->tri_tests_closest = (int64_t)h[S_TRIS_C];
+    st->node_visits_shadow = (int64_t)h[S_NODES_S];
+    st->tri_tests_shadow = (int64_t)h[S_TRIS_S];
   }
-  s->last_nodes = h[S_NODES];
-  s->last_tris = h[S_TRIS];
+  s->last_nodes = h[S_NODES_C];
+  s->last_tris = h[S_TRIS_C];
   return ORION_OK;
 }
+
+// Per-launch HIP events (recorded on the launch stream) for kernel timing.
+struct LaunchTimer {
+  bool on = false;
+  hipStream_t st = nullptr;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> kind;  // 0 closest, 1 shadow, 2 shade
+  int begin(int k) {
+    if (!on) return ORION_OK;
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventRecord(e, st));
+    ev.push_back(e);
+    kind.push_back(k);
+    return ORION_OK;
+  }
+  int end() {
+    if (!on) return ORION_OK;
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventRecord(e, st));
+    ev.push_back(e);
+    kind.push_back(-1);
+    return ORION_OK;
+  }
+  // call after the stream has been synchronised
+  int collect(orion_render_stats *s) {
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      if (!s) continue;
+      if (kind[i] == 0) { s->trace_closest_ms += ms; s->trace_closest_launches++; }
+      else if (kind[i] == 1) { s->trace_shadow_ms += ms; s->trace_shadow_launches++; }
+      else { s->shade_ms += ms; s->shade_launches++; }
+    }
+    return ORION_OK;
+  }
+  ~LaunchTimer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
 
 int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, const orion_tuning *tune,
                       orion_render_stats *stats) {
@@ -385,6 +430,9 @@ int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, const orion_t
   int bpc = tune && tune->persistent_blocks > 0 ? tune->persistent_blocks : 6;
   int trace_blocks = c->n_cu * bpc;
   HIP_TRY(hipMemsetAsync(s->stats.ptr, 0, S_COUNT * 8, c->stream));
+  LaunchTimer tm;
+  tm.on = tune && tune->profile;
+  tm.st = c->stream;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   Wave w = s->wave.wave;
   w.ctrl = (int *)s->ctrl.ptr;
@@ -399,10 +447,16 @@ int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, const orion_t
     launch_raygen(s->d, w, c->stream);
     int cur = 0;
     for (int it = 0; it <= maxDepth; ++it) {
+      if ((rc = tm.begin(0))) return rc;
       launch_trace(s->d, w, false, trace_blocks, c->stream);
+      if ((rc = tm.end())) return rc;
+      if ((rc = tm.begin(1))) return rc;
       launch_trace(s->d, w, true, trace_blocks, c->stream);
+      if ((rc = tm.end())) return rc;
       launch_between(w, cur ^ 1, c->stream);
+      if ((rc = tm.begin(2))) return rc;
       launch_shade(s->d, w, cur, maxDepth, c->stream);
+      if ((rc = tm.end())) return rc;
       cur ^= 1;
     }
     launch_film(w, Sk, (float *)s->film.ptr, (int *)s->count.ptr, c->stream);
@@ -412,15 +466,9 @@ int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, const orion_t
   HIP_TRY(hipStreamSynchronize(c->stream));
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if ((rc = tm.collect(stats))) return rc;
   return read_stats(s, stats, ms);
-}
-
-int orion_render_sppm(orion_scene s, int n_iters, int photons, float r0, int maxDepth, int hash_size,
-                      const orion_sppm_debug *dbg, orion_render_stats *stats) {
-  if (!s) { set_error("null scene"); return ORION_ERR_ARG; }
-  (void)n_iters; (void)photons; (void)r0; (void)maxDepth; (void)hash_size; (void)dbg; (void)stats;
-  set_error("SPPM kernels are not built into this library yet");
-  return ORION_ERR_UNSUPPORTED;
 }
 
 }  // extern "C"

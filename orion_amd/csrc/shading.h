@@ -1,0 +1,241 @@
+// Device shading shared by the path tracer and SPPM: camera rays, light
+// selection and sampling, and the deferred direct-lighting estimate
+// (uniformSampleOneLight + estimateDirect, core/integrator.cpp:8-90) split
+// into a setup half (before the shadow / MIS rays are traced) and a resolve
+// half (after).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_scene.h"
+#include "wavefront.h"
+
+namespace orion_amd {
+namespace dev {
+
+// pinhole.cpp:8-33; next2() = (x: dim d+1, y: dim d) as g++ builds
+// Point2f(next(), next()) (sampler.h:31).
+__device__ __forceinline__ void camera_ray(const DScene &sc, int i, int j, uint32_t index, V3 *o, V3 *d) {
+  float uy = sobol(sc.sobol, index, 5);
+  float ux = sobol(sc.sobol, index, 6);
+  float rx = (float)i + ux, ry = (float)j + uy;
+  const float *c = sc.cam;
+  float width = c[14], height = c[15];
+  float xx = (2 * rx / width - 1) * c[13] * c[12];
+  float yy = (1 - 2 * ry / height) * c[12];
+  V3 dir = normalize(mk(xx, yy, -1.0f));
+  *o = mk(c[0], c[1], c[2]);
+  *d = mat3(c + 3, dir);
+}
+
+// Distribution1D::sampleDiscrete (sampling.h:110-120) over light power.
+__device__ __forceinline__ int sample_light(const DScene &sc, float u, float *pdf) {
+  int size = sc.n_lights + 1;
+  int first = 0, len = size;
+  while (len > 0) {
+    int half = len >> 1, mid = first + half;
+    if (sc.light_cdf[mid] <= u) { first = mid + 1; len -= half + 1; }
+    else len = half;
+  }
+  int off = first - 1;
+  off = off < 0 ? 0 : (off > size - 2 ? size - 2 : off);
+  *pdf = (sc.light_func_int > 0) ? sc.light_func[off] / (sc.light_func_int * sc.n_lights) : 0;
+  return off;
+}
+
+__device__ __forceinline__ float spot_falloff(const float *L, V3 w) {  // spot.cpp:10-19
+  V3 wl = normalize(mat3(L + 11, w));
+  float cosTheta = wl.z;
+  if (cosTheta < L[9]) return 0;
+  if (cosTheta > L[10]) return 1;
+  float delta = (cosTheta - L[9]) / (L[10] - L[9]);
+  return (delta * delta) * (delta * delta);
+}
+
+// Triangle::sample (triangle.cpp:85-106): point, face-forwarded ng
+__device__ __forceinline__ void tri_sample(const DScene &sc, int prim, float u0, float u1, V3 *p, V3 *ng) {
+  V3 p0, p1, p2, n0, n1, n2;
+  int m_, l_;
+  load_prim(sc, prim, &p0, &p1, &p2, &n0, &n1, &n2, &m_, &l_);
+  float su0 = sqrtf(u0);
+  float b0 = 1 - su0, b1 = u1 * su0;
+  *p = add(add(mul(p0, b0), mul(p1, b1)), mul(p2, 1 - b0 - b1));
+  V3 g = normalize(cross(sub(p1, p0), sub(p2, p0)));
+  V3 ns = add(add(mul(n0, b0), mul(n1, b1)), mul(n2, 1.0f - b0 - b1));
+  *ng = dot(g, ns) < 0.f ? neg(g) : g;
+}
+
+// Light::sample_Li; *so / *sd = the ShadowTester ray (intersection.h:40-44)
+__device__ inline Sp light_sample_li(const DScene &sc, int l, V3 refp, V3 refng, float u0, float u1, V3 *wi,
+                                     float *pdf, V3 *so, V3 *sd) {
+  const float *L = sc.lights + 32 * (size_t)l;
+  int type = (int)L[0];
+  Sp Le = sp3(L[2], L[3], L[4]);
+  V3 target;
+  if (type == 0) {  // diffuse.cpp:13-25 + triangle.cpp:108-121
+    V3 p, ng;
+    tri_sample(sc, (int)L[1], u0, u1, &p, &ng);
+    float pd = 1.0f / sc.light_area[l];
+    V3 w = sub(p, refp);
+    if (lensq(w) == 0) pd = 0;
+    else {
+      w = normalize(w);
+      pd *= lensq(sub(refp, p)) / absdot(ng, neg(w));
+      if (is_inf(pd)) pd = 0;
+    }
+    if (pd == 0 || lensq(sub(p, refp)) == 0) { *pdf = 0; return sp(0); }
+    *pdf = pd;
+    *wi = normalize(sub(p, refp));
+    target = p;
+    Le = dot(ng, neg(*wi)) > 0 ? Le : sp(0);
+  } else {  // point.cpp:10-19, spot.cpp:20-29
+    V3 pos = mk(L[5], L[6], L[7]);
+    *wi = normalize(sub(pos, refp));
+    *pdf = 1.0f;
+    target = pos;
+    if (type == 1) Le = Le / lensq(sub(pos, refp));
+    else Le = Le * spot_falloff(L, neg(*wi)) / lensq(sub(pos, refp));
+  }
+  *so = offset_ray_origin(refp, refng, sub(target, refp));
+  *sd = sub(target, *so);
+  return Le;
+}
+
+// Light::sample_Le (diffuse.cpp:32-45, point.cpp:20-27, spot.cpp:30-38)
+__device__ inline Sp light_sample_le(const DScene &sc, int l, float a0, float a1, float b0, float b1, V3 *ro,
+                                     V3 *rd, V3 *nLight, float *pdfPos, float *pdfDir) {
+  const float *L = sc.lights + 32 * (size_t)l;
+  int type = (int)L[0];
+  Sp Le = sp3(L[2], L[3], L[4]);
+  if (type == 0) {
+    V3 p, ng;
+    tri_sample(sc, (int)L[1], a0, a1, &p, &ng);
+    *pdfPos = 1.0f / sc.light_area[l];
+    *nLight = ng;
+    V3 w = cosine_hemisphere(b0, b1);
+    *pdfDir = w.z * kInvPi;
+    V3 x, y;
+    coordinate_system(ng, &x, &y);
+    w = add(add(mul(x, w.x), mul(y, w.y)), mul(ng, w.z));
+    *ro = offset_ray_origin(p, ng, w);
+    *rd = w;
+    return dot(ng, w) > 0 ? Le : sp(0);
+  }
+  V3 pos = mk(L[5], L[6], L[7]);
+  *ro = pos;
+  *pdfPos = 1;
+  if (type == 1) {
+    *rd = uniform_sphere(a0, a1);
+    *nLight = *rd;
+    *pdfDir = kInv4Pi;
+    return Le;
+  }
+  V3 w = uniform_cone(a0, a1, L[9]);
+  *rd = mat3(L + 20, w);
+  *nLight = *rd;
+  *pdfDir = 1 / (2 * kPi * (1 - L[9]));
+  return Le * spot_falloff(L, *rd);
+}
+
+// Second half of estimateDirect: adds the pending light-sampled term (when
+// its shadow ray was unoccluded) and the BSDF-sampled term (when its MIS ray
+// hit the sampled light's triangle) to L, as L += beta * (Ld / lightPdf).
+__device__ inline void resolve_direct(const DScene &sc, const Wave &w, int p, int fl, Sp *L) {
+  const int N = w.N;
+  Sp Ld = sp(0);
+  if ((fl & F_SHADOW) && w.vis[p]) acc(Ld, sp3(w.Ar[p], w.Ag[p], w.Ab[p]));
+  if (fl & F_MIS) {
+    int hp = w.hprim[N + p];
+    int l = w.nlight[p];
+    const float *LR = sc.lights + 32 * (size_t)l;
+    if (hp >= 0 && hp == (int)LR[1]) {
+      // Triangle::pdf (triangle.cpp:123-135) on the MIS ray's hit
+      Isect in;
+      make_isect(sc, hp, w.hu[N + p], w.hv[N + p], &in);
+      V3 wi = mk(w.dx[N + p], w.dy[N + p], w.dz[N + p]);
+      V3 pp = mk(w.px[p], w.py[p], w.pz[p]);
+      float lpdf = lensq(sub(pp, in.p)) / (absdot(in.ng, neg(wi)) * sc.light_area[l]);
+      if (is_inf(lpdf)) lpdf = 0;
+      if (lpdf != 0) {
+        float bp = w.bpdf[p];
+        float weight = power_heuristic(bp, lpdf);
+        Sp Lm = dot(in.ng, neg(wi)) > 0 ? sp3(LR[2], LR[3], LR[4]) : sp(0);
+        if (!black(Lm)) acc(Ld, sp3(w.fr[p], w.fg[p], w.fb[p]) * Lm * weight / bp);
+      }
+    }
+  }
+  acc(*L, sp3(w.nbr[p], w.nbg[p], w.nbb[p]) * (Ld / w.sel[p]));
+}
+
+// First half: light choice (power CDF, or uniform when `uniform`), light and
+// BSDF samples, the light-sampled term and the two rays.  Draws exactly the
+// reference's dimensions.  Returns the F_PEND | F_SHADOW | F_MIS bits.
+__device__ inline int setup_direct(const DScene &sc, const Wave &w, int p, const Bsdf &bs, const Isect &is, V3 d,
+                                   Sp beta, uint32_t index, int *dim, bool uniform) {
+  if (sc.n_lights == 0) return 0;
+  const int N = w.N;
+  float sel;
+  int l;
+  if (!uniform) {
+    l = sample_light(sc, sobol(sc.sobol, index, (*dim)++), &sel);
+    if (sel == 0) return 0;
+  } else {
+    int nl = sc.n_lights;
+    l = min((int)(sobol(sc.sobol, index, (*dim)++) * nl), nl - 1);
+    sel = 1.0f / nl;
+  }
+  float lu1 = sobol(sc.sobol, index, (*dim)++), lu0 = sobol(sc.sobol, index, (*dim)++);
+  float bu1 = sobol(sc.sobol, index, (*dim)++), bu0 = sobol(sc.sobol, index, (*dim)++);
+  const int nonspec = ALL & ~SPEC;
+  V3 wo = normalize(neg(d));
+  V3 wi = mk(0, 0, 0), so, sdir;
+  float lpdf = 0;
+  Sp Li = light_sample_li(sc, l, is.p, is.ng, lu0, lu1, &wi, &lpdf, &so, &sdir);
+  bool delta = (int)sc.lights[32 * (size_t)l] != 0;
+  bool hs = false, hm = false;
+  if (lpdf > 0 && !black(Li)) {
+    Sp f = bsdf_f(bs, wi, wo, nonspec) * absdot(wi, is.ns);
+    float bpdf = bsdf_pdf(bs, wi, wo, nonspec);
+    if (!black(f)) {
+      Sp A;
+      if (delta) A = f * Li / lpdf;
+      else A = f * Li * power_heuristic(lpdf, bpdf) / lpdf;
+      w.Ar[p] = A.r; w.Ag[p] = A.g; w.Ab[p] = A.b;
+      int s2 = 2 * N + p;
+      w.ox[s2] = so.x; w.oy[s2] = so.y; w.oz[s2] = so.z;
+      w.dx[s2] = sdir.x; w.dy[s2] = sdir.y; w.dz[s2] = sdir.z;
+      w.tmax[s2] = 1 - kShadowEps;
+      hs = true;
+    }
+  }
+  if (!delta) {
+    float bp2 = 0;
+    int st = 0;
+    V3 wi2 = wi;
+    Sp f2 = bsdf_sample(bs, &wi2, wo, bu0, bu1, &bp2, nonspec, &st);
+    mulby(f2, sp(absdot(wi2, is.ns)));
+    if (!black(f2) && bp2 > 0) {
+      V3 o2 = offset_ray_origin(is.p, is.ng, wi2);
+      int s1 = N + p;
+      w.ox[s1] = o2.x; w.oy[s1] = o2.y; w.oz[s1] = o2.z;
+      w.dx[s1] = wi2.x; w.dy[s1] = wi2.y; w.dz[s1] = wi2.z;
+      w.tmax[s1] = kFltMax;
+      w.fr[p] = f2.r; w.fg[p] = f2.g; w.fb[p] = f2.b;
+      w.bpdf[p] = bp2;
+      w.px[p] = is.p.x; w.py[p] = is.p.y; w.pz[p] = is.p.z;
+      w.nlight[p] = l;
+      hm = true;
+    }
+  }
+  if (!(hs || hm)) return 0;
+  w.sel[p] = sel;
+  w.nbr[p] = beta.r; w.nbg[p] = beta.g; w.nbb[p] = beta.b;
+  return F_PEND | (hs ? F_SHADOW : 0) | (hm ? F_MIS : 0);
+}
+
+__device__ __forceinline__ Sp area_emission(const DScene &sc, const Isect &is, V3 w) {  // diffuse.cpp:53-56
+  const float *LR = sc.lights + 32 * (size_t)is.light;
+  return dot(is.ng, w) > 0 ? sp3(LR[2], LR[3], LR[4]) : sp(0);
+}
+
+}  // namespace dev
+}  // namespace orion_amd

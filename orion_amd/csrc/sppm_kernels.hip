@@ -1,16 +1,882 @@
 // SPPM (SPPM::render, integrator/sppm.cpp:58-322) on gfx950.
+//
+// Per iteration:
+//   camera pass   wavefront over pixels (closest + shadow + MIS rays, the
+//                 deferred direct lighting of the path tracer with a uniform
+//                 light pick), storing a visible point at the first diffuse
+//                 hit (or glossy hit at the last depth)
+//   grid          bounds / resolution reduction, per visible point the cells
+//                 its radius overlaps, (hash key, pixel) pairs, a stable LSD
+//                 radix sort of the keys (8-bit digits, wave64 ballot ranking),
+//                 per-cell [start, end) ranges
+//   photon pass   wavefront over photons: light emission, closest hits,
+//                 deposits into the visible points of the hashed cell (float
+//                 atomics for Phi, integer atomics for M and per-cell counts),
+//                 BSDF sampling in Photon mode, Russian roulette
+//   update        radius / N / tau update, then the per-iteration film add
+// The hash keeps the reference's collisions: a visible point overlapping two
+// cells that hash alike is listed (and deposited into) twice, as with orion's
+// per-cell linked lists.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/orion_amd.h"
 #include "device_scene.h"
+#include "host_scene.h"
 #include "kernels.h"
 #include "runtime.h"
+#include "shading.h"
 
 namespace orion_amd {
+using namespace dev;
+
+#define HIP_TRY(expr)                                                         \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));           \
+      return ORION_ERR_HIP;                                                   \
+    }                                                                         \
+  } while (0)
+
+struct Pix {  // per-pixel state, SoA
+  float *radius, *N, *Ldr, *Ldg, *Ldb, *taur, *taug, *taub, *Phir, *Phig, *Phib;
+  int *M;
+  // visible point
+  float *vpx, *vpy, *vpz, *vwox, *vwoy, *vwoz;
+  float *vng, *vns, *vu, *vv;  // 3 floats per pixel each (xyz interleaved)
+  int *vmat;
+  float *vbr, *vbg, *vbb;
+};
+
+struct Grid {
+  float *fctl;      // [0..2] lo, [3..5] hi, [6] maxRadius
+  int *ictl;        // [0..2] gridRes, [3] any, [4..9] sortable lo/hi, [10] maxRadius bits, [11] n_pairs
+  int *cnt;         // per pixel pair counts -> exclusive offsets (npix + 1)
+  uint32_t *keys[2];
+  int *vals[2];
+  int *cell_start, *cell_end, *photon_cell;
+  int *hist;        // radix histogram (256 x tiles) -> offsets
+  int *scan_tmp;
+  int hash_size;
+};
 
 struct SppmBuffers {
-  DevBuf mem;
+  DevBuf pix_mem, grid_mem, pair_mem, hist_mem, scan_mem;
+  Pix px{};
+  Grid g{};
+  int npix = 0, hash_size = 0;
+  size_t pair_cap = 0, hist_cap = 0, scan_cap = 0;
 };
 
 void destroy_sppm(SppmBuffers *b) { delete b; }
 
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ int f2key(float f) {  // order-preserving float -> int
+  int i = __float_as_int(f);
+  return i >= 0 ? i : (i ^ 0x7fffffff);
+}
+__device__ __forceinline__ float key2f(int k) { return __int_as_float(k >= 0 ? k : (k ^ 0x7fffffff)); }
+
+// ToGrid (sppm.cpp:40-50): truncation toward zero, then clamp
+__device__ __forceinline__ bool to_grid(V3 p, const float *lo, const float *hi, const int *res, int *c) {
+  bool in = true;
+  float o[3] = {p.x - lo[0], p.y - lo[1], p.z - lo[2]};
+  for (int a = 0; a < 3; ++a)
+    if (hi[a] > lo[a]) o[a] /= hi[a] - lo[a];
+  for (int a = 0; a < 3; ++a) {
+    c[a] = (int)(res[a] * o[a]);
+    in &= (c[a] >= 0 && c[a] < res[a]);
+    c[a] = c[a] < 0 ? 0 : (c[a] > res[a] - 1 ? res[a] - 1 : c[a]);
+  }
+  return in;
+}
+// hash (sppm.cpp:52-56); int products wrap, so they are formed in uint32
+__device__ __forceinline__ uint32_t grid_hash(int x, int y, int z, int hs) {
+  uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u);
+  return h % (uint32_t)hs;
+}
+
+__device__ __forceinline__ V3 ld3(const float *a, int i) { return mk(a[3 * i], a[3 * i + 1], a[3 * i + 2]); }
+__device__ __forceinline__ void st3(float *a, int i, V3 v) { a[3 * i] = v.x; a[3 * i + 1] = v.y; a[3 * i + 2] = v.z; }
+
+// ---------------------------------------------------------------- camera pass
+__global__ void k_cam_raygen(DScene sc, Wave w, Pix px, int iter) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) {
+    w.ctrl[C_PATH0] = w.N;
+    w.ctrl[C_RAY] = w.N;
+    w.ctrl[C_SHADOW] = 0;
+    w.ctrl[C_WORK_RAY] = 0;
+    w.ctrl[C_WORK_SHADOW] = 0;
+  }
+  if (p >= w.N) return;
+  int j = p / w.W, i = p - j * w.W;
+  uint32_t index = (uint32_t)iter * (uint32_t)w.npix + (uint32_t)p;  // sppm.cpp:86
+  V3 o, d;
+  camera_ray(sc, i, j, index, &o, &d);
+  w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
+  w.dx[p] = d.x; w.dy[p] = d.y; w.dz[p] = d.z;
+  w.tmax[p] = kFltMax;
+  w.br[p] = 1.f; w.bg[p] = 1.f; w.bb[p] = 1.f;
+  w.dim[p] = 7;
+  w.bounce[p] = 0;
+  w.flags[p] = F_ALIVE;
+  w.qpath[0][p] = p;
+  w.qray[p] = p;
+}
+
+// One camera-pass step (sppm.cpp:93-138).  w.L* alias the pixels' Ld.
+__global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, int cur, int maxDepth, int iter) {
+  const int nxt = cur ^ 1;
+  const int count = w.ctrl[C_PATH0 + cur];
+  const int N = w.N;
+  for (int base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+    int i = base + threadIdx.x;
+    bool act = i < count;
+    int p = act ? w.qpath[cur][i] : 0;
+    bool want_ray0 = false, want_ray1 = false, want_shadow = false, keep = false;
+    if (act) {
+      int fl = w.flags[p];
+      Sp L = sp3(w.Lr[p], w.Lg[p], w.Lb[p]);
+      if (fl & F_PEND) {
+        resolve_direct(sc, w, p, fl, &L);
+        fl &= ~(F_PEND | F_SHADOW | F_MIS);
+      }
+      if (fl & F_ALIVE) {
+        fl &= ~F_ALIVE;
+        int depth = w.bounce[p];
+        int hp = w.hprim[p];
+        if (hp >= 0) {
+          Sp beta = sp3(w.br[p], w.bg[p], w.bb[p]);
+          V3 d = mk(w.dx[p], w.dy[p], w.dz[p]);
+          Isect is;
+          make_isect(sc, hp, w.hu[p], w.hv[p], &is);
+          Bsdf bs;
+          make_bsdf(sc.mats + 16 * (size_t)is.mat, is.ng, is.ns, is.u, is.v, false, &bs);
+          V3 wo = neg(d);
+          if ((depth == 0 || (fl & F_SPEC)) && is.light >= 0) acc(L, beta * area_emission(sc, is, wo));
+          uint32_t index = (uint32_t)iter * (uint32_t)w.npix + (uint32_t)p;
+          int dim = w.dim[p];
+          int nf = setup_direct(sc, w, p, bs, is, d, beta, index, &dim, true);
+          fl |= nf;
+          want_shadow = (nf & F_SHADOW) != 0;
+          want_ray1 = (nf & F_MIS) != 0;
+          bool isDiffuse = bsdf_num(bs, DIFF | REFL | TRANS) > 0;
+          bool isGlossy = bsdf_num(bs, GLOSSY | REFL | TRANS) > 0;
+          if (isDiffuse || (isGlossy && depth == maxDepth - 1)) {
+            px.vpx[p] = is.p.x; px.vpy[p] = is.p.y; px.vpz[p] = is.p.z;
+            px.vwox[p] = wo.x; px.vwoy[p] = wo.y; px.vwoz[p] = wo.z;
+            st3(px.vng, p, is.ng); st3(px.vns, p, is.ns); st3(px.vu, p, is.u); st3(px.vv, p, is.v);
+            px.vmat[p] = is.mat;
+            px.vbr[p] = beta.r; px.vbg[p] = beta.g; px.vbb[p] = beta.b;
+          } else if (depth < maxDepth - 1) {
+            float u1 = sobol(sc.sobol, index, dim++), u0 = sobol(sc.sobol, index, dim++);
+            V3 wi = mk(0, 0, 0);
+            float pdf = 0;
+            int type = 0;
+            Sp f = bsdf_sample(bs, &wi, wo, u0, u1, &pdf, ALL, &type);
+            if (!(pdf == 0.f || black(f))) {
+              fl = (type & SPEC) ? (fl | F_SPEC) : (fl & ~F_SPEC);
+              mulby(beta, f * absdot(wi, is.ns) / pdf);
+              bool alive = true;
+              if (intensity(beta) < 0.25f) {
+                float cp = mn(1.0f, intensity(beta));
+                if (sobol(sc.sobol, index, dim++) > cp) alive = false;
+                else beta = beta / cp;
+              }
+              if (alive) {
+                V3 o = offset_ray_origin(is.p, is.ng, wi);
+                w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
+                w.dx[p] = wi.x; w.dy[p] = wi.y; w.dz[p] = wi.z;
+                w.tmax[p] = kFltMax;
+                w.bounce[p] = depth + 1;
+                fl |= F_ALIVE;
+                want_ray0 = true;
+              }
+            }
+          }
+          w.br[p] = beta.r; w.bg[p] = beta.g; w.bb[p] = beta.b;
+          w.dim[p] = dim;
+        }
+      }
+      w.Lr[p] = L.r; w.Lg[p] = L.g; w.Lb[p] = L.b;
+      w.flags[p] = fl;
+      keep = (fl & (F_ALIVE | F_PEND)) != 0;
+    }
+    int a = wave_append(&w.ctrl[C_PATH0 + nxt], keep);
+    if (keep) w.qpath[nxt][a] = p;
+    a = wave_append(&w.ctrl[C_RAY], want_ray0);
+    if (want_ray0) w.qray[a] = p;
+    a = wave_append(&w.ctrl[C_RAY], want_ray1);
+    if (want_ray1) w.qray[a] = N + p;
+    a = wave_append(&w.ctrl[C_SHADOW], want_shadow);
+    if (want_shadow) w.qshadow[a] = p;
+  }
+}
+
+// ---------------------------------------------------------------- grid
+__global__ void k_grid_init(Grid g) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const int big = 0x7f7fffff;  // FLT_MAX as a sortable key
+    for (int a = 0; a < 3; ++a) { g.ictl[4 + a] = big; g.ictl[7 + a] = f2key(-3.40282347e+38f); }
+    g.ictl[3] = 0;
+    g.ictl[10] = 0;
+    g.ictl[11] = 0;
+  }
+}
+
+// bounds = union of expand(vp.p, r) over visible points (sppm.cpp:149-157)
+__global__ __launch_bounds__(256) void k_grid_bounds(Pix px, Grid g, int npix) {
+  __shared__ int red[8][7];
+  int lo[3], hi[3], mr = 0, any = 0;
+  for (int a = 0; a < 3; ++a) { lo[a] = 0x7f7fffff; hi[a] = f2key(-3.40282347e+38f); }
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+    if (black(sp3(px.vbr[p], px.vbg[p], px.vbb[p]))) continue;
+    float r = px.radius[p];
+    float q[3] = {px.vpx[p], px.vpy[p], px.vpz[p]};
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = min(lo[a], f2key(q[a] - r));
+      hi[a] = max(hi[a], f2key(q[a] + r));
+    }
+    mr = max(mr, f2key(r));
+    any = 1;
+  }
+  int vals[8] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], mr, any};
+  for (int off = 32; off > 0; off >>= 1)
+    for (int k = 0; k < 8; ++k) {
+      int o = __shfl_down(vals[k], off);
+      vals[k] = k < 3 ? min(vals[k], o) : max(vals[k], o);
+    }
+  if (vals[7] == 0 && lane_id() == 0) return;
+  if (lane_id() == 0) {
+    for (int a = 0; a < 3; ++a) { atomicMin(&g.ictl[4 + a], vals[a]); atomicMax(&g.ictl[7 + a], vals[3 + a]); }
+    atomicMax(&g.ictl[10], vals[6]);
+    atomicMax(&g.ictl[3], 1);
+  }
+  (void)red;
+}
+
+// grid resolution (sppm.cpp:159-163)
+__global__ void k_grid_res(Grid g) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (!g.ictl[3]) { g.ictl[0] = g.ictl[1] = g.ictl[2] = 1; return; }
+  float lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) { lo[a] = key2f(g.ictl[4 + a]); hi[a] = key2f(g.ictl[7 + a]); }
+  float maxRadius = key2f(g.ictl[10]);
+  float diag[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+  float maxDiag = mx(diag[0], mx(diag[1], diag[2]));
+  int baseGridRes = (int)(maxDiag / maxRadius);
+  for (int a = 0; a < 3; ++a) {
+    int r = (int)(baseGridRes * diag[a] / maxDiag);
+    g.ictl[a] = r > 1 ? r : 1;
+    g.fctl[a] = lo[a];
+    g.fctl[3 + a] = hi[a];
+  }
+  g.fctl[6] = maxRadius;
+}
+
+__device__ __forceinline__ void vp_cells(const Pix &px, const Grid &g, int p, int *pmin, int *pmax) {
+  float r = px.radius[p];
+  V3 q = mk(px.vpx[p], px.vpy[p], px.vpz[p]);
+  to_grid(sub(q, mk(r, r, r)), g.fctl, g.fctl + 3, g.ictl, pmin);
+  to_grid(add(q, mk(r, r, r)), g.fctl, g.fctl + 3, g.ictl, pmax);
+}
+
+__global__ void k_grid_count(Pix px, Grid g, int npix) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > npix) return;
+  int c = 0;
+  if (p < npix && g.ictl[3] && !black(sp3(px.vbr[p], px.vbg[p], px.vbb[p]))) {
+    int a[3], b[3];
+    vp_cells(px, g, p, a, b);
+    c = (b[2] - a[2] + 1) * (b[1] - a[1] + 1) * (b[0] - a[0] + 1);
+  }
+  g.cnt[p] = c;
+}
+
+// (key, pixel) pairs in the reference's z, y, x cell order (sppm.cpp:176-193)
+__global__ void k_grid_emit(Pix px, Grid g, int npix) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  int off = g.cnt[p], end = g.cnt[p + 1];
+  if (off == end) return;
+  int a[3], b[3];
+  vp_cells(px, g, p, a, b);
+  for (int z = a[2]; z <= b[2]; ++z)
+    for (int y = a[1]; y <= b[1]; ++y)
+      for (int x = a[0]; x <= b[0]; ++x) {
+        g.keys[0][off] = grid_hash(x, y, z, g.hash_size);
+        g.vals[0][off] = p;
+        ++off;
+      }
+}
+
+// ---------------------------------------------------------------- scan (exclusive, int)
+constexpr int kScanBlock = 256, kScanItems = 4, kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ int block_excl_scan(int v, int *lds, int *total) {
+  int lane = lane_id(), wv = threadIdx.x / 64;
+  int x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    int y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) lds[wv] = x;
+  __syncthreads();
+  int nw = blockDim.x / 64;
+  int pre = 0, tot = 0;
+  for (int k = 0; k < nw; ++k) {
+    int s = lds[k];
+    if (k < wv) pre += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + x - v;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const int *in, int *out, int *sums, int n) {
+  __shared__ int lds[8];
+  int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int v[kScanItems], s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0;
+    s += v[k];
+  }
+  int tot;
+  int pre = block_excl_scan(s, lds, &tot);
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = pre;
+    pre += v[k];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+__global__ void k_scan_add(int *out, const int *sums, int n) {
+  int i = blockIdx.x * kScanTile + threadIdx.x;
+  int add = sums[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k, i += kScanBlock)
+    if (i < n) out[i] += add;
+}
+
+// exclusive scan of n ints (in may equal out); tmp must hold scan_tmp_ints(n)
+static size_t scan_tmp_ints(size_t n) {
+  size_t t = 0;
+  while (n > 1) {
+    n = (n + kScanTile - 1) / kScanTile;
+    t += n + 1;
+  }
+  return t + 1;
+}
+static void device_scan(const int *in, int *out, int n, int *tmp, hipStream_t st) {
+  int nb = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_tiles, dim3(std::max(nb, 1)), dim3(kScanBlock), 0, st, in, out, tmp, n);
+  if (nb > 1) {
+    device_scan(tmp, tmp, nb, tmp + nb + 1, st);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kScanBlock), 0, st, out, tmp, n);
+  }
+}
+
+// ---------------------------------------------------------------- stable LSD radix sort (8-bit digits)
+constexpr int kSortBlock = 256, kSortRows = 16, kSortTile = kSortRows * 64;  // 1024 keys per tile
+
+__global__ __launch_bounds__(kSortBlock) void k_sort_hist(const uint32_t *keys, int n, int shift, int *hist, int ntiles) {
+  __shared__ int h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  int base = blockIdx.x * kSortTile;
+  for (int k = threadIdx.x; k < kSortTile; k += kSortBlock)
+    if (base + k < n) atomicAdd(&h[(keys[base + k] >> shift) & 255], 1);
+  __syncthreads();
+  hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];  // digit-major
+}
+
+// Scatter with stable ranks: row r of the tile (64 keys) computes, per key,
+// the lanes holding the same digit with 8 ballots; rows are combined in
+// order through an LDS table of per-row digit counts.
+__global__ __launch_bounds__(kSortBlock) void k_sort_scatter(const uint32_t *kin, const int *vin, uint32_t *kout,
+                                                             int *vout, int n, int shift, const int *offs,
+                                                             int ntiles) {
+  __shared__ int rowcnt[kSortRows][256];
+  __shared__ int base_off[256];
+  for (int k = threadIdx.x; k < kSortRows * 256; k += kSortBlock) (&rowcnt[0][0])[k] = 0;
+  base_off[threadIdx.x] = offs[threadIdx.x * ntiles + blockIdx.x];
+  __syncthreads();
+  const int lane = lane_id(), wv = threadIdx.x / 64;
+  const int tile0 = blockIdx.x * kSortTile;
+  const int rows_per_wave = kSortRows / (kSortBlock / 64);
+  int digit[4], rank[4];
+  bool valid[4];
+  unsigned long long below = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+  for (int q = 0; q < rows_per_wave; ++q) {
+    int row = wv * rows_per_wave + q;
+    int idx = tile0 + row * 64 + lane;
+    valid[q] = idx < n;
+    int d = valid[q] ? (int)((kin[idx] >> shift) & 255) : 256 + lane;  // invalid lanes never match
+    unsigned long long m = __ballot(valid[q]);
+    for (int b = 0; b < 8; ++b) {
+      unsigned long long bal = __ballot((d >> b) & 1);
+      m &= ((d >> b) & 1) ? bal : ~bal;
+    }
+    digit[q] = d;
+    rank[q] = __popcll(m & below);
+    if (valid[q] && rank[q] == 0) rowcnt[row][d] = __popcll(m);
+  }
+  __syncthreads();
+  // exclusive prefix over rows, per digit (thread = digit)
+  {
+    int d = threadIdx.x, acc = 0;
+    for (int r = 0; r < kSortRows; ++r) {
+      int c = rowcnt[r][d];
+      rowcnt[r][d] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  for (int q = 0; q < rows_per_wave; ++q) {
+    if (!valid[q]) continue;
+    int row = wv * rows_per_wave + q;
+    int idx = tile0 + row * 64 + lane;
+    int pos = base_off[digit[q]] + rowcnt[row][digit[q]] + rank[q];
+    kout[pos] = kin[idx];
+    vout[pos] = vin[idx];
+  }
+}
+
+__global__ void k_cell_clear(Grid g) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < g.hash_size) { g.cell_start[i] = 0; g.cell_end[i] = 0; g.photon_cell[i] = 0; }
+}
+__global__ void k_cell_ranges(Grid g, const uint32_t *keys, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k = keys[i];
+  if (i == 0 || keys[i - 1] != k) g.cell_start[k] = i;
+  if (i == n - 1 || keys[i + 1] != k) g.cell_end[k] = i + 1;
+}
+__global__ void k_cell_counts(Grid g, int *out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < g.hash_size) out[i] = g.cell_end[i] - g.cell_start[i];
+}
+
+// ---------------------------------------------------------------- photon pass
+__global__ void k_photon_init(DScene sc, Wave w, uint32_t index0, int P) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) {
+    w.ctrl[C_WORK_RAY] = 0;
+    w.ctrl[C_SHADOW] = 0;
+  }
+  if (p >= P) return;
+  uint32_t index = index0 + (uint32_t)p;  // sppm.cpp:203
+  float lightPdf;
+  int l = sample_light(sc, sobol(sc.sobol, index, 5), &lightPdf);
+  float a1 = sobol(sc.sobol, index, 6), a0 = sobol(sc.sobol, index, 7);
+  float b1 = sobol(sc.sobol, index, 8), b0 = sobol(sc.sobol, index, 9);
+  V3 o, d, nl;
+  float pdfPos, pdfDir;
+  Sp Le = light_sample_le(sc, l, a0, a1, b0, b1, &o, &d, &nl, &pdfPos, &pdfDir);
+  bool alive = !(pdfPos == 0 || pdfDir == 0 || black(Le));
+  Sp beta = sp(0);
+  if (alive) {
+    beta = Le * absdot(nl, d) / (lightPdf * pdfPos * pdfDir);
+    alive = !black(beta);
+  }
+  w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
+  w.dx[p] = d.x; w.dy[p] = d.y; w.dz[p] = d.z;
+  w.tmax[p] = kFltMax;
+  w.br[p] = beta.r; w.bg[p] = beta.g; w.bb[p] = beta.b;
+  w.dim[p] = 10;
+  int a = wave_append(&w.ctrl[C_RAY], alive);
+  if (alive) w.qray[a] = p;
+}
+
+__global__ void k_photon_between(Wave w, unsigned long long *stats) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (stats) stats[S_CLOSEST] += (unsigned long long)w.ctrl[C_RAY];
+    w.ctrl[C_PATH0] = w.ctrl[C_RAY];  // photons to shade = photons just traced
+    w.ctrl[C_SHADOW] = 0;             // next queue length
+    w.ctrl[C_WORK_RAY] = 0;
+  }
+}
+
+// One photon bounce (sppm.cpp:225-271).  Reads the traced queue (qray,
+// count in C_PATH0), appends survivors to qpath[0] (count in C_SHADOW),
+// then the host swaps them back for the next trace.
+__global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px, Grid g, uint32_t index0,
+                                                      int depth, unsigned long long *stats) {
+  const int count = w.ctrl[C_PATH0];
+  unsigned long long ndep = 0;
+  for (int base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+    int i = base + threadIdx.x;
+    bool keep = false;
+    int p = 0;
+    if (i < count) {
+      p = w.qray[i];
+      int hp = w.hprim[p];
+      if (hp >= 0) {
+        Isect is;
+        make_isect(sc, hp, w.hu[p], w.hv[p], &is);
+        V3 d = mk(w.dx[p], w.dy[p], w.dz[p]);
+        Sp beta = sp3(w.br[p], w.bg[p], w.bb[p]);
+        if (depth > 0) {
+          int c[3];
+          if (to_grid(is.p, g.fctl, g.fctl + 3, g.ictl, c)) {
+            uint32_t h = grid_hash(c[0], c[1], c[2], g.hash_size);
+            atomicAdd(&g.photon_cell[h], 1);
+            int e0 = g.cell_start[h], e1 = g.cell_end[h];
+            for (int e = e0; e < e1; ++e) {
+              int q = g.vals[0][e];
+              float r = px.radius[q];
+              V3 vp = mk(px.vpx[q], px.vpy[q], px.vpz[q]);
+              if (lensq(sub(vp, is.p)) > r * r) continue;
+              Bsdf vb;
+              make_bsdf(sc.mats + 16 * (size_t)px.vmat[q], ld3(px.vng, q), ld3(px.vns, q), ld3(px.vu, q),
+                        ld3(px.vv, q), false, &vb);
+              Sp Phi = beta * bsdf_f(vb, neg(d), mk(px.vwox[q], px.vwoy[q], px.vwoz[q]), ALL);
+              atomicAdd(&px.Phir[q], Phi.r);
+              atomicAdd(&px.Phig[q], Phi.g);
+              atomicAdd(&px.Phib[q], Phi.b);
+              atomicAdd(&px.M[q], 1);
+              ++ndep;
+            }
+          }
+        }
+        Bsdf bs;
+        make_bsdf(sc.mats + 16 * (size_t)is.mat, is.ng, is.ns, is.u, is.v, true, &bs);
+        uint32_t index = index0 + (uint32_t)p;
+        int dim = w.dim[p];
+        float u1 = sobol(sc.sobol, index, dim++), u0 = sobol(sc.sobol, index, dim++);
+        V3 wi = mk(0, 0, 0);
+        float pdf = 0;
+        int flags = 0;
+        Sp fr = bsdf_sample(bs, &wi, neg(d), u0, u1, &pdf, ALL, &flags);
+        if (!(black(fr) || pdf == 0)) {
+          Sp bnew = beta * fr * absdot(wi, is.ns) / pdf;
+          float q = mx(0.0f, 1 - intensity(bnew) / intensity(beta));
+          if (!(sobol(sc.sobol, index, dim++) < q)) {
+            beta = bnew / (1 - q);
+            V3 o = offset_ray_origin(is.p, is.ng, wi);
+            w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
+            w.dx[p] = wi.x; w.dy[p] = wi.y; w.dz[p] = wi.z;
+            w.br[p] = beta.r; w.bg[p] = beta.g; w.bb[p] = beta.b;
+            keep = true;
+          }
+        }
+        w.dim[p] = dim;
+      }
+    }
+    int a = wave_append(&w.ctrl[C_SHADOW], keep);
+    if (keep) w.qpath[0][a] = p;
+  }
+  for (int off = 32; off > 0; off >>= 1) ndep += __shfl_down(ndep, off);
+  if (lane_id() == 0 && ndep && stats) atomicAdd(&stats[S_DEPOSITS], ndep);
+}
+
+__global__ void k_photon_swap(Wave w) {
+  // survivors (qpath[0], count C_SHADOW) become the next trace queue
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    w.ctrl[C_RAY] = w.ctrl[C_SHADOW];
+    w.ctrl[C_WORK_RAY] = 0;
+  }
+}
+__global__ void k_copy_queue(Wave w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < w.ctrl[C_RAY]) w.qray[i] = w.qpath[0][i];
+}
+
+// ---------------------------------------------------------------- update + film
+__global__ void k_pixel_update(Pix px, int npix, int *M_out) {  // sppm.cpp:286-310
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  int M = px.M[i];
+  if (M_out) M_out[i] = M;
+  if (M > 0) {
+    float gamma = (float)2 / (float)3;
+    float N = px.N[i], r = px.radius[i];
+    float Nnew = N + gamma * M;
+    float Rnew = r * sqrtf(Nnew / (N + M));
+    Sp Phi = sp3(px.Phir[i], px.Phig[i], px.Phib[i]);
+    Sp tau = sp3(px.taur[i], px.taug[i], px.taub[i]);
+    tau = (tau + sp3(px.vbr[i], px.vbg[i], px.vbb[i]) * Phi) * (Rnew * Rnew) / (r * r);
+    px.taur[i] = tau.r; px.taug[i] = tau.g; px.taub[i] = tau.b;
+    px.N[i] = Nnew;
+    px.radius[i] = Rnew;
+    px.M[i] = 0;
+    px.Phir[i] = 0; px.Phig[i] = 0; px.Phib[i] = 0;
+  }
+  px.vbr[i] = 0.f; px.vbg[i] = 0.f; px.vbb[i] = 0.f;
+}
+__global__ void k_sppm_film(Pix px, int npix, float nIterF, float NpF, float *film, int *count) {  // sppm.cpp:313-320
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  float r = px.radius[i];
+  Sp L = sp3(px.Ldr[i], px.Ldg[i], px.Ldb[i]) / nIterF;
+  acc(L, sp3(px.taur[i], px.taug[i], px.taub[i]) / (NpF * kPi * r * r));
+  film[3 * i] += L.r;
+  film[3 * i + 1] += L.g;
+  film[3 * i + 2] += L.b;
+  count[i] += 1;
+}
+__global__ void k_pix_init(Pix px, int npix, float r0) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  px.radius[i] = r0;
+  px.N[i] = 0;
+  px.Ldr[i] = px.Ldg[i] = px.Ldb[i] = 0;
+  px.taur[i] = px.taug[i] = px.taub[i] = 0;
+  px.Phir[i] = px.Phig[i] = px.Phib[i] = 0;
+  px.M[i] = 0;
+  px.vbr[i] = px.vbg[i] = px.vbb[i] = 0;
+}
+
+// ---------------------------------------------------------------- host driver
+static int ensure_buffers(orion_scene s, int hash_size) {
+  if (!s->sppm) s->sppm = new SppmBuffers();
+  SppmBuffers &b = *s->sppm;
+  int npix = s->W * s->H;
+  if (b.npix != npix) {
+    size_t per = 12 + 6 + 12 + 1 + 3;  // pixel state, vp p/wo, frame, mat, beta
+    int rc = b.pix_mem.ensure(per * (size_t)npix * 4 + 256);
+    if (rc) return rc;
+    char *m = (char *)b.pix_mem.ptr;
+    size_t off = 0;
+    auto take = [&](size_t n) { void *q = m + off; off += n * 4; return q; };
+    Pix &p = b.px;
+    p.radius = (float *)take(npix); p.N = (float *)take(npix);
+    p.Ldr = (float *)take(npix); p.Ldg = (float *)take(npix); p.Ldb = (float *)take(npix);
+    p.taur = (float *)take(npix); p.taug = (float *)take(npix); p.taub = (float *)take(npix);
+    p.Phir = (float *)take(npix); p.Phig = (float *)take(npix); p.Phib = (float *)take(npix);
+    p.M = (int *)take(npix);
+    p.vpx = (float *)take(npix); p.vpy = (float *)take(npix); p.vpz = (float *)take(npix);
+    p.vwox = (float *)take(npix); p.vwoy = (float *)take(npix); p.vwoz = (float *)take(npix);
+    p.vng = (float *)take(3 * (size_t)npix); p.vns = (float *)take(3 * (size_t)npix);
+    p.vu = (float *)take(3 * (size_t)npix); p.vv = (float *)take(3 * (size_t)npix);
+    p.vmat = (int *)take(npix);
+    p.vbr = (float *)take(npix); p.vbg = (float *)take(npix); p.vbb = (float *)take(npix);
+    b.npix = npix;
+  }
+  if (b.hash_size != hash_size || !b.grid_mem.ptr) {
+    size_t ints = 64 + (size_t)(npix + 1) + 3 * (size_t)hash_size;
+    int rc = b.grid_mem.ensure(ints * 4 + 256);
+    if (rc) return rc;
+    int *m = (int *)b.grid_mem.ptr;
+    Grid &g = b.g;
+    g.fctl = (float *)m;
+    g.ictl = m + 16;
+    g.cnt = m + 64;
+    g.cell_start = g.cnt + npix + 1;
+    g.cell_end = g.cell_start + hash_size;
+    g.photon_cell = g.cell_end + hash_size;
+    g.hash_size = hash_size;
+    b.hash_size = hash_size;
+  }
+  size_t need_scan = scan_tmp_ints(npix + 1) + 16;
+  if (b.scan_cap < need_scan) {
+    int rc = b.scan_mem.ensure(need_scan * 4);
+    if (rc) return rc;
+    b.scan_cap = need_scan;
+    b.g.scan_tmp = (int *)b.scan_mem.ptr;
+  }
+  return ORION_OK;
+}
+
+static int ensure_pairs(SppmBuffers &b, size_t n) {
+  if (b.pair_cap < n) {
+    size_t cap = std::max(n + n / 4, (size_t)1 << 16);
+    int rc = b.pair_mem.ensure(cap * 4 * 4);
+    if (rc) return rc;
+    uint32_t *m = (uint32_t *)b.pair_mem.ptr;
+    b.g.keys[0] = m;
+    b.g.keys[1] = m + cap;
+    b.g.vals[0] = (int *)(m + 2 * cap);
+    b.g.vals[1] = (int *)(m + 3 * cap);
+    b.pair_cap = cap;
+  }
+  size_t tiles = (n + kSortTile - 1) / kSortTile + 1;
+  size_t hn = 256 * tiles;
+  size_t need_scan = scan_tmp_ints(hn) + 16;
+  if (b.hist_cap < hn) {
+    int rc = b.hist_mem.ensure(hn * 4);
+    if (rc) return rc;
+    b.hist_cap = hn;
+    b.g.hist = (int *)b.hist_mem.ptr;
+  }
+  if (b.scan_cap < need_scan) {
+    int rc = b.scan_mem.ensure(need_scan * 4);
+    if (rc) return rc;
+    b.scan_cap = need_scan;
+    b.g.scan_tmp = (int *)b.scan_mem.ptr;
+  }
+  return ORION_OK;
+}
+
 }  // namespace orion_amd
+
+using namespace orion_amd;
+
+extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float r0, int maxDepth, int hash_size,
+                                 const orion_sppm_debug *dbg, orion_render_stats *stats) {
+  if (!s) { set_error("null scene"); return ORION_ERR_ARG; }
+  if (n_iters <= 0 || maxDepth <= 0 || !(r0 > 0)) { set_error("bad SPPM parameters"); return ORION_ERR_ARG; }
+  const int npix = s->W * s->H;
+  if (photons < 0) photons = npix;  // sppm.cpp:329-330
+  if (hash_size <= 0) hash_size = npix;  // sppm.cpp:147
+  if (hash_size > (1 << 24)) { set_error("hash_size above 2^24"); return ORION_ERR_UNSUPPORTED; }
+  // the reference forms photon indices in int: nIterations * nPixels + iter * P + p
+  if ((int64_t)n_iters * npix + (int64_t)n_iters * photons >= ((int64_t)1 << 31)) {
+    set_error("SPPM sample indices exceed the reference's int range");
+    return ORION_ERR_UNSUPPORTED;
+  }
+  if (s->d.n_lights == 0) { set_error("SPPM needs at least one light"); return ORION_ERR_ARG; }
+  orion_ctx c = s->ctx;
+  hipStream_t st = c->stream;
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = ensure_buffers(s, hash_size);
+  if (rc) return rc;
+  SppmBuffers &b = *s->sppm;
+  Grid &g = b.g;
+  if ((rc = s->wave.ensure(std::max(npix, photons)))) return rc;
+  Wave w = s->wave.wave;
+  w.ctrl = (int *)s->ctrl.ptr;
+  w.stats = (unsigned long long *)s->stats.ptr;
+  w.npix = npix;
+  w.W = s->W;
+  w.H = s->H;
+  w.k0 = 0;
+  unsigned long long *dstats = (unsigned long long *)s->stats.ptr;
+  HIP_TRY(hipMemsetAsync(s->stats.ptr, 0, S_COUNT * 8, st));
+  HIP_TRY(hipEventRecord(c->ev0, st));
+  const int TB = 256;
+  hipLaunchKernelGGL(k_pix_init, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, b.px, npix, r0);
+  int trace_blocks = c->n_cu * 6;
+  const float nIterF = (float)n_iters;
+  const float NpF = (float)((uint64_t)n_iters * (uint64_t)photons);
+  const bool want_dbg = dbg && dbg->debug_iter >= 0 && dbg->debug_iter < n_iters;
+  for (int iter = 0; iter < n_iters; ++iter) {
+    // --- camera pass: the wave's radiance arrays are the pixels' Ld
+    Wave cw = w;
+    cw.N = npix;
+    cw.Lr = b.px.Ldr; cw.Lg = b.px.Ldg; cw.Lb = b.px.Ldb;
+    hipLaunchKernelGGL(k_cam_raygen, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, s->d, cw, b.px, iter);
+    int cur = 0;
+    for (int it = 0; it <= maxDepth; ++it) {
+      launch_trace(s->d, cw, false, trace_blocks, st);
+      launch_trace(s->d, cw, true, trace_blocks, st);
+      launch_between(cw, cur ^ 1, st);
+      int blocks = std::min((npix + 255) / 256, 8192);
+      hipLaunchKernelGGL(k_cam_shade, dim3(blocks), dim3(256), 0, st, s->d, cw, b.px, cur, maxDepth, iter);
+      cur ^= 1;
+    }
+    // --- grid
+    hipLaunchKernelGGL(k_grid_init, dim3(1), dim3(64), 0, st, g);
+    hipLaunchKernelGGL(k_grid_bounds, dim3(std::min((npix + 255) / 256, 1024)), dim3(256), 0, st, b.px, g, npix);
+    hipLaunchKernelGGL(k_grid_res, dim3(1), dim3(64), 0, st, g);
+    hipLaunchKernelGGL(k_grid_count, dim3((npix + 1 + TB - 1) / TB), dim3(TB), 0, st, b.px, g, npix);
+    device_scan(g.cnt, g.cnt, npix + 1, g.scan_tmp, st);
+    int n_pairs = 0;
+    HIP_TRY(hipMemcpyAsync(&n_pairs, g.cnt + npix, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (n_pairs < 0) { set_error("visible-point cell pairs overflow"); return ORION_ERR_UNSUPPORTED; }
+    if ((rc = ensure_pairs(b, (size_t)n_pairs + 1))) return rc;
+    hipLaunchKernelGGL(k_grid_emit, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, b.px, g, npix);
+    int bits = 1;
+    while ((1 << bits) < hash_size) ++bits;
+    int ntiles = (n_pairs + kSortTile - 1) / kSortTile;
+    int src = 0;
+    if (n_pairs > 0) {
+      for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(kSortBlock), 0, st, g.keys[src], n_pairs, shift, g.hist,
+                           ntiles);
+        device_scan(g.hist, g.hist, 256 * ntiles, g.scan_tmp, st);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(kSortBlock), 0, st, g.keys[src], g.vals[src],
+                           g.keys[src ^ 1], g.vals[src ^ 1], n_pairs, shift, g.hist, ntiles);
+        src ^= 1;
+      }
+    }
+    if (src == 1) {  // keep the sorted pairs in slot 0
+      HIP_TRY(hipMemcpyAsync(g.keys[0], g.keys[1], (size_t)n_pairs * 4, hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipMemcpyAsync(g.vals[0], g.vals[1], (size_t)n_pairs * 4, hipMemcpyDeviceToDevice, st));
+    }
+    hipLaunchKernelGGL(k_cell_clear, dim3((hash_size + TB - 1) / TB), dim3(TB), 0, st, g);
+    if (n_pairs > 0)
+      hipLaunchKernelGGL(k_cell_ranges, dim3((n_pairs + TB - 1) / TB), dim3(TB), 0, st, g, g.keys[0], n_pairs);
+    if (want_dbg && iter == dbg->debug_iter) {
+      if (dbg->vp_cell_count) {
+        DevBuf tmp;
+        if ((rc = tmp.ensure((size_t)hash_size * 4))) return rc;
+        hipLaunchKernelGGL(k_cell_counts, dim3((hash_size + TB - 1) / TB), dim3(TB), 0, st, g, (int *)tmp.ptr);
+        HIP_TRY(hipMemcpyAsync(dbg->vp_cell_count, tmp.ptr, (size_t)hash_size * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+      }
+      int ictl[16];
+      float fctl[16];
+      HIP_TRY(hipMemcpyAsync(ictl, g.ictl, sizeof(ictl), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipMemcpyAsync(fctl, g.fctl, sizeof(fctl), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      if (dbg->grid_info) {
+        dbg->grid_info[0] = ictl[0]; dbg->grid_info[1] = ictl[1]; dbg->grid_info[2] = ictl[2];
+        dbg->grid_info[3] = n_pairs;
+      }
+      if (dbg->grid_bounds)
+        for (int a = 0; a < 6; ++a) dbg->grid_bounds[a] = fctl[a];
+    }
+    // --- photon pass (sppm.cpp:199-273)
+    if (photons > 0) {
+      Wave pw = w;
+      pw.N = photons;
+      uint32_t index0 = (uint32_t)(n_iters * npix + iter * photons);
+      HIP_TRY(hipMemsetAsync(w.ctrl + C_RAY, 0, 4, st));
+      hipLaunchKernelGGL(k_photon_init, dim3((photons + TB - 1) / TB), dim3(TB), 0, st, s->d, pw, index0, photons);
+      for (int depth = 0; depth < maxDepth; ++depth) {
+        launch_trace(s->d, pw, false, trace_blocks, st);
+        hipLaunchKernelGGL(k_photon_between, dim3(1), dim3(64), 0, st, pw, dstats);
+        int blocks = std::min((photons + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_photon_shade, dim3(blocks), dim3(256), 0, st, s->d, pw, b.px, g, index0, depth, dstats);
+        hipLaunchKernelGGL(k_photon_swap, dim3(1), dim3(64), 0, st, pw);
+        hipLaunchKernelGGL(k_copy_queue, dim3((photons + TB - 1) / TB), dim3(TB), 0, st, pw);
+      }
+    }
+    if (want_dbg && iter == dbg->debug_iter && dbg->photon_cell_count)
+      HIP_TRY(hipMemcpyAsync(dbg->photon_cell_count, g.photon_cell, (size_t)hash_size * 4, hipMemcpyDeviceToHost, st));
+    // --- update + film
+    DevBuf mdbg;
+    int *M_out = nullptr;
+    if (want_dbg && iter == dbg->debug_iter && dbg->M) {
+      if ((rc = mdbg.ensure((size_t)npix * 4))) return rc;
+      M_out = (int *)mdbg.ptr;
+    }
+    hipLaunchKernelGGL(k_pixel_update, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, b.px, npix, M_out);
+    hipLaunchKernelGGL(k_sppm_film, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, b.px, npix, nIterF, NpF,
+                       (float *)s->film.ptr, (int *)s->count.ptr);
+    if (M_out) {
+      HIP_TRY(hipMemcpyAsync(dbg->M, M_out, (size_t)npix * 4, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(c->ev1, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  unsigned long long h[S_COUNT];
+  HIP_TRY(hipMemcpy(h, s->stats.ptr, sizeof(h), hipMemcpyDeviceToHost));
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->closest_rays = (int64_t)h[S_CLOSEST];
+    stats->shadow_rays = (int64_t)h[S_SHADOW];
+    stats->ref_equiv_rays = (int64_t)(h[S_CLOSEST] + h[S_SHADOW] + h[S_REFEQ]);
+    stats->photons = (int64_t)n_iters * photons;
+    stats->paths = (int64_t)n_iters * npix;
+    stats->deposits = (int64_t)h[S_DEPOSITS];
+    stats->ms = ms;
+    stats->node_visits_closest = (int64_t)h[S_NODES_C];
+    stats->tri_tests_closest = (int64_t)h[S_TRIS_C];
+    stats->node_visits_shadow = (int64_t)h[S_NODES_S];
+    stats->tri_tests_shadow = (int64_t)h[S_TRIS_S];
+  }
+  return ORION_OK;
+}

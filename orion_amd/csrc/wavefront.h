@@ -25,8 +25,8 @@ enum : int {
 };
 // 64-bit statistics
 enum : int {
-  S_CLOSEST = 0, S_SHADOW = 1, S_REFEQ = 2, S_PATHS = 3, S_NODES = 4, S_TRIS = 5,
-  S_PHOTONS = 6, S_DEPOSITS = 7, S_COUNT = 8,
+  S_CLOSEST = 0, S_SHADOW = 1, S_REFEQ = 2, S_PATHS = 3, S_NODES_C = 4, S_TRIS_C = 5,
+  S_PHOTONS = 6, S_DEPOSITS = 7, S_NODES_S = 8, S_TRIS_S = 9, S_COUNT = 12,
 };
 
 struct Wave {

@@ -1210,7 +1210,7 @@ int oracle_path_samples(void *s, int n, const int *pixel, const int *kk, int max
 int oracle_render_sppm(void *s, int nIterations, int photonPerIteration, float initialSearchRadius,
                        int maxDepth, int hashSizeArg, float *rgb, int *count, int threads,
                        int debug_iter, int *vp_cell_count, int *photon_cell_count, int *M_debug,
-                       int *grid_info, float *grid_bounds) {
+                       int *grid_info, float *grid_bounds, float *vp_out) {
   const Scene &sc = *(const Scene *)s;
   int W = sc.W, H = sc.H;
   int nPixels = W * H;
@@ -1313,6 +1313,13 @@ int oracle_render_sppm(void *s, int nIterations, int photonPerIteration, float i
       }
       if (grid_bounds)
         for (int a = 0; a < 3; ++a) { grid_bounds[a] = blo[a]; grid_bounds[3 + a] = bhi[a]; }
+      if (vp_out)
+        for (int i = 0; i < nPixels; ++i) {
+          const Pixel &px = pixels[i];
+          vp_out[5 * i] = px.vp.p.x; vp_out[5 * i + 1] = px.vp.p.y; vp_out[5 * i + 2] = px.vp.p.z;
+          vp_out[5 * i + 3] = px.radius;
+          vp_out[5 * i + 4] = black(px.vp.beta) ? 0.f : 1.f;
+        }
       if (photon_cell_count) std::memset(photon_cell_count, 0, sizeof(int) * hashSize);
     }
     // photon pass, sppm.cpp:199-273

@@ -55,13 +55,13 @@ int oracle_path_samples(void *scene, int n, const int *pixel, const int *k,
  * iteration debug_iter: vp_cell_count[hash_size] (visible-point entries per
  * hash cell), photon_cell_count[hash_size] (in-bounds photon deposits per
  * cell), M_debug[nPixels] (photon count per pixel before the update),
- * grid_info[8] = gridRes xyz, n_pairs, bounds lo/hi packed as floats in
- * grid_bounds[6]. */
+ * grid_info[8] = gridRes xyz, n_pairs; grid_bounds[6] = bounds lo, hi;
+ * vp_out[nPixels * 5] = visible point x, y, z, radius, valid. */
 int oracle_render_sppm(void *scene, int n_iters, int photons_per_iter,
                        float init_radius, int max_depth, int hash_size,
                        float *rgb_sum, int *count, int threads, int debug_iter,
                        int *vp_cell_count, int *photon_cell_count, int *M_debug,
-                       int *grid_info, float *grid_bounds);
+                       int *grid_info, float *grid_bounds, float *vp_out);
 
 /* Building blocks exposed for the unit pins. */
 float oracle_sobol(uint64_t index, int dim);

@@ -36,7 +36,7 @@ class Oracle:
         L.oracle_render_sppm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int,
                                          C.c_int, f32p, i32p, C.c_int, C.c_int,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                         C.c_void_p]
+                                         C.c_void_p, C.c_void_p]
         L.oracle_sobol.restype = C.c_float
         L.oracle_sobol.argtypes = [C.c_uint64, C.c_int]
         L.oracle_camera_ray.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint64, f32p]
@@ -107,12 +107,13 @@ class OracleScene:
             dbg = dict(vp_cell_count=np.zeros(hs, np.int32),
                        photon_cell_count=np.zeros(hs, np.int32),
                        M=np.zeros(W * H, np.int32), grid_info=np.zeros(8, np.int32),
-                       grid_bounds=np.zeros(6, np.float32))
+                       grid_bounds=np.zeros(6, np.float32), vp=np.zeros(W * H * 5, np.float32))
         self.o.L.oracle_render_sppm(self.h, iters, photons, radius, max_depth, hash_size, rgb,
                                     cnt, threads, debug_iter,
                                     _opt(dbg.get("vp_cell_count")),
                                     _opt(dbg.get("photon_cell_count")), _opt(dbg.get("M")),
-                                    _opt(dbg.get("grid_info")), _opt(dbg.get("grid_bounds")))
+                                    _opt(dbg.get("grid_info")), _opt(dbg.get("grid_bounds")),
+                                    _opt(dbg.get("vp")))
         return rgb.reshape(H, W, 3), cnt.reshape(H, W), dbg
 
     def camera_ray(self, i, j, index):

@@ -1,0 +1,150 @@
+"""Host side of the boundary (CPU): the C-ABI library loads and exports every
+symbol include/orion_amd.h declares; the .ori loader's resolved records equal
+the reference's own transform / camera / light code (oracle/_ref); writer ->
+loader round trips are bit-exact; errors come back as status codes."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import orion_amd as oa
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_header_symbols():
+    import ctypes
+    hdr = (ROOT / "include" / "orion_amd.h").read_text()
+    names = set(re.findall(r"\b(orion_[a-z_]+)\s*\(", hdr))
+    lib = ctypes.CDLL(str(oa.LIB_PATH))
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(oa.exported_symbols()) <= names
+
+
+def test_scene_sizes():
+    assert oa.HostScene.generate("cornell").n_tris == 36
+    assert oa.HostScene.generate("caustics").n_tris == 81920 + 2
+    hs = oa.HostScene.generate("icosphere", params=[4, 64, 64])
+    assert hs.n_tris == 20 * 4 ** 4 + 4
+
+
+def test_full_size_icosphere():
+    hs = oa.HostScene.generate("icosphere")
+    assert hs.n_tris == 1310724 and hs.width == 1024 and hs.info.spp == 64 and hs.info.max_depth == 8
+    n = hs.tri_pn.reshape(-1, 18)[:, 9:]
+    assert np.isfinite(hs.tri_pn).all() and np.allclose(np.linalg.norm(n.reshape(-1, 3), axis=1), 1, atol=1e-5)
+
+
+def test_create_transform_matches_reference(ref):
+    rng = np.random.default_rng(3)
+    chains = [
+        [("translate", "1,2,3")],
+        [("scale", "2,0.5,1.5"), ("rotate", "0,1,0,30")],
+        [("rotate", "1,1,0,45"), ("translate", "-0.5,0.25,4"), ("scale", "1,1,-1")],
+        [("lookat", "0,1,5, 0,0,0, 0,1,0")],
+        [("translate", "1,0,0"), ("store", "1"), ("rotate", "0,0,1,90"), ("reuse", "1")],
+    ]
+    for _ in range(10):
+        chains.append([("rotate", ",".join(f"{x:.4f}" for x in rng.normal(size=3)) + f",{rng.uniform(-180, 180):.3f}"),
+                       ("translate", ",".join(f"{x:.3f}" for x in rng.normal(size=3)))])
+    for ch in chains:
+        m, mi = oa.create_transform(ch)
+        rm, rmi = ref.create_transform(ch)
+        np.testing.assert_array_equal(m.view(np.int32), rm.view(np.int32))
+        np.testing.assert_array_equal(mi.view(np.int32), rmi.view(np.int32))
+
+
+def test_camera_record_matches_reference(ref):
+    rng = np.random.default_rng(9)
+    for _ in range(50):
+        pos = rng.normal(size=3) * 4
+        look = rng.normal(size=3)
+        up = np.array([0, 1, 0]) + 0.1 * rng.normal(size=3)
+        la = np.concatenate([pos, look, up]).astype(np.float32)
+        fov = float(rng.uniform(10, 90))
+        W, H = int(rng.integers(16, 2048)), int(rng.integers(16, 2048))
+        rec = oa.camera_record(la, fov, W, H)
+        m, ta, ray = ref.camera(la, fov, W, H)
+        # origin = the transform applied to (0,0,0), including the w divide (transform.h:118-131)
+        np.testing.assert_array_equal(rec[:3].view(np.int32), ray[:3].view(np.int32))
+        np.testing.assert_array_equal(rec[3:12].view(np.int32), m[:3, :3].reshape(-1).view(np.int32))
+        assert rec[12] == ta[0] and rec[13] == ta[1]
+
+
+@pytest.mark.parametrize("scene,params,limit", [("cornell", None, 1000), ("icosphere", [3, 64, 48], 100),
+                                                ("caustics", [2, 40, 30], 10 ** 9)])
+def test_ori_round_trip(tmp_path, scene, params, limit):
+    hs = oa.HostScene.generate(scene, params=params)
+    path = tmp_path / "scene.ori"
+    hs.write_ori(path, inline_limit=limit)
+    back = oa.HostScene.load_ori(path)
+    for name in ("tri_pn", "tri_light", "lights", "camera"):
+        a, b = getattr(hs, name), getattr(back, name)
+        assert a.shape == b.shape, name
+        np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32), err_msg=name)
+    # the loader makes one material per Model (parser.cpp:95-96), so indices are
+    # renumbered; the resolved material of every triangle is unchanged
+    ma = hs.mats.reshape(-1, 16)[hs.tri_mat]
+    mb = back.mats.reshape(-1, 16)[back.tri_mat]
+    np.testing.assert_array_equal(ma.view(np.int32), mb.view(np.int32))
+    assert back.info.integrator == hs.info.integrator and back.info.max_depth == hs.info.max_depth
+
+
+def test_loader_schema_features(tmp_path):
+    """Transforms on models, named materials, metal presets, one-normal meshHandy."""
+    (tmp_path / "s.ori").write_text("""<?xml version="1.0"?>
+<Scene>
+  <Resource filename="./"/>
+  <Camera type="pinhole"><Transform><Param lookat="0,0,5, 0,0,0, 0,1,0"/></Transform>
+    <Param fov="40"/><Param filmsize="32,24"/></Camera>
+  <!-- a comment -->
+  <Model type="meshHandy">
+    <Transform><Param scale="2,2,2"/><Param translate="0,1,0"/></Transform>
+    <Param p="0,0,0, 1,0,0, 0,1,0"/><Param n="0,0,1"/>
+    <Material type="metal" name="gold1"><Param metalName="gold"/>
+      <roughness><Param type="float"/><Param value="0.2"/></roughness></Material>
+  </Model>
+  <Model type="meshHandy">
+    <Param p="0,0,1, 1,0,1, 0,1,1"/><Param n="0,0,1, 0,0,1, 0,0,1"/>
+    <Material named="gold1"/>
+    <AreaLight><Param Le="4"/><Param nLightSamples="1"/></AreaLight>
+  </Model>
+  <Light type="point"><Param I="1,2,3"/><Param from="0,4,0"/></Light>
+  <Integrator type="path"><Param maxDepth="5"/><Param nSamplesPerPixel="4"/></Integrator>
+  <Sampler type="sobol"/>
+</Scene>""")
+    hs = oa.HostScene.load_ori(tmp_path / "s.ori")
+    assert hs.n_tris == 2 and hs.n_mats == 2 and hs.n_lights == 2
+    p = hs.tri_pn.reshape(-1, 18)
+    np.testing.assert_array_equal(p[0, :9], [0, 1, 0, 2, 1, 0, 0, 3, 0])
+    m = hs.mats.reshape(-1, 16)
+    np.testing.assert_array_equal(m[0], m[1])
+    assert m[0][0] == 1 and m[0][7] == np.float32(0.2)
+    L = hs.lights.reshape(-1, 32)
+    assert L[0][0] == 0 and L[0][1] == 1 and (L[0][2:5] == 4).all()
+    assert L[1][0] == 1 and (L[1][2:5] == [1, 2, 3]).all() and (L[1][5:8] == [0, 4, 0]).all()
+    assert list(hs.tri_light) == [-1, 0]
+    assert hs.info.spp == 4 and hs.info.max_depth == 5 and hs.width == 32 and hs.height == 24
+
+
+@pytest.mark.parametrize("body,frag", [
+    ("<Scene><Camera type='pinhole'><Param fov='40'/><Param filmsize='4,4'/></Camera>"
+     "<Model type='meshHandy'><Param p='0,0,0,1,0,0,0,1,0'/><Param n='0,0,1'/>"
+     "<Material type='plastic'/></Model><Integrator type='path'><Param maxDepth='1'/>"
+     "<Param nSamplesPerPixel='1'/></Integrator></Scene>", "plastic"),
+    ("<Scene><Camera type='pinhole'><Param fov='40'/><Param filmsize='4,4'/></Camera>"
+     "<Model type='meshHandy'><Param p='0,0,0,1,0,0,0,1,0'/>"
+     "<Material type='matte'><Kd><Param type='constant'/><Param color='1'/></Kd></Material></Model>"
+     "<Integrator type='path'><Param maxDepth='1'/><Param nSamplesPerPixel='1'/></Integrator></Scene>",
+     "normals"),
+    ("<Scene><Camera type='pinhole'>", "XML"),
+])
+def test_loader_errors(tmp_path, body, frag):
+    (tmp_path / "bad.ori").write_text(body)
+    with pytest.raises(oa.OrionError) as e:
+        oa.HostScene.load_ori(tmp_path / "bad.ori")
+    assert frag in str(e.value)
+    with pytest.raises(oa.OrionError):
+        oa.HostScene.load_ori(tmp_path / "missing.ori")

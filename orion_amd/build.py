@@ -61,6 +61,11 @@ def build_library(verbose=False):
     tmp = str(LIB) + ".tmp"
     _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lpthread"])
     os.replace(tmp, LIB)
+    # the C++ host layer's driver (orion_amd/host), linked against the library
+    drv = PKG / "orion_render"
+    _run(["g++", "-O2", "-std=c++17", str(PKG / "host" / "render_ori.cpp"), "-o", str(drv) + ".tmp",
+          f"-L{PKG}", "-lorion_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(str(drv) + ".tmp", drv)
     return LIB
 
 

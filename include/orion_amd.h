@@ -181,7 +181,55 @@ int orion_render_sppm(orion_scene s, int n_iters, int photons_per_iter, float in
                       int max_depth, int hash_size, const orion_sppm_debug *dbg,
                       orion_render_stats *stats);
 
-/* Raw device pointers of the last wavefront's queues etc. are internal. */
+/* ---------------------------------------------------------------- batched launch
+ * No counterpart in orion: many render jobs -- each its own resident scene,
+ * film size, spp and depth -- packed into shared wavefronts that one
+ * persistent traversal kernel drains.  A job's camera is its scene's view
+ * with the job's film size (aspect = width / height). */
+typedef struct {
+  orion_scene scene;   /* resident scene (BVH, materials, lights, view) */
+  int width, height;   /* film of this job */
+  int spp;             /* samples [0, spp) per pixel */
+  int max_depth;
+} orion_job_desc;
+
+/* Renders every job to completion.  rgb_out[j] (width*height*3 floats) and
+ * count_out[j] (width*height ints) receive job j's film sums and counts;
+ * each equals what orion_render_path gives for that job alone, bit for bit. */
+int orion_render_batch(orion_ctx c, int n_jobs, const orion_job_desc *jobs, float *const *rgb_out,
+                       int *const *count_out, const orion_tuning *tune, orion_render_stats *stats);
+
+/* ---------------------------------------------------------------- serving
+ * No counterpart in orion: a request loop on its own host thread.  Jobs are
+ * admitted when their film fits the HBM budget, merged into the running
+ * wavefronts (continuous batching: a job joins at the next wave), and
+ * retired when their last sample is accumulated. */
+typedef struct orion_server_s *orion_server;
+int orion_server_create(orion_ctx c, int64_t hbm_budget_bytes, const orion_tuning *tune, orion_server *out);
+/* thread-safe; *job_id identifies the request */
+int orion_server_submit(orion_server s, const orion_job_desc *job, uint64_t *job_id);
+/* done = 1 once finished; then copies the film (either pointer may be NULL)
+ * and the request's times in seconds since the server started:
+ * times[0] submitted, [1] admitted, [2] finished. */
+int orion_server_poll(orion_server s, uint64_t job_id, int *done, float *rgb, int *count, double times[3]);
+/* blocks until every submitted job has finished */
+int orion_server_drain(orion_server s);
+typedef struct {
+  int64_t jobs_done, waves, paths, closest_rays, shadow_rays;
+  int active_jobs, queued_jobs;
+  int64_t hbm_in_use;
+  double busy_ms;
+  int64_t mixed_waves;  /* waves that carried more than one job */
+  int max_jobs_in_wave;
+} orion_server_stats;
+int orion_server_get_stats(orion_server s, orion_server_stats *st);
+int orion_server_destroy(orion_server s);
+
+/* ---------------------------------------------------------------- autotuner
+ * Times short renders of `scene` over candidate wave sizes, persistent
+ * traversal blocks per CU and graph capture, and returns the fastest
+ * setting (also cached per scene for later calls with tune == NULL). */
+int orion_autotune(orion_scene s, int max_depth, orion_tuning *best);
 
 #ifdef __cplusplus
 }

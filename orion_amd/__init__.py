@@ -61,6 +61,18 @@ class SppmDebug(C.Structure):
                 ("grid_bounds", C.c_void_p)]
 
 
+class JobDesc(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("width", C.c_int), ("height", C.c_int), ("spp", C.c_int),
+                ("max_depth", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("jobs_done", C.c_int64), ("waves", C.c_int64), ("paths", C.c_int64),
+                ("closest_rays", C.c_int64), ("shadow_rays", C.c_int64), ("active_jobs", C.c_int),
+                ("queued_jobs", C.c_int), ("hbm_in_use", C.c_int64), ("busy_ms", C.c_double),
+                ("mixed_waves", C.c_int64), ("max_jobs_in_wave", C.c_int)]
+
+
 # name -> (restype, argtypes); the header's full export list
 _SIGS = {
     "orion_last_error": (C.c_char_p, []),
@@ -93,6 +105,18 @@ _SIGS = {
     "orion_film_read": (C.c_int, [C.c_void_p, f32p, i32p]),
     "orion_render_sppm": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
                                     C.POINTER(SppmDebug), C.POINTER(RenderStats)]),
+    "orion_render_batch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(JobDesc),
+                                     C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                     C.POINTER(Tuning), C.POINTER(RenderStats)]),
+    "orion_server_create": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Tuning),
+                                      C.POINTER(C.c_void_p)]),
+    "orion_server_submit": (C.c_int, [C.c_void_p, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),
+    "orion_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), C.c_void_p,
+                                    C.c_void_p, C.POINTER(C.c_double)]),
+    "orion_server_drain": (C.c_int, [C.c_void_p]),
+    "orion_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    "orion_server_destroy": (C.c_int, [C.c_void_p]),
+    "orion_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Tuning)]),
 }
 
 _lib = None
@@ -293,9 +317,16 @@ class DeviceScene:
     def render_path(self, k0, k1, max_depth, spp_per_wave=0, persistent_blocks=0, profile=False,
                     use_graph=False):
         st = RenderStats()
-        tune = Tuning(spp_per_wave, 0, persistent_blocks, int(use_graph), int(profile))
-        _check(lib().orion_render_path(self.h, k0, k1, max_depth, C.byref(tune), C.byref(st)))
+        tune = None  # NULL: the scene's autotuned setting, if any, else the defaults
+        if spp_per_wave or persistent_blocks or use_graph or profile:
+            tune = C.byref(Tuning(spp_per_wave, 0, persistent_blocks, int(use_graph), int(profile)))
+        _check(lib().orion_render_path(self.h, k0, k1, max_depth, tune, C.byref(st)))
         return st
+
+    def autotune(self, max_depth):
+        best = Tuning()
+        _check(lib().orion_autotune(self.h, max_depth, C.byref(best)))
+        return best
 
     def film_clear(self):
         _check(lib().orion_film_clear(self.h))
@@ -327,6 +358,81 @@ class DeviceScene:
     def close(self):
         if self.h:
             lib().orion_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _tuning(spp_per_wave=0, persistent_blocks=0, use_graph=False, profile=False):
+    return Tuning(spp_per_wave, 0, persistent_blocks, int(use_graph), int(profile))
+
+
+def render_batch(dev, jobs, spp_per_wave=0, persistent_blocks=0, use_graph=False, profile=False):
+    """Renders jobs = [(DeviceScene, width, height, spp, max_depth), ...] in shared
+    wavefronts; returns ([(rgb HxWx3, count HxW), ...], RenderStats)."""
+    n = len(jobs)
+    descs = (JobDesc * max(n, 1))()
+    outs = []
+    rgb_p = (C.c_void_p * max(n, 1))()
+    cnt_p = (C.c_void_p * max(n, 1))()
+    for i, (sc, W, H, spp, depth) in enumerate(jobs):
+        descs[i] = JobDesc(sc.h.value, W, H, spp, depth)
+        rgb = np.zeros((H, W, 3), np.float32)
+        cnt = np.zeros((H, W), np.int32)
+        outs.append((rgb, cnt))
+        rgb_p[i] = rgb.ctypes.data
+        cnt_p[i] = cnt.ctypes.data
+    st = RenderStats()
+    tune = _tuning(spp_per_wave, persistent_blocks, use_graph, profile)
+    _check(lib().orion_render_batch(dev.h, n, descs, rgb_p, cnt_p, C.byref(tune), C.byref(st)))
+    return outs, st
+
+
+class Server:
+    """Request loop over one device: submit jobs, poll or drain, read films and
+    per-request times (submitted, admitted, finished; seconds since start)."""
+
+    def __init__(self, dev, hbm_budget_bytes, spp_per_wave=0, use_graph=False):
+        self.dev = dev
+        self.h = C.c_void_p()
+        self._shape = {}
+        tune = _tuning(spp_per_wave, 0, use_graph)
+        _check(lib().orion_server_create(dev.h, int(hbm_budget_bytes), C.byref(tune), C.byref(self.h)))
+
+    def submit(self, scene, width, height, spp, max_depth):
+        jid = C.c_uint64()
+        d = JobDesc(scene.h.value, width, height, spp, max_depth)
+        _check(lib().orion_server_submit(self.h, C.byref(d), C.byref(jid)))
+        self._shape[jid.value] = (height, width)
+        return jid.value
+
+    def poll(self, jid, want_film=True):
+        """Returns None while running, else (rgb, count, times)."""
+        H, W = self._shape[jid]
+        done = C.c_int()
+        times = (C.c_double * 3)()
+        rgb = np.zeros((H, W, 3), np.float32) if want_film else None
+        cnt = np.zeros((H, W), np.int32) if want_film else None
+        _check(lib().orion_server_poll(self.h, jid, C.byref(done), _ptr(rgb), _ptr(cnt), times))
+        if not done.value:
+            return None
+        return rgb, cnt, tuple(times)
+
+    def drain(self):
+        _check(lib().orion_server_drain(self.h))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().orion_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self.h:
+            _check(lib().orion_server_destroy(self.h))
             self.h = C.c_void_p()
 
     def __del__(self):

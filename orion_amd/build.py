@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "liborion_amd.so"
 
-HIP_SOURCES = ["runtime.hip", "path_kernels.hip", "sppm_kernels.hip"]
+HIP_SOURCES = ["runtime.hip", "path_kernels.hip", "sppm_kernels.hip", "batch.hip"]
 CPP_SOURCES = ["api_host.cpp", "host_records.cpp", "ori_loader.cpp", "scenegen.cpp", "bvh.cpp"]
 
 ARCH = os.environ.get("ORION_AMD_ARCH", "gfx950")

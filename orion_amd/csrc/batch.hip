@@ -1,0 +1,726 @@
+// Wave runner shared by the plain renderer, the batched launch and the
+// serving loop; the autotuner.
+//
+// A wave = a device job table + per-wave sizes in the control block +
+// raygen, (maxDepth + 1) x [closest trace, any trace, queue reset, shade],
+// film.  Every launch argument is fixed for a given (buffers, depth, stack
+// variant, grid), so the sequence is captured once as a hipGraph and
+// replayed per wave after the job table is copied in.  Queue counters are
+// reset by kernels, not hipMemsetAsync, inside the captured region.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/orion_amd.h"
+#include "host_scene.h"
+#include "kernels.h"
+#include "runtime.h"
+
+namespace orion_amd {
+
+#define HIP_TRY(expr)                                                         \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));           \
+      return ORION_ERR_HIP;                                                   \
+    }                                                                         \
+  } while (0)
+
+constexpr int kDefaultWavePaths = 1 << 21;
+
+// Per-launch HIP events recorded on the launch stream (kernel timing).
+struct LaunchTimer {
+  bool on = false;
+  hipStream_t st = nullptr;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> kind;  // 0 closest, 1 shadow, 2 shade
+  int mark(int k) {
+    if (!on) return ORION_OK;
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipEventRecord(e, st));
+    ev.push_back(e);
+    kind.push_back(k);
+    return ORION_OK;
+  }
+  int collect(orion_render_stats *s) {
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      if (!s) continue;
+      if (kind[i] == 0) { s->trace_closest_ms += ms; s->trace_closest_launches++; }
+      else if (kind[i] == 1) { s->trace_shadow_ms += ms; s->trace_shadow_launches++; }
+      else { s->shade_ms += ms; s->shade_launches++; }
+    }
+    return ORION_OK;
+  }
+  ~LaunchTimer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
+
+struct GraphKey {
+  int depth, stack, multi, blocks, cap;
+  void *mem;
+  bool operator==(const GraphKey &o) const {
+    return depth == o.depth && stack == o.stack && multi == o.multi && blocks == o.blocks && cap == o.cap &&
+           mem == o.mem;
+  }
+};
+
+struct WaveRunner {
+  WaveBuffers wb;
+  DevBuf jobs_dev, ctrl, stats;
+  Job *h_jobs = nullptr;
+  int *h_ctrl = nullptr;
+  int max_jobs = 0;
+  hipEvent_t staged = nullptr;
+  std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
+
+  ~WaveRunner() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+    if (h_jobs) (void)hipHostFree(h_jobs);
+    if (h_ctrl) (void)hipHostFree(h_ctrl);
+    if (staged) (void)hipEventDestroy(staged);
+  }
+  int init(int njobs) {
+    if (njobs <= max_jobs && staged) return ORION_OK;
+    int cap = std::max(njobs, std::max(64, max_jobs * 2));
+    if (h_jobs) { HIP_TRY(hipEventSynchronize(staged)); (void)hipHostFree(h_jobs); }
+    HIP_TRY(hipHostMalloc((void **)&h_jobs, sizeof(Job) * cap, hipHostMallocDefault));
+    if (!h_ctrl) HIP_TRY(hipHostMalloc((void **)&h_ctrl, 64, hipHostMallocDefault));
+    int rc = jobs_dev.ensure(sizeof(Job) * cap);
+    if (rc) return rc;
+    if ((rc = ctrl.ensure(64 * 4))) return rc;
+    if ((rc = stats.ensure(S_COUNT * 8))) return rc;
+    HIP_TRY(hipMemset(ctrl.ptr, 0, 64 * 4));
+    if (!staged) HIP_TRY(hipEventCreateWithFlags(&staged, hipEventDisableTiming));
+    max_jobs = cap;
+    drop_graphs();
+    return ORION_OK;
+  }
+  void drop_graphs() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+  }
+  int ensure_cap(int paths) {
+    if (paths <= wb.cap && wb.mem.ptr) return ORION_OK;
+    drop_graphs();
+    return wb.ensure(paths);
+  }
+  Wave wave() {
+    Wave w = wb.wave;
+    w.ctrl = (int *)ctrl.ptr;
+    w.stats = (unsigned long long *)stats.ptr;
+    return w;
+  }
+  int reset_stats(hipStream_t st) {
+    HIP_TRY(hipMemsetAsync(stats.ptr, 0, S_COUNT * 8, st));
+    return ORION_OK;
+  }
+  int read_stats(orion_render_stats *o) {
+    unsigned long long h[S_COUNT];
+    HIP_TRY(hipMemcpy(h, stats.ptr, sizeof(h), hipMemcpyDeviceToHost));
+    if (!o) return ORION_OK;
+    o->paths += (int64_t)h[S_PATHS];
+    o->closest_rays += (int64_t)h[S_CLOSEST];
+    o->shadow_rays += (int64_t)h[S_SHADOW];
+    o->ref_equiv_rays += (int64_t)(h[S_CLOSEST] + h[S_SHADOW] + h[S_REFEQ]);
+    o->node_visits_closest += (int64_t)h[S_NODES_C];
+    o->tri_tests_closest += (int64_t)h[S_TRIS_C];
+    o->node_visits_shadow += (int64_t)h[S_NODES_S];
+    o->tri_tests_shadow += (int64_t)h[S_TRIS_S];
+    return ORION_OK;
+  }
+
+  void enqueue_wave(const Job *dj, const Wave &w, int depth, int stack, bool multi, int blocks,
+                    hipStream_t st, LaunchTimer *tm) {
+    launch_raygen(dj, w, wb.cap, multi, st);
+    int cur = 0;
+    for (int it = 0; it <= depth; ++it) {
+      if (tm) tm->mark(0);
+      launch_trace(dj, w, false, stack, blocks, multi, st);
+      if (tm) { tm->mark(-1); tm->mark(1); }
+      launch_trace(dj, w, true, stack, blocks, multi, st);
+      if (tm) tm->mark(-1);
+      launch_between(w, cur ^ 1, st);
+      if (tm) tm->mark(2);
+      launch_shade(dj, w, cur, wb.cap, multi, st);
+      if (tm) tm->mark(-1);
+      cur ^= 1;
+    }
+    launch_film(dj, w, std::max(wb.cap, 1 << 20), multi, st);
+  }
+
+  // Runs one wave: jobs[0..n) with path0 / pix0 already assigned.
+  int run(hipStream_t st, const Job *jobs, int n, int npaths, int npix, int depth, int stack, bool multi,
+          int blocks, bool use_graph, LaunchTimer *tm) {
+    int rc = init(n);
+    if (rc) return rc;
+    if ((rc = ensure_cap(npaths))) return rc;
+    // stage the job table and the wave sizes (pinned; the previous copy
+    // must have completed before the staging area is rewritten)
+    HIP_TRY(hipEventSynchronize(staged));
+    std::memcpy(h_jobs, jobs, sizeof(Job) * n);
+    h_ctrl[0] = npaths;
+    h_ctrl[1] = npix;
+    h_ctrl[2] = n;
+    HIP_TRY(hipMemcpyAsync(jobs_dev.ptr, h_jobs, sizeof(Job) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync((int *)ctrl.ptr + C_NPATHS, h_ctrl, 3 * sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(staged, st));
+    const Job *dj = (const Job *)jobs_dev.ptr;
+    Wave w = wave();
+    if (!use_graph || tm) {
+      enqueue_wave(dj, w, depth, stack, multi, blocks, st, tm);
+      HIP_TRY(hipGetLastError());
+      return ORION_OK;
+    }
+    GraphKey key{depth, stack, multi ? 1 : 0, blocks, wb.cap, wb.mem.ptr};
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : graphs)
+      if (g.first == key) exec = g.second;
+    if (!exec) {
+      hipGraph_t graph;
+      HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      enqueue_wave(dj, w, depth, stack, multi, blocks, st, nullptr);
+      HIP_TRY(hipStreamEndCapture(st, &graph));
+      hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        return ORION_ERR_HIP;
+      }
+      graphs.push_back({key, exec});
+    }
+    HIP_TRY(hipGraphLaunch(exec, st));
+    return ORION_OK;
+  }
+};
+
+void destroy_runner(WaveRunner *r) { delete r; }
+
+static int trace_blocks(orion_ctx c, int stack, const orion_tuning *t) {
+  int bpc = t && t->persistent_blocks > 0 ? t->persistent_blocks : trace_blocks_per_cu(stack, false);
+  return c->n_cu * bpc;
+}
+
+static Job make_job(orion_scene s, int W, int H, int maxDepth, float *film, int *count) {
+  Job J;
+  std::memset(&J, 0, sizeof(J));
+  J.sc = s->d;
+  // the job's camera: the scene's view with this film size (camera.h:31)
+  J.sc.cam[13] = (float)W / H;
+  J.sc.cam[14] = (float)W;
+  J.sc.cam[15] = (float)H;
+  J.W = W;
+  J.H = H;
+  J.npix = W * H;
+  J.maxDepth = maxDepth;
+  J.film = film;
+  J.count = count;
+  return J;
+}
+
+static const orion_tuning *effective_tuning(orion_scene s, const orion_tuning *t) {
+  if (t) return t;
+  return s->has_tuned ? &s->tuned : nullptr;
+}
+
+}  // namespace orion_amd
+
+using namespace orion_amd;
+
+// ======================================================================= plain renderer
+extern "C" int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, const orion_tuning *tune,
+                                 orion_render_stats *stats) {
+  if (!s) { set_error("null scene"); return ORION_ERR_ARG; }
+  if (k0 < 0 || k1 < k0 || maxDepth < 0) { set_error("bad sample range / depth"); return ORION_ERR_ARG; }
+  const int npix = s->W * s->H;
+  if ((uint64_t)k1 * (uint64_t)npix > 0xffffffffull) {
+    set_error("sample index k * W * H must stay below 2^32");
+    return ORION_ERR_UNSUPPORTED;
+  }
+  if (maxDepth > 60) { set_error("maxDepth above 60 exceeds the Sobol dimension budget"); return ORION_ERR_UNSUPPORTED; }
+  orion_ctx c = s->ctx;
+  HIP_TRY(hipSetDevice(c->device));
+  tune = effective_tuning(s, tune);
+  if (!s->runner) s->runner = new WaveRunner();
+  WaveRunner &R = *s->runner;
+  int S = tune && tune->spp_per_wave > 0 ? tune->spp_per_wave : std::max(1, kDefaultWavePaths / npix);
+  S = std::min(S, std::max(1, k1 - k0));
+  int rc;
+  if ((rc = R.init(1))) return rc;
+  if ((rc = R.ensure_cap(S * npix))) return rc;
+  if ((rc = R.reset_stats(c->stream))) return rc;
+  const int stack = s->d.stack_entries;
+  const int blocks = trace_blocks(c, stack, tune);
+  const bool graph = tune && tune->use_graph;
+  LaunchTimer tm;
+  tm.on = tune && tune->profile;
+  tm.st = c->stream;
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  for (int k = k0; k < k1; k += S) {
+    int Sk = std::min(S, k1 - k);
+    Job J = make_job(s, s->W, s->H, maxDepth, (float *)s->film.ptr, (int *)s->count.ptr);
+    J.S = Sk;
+    J.k0 = (uint32_t)k;
+    if ((rc = R.run(c->stream, &J, 1, Sk * npix, npix, maxDepth, stack, false, blocks, graph, tm.on ? &tm : nullptr)))
+      return rc;
+  }
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->ms = ms;
+  }
+  if ((rc = tm.collect(stats))) return rc;
+  return R.read_stats(stats);
+}
+
+// ======================================================================= batched launch
+namespace orion_amd {
+
+struct JobState {
+  orion_job_desc d;
+  int next_k = 0;
+  DevBuf film, count;
+};
+
+constexpr int kMaxWavePaths = 1 << 23;
+
+// Picks each active job's next sample slab for one wave.  The wave holds
+// max(target, one sample of every active job) paths, at most kMaxWavePaths
+// (or one job's film if that is larger).  First every job gets one sample,
+// visited from a rotating start (*rr) so that a job left out of a full wave
+// goes first in the next one; the capacity left is then split evenly, in
+// whole samples.  taken[i] = samples given to act[i].
+static std::vector<Job> compose_wave(std::vector<JobState *> &act, int target, int *rr, int *npaths, int *npix,
+                                     int *depth, int *stack, std::vector<int> *taken) {
+  const int n_act = (int)act.size();
+  taken->assign(n_act, 0);
+  int64_t sum_np = 0;
+  int max_np = 1;
+  for (JobState *js : act)
+    if (js->d.spp > js->next_k) {
+      int np = js->d.width * js->d.height;
+      sum_np += np;
+      max_np = std::max(max_np, np);
+    }
+  const int64_t cap = std::max<int64_t>(max_np, std::min<int64_t>(std::max<int64_t>(target, sum_np),
+                                                                   std::max(target, kMaxWavePaths)));
+  // one sample each, rotating start
+  std::vector<int> order;
+  int64_t paths = 0;
+  const int start = n_act ? *rr % n_act : 0;
+  for (int t = 0; t < n_act; ++t) {
+    const int i = (start + t) % n_act;
+    JobState *js = act[i];
+    const int64_t np = (int64_t)js->d.width * js->d.height;
+    if (js->d.spp <= js->next_k || paths + np > cap) continue;
+    (*taken)[i] = 1;
+    paths += np;
+    order.push_back(i);
+  }
+  if (!order.empty()) *rr = order.back() + 1;
+  // the rest of the capacity, evenly
+  if (!order.empty()) {
+    const int64_t share = (cap - paths) / (int64_t)order.size();
+    for (int i : order) {
+      JobState *js = act[i];
+      const int64_t np = (int64_t)js->d.width * js->d.height;
+      const int extra = (int)std::min<int64_t>(js->d.spp - js->next_k - 1, share / np);
+      (*taken)[i] += extra;
+      paths += extra * np;
+    }
+  }
+  std::vector<Job> jobs;
+  int64_t p0 = 0;
+  int pix = 0;
+  *depth = 0;
+  *stack = 16;
+  for (int i : order) {
+    JobState *js = act[i];
+    const int np = js->d.width * js->d.height;
+    Job J = make_job(js->d.scene, js->d.width, js->d.height, js->d.max_depth, (float *)js->film.ptr,
+                     (int *)js->count.ptr);
+    J.S = (*taken)[i];
+    J.k0 = (uint32_t)js->next_k;
+    J.path0 = (int)p0;
+    J.pix0 = pix;
+    p0 += (int64_t)J.S * np;
+    pix += np;
+    *depth = std::max(*depth, js->d.max_depth);
+    *stack = std::max(*stack, js->d.scene->d.stack_entries);
+    jobs.push_back(J);
+  }
+  *npaths = (int)p0;
+  *npix = pix;
+  return jobs;
+}
+
+static int check_desc(const orion_job_desc &d) {
+  if (!d.scene || d.width <= 0 || d.height <= 0 || d.spp < 0 || d.max_depth < 0 || d.max_depth > 60) {
+    set_error("bad job description");
+    return ORION_ERR_ARG;
+  }
+  if ((uint64_t)d.spp * (uint64_t)d.width * (uint64_t)d.height > 0xffffffffull) {
+    set_error("sample index k * W * H must stay below 2^32");
+    return ORION_ERR_UNSUPPORTED;
+  }
+  return ORION_OK;
+}
+
+}  // namespace orion_amd
+
+extern "C" int orion_render_batch(orion_ctx c, int n, const orion_job_desc *descs, float *const *rgb_out,
+                                  int *const *count_out, const orion_tuning *tune, orion_render_stats *stats) {
+  if (!c || n < 0 || (n && (!descs || !rgb_out || !count_out))) { set_error("bad argument"); return ORION_ERR_ARG; }
+  for (int j = 0; j < n; ++j) {
+    int rc = check_desc(descs[j]);
+    if (rc) return rc;
+    if (descs[j].scene->ctx != c) { set_error("job scene belongs to another context"); return ORION_ERR_ARG; }
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  if (!c->runner) c->runner = new WaveRunner();
+  WaveRunner &R = *c->runner;
+  std::vector<JobState> js(n);
+  int max_np = 1;
+  for (int j = 0; j < n; ++j) {
+    js[j].d = descs[j];
+    size_t np = (size_t)descs[j].width * descs[j].height;
+    max_np = std::max(max_np, (int)np);
+    int rc;
+    if ((rc = js[j].film.ensure(np * 12))) return rc;
+    if ((rc = js[j].count.ensure(np * 4))) return rc;
+    HIP_TRY(hipMemsetAsync(js[j].film.ptr, 0, np * 12, c->stream));
+    HIP_TRY(hipMemsetAsync(js[j].count.ptr, 0, np * 4, c->stream));
+  }
+  int target = tune && tune->spp_per_wave > 0 ? tune->spp_per_wave * max_np : kDefaultWavePaths;
+  int rc;
+  if ((rc = R.init(std::max(n, 1)))) return rc;
+  if ((rc = R.reset_stats(c->stream))) return rc;
+  const bool graph = tune && tune->use_graph;
+  LaunchTimer tm;
+  tm.on = tune && tune->profile;
+  tm.st = c->stream;
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  std::vector<JobState *> act;
+  for (auto &j : js) act.push_back(&j);
+  int rr = 0;
+  for (;;) {
+    int npaths, npix, depth, stack;
+    std::vector<int> taken;
+    std::vector<Job> jobs = compose_wave(act, target, &rr, &npaths, &npix, &depth, &stack, &taken);
+    if (jobs.empty()) break;
+    int blocks = trace_blocks(c, stack, tune);
+    if ((rc = R.run(c->stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, stack, jobs.size() > 1, blocks, graph,
+                    tm.on ? &tm : nullptr)))
+      return rc;
+    for (size_t i = 0; i < act.size(); ++i) act[i]->next_k += taken[i];
+    act.erase(std::remove_if(act.begin(), act.end(), [](JobState *j) { return j->next_k >= j->d.spp; }), act.end());
+  }
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  for (int j = 0; j < n; ++j) {
+    size_t np = (size_t)descs[j].width * descs[j].height;
+    HIP_TRY(hipMemcpyAsync(rgb_out[j], js[j].film.ptr, np * 12, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(count_out[j], js[j].count.ptr, np * 4, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->ms = ms;
+  }
+  if ((rc = tm.collect(stats))) return rc;
+  return R.read_stats(stats);
+}
+
+// ======================================================================= serving
+namespace orion_amd {
+struct ServerJob {
+  uint64_t id;
+  JobState st;
+  int state = 0;  // 0 queued, 1 active, 2 done, 3 failed
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  int64_t bytes = 0;
+  std::vector<float> rgb;
+  std::vector<int> cnt;
+};
+}  // namespace orion_amd
+
+struct orion_server_s {
+  orion_ctx ctx = nullptr;
+  hipStream_t stream = nullptr;  // the serving loop's own stream
+  int64_t budget = 0, in_use = 0;
+  orion_tuning tune{};
+  bool has_tune = false;
+  std::mutex mu;
+  std::condition_variable cv, cv_done;
+  std::deque<orion_amd::ServerJob *> queue;
+  std::vector<orion_amd::ServerJob *> active;
+  std::map<uint64_t, orion_amd::ServerJob *> all;
+  uint64_t next_id = 1;
+  bool stop = false;
+  int error = 0;
+  std::string error_msg;
+  std::thread th;
+  orion_amd::WaveRunner runner;
+  std::chrono::steady_clock::time_point t0;
+  int64_t jobs_done = 0, waves = 0, paths = 0, closest = 0, shadow = 0, mixed_waves = 0;
+  int max_jobs_in_wave = 0;
+  double busy_ms = 0;
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+  void loop();
+};
+
+void orion_server_s::loop() {
+  using namespace orion_amd;
+  (void)hipSetDevice(ctx->device);
+  const orion_tuning *tn = has_tune ? &tune : nullptr;
+  // wave size in paths: spp_per_wave x 2^18 (a 512x512 film's worth)
+  int target = tn && tn->spp_per_wave > 0 ? tn->spp_per_wave * (1 << 18) : kDefaultWavePaths;
+  int rr = 0;
+  for (;;) {
+    std::vector<ServerJob *> admitted;
+    {
+      std::unique_lock<std::mutex> lk(mu);
+      cv.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
+      if (stop && active.empty() && queue.empty()) return;
+      // admission against the HBM budget (FIFO; one job is always admitted
+      // when nothing runs, so an oversized job cannot stall the queue)
+      while (!queue.empty()) {
+        ServerJob *j = queue.front();
+        if (!active.empty() && in_use + j->bytes > budget) break;
+        queue.pop_front();
+        in_use += j->bytes;
+        admitted.push_back(j);
+        active.push_back(j);
+      }
+    }
+    for (ServerJob *j : admitted) {
+      size_t np = (size_t)j->st.d.width * j->st.d.height;
+      int rc = j->st.film.ensure(np * 12);
+      if (!rc) rc = j->st.count.ensure(np * 4);
+      if (!rc && (hipMemsetAsync(j->st.film.ptr, 0, np * 12, stream) != hipSuccess ||
+                  hipMemsetAsync(j->st.count.ptr, 0, np * 4, stream) != hipSuccess))
+        rc = ORION_ERR_HIP;
+      std::lock_guard<std::mutex> lk(mu);
+      j->t_admit = now();
+      if (rc) {
+        j->state = 3;
+        error = rc;
+        error_msg = orion_last_error();
+        in_use -= j->bytes;
+        active.erase(std::find(active.begin(), active.end(), j));
+      } else {
+        j->state = 1;
+      }
+    }
+    if (!admitted.empty()) cv_done.notify_all();
+    std::vector<JobState *> act;
+    std::vector<ServerJob *> owners;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (ServerJob *j : active)
+        if (j->state == 1) { act.push_back(&j->st); owners.push_back(j); }
+    }
+    if (act.empty()) continue;
+    int npaths, npix, depth, stack;
+    std::vector<int> taken;
+    std::vector<Job> jobs = compose_wave(act, target, &rr, &npaths, &npix, &depth, &stack, &taken);
+    auto w0 = std::chrono::steady_clock::now();
+    int rc = runner.init(std::max(1, (int)jobs.size()));
+    if (!rc) rc = runner.reset_stats(stream);
+    if (!rc && !jobs.empty())
+      rc = runner.run(stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, stack, jobs.size() > 1,
+                      trace_blocks(ctx, stack, tn), tn && tn->use_graph, nullptr);
+    if (!rc && hipStreamSynchronize(stream) != hipSuccess) rc = ORION_ERR_HIP;
+    orion_render_stats ws{};
+    if (!rc) rc = runner.read_stats(&ws);
+    double wms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    std::vector<ServerJob *> finished;
+    for (size_t i = 0; i < act.size(); ++i) {
+      act[i]->next_k += taken[i];
+      if (act[i]->next_k >= act[i]->d.spp) finished.push_back(owners[i]);
+    }
+    for (ServerJob *j : finished) {
+      size_t np = (size_t)j->st.d.width * j->st.d.height;
+      j->rgb.resize(np * 3);
+      j->cnt.resize(np);
+      if (!rc && (hipMemcpy(j->rgb.data(), j->st.film.ptr, np * 12, hipMemcpyDeviceToHost) != hipSuccess ||
+                  hipMemcpy(j->cnt.data(), j->st.count.ptr, np * 4, hipMemcpyDeviceToHost) != hipSuccess))
+        rc = ORION_ERR_HIP;
+      j->st.film.release();
+      j->st.count.release();
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      waves++;
+      if (jobs.size() > 1) mixed_waves++;
+      max_jobs_in_wave = std::max(max_jobs_in_wave, (int)jobs.size());
+      busy_ms += wms;
+      paths += ws.paths;
+      closest += ws.closest_rays;
+      shadow += ws.shadow_rays;
+      if (rc) { error = rc; error_msg = orion_last_error(); }
+      for (ServerJob *j : finished) {
+        j->state = rc ? 3 : 2;
+        j->t_done = now();
+        in_use -= j->bytes;
+        jobs_done++;
+        active.erase(std::find(active.begin(), active.end(), j));
+      }
+    }
+    cv_done.notify_all();
+  }
+}
+
+extern "C" {
+
+int orion_server_create(orion_ctx c, int64_t budget, const orion_tuning *tune, orion_server *out) {
+  if (!c || !out || budget <= 0) { set_error("bad argument"); return ORION_ERR_ARG; }
+  HIP_TRY(hipSetDevice(c->device));
+  auto *s = new orion_server_s();
+  s->ctx = c;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    set_error(std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+    return ORION_ERR_HIP;
+  }
+  s->budget = budget;
+  if (tune) { s->tune = *tune; s->has_tune = true; }
+  s->t0 = std::chrono::steady_clock::now();
+  s->th = std::thread([s] { s->loop(); });
+  *out = s;
+  return ORION_OK;
+}
+
+int orion_server_submit(orion_server s, const orion_job_desc *d, uint64_t *id) {
+  if (!s || !d || !id) { set_error("bad argument"); return ORION_ERR_ARG; }
+  int rc = orion_amd::check_desc(*d);
+  if (rc) return rc;
+  auto *j = new orion_amd::ServerJob();
+  j->st.d = *d;
+  j->bytes = (int64_t)d->width * d->height * 16;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) { delete j; set_error("server stopped"); return ORION_ERR_BUSY; }
+    j->id = s->next_id++;
+    j->t_submit = s->now();
+    s->queue.push_back(j);
+    s->all[j->id] = j;
+    *id = j->id;
+  }
+  s->cv.notify_one();
+  return ORION_OK;
+}
+
+int orion_server_poll(orion_server s, uint64_t id, int *done, float *rgb, int *count, double times[3]) {
+  if (!s || !done) { set_error("bad argument"); return ORION_ERR_ARG; }
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->all.find(id);
+  if (it == s->all.end()) { set_error("unknown job id"); return ORION_ERR_ARG; }
+  orion_amd::ServerJob *j = it->second;
+  if (j->state == 3) { set_error("job failed: " + s->error_msg); return s->error ? s->error : ORION_ERR_HIP; }
+  *done = j->state == 2;
+  if (times) { times[0] = j->t_submit; times[1] = j->t_admit; times[2] = j->t_done; }
+  if (*done) {
+    if (rgb) std::memcpy(rgb, j->rgb.data(), j->rgb.size() * 4);
+    if (count) std::memcpy(count, j->cnt.data(), j->cnt.size() * 4);
+  }
+  return ORION_OK;
+}
+
+int orion_server_drain(orion_server s) {
+  if (!s) { set_error("bad argument"); return ORION_ERR_ARG; }
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [&] { return s->queue.empty() && s->active.empty(); });
+  if (s->error) { set_error(s->error_msg); return s->error; }
+  return ORION_OK;
+}
+
+int orion_server_get_stats(orion_server s, orion_server_stats *st) {
+  if (!s || !st) { set_error("bad argument"); return ORION_ERR_ARG; }
+  std::lock_guard<std::mutex> lk(s->mu);
+  st->jobs_done = s->jobs_done;
+  st->waves = s->waves;
+  st->paths = s->paths;
+  st->closest_rays = s->closest;
+  st->shadow_rays = s->shadow;
+  st->active_jobs = (int)s->active.size();
+  st->queued_jobs = (int)s->queue.size();
+  st->hbm_in_use = s->in_use;
+  st->busy_ms = s->busy_ms;
+  st->mixed_waves = s->mixed_waves;
+  st->max_jobs_in_wave = s->max_jobs_in_wave;
+  return ORION_OK;
+}
+
+int orion_server_destroy(orion_server s) {
+  if (!s) return ORION_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  if (s->th.joinable()) s->th.join();
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->stream);
+  for (auto &kv : s->all) delete kv.second;
+  s->runner.drop_graphs();
+  (void)hipStreamDestroy(s->stream);
+  delete s;
+  return ORION_OK;
+}
+
+// ======================================================================= autotuner
+int orion_autotune(orion_scene s, int maxDepth, orion_tuning *best) {
+  if (!s || !best) { set_error("bad argument"); return ORION_ERR_ARG; }
+  const int npix = s->W * s->H;
+  const int waves[] = {1 << 19, 1 << 20, 1 << 21, 1 << 22};
+  const int bpcs[] = {0, 4, 8, 16};
+  double best_rate = 0;
+  orion_tuning b{};
+  for (int wpaths : waves) {
+    int S = std::max(1, wpaths / npix);
+    if ((uint64_t)2 * S * (uint64_t)npix > 0xffffffffull) continue;
+    for (int bpc : bpcs) {
+      for (int graph = 0; graph < 2; ++graph) {
+        orion_tuning t{};
+        t.spp_per_wave = S;
+        t.persistent_blocks = bpc;
+        t.use_graph = graph;
+        orion_render_stats st{};
+        // warm (graph capture / allocation), then time one wave
+        int rc = orion_render_path(s, 0, S, maxDepth, &t, &st);
+        if (rc) return rc;
+        rc = orion_render_path(s, S, 2 * S, maxDepth, &t, &st);
+        if (rc) return rc;
+        double rate = st.ms > 0 ? st.paths / st.ms : 0;
+        if (rate > best_rate) { best_rate = rate; b = t; }
+      }
+    }
+  }
+  int rc = orion_film_clear(s);
+  if (rc) return rc;
+  s->tuned = b;
+  s->has_tuned = true;
+  *best = b;
+  return ORION_OK;
+}
+
+}  // extern "C"

@@ -16,7 +16,7 @@
 
 namespace orion_amd {
 
-constexpr int kStackEntries = kBvhMaxDepth + 2;
+constexpr int kStackEntries = kBvhMaxDepth + 2;  // worst case; kernels use the scene's depth + 2
 
 struct DScene {
   const float4 *nodes;   // 4 x float4 per node (bvh.h)
@@ -31,30 +31,25 @@ struct DScene {
   float light_func_int;
   int n_lights;
   int n_prims;
+  int stack_entries;  // traversal stack depth this scene needs (BVH depth + 2)
   float cam[16];
 };
 
 namespace dev {
 
-// Robust slab test (Ize 2013 / pbrt-v3): t_far scaled by 1 + 2 gamma(3).
-__device__ __forceinline__ bool slab(float lo, float hi, float o, float inv, float &t0, float &t1) {
-  float tn = (lo - o) * inv;
-  float tf = (hi - o) * inv;
-  if (tn > tf) { float t = tn; tn = tf; tf = t; }
-  tf *= 1.00000035762786865234375f;  // 1 + 2 * gamma(3), gamma(3) = 3 eps / (1 - 3 eps)
-  t0 = tn > t0 ? tn : t0;
-  t1 = tf < t1 ? tf : t1;
-  return t0 <= t1;
-}
-
+// Robust slab test (Ize 2013 / pbrt-v3): the exit distance is scaled by
+// 1 + 2 gamma(3) so rounding never culls a box the exact test would enter.
+// The scale is applied once to min(t_far) -- identical to scaling each slab
+// first, as multiplication by a positive constant is monotonic.
 __device__ __forceinline__ bool box_hit(float lox, float hix, float loy, float hiy, float loz, float hiz,
                                         V3 o, V3 inv, float tmin, float tmax, float *tnear) {
-  float t0 = tmin, t1 = tmax;
-  if (!slab(lox, hix, o.x, inv.x, t0, t1)) return false;
-  if (!slab(loy, hiy, o.y, inv.y, t0, t1)) return false;
-  if (!slab(loz, hiz, o.z, inv.z, t0, t1)) return false;
+  float ax = (lox - o.x) * inv.x, bx = (hix - o.x) * inv.x;
+  float ay = (loy - o.y) * inv.y, by = (hiy - o.y) * inv.y;
+  float az = (loz - o.z) * inv.z, bz = (hiz - o.z) * inv.z;
+  float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+  float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz)) * 1.00000035762786865234375f;
   *tnear = t0;
-  return true;
+  return t0 <= fminf(t1, tmax);
 }
 
 // Triangle test: identical operation sequence to the CPU restatement.
@@ -88,7 +83,10 @@ struct HitRec {
 template <bool ANY>
 __device__ inline bool traverse(const DScene &sc, V3 o, V3 d, float tmin, float tmax, HitRec *h, int *stk,
                                 int stride, uint32_t *node_visits, uint32_t *tri_tests) {
-  V3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  // |d| components below 1e-35 are clamped so (lo - o) * inv is never 0 * inf
+  // (a NaN would drop a slab); the padded boxes absorb the change
+  auto safe = [](float x) { return fabsf(x) > 1e-35f ? x : copysignf(1e-35f, x); };
+  V3 inv = mk(1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z));
   bool found = false;
   float best = tmax;
   h->prim = -1;

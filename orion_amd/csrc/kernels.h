@@ -9,19 +9,17 @@ namespace orion_amd {
 
 constexpr int kTraceBlock = 128;  // threads per traversal block (2 wave64)
 
-void launch_raygen(const DScene &sc, const Wave &w, hipStream_t st);
-void launch_trace(const DScene &sc, const Wave &w, bool any, int blocks, hipStream_t st);
+// Wave launchers: `jobs` is the device job table, `multi` selects the
+// per-path job lookup (false: every path belongs to job 0).
+void launch_raygen(const Job *jobs, const Wave &w, int max_paths, bool multi, hipStream_t st);
+void launch_trace(const Job *jobs, const Wave &w, bool any, int stack_entries, int blocks, bool multi,
+                  hipStream_t st);
+int stack_variant(int bvh_depth);
+int trace_blocks_per_cu(int stack_entries, bool any);
 void launch_between(const Wave &w, int nxt, hipStream_t st);
-void launch_shade(const DScene &sc, const Wave &w, int cur, int maxDepth, hipStream_t st);
-void launch_film(const Wave &w, int S, float *film, int *count, hipStream_t st);
+void launch_shade(const Job *jobs, const Wave &w, int cur, int max_paths, bool multi, hipStream_t st);
+void launch_film(const Job *jobs, const Wave &w, int max_pix, bool multi, hipStream_t st);
 void launch_intersect_batch(const DScene &sc, int n, const float *rays, float *t, int *prim, float *u, float *v,
                             int *tie, int *occluded, bool any, hipStream_t st);
-
-// SPPM (sppm_kernels.hip)
-struct SppmState;
-int sppm_render(const DScene &sc, SppmState *st, int n_iters, int photons, float r0, int maxDepth,
-                int hash_size, int debug_iter, int *vp_cell_count, int *photon_cell_count, int *M,
-                int *grid_info, float *grid_bounds, float *film, int *count, unsigned long long *stats,
-                hipStream_t stream, int W, int H);
 
 }  // namespace orion_amd

@@ -8,6 +8,12 @@
 // bounce b, direct light of bounce b, emission of bounce b + 1, ...), then
 // shades the new hit: emission, light sampling, BSDF sampling, Russian
 // roulette.  Queues are compacted with wave64 ballots (one atomic per wave).
+//
+// A wave may pack several jobs (device job table, wavefront.h): every path
+// carries its job index and reads that job's scene, camera, depth and film.
+// MULTI = false is the one-job fast path (job 0, no per-path job reads).
+// Per-wave sizes live in the control block, so a wave's launch sequence has
+// fixed arguments and can be captured once as a hipGraph.
 #include <hip/hip_runtime.h>
 
 #include "device_scene.h"
@@ -18,21 +24,36 @@
 namespace orion_amd {
 using namespace dev;
 
-__global__ void k_raygen(DScene sc, Wave w) {
+__device__ __forceinline__ int find_job(const Job *jobs, int njobs, int p) {
+  int lo = 0, hi = njobs - 1;  // last job with path0 <= p
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].path0 <= p) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool MULTI>
+__global__ void k_raygen(const Job *jobs, Wave w) {
+  const int npaths = w.ctrl[C_NPATHS];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p == 0) {
-    w.ctrl[C_PATH0] = w.N;
-    w.ctrl[C_RAY] = w.N;
+    w.ctrl[C_PATH0] = npaths;
+    w.ctrl[C_RAY] = npaths;
     w.ctrl[C_SHADOW] = 0;
     w.ctrl[C_WORK_RAY] = 0;
     w.ctrl[C_WORK_SHADOW] = 0;
   }
-  if (p >= w.N) return;
-  int s = p / w.npix, pix = p - s * w.npix;
-  int j = pix / w.W, i = pix - j * w.W;
-  uint32_t index = (w.k0 + (uint32_t)s) * (uint32_t)w.npix + (uint32_t)pix;
+  if (p >= npaths) return;
+  int jb = MULTI ? find_job(jobs, w.ctrl[C_NJOBS], p) : 0;
+  const Job &J = jobs[jb];
+  int q = p - J.path0;
+  int s = q / J.npix, pix = q - s * J.npix;
+  int j = pix / J.W, i = pix - j * J.W;
+  uint32_t index = (J.k0 + (uint32_t)s) * (uint32_t)J.npix + (uint32_t)pix;  // path.cpp:26
   V3 o, d;
-  camera_ray(sc, i, j, index, &o, &d);
+  camera_ray(J.sc, i, j, index, &o, &d);
   w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
   w.dx[p] = d.x; w.dy[p] = d.y; w.dz[p] = d.z;
   w.tmax[p] = kFltMax;
@@ -41,15 +62,16 @@ __global__ void k_raygen(DScene sc, Wave w) {
   w.dim[p] = 7;
   w.bounce[p] = 0;
   w.flags[p] = F_ALIVE;
+  if (MULTI) w.job[p] = jb;
   w.qpath[0][p] = p;
   w.qray[p] = p;
 }
 
 // ---------------------------------------------------------------- traversal
 // Persistent: each wave pulls 64-ray tiles from the queue until it drains.
-template <bool ANY>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(DScene sc, Wave w) {
-  __shared__ int stack[kStackEntries * kTraceBlock];
+template <bool ANY, int STACK, bool MULTI>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w) {
+  __shared__ int stack[STACK * kTraceBlock];
   int *stk = stack + threadIdx.x;
   const int count = w.ctrl[ANY ? C_SHADOW : C_RAY];
   int *work = w.ctrl + (ANY ? C_WORK_SHADOW : C_WORK_RAY);
@@ -65,6 +87,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(DScene sc, Wave w) {
     if (q < count) {
       int r = queue[q];
       int slot = ANY ? 2 * w.N + r : r;  // shadow rays live in slot 2
+      int p = ANY ? r : (r >= w.N ? r - w.N : r);
+      const DScene &sc = jobs[MULTI ? w.job[p] : 0].sc;
       V3 o = mk(w.ox[slot], w.oy[slot], w.oz[slot]);
       V3 d = mk(w.dx[slot], w.dy[slot], w.dz[slot]);
       HitRec h;
@@ -106,13 +130,9 @@ __global__ void k_between(Wave w, int nxt) {
   }
 }
 
-__device__ __forceinline__ uint32_t path_index(const Wave &w, int p) {
-  int s = p / w.npix;
-  return (w.k0 + (uint32_t)s) * (uint32_t)w.npix + (uint32_t)(p - s * w.npix);
-}
-
 // ---------------------------------------------------------------- shade
-__global__ __launch_bounds__(256) void k_shade(DScene sc, Wave w, int cur, int maxDepth) {
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_shade(const Job *jobs, Wave w, int cur) {
   const int nxt = cur ^ 1;
   const int count = w.ctrl[C_PATH0 + cur];
   const int N = w.N;
@@ -122,6 +142,8 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, Wave w, int cur, int m
     int p = act ? w.qpath[cur][i] : 0;
     bool want_ray0 = false, want_ray1 = false, want_shadow = false, keep = false, done = false;
     if (act) {
+      const Job &J = jobs[MULTI ? w.job[p] : 0];
+      const DScene &sc = J.sc;
       int fl = w.flags[p];
       Sp L = sp3(w.Lr[p], w.Lg[p], w.Lb[p]);
       // 1) resolve the direct-lighting estimate of the previous bounce
@@ -138,12 +160,14 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, Wave w, int cur, int m
         Isect is;
         if (hp >= 0) make_isect(sc, hp, w.hu[p], w.hv[p], &is);
         if ((b == 0 || (fl & F_SPEC)) && hp >= 0 && is.light >= 0) acc(L, beta * area_emission(sc, is, neg(d)));
-        if (hp < 0 || b >= maxDepth) {
+        if (hp < 0 || b >= J.maxDepth) {
           fl &= ~F_ALIVE;
         } else {
           Bsdf bs;
           make_bsdf(sc.mats + 16 * (size_t)is.mat, is.ng, is.ns, is.u, is.v, false, &bs);
-          uint32_t index = path_index(w, p);
+          int q = p - J.path0;
+          int s = q / J.npix;
+          uint32_t index = (J.k0 + (uint32_t)s) * (uint32_t)J.npix + (uint32_t)(q - s * J.npix);
           int dim = w.dim[p];
           if (bsdf_num(bs, ALL & ~SPEC) > 0) {
             int nf = setup_direct(sc, w, p, bs, is, d, beta, index, &dim, false);
@@ -165,9 +189,9 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, Wave w, int cur, int m
             V3 o = offset_ray_origin(is.p, is.ng, wi);
             bool alive = true;
             if (b > 3) {
-              float q = mx(0.05f, 1 - maxc(beta));
-              if (sobol(sc.sobol, index, dim++) < q) alive = false;
-              else beta = beta / (1 - q);
+              float qq = mx(0.05f, 1 - maxc(beta));
+              if (sobol(sc.sobol, index, dim++) < qq) alive = false;
+              else beta = beta / (1 - qq);
             }
             if (alive) {
               w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
@@ -206,28 +230,43 @@ __global__ __launch_bounds__(256) void k_shade(DScene sc, Wave w, int cur, int m
 }
 
 // RenderTarget::acumulate in sample order per pixel (rendertarget.cpp:22-27)
-__global__ void k_film(Wave w, int S, float *film, int *count) {
-  int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (pix >= w.npix) return;
-  float r = film[3 * pix], g = film[3 * pix + 1], b = film[3 * pix + 2];
-  for (int s = 0; s < S; ++s) {
-    int p = s * w.npix + pix;
-    r += w.Lr[p];
-    g += w.Lg[p];
-    b += w.Lb[p];
+template <bool MULTI>
+__global__ void k_film(const Job *jobs, Wave w) {
+  const int total = w.ctrl[C_NPIX];
+  const int njobs = w.ctrl[C_NJOBS];
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    int jb = 0;
+    if (MULTI) {
+      int lo = 0, hi = njobs - 1;
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].pix0 <= g) lo = mid;
+        else hi = mid - 1;
+      }
+      jb = lo;
+    }
+    const Job &J = jobs[jb];
+    int pix = g - J.pix0;
+    float r = J.film[3 * pix], gg = J.film[3 * pix + 1], b = J.film[3 * pix + 2];
+    for (int s = 0; s < J.S; ++s) {
+      int p = J.path0 + s * J.npix + pix;
+      r += w.Lr[p];
+      gg += w.Lg[p];
+      b += w.Lb[p];
+    }
+    J.film[3 * pix] = r;
+    J.film[3 * pix + 1] = gg;
+    J.film[3 * pix + 2] = b;
+    J.count[pix] += J.S;
   }
-  film[3 * pix] = r;
-  film[3 * pix + 1] = g;
-  film[3 * pix + 2] = b;
-  count[pix] += S;
 }
 
 // ---------------------------------------------------------------- batch intersection API
-template <bool ANY>
+template <bool ANY, int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_intersect_batch(DScene sc, int n, const float *rays, float *t,
                                                                  int *prim, float *u, float *v, int *tie,
                                                                  int *occluded) {
-  __shared__ int stack[kStackEntries * kTraceBlock];
+  __shared__ int stack[STACK * kTraceBlock];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   V3 o = mk(rays[i], rays[n + i], rays[2 * n + i]);
@@ -247,29 +286,70 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect_batch(DScene sc, int 
 }
 
 // ---------------------------------------------------------------- launchers
-void launch_raygen(const DScene &sc, const Wave &w, hipStream_t st) {
-  hipLaunchKernelGGL(k_raygen, dim3((w.N + 255) / 256), dim3(256), 0, st, sc, w);
+// Stack-depth variants: the smallest that holds the BVH depth + 2, so LDS
+// (STACK x 128 lanes x 4 B per block) caps occupancy no more than needed.
+#define ORION_STACK_DISPATCH(S, CALL)                     \
+  do {                                                    \
+    if ((S) <= 16) { constexpr int ST = 16; CALL; }       \
+    else if ((S) <= 24) { constexpr int ST = 24; CALL; }  \
+    else if ((S) <= 32) { constexpr int ST = 32; CALL; }  \
+    else { constexpr int ST = kStackEntries; CALL; }      \
+  } while (0)
+
+int stack_variant(int depth) {
+  int need = depth + 2;
+  return need <= 16 ? 16 : need <= 24 ? 24 : need <= 32 ? 32 : kStackEntries;
 }
-void launch_trace(const DScene &sc, const Wave &w, bool any, int blocks, hipStream_t st) {
-  if (any) hipLaunchKernelGGL(k_trace<true>, dim3(blocks), dim3(kTraceBlock), 0, st, sc, w);
-  else hipLaunchKernelGGL(k_trace<false>, dim3(blocks), dim3(kTraceBlock), 0, st, sc, w);
+
+int trace_blocks_per_cu(int stack_entries, bool any) {
+  static int cache[2][4] = {{0}};
+  int vi = stack_entries <= 16 ? 0 : stack_entries <= 24 ? 1 : stack_entries <= 32 ? 2 : 3;
+  int &c = cache[any ? 1 : 0][vi];
+  if (c == 0) {
+    int nb = 0;
+    hipError_t e = hipSuccess;
+    if (any) ORION_STACK_DISPATCH(stack_entries, e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<true, ST, true>, kTraceBlock, 0));
+    else ORION_STACK_DISPATCH(stack_entries, e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false, ST, true>, kTraceBlock, 0));
+    c = (e == hipSuccess && nb > 0) ? nb : 4;
+  }
+  return c;
+}
+
+void launch_raygen(const Job *jobs, const Wave &w, int max_paths, bool multi, hipStream_t st) {
+  dim3 g((max_paths + 255) / 256);
+  if (multi) hipLaunchKernelGGL(k_raygen<true>, g, dim3(256), 0, st, jobs, w);
+  else hipLaunchKernelGGL(k_raygen<false>, g, dim3(256), 0, st, jobs, w);
+}
+void launch_trace(const Job *jobs, const Wave &w, bool any, int stack_entries, int blocks, bool multi,
+                  hipStream_t st) {
+  if (multi) {
+    if (any) ORION_STACK_DISPATCH(stack_entries, hipLaunchKernelGGL((k_trace<true, ST, true>), dim3(blocks), dim3(kTraceBlock), 0, st, jobs, w));
+    else ORION_STACK_DISPATCH(stack_entries, hipLaunchKernelGGL((k_trace<false, ST, true>), dim3(blocks), dim3(kTraceBlock), 0, st, jobs, w));
+  } else {
+    if (any) ORION_STACK_DISPATCH(stack_entries, hipLaunchKernelGGL((k_trace<true, ST, false>), dim3(blocks), dim3(kTraceBlock), 0, st, jobs, w));
+    else ORION_STACK_DISPATCH(stack_entries, hipLaunchKernelGGL((k_trace<false, ST, false>), dim3(blocks), dim3(kTraceBlock), 0, st, jobs, w));
+  }
 }
 void launch_between(const Wave &w, int nxt, hipStream_t st) {
   hipLaunchKernelGGL(k_between, dim3(1), dim3(64), 0, st, w, nxt);
 }
-void launch_shade(const DScene &sc, const Wave &w, int cur, int maxDepth, hipStream_t st) {
-  int blocks = (w.N + 255) / 256;
+void launch_shade(const Job *jobs, const Wave &w, int cur, int max_paths, bool multi, hipStream_t st) {
+  int blocks = (max_paths + 255) / 256;
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, st, sc, w, cur, maxDepth);
+  if (multi) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, st, jobs, w, cur);
+  else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, st, jobs, w, cur);
 }
-void launch_film(const Wave &w, int S, float *film, int *count, hipStream_t st) {
-  hipLaunchKernelGGL(k_film, dim3((w.npix + 255) / 256), dim3(256), 0, st, w, S, film, count);
+void launch_film(const Job *jobs, const Wave &w, int max_pix, bool multi, hipStream_t st) {
+  int blocks = (max_pix + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (multi) hipLaunchKernelGGL(k_film<true>, dim3(blocks), dim3(256), 0, st, jobs, w);
+  else hipLaunchKernelGGL(k_film<false>, dim3(blocks), dim3(256), 0, st, jobs, w);
 }
 void launch_intersect_batch(const DScene &sc, int n, const float *rays, float *t, int *prim, float *u, float *v,
                             int *tie, int *occluded, bool any, hipStream_t st) {
   dim3 g((n + kTraceBlock - 1) / kTraceBlock);
-  if (any) hipLaunchKernelGGL(k_intersect_batch<true>, g, dim3(kTraceBlock), 0, st, sc, n, rays, t, prim, u, v, tie, occluded);
-  else hipLaunchKernelGGL(k_intersect_batch<false>, g, dim3(kTraceBlock), 0, st, sc, n, rays, t, prim, u, v, tie, occluded);
+  if (any) ORION_STACK_DISPATCH(sc.stack_entries, hipLaunchKernelGGL((k_intersect_batch<true, ST>), g, dim3(kTraceBlock), 0, st, sc, n, rays, t, prim, u, v, tie, occluded));
+  else ORION_STACK_DISPATCH(sc.stack_entries, hipLaunchKernelGGL((k_intersect_batch<false, ST>), g, dim3(kTraceBlock), 0, st, sc, n, rays, t, prim, u, v, tie, occluded));
 }
 
 }  // namespace orion_amd

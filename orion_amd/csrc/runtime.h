@@ -33,6 +33,8 @@ struct WaveBuffers {
 
 struct SppmBuffers;
 void destroy_sppm(SppmBuffers *b);
+struct WaveRunner;
+void destroy_runner(WaveRunner *r);
 
 }  // namespace orion_amd
 
@@ -42,6 +44,7 @@ struct orion_ctx_s {
   std::string arch;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  orion_amd::WaveRunner *runner = nullptr;  // batched launches
 };
 
 struct orion_scene_s {
@@ -53,7 +56,9 @@ struct orion_scene_s {
   orion_amd::DScene d{};
   orion_amd::WaveBuffers wave;
   orion_amd::SppmBuffers *sppm = nullptr;
+  orion_amd::WaveRunner *runner = nullptr;  // plain path renders
+  orion_tuning tuned{};
+  bool has_tuned = false;
   double build_ms = 0, upload_ms = 0;
-  unsigned long long last_nodes = 0, last_tris = 0;
   ~orion_scene_s();
 };

@@ -59,7 +59,7 @@ int WaveBuffers::ensure(int N) {
   if (N <= cap && mem.ptr) return ORION_OK;
   // floats/ints per path: rays 7 x 3 slots, hits 4 x 2, vis 1, state 9,
   // pending 15, queues 2 + 2 + 1
-  const size_t per = 21 + 8 + 1 + 9 + 15 + 5;
+  const size_t per = 21 + 8 + 1 + 9 + 15 + 5 + 1;
   int rc = mem.ensure(per * (size_t)N * 4 + 256);
   if (rc) return rc;
   cap = N;
@@ -82,6 +82,8 @@ int WaveBuffers::ensure(int N) {
   w.bpdf = (float *)take(N); w.sel = (float *)take(N);
   w.nbr = (float *)take(N); w.nbg = (float *)take(N); w.nbb = (float *)take(N);
   w.nlight = (int *)take(N);
+  w.job = (int *)take(N);
+  w.N = N;  // slot stride
   w.qpath[0] = (int *)take(N); w.qpath[1] = (int *)take(N);
   w.qray = (int *)take(2 * (size_t)N);
   w.qshadow = (int *)take(N);
@@ -123,6 +125,7 @@ int orion_ctx_destroy(orion_ctx c) {
   if (!c) return ORION_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  orion_amd::destroy_runner(c->runner);
   (void)hipEventDestroy(c->ev0);
   (void)hipEventDestroy(c->ev1);
   (void)hipStreamDestroy(c->stream);
@@ -248,6 +251,7 @@ int orion_scene_upload(orion_ctx c, orion_hscene hs, orion_scene *out) {
   d.light_func_int = funcInt;
   d.n_lights = nl;
   d.n_prims = n;
+  d.stack_entries = stack_variant(sc->bvh.depth);
   std::memcpy(d.cam, s.camera, sizeof(d.cam));
   sc->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   sc->upload_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
@@ -349,128 +353,9 @@ int orion_film_read(orion_scene s, float *rgb, int *count) {
   return ORION_OK;
 }
 
-static int read_stats(orion_scene s, orion_render_stats *st, float ms) {
-  unsigned long long h[S_COUNT];
-  HIP_TRY(hipMemcpy(h, s->stats.ptr, sizeof(h), hipMemcpyDeviceToHost));
-  if (st) {
-    st->paths = (int64_t)h[S_PATHS];
-    st->closest_rays = (int64_t)h[S_CLOSEST];
-    st->shadow_rays = (int64_t)h[S_SHADOW];
-    st->ref_equiv_rays = (int64_t)(h[S_CLOSEST] + h[S_SHADOW] + h[S_REFEQ]);
-    st->photons = (int64_t)h[S_PHOTONS];
-    st->deposits = (int64_t)h[S_DEPOSITS];
-    st->ms = ms;
-    st->node_visits_closest = (int64_t)h[S_NODES_C];
-    st->tri_tests_closest = (int64_t)h[S_TRIS_C];
-    st->node_visits_shadow = (int64_t)h[S_NODES_S];
-    st->tri_tests_shadow = (int64_t)h[S_TRIS_S];
-  }
-  s->last_nodes = h[S_NODES_C];
-  s->last_tris = h[S_TRIS_C];
-  return ORION_OK;
-}
-
-// Per-launch HIP events (recorded on the launch stream) for kernel timing.
-struct LaunchTimer {
-  bool on = false;
-  hipStream_t st = nullptr;
-  std::vector<hipEvent_t> ev;
-  std::vector<int> kind;  // 0 closest, 1 shadow, 2 shade
-  int begin(int k) {
-    if (!on) return ORION_OK;
-    hipEvent_t e;
-    HIP_TRY(hipEventCreate(&e));
-    HIP_TRY(hipEventRecord(e, st));
-    ev.push_back(e);
-    kind.push_back(k);
-    return ORION_OK;
-  }
-  int end() {
-    if (!on) return ORION_OK;
-    hipEvent_t e;
-    HIP_TRY(hipEventCreate(&e));
-    HIP_TRY(hipEventRecord(e, st));
-    ev.push_back(e);
-    kind.push_back(-1);
-    return ORION_OK;
-  }
-  // call after the stream has been synchronised
-  int collect(orion_render_stats *s) {
-    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
-      float ms = 0;
-      HIP_TRY(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-      if (!s) continue;
-      if (kind[i] == 0) { s->trace_closest_ms += ms; s->trace_closest_launches++; }
-      else if (kind[i] == 1) { s->trace_shadow_ms += ms; s->trace_shadow_launches++; }
-      else { s->shade_ms += ms; s->shade_launches++; }
-    }
-    return ORION_OK;
-  }
-  ~LaunchTimer() {
-    for (auto e : ev) (void)hipEventDestroy(e);
-  }
-};
-
-int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, const orion_tuning *tune,
-                      orion_render_stats *stats) {
-  if (!s) { set_error("null scene"); return ORION_ERR_ARG; }
-  if (k0 < 0 || k1 < k0 || maxDepth < 0) { set_error("bad sample range / depth"); return ORION_ERR_ARG; }
-  const int npix = s->W * s->H;
-  if ((uint64_t)k1 * (uint64_t)npix > 0xffffffffull) {
-    set_error("sample index k * W * H must stay below 2^32");
-    return ORION_ERR_UNSUPPORTED;
-  }
-  if (maxDepth > 60) { set_error("maxDepth above 60 exceeds the Sobol dimension budget"); return ORION_ERR_UNSUPPORTED; }
-  orion_ctx c = s->ctx;
-  HIP_TRY(hipSetDevice(c->device));
-  int S = tune && tune->spp_per_wave > 0 ? tune->spp_per_wave : std::max(1, (1 << 21) / npix);
-  S = std::min(S, std::max(1, k1 - k0));
-  int rc = s->wave.ensure(S * npix);
-  if (rc) return rc;
-  int bpc = tune && tune->persistent_blocks > 0 ? tune->persistent_blocks : 6;
-  int trace_blocks = c->n_cu * bpc;
-  HIP_TRY(hipMemsetAsync(s->stats.ptr, 0, S_COUNT * 8, c->stream));
-  LaunchTimer tm;
-  tm.on = tune && tune->profile;
-  tm.st = c->stream;
-  HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  Wave w = s->wave.wave;
-  w.ctrl = (int *)s->ctrl.ptr;
-  w.stats = (unsigned long long *)s->stats.ptr;
-  w.npix = npix;
-  w.W = s->W;
-  w.H = s->H;
-  for (int k = k0; k < k1; k += S) {
-    int Sk = std::min(S, k1 - k);
-    w.N = Sk * npix;
-    w.k0 = (uint32_t)k;
-    launch_raygen(s->d, w, c->stream);
-    int cur = 0;
-    for (int it = 0; it <= maxDepth; ++it) {
-      if ((rc = tm.begin(0))) return rc;
-      launch_trace(s->d, w, false, trace_blocks, c->stream);
-      if ((rc = tm.end())) return rc;
-      if ((rc = tm.begin(1))) return rc;
-      launch_trace(s->d, w, true, trace_blocks, c->stream);
-      if ((rc = tm.end())) return rc;
-      launch_between(w, cur ^ 1, c->stream);
-      if ((rc = tm.begin(2))) return rc;
-      launch_shade(s->d, w, cur, maxDepth, c->stream);
-      if ((rc = tm.end())) return rc;
-      cur ^= 1;
-    }
-    launch_film(w, Sk, (float *)s->film.ptr, (int *)s->count.ptr, c->stream);
-    HIP_TRY(hipGetLastError());
-  }
-  HIP_TRY(hipEventRecord(c->ev1, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  if (stats) std::memset(stats, 0, sizeof(*stats));
-  if ((rc = tm.collect(stats))) return rc;
-  return read_stats(s, stats, ms);
-}
-
 }  // extern "C"
 
-orion_scene_s::~orion_scene_s() { orion_amd::destroy_sppm(sppm); }
+orion_scene_s::~orion_scene_s() {
+  orion_amd::destroy_runner(runner);
+  orion_amd::destroy_sppm(sppm);
+}

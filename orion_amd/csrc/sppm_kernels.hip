@@ -66,7 +66,7 @@ struct Grid {
 };
 
 struct SppmBuffers {
-  DevBuf pix_mem, grid_mem, pair_mem, hist_mem, scan_mem;
+  DevBuf pix_mem, grid_mem, pair_mem, hist_mem, scan_mem, job;
   Pix px{};
   Grid g{};
   int npix = 0, hash_size = 0;
@@ -748,11 +748,19 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   w.H = s->H;
   w.k0 = 0;
   unsigned long long *dstats = (unsigned long long *)s->stats.ptr;
+  // one-entry job table for the shared traversal kernels
+  Job job1;
+  std::memset(&job1, 0, sizeof(job1));
+  job1.sc = s->d;
+  if ((rc = b.job.ensure(sizeof(Job)))) return rc;
+  HIP_TRY(hipMemcpy(b.job.ptr, &job1, sizeof(Job), hipMemcpyHostToDevice));
+  const Job *dj = (const Job *)b.job.ptr;
+  const int stack = s->d.stack_entries;
   HIP_TRY(hipMemsetAsync(s->stats.ptr, 0, S_COUNT * 8, st));
   HIP_TRY(hipEventRecord(c->ev0, st));
   const int TB = 256;
   hipLaunchKernelGGL(k_pix_init, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, b.px, npix, r0);
-  int trace_blocks = c->n_cu * 6;
+  int trace_blocks = c->n_cu * trace_blocks_per_cu(s->d.stack_entries, false);
   const float nIterF = (float)n_iters;
   const float NpF = (float)((uint64_t)n_iters * (uint64_t)photons);
   const bool want_dbg = dbg && dbg->debug_iter >= 0 && dbg->debug_iter < n_iters;
@@ -764,8 +772,8 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     hipLaunchKernelGGL(k_cam_raygen, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, s->d, cw, b.px, iter);
     int cur = 0;
     for (int it = 0; it <= maxDepth; ++it) {
-      launch_trace(s->d, cw, false, trace_blocks, st);
-      launch_trace(s->d, cw, true, trace_blocks, st);
+      launch_trace(dj, cw, false, stack, trace_blocks, false, st);
+      launch_trace(dj, cw, true, stack, trace_blocks, false, st);
       launch_between(cw, cur ^ 1, st);
       int blocks = std::min((npix + 255) / 256, 8192);
       hipLaunchKernelGGL(k_cam_shade, dim3(blocks), dim3(256), 0, st, s->d, cw, b.px, cur, maxDepth, iter);
@@ -832,7 +840,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
       HIP_TRY(hipMemsetAsync(w.ctrl + C_RAY, 0, 4, st));
       hipLaunchKernelGGL(k_photon_init, dim3((photons + TB - 1) / TB), dim3(TB), 0, st, s->d, pw, index0, photons);
       for (int depth = 0; depth < maxDepth; ++depth) {
-        launch_trace(s->d, pw, false, trace_blocks, st);
+        launch_trace(dj, pw, false, stack, trace_blocks, false, st);
         hipLaunchKernelGGL(k_photon_between, dim3(1), dim3(64), 0, st, pw, dstats);
         int blocks = std::min((photons + 255) / 256, 8192);
         hipLaunchKernelGGL(k_photon_shade, dim3(blocks), dim3(256), 0, st, s->d, pw, b.px, g, index0, depth, dstats);

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_scene.h"
+
 namespace orion_amd {
 
 // flags
@@ -21,6 +23,9 @@ enum : int {
   C_RAY = 2,                  // closest-hit queue length
   C_SHADOW = 3,               // any-hit queue length
   C_WORK_RAY = 4, C_WORK_SHADOW = 5,
+  C_NPATHS = 6,               // camera paths in the current wave (all jobs)
+  C_NPIX = 7,                 // film pixels of all jobs in the wave
+  C_NJOBS = 8,                // jobs in the current wave
   C_INTS = 16,
 };
 // 64-bit statistics
@@ -29,11 +34,29 @@ enum : int {
   S_PHOTONS = 6, S_DEPOSITS = 7, S_NODES_S = 8, S_TRIS_S = 9, S_COUNT = 12,
 };
 
+// One render job of a wave (device job table).  A wave packs the next
+// sample slabs of any number of jobs -- each with its own scene, camera,
+// film, sample range and depth -- into one set of queues; path ids
+// [path0, path0 + S * npix) belong to the job, sample s of pixel q at
+// path0 + s * npix + q.
+struct Job {
+  DScene sc;          // geometry, materials, lights; sc.cam is this job's camera
+  int W, H, npix;
+  int S;              // samples of this job in the wave
+  uint32_t k0;        // sample index of the job's first sample in the wave
+  int path0;          // first path id of the job
+  int pix0;           // first film pixel of the job (prefix over jobs)
+  int maxDepth;
+  float *film;        // W * H * 3 sums
+  int *count;         // W * H sample counts
+};
+
 struct Wave {
-  int N;          // paths in the wave
-  int npix;       // pixels per sample slab
-  uint32_t k0;    // first sample index of the wave
+  int N;          // slot stride: capacity of the per-path arrays
+  int npix;       // (SPPM) pixels of the image
+  uint32_t k0;
   int W, H;
+  int *job;       // job index per path (multi-job waves)
   // rays: slot s of path p at s * N + p; slots 0 continuation, 1 MIS, 2 shadow
   float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
   // hits for slots 0 and 1

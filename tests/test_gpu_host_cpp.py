@@ -34,5 +34,5 @@ def test_cpp_driver_renders_ori(tmp_path, oracle):
     img = read_pfm(out)
     back = oa.HostScene.load_ori(ori)
     ref, cnt, _ = oracle.scene(back).render_path(back.info.spp, back.info.max_depth)
-    ref = ref / cnt[..., None]
+    ref = ref / cnt[..., None].astype(np.float32)  # float / (float)count, as getAveraged
     np.testing.assert_array_equal(img.view(np.int32), ref.view(np.int32))

@@ -20,6 +20,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -43,6 +45,12 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU-baseline work (bounded sample)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-jobs", type=int, default=64, help="batched-launch leg (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=24, help="serving leg (0 = skip)")
+    ap.add_argument("--serve-load", type=float, default=0.9,
+                    help="offered load of the Poisson stream, as a fraction of the plain path's "
+                         "one-at-a-time capacity")
+    ap.add_argument("--hbm-budget-gib", type=float, default=16.0)
     return ap.parse_args()
 
 
@@ -77,6 +85,136 @@ def cpu_baseline(hs, depth, target_s):
     }
 
 
+def batch_leg(oa, dev, n_jobs, seed):
+    """Batched launch of n_jobs seeded procedural jobs (displaced icosphere of
+    1k-500k triangles, log-uniform; film sides uniform in [128, 1024]; spp
+    uniform in [4, 64]; depth 5-8; material drawn per mesh) against the plain
+    path rendering the same jobs one at a time.  The tuning comes from the
+    autotuner on the same mix."""
+    rng = np.random.default_rng(seed)
+    jobs, scenes = [], []
+    t0 = time.perf_counter()
+    for i in range(n_jobs):
+        tris = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+        W, H = (int(x) for x in rng.integers(128, 1025, 2))
+        spp, depth = int(rng.integers(4, 65)), int(rng.integers(5, 9))
+        ds = oa.DeviceScene(dev, oa.HostScene.generate("batch_job", seed=seed * 1000 + i,
+                                                       params=[tris, W, H, 3]))
+        scenes.append(ds)
+        jobs.append((ds, W, H, spp, depth))
+    setup_s = time.perf_counter() - t0
+    paths = sum(W * H * spp for _, W, H, spp, _ in jobs)
+    tris_tot = sum(ds.stats()["n_tris"] for ds in scenes)
+    # plain: one job at a time through orion_render_path, default waves
+    for ds, W, H, spp, depth in jobs[:2]:  # warm
+        ds.render_path(0, 1, depth)
+    plain_films = []
+    t_plain = 0.0
+    for ds, W, H, spp, depth in jobs:
+        ds.film_clear()
+        t0 = time.perf_counter()
+        ds.render_path(0, spp, depth)
+        t_plain += time.perf_counter() - t0
+        plain_films.append(ds.film()[0])
+    t0 = time.perf_counter()
+    tune = oa.autotune(dev, jobs)
+    tune_s = time.perf_counter() - t0
+    oa.render_batch(dev, jobs[:4], tuning=tune)  # warm
+    t0 = time.perf_counter()
+    outs, st = oa.render_batch(dev, jobs, tuning=tune)
+    t_batch = time.perf_counter() - t0
+    equal = sum(int(np.array_equal(a.view(np.int32), b.view(np.int32)))
+                for a, (b, _) in zip(plain_films, outs))
+    return {
+        "workload": f"{n_jobs} jobs, displaced icosphere 1k-500k tris log-uniform "
+                    f"({tris_tot} tris total), films 128..1024 per side, 4-64 spp, depth 5-8, "
+                    f"Lambert/GGX/dielectric per mesh",
+        "paths": int(paths),
+        "batched_paths_per_s": round(paths / t_batch, 1),
+        "plain_paths_per_s": round(paths / t_plain, 1),
+        "speedup": round(t_plain / t_batch, 3),
+        "batched_s": round(t_batch, 3), "plain_s": round(t_plain, 3),
+        "batched_rays_per_s": round((st.closest_rays + st.shadow_rays) / t_batch, 1),
+        "films_bitwise_equal_to_plain": f"{equal}/{n_jobs}",
+        "tuning": {"wave_paths": tune.wave_paths, "persistent_blocks": tune.persistent_blocks,
+                   "use_graph": tune.use_graph},
+        "autotune_s": round(tune_s, 2), "setup_s": round(setup_s, 2),
+    }
+
+
+def serve_leg(oa, dev, n_req, load, budget_gib, seed):
+    """Poisson stream of render requests against 4 resident scenes (~8.2M
+    triangles: displaced icospheres of 5.2M, 1.3M, 1.3M and 0.33M triangles,
+    material drawn per mesh); each request: film sides uniform in [256, 1024],
+    spp uniform in [16, 256], depth 5-8.  The plain baseline renders each
+    request alone (one at a time, first come first served); its latencies
+    follow from the measured service times on the same arrival schedule.  The
+    arrival rate is `load` x the plain path's capacity."""
+    rng = np.random.default_rng(seed)
+    scenes = []
+    for i, tris in enumerate([5.3e6, 1.4e6, 1.4e6, 3.4e5]):
+        scenes.append(oa.DeviceScene(dev, oa.HostScene.generate(
+            "batch_job", seed=seed * 100 + i, params=[tris, 512, 512, 3])))
+    tris_tot = sum(ds.stats()["n_tris"] for ds in scenes)
+    reqs = []
+    for i in range(n_req):
+        ds = scenes[int(rng.integers(0, 4))]
+        W, H = (int(x) for x in rng.integers(256, 1025, 2))
+        reqs.append((ds, W, H, int(rng.integers(16, 257)), int(rng.integers(5, 9))))
+    paths = sum(W * H * spp for _, W, H, spp, _ in reqs)
+    oa.render_batch(dev, [(scenes[0], 256, 256, 1, 5)])  # warm
+    service = []
+    for r in reqs:
+        t0 = time.perf_counter()
+        oa.render_batch(dev, [r])
+        service.append(time.perf_counter() - t0)
+    capacity = n_req / sum(service)          # requests/s, plain
+    qps = load * capacity
+    gaps = rng.exponential(1.0 / qps, n_req)
+    arrive = np.cumsum(gaps) - gaps[0]
+    # plain FIFO on the same schedule
+    free, lat_plain = 0.0, []
+    for a, sv in zip(arrive, service):
+        start = max(a, free)
+        free = start + sv
+        lat_plain.append(free - a)
+    plain_span = free - arrive[0]
+    tune = oa.autotune(dev, [(ds, W, H, min(spp, 2), d) for ds, W, H, spp, d in reqs[:8]])
+    srv = oa.Server(dev, int(budget_gib * (1 << 30)), tuning=tune)
+    t0 = time.perf_counter()
+    ids = []
+    for a, r in zip(arrive, reqs):
+        dt = t0 + a - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(srv.submit(*r))
+    srv.drain()
+    lat, done = [], []
+    for jid in ids:
+        _, _, (ts, ta, tf) = srv.poll(jid, want_film=False)
+        lat.append(tf - ts)
+        done.append(tf)
+    sst = srv.stats()
+    srv.close()
+    span = max(done) - 0.0
+    return {
+        "workload": f"{n_req} requests, Poisson at {qps:.2f}/s ({load:g} x plain capacity), "
+                    f"4 resident scenes ({tris_tot} tris), films 256..1024 per side, "
+                    f"16-256 spp, depth 5-8, HBM budget {budget_gib:g} GiB",
+        "paths": int(paths),
+        "p50_s": round(float(np.percentile(lat, 50)), 3),
+        "p99_s": round(float(np.percentile(lat, 99)), 3),
+        "plain_p50_s": round(float(np.percentile(lat_plain, 50)), 3),
+        "plain_p99_s": round(float(np.percentile(lat_plain, 99)), 3),
+        "served_paths_per_s_busy": round(paths / (sst["busy_ms"] / 1e3), 1),
+        "plain_paths_per_s": round(paths / sum(service), 1),
+        "speedup_busy": round(sum(service) / (sst["busy_ms"] / 1e3), 3),
+        "served_span_s": round(span, 3), "plain_span_s": round(plain_span, 3),
+        "waves": sst["waves"], "mixed_waves": sst["mixed_waves"],
+        "max_jobs_in_wave": sst["max_jobs_in_wave"],
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -106,6 +244,7 @@ def main():
         tot = d if tot is None else {key: tot[key] + d[key] for key in d}
     dev.synchronize()
     el = time.perf_counter() - t0
+    print(f"# main leg done: {a.steps} steps in {el:.2f} s", file=sys.stderr, flush=True)
     paths = tot["paths"]
     value = paths / el
     rays = tot["closest_rays"] + tot["shadow_rays"]
@@ -145,6 +284,12 @@ def main():
         "roofline": roofline,
         "scene_gen_s": round(gen_s, 2), "bvh_build_ms": round(sst["build_ms"], 1),
     }
+    if a.batch_jobs > 0:
+        out["batched"] = batch_leg(oa, dev, a.batch_jobs, a.seed)
+        print(f"# batched leg done: {out['batched']['speedup']}x", file=sys.stderr, flush=True)
+    if a.serve_requests > 0:
+        out["serving"] = serve_leg(oa, dev, a.serve_requests, a.serve_load, a.hbm_budget_gib, a.seed)
+        print("# serving leg done", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(hs, a.depth, a.cpu_seconds)
     print(json.dumps(out))

@@ -150,8 +150,8 @@ typedef struct {
 } orion_render_stats;
 
 typedef struct {
-  int spp_per_wave;     /* samples per pixel processed per wavefront (0 = auto) */
-  int block;            /* threads per block for traversal (0 = auto) */
+  int spp_per_wave;     /* plain renders: samples per pixel per wavefront (0 = from wave_paths) */
+  int wave_paths;       /* paths per wavefront (0 = auto, 2^21); batched and served waves */
   int persistent_blocks;/* blocks per CU for persistent traversal (0 = auto) */
   int use_graph;        /* capture the bounce loop into a hipGraph */
   int profile;          /* time every traversal / shade launch with HIP events */
@@ -226,10 +226,12 @@ int orion_server_get_stats(orion_server s, orion_server_stats *st);
 int orion_server_destroy(orion_server s);
 
 /* ---------------------------------------------------------------- autotuner
- * Times short renders of `scene` over candidate wave sizes, persistent
- * traversal blocks per CU and graph capture, and returns the fastest
- * setting (also cached per scene for later calls with tune == NULL). */
-int orion_autotune(orion_scene s, int max_depth, orion_tuning *best);
+ * No counterpart in orion.  Times short batched renders of `jobs` (each
+ * cut to its first two samples per pixel) over candidate wave sizes,
+ * persistent traversal blocks per CU and graph capture, and returns the
+ * fastest setting for orion_render_batch / orion_server_create; with one
+ * job whose film is its scene's, also for orion_render_path. */
+int orion_autotune(orion_ctx c, int n_jobs, const orion_job_desc *jobs, orion_tuning *best);
 
 #ifdef __cplusplus
 }

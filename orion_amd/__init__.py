@@ -51,7 +51,7 @@ class RenderStats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("spp_per_wave", C.c_int), ("block", C.c_int), ("persistent_blocks", C.c_int),
+    _fields_ = [("spp_per_wave", C.c_int), ("wave_paths", C.c_int), ("persistent_blocks", C.c_int),
                 ("use_graph", C.c_int), ("profile", C.c_int)]
 
 
@@ -116,7 +116,7 @@ _SIGS = {
     "orion_server_drain": (C.c_int, [C.c_void_p]),
     "orion_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "orion_server_destroy": (C.c_int, [C.c_void_p]),
-    "orion_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(Tuning)]),
+    "orion_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(JobDesc), C.POINTER(Tuning)]),
 }
 
 _lib = None
@@ -315,18 +315,17 @@ class DeviceScene:
         return occ
 
     def render_path(self, k0, k1, max_depth, spp_per_wave=0, persistent_blocks=0, profile=False,
-                    use_graph=False):
+                    use_graph=False, wave_paths=0, tuning=None):
+        if tuning is not None:
+            spp_per_wave, wave_paths = tuning.spp_per_wave, tuning.wave_paths
+            persistent_blocks, use_graph = tuning.persistent_blocks, bool(tuning.use_graph)
         st = RenderStats()
-        tune = None  # NULL: the scene's autotuned setting, if any, else the defaults
-        if spp_per_wave or persistent_blocks or use_graph or profile:
-            tune = C.byref(Tuning(spp_per_wave, 0, persistent_blocks, int(use_graph), int(profile)))
+        tune = None  # NULL: the defaults
+        if spp_per_wave or wave_paths or persistent_blocks or use_graph or profile:
+            tune = C.byref(Tuning(spp_per_wave, wave_paths, persistent_blocks, int(use_graph),
+                                  int(profile)))
         _check(lib().orion_render_path(self.h, k0, k1, max_depth, tune, C.byref(st)))
         return st
-
-    def autotune(self, max_depth):
-        best = Tuning()
-        _check(lib().orion_autotune(self.h, max_depth, C.byref(best)))
-        return best
 
     def film_clear(self):
         _check(lib().orion_film_clear(self.h))
@@ -367,27 +366,40 @@ class DeviceScene:
             pass
 
 
-def _tuning(spp_per_wave=0, persistent_blocks=0, use_graph=False, profile=False):
-    return Tuning(spp_per_wave, 0, persistent_blocks, int(use_graph), int(profile))
+def _descs(jobs):
+    descs = (JobDesc * max(len(jobs), 1))()
+    for i, (sc, W, H, spp, depth) in enumerate(jobs):
+        descs[i] = JobDesc(sc.h.value, W, H, spp, depth)
+    return descs
 
 
-def render_batch(dev, jobs, spp_per_wave=0, persistent_blocks=0, use_graph=False, profile=False):
+def autotune(dev, jobs):
+    """Fastest Tuning for the job mix [(DeviceScene, width, height, spp, max_depth), ...]."""
+    best = Tuning()
+    _check(lib().orion_autotune(dev.h, len(jobs), _descs(jobs), C.byref(best)))
+    return best
+
+
+def render_batch(dev, jobs, wave_paths=0, persistent_blocks=0, use_graph=False, profile=False,
+                 tuning=None):
     """Renders jobs = [(DeviceScene, width, height, spp, max_depth), ...] in shared
     wavefronts; returns ([(rgb HxWx3, count HxW), ...], RenderStats)."""
+    if tuning is not None:
+        wave_paths, persistent_blocks = tuning.wave_paths, tuning.persistent_blocks
+        use_graph = bool(tuning.use_graph)
     n = len(jobs)
-    descs = (JobDesc * max(n, 1))()
+    descs = _descs(jobs)
     outs = []
     rgb_p = (C.c_void_p * max(n, 1))()
     cnt_p = (C.c_void_p * max(n, 1))()
     for i, (sc, W, H, spp, depth) in enumerate(jobs):
-        descs[i] = JobDesc(sc.h.value, W, H, spp, depth)
         rgb = np.zeros((H, W, 3), np.float32)
         cnt = np.zeros((H, W), np.int32)
         outs.append((rgb, cnt))
         rgb_p[i] = rgb.ctypes.data
         cnt_p[i] = cnt.ctypes.data
     st = RenderStats()
-    tune = _tuning(spp_per_wave, persistent_blocks, use_graph, profile)
+    tune = Tuning(0, wave_paths, persistent_blocks, int(use_graph), int(profile))
     _check(lib().orion_render_batch(dev.h, n, descs, rgb_p, cnt_p, C.byref(tune), C.byref(st)))
     return outs, st
 
@@ -396,11 +408,15 @@ class Server:
     """Request loop over one device: submit jobs, poll or drain, read films and
     per-request times (submitted, admitted, finished; seconds since start)."""
 
-    def __init__(self, dev, hbm_budget_bytes, spp_per_wave=0, use_graph=False):
+    def __init__(self, dev, hbm_budget_bytes, wave_paths=0, use_graph=False, persistent_blocks=0,
+                 tuning=None):
+        if tuning is not None:
+            wave_paths, persistent_blocks = tuning.wave_paths, tuning.persistent_blocks
+            use_graph = bool(tuning.use_graph)
         self.dev = dev
         self.h = C.c_void_p()
         self._shape = {}
-        tune = _tuning(spp_per_wave, 0, use_graph)
+        tune = Tuning(0, wave_paths, persistent_blocks, int(use_graph), 0)
         _check(lib().orion_server_create(dev.h, int(hbm_budget_bytes), C.byref(tune), C.byref(self.h)))
 
     def submit(self, scene, width, height, spp, max_depth):

@@ -231,10 +231,6 @@ static Job make_job(orion_scene s, int W, int H, int maxDepth, float *film, int 
   return J;
 }
 
-static const orion_tuning *effective_tuning(orion_scene s, const orion_tuning *t) {
-  if (t) return t;
-  return s->has_tuned ? &s->tuned : nullptr;
-}
 
 }  // namespace orion_amd
 
@@ -253,10 +249,10 @@ extern "C" int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, co
   if (maxDepth > 60) { set_error("maxDepth above 60 exceeds the Sobol dimension budget"); return ORION_ERR_UNSUPPORTED; }
   orion_ctx c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
-  tune = effective_tuning(s, tune);
   if (!s->runner) s->runner = new WaveRunner();
   WaveRunner &R = *s->runner;
-  int S = tune && tune->spp_per_wave > 0 ? tune->spp_per_wave : std::max(1, kDefaultWavePaths / npix);
+  const int wave_paths = tune && tune->wave_paths > 0 ? tune->wave_paths : kDefaultWavePaths;
+  int S = tune && tune->spp_per_wave > 0 ? tune->spp_per_wave : std::max(1, wave_paths / npix);
   S = std::min(S, std::max(1, k1 - k0));
   int rc;
   if ((rc = R.init(1))) return rc;
@@ -407,7 +403,7 @@ extern "C" int orion_render_batch(orion_ctx c, int n, const orion_job_desc *desc
     HIP_TRY(hipMemsetAsync(js[j].film.ptr, 0, np * 12, c->stream));
     HIP_TRY(hipMemsetAsync(js[j].count.ptr, 0, np * 4, c->stream));
   }
-  int target = tune && tune->spp_per_wave > 0 ? tune->spp_per_wave * max_np : kDefaultWavePaths;
+  int target = tune && tune->wave_paths > 0 ? tune->wave_paths : kDefaultWavePaths;
   int rc;
   if ((rc = R.init(std::max(n, 1)))) return rc;
   if ((rc = R.reset_stats(c->stream))) return rc;
@@ -490,8 +486,7 @@ void orion_server_s::loop() {
   using namespace orion_amd;
   (void)hipSetDevice(ctx->device);
   const orion_tuning *tn = has_tune ? &tune : nullptr;
-  // wave size in paths: spp_per_wave x 2^18 (a 512x512 film's worth)
-  int target = tn && tn->spp_per_wave > 0 ? tn->spp_per_wave * (1 << 18) : kDefaultWavePaths;
+  int target = tn && tn->wave_paths > 0 ? tn->wave_paths : kDefaultWavePaths;
   int rr = 0;
   for (;;) {
     std::vector<ServerJob *> admitted;
@@ -688,37 +683,39 @@ int orion_server_destroy(orion_server s) {
 }
 
 // ======================================================================= autotuner
-int orion_autotune(orion_scene s, int maxDepth, orion_tuning *best) {
-  if (!s || !best) { set_error("bad argument"); return ORION_ERR_ARG; }
-  const int npix = s->W * s->H;
-  const int waves[] = {1 << 19, 1 << 20, 1 << 21, 1 << 22};
-  const int bpcs[] = {0, 4, 8, 16};
-  double best_rate = 0;
+int orion_autotune(orion_ctx c, int n, const orion_job_desc *jobs, orion_tuning *best) {
+  if (!c || n <= 0 || !jobs || !best) { set_error("bad argument"); return ORION_ERR_ARG; }
+  std::vector<orion_job_desc> trial(jobs, jobs + n);
+  std::vector<std::vector<float>> rgb(n);
+  std::vector<std::vector<int>> cnt(n);
+  std::vector<float *> rp(n);
+  std::vector<int *> cp(n);
+  for (int j = 0; j < n; ++j) {
+    trial[j].spp = std::min(trial[j].spp, 2);
+    size_t np = (size_t)trial[j].width * trial[j].height;
+    rgb[j].resize(np * 3);
+    cnt[j].resize(np);
+    rp[j] = rgb[j].data();
+    cp[j] = cnt[j].data();
+  }
+  const int wave_sizes[] = {1 << 20, 1 << 21, 1 << 22, 1 << 23};
+  const int bpcs[] = {0, 2, 4};
+  double best_ms = 1e30;
   orion_tuning b{};
-  for (int wpaths : waves) {
-    int S = std::max(1, wpaths / npix);
-    if ((uint64_t)2 * S * (uint64_t)npix > 0xffffffffull) continue;
-    for (int bpc : bpcs) {
+  for (int wp : wave_sizes)
+    for (int bpc : bpcs)
       for (int graph = 0; graph < 2; ++graph) {
         orion_tuning t{};
-        t.spp_per_wave = S;
+        t.wave_paths = wp;
         t.persistent_blocks = bpc;
         t.use_graph = graph;
         orion_render_stats st{};
-        // warm (graph capture / allocation), then time one wave
-        int rc = orion_render_path(s, 0, S, maxDepth, &t, &st);
+        // the first call captures graphs and sizes buffers; time the second
+        int rc = orion_render_batch(c, n, trial.data(), rp.data(), cp.data(), &t, &st);
+        if (!rc) rc = orion_render_batch(c, n, trial.data(), rp.data(), cp.data(), &t, &st);
         if (rc) return rc;
-        rc = orion_render_path(s, S, 2 * S, maxDepth, &t, &st);
-        if (rc) return rc;
-        double rate = st.ms > 0 ? st.paths / st.ms : 0;
-        if (rate > best_rate) { best_rate = rate; b = t; }
+        if (st.ms < best_ms) { best_ms = st.ms; b = t; }
       }
-    }
-  }
-  int rc = orion_film_clear(s);
-  if (rc) return rc;
-  s->tuned = b;
-  s->has_tuned = true;
   *best = b;
   return ORION_OK;
 }

@@ -57,8 +57,6 @@ struct orion_scene_s {
   orion_amd::WaveBuffers wave;
   orion_amd::SppmBuffers *sppm = nullptr;
   orion_amd::WaveRunner *runner = nullptr;  // plain path renders
-  orion_tuning tuned{};
-  bool has_tuned = false;
   double build_ms = 0, upload_ms = 0;
   ~orion_scene_s();
 };

@@ -39,8 +39,8 @@ def job_list(sc):
 def test_batch_alone_equals_mix(device, scenes):
     jobs = job_list(scenes)
     alone = [oa.render_batch(device, [j])[0][0] for j in jobs]
-    for spw in (0, 1, 3):  # default waves, one sample per job per wave, three
-        mixed, st = oa.render_batch(device, jobs, spp_per_wave=spw)
+    for wp in (0, 1, 30000):  # default waves, one sample per job per wave, a few
+        mixed, st = oa.render_batch(device, jobs, wave_paths=wp)
         assert st.paths == sum(W * H * spp for _, W, H, spp, _ in jobs)
         for (rgb, cnt), (argb, acnt), (_, W, H, spp, _) in zip(mixed, alone, jobs):
             assert (cnt == spp).all()
@@ -64,7 +64,7 @@ def test_batch_job_matches_oracle(device, scenes, oracle):
     from test_gpu_parity import image_compare, MEAN_TOL, PIX_FRAC
     hs = oa.HostScene.generate("batch_job", seed=13, params=[1500, 72, 40, 0])
     ref, rcnt, _ = oracle.scene(hs).render_path(7, 6)
-    mixed, _ = oa.render_batch(device, job_list(scenes), spp_per_wave=1)
+    mixed, _ = oa.render_batch(device, job_list(scenes), wave_paths=1)
     rgb, cnt = mixed[4]
     np.testing.assert_array_equal(cnt, rcnt)
     frac, mean_err, _ = image_compare(rgb, ref)
@@ -88,9 +88,9 @@ def test_graph_replay_equals_eager(device, scenes):
         assert (cnt == k + 2).all()
         scenes["lambert"].render_path(0, 1, 6)  # eager work on the same stream between replays
     jobs = job_list(scenes)
-    a, _ = oa.render_batch(device, jobs, spp_per_wave=1)
+    a, _ = oa.render_batch(device, jobs, wave_paths=1)
     for _ in range(3):
-        g, _ = oa.render_batch(device, jobs, spp_per_wave=1, use_graph=True)
+        g, _ = oa.render_batch(device, jobs, wave_paths=1, use_graph=True)
         for (x, _), (y, _) in zip(a, g):
             np.testing.assert_array_equal(bits(x), bits(y))
 
@@ -102,7 +102,7 @@ def test_server_results_equal_alone(device, scenes, budget):
     film_bytes = [W * H * 16 for _, W, H, _, _ in jobs]
     # "one_job": no two films fit, so jobs run one after another
     nbytes = 100 * max(film_bytes) if budget == "wide" else min(film_bytes)
-    srv = oa.Server(device, nbytes, spp_per_wave=1)
+    srv = oa.Server(device, nbytes, wave_paths=1)
     ids = [srv.submit(*j) for j in jobs]
     srv.drain()
     st = srv.stats()
@@ -121,10 +121,10 @@ def test_server_results_equal_alone(device, scenes, budget):
 def test_server_continuous_batching(device, scenes):
     """a request that arrives while another runs joins its waves"""
     sc = scenes["gold"]
-    big = (sc, 512, 512, 64, 5)  # 64 waves of 2^18 paths
+    big = (sc, 512, 512, 64, 5)  # 64 waves of one sample
     small = (scenes["cornell"], 48, 48, 4, 5)
     alone_small = oa.render_batch(device, [small])[0][0]
-    srv = oa.Server(device, 1 << 30, spp_per_wave=1)
+    srv = oa.Server(device, 1 << 30, wave_paths=1 << 18)
     a = srv.submit(*big)
     t0 = time.time()
     while srv.stats()["waves"] < 2 and time.time() - t0 < 30:
@@ -141,12 +141,17 @@ def test_server_continuous_batching(device, scenes):
 
 
 def test_autotune(device, scenes):
+    jobs = job_list(scenes)
+    best = oa.autotune(device, jobs)
+    assert best.wave_paths >= 1 << 20 and best.persistent_blocks >= 0
+    a, _ = oa.render_batch(device, jobs)
+    t, _ = oa.render_batch(device, jobs, tuning=best)
+    for (x, _), (y, _) in zip(a, t):
+        np.testing.assert_array_equal(bits(x), bits(y))
     sc = scenes["glass"]
-    best = sc.autotune(8)
-    assert best.spp_per_wave >= 1
     sc.film_clear()
     sc.render_path(0, 4, 8)
-    tuned = sc.film()[0].copy()
+    plain = sc.film()[0].copy()
     sc.film_clear()
-    sc.render_path(0, 4, 8, spp_per_wave=1)
-    np.testing.assert_array_equal(bits(tuned), bits(sc.film()[0]))
+    sc.render_path(0, 4, 8, tuning=best)
+    np.testing.assert_array_equal(bits(plain), bits(sc.film()[0]))

@@ -4,8 +4,8 @@
 Workload (one GPU): level-8 icosphere (1,310,720 triangles + ground + light),
 gold GGX per region, quad area light + point light, 1024 x 1024, depth 8, RR
 after bounce 3, Sobol.  A step is one wavefront pass of `--spp-per-step`
-samples per pixel over the whole image (2 spp = 2,097,152 camera paths); the
-default 32 steps render the configuration's full 64 spp.  The scene (BVH,
+samples per pixel over the whole image (32 spp = 33,554,432 camera paths in
+flight); the default 2 steps render the configuration's full 64 spp.  The scene (BVH,
 triangles, shading data, Sobol table) is resident in HBM before timing.
 
 Prints one JSON line: paths/s as `value`, rays/s beside it, the roofline of
@@ -27,7 +27,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "paths/s + rays/s at 1.3M tris; served-render p50/p99 latency vs plain GPU"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 64      # BVH2 node: both children's boxes + child links
+NODE_BYTES = 64      # 4-wide node: origin, grid steps, 4 child links, 8-bit child boxes
 TRI_BYTES = 48       # p0, e1, e2 + primitive id
 RAY_IO_BYTES = 48    # 32 B ray in (o, d, tmax, id) + 16 B hit out (t, prim, u, v)
 
@@ -35,9 +35,9 @@ RAY_IO_BYTES = 48    # 32 B ray in (o, d, tmax, id) + 16 B hit out (t, prim, u, 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=32)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--spp-per-step", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--spp-per-step", type=int, default=32)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--depth", type=int, default=8)
@@ -51,6 +51,9 @@ def parse():
                     help="offered load of the Poisson stream, as a fraction of the plain path's "
                          "one-at-a-time capacity")
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this size (a known-bytes "
+                         "kernel to scale the FETCH_SIZE / WRITE_SIZE counters by)")
     return ap.parse_args()
 
 
@@ -136,8 +139,8 @@ def batch_leg(oa, dev, n_jobs, seed):
         "batched_s": round(t_batch, 3), "plain_s": round(t_plain, 3),
         "batched_rays_per_s": round((st.closest_rays + st.shadow_rays) / t_batch, 1),
         "films_bitwise_equal_to_plain": f"{equal}/{n_jobs}",
-        "tuning": {"wave_paths": tune.wave_paths, "persistent_blocks": tune.persistent_blocks,
-                   "use_graph": tune.use_graph},
+        "tuning": {"wave_paths": tune.wave_paths, "rays_per_wave": tune.rays_per_wave,
+                   "lds_ring": tune.lds_ring, "use_graph": tune.use_graph},
         "autotune_s": round(tune_s, 2), "setup_s": round(setup_s, 2),
     }
 
@@ -223,6 +226,14 @@ def main():
               f"film reduction; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
     import orion_amd as oa
+    if a.copy_calibration_mib > 0:
+        import torch
+        src = torch.ones(a.copy_calibration_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     t0 = time.perf_counter()
     hs = oa.HostScene.generate("icosphere", seed=a.seed, params=[a.level, a.res, a.res])
@@ -260,10 +271,12 @@ def main():
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-        "kernel": "k_trace<closest> (BVH2 traversal, LDS stack)",
+        "kernel": "k_trace<closest> (4-wide quantized BVH, LDS ring stack)",
         "bytes_per_ray": round(bytes_alg / max(tot["closest_rays"], 1), 1),
         "nodes_per_ray": round(tot["node_visits_closest"] / max(tot["closest_rays"], 1), 2),
         "tris_per_ray": round(tot["tri_tests_closest"] / max(tot["closest_rays"], 1), 2),
+        "shadow_nodes_per_ray": round(tot["node_visits_shadow"] / max(tot["shadow_rays"], 1), 2),
+        "shadow_tris_per_ray": round(tot["tri_tests_shadow"] / max(tot["shadow_rays"], 1), 2),
         "kernel_ms": {k: round(v, 2) for k, v in kern.items()},
     }
     out = {
@@ -277,6 +290,8 @@ def main():
                         f"{a.res}x{a.res}, {S} spp per step, depth {a.depth}, RR after bounce 3",
             "triangles": hs.n_tris, "width": a.res, "height": a.res, "spp": S * a.steps,
             "max_depth": a.depth, "bvh_nodes": sst["n_nodes"], "bvh_depth": sst["bvh_depth"],
+            "stack_bound": sst["stack_bound"],
+            "trace_waves_per_simd": sst["trace_waves_per_simd"],
         },
         "rays_per_s": round(rays / el, 1),
         "ref_equiv_rays_per_s": round(tot["ref_equiv_rays"] / el, 1),

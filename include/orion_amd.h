@@ -95,6 +95,11 @@ int orion_hscene_records(orion_hscene s, const float **tri_pn, const int **tri_m
 /* Writes an .ori (+ .obj for meshes above mesh_inline_limit triangles). */
 int orion_hscene_write_ori(orion_hscene s, const char *path, int mesh_inline_limit);
 void orion_hscene_destroy(orion_hscene s);
+/* No counterpart in orion (diagnostic): builds the BVH that orion_scene_upload
+ * would build and checks it structurally on the host -- each node reached once,
+ * each triangle in exactly one leaf, every decoded (quantized) child box
+ * containing its subtree's vertices.  Fills n_nodes / depth / stack_bound. */
+int orion_hscene_bvh_check(orion_hscene s, int64_t *n_nodes, int *depth, int *stack_bound);
 
 /* Resolved camera record from a lookat + fov (transform.cpp:197-217,
  * pinhole.cpp:35-48); exposed for the loader's parity tests. */
@@ -120,6 +125,8 @@ typedef struct {
   int64_t n_nodes, n_tris, node_bytes, tri_bytes, shade_bytes;
   int bvh_depth, node_format;
   double build_ms, upload_ms;
+  int stack_bound;          /* traversal stack entries that can be live at once */
+  int trace_waves_per_simd; /* occupancy of the closest-hit traversal (default LDS ring) */
 } orion_scene_stats;
 int orion_scene_get_stats(orion_scene s, orion_scene_stats *st);
 
@@ -147,14 +154,16 @@ typedef struct {
    * events recorded on the stream the kernels run on) */
   double trace_closest_ms, trace_shadow_ms, shade_ms;
   int64_t trace_closest_launches, trace_shadow_launches, shade_launches;
+  int64_t stack_overflows;  /* traversal stack entries dropped (0 for a well-formed BVH) */
 } orion_render_stats;
 
 typedef struct {
   int spp_per_wave;     /* plain renders: samples per pixel per wavefront (0 = from wave_paths) */
   int wave_paths;       /* paths per wavefront (0 = auto, 2^21); batched and served waves */
-  int persistent_blocks;/* blocks per CU for persistent traversal (0 = auto) */
+  int rays_per_wave;    /* traversal queue entries per wave64 (0 = 256) */
   int use_graph;        /* capture the bounce loop into a hipGraph */
   int profile;          /* time every traversal / shade launch with HIP events */
+  int lds_ring;         /* traversal stack entries per lane kept in LDS: 8, 16, 32 (0 = 16) */
 } orion_tuning;
 
 /* PathTracing::render over samples [k0, k1): accumulates into the scene's film

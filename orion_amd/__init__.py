@@ -33,7 +33,8 @@ class SceneInfo(C.Structure):
 class SceneStats(C.Structure):
     _fields_ = [("n_nodes", C.c_int64), ("n_tris", C.c_int64), ("node_bytes", C.c_int64),
                 ("tri_bytes", C.c_int64), ("shade_bytes", C.c_int64), ("bvh_depth", C.c_int),
-                ("node_format", C.c_int), ("build_ms", C.c_double), ("upload_ms", C.c_double)]
+                ("node_format", C.c_int), ("build_ms", C.c_double), ("upload_ms", C.c_double),
+                ("stack_bound", C.c_int), ("trace_waves_per_simd", C.c_int)]
 
 
 class RenderStats(C.Structure):
@@ -44,15 +45,15 @@ class RenderStats(C.Structure):
                 ("tri_tests_shadow", C.c_int64), ("trace_closest_ms", C.c_double),
                 ("trace_shadow_ms", C.c_double), ("shade_ms", C.c_double),
                 ("trace_closest_launches", C.c_int64), ("trace_shadow_launches", C.c_int64),
-                ("shade_launches", C.c_int64)]
+                ("shade_launches", C.c_int64), ("stack_overflows", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Tuning(C.Structure):
-    _fields_ = [("spp_per_wave", C.c_int), ("wave_paths", C.c_int), ("persistent_blocks", C.c_int),
-                ("use_graph", C.c_int), ("profile", C.c_int)]
+    _fields_ = [("spp_per_wave", C.c_int), ("wave_paths", C.c_int), ("rays_per_wave", C.c_int),
+                ("use_graph", C.c_int), ("profile", C.c_int), ("lds_ring", C.c_int)]
 
 
 class SppmDebug(C.Structure):
@@ -86,6 +87,8 @@ _SIGS = {
     "orion_hscene_records": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_void_p)] * 6),
     "orion_hscene_write_ori": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "orion_hscene_destroy": (None, [C.c_void_p]),
+    "orion_hscene_bvh_check": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]),
     "orion_camera_record": (C.c_int, [f32p, C.c_float, C.c_int, C.c_int, f32p]),
     "orion_create_transform": (C.c_int, [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p),
                                          f32p, f32p]),
@@ -227,6 +230,13 @@ class HostScene:
     def height(self):
         return self.info.height
 
+    def bvh_check(self):
+        """Builds and structurally checks the device BVH on the host; returns
+        (n_nodes, depth, stack_bound) or raises OrionError."""
+        nn, d, sb = C.c_int64(), C.c_int(), C.c_int()
+        _check(lib().orion_hscene_bvh_check(self.h, C.byref(nn), C.byref(d), C.byref(sb)))
+        return nn.value, d.value, sb.value
+
     def write_ori(self, path, inline_limit=1000):
         _check(lib().orion_hscene_write_ori(self.h, str(path).encode(), inline_limit))
 
@@ -314,16 +324,17 @@ class DeviceScene:
         _check(lib().orion_intersect_any(self.h, n, _ptr(rays), _ptr(occ), 0))
         return occ
 
-    def render_path(self, k0, k1, max_depth, spp_per_wave=0, persistent_blocks=0, profile=False,
-                    use_graph=False, wave_paths=0, tuning=None):
+    def render_path(self, k0, k1, max_depth, spp_per_wave=0, rays_per_wave=0, profile=False,
+                    use_graph=False, wave_paths=0, lds_ring=0, tuning=None):
         if tuning is not None:
             spp_per_wave, wave_paths = tuning.spp_per_wave, tuning.wave_paths
-            persistent_blocks, use_graph = tuning.persistent_blocks, bool(tuning.use_graph)
+            rays_per_wave, use_graph = tuning.rays_per_wave, bool(tuning.use_graph)
+            lds_ring = tuning.lds_ring
         st = RenderStats()
         tune = None  # NULL: the defaults
-        if spp_per_wave or wave_paths or persistent_blocks or use_graph or profile:
-            tune = C.byref(Tuning(spp_per_wave, wave_paths, persistent_blocks, int(use_graph),
-                                  int(profile)))
+        if spp_per_wave or wave_paths or rays_per_wave or use_graph or profile or lds_ring:
+            tune = C.byref(Tuning(spp_per_wave, wave_paths, rays_per_wave, int(use_graph),
+                                  int(profile), lds_ring))
         _check(lib().orion_render_path(self.h, k0, k1, max_depth, tune, C.byref(st)))
         return st
 
@@ -380,13 +391,13 @@ def autotune(dev, jobs):
     return best
 
 
-def render_batch(dev, jobs, wave_paths=0, persistent_blocks=0, use_graph=False, profile=False,
-                 tuning=None):
+def render_batch(dev, jobs, wave_paths=0, rays_per_wave=0, use_graph=False, profile=False,
+                 lds_ring=0, tuning=None):
     """Renders jobs = [(DeviceScene, width, height, spp, max_depth), ...] in shared
     wavefronts; returns ([(rgb HxWx3, count HxW), ...], RenderStats)."""
     if tuning is not None:
-        wave_paths, persistent_blocks = tuning.wave_paths, tuning.persistent_blocks
-        use_graph = bool(tuning.use_graph)
+        wave_paths, rays_per_wave = tuning.wave_paths, tuning.rays_per_wave
+        use_graph, lds_ring = bool(tuning.use_graph), tuning.lds_ring
     n = len(jobs)
     descs = _descs(jobs)
     outs = []
@@ -399,7 +410,7 @@ def render_batch(dev, jobs, wave_paths=0, persistent_blocks=0, use_graph=False, 
         rgb_p[i] = rgb.ctypes.data
         cnt_p[i] = cnt.ctypes.data
     st = RenderStats()
-    tune = Tuning(0, wave_paths, persistent_blocks, int(use_graph), int(profile))
+    tune = Tuning(0, wave_paths, rays_per_wave, int(use_graph), int(profile), lds_ring)
     _check(lib().orion_render_batch(dev.h, n, descs, rgb_p, cnt_p, C.byref(tune), C.byref(st)))
     return outs, st
 
@@ -408,15 +419,15 @@ class Server:
     """Request loop over one device: submit jobs, poll or drain, read films and
     per-request times (submitted, admitted, finished; seconds since start)."""
 
-    def __init__(self, dev, hbm_budget_bytes, wave_paths=0, use_graph=False, persistent_blocks=0,
-                 tuning=None):
+    def __init__(self, dev, hbm_budget_bytes, wave_paths=0, use_graph=False, rays_per_wave=0,
+                 lds_ring=0, tuning=None):
         if tuning is not None:
-            wave_paths, persistent_blocks = tuning.wave_paths, tuning.persistent_blocks
-            use_graph = bool(tuning.use_graph)
+            wave_paths, rays_per_wave = tuning.wave_paths, tuning.rays_per_wave
+            use_graph, lds_ring = bool(tuning.use_graph), tuning.lds_ring
         self.dev = dev
         self.h = C.c_void_p()
         self._shape = {}
-        tune = Tuning(0, wave_paths, persistent_blocks, int(use_graph), 0)
+        tune = Tuning(0, wave_paths, rays_per_wave, int(use_graph), 0, lds_ring)
         _check(lib().orion_server_create(dev.h, int(hbm_budget_bytes), C.byref(tune), C.byref(self.h)))
 
     def submit(self, scene, width, height, spp, max_depth):

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "../../include/orion_amd.h"
+#include "bvh.h"
 #include "host_math.h"
 #include "host_scene.h"
 
@@ -88,6 +89,20 @@ int orion_hscene_write_ori(orion_hscene h, const char *path, int inline_limit) {
 }
 
 void orion_hscene_destroy(orion_hscene h) { delete reinterpret_cast<HostScene *>(h); }
+
+int orion_hscene_bvh_check(orion_hscene h, int64_t *n_nodes, int *depth, int *stack_bound) {
+  if (!h) { set_error("null scene"); return ORION_ERR_ARG; }
+  const HostScene &s = *reinterpret_cast<HostScene *>(h);
+  int n = (int)s.tri_mat.size();
+  Bvh b;
+  build_bvh(s.tri_pn.data(), n, &b);
+  std::string err = check_bvh(b, s.tri_pn.data(), n);
+  if (n_nodes) *n_nodes = (int64_t)b.nodes.size();
+  if (depth) *depth = b.depth;
+  if (stack_bound) *stack_bound = b.stack_bound;
+  if (!err.empty()) { set_error("BVH check: " + err); return ORION_ERR_ARG; }
+  return ORION_OK;
+}
 
 int orion_camera_record(const float lookat[9], float fov, int W, int H, float rec[16]) {
   if (!lookat || !rec || W <= 0 || H <= 0) { set_error("bad argument"); return ORION_ERR_ARG; }

@@ -36,7 +36,11 @@ namespace orion_amd {
     }                                                                         \
   } while (0)
 
-constexpr int kDefaultWavePaths = 1 << 21;
+// Paths per wave: the traversal launches are latency-bound (a launch lasts
+// at least as long as its slowest rays), so the wave is made large enough
+// that each bounce's launch is dominated by throughput (32 samples per pixel
+// of a 1024x1024 film; ~240 B of wave state per path = 8 GB).
+constexpr int kDefaultWavePaths = 1 << 25;
 
 // Per-launch HIP events recorded on the launch stream (kernel timing).
 struct LaunchTimer {
@@ -70,17 +74,18 @@ struct LaunchTimer {
 };
 
 struct GraphKey {
-  int depth, stack, multi, blocks, cap;
-  void *mem;
+  int depth, ring, multi, chunk, cap, spill_cap, seg, nseg;
+  void *mem, *spill;
   bool operator==(const GraphKey &o) const {
-    return depth == o.depth && stack == o.stack && multi == o.multi && blocks == o.blocks && cap == o.cap &&
-           mem == o.mem;
+    return depth == o.depth && ring == o.ring && multi == o.multi && chunk == o.chunk && cap == o.cap &&
+           spill_cap == o.spill_cap && seg == o.seg && nseg == o.nseg && mem == o.mem && spill == o.spill;
   }
 };
 
 struct WaveRunner {
   WaveBuffers wb;
-  DevBuf jobs_dev, ctrl, stats;
+  DevBuf jobs_dev, ctrl, stats, spill;
+  int spill_cap = 0;
   Job *h_jobs = nullptr;
   int *h_ctrl = nullptr;
   int max_jobs = 0;
@@ -102,7 +107,7 @@ struct WaveRunner {
     int rc = jobs_dev.ensure(sizeof(Job) * cap);
     if (rc) return rc;
     if ((rc = ctrl.ensure(64 * 4))) return rc;
-    if ((rc = stats.ensure(S_COUNT * 8))) return rc;
+    if ((rc = stats.ensure(kStatBytes))) return rc;
     HIP_TRY(hipMemset(ctrl.ptr, 0, 64 * 4));
     if (!staged) HIP_TRY(hipEventCreateWithFlags(&staged, hipEventDisableTiming));
     max_jobs = cap;
@@ -118,19 +123,33 @@ struct WaveRunner {
     drop_graphs();
     return wb.ensure(paths);
   }
+  // traversal stack spill area for a wave of nseg segments of seg paths
+  int ensure_spill(int stack_bound, int ring, int chunk, int seg, int nseg) {
+    size_t need = trace_spill_ints(stack_bound, ring, trace_grid(nseg, seg, false, chunk)) * 4 + 4;
+    if (need > spill.bytes) {
+      drop_graphs();
+      int rc = spill.ensure(need);
+      if (rc) return rc;
+    }
+    spill_cap = std::max(0, stack_bound - ring_variant(ring));
+    return ORION_OK;
+  }
   Wave wave() {
     Wave w = wb.wave;
     w.ctrl = (int *)ctrl.ptr;
     w.stats = (unsigned long long *)stats.ptr;
+    w.spill = (int *)spill.ptr;
+    w.spill_cap = spill_cap;
     return w;
   }
   int reset_stats(hipStream_t st) {
-    HIP_TRY(hipMemsetAsync(stats.ptr, 0, S_COUNT * 8, st));
+    HIP_TRY(hipMemsetAsync(stats.ptr, 0, kStatBytes, st));
     return ORION_OK;
   }
   int read_stats(orion_render_stats *o) {
     unsigned long long h[S_COUNT];
-    HIP_TRY(hipMemcpy(h, stats.ptr, sizeof(h), hipMemcpyDeviceToHost));
+    int rc = read_striped_stats(stats.ptr, h);
+    if (rc) return rc;
     if (!o) return ORION_OK;
     o->paths += (int64_t)h[S_PATHS];
     o->closest_rays += (int64_t)h[S_CLOSEST];
@@ -140,22 +159,22 @@ struct WaveRunner {
     o->tri_tests_closest += (int64_t)h[S_TRIS_C];
     o->node_visits_shadow += (int64_t)h[S_NODES_S];
     o->tri_tests_shadow += (int64_t)h[S_TRIS_S];
+    o->stack_overflows += (int64_t)h[S_OVERFLOW];
     return ORION_OK;
   }
 
-  void enqueue_wave(const Job *dj, const Wave &w, int depth, int stack, bool multi, int blocks,
+  void enqueue_wave(const Job *dj, const Wave &w, int depth, int ring, bool multi, int chunk,
                     hipStream_t st, LaunchTimer *tm) {
-    launch_raygen(dj, w, wb.cap, multi, st);
+    launch_raygen(dj, w, multi, st);
     int cur = 0;
     for (int it = 0; it <= depth; ++it) {
       if (tm) tm->mark(0);
-      launch_trace(dj, w, false, stack, blocks, multi, st);
+      launch_trace(dj, w, false, ring, chunk, multi, st);
       if (tm) { tm->mark(-1); tm->mark(1); }
-      launch_trace(dj, w, true, stack, blocks, multi, st);
+      launch_trace(dj, w, true, ring, chunk, multi, st);
       if (tm) tm->mark(-1);
-      launch_between(w, cur ^ 1, st);
       if (tm) tm->mark(2);
-      launch_shade(dj, w, cur, wb.cap, multi, st);
+      launch_shade(dj, w, cur, multi, st);
       if (tm) tm->mark(-1);
       cur ^= 1;
     }
@@ -163,11 +182,14 @@ struct WaveRunner {
   }
 
   // Runs one wave: jobs[0..n) with path0 / pix0 already assigned.
-  int run(hipStream_t st, const Job *jobs, int n, int npaths, int npix, int depth, int stack, bool multi,
-          int blocks, bool use_graph, LaunchTimer *tm) {
+  int run(hipStream_t st, const Job *jobs, int n, int npaths, int npix, int depth, int ring, int stack_bound,
+          bool multi, int chunk, bool use_graph, LaunchTimer *tm) {
     int rc = init(n);
     if (rc) return rc;
     if ((rc = ensure_cap(npaths))) return rc;
+    int seg, nseg;
+    choose_segments(npaths, &seg, &nseg);
+    if ((rc = ensure_spill(stack_bound, ring, chunk, seg, nseg))) return rc;
     // stage the job table and the wave sizes (pinned; the previous copy
     // must have completed before the staging area is rewritten)
     HIP_TRY(hipEventSynchronize(staged));
@@ -180,19 +202,21 @@ struct WaveRunner {
     HIP_TRY(hipEventRecord(staged, st));
     const Job *dj = (const Job *)jobs_dev.ptr;
     Wave w = wave();
+    w.seg = seg;
+    w.nseg = nseg;
     if (!use_graph || tm) {
-      enqueue_wave(dj, w, depth, stack, multi, blocks, st, tm);
+      enqueue_wave(dj, w, depth, ring, multi, chunk, st, tm);
       HIP_TRY(hipGetLastError());
       return ORION_OK;
     }
-    GraphKey key{depth, stack, multi ? 1 : 0, blocks, wb.cap, wb.mem.ptr};
+    GraphKey key{depth, ring, multi ? 1 : 0, chunk, wb.cap, w.spill_cap, seg, nseg, wb.mem.ptr, spill.ptr};
     hipGraphExec_t exec = nullptr;
     for (auto &g : graphs)
       if (g.first == key) exec = g.second;
     if (!exec) {
       hipGraph_t graph;
       HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      enqueue_wave(dj, w, depth, stack, multi, blocks, st, nullptr);
+      enqueue_wave(dj, w, depth, ring, multi, chunk, st, nullptr);
       HIP_TRY(hipStreamEndCapture(st, &graph));
       hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
@@ -209,10 +233,8 @@ struct WaveRunner {
 
 void destroy_runner(WaveRunner *r) { delete r; }
 
-static int trace_blocks(orion_ctx c, int stack, const orion_tuning *t) {
-  int bpc = t && t->persistent_blocks > 0 ? t->persistent_blocks : trace_blocks_per_cu(stack, false);
-  return c->n_cu * bpc;
-}
+static int ring_of(const orion_tuning *t) { return ring_variant(t ? t->lds_ring : 0); }
+static int chunk_of(const orion_tuning *t) { return trace_chunk(t ? t->rays_per_wave : 0); }
 
 static Job make_job(orion_scene s, int W, int H, int maxDepth, float *film, int *count) {
   Job J;
@@ -249,8 +271,8 @@ extern "C" int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, co
   if (maxDepth > 60) { set_error("maxDepth above 60 exceeds the Sobol dimension budget"); return ORION_ERR_UNSUPPORTED; }
   orion_ctx c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
-  if (!s->runner) s->runner = new WaveRunner();
-  WaveRunner &R = *s->runner;
+  if (!c->runner) c->runner = new WaveRunner();
+  WaveRunner &R = *c->runner;
   const int wave_paths = tune && tune->wave_paths > 0 ? tune->wave_paths : kDefaultWavePaths;
   int S = tune && tune->spp_per_wave > 0 ? tune->spp_per_wave : std::max(1, wave_paths / npix);
   S = std::min(S, std::max(1, k1 - k0));
@@ -258,8 +280,8 @@ extern "C" int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, co
   if ((rc = R.init(1))) return rc;
   if ((rc = R.ensure_cap(S * npix))) return rc;
   if ((rc = R.reset_stats(c->stream))) return rc;
-  const int stack = s->d.stack_entries;
-  const int blocks = trace_blocks(c, stack, tune);
+  const int ring = ring_of(tune);
+  const int chunk = chunk_of(tune);
   const bool graph = tune && tune->use_graph;
   LaunchTimer tm;
   tm.on = tune && tune->profile;
@@ -270,7 +292,8 @@ extern "C" int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, co
     Job J = make_job(s, s->W, s->H, maxDepth, (float *)s->film.ptr, (int *)s->count.ptr);
     J.S = Sk;
     J.k0 = (uint32_t)k;
-    if ((rc = R.run(c->stream, &J, 1, Sk * npix, npix, maxDepth, stack, false, blocks, graph, tm.on ? &tm : nullptr)))
+    if ((rc = R.run(c->stream, &J, 1, Sk * npix, npix, maxDepth, ring, s->d.stack_bound, false, chunk, graph,
+                    tm.on ? &tm : nullptr)))
       return rc;
   }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
@@ -294,7 +317,7 @@ struct JobState {
   DevBuf film, count;
 };
 
-constexpr int kMaxWavePaths = 1 << 23;
+constexpr int kMaxWavePaths = 1 << 25;
 
 // Picks each active job's next sample slab for one wave.  The wave holds
 // max(target, one sample of every active job) paths, at most kMaxWavePaths
@@ -345,7 +368,7 @@ static std::vector<Job> compose_wave(std::vector<JobState *> &act, int target, i
   int64_t p0 = 0;
   int pix = 0;
   *depth = 0;
-  *stack = 16;
+  *stack = 1;
   for (int i : order) {
     JobState *js = act[i];
     const int np = js->d.width * js->d.height;
@@ -358,7 +381,7 @@ static std::vector<Job> compose_wave(std::vector<JobState *> &act, int target, i
     p0 += (int64_t)J.S * np;
     pix += np;
     *depth = std::max(*depth, js->d.max_depth);
-    *stack = std::max(*stack, js->d.scene->d.stack_entries);
+    *stack = std::max(*stack, js->d.scene->d.stack_bound);
     jobs.push_back(J);
   }
   *npaths = (int)p0;
@@ -420,8 +443,8 @@ extern "C" int orion_render_batch(orion_ctx c, int n, const orion_job_desc *desc
     std::vector<int> taken;
     std::vector<Job> jobs = compose_wave(act, target, &rr, &npaths, &npix, &depth, &stack, &taken);
     if (jobs.empty()) break;
-    int blocks = trace_blocks(c, stack, tune);
-    if ((rc = R.run(c->stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, stack, jobs.size() > 1, blocks, graph,
+    if ((rc = R.run(c->stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, ring_of(tune), stack,
+                    jobs.size() > 1, chunk_of(tune), graph,
                     tm.on ? &tm : nullptr)))
       return rc;
     for (size_t i = 0; i < act.size(); ++i) act[i]->next_k += taken[i];
@@ -540,8 +563,8 @@ void orion_server_s::loop() {
     int rc = runner.init(std::max(1, (int)jobs.size()));
     if (!rc) rc = runner.reset_stats(stream);
     if (!rc && !jobs.empty())
-      rc = runner.run(stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, stack, jobs.size() > 1,
-                      trace_blocks(ctx, stack, tn), tn && tn->use_graph, nullptr);
+      rc = runner.run(stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, ring_of(tn), stack,
+                      jobs.size() > 1, chunk_of(tn), tn && tn->use_graph, nullptr);
     if (!rc && hipStreamSynchronize(stream) != hipSuccess) rc = ORION_ERR_HIP;
     orion_render_stats ws{};
     if (!rc) rc = runner.read_stats(&ws);
@@ -685,37 +708,54 @@ int orion_server_destroy(orion_server s) {
 // ======================================================================= autotuner
 int orion_autotune(orion_ctx c, int n, const orion_job_desc *jobs, orion_tuning *best) {
   if (!c || n <= 0 || !jobs || !best) { set_error("bad argument"); return ORION_ERR_ARG; }
-  std::vector<orion_job_desc> trial(jobs, jobs + n);
+  std::vector<orion_job_desc> trial_jobs(jobs, jobs + n);
   std::vector<std::vector<float>> rgb(n);
   std::vector<std::vector<int>> cnt(n);
   std::vector<float *> rp(n);
   std::vector<int *> cp(n);
   for (int j = 0; j < n; ++j) {
-    trial[j].spp = std::min(trial[j].spp, 2);
-    size_t np = (size_t)trial[j].width * trial[j].height;
+    trial_jobs[j].spp = std::min(trial_jobs[j].spp, 2);
+    size_t np = (size_t)trial_jobs[j].width * trial_jobs[j].height;
     rgb[j].resize(np * 3);
     cnt[j].resize(np);
     rp[j] = rgb[j].data();
     cp[j] = cnt[j].data();
   }
-  const int wave_sizes[] = {1 << 20, 1 << 21, 1 << 22, 1 << 23};
-  const int bpcs[] = {0, 2, 4};
+  // coordinate descent: wave size, then traversal chunk, LDS ring, graph
   double best_ms = 1e30;
   orion_tuning b{};
-  for (int wp : wave_sizes)
-    for (int bpc : bpcs)
-      for (int graph = 0; graph < 2; ++graph) {
-        orion_tuning t{};
-        t.wave_paths = wp;
-        t.persistent_blocks = bpc;
-        t.use_graph = graph;
-        orion_render_stats st{};
-        // the first call captures graphs and sizes buffers; time the second
-        int rc = orion_render_batch(c, n, trial.data(), rp.data(), cp.data(), &t, &st);
-        if (!rc) rc = orion_render_batch(c, n, trial.data(), rp.data(), cp.data(), &t, &st);
-        if (rc) return rc;
-        if (st.ms < best_ms) { best_ms = st.ms; b = t; }
-      }
+  auto trial = [&](const orion_tuning &t) -> int {
+    orion_render_stats st{};
+    // the first call captures graphs and sizes buffers; time the second
+    int rc = orion_render_batch(c, n, trial_jobs.data(), rp.data(), cp.data(), &t, &st);
+    if (!rc) rc = orion_render_batch(c, n, trial_jobs.data(), rp.data(), cp.data(), &t, &st);
+    if (rc) return rc;
+    if (st.ms < best_ms) { best_ms = st.ms; b = t; }
+    return ORION_OK;
+  };
+  int rc;
+  for (int wp : {1 << 22, 1 << 23, 1 << 24, 1 << 25}) {
+    orion_tuning t{};
+    t.wave_paths = wp;
+    if ((rc = trial(t))) return rc;
+  }
+  for (int chunk : {128, 512}) {
+    orion_tuning t = b;
+    t.rays_per_wave = chunk;
+    if ((rc = trial(t))) return rc;
+  }
+  for (int ring : {8, 32}) {
+    orion_tuning t = b;
+    t.lds_ring = ring;
+    if ((rc = trial(t))) return rc;
+  }
+  {
+    orion_tuning t = b;
+    t.use_graph = 1;
+    if ((rc = trial(t))) return rc;
+  }
+  if (b.rays_per_wave == 0) b.rays_per_wave = kDefaultChunk;
+  if (b.lds_ring == 0) b.lds_ring = kDefaultRing;
   *best = b;
   return ORION_OK;
 }

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <future>
 #include <limits>
+#include <string>
 #include <vector>
 
 namespace orion_amd {
@@ -146,6 +147,115 @@ std::vector<TNode> build(Ctx &c, int first, int count, int depth) {
   return out;
 }
 
+int leaf_code(const TNode &l) { return ~(l.first * 8 + (l.count - 1)); }
+
+// Grid step of one axis: the smallest power of two 2^e with
+// fmaf(255, 2^e, lo) >= hi, as a biased exponent byte.
+uint32_t grid_exponent(float lo, float hi) {
+  double ext = (double)hi - (double)lo;
+  int e = ext > 0 ? (int)std::ceil(std::log2(ext / 255.0)) : -126;
+  e = std::max(-126, std::min(127, e));
+  for (;;) {
+    float step;
+    uint32_t bits = (uint32_t)(e + 127) << 23;
+    std::memcpy(&step, &bits, 4);
+    if (std::fmaf(255.0f, step, lo) >= hi || e >= 127) break;
+    ++e;
+  }
+  return (uint32_t)(e + 127);
+}
+
+float step_of(uint32_t ebyte) {
+  uint32_t bits = ebyte << 23;
+  float f;
+  std::memcpy(&f, &bits, 4);
+  return f;
+}
+
+// Quantizes [clo, chi] on one axis to bytes whose decoded planes enclose it.
+void quantize(float clo, float chi, float org, float step, uint8_t *qlo, uint8_t *qhi) {
+  int lo = (int)std::floor(((double)clo - org) / step);
+  lo = std::max(0, std::min(255, lo));
+  while (lo > 0 && std::fmaf((float)lo, step, org) > clo) --lo;
+  int hi = (int)std::ceil(((double)chi - org) / step);
+  hi = std::max(0, std::min(255, hi));
+  while (hi < 255 && std::fmaf((float)hi, step, org) < chi) ++hi;
+  *qlo = (uint8_t)lo;
+  *qhi = (uint8_t)hi;
+}
+
+// Collapses the binary tree into 4-wide nodes (a node's slots take the
+// children of its largest-area inner child until there are four) and writes
+// them breadth first, siblings contiguous.
+void emit_wide(const std::vector<TNode> &t, Bvh *out) {
+  out->nodes.clear();
+  out->depth = 0;
+  out->stack_bound = 1;
+  BvhNode root{};
+  if (t.empty()) {
+    for (int k = 0; k < 4; ++k) root.child[k] = kLeafEmpty;
+    for (int k = 0; k < 4; ++k) { root.q[k] = 255; root.q[4 + k] = 0; }
+    out->nodes.push_back(root);
+    return;
+  }
+  out->nodes.push_back(root);
+  struct Item { int tnode, slot, depth; };
+  std::vector<Item> queue;
+  queue.push_back({0, 0, 0});
+  for (size_t qi = 0; qi < queue.size(); ++qi) {
+    Item it = queue[qi];
+    out->depth = std::max(out->depth, it.depth);
+    int kids[4], nk = 0;
+    if (t[it.tnode].count > 0) {
+      kids[nk++] = it.tnode;  // a lone leaf at the root
+    } else {
+      kids[nk++] = t[it.tnode].kid[0];
+      kids[nk++] = t[it.tnode].kid[1];
+      while (nk < 4) {
+        int best = -1;
+        float ba = -1;
+        for (int k = 0; k < nk; ++k)
+          if (t[kids[k]].count == 0 && t[kids[k]].box.area() > ba) { ba = t[kids[k]].box.area(); best = k; }
+        if (best < 0) break;
+        int open = kids[best];
+        kids[best] = t[open].kid[0];
+        kids[nk++] = t[open].kid[1];
+      }
+    }
+    Box nb;
+    for (int k = 0; k < nk; ++k) nb.grow(t[kids[k]].box);
+    BvhNode g{};
+    uint32_t eb[3];
+    float step[3];
+    for (int a = 0; a < 3; ++a) {
+      g.org[a] = nb.lo[a];
+      eb[a] = grid_exponent(nb.lo[a], nb.hi[a]);
+      step[a] = step_of(eb[a]);
+    }
+    g.exps = eb[0] | (eb[1] << 8) | (eb[2] << 16);
+    for (int k = 0; k < 4; ++k) {
+      if (k >= nk) {
+        g.child[k] = kLeafEmpty;
+        for (int a = 0; a < 3; ++a) { g.q[8 * a + k] = 255; g.q[8 * a + 4 + k] = 0; }
+        continue;
+      }
+      const TNode &c = t[kids[k]];
+      for (int a = 0; a < 3; ++a) quantize(c.box.lo[a], c.box.hi[a], g.org[a], step[a], &g.q[8 * a + k], &g.q[8 * a + 4 + k]);
+      if (c.count > 0) {
+        g.child[k] = leaf_code(c);
+      } else {
+        g.child[k] = (int)out->nodes.size();
+        out->nodes.push_back(BvhNode{});
+        queue.push_back({kids[k], g.child[k], it.depth + 1});
+      }
+    }
+    out->nodes[it.slot] = g;
+  }
+  // a depth-first walk keeps at most three siblings per level waiting
+  // (levels 1 .. depth + 1)
+  out->stack_bound = 3 * (out->depth + 1);
+}
+
 }  // namespace
 
 void build_bvh(const float *tri_pn, int n, Bvh *out) {
@@ -171,9 +281,7 @@ void build_bvh(const float *tri_pn, int n, Bvh *out) {
   }
   std::vector<TNode> t;
   if (n > 0) t = build(c, 0, n, 0);
-  // flatten: inner nodes get GPU node slots in depth-first order; leaves are
-  // encoded in the parent's child field
-  out->nodes.clear();
+  // triangles in leaf order; leaves are encoded in their parent's child slot
   out->tris.resize(n);
   out->prim = c.order;
   for (int s = 0; s < n; ++s) {
@@ -189,55 +297,70 @@ void build_bvh(const float *tri_pn, int n, Bvh *out) {
     }
     bt.v[7] = 0; bt.v[11] = 0;
   }
-  auto leaf_code = [](const TNode &l) { return ~(l.first * 8 + (l.count - 1)); };
-  auto empty_box = [](float *dst) { dst[0] = kInfF; dst[1] = -kInfF; dst[2] = kInfF; dst[3] = -kInfF; };
+  emit_wide(t, out);
+  out->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+std::string check_bvh(const Bvh &b, const float *tri_pn, int n) {
+  if (b.nodes.empty()) return "no root node";
+  if ((int)b.tris.size() != n || (int)b.prim.size() != n) return "triangle count mismatch";
+  std::vector<int> seen(n, 0);
+  struct Item { int node, depth; };
+  std::vector<Item> st{{0, 0}};
   int max_depth = 0;
-  if (n == 0) {
-    BvhNode e;
-    empty_box(e.a); empty_box(e.b);
-    e.c[0] = kInfF; e.c[1] = -kInfF; e.c[2] = kInfF; e.c[3] = -kInfF;
-    e.d[0] = e.d[1] = ~0; e.d[2] = e.d[3] = 0;
-    out->nodes.push_back(e);
-  } else if (t[0].count > 0) {
-    // single leaf: a root whose second child is empty
-    BvhNode e;
-    const Box &b = t[0].box;
-    e.a[0] = b.lo[0]; e.a[1] = b.hi[0]; e.a[2] = b.lo[1]; e.a[3] = b.hi[1];
-    empty_box(e.b);
-    e.c[0] = b.lo[2]; e.c[1] = b.hi[2]; e.c[2] = kInfF; e.c[3] = -kInfF;
-    e.d[0] = leaf_code(t[0]); e.d[1] = leaf_code(t[0]); e.d[2] = e.d[3] = 0;
-    out->nodes.push_back(e);
-  } else {
-    std::vector<int> slot(t.size(), -1);
-    std::vector<std::pair<int, int>> stack;  // (tnode, depth)
-    stack.push_back({0, 0});
-    // first pass: assign slots in DFS pre-order (child 0 first)
-    int next = 0;
-    std::vector<int> dfs;
-    while (!stack.empty()) {
-      auto [i, d] = stack.back();
-      stack.pop_back();
-      if (t[i].count > 0) continue;
-      slot[i] = next++;
-      dfs.push_back(i);
-      max_depth = std::max(max_depth, d);
-      stack.push_back({t[i].kid[1], d + 1});
-      stack.push_back({t[i].kid[0], d + 1});
-    }
-    out->nodes.resize(next);
-    for (int i : dfs) {
-      BvhNode &g = out->nodes[slot[i]];
-      const TNode &c0 = t[t[i].kid[0]], &c1 = t[t[i].kid[1]];
-      g.a[0] = c0.box.lo[0]; g.a[1] = c0.box.hi[0]; g.a[2] = c0.box.lo[1]; g.a[3] = c0.box.hi[1];
-      g.b[0] = c1.box.lo[0]; g.b[1] = c1.box.hi[0]; g.b[2] = c1.box.lo[1]; g.b[3] = c1.box.hi[1];
-      g.c[0] = c0.box.lo[2]; g.c[1] = c0.box.hi[2]; g.c[2] = c1.box.lo[2]; g.c[3] = c1.box.hi[2];
-      g.d[0] = c0.count > 0 ? leaf_code(c0) : slot[t[i].kid[0]];
-      g.d[1] = c1.count > 0 ? leaf_code(c1) : slot[t[i].kid[1]];
-      g.d[2] = g.d[3] = 0;
+  std::vector<char> visited(b.nodes.size(), 0);
+  while (!st.empty()) {
+    Item it = st.back();
+    st.pop_back();
+    if (it.node < 0 || it.node >= (int)b.nodes.size()) return "child index out of range";
+    if (visited[it.node]++) return "node reached twice";
+    max_depth = std::max(max_depth, it.depth);
+    const BvhNode &g = b.nodes[it.node];
+    float step[3];
+    for (int a = 0; a < 3; ++a) step[a] = step_of((g.exps >> (8 * a)) & 0xffu);
+    for (int k = 0; k < 4; ++k) {
+      int c = g.child[k];
+      if (c == kLeafEmpty) continue;
+      float lo[3], hi[3];
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::fmaf((float)g.q[8 * a + k], step[a], g.org[a]);
+        hi[a] = std::fmaf((float)g.q[8 * a + 4 + k], step[a], g.org[a]);
+      }
+      // every triangle below this slot lies inside the decoded box
+      std::vector<int> sub{c};
+      while (!sub.empty()) {
+        int x = sub.back();
+        sub.pop_back();
+        if (x >= 0) {
+          if (x >= (int)b.nodes.size()) return "child index out of range";
+          for (int kk = 0; kk < 4; ++kk)
+            if (b.nodes[x].child[kk] != kLeafEmpty) sub.push_back(b.nodes[x].child[kk]);
+          continue;
+        }
+        int code = ~x, first = code >> 3, cnt = (code & 7) + 1;
+        if (first < 0 || first + cnt > n) return "leaf range out of bounds";
+        for (int s2 = first; s2 < first + cnt; ++s2) {
+          const float *q = tri_pn + 18 * (size_t)b.prim[s2];
+          for (int v = 0; v < 3; ++v)
+            for (int a = 0; a < 3; ++a)
+              if (!(q[3 * v + a] >= lo[a] && q[3 * v + a] <= hi[a])) return "decoded box misses a triangle";
+        }
+      }
+      if (c >= 0) {
+        st.push_back({c, it.depth + 1});
+      } else {
+        int code = ~c, first = code >> 3, cnt = (code & 7) + 1;
+        for (int s2 = first; s2 < first + cnt; ++s2) seen[s2]++;
+      }
     }
   }
-  out->depth = max_depth;
-  out->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (int i = 0; i < n; ++i)
+    if (seen[i] != 1) return "triangle slot not referenced exactly once";
+  for (size_t i = 0; i < visited.size(); ++i)
+    if (!visited[i]) return "unreachable node";
+  if (max_depth != b.depth) return "depth mismatch";
+  if (b.stack_bound != 3 * (b.depth + 1)) return "stack bound mismatch";
+  return "";
 }
 
 }  // namespace orion_amd

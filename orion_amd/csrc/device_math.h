@@ -132,6 +132,41 @@ __device__ __forceinline__ V3 mat3(const float *m, V3 v) {
 // reflection + GGX transmission when rough).
 enum { BX_LAMBERT = 0, BX_MF_REFL = 1, BX_MF_TRANS = 2, BX_FRESNEL_SPEC = 3 };
 enum { FR_CONDUCTOR = 0, FR_DIELECTRIC = 1 };
+// Load through a global-address-space pointer: scene arrays reach the
+// kernels inside a job table in memory, where the compiler cannot prove
+// they are global and would emit flat loads (which also hold the LDS
+// counter, serialising the LDS stack behind outstanding fetches).
+template <class T>
+__device__ __forceinline__ T gld(const T *p) {
+  typedef const __attribute__((address_space(1))) T gT;
+  return *(gT *)p;
+}
+// vector types go through clang's native vectors (HIP's vector classes would
+// copy through a generic reference)
+typedef float gf4 __attribute__((ext_vector_type(4)));
+typedef unsigned int gu4 __attribute__((ext_vector_type(4)));
+typedef unsigned int gu2 __attribute__((ext_vector_type(2)));
+template <>
+__device__ __forceinline__ float4 gld<float4>(const float4 *p) {
+  gf4 v = *(const __attribute__((address_space(1))) gf4 *)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+template <>
+__device__ __forceinline__ int4 gld<int4>(const int4 *p) {
+  gu4 v = *(const __attribute__((address_space(1))) gu4 *)p;
+  return make_int4((int)v.x, (int)v.y, (int)v.z, (int)v.w);
+}
+template <>
+__device__ __forceinline__ uint4 gld<uint4>(const uint4 *p) {
+  gu4 v = *(const __attribute__((address_space(1))) gu4 *)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+template <>
+__device__ __forceinline__ uint2 gld<uint2>(const uint2 *p) {
+  gu2 v = *(const __attribute__((address_space(1))) gu2 *)p;
+  return make_uint2(v.x, v.y);
+}
+
 struct Lobe {
   int kind, type, fresnel, photon;
   Sp R, T, etaT, k;
@@ -315,14 +350,16 @@ __device__ inline Sp lobe_sample(const Lobe &b, V3 wo, float u0, float u1, V3 *w
 
 __device__ __forceinline__ int bsdf_num(const Bsdf &bs, int type) {
   int n = 0;
-  for (int i = 0; i < bs.n; ++i) n += match(bs.b[i], type) ? 1 : 0;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) n += (i < bs.n && match(bs.b[i], type)) ? 1 : 0;
   return n;
 }
 __device__ inline Sp bsdf_f(const Bsdf &bs, V3 wi, V3 wo, int type) {  // bsdf.cpp:17-29
   Sp r = sp(0);
   V3 swi = bs.to_local(wi), swo = bs.to_local(wo);
-  for (int i = 0; i < bs.n; ++i)
-    if (match(bs.b[i], type)) acc(r, lobe_f(bs.b[i], swi, swo));
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+    if (i < bs.n && match(bs.b[i], type)) acc(r, lobe_f(bs.b[i], swi, swo));
   return r;
 }
 __device__ inline float bsdf_pdf(const Bsdf &bs, V3 wi, V3 wo, int type) {  // bsdf.cpp:85-103
@@ -331,8 +368,9 @@ __device__ inline float bsdf_pdf(const Bsdf &bs, V3 wi, V3 wo, int type) {  // b
   if (swo.z == 0) return 0;
   float pdf = 0;
   int m = 0;
-  for (int i = 0; i < bs.n; ++i)
-    if (match(bs.b[i], type)) { ++m; pdf += lobe_pdf(bs.b[i], swi, swo); }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+    if (i < bs.n && match(bs.b[i], type)) { ++m; pdf += lobe_pdf(bs.b[i], swi, swo); }
   return m > 0 ? pdf / m : 0.f;
 }
 // bsdf.cpp:31-83; *pdf and *stype stay untouched on the swo.z == 0 exit
@@ -340,10 +378,11 @@ __device__ inline Sp bsdf_sample(const Bsdf &bs, V3 *wi, V3 wo, float u0, float 
   int matching = bsdf_num(bs, type);
   if (matching == 0) { *pdf = 0; *stype = 0; return sp(0); }
   int pos = min((int)floorf(u0 * matching), matching - 1);
-  int count = pos, sel = 0;
-  for (int i = 0; i < bs.n; ++i)
-    if (match(bs.b[i], type) && count-- == 0) { sel = i; break; }
-  const Lobe &bx = bs.b[sel];
+  // the pos-th matching lobe (at most two lobes: constant indices only, so
+  // the lobes stay in registers)
+  const bool m0 = bs.n > 0 && match(bs.b[0], type);
+  const int sel = (m0 && pos == 0) ? 0 : 1;
+  const Lobe bx = sel == 0 ? bs.b[0] : bs.b[1];
   float nu0 = mn(u0 * matching - pos, kOneMinusEps);
   V3 swi = mk(0, 0, 0), swo = bs.to_local(wo);
   if (swo.z == 0) return sp(0);
@@ -353,16 +392,19 @@ __device__ inline Sp bsdf_sample(const Bsdf &bs, V3 *wi, V3 wo, float u0, float 
   if (*pdf == 0) { *stype = 0; return sp(0); }
   *wi = bs.to_world(swi);
   bool hasSpecular = (bx.type & SPEC) != 0;
-  if (!hasSpecular && matching > 1)
-    for (int i = 0; i < bs.n; ++i)
-      if (i != sel && match(bs.b[i], type)) *pdf += lobe_pdf(bs.b[i], swi, swo);
+  if (!hasSpecular && matching > 1) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      if (i < bs.n && i != sel && match(bs.b[i], type)) *pdf += lobe_pdf(bs.b[i], swi, swo);
+  }
   if (matching > 1) *pdf /= matching;
   if (!hasSpecular && matching > 1) {
     bool refl = dot(*wi, bs.ng) * dot(wo, bs.ng) > 0;
     f = sp(0);
-    for (int i = 0; i < bs.n; ++i) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
       bool hr = (bs.b[i].type & REFL) != 0, ht = (bs.b[i].type & TRANS) != 0;
-      if (match(bs.b[i], type) && ((refl && hr) || (!refl && ht))) acc(f, lobe_f(bs.b[i], swi, swo));
+      if (i < bs.n && match(bs.b[i], type) && ((refl && hr) || (!refl && ht))) acc(f, lobe_f(bs.b[i], swi, swo));
     }
   }
   return f;
@@ -372,40 +414,50 @@ __device__ inline Sp bsdf_sample(const Bsdf &bs, V3 *wi, V3 wo, float u0, float 
 // alpha comes precomputed in the record.
 __device__ inline void make_bsdf(const float *m, V3 ng, V3 ns, V3 u, V3 v, bool photon, Bsdf *bs) {
   bs->ng = ng; bs->ns = ns; bs->u = u; bs->v = v; bs->n = 0;
-  int type = (int)m[0];
+  // lobes are appended through constant indices so they stay in registers
+  auto add = [&](const Lobe &l) {
+    if (bs->n == 0) bs->b[0] = l;
+    else bs->b[1] = l;
+    bs->n++;
+  };
+  Lobe b{};
+  b.photon = photon;
+  int type = (int)gld(m + 0);
   if (type == 0) {
-    Sp r = sp3(clampf(m[1], 0, kFltMax), clampf(m[2], 0, kFltMax), clampf(m[3], 0, kFltMax));
+    Sp r = sp3(clampf(gld(m + 1), 0, kFltMax), clampf(gld(m + 2), 0, kFltMax), clampf(gld(m + 3), 0, kFltMax));
     if (!black(r)) {
-      Lobe &b = bs->b[bs->n++];
       b.kind = BX_LAMBERT; b.type = REFL | DIFF; b.R = r;
-      b.fresnel = 0; b.photon = photon; b.alpha = 0; b.etaA = b.etaB = 0;
+      b.fresnel = 0; b.alpha = 0; b.etaA = b.etaB = 0;
+      add(b);
     }
   } else if (type == 1) {
-    Lobe &b = bs->b[bs->n++];
     b.kind = BX_MF_REFL; b.type = REFL | GLOSSY; b.R = sp(1.0f);
-    b.alpha = m[8];
-    b.fresnel = FR_CONDUCTOR; b.photon = photon;
-    b.etaT = sp3(m[1], m[2], m[3]);
-    b.k = sp3(m[4], m[5], m[6]);
+    b.alpha = gld(m + 8);
+    b.fresnel = FR_CONDUCTOR;
+    b.etaT = sp3(gld(m + 1), gld(m + 2), gld(m + 3));
+    b.k = sp3(gld(m + 4), gld(m + 5), gld(m + 6));
     b.etaA = b.etaB = 0;
+    add(b);
   } else {
-    Sp R = sp3(m[1], m[2], m[3]), T = sp3(m[4], m[5], m[6]);
-    float rough = m[7], alpha = m[8], e = m[9];
+    Sp R = sp3(gld(m + 1), gld(m + 2), gld(m + 3)), T = sp3(gld(m + 4), gld(m + 5), gld(m + 6));
+    float rough = gld(m + 7), alpha = gld(m + 8), e = gld(m + 9);
     if (black(R) && black(T)) return;
     if (rough == 0) {
-      Lobe &b = bs->b[bs->n++];
       b.kind = BX_FRESNEL_SPEC; b.type = REFL | TRANS | SPEC;
-      b.R = R; b.T = T; b.etaA = 1.0f; b.etaB = e; b.photon = photon; b.fresnel = FR_DIELECTRIC; b.alpha = 0;
+      b.R = R; b.T = T; b.etaA = 1.0f; b.etaB = e; b.fresnel = FR_DIELECTRIC; b.alpha = 0;
+      add(b);
     } else {
       if (!black(R)) {
-        Lobe &b = bs->b[bs->n++];
-        b.kind = BX_MF_REFL; b.type = REFL | GLOSSY; b.R = R; b.alpha = alpha;
-        b.fresnel = FR_DIELECTRIC; b.etaA = 1.0f; b.etaB = e; b.photon = photon;
+        Lobe br = b;
+        br.kind = BX_MF_REFL; br.type = REFL | GLOSSY; br.R = R; br.alpha = alpha;
+        br.fresnel = FR_DIELECTRIC; br.etaA = 1.0f; br.etaB = e;
+        add(br);
       }
       if (!black(T)) {
-        Lobe &b = bs->b[bs->n++];
-        b.kind = BX_MF_TRANS; b.type = TRANS | GLOSSY; b.T = T; b.alpha = alpha;
-        b.etaA = 1.0f; b.etaB = e; b.photon = photon; b.fresnel = FR_DIELECTRIC;
+        Lobe bt = b;
+        bt.kind = BX_MF_TRANS; bt.type = TRANS | GLOSSY; bt.T = T; bt.alpha = alpha;
+        bt.etaA = 1.0f; bt.etaB = e; bt.fresnel = FR_DIELECTRIC;
+        add(bt);
       }
     }
   }

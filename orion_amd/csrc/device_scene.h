@@ -16,8 +16,6 @@
 
 namespace orion_amd {
 
-constexpr int kStackEntries = kBvhMaxDepth + 2;  // worst case; kernels use the scene's depth + 2
-
 struct DScene {
   const float4 *nodes;   // 4 x float4 per node (bvh.h)
   const float4 *tris;    // 3 x float4 per leaf slot
@@ -27,11 +25,11 @@ struct DScene {
   const float *light_cdf;   // n_lights + 1
   const float *light_func;  // n_lights
   const float *light_area;  // n_lights (area lights; 0 otherwise)
-  const uint32_t *sobol;    // 1024 x 32 generator columns
+  const uint32_t *sobol;    // 1024 dims x 4 bytes x 256: Sobol byte tables
   float light_func_int;
   int n_lights;
   int n_prims;
-  int stack_entries;  // traversal stack depth this scene needs (BVH depth + 2)
+  int stack_bound;    // traversal stack entries that can be live at once
   float cam[16];
 };
 
@@ -78,75 +76,193 @@ struct HitRec {
   int tie;
 };
 
-// Stack: `stk` points at this lane's first entry, consecutive entries are
-// `stride` ints apart (LDS column per lane: conflict-free across a wave).
-template <bool ANY>
-__device__ inline bool traverse(const DScene &sc, V3 o, V3 d, float tmin, float tmax, HitRec *h, int *stk,
-                                int stride, uint32_t *node_visits, uint32_t *tri_tests) {
-  // |d| components below 1e-35 are clamped so (lo - o) * inv is never 0 * inf
-  // (a NaN would drop a slab); the padded boxes absorb the change
+// Traversal stack: the newest RING entries of each lane live in an LDS ring
+// (`ring` = the lane's first slot, slots kRingStride ints apart, one LDS
+// column per lane so a wave's accesses are conflict-free); older entries
+// move to the lane's spill area in HBM (`spill`, entries `spill_stride` ints
+// apart, `spill_cap` entries).  A DFS stack of a 4-wide tree holds at most
+// three siblings per level, so stack_bound entries never overflow; an
+// overflow (a malformed tree) is counted in *overflow and the entry dropped.
+constexpr int kRingStride = 128;  // threads per traversal block
+
+template <int RING>
+struct TravStack {
+  int *ring;
+  int *spill;
+  int spill_stride, spill_cap;
+  int top = 0, base = 0;  // ring holds [base, top); spill holds [0, base)
+  __device__ __forceinline__ void push(int x, uint32_t *overflow) {
+    if (top - base == RING) {
+      if (base < spill_cap) spill[spill_stride * base] = ring[kRingStride * (base & (RING - 1))];
+      else ++*overflow;
+      ++base;
+    }
+    ring[kRingStride * (top & (RING - 1))] = x;
+    ++top;
+  }
+  __device__ __forceinline__ int pop() {
+    if (top > base) {
+      --top;
+      return ring[kRingStride * (top & (RING - 1))];
+    }
+    if (base > 0) {
+      --base;
+      --top;
+      return base < spill_cap ? spill[spill_stride * base] : kLeafEmpty;
+    }
+    return kLeafEmpty;
+  }
+};
+
+__device__ __forceinline__ float q2f(uint32_t word, int k) { return (float)((word >> (8 * k)) & 0xffu); }
+
+// One ray in flight in the traversal engine.
+struct TRay {
+  V3 o, d, inv;
+  float tmin, tmax;
+  const float4 *nodes, *tris;
+  int tag;  // caller's id (queue entry)
+};
+
+// |d| components below 1e-35 are clamped so (lo - o) * inv is never 0 * inf
+// (a NaN would drop a slab); the padded boxes absorb the change.
+__device__ __forceinline__ V3 safe_inv(V3 d) {
   auto safe = [](float x) { return fabsf(x) > 1e-35f ? x : copysignf(1e-35f, x); };
-  V3 inv = mk(1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z));
+  return mk(1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z));
+}
+
+// Inner node: decode the four child boxes (fmaf(q, step, origin), exactly as
+// the builder evaluated them), slab-test them, continue with the nearest hit
+// child and push the others far-to-near (any-hit: in slot order).  Returns
+// the next node (a leaf code or kLeafEmpty when the stack is exhausted).
+template <bool ANY, int RING>
+__device__ __forceinline__ int node_step(const TRay &ray, float best, int node, TravStack<RING> &st,
+                                         uint32_t *overflow) {
+  const float4 *nd = ray.nodes + 4 * (size_t)node;
+  const float4 A = gld(nd);
+  const int4 C = gld(reinterpret_cast<const int4 *>(nd + 1));
+  const uint4 Q = gld(reinterpret_cast<const uint4 *>(nd + 2));
+  const uint2 R = gld(reinterpret_cast<const uint2 *>(nd + 3));
+  const uint32_t eb = __float_as_uint(A.w);
+  const float sx = __uint_as_float((eb & 0xffu) << 23);
+  const float sy = __uint_as_float(((eb >> 8) & 0xffu) << 23);
+  const float sz = __uint_as_float(((eb >> 16) & 0xffu) << 23);
+  const int cc[4] = {C.x, C.y, C.z, C.w};
+  float tk[4];
+  int ck[4];
+  int nh = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    float lox = fmaf(q2f(Q.x, k), sx, A.x), hix = fmaf(q2f(Q.y, k), sx, A.x);
+    float loy = fmaf(q2f(Q.z, k), sy, A.y), hiy = fmaf(q2f(Q.w, k), sy, A.y);
+    float loz = fmaf(q2f(R.x, k), sz, A.z), hiz = fmaf(q2f(R.y, k), sz, A.z);
+    float t;
+    bool hit = box_hit(lox, hix, loy, hiy, loz, hiz, ray.o, ray.inv, ray.tmin, best, &t) & (cc[k] != kLeafEmpty);
+    tk[k] = hit ? t : __builtin_inff();
+    ck[k] = cc[k];
+    nh += hit ? 1 : 0;
+  }
+  auto cs = [&](int a, int b) {
+    // closest: ascending entry distance; any: hit children first
+    bool sw = ANY ? (!(tk[a] < __builtin_inff()) && tk[b] < __builtin_inff()) : (tk[b] < tk[a]);
+    float ta = tk[a], tb = tk[b];
+    int ca = ck[a], cb = ck[b];
+    tk[a] = sw ? tb : ta; tk[b] = sw ? ta : tb;
+    ck[a] = sw ? cb : ca; ck[b] = sw ? ca : cb;
+  };
+  cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
+  if (nh > 3) st.push(ck[3], overflow);
+  if (nh > 2) st.push(ck[2], overflow);
+  if (nh > 1) st.push(ck[1], overflow);
+  return nh > 0 ? ck[0] : st.pop();
+}
+
+// Leaf: triangle tests, keeping the lexicographic minimum (t, prim) and the
+// tie flag.  Any-hit returns true at the first accepted triangle.
+template <bool ANY>
+__device__ __forceinline__ bool leaf_step(const TRay &ray, float *best, int node, HitRec *h, bool *found,
+                                          uint32_t *tri_tests) {
+  const int code = ~node;
+  const int first = code >> 3, cnt = (code & 7) + 1;
+  for (int k = first; k < first + cnt; ++k) {
+    const float4 *tr = ray.tris + 3 * (size_t)k;
+    float4 a = gld(tr), b = gld(tr + 1), c = gld(tr + 2);
+    ++*tri_tests;
+    float t, u, v;
+    if (!tri_test(a, b, c, ray.o, ray.d, ray.tmin, ANY ? ray.tmax : *best, &t, &u, &v)) continue;
+    int prim = __float_as_int(a.w);
+    if (ANY) {
+      h->t = t; h->u = u; h->v = v; h->prim = prim;
+      *found = true;
+      return true;
+    }
+    if (!*found || t < *best) {
+      *best = t; h->t = t; h->u = u; h->v = v; h->prim = prim; h->tie = 0; *found = true;
+    } else if (t == *best) {
+      h->tie = 1;
+      if (prim < h->prim) { h->u = u; h->v = v; h->prim = prim; }
+    }
+  }
+  return false;
+}
+
+// Traversal engine for one wave over queue entries [begin, end): a lane
+// whose ray finishes takes the next entry once at least kRefillMin lanes are
+// idle, so the wave's time is the sum of its rays' work spread over 64 lanes
+// rather than the longest ray of each group of 64.  Per step every active
+// lane descends through inner nodes to its next leaf, then tests that leaf.
+// Waves own fixed ranges (no shared work counter: device-scope atomics on
+// one address serialise across the XCDs and cost more than they balance).
+// IO: void load(int entry, TRay *ray); void store(const TRay &, bool hit,
+// const HitRec &).
+constexpr int kRefillMin = 8;
+
+template <bool ANY, int RING, class IO>
+__device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> &st, uint32_t *node_visits,
+                                    uint32_t *tri_tests, uint32_t *overflow) {
+  const int lane = (int)__lane_id();
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int next = begin;  // wave-uniform: next unassigned entry
+  bool active = false;
+  TRay ray;
+  HitRec h;
+  h.prim = -1;
+  h.tie = 0;
   bool found = false;
-  float best = tmax;
-  h->prim = -1;
-  h->tie = 0;
-  int sp = 0;
-  int node = 0;
+  float best = 0.f;
+  int node = kLeafEmpty;
   uint32_t nv = 0, nt = 0;
   for (;;) {
-    if (node >= 0) {
-      // inner node: test both children
-      const float4 *nd = sc.nodes + 4 * (size_t)node;
-      float4 A = nd[0], B = nd[1], C = nd[2];
-      int4 D = *reinterpret_cast<const int4 *>(nd + 3);
-      ++nv;
-      float t0, t1;
-      // an empty child slot has lo.x > hi.x and is never entered
-      bool h0 = A.x <= A.y && box_hit(A.x, A.y, A.z, A.w, C.x, C.y, o, inv, tmin, best, &t0);
-      bool h1 = B.x <= B.y && box_hit(B.x, B.y, B.z, B.w, C.z, C.w, o, inv, tmin, best, &t1);
-      if (h0 && h1) {
-        int nearc = D.x, farc = D.y;
-        if (t1 < t0) { nearc = D.y; farc = D.x; }
-        stk[stride * sp++] = farc;
-        node = nearc;
-        continue;
-      } else if (h0) {
-        node = D.x;
-        continue;
-      } else if (h1) {
-        node = D.y;
-        continue;
+    const uint64_t idle = __ballot(!active);
+    const int nidle = __popcll(idle);
+    if (next < end && (nidle >= kRefillMin || nidle == 64)) {
+      const int entry = next + __popcll(idle & below);
+      if (!active && entry < end) {
+        io.load(entry, &ray);
+        ray.inv = safe_inv(ray.d);
+        active = true;
+        found = false;
+        best = ray.tmax;
+        h.prim = -1;
+        h.tie = 0;
+        node = 0;
+        st.top = st.base = 0;
       }
-    } else {
-      // leaf
-      int code = ~node;
-      int first = code >> 3, cnt = (code & 7) + 1;
-      for (int k = first; k < first + cnt; ++k) {
-        const float4 *tr = sc.tris + 3 * (size_t)k;
-        float4 a = tr[0], b = tr[1], c = tr[2];
-        ++nt;
-        float t, u, v;
-        if (!tri_test(a, b, c, o, d, tmin, ANY ? tmax : best, &t, &u, &v)) continue;
-        int prim = __float_as_int(a.w);
-        if (ANY) {
-          h->t = t; h->u = u; h->v = v; h->prim = prim;
-          if (node_visits) { *node_visits += nv; *tri_tests += nt; }
-          return true;
-        }
-        if (!found || t < best) {
-          best = t; h->t = t; h->u = u; h->v = v; h->prim = prim; h->tie = 0; found = true;
-        } else if (t == best) {
-          h->tie = 1;
-          if (prim < h->prim) { h->u = u; h->v = v; h->prim = prim; }
-        }
-      }
+      next = min(next + nidle, end);
     }
-    if (sp == 0) break;
-    node = stk[stride * --sp];
+    if (__ballot(active) == 0) break;
+    while (node >= 0) {
+      ++nv;
+      node = node_step<ANY, RING>(ray, best, node, st, overflow);
+    }
+    if (node != kLeafEmpty) node = leaf_step<ANY>(ray, &best, node, &h, &found, &nt) ? kLeafEmpty : st.pop();
+    if (active && node == kLeafEmpty) {
+      io.store(ray, found, h);
+      active = false;
+    }
   }
-  if (node_visits) { *node_visits += nv; *tri_tests += nt; }
-  return found;
+  *node_visits += nv;
+  *tri_tests += nt;
 }
 
 // Hit reconstruction, core/kernel.cpp:127-172.
@@ -157,7 +273,7 @@ struct Isect {
 __device__ __forceinline__ void load_prim(const DScene &sc, int prim, V3 *p0, V3 *p1, V3 *p2, V3 *n0, V3 *n1,
                                           V3 *n2, int *mat, int *light) {
   const float4 *s = sc.shade + 5 * (size_t)prim;
-  float4 a = s[0], b = s[1], c = s[2], d = s[3], e = s[4];
+  float4 a = gld(s), b = gld(s + 1), c = gld(s + 2), d = gld(s + 3), e = gld(s + 4);
   *p0 = mk(a.x, a.y, a.z); *p1 = mk(a.w, b.x, b.y); *p2 = mk(b.z, b.w, c.x);
   *n0 = mk(c.y, c.z, c.w); *n1 = mk(d.x, d.y, d.z); *n2 = mk(d.w, e.x, e.y);
   *mat = __float_as_int(e.z);
@@ -179,15 +295,13 @@ __device__ inline void make_isect(const DScene &sc, int prim, float bu, float bv
 }
 
 // Sobol (sobol.h:28-43): XOR of generator columns for the set bits of the
-// index, float(result) * 2^-32.  Indices on this path are < 2^32.
+// index, float(result) * 2^-32.  Indices on this path are < 2^32.  `tab`
+// holds per-dimension byte tables (4 x 256 words: the XOR of the columns of
+// each byte value), so the XOR over the set bits is four independent loads.
 __device__ __forceinline__ float sobol(const uint32_t *tab, uint32_t index, int dim) {
-  const uint32_t *m = tab + (size_t)dim * 32;
-  uint32_t r = 0;
-  while (index) {
-    int b = __builtin_ctz(index);
-    r ^= m[b];
-    index &= index - 1;
-  }
+  const uint32_t *m = tab + (size_t)dim * 1024;
+  uint32_t r = gld(m + (index & 0xffu)) ^ gld(m + 256 + ((index >> 8) & 0xffu)) ^
+               gld(m + 512 + ((index >> 16) & 0xffu)) ^ gld(m + 768 + (index >> 24));
   return (float)r * (1.f / 4294967296.0f);
 }
 

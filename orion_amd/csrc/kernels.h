@@ -7,19 +7,32 @@
 
 namespace orion_amd {
 
-constexpr int kTraceBlock = 128;  // threads per traversal block (2 wave64)
+constexpr int kTraceBlock = dev::kRingStride;  // threads per traversal block (2 wave64)
+constexpr int kDefaultRing = 16;          // LDS ring entries per lane (8, 16 or 32)
+constexpr int kDefaultChunk = 256;        // queue entries per traversal wave64
 
 // Wave launchers: `jobs` is the device job table, `multi` selects the
 // per-path job lookup (false: every path belongs to job 0).
-void launch_raygen(const Job *jobs, const Wave &w, int max_paths, bool multi, hipStream_t st);
-void launch_trace(const Job *jobs, const Wave &w, bool any, int stack_entries, int blocks, bool multi,
-                  hipStream_t st);
-int stack_variant(int bvh_depth);
-int trace_blocks_per_cu(int stack_entries, bool any);
+void launch_raygen(const Job *jobs, const Wave &w, bool multi, hipStream_t st);
+// ring: LDS ring entries per lane (8, 16 or 32); chunk: queue entries per
+// wave64.  w.spill must hold w.spill_cap entries for each of
+// trace_grid(w.nseg, w.seg, any, chunk) * kTraceBlock threads.
+void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk, bool multi, hipStream_t st);
+int ring_variant(int ring);
+int trace_chunk(int chunk);
+// blocks of kTraceBlock threads covering nseg queue segments (closest
+// queues: 2 seg entries per segment; any-hit: seg)
+int trace_grid(int nseg, int seg, bool any, int chunk);
+// spill area (ints) for `blocks` traversal blocks; the per-thread capacity
+// is max(0, stack_bound - ring)
+size_t trace_spill_ints(int stack_bound, int ring, int blocks);
+int trace_waves_per_simd(int ring);  // occupancy of the closest-hit kernel
+// single-queue waves (SPPM): account the traced rays, reset the queues
 void launch_between(const Wave &w, int nxt, hipStream_t st);
-void launch_shade(const Job *jobs, const Wave &w, int cur, int max_paths, bool multi, hipStream_t st);
+void launch_shade(const Job *jobs, const Wave &w, int cur, bool multi, hipStream_t st);
 void launch_film(const Job *jobs, const Wave &w, int max_pix, bool multi, hipStream_t st);
 void launch_intersect_batch(const DScene &sc, int n, const float *rays, float *t, int *prim, float *u, float *v,
-                            int *tie, int *occluded, bool any, hipStream_t st);
+                            int *tie, int *occluded, bool any, int *spill, int spill_cap,
+                            unsigned long long *stats, hipStream_t st);
 
 }  // namespace orion_amd

@@ -34,16 +34,21 @@ __device__ __forceinline__ int find_job(const Job *jobs, int njobs, int p) {
   return lo;
 }
 
+// One thread per path slot of the wave's capacity: camera rays for the
+// wave's paths, and every segment's initial queue lengths (all paths in the
+// path and closest queues, none in the shadow queue).
 template <bool MULTI>
 __global__ void k_raygen(const Job *jobs, Wave w) {
   const int npaths = w.ctrl[C_NPATHS];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) {
-    w.ctrl[C_PATH0] = npaths;
-    w.ctrl[C_RAY] = npaths;
-    w.ctrl[C_SHADOW] = 0;
-    w.ctrl[C_WORK_RAY] = 0;
-    w.ctrl[C_WORK_SHADOW] = 0;
+  if (p >= w.nseg * w.seg) return;
+  if (p % w.seg == 0) {
+    const int sg = p / w.seg;
+    const int n = max(0, min(w.seg, npaths - p));
+    w.cnt_path[0][sg] = n;
+    w.cnt_ray[sg] = n;
+    w.cnt_shadow[sg] = 0;
+    stat_add(w.stats, S_CLOSEST, (unsigned long long)n);
   }
   if (p >= npaths) return;
   int jb = MULTI ? find_job(jobs, w.ctrl[C_NJOBS], p) : 0;
@@ -63,54 +68,78 @@ __global__ void k_raygen(const Job *jobs, Wave w) {
   w.bounce[p] = 0;
   w.flags[p] = F_ALIVE;
   if (MULTI) w.job[p] = jb;
+  const int sg = p / w.seg, j0 = p - sg * w.seg;
   w.qpath[0][p] = p;
-  w.qray[p] = p;
+  w.qray[2 * sg * w.seg + j0] = p;
 }
 
 // ---------------------------------------------------------------- traversal
-// Persistent: each wave pulls 64-ray tiles from the queue until it drains.
-template <bool ANY, int STACK, bool MULTI>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w) {
-  __shared__ int stack[STACK * kTraceBlock];
-  int *stk = stack + threadIdx.x;
-  const int count = w.ctrl[ANY ? C_SHADOW : C_RAY];
-  int *work = w.ctrl + (ANY ? C_WORK_SHADOW : C_WORK_RAY);
-  const int *queue = ANY ? w.qshadow : w.qray;
-  int lane = lane_id();
-  uint32_t nv = 0, nt = 0;
-  for (;;) {
-    int base = 0;
-    if (lane == 0) base = atomicAdd(work, 64);
-    base = __shfl(base, 0);
-    if (base >= count) break;
-    int q = base + lane;
-    if (q < count) {
-      int r = queue[q];
-      int slot = ANY ? 2 * w.N + r : r;  // shadow rays live in slot 2
-      int p = ANY ? r : (r >= w.N ? r - w.N : r);
-      const DScene &sc = jobs[MULTI ? w.job[p] : 0].sc;
-      V3 o = mk(w.ox[slot], w.oy[slot], w.oz[slot]);
-      V3 d = mk(w.dx[slot], w.dy[slot], w.dz[slot]);
-      HitRec h;
-      bool hit = traverse<ANY>(sc, o, d, 0.0f, w.tmax[slot], &h, stk, kTraceBlock, &nv, &nt);
-      if (ANY) {
-        w.vis[r] = hit ? 0 : 1;
-      } else {
-        w.hprim[r] = hit ? h.prim : -1;
-        w.ht[r] = h.t;
-        w.hu[r] = h.u;
-        w.hv[r] = h.v;
-      }
+// Wave queues as traversal-engine IO.  Closest rays: entries r < N are
+// continuation rays (slot 0), N + p MIS rays (slot 1); shadow entries are
+// path ids whose ray lives in slot 2.
+template <bool ANY, bool MULTI>
+struct WaveRays {
+  const Job *jobs;
+  const Wave *w;
+  int base;  // first queue entry of the segment
+  __device__ void load(int q, TRay *ray) const {
+    const int r = (ANY ? w->qshadow : w->qray)[base + q];
+    const int slot = ANY ? 2 * w->N + r : r;
+    const int p = ANY ? r : (r >= w->N ? r - w->N : r);
+    const DScene &sc = jobs[MULTI ? w->job[p] : 0].sc;
+    ray->o = mk(w->ox[slot], w->oy[slot], w->oz[slot]);
+    ray->d = mk(w->dx[slot], w->dy[slot], w->dz[slot]);
+    ray->tmin = 0.0f;
+    ray->tmax = w->tmax[slot];
+    ray->nodes = sc.nodes;
+    ray->tris = sc.tris;
+    ray->tag = r;
+  }
+  __device__ void store(const TRay &ray, bool hit, const HitRec &h) const {
+    const int r = ray.tag;
+    if (ANY) {
+      w->vis[r] = hit ? 0 : 1;
+    } else {
+      w->hprim[r] = hit ? h.prim : -1;
+      w->ht[r] = hit ? h.t : 0.f;
+      w->hu[r] = hit ? h.u : 0.f;
+      w->hv[r] = hit ? h.v : 0.f;
     }
   }
+};
+
+// Wave64 number g traces entries [c * chunk, (c + 1) * chunk) of queue
+// segment g / (waves per segment), c = g mod (waves per segment); the grid
+// covers every segment at full length and waves past a segment's length exit.
+template <bool ANY, int RING, bool MULTI>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w, int chunk) {
+  __shared__ int ring[RING * kTraceBlock];
+  const int seg_entries = ANY ? w.seg : 2 * w.seg;
+  const int wps = (seg_entries + chunk - 1) / chunk;
+  const int gw = xcd_block() * (kTraceBlock / 64) + (int)(threadIdx.x >> 6);
+  const int sg = gw / wps;
+  if (sg >= w.nseg) return;
+  const int count = (ANY ? w.cnt_shadow : w.cnt_ray)[sg];
+  const int begin = (gw - sg * wps) * chunk;
+  if (begin >= count) return;
+  TravStack<RING> stk;
+  stk.ring = ring + threadIdx.x;
+  stk.spill_stride = gridDim.x * kTraceBlock;
+  stk.spill = w.spill + blockIdx.x * kTraceBlock + threadIdx.x;
+  stk.spill_cap = w.spill_cap;
+  uint32_t ovf = 0, nv = 0, nt = 0;
+  WaveRays<ANY, MULTI> io{jobs, &w, sg * seg_entries};
+  trace_engine<ANY, RING>(io, begin, min(begin + chunk, count), stk, &nv, &nt, &ovf);
+  const int lane = (int)__lane_id();
   // per-wave reduction of traversal counters (for the roofline's n_node / n_tri)
   for (int off = 32; off > 0; off >>= 1) {
     nv += __shfl_down(nv, off);
     nt += __shfl_down(nt, off);
   }
-  if (lane == 0 && w.stats) {
-    atomicAdd(&w.stats[ANY ? S_NODES_S : S_NODES_C], (unsigned long long)nv);
-    atomicAdd(&w.stats[ANY ? S_TRIS_S : S_TRIS_C], (unsigned long long)nt);
+  if (ovf) stat_add(w.stats, S_OVERFLOW, ovf);
+  if (lane == 0) {
+    stat_add(w.stats, ANY ? S_NODES_S : S_NODES_C, nv);
+    stat_add(w.stats, ANY ? S_TRIS_S : S_TRIS_C, nt);
   }
 }
 
@@ -118,10 +147,8 @@ __global__ void k_between(Wave w, int nxt) {
   // runs after both traces, before shade: account the rays just traced and
   // reset the queues the shade kernel appends to
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    if (w.stats) {
-      w.stats[S_CLOSEST] += (unsigned long long)w.ctrl[C_RAY];
-      w.stats[S_SHADOW] += (unsigned long long)w.ctrl[C_SHADOW];
-    }
+    stat_add(w.stats, S_CLOSEST, (unsigned long long)w.ctrl[C_RAY]);
+    stat_add(w.stats, S_SHADOW, (unsigned long long)w.ctrl[C_SHADOW]);
     w.ctrl[C_RAY] = 0;
     w.ctrl[C_SHADOW] = 0;
     w.ctrl[C_WORK_RAY] = 0;
@@ -132,14 +159,22 @@ __global__ void k_between(Wave w, int nxt) {
 
 // ---------------------------------------------------------------- shade
 template <bool MULTI>
-__global__ __launch_bounds__(256) void k_shade(const Job *jobs, Wave w, int cur) {
+// Block sg shades the live paths of segment sg and writes the segment's
+// next path, closest-ray and shadow-ray queues.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_shade(const Job *jobs, Wave w, int cur) {
+  __shared__ int lcnt[3];  // next path queue, closest rays, shadow rays
   const int nxt = cur ^ 1;
-  const int count = w.ctrl[C_PATH0 + cur];
+  const int sg = xcd_block();
+  const int seg0 = sg * w.seg;
+  const int count = w.cnt_path[cur][sg];
   const int N = w.N;
-  for (int base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+  if (threadIdx.x < 3) lcnt[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t n_done = 0, n_mis = 0;
+  for (int base = 0; base < count; base += blockDim.x) {
     int i = base + threadIdx.x;
     bool act = i < count;
-    int p = act ? w.qpath[cur][i] : 0;
+    int p = act ? w.qpath[cur][seg0 + i] : 0;
     bool want_ray0 = false, want_ray1 = false, want_shadow = false, keep = false, done = false;
     if (act) {
       const Job &J = jobs[MULTI ? w.job[p] : 0];
@@ -212,20 +247,38 @@ __global__ __launch_bounds__(256) void k_shade(const Job *jobs, Wave w, int cur)
       keep = (fl & (F_ALIVE | F_PEND)) != 0;
       done = !keep;
     }
-    int a = wave_append(&w.ctrl[C_PATH0 + nxt], keep);
-    if (keep) w.qpath[nxt][a] = p;
-    a = wave_append(&w.ctrl[C_RAY], want_ray0);
-    if (want_ray0) w.qray[a] = p;
-    a = wave_append(&w.ctrl[C_RAY], want_ray1);
-    if (want_ray1) w.qray[a] = N + p;
-    a = wave_append(&w.ctrl[C_SHADOW], want_shadow);
-    if (want_shadow) w.qshadow[a] = p;
-    unsigned long long dm = __ballot(done);
-    unsigned long long mm = __ballot(want_ray1);
-    if (lane_id() == 0 && w.stats) {
-      if (dm) atomicAdd(&w.stats[S_PATHS], (unsigned long long)__popcll(dm));
-      if (mm) atomicAdd(&w.stats[S_REFEQ], (unsigned long long)__popcll(mm));
-    }
+    int a = lds_append(&lcnt[0], keep);
+    if (keep) w.qpath[nxt][seg0 + a] = p;
+    a = lds_append(&lcnt[1], want_ray0);
+    if (want_ray0) w.qray[2 * seg0 + a] = p;
+    a = lds_append(&lcnt[1], want_ray1);
+    if (want_ray1) w.qray[2 * seg0 + a] = N + p;
+    a = lds_append(&lcnt[2], want_shadow);
+    if (want_shadow) w.qshadow[seg0 + a] = p;
+    n_done += done ? 1 : 0;
+    n_mis += want_ray1 ? 1 : 0;
+  }
+  // block totals: queue lengths of the segment and striped statistics
+  for (int off = 32; off > 0; off >>= 1) {
+    n_done += __shfl_down(n_done, off);
+    n_mis += __shfl_down(n_mis, off);
+  }
+  __shared__ uint32_t ltot[2];
+  if (threadIdx.x < 2) ltot[threadIdx.x] = 0;
+  __syncthreads();
+  if (lane_id() == 0) {
+    atomicAdd(&ltot[0], n_done);
+    atomicAdd(&ltot[1], n_mis);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    w.cnt_path[nxt][sg] = lcnt[0];
+    w.cnt_ray[sg] = lcnt[1];
+    w.cnt_shadow[sg] = lcnt[2];
+    stat_add(w.stats, S_CLOSEST, (unsigned long long)lcnt[1]);
+    stat_add(w.stats, S_SHADOW, (unsigned long long)lcnt[2]);
+    stat_add(w.stats, S_PATHS, ltot[0]);
+    stat_add(w.stats, S_REFEQ, ltot[1]);
   }
 }
 
@@ -262,80 +315,112 @@ __global__ void k_film(const Job *jobs, Wave w) {
 }
 
 // ---------------------------------------------------------------- batch intersection API
-template <bool ANY, int STACK>
+struct SoaRays {
+  const DScene *sc;
+  int n;
+  const float *rays;
+  float *t, *u, *v;
+  int *prim, *tie, *occluded;
+  __device__ void load(int i, TRay *ray) const {
+    ray->o = mk(rays[i], rays[n + i], rays[2 * n + i]);
+    ray->d = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+    ray->tmin = rays[6 * n + i];
+    ray->tmax = rays[7 * n + i];
+    ray->nodes = sc->nodes;
+    ray->tris = sc->tris;
+    ray->tag = i;
+  }
+  __device__ void store(const TRay &ray, bool hit, const HitRec &h) const {
+    const int i = ray.tag;
+    if (occluded) {
+      occluded[i] = hit ? 1 : 0;
+    } else {
+      t[i] = hit ? h.t : 0.f;
+      prim[i] = hit ? h.prim : -1;
+      if (u) u[i] = hit ? h.u : 0.f;
+      if (v) v[i] = hit ? h.v : 0.f;
+      if (tie) tie[i] = hit ? h.tie : 0;
+    }
+  }
+};
+
+template <bool ANY, int RING>
 __global__ __launch_bounds__(kTraceBlock) void k_intersect_batch(DScene sc, int n, const float *rays, float *t,
                                                                  int *prim, float *u, float *v, int *tie,
-                                                                 int *occluded) {
-  __shared__ int stack[STACK * kTraceBlock];
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  V3 o = mk(rays[i], rays[n + i], rays[2 * n + i]);
-  V3 d = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
-  float tmin = rays[6 * n + i], tmx = rays[7 * n + i];
-  HitRec h;
-  bool hit = traverse<ANY>(sc, o, d, tmin, tmx, &h, stack + threadIdx.x, kTraceBlock, nullptr, nullptr);
-  if (ANY) {
-    occluded[i] = hit ? 1 : 0;
-  } else {
-    t[i] = hit ? h.t : 0.f;
-    prim[i] = hit ? h.prim : -1;
-    if (u) u[i] = hit ? h.u : 0.f;
-    if (v) v[i] = hit ? h.v : 0.f;
-    if (tie) tie[i] = hit ? h.tie : 0;
-  }
+                                                                 int *occluded, int *spill, int spill_cap,
+                                                                 int chunk, unsigned long long *stats) {
+  __shared__ int ring[RING * kTraceBlock];
+  const int begin = (blockIdx.x * (kTraceBlock / 64) + (int)(threadIdx.x >> 6)) * chunk;
+  if (begin >= n) return;
+  TravStack<RING> stk;
+  stk.ring = ring + threadIdx.x;
+  stk.spill_stride = gridDim.x * kTraceBlock;
+  stk.spill = spill + blockIdx.x * kTraceBlock + threadIdx.x;
+  stk.spill_cap = spill_cap;
+  uint32_t ovf = 0, nv = 0, nt = 0;
+  SoaRays io{&sc, n, rays, t, u, v, prim, tie, ANY ? occluded : nullptr};
+  trace_engine<ANY, RING>(io, begin, min(begin + chunk, n), stk, &nv, &nt, &ovf);
+  if (ovf) stat_add(stats, S_OVERFLOW, ovf);
 }
 
 // ---------------------------------------------------------------- launchers
-// Stack-depth variants: the smallest that holds the BVH depth + 2, so LDS
-// (STACK x 128 lanes x 4 B per block) caps occupancy no more than needed.
-#define ORION_STACK_DISPATCH(S, CALL)                     \
-  do {                                                    \
-    if ((S) <= 16) { constexpr int ST = 16; CALL; }       \
-    else if ((S) <= 24) { constexpr int ST = 24; CALL; }  \
-    else if ((S) <= 32) { constexpr int ST = 32; CALL; }  \
-    else { constexpr int ST = kStackEntries; CALL; }      \
+// LDS ring variants: RING x 128 lanes x 4 B of LDS per block.
+#define ORION_RING_DISPATCH(S, CALL)                     \
+  do {                                                   \
+    if ((S) <= 8) { constexpr int ST = 8; CALL; }        \
+    else if ((S) <= 16) { constexpr int ST = 16; CALL; } \
+    else { constexpr int ST = 32; CALL; }                \
   } while (0)
 
-int stack_variant(int depth) {
-  int need = depth + 2;
-  return need <= 16 ? 16 : need <= 24 ? 24 : need <= 32 ? 32 : kStackEntries;
+int ring_variant(int ring) {
+  if (ring <= 0) return kDefaultRing;
+  return ring <= 8 ? 8 : ring <= 16 ? 16 : 32;
 }
 
-int trace_blocks_per_cu(int stack_entries, bool any) {
-  static int cache[2][4] = {{0}};
-  int vi = stack_entries <= 16 ? 0 : stack_entries <= 24 ? 1 : stack_entries <= 32 ? 2 : 3;
-  int &c = cache[any ? 1 : 0][vi];
-  if (c == 0) {
-    int nb = 0;
-    hipError_t e = hipSuccess;
-    if (any) ORION_STACK_DISPATCH(stack_entries, e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<true, ST, true>, kTraceBlock, 0));
-    else ORION_STACK_DISPATCH(stack_entries, e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false, ST, true>, kTraceBlock, 0));
-    c = (e == hipSuccess && nb > 0) ? nb : 4;
-  }
-  return c;
+int trace_waves_per_simd(int ring) {
+  ring = ring_variant(ring);
+  int nb = 0;
+  hipError_t e = hipSuccess;
+  ORION_RING_DISPATCH(ring, e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false, ST, false>, kTraceBlock, 0));
+  return e == hipSuccess ? nb * (kTraceBlock / 64) / 4 : 0;
 }
 
-void launch_raygen(const Job *jobs, const Wave &w, int max_paths, bool multi, hipStream_t st) {
-  dim3 g((max_paths + 255) / 256);
+size_t trace_spill_ints(int stack_bound, int ring, int blocks) {
+  int cap = std::max(0, stack_bound - ring_variant(ring));
+  return (size_t)cap * blocks * kTraceBlock;
+}
+
+int trace_chunk(int chunk) { return chunk > 0 ? std::min(std::max(chunk, 64), 4096) : kDefaultChunk; }
+
+int trace_grid(int nseg, int seg, bool any, int chunk) {
+  chunk = trace_chunk(chunk);
+  const int64_t seg_entries = any ? seg : 2 * (int64_t)seg;
+  const int64_t waves = nseg * ((seg_entries + chunk - 1) / chunk);
+  return (int)std::max<int64_t>(1, (waves + kTraceBlock / 64 - 1) / (kTraceBlock / 64));
+}
+
+void launch_raygen(const Job *jobs, const Wave &w, bool multi, hipStream_t st) {
+  dim3 g(((size_t)w.nseg * w.seg + 255) / 256);
   if (multi) hipLaunchKernelGGL(k_raygen<true>, g, dim3(256), 0, st, jobs, w);
   else hipLaunchKernelGGL(k_raygen<false>, g, dim3(256), 0, st, jobs, w);
 }
-void launch_trace(const Job *jobs, const Wave &w, bool any, int stack_entries, int blocks, bool multi,
-                  hipStream_t st) {
+void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk, bool multi, hipStream_t st) {
+  ring = ring_variant(ring);
+  chunk = trace_chunk(chunk);
+  const dim3 g(trace_grid(w.nseg, w.seg, any, chunk));
   if (multi) {
-    if (any) ORION_STACK_DISPATCH(stack_entries, hipLaunchKernelGGL((k_trace<true, ST, true>), dim3(blocks), dim3(kTraceBlock), 0, st, jobs, w));
-    else ORION_STACK_DISPATCH(stack_entries, hipLaunchKernelGGL((k_trace<false, ST, true>), dim3(blocks), dim3(kTraceBlock), 0, st, jobs, w));
+    if (any) ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<true, ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
+    else ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<false, ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
   } else {
-    if (any) ORION_STACK_DISPATCH(stack_entries, hipLaunchKernelGGL((k_trace<true, ST, false>), dim3(blocks), dim3(kTraceBlock), 0, st, jobs, w));
-    else ORION_STACK_DISPATCH(stack_entries, hipLaunchKernelGGL((k_trace<false, ST, false>), dim3(blocks), dim3(kTraceBlock), 0, st, jobs, w));
+    if (any) ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<true, ST, false>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
+    else ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<false, ST, false>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
   }
 }
 void launch_between(const Wave &w, int nxt, hipStream_t st) {
   hipLaunchKernelGGL(k_between, dim3(1), dim3(64), 0, st, w, nxt);
 }
-void launch_shade(const Job *jobs, const Wave &w, int cur, int max_paths, bool multi, hipStream_t st) {
-  int blocks = (max_paths + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
+void launch_shade(const Job *jobs, const Wave &w, int cur, bool multi, hipStream_t st) {
+  const int blocks = w.nseg;  // one block per segment
   if (multi) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, st, jobs, w, cur);
   else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, st, jobs, w, cur);
 }
@@ -346,10 +431,17 @@ void launch_film(const Job *jobs, const Wave &w, int max_pix, bool multi, hipStr
   else hipLaunchKernelGGL(k_film<false>, dim3(blocks), dim3(256), 0, st, jobs, w);
 }
 void launch_intersect_batch(const DScene &sc, int n, const float *rays, float *t, int *prim, float *u, float *v,
-                            int *tie, int *occluded, bool any, hipStream_t st) {
-  dim3 g((n + kTraceBlock - 1) / kTraceBlock);
-  if (any) ORION_STACK_DISPATCH(sc.stack_entries, hipLaunchKernelGGL((k_intersect_batch<true, ST>), g, dim3(kTraceBlock), 0, st, sc, n, rays, t, prim, u, v, tie, occluded));
-  else ORION_STACK_DISPATCH(sc.stack_entries, hipLaunchKernelGGL((k_intersect_batch<false, ST>), g, dim3(kTraceBlock), 0, st, sc, n, rays, t, prim, u, v, tie, occluded));
+                            int *tie, int *occluded, bool any, int *spill, int spill_cap,
+                            unsigned long long *stats, hipStream_t st) {
+  constexpr int ST = kDefaultRing;
+  const int chunk = kDefaultChunk;
+  const dim3 g(trace_grid(1, n, true, chunk));
+  if (any)
+    hipLaunchKernelGGL((k_intersect_batch<true, ST>), g, dim3(kTraceBlock), 0, st, sc, n, rays, t, prim, u, v, tie,
+                       occluded, spill, spill_cap, chunk, stats);
+  else
+    hipLaunchKernelGGL((k_intersect_batch<false, ST>), g, dim3(kTraceBlock), 0, st, sc, n, rays, t, prim, u, v, tie,
+                       occluded, spill, spill_cap, chunk, stats);
 }
 
 }  // namespace orion_amd

@@ -35,6 +35,8 @@ struct SppmBuffers;
 void destroy_sppm(SppmBuffers *b);
 struct WaveRunner;
 void destroy_runner(WaveRunner *r);
+// sums the striped device statistics (kStatBytes at dev) into out[S_COUNT]
+int read_striped_stats(const void *dev, unsigned long long *out);
 
 }  // namespace orion_amd
 
@@ -44,7 +46,7 @@ struct orion_ctx_s {
   std::string arch;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  orion_amd::WaveRunner *runner = nullptr;  // batched launches
+  orion_amd::WaveRunner *runner = nullptr;  // wave buffers of every render on this stream
 };
 
 struct orion_scene_s {
@@ -53,10 +55,10 @@ struct orion_scene_s {
   orion_amd::Bvh bvh;
   orion_amd::DevBuf nodes, tris, shade, mats, lights, light_cdf, light_func, light_area, sobol;
   orion_amd::DevBuf film, count, ctrl, stats;
+  orion_amd::DevBuf spill;  // traversal stack spill area of the intersection API
   orion_amd::DScene d{};
   orion_amd::WaveBuffers wave;
   orion_amd::SppmBuffers *sppm = nullptr;
-  orion_amd::WaveRunner *runner = nullptr;  // plain path renders
   double build_ms = 0, upload_ms = 0;
   ~orion_scene_s();
 };

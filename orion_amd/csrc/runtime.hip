@@ -19,6 +19,28 @@
 #define ORION_SOBOL_STORAGE static const
 #include "sobol_matrices.inc"
 
+namespace {
+// Sobol generator matrices as byte tables: entry [dim][j][v] is the XOR of
+// the columns 8 j + b of dimension dim for the set bits b of v, so a sample
+// is the XOR of four independent lookups (one per byte of the index)
+// instead of a chain over the index's set bits.
+const std::vector<uint32_t> &sobol_byte_tables() {
+  static const std::vector<uint32_t> tab = [] {
+    std::vector<uint32_t> t((size_t)1024 * 4 * 256);
+    for (int dim = 0; dim < 1024; ++dim)
+      for (int j = 0; j < 4; ++j)
+        for (int v = 0; v < 256; ++v) {
+          uint32_t r = 0;
+          for (int b = 0; b < 8; ++b)
+            if (v >> b & 1) r ^= orion_sobol_matrices[dim * 32 + 8 * j + b];
+          t[((size_t)dim * 4 + j) * 256 + v] = r;
+        }
+    return t;
+  }();
+  return tab;
+}
+}  // namespace
+
 namespace orion_amd {
 
 #define HIP_TRY(expr)                                                                   \
@@ -60,7 +82,8 @@ int WaveBuffers::ensure(int N) {
   // floats/ints per path: rays 7 x 3 slots, hits 4 x 2, vis 1, state 9,
   // pending 15, queues 2 + 2 + 1
   const size_t per = 21 + 8 + 1 + 9 + 15 + 5 + 1;
-  int rc = mem.ensure(per * (size_t)N * 4 + 256);
+  const size_t nseg = ((size_t)N + kSegMin - 1) / kSegMin + 1;
+  int rc = mem.ensure(per * (size_t)N * 4 + 4 * nseg * 4 + 256);
   if (rc) return rc;
   cap = N;
   char *b = (char *)mem.ptr;
@@ -87,6 +110,24 @@ int WaveBuffers::ensure(int N) {
   w.qpath[0] = (int *)take(N); w.qpath[1] = (int *)take(N);
   w.qray = (int *)take(2 * (size_t)N);
   w.qshadow = (int *)take(N);
+  choose_segments(N, &w.seg, &w.nseg);
+  w.cnt_path[0] = (int *)take(nseg); w.cnt_path[1] = (int *)take(nseg);
+  w.cnt_ray = (int *)take(nseg); w.cnt_shadow = (int *)take(nseg);
+  return ORION_OK;
+}
+
+int read_striped_stats(const void *dev, unsigned long long *out) {
+  std::vector<unsigned long long> h((size_t)S_COUNT * kStatStripes);
+  hipError_t e = hipMemcpy(h.data(), dev, kStatBytes, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMemcpy (statistics): ") + hipGetErrorString(e));
+    return ORION_ERR_HIP;
+  }
+  for (int s = 0; s < S_COUNT; ++s) {
+    unsigned long long v = 0;
+    for (int k = 0; k < kStatStripes; ++k) v += h[(size_t)s * kStatStripes + k];
+    out[s] = v;
+  }
   return ORION_OK;
 }
 
@@ -227,11 +268,12 @@ int orion_scene_upload(orion_ctx c, orion_hscene hs, orion_scene *out) {
   if ((rc = up(sc->light_cdf, cdf.data(), cdf.size() * 4))) return fail(rc);
   if ((rc = up(sc->light_func, func.data(), func.size() * 4))) return fail(rc);
   if ((rc = up(sc->light_area, area.data(), area.size() * 4))) return fail(rc);
-  if ((rc = up(sc->sobol, orion_sobol_matrices, sizeof(orion_sobol_matrices)))) return fail(rc);
+  const std::vector<uint32_t> &sob = sobol_byte_tables();
+  if ((rc = up(sc->sobol, sob.data(), sob.size() * 4))) return fail(rc);
   if ((rc = sc->film.ensure((size_t)W * H * 3 * 4))) return fail(rc);
   if ((rc = sc->count.ensure((size_t)W * H * 4))) return fail(rc);
   if ((rc = sc->ctrl.ensure(64 * 4))) return fail(rc);
-  if ((rc = sc->stats.ensure(S_COUNT * 8))) return fail(rc);
+  if ((rc = sc->stats.ensure(kStatBytes))) return fail(rc);
   hipError_t e1 = hipMemset(sc->film.ptr, 0, (size_t)W * H * 3 * 4);
   hipError_t e2 = hipMemset(sc->count.ptr, 0, (size_t)W * H * 4);
   hipError_t e3 = hipMemset(sc->ctrl.ptr, 0, 64 * 4);
@@ -251,7 +293,7 @@ int orion_scene_upload(orion_ctx c, orion_hscene hs, orion_scene *out) {
   d.light_func_int = funcInt;
   d.n_lights = nl;
   d.n_prims = n;
-  d.stack_entries = stack_variant(sc->bvh.depth);
+  d.stack_bound = sc->bvh.stack_bound;
   std::memcpy(d.cam, s.camera, sizeof(d.cam));
   sc->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   sc->upload_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
@@ -275,9 +317,25 @@ int orion_scene_get_stats(orion_scene s, orion_scene_stats *st) {
   st->tri_bytes = st->n_tris * (int64_t)sizeof(BvhTri);
   st->shade_bytes = st->n_tris * 80;
   st->bvh_depth = s->bvh.depth;
-  st->node_format = 2;
+  st->node_format = 4;
   st->build_ms = s->build_ms;
   st->upload_ms = s->upload_ms;
+  st->stack_bound = s->d.stack_bound;
+  HIP_TRY(hipSetDevice(s->ctx->device));
+  st->trace_waves_per_simd = trace_waves_per_simd(kDefaultRing);
+  return ORION_OK;
+}
+
+// Persistent intersection launch with the scene's own stack spill area.
+static int intersect_launch(orion_scene s, int n, const float *rays, float *t, int *prim, float *u, float *v,
+                            int *tie, int *occ, bool any) {
+  orion_ctx c = s->ctx;
+  int rc = s->spill.ensure(
+      trace_spill_ints(s->d.stack_bound, kDefaultRing, trace_grid(1, n, true, kDefaultChunk)) * 4 + 4);
+  if (rc) return rc;
+  launch_intersect_batch(s->d, n, rays, t, prim, u, v, tie, occ, any, (int *)s->spill.ptr,
+                         std::max(0, s->d.stack_bound - kDefaultRing), nullptr, c->stream);
+  HIP_TRY(hipGetLastError());
   return ORION_OK;
 }
 
@@ -288,8 +346,8 @@ int orion_intersect_closest(orion_scene s, int n, const float *rays, float *t, i
   orion_ctx c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
   if (on_device) {
-    launch_intersect_batch(s->d, n, rays, t, prim, u, v, tie, nullptr, false, c->stream);
-    HIP_TRY(hipGetLastError());
+    int rc = intersect_launch(s, n, rays, t, prim, u, v, tie, nullptr, false);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return ORION_OK;
   }
@@ -300,8 +358,7 @@ int orion_intersect_closest(orion_scene s, int n, const float *rays, float *t, i
   float *dt = dr + 8 * (size_t)n, *du = dt + n, *dv = du + n;
   int *dp = (int *)(dv + n), *dtie = dp + n;
   HIP_TRY(hipMemcpyAsync(dr, rays, (size_t)n * 8 * 4, hipMemcpyHostToDevice, c->stream));
-  launch_intersect_batch(s->d, n, dr, dt, dp, du, dv, dtie, nullptr, false, c->stream);
-  HIP_TRY(hipGetLastError());
+  if ((rc = intersect_launch(s, n, dr, dt, dp, du, dv, dtie, nullptr, false))) return rc;
   HIP_TRY(hipMemcpyAsync(t, dt, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(prim, dp, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   if (u) HIP_TRY(hipMemcpyAsync(u, du, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -317,8 +374,8 @@ int orion_intersect_any(orion_scene s, int n, const float *rays, int *occluded, 
   orion_ctx c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
   if (on_device) {
-    launch_intersect_batch(s->d, n, rays, nullptr, nullptr, nullptr, nullptr, nullptr, occluded, true, c->stream);
-    HIP_TRY(hipGetLastError());
+    int rc = intersect_launch(s, n, rays, nullptr, nullptr, nullptr, nullptr, nullptr, occluded, true);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return ORION_OK;
   }
@@ -328,8 +385,7 @@ int orion_intersect_any(orion_scene s, int n, const float *rays, int *occluded, 
   float *dr = (float *)buf.ptr;
   int *docc = (int *)(dr + 8 * (size_t)n);
   HIP_TRY(hipMemcpyAsync(dr, rays, (size_t)n * 8 * 4, hipMemcpyHostToDevice, c->stream));
-  launch_intersect_batch(s->d, n, dr, nullptr, nullptr, nullptr, nullptr, nullptr, docc, true, c->stream);
-  HIP_TRY(hipGetLastError());
+  if ((rc = intersect_launch(s, n, dr, nullptr, nullptr, nullptr, nullptr, nullptr, docc, true))) return rc;
   HIP_TRY(hipMemcpyAsync(occluded, docc, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return ORION_OK;
@@ -355,7 +411,4 @@ int orion_film_read(orion_scene s, float *rgb, int *count) {
 
 }  // extern "C"
 
-orion_scene_s::~orion_scene_s() {
-  orion_amd::destroy_runner(runner);
-  orion_amd::destroy_sppm(sppm);
-}
+orion_scene_s::~orion_scene_s() { orion_amd::destroy_sppm(sppm); }

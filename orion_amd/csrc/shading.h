@@ -33,21 +33,21 @@ __device__ __forceinline__ int sample_light(const DScene &sc, float u, float *pd
   int first = 0, len = size;
   while (len > 0) {
     int half = len >> 1, mid = first + half;
-    if (sc.light_cdf[mid] <= u) { first = mid + 1; len -= half + 1; }
+    if (gld(sc.light_cdf + mid) <= u) { first = mid + 1; len -= half + 1; }
     else len = half;
   }
   int off = first - 1;
   off = off < 0 ? 0 : (off > size - 2 ? size - 2 : off);
-  *pdf = (sc.light_func_int > 0) ? sc.light_func[off] / (sc.light_func_int * sc.n_lights) : 0;
+  *pdf = (sc.light_func_int > 0) ? gld(sc.light_func + off) / (sc.light_func_int * sc.n_lights) : 0;
   return off;
 }
 
 __device__ __forceinline__ float spot_falloff(const float *L, V3 w) {  // spot.cpp:10-19
   V3 wl = normalize(mat3(L + 11, w));
   float cosTheta = wl.z;
-  if (cosTheta < L[9]) return 0;
-  if (cosTheta > L[10]) return 1;
-  float delta = (cosTheta - L[9]) / (L[10] - L[9]);
+  if (cosTheta < gld(L + 9)) return 0;
+  if (cosTheta > gld(L + 10)) return 1;
+  float delta = (cosTheta - gld(L + 9)) / (gld(L + 10) - gld(L + 9));
   return (delta * delta) * (delta * delta);
 }
 
@@ -68,13 +68,13 @@ __device__ __forceinline__ void tri_sample(const DScene &sc, int prim, float u0,
 __device__ inline Sp light_sample_li(const DScene &sc, int l, V3 refp, V3 refng, float u0, float u1, V3 *wi,
                                      float *pdf, V3 *so, V3 *sd) {
   const float *L = sc.lights + 32 * (size_t)l;
-  int type = (int)L[0];
-  Sp Le = sp3(L[2], L[3], L[4]);
+  int type = (int)gld(L + 0);
+  Sp Le = sp3(gld(L + 2), gld(L + 3), gld(L + 4));
   V3 target;
   if (type == 0) {  // diffuse.cpp:13-25 + triangle.cpp:108-121
     V3 p, ng;
-    tri_sample(sc, (int)L[1], u0, u1, &p, &ng);
-    float pd = 1.0f / sc.light_area[l];
+    tri_sample(sc, (int)gld(L + 1), u0, u1, &p, &ng);
+    float pd = 1.0f / gld(sc.light_area + l);
     V3 w = sub(p, refp);
     if (lensq(w) == 0) pd = 0;
     else {
@@ -88,7 +88,7 @@ __device__ inline Sp light_sample_li(const DScene &sc, int l, V3 refp, V3 refng,
     target = p;
     Le = dot(ng, neg(*wi)) > 0 ? Le : sp(0);
   } else {  // point.cpp:10-19, spot.cpp:20-29
-    V3 pos = mk(L[5], L[6], L[7]);
+    V3 pos = mk(gld(L + 5), gld(L + 6), gld(L + 7));
     *wi = normalize(sub(pos, refp));
     *pdf = 1.0f;
     target = pos;
@@ -104,12 +104,12 @@ __device__ inline Sp light_sample_li(const DScene &sc, int l, V3 refp, V3 refng,
 __device__ inline Sp light_sample_le(const DScene &sc, int l, float a0, float a1, float b0, float b1, V3 *ro,
                                      V3 *rd, V3 *nLight, float *pdfPos, float *pdfDir) {
   const float *L = sc.lights + 32 * (size_t)l;
-  int type = (int)L[0];
-  Sp Le = sp3(L[2], L[3], L[4]);
+  int type = (int)gld(L + 0);
+  Sp Le = sp3(gld(L + 2), gld(L + 3), gld(L + 4));
   if (type == 0) {
     V3 p, ng;
-    tri_sample(sc, (int)L[1], a0, a1, &p, &ng);
-    *pdfPos = 1.0f / sc.light_area[l];
+    tri_sample(sc, (int)gld(L + 1), a0, a1, &p, &ng);
+    *pdfPos = 1.0f / gld(sc.light_area + l);
     *nLight = ng;
     V3 w = cosine_hemisphere(b0, b1);
     *pdfDir = w.z * kInvPi;
@@ -120,7 +120,7 @@ __device__ inline Sp light_sample_le(const DScene &sc, int l, float a0, float a1
     *rd = w;
     return dot(ng, w) > 0 ? Le : sp(0);
   }
-  V3 pos = mk(L[5], L[6], L[7]);
+  V3 pos = mk(gld(L + 5), gld(L + 6), gld(L + 7));
   *ro = pos;
   *pdfPos = 1;
   if (type == 1) {
@@ -129,10 +129,10 @@ __device__ inline Sp light_sample_le(const DScene &sc, int l, float a0, float a1
     *pdfDir = kInv4Pi;
     return Le;
   }
-  V3 w = uniform_cone(a0, a1, L[9]);
+  V3 w = uniform_cone(a0, a1, gld(L + 9));
   *rd = mat3(L + 20, w);
   *nLight = *rd;
-  *pdfDir = 1 / (2 * kPi * (1 - L[9]));
+  *pdfDir = 1 / (2 * kPi * (1 - gld(L + 9)));
   return Le * spot_falloff(L, *rd);
 }
 
@@ -147,18 +147,18 @@ __device__ inline void resolve_direct(const DScene &sc, const Wave &w, int p, in
     int hp = w.hprim[N + p];
     int l = w.nlight[p];
     const float *LR = sc.lights + 32 * (size_t)l;
-    if (hp >= 0 && hp == (int)LR[1]) {
+    if (hp >= 0 && hp == (int)gld(LR + 1)) {
       // Triangle::pdf (triangle.cpp:123-135) on the MIS ray's hit
       Isect in;
       make_isect(sc, hp, w.hu[N + p], w.hv[N + p], &in);
       V3 wi = mk(w.dx[N + p], w.dy[N + p], w.dz[N + p]);
       V3 pp = mk(w.px[p], w.py[p], w.pz[p]);
-      float lpdf = lensq(sub(pp, in.p)) / (absdot(in.ng, neg(wi)) * sc.light_area[l]);
+      float lpdf = lensq(sub(pp, in.p)) / (absdot(in.ng, neg(wi)) * gld(sc.light_area + l));
       if (is_inf(lpdf)) lpdf = 0;
       if (lpdf != 0) {
         float bp = w.bpdf[p];
         float weight = power_heuristic(bp, lpdf);
-        Sp Lm = dot(in.ng, neg(wi)) > 0 ? sp3(LR[2], LR[3], LR[4]) : sp(0);
+        Sp Lm = dot(in.ng, neg(wi)) > 0 ? sp3(gld(LR + 2), gld(LR + 3), gld(LR + 4)) : sp(0);
         if (!black(Lm)) acc(Ld, sp3(w.fr[p], w.fg[p], w.fb[p]) * Lm * weight / bp);
       }
     }
@@ -190,7 +190,7 @@ __device__ inline int setup_direct(const DScene &sc, const Wave &w, int p, const
   V3 wi = mk(0, 0, 0), so, sdir;
   float lpdf = 0;
   Sp Li = light_sample_li(sc, l, is.p, is.ng, lu0, lu1, &wi, &lpdf, &so, &sdir);
-  bool delta = (int)sc.lights[32 * (size_t)l] != 0;
+  bool delta = (int)gld(sc.lights + 32 * (size_t)l) != 0;
   bool hs = false, hm = false;
   if (lpdf > 0 && !black(Li)) {
     Sp f = bsdf_f(bs, wi, wo, nonspec) * absdot(wi, is.ns);
@@ -234,7 +234,7 @@ __device__ inline int setup_direct(const DScene &sc, const Wave &w, int p, const
 
 __device__ __forceinline__ Sp area_emission(const DScene &sc, const Isect &is, V3 w) {  // diffuse.cpp:53-56
   const float *LR = sc.lights + 32 * (size_t)is.light;
-  return dot(is.ng, w) > 0 ? sp3(LR[2], LR[3], LR[4]) : sp(0);
+  return dot(is.ng, w) > 0 ? sp3(gld(LR + 2), gld(LR + 3), gld(LR + 4)) : sp(0);
 }
 
 }  // namespace dev

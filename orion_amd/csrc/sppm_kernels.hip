@@ -66,7 +66,7 @@ struct Grid {
 };
 
 struct SppmBuffers {
-  DevBuf pix_mem, grid_mem, pair_mem, hist_mem, scan_mem, job;
+  DevBuf pix_mem, grid_mem, pair_mem, hist_mem, scan_mem, job, spill;
   Pix px{};
   Grid g{};
   int npix = 0, hash_size = 0;
@@ -496,7 +496,7 @@ __global__ void k_photon_init(DScene sc, Wave w, uint32_t index0, int P) {
 
 __global__ void k_photon_between(Wave w, unsigned long long *stats) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    if (stats) stats[S_CLOSEST] += (unsigned long long)w.ctrl[C_RAY];
+    stat_add(stats, S_CLOSEST, (unsigned long long)w.ctrl[C_RAY]);
     w.ctrl[C_PATH0] = w.ctrl[C_RAY];  // photons to shade = photons just traced
     w.ctrl[C_SHADOW] = 0;             // next queue length
     w.ctrl[C_WORK_RAY] = 0;
@@ -573,7 +573,7 @@ __global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px,
     if (keep) w.qpath[0][a] = p;
   }
   for (int off = 32; off > 0; off >>= 1) ndep += __shfl_down(ndep, off);
-  if (lane_id() == 0 && ndep && stats) atomicAdd(&stats[S_DEPOSITS], ndep);
+  if (lane_id() == 0) stat_add(stats, S_DEPOSITS, ndep);
 }
 
 __global__ void k_photon_swap(Wave w) {
@@ -755,12 +755,19 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   if ((rc = b.job.ensure(sizeof(Job)))) return rc;
   HIP_TRY(hipMemcpy(b.job.ptr, &job1, sizeof(Job), hipMemcpyHostToDevice));
   const Job *dj = (const Job *)b.job.ptr;
-  const int stack = s->d.stack_entries;
-  HIP_TRY(hipMemsetAsync(s->stats.ptr, 0, S_COUNT * 8, st));
+  const int ring = kDefaultRing;
+  HIP_TRY(hipMemsetAsync(s->stats.ptr, 0, kStatBytes, st));
   HIP_TRY(hipEventRecord(c->ev0, st));
   const int TB = 256;
   hipLaunchKernelGGL(k_pix_init, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, b.px, npix, r0);
-  int trace_blocks = c->n_cu * trace_blocks_per_cu(s->d.stack_entries, false);
+  const int chunk = kDefaultChunk;
+  {
+    int blocks = trace_grid(1, npix, false, chunk);
+    if (photons > 0) blocks = std::max(blocks, trace_grid(1, photons, false, chunk));
+    if ((rc = b.spill.ensure(trace_spill_ints(s->d.stack_bound, ring, blocks) * 4 + 4))) return rc;
+  }
+  w.spill = (int *)b.spill.ptr;
+  w.spill_cap = std::max(0, s->d.stack_bound - ring);
   const float nIterF = (float)n_iters;
   const float NpF = (float)((uint64_t)n_iters * (uint64_t)photons);
   const bool want_dbg = dbg && dbg->debug_iter >= 0 && dbg->debug_iter < n_iters;
@@ -768,12 +775,16 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     // --- camera pass: the wave's radiance arrays are the pixels' Ld
     Wave cw = w;
     cw.N = npix;
+    cw.seg = npix;  // one queue segment; its lengths live in the control block
+    cw.nseg = 1;
+    cw.cnt_ray = w.ctrl + C_RAY;
+    cw.cnt_shadow = w.ctrl + C_SHADOW;
     cw.Lr = b.px.Ldr; cw.Lg = b.px.Ldg; cw.Lb = b.px.Ldb;
     hipLaunchKernelGGL(k_cam_raygen, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, s->d, cw, b.px, iter);
     int cur = 0;
     for (int it = 0; it <= maxDepth; ++it) {
-      launch_trace(dj, cw, false, stack, trace_blocks, false, st);
-      launch_trace(dj, cw, true, stack, trace_blocks, false, st);
+      launch_trace(dj, cw, false, ring, chunk, false, st);
+      launch_trace(dj, cw, true, ring, chunk, false, st);
       launch_between(cw, cur ^ 1, st);
       int blocks = std::min((npix + 255) / 256, 8192);
       hipLaunchKernelGGL(k_cam_shade, dim3(blocks), dim3(256), 0, st, s->d, cw, b.px, cur, maxDepth, iter);
@@ -836,11 +847,15 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     if (photons > 0) {
       Wave pw = w;
       pw.N = photons;
+      pw.seg = photons;
+      pw.nseg = 1;
+      pw.cnt_ray = w.ctrl + C_RAY;
+      pw.cnt_shadow = w.ctrl + C_SHADOW;
       uint32_t index0 = (uint32_t)(n_iters * npix + iter * photons);
       HIP_TRY(hipMemsetAsync(w.ctrl + C_RAY, 0, 4, st));
       hipLaunchKernelGGL(k_photon_init, dim3((photons + TB - 1) / TB), dim3(TB), 0, st, s->d, pw, index0, photons);
       for (int depth = 0; depth < maxDepth; ++depth) {
-        launch_trace(dj, pw, false, stack, trace_blocks, false, st);
+        launch_trace(dj, pw, false, ring, chunk, false, st);
         hipLaunchKernelGGL(k_photon_between, dim3(1), dim3(64), 0, st, pw, dstats);
         int blocks = std::min((photons + 255) / 256, 8192);
         hipLaunchKernelGGL(k_photon_shade, dim3(blocks), dim3(256), 0, st, s->d, pw, b.px, g, index0, depth, dstats);
@@ -871,7 +886,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   unsigned long long h[S_COUNT];
-  HIP_TRY(hipMemcpy(h, s->stats.ptr, sizeof(h), hipMemcpyDeviceToHost));
+  if ((rc = read_striped_stats(s->stats.ptr, h))) return rc;
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->closest_rays = (int64_t)h[S_CLOSEST];

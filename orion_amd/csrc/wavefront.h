@@ -28,11 +28,30 @@ enum : int {
   C_NJOBS = 8,                // jobs in the current wave
   C_INTS = 16,
 };
-// 64-bit statistics
+// 64-bit statistics, each striped over kStatStripes counters (a block adds
+// to stripe blockIdx % kStatStripes) so no single address takes every
+// block's atomics; readers sum the stripes.
 enum : int {
   S_CLOSEST = 0, S_SHADOW = 1, S_REFEQ = 2, S_PATHS = 3, S_NODES_C = 4, S_TRIS_C = 5,
-  S_PHOTONS = 6, S_DEPOSITS = 7, S_NODES_S = 8, S_TRIS_S = 9, S_COUNT = 12,
+  S_PHOTONS = 6, S_DEPOSITS = 7, S_NODES_S = 8, S_TRIS_S = 9, S_OVERFLOW = 10, S_COUNT = 12,
 };
+constexpr int kStatStripes = 64;
+constexpr size_t kStatBytes = (size_t)S_COUNT * kStatStripes * 8;
+
+// Queue segments: path ids [s * seg, (s + 1) * seg) form segment s.  Each
+// segment has its own path, closest-ray (2 seg entries) and shadow-ray
+// queues with their own lengths, filled by the one block that shades the
+// segment through LDS counters, so appends never contend on a shared device
+// counter.  seg is a power of two in [kSegMin, kSegMax] chosen per wave:
+// large enough that a traversal wave's rays come from neighbouring pixels,
+// small enough that a wave still has about 2048 shading blocks.
+constexpr int kSegMin = 512, kSegMax = 16384;
+inline void choose_segments(int npaths, int *seg, int *nseg) {
+  int s = kSegMin;
+  while (s < kSegMax && npaths / (2 * s) >= 2048) s *= 2;
+  *seg = s;
+  *nseg = npaths > 0 ? (npaths + s - 1) / s : 1;
+}
 
 // One render job of a wave (device job table).  A wave packs the next
 // sample slabs of any number of jobs -- each with its own scene, camera,
@@ -80,9 +99,50 @@ struct Wave {
   int *qshadow;   // path ids
   int *ctrl;
   unsigned long long *stats;
+  // traversal stack spill area: spill_cap entries per traversal thread
+  int *spill;
+  int spill_cap;
+  // queue segments: `seg` paths per segment (closest queues hold 2 seg
+  // entries per segment), nseg segments in use, lengths per segment
+  int seg, nseg;
+  int *cnt_path[2];
+  int *cnt_ray, *cnt_shadow;
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
+// b % 8) and each XCD has its own L2.  xcd_block maps the hardware block id
+// to a logical one so that each XCD takes a contiguous range of logical
+// blocks: neighbouring queue segments (neighbouring pixels, coherent rays)
+// then share one L2 instead of being fetched into all eight.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int G = gridDim.x, b = blockIdx.x;
+  const int per = G / kXcds, rem = G % kXcds;
+  const int x = b % kXcds, k = b / kXcds;
+  return x * per + min(x, rem) + k;
+}
+
+__device__ __forceinline__ void stat_add(unsigned long long *stats, int s, unsigned long long v) {
+  if (stats && v) atomicAdd(&stats[s * kStatStripes + (blockIdx.x & (kStatStripes - 1))], v);
+}
+
+// Wave-aggregated append of pred lanes to an LDS counter; returns each
+// lane's slot (lane order).  Every lane of the wave must call it.
+__device__ __forceinline__ int lds_append(int *counter, bool pred) {
+  unsigned long long mask = __ballot(pred);
+  int lane = lane_id();
+  int n = __popcll(mask);
+  int base = 0;
+  if (n) {
+    int leader = __ffsll((long long)mask) - 1;
+    if (lane == leader) base = atomicAdd(counter, n);
+    base = __shfl(base, leader);
+  }
+  unsigned long long below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+  return base + __popcll(below);
+}
 
 // Aggregated append: one atomic per wave, slots in lane order.  Must be
 // called by every lane of the wave (outside divergent code).

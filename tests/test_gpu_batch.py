@@ -143,7 +143,7 @@ def test_server_continuous_batching(device, scenes):
 def test_autotune(device, scenes):
     jobs = job_list(scenes)
     best = oa.autotune(device, jobs)
-    assert best.wave_paths >= 1 << 20 and best.persistent_blocks >= 0
+    assert best.wave_paths >= 1 << 20 and best.rays_per_wave >= 64 and best.lds_ring in (8, 16, 32)
     a, _ = oa.render_batch(device, jobs)
     t, _ = oa.render_batch(device, jobs, tuning=best)
     for (x, _), (y, _) in zip(a, t):

@@ -148,3 +148,31 @@ def test_loader_errors(tmp_path, body, frag):
     assert frag in str(e.value)
     with pytest.raises(oa.OrionError):
         oa.HostScene.load_ori(tmp_path / "missing.ori")
+
+
+@pytest.mark.parametrize("name,params", [("cornell", None), ("icosphere", [5, 64, 64]),
+                                         ("caustics", [4, 64, 64]),
+                                         ("batch_job", [300, 64, 64, 0]),
+                                         ("batch_job", [2e5, 64, 64, 1])])
+def test_bvh_structure(name, params):
+    """the 4-wide quantized BVH: decoded child boxes contain their subtrees,
+    every triangle sits in one leaf, depth and stack bound consistent"""
+    hs = oa.HostScene.generate(name, seed=3, params=params)
+    n_nodes, depth, bound = hs.bvh_check()
+    assert n_nodes >= 1 and bound == 3 * (depth + 1)
+    # a 4-wide tree over n leaves of <= 8 triangles has at least n / 8 / 3 inner nodes
+    assert n_nodes >= hs.n_tris / 24
+
+
+def test_bvh_tiny_scenes():
+    tri = np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0] + [0, 0, 1] * 3], np.float32)
+    for n in (1, 2, 5, 9, 17):
+        rng = np.random.default_rng(n)
+        pn = np.repeat(tri, n, axis=0)
+        pn[:, :9] += rng.random((n, 1)).astype(np.float32)
+        mats = np.zeros(16, np.float32)
+        cam = oa.camera_record([0, 0, 5, 0, 0, 0, 0, 1, 0], 40, 8, 8)
+        hs = oa.HostScene.from_records(pn, np.zeros(n, np.int32), -np.ones(n, np.int32), mats,
+                                       np.zeros(0, np.float32), cam)
+        nn, depth, bound = hs.bvh_check()
+        assert nn >= 1

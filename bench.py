@@ -30,6 +30,25 @@ HBM_PEAK_GBS = 8000.0
 NODE_BYTES = 64      # 4-wide node: origin, grid steps, 4 child links, 8-bit child boxes
 TRI_BYTES = 48       # p0, e1, e2 + primitive id
 RAY_IO_BYTES = 48    # 32 B ray in (o, d, tmax, id) + 16 B hit out (t, prim, u, v)
+SHADOW_IO_BYTES = 36  # 32 B ray in + 4 B visibility out
+TRACE_KERNELS = {"closest": "k_trace<false, 16, false>", "shadow": "k_trace<true, 16, false>"}
+TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_icosphere_L8_1024x1024_32spp.json"
+
+
+def load_traffic(a):
+    """HBM bytes per dispatch measured with rocprofv3 --pmc (FETCH_SIZE /
+    WRITE_SIZE, scaled by a 256 MiB copy) for this exact configuration, or
+    None when the configuration differs or the measurement is absent."""
+    if not (a.level == 8 and a.res == 1024 and a.depth == 8 and a.spp_per_step == 32):
+        return None
+    try:
+        d = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    d["source"] = f"{TRAFFIC_FILE.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE x " \
+                  f"{d['copy_calibration']['fetch_scale']}, WRITE_SIZE x " \
+                  f"{d['copy_calibration']['write_scale']}, per dispatch)"
+    return d
 
 
 def parse():
@@ -108,17 +127,27 @@ def batch_leg(oa, dev, n_jobs, seed):
     setup_s = time.perf_counter() - t0
     paths = sum(W * H * spp for _, W, H, spp, _ in jobs)
     tris_tot = sum(ds.stats()["n_tris"] for ds in scenes)
-    # plain: one job at a time through orion_render_path, default waves
+    # plain: one job at a time through orion_render_path (render + film
+    # readback, as the batched call returns the films), (a) one sample pass
+    # over the film per wave -- orion's own loop order, PathTracing::render
+    # -- and (b) this library's default multi-sample waves
     for ds, W, H, spp, depth in jobs[:2]:  # warm
         ds.render_path(0, 1, depth)
+    t_plain1 = 0.0
+    for ds, W, H, spp, depth in jobs:
+        ds.film_clear()
+        t0 = time.perf_counter()
+        ds.render_path(0, spp, depth, spp_per_wave=1)
+        ds.film()
+        t_plain1 += time.perf_counter() - t0
     plain_films = []
     t_plain = 0.0
     for ds, W, H, spp, depth in jobs:
         ds.film_clear()
         t0 = time.perf_counter()
         ds.render_path(0, spp, depth)
-        t_plain += time.perf_counter() - t0
         plain_films.append(ds.film()[0])
+        t_plain += time.perf_counter() - t0
     t0 = time.perf_counter()
     tune = oa.autotune(dev, jobs)
     tune_s = time.perf_counter() - t0
@@ -134,13 +163,18 @@ def batch_leg(oa, dev, n_jobs, seed):
                     f"Lambert/GGX/dielectric per mesh",
         "paths": int(paths),
         "batched_paths_per_s": round(paths / t_batch, 1),
-        "plain_paths_per_s": round(paths / t_plain, 1),
-        "speedup": round(t_plain / t_batch, 3),
-        "batched_s": round(t_batch, 3), "plain_s": round(t_plain, 3),
+        "plain_paths_per_s": round(paths / t_plain1, 1),
+        "speedup": round(t_plain1 / t_batch, 3),
+        "plain": "one job at a time, one sample pass over the film per wave (orion's loop order)",
+        "plain_multisample_wave_paths_per_s": round(paths / t_plain, 1),
+        "speedup_vs_plain_multisample_wave": round(t_plain / t_batch, 3),
+        "batched_s": round(t_batch, 3), "plain_s": round(t_plain1, 3),
+        "plain_multisample_wave_s": round(t_plain, 3),
         "batched_rays_per_s": round((st.closest_rays + st.shadow_rays) / t_batch, 1),
         "films_bitwise_equal_to_plain": f"{equal}/{n_jobs}",
         "tuning": {"wave_paths": tune.wave_paths, "rays_per_wave": tune.rays_per_wave,
-                   "lds_ring": tune.lds_ring, "use_graph": tune.use_graph},
+                   "lds_ring": tune.lds_ring, "queue_segment": tune.queue_segment,
+                   "use_graph": tune.use_graph},
         "autotune_s": round(tune_s, 2), "setup_s": round(setup_s, 2),
     }
 
@@ -149,10 +183,13 @@ def serve_leg(oa, dev, n_req, load, budget_gib, seed):
     """Poisson stream of render requests against 4 resident scenes (~8.2M
     triangles: displaced icospheres of 5.2M, 1.3M, 1.3M and 0.33M triangles,
     material drawn per mesh); each request: film sides uniform in [256, 1024],
-    spp uniform in [16, 256], depth 5-8.  The plain baseline renders each
-    request alone (one at a time, first come first served); its latencies
-    follow from the measured service times on the same arrival schedule.  The
-    arrival rate is `load` x the plain path's capacity."""
+    spp uniform in [16, 256], depth 5-8.  The plain baselines render each
+    request alone (one at a time, first come first served) -- one sample pass
+    over the film per wave (orion's loop order), and this library's default
+    multi-sample waves; their latencies follow from the measured service
+    times on the same arrival schedule.  The arrival rate is `load` x the
+    multi-sample plain path's capacity (the one-sample-per-wave plain path
+    cannot keep up with it, so its queue grows over the stream)."""
     rng = np.random.default_rng(seed)
     scenes = []
     for i, tris in enumerate([5.3e6, 1.4e6, 1.4e6, 3.4e5]):
@@ -166,22 +203,29 @@ def serve_leg(oa, dev, n_req, load, budget_gib, seed):
         reqs.append((ds, W, H, int(rng.integers(16, 257)), int(rng.integers(5, 9))))
     paths = sum(W * H * spp for _, W, H, spp, _ in reqs)
     oa.render_batch(dev, [(scenes[0], 256, 256, 1, 5)])  # warm
-    service = []
+    service, service1 = [], []
     for r in reqs:
         t0 = time.perf_counter()
         oa.render_batch(dev, [r])
         service.append(time.perf_counter() - t0)
-    capacity = n_req / sum(service)          # requests/s, plain
+        t0 = time.perf_counter()
+        oa.render_batch(dev, [r], wave_paths=r[1] * r[2])  # one sample pass per wave
+        service1.append(time.perf_counter() - t0)
+    capacity = n_req / sum(service)          # requests/s, multi-sample plain
     qps = load * capacity
     gaps = rng.exponential(1.0 / qps, n_req)
     arrive = np.cumsum(gaps) - gaps[0]
-    # plain FIFO on the same schedule
-    free, lat_plain = 0.0, []
-    for a, sv in zip(arrive, service):
-        start = max(a, free)
-        free = start + sv
-        lat_plain.append(free - a)
-    plain_span = free - arrive[0]
+
+    def fifo(svc):  # plain first-come-first-served on the same schedule
+        free, lat = 0.0, []
+        for a, sv in zip(arrive, svc):
+            start = max(a, free)
+            free = start + sv
+            lat.append(free - a)
+        return lat, free - arrive[0]
+
+    lat_plain1, plain1_span = fifo(service1)
+    lat_plain, plain_span = fifo(service)
     tune = oa.autotune(dev, [(ds, W, H, min(spp, 2), d) for ds, W, H, spp, d in reqs[:8]])
     srv = oa.Server(dev, int(budget_gib * (1 << 30)), tuning=tune)
     t0 = time.perf_counter()
@@ -201,18 +245,24 @@ def serve_leg(oa, dev, n_req, load, budget_gib, seed):
     srv.close()
     span = max(done) - 0.0
     return {
-        "workload": f"{n_req} requests, Poisson at {qps:.2f}/s ({load:g} x plain capacity), "
-                    f"4 resident scenes ({tris_tot} tris), films 256..1024 per side, "
+        "workload": f"{n_req} requests, Poisson at {qps:.2f}/s ({load:g} x multi-sample plain "
+                    f"capacity), 4 resident scenes ({tris_tot} tris), films 256..1024 per side, "
                     f"16-256 spp, depth 5-8, HBM budget {budget_gib:g} GiB",
         "paths": int(paths),
         "p50_s": round(float(np.percentile(lat, 50)), 3),
         "p99_s": round(float(np.percentile(lat, 99)), 3),
-        "plain_p50_s": round(float(np.percentile(lat_plain, 50)), 3),
-        "plain_p99_s": round(float(np.percentile(lat_plain, 99)), 3),
+        "plain": "one request at a time, one sample pass over the film per wave (orion's loop order)",
+        "plain_p50_s": round(float(np.percentile(lat_plain1, 50)), 3),
+        "plain_p99_s": round(float(np.percentile(lat_plain1, 99)), 3),
+        "plain_multisample_wave_p50_s": round(float(np.percentile(lat_plain, 50)), 3),
+        "plain_multisample_wave_p99_s": round(float(np.percentile(lat_plain, 99)), 3),
         "served_paths_per_s_busy": round(paths / (sst["busy_ms"] / 1e3), 1),
-        "plain_paths_per_s": round(paths / sum(service), 1),
-        "speedup_busy": round(sum(service) / (sst["busy_ms"] / 1e3), 3),
-        "served_span_s": round(span, 3), "plain_span_s": round(plain_span, 3),
+        "plain_paths_per_s": round(paths / sum(service1), 1),
+        "plain_multisample_wave_paths_per_s": round(paths / sum(service), 1),
+        "speedup_busy": round(sum(service1) / (sst["busy_ms"] / 1e3), 3),
+        "speedup_busy_vs_plain_multisample_wave": round(sum(service) / (sst["busy_ms"] / 1e3), 3),
+        "served_span_s": round(span, 3), "plain_span_s": round(plain1_span, 3),
+        "plain_multisample_wave_span_s": round(plain_span, 3),
         "waves": sst["waves"], "mixed_waves": sst["mixed_waves"],
         "max_jobs_in_wave": sst["max_jobs_in_wave"],
     }
@@ -259,26 +309,43 @@ def main():
     paths = tot["paths"]
     value = paths / el
     rays = tot["closest_rays"] + tot["shadow_rays"]
-    # roofline of the dominant kernel: closest-hit traversal
+    # roofline of the traversal kernels; the headline is the one with the
+    # larger device time (the dominant kernel)
     kern = {"trace_closest": tot["trace_closest_ms"], "trace_shadow": tot["trace_shadow_ms"],
             "shade": tot["shade_ms"]}
-    launches = max(tot["trace_closest_launches"], 1)
-    bytes_alg = (tot["closest_rays"] * RAY_IO_BYTES + tot["node_visits_closest"] * NODE_BYTES
-                 + tot["tri_tests_closest"] * TRI_BYTES)
-    per_launch_bytes = bytes_alg / launches
-    per_launch_s = tot["trace_closest_ms"] / 1e3 / launches
-    achieved = per_launch_bytes / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
-    roofline = {
-        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-        "kernel": "k_trace<closest> (4-wide quantized BVH, LDS ring stack)",
-        "bytes_per_ray": round(bytes_alg / max(tot["closest_rays"], 1), 1),
-        "nodes_per_ray": round(tot["node_visits_closest"] / max(tot["closest_rays"], 1), 2),
-        "tris_per_ray": round(tot["tri_tests_closest"] / max(tot["closest_rays"], 1), 2),
-        "shadow_nodes_per_ray": round(tot["node_visits_shadow"] / max(tot["shadow_rays"], 1), 2),
-        "shadow_tris_per_ray": round(tot["tri_tests_shadow"] / max(tot["shadow_rays"], 1), 2),
-        "kernel_ms": {k: round(v, 2) for k, v in kern.items()},
-    }
+    traffic = load_traffic(a)
+
+    def trace_roofline(kind):
+        rays = tot[f"{kind}_rays"]
+        launches = max(tot[f"trace_{kind}_launches"], 1)
+        io = RAY_IO_BYTES if kind == "closest" else SHADOW_IO_BYTES
+        nodes, tris = tot[f"node_visits_{kind}"], tot[f"tri_tests_{kind}"]
+        bytes_alg = rays * io + nodes * NODE_BYTES + tris * TRI_BYTES
+        per_launch_s = tot[f"trace_{kind}_ms"] / 1e3 / launches
+        achieved = bytes_alg / launches / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
+        k = TRACE_KERNELS[kind]
+        t = traffic["kernels"].get(k) if traffic else None
+        return {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": t["hbm_bytes_per_dispatch"] if t else None,
+            "kernel": f"{k} ({'closest-hit' if kind == 'closest' else 'any-hit'} traversal, 4-wide "
+                      f"quantized BVH, LDS ring stack)",
+            "algorithmic_bytes_per_launch": round(bytes_alg / launches),
+            "launch_ms": round(per_launch_s * 1e3, 4),
+            "bytes_per_ray": round(bytes_alg / max(rays, 1), 1),
+            "nodes_per_ray": round(nodes / max(rays, 1), 2),
+            "tris_per_ray": round(tris / max(rays, 1), 2),
+            "l2_hit_rate": t["l2_hit_rate"] if t else None,
+        }
+
+    rl = {kind: trace_roofline(kind) for kind in ("closest", "shadow")}
+    dom = "closest" if tot["trace_closest_ms"] >= tot["trace_shadow_ms"] else "shadow"
+    roofline = dict(rl[dom])
+    roofline["kernel_ms"] = {k: round(v, 2) for k, v in kern.items()}
+    roofline["other_trace_kernel"] = rl["shadow" if dom == "closest" else "closest"]
+    if traffic:
+        roofline["traffic_source"] = traffic["source"]
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "paths/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(el * 1e3 / a.steps, 3),

@@ -164,6 +164,7 @@ typedef struct {
   int use_graph;        /* capture the bounce loop into a hipGraph */
   int profile;          /* time every traversal / shade launch with HIP events */
   int lds_ring;         /* traversal stack entries per lane kept in LDS: 8, 16, 32 (0 = 16) */
+  int queue_segment;    /* paths per queue segment, a power of two in [512, 16384] (0 = by wave size) */
 } orion_tuning;
 
 /* PathTracing::render over samples [k0, k1): accumulates into the scene's film
@@ -215,6 +216,13 @@ int orion_render_batch(orion_ctx c, int n_jobs, const orion_job_desc *jobs, floa
  * retired when their last sample is accumulated. */
 typedef struct orion_server_s *orion_server;
 int orion_server_create(orion_ctx c, int64_t hbm_budget_bytes, const orion_tuning *tune, orion_server *out);
+/* Wave composition: ORION_SERVE_PACK (default) fills each wave with the jobs
+ * of least remaining work first, as many samples each as fit; ORION_SERVE_FAIR
+ * gives every admitted job at least one sample per wave (all images refine
+ * together).  Either way a job's film is bit-identical to rendering it alone. */
+#define ORION_SERVE_PACK 0
+#define ORION_SERVE_FAIR 1
+int orion_server_set_policy(orion_server s, int policy);
 /* thread-safe; *job_id identifies the request */
 int orion_server_submit(orion_server s, const orion_job_desc *job, uint64_t *job_id);
 /* done = 1 once finished; then copies the film (either pointer may be NULL)

@@ -53,7 +53,8 @@ class RenderStats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("spp_per_wave", C.c_int), ("wave_paths", C.c_int), ("rays_per_wave", C.c_int),
-                ("use_graph", C.c_int), ("profile", C.c_int), ("lds_ring", C.c_int)]
+                ("use_graph", C.c_int), ("profile", C.c_int), ("lds_ring", C.c_int),
+                ("queue_segment", C.c_int)]
 
 
 class SppmDebug(C.Structure):
@@ -113,6 +114,7 @@ _SIGS = {
                                      C.POINTER(Tuning), C.POINTER(RenderStats)]),
     "orion_server_create": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Tuning),
                                       C.POINTER(C.c_void_p)]),
+    "orion_server_set_policy": (C.c_int, [C.c_void_p, C.c_int]),
     "orion_server_submit": (C.c_int, [C.c_void_p, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),
     "orion_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), C.c_void_p,
                                     C.c_void_p, C.POINTER(C.c_double)]),
@@ -325,16 +327,16 @@ class DeviceScene:
         return occ
 
     def render_path(self, k0, k1, max_depth, spp_per_wave=0, rays_per_wave=0, profile=False,
-                    use_graph=False, wave_paths=0, lds_ring=0, tuning=None):
+                    use_graph=False, wave_paths=0, lds_ring=0, queue_segment=0, tuning=None):
         if tuning is not None:
             spp_per_wave, wave_paths = tuning.spp_per_wave, tuning.wave_paths
             rays_per_wave, use_graph = tuning.rays_per_wave, bool(tuning.use_graph)
-            lds_ring = tuning.lds_ring
+            lds_ring, queue_segment = tuning.lds_ring, tuning.queue_segment
         st = RenderStats()
         tune = None  # NULL: the defaults
-        if spp_per_wave or wave_paths or rays_per_wave or use_graph or profile or lds_ring:
+        if spp_per_wave or wave_paths or rays_per_wave or use_graph or profile or lds_ring or queue_segment:
             tune = C.byref(Tuning(spp_per_wave, wave_paths, rays_per_wave, int(use_graph),
-                                  int(profile), lds_ring))
+                                  int(profile), lds_ring, queue_segment))
         _check(lib().orion_render_path(self.h, k0, k1, max_depth, tune, C.byref(st)))
         return st
 
@@ -392,12 +394,13 @@ def autotune(dev, jobs):
 
 
 def render_batch(dev, jobs, wave_paths=0, rays_per_wave=0, use_graph=False, profile=False,
-                 lds_ring=0, tuning=None):
+                 lds_ring=0, queue_segment=0, tuning=None):
     """Renders jobs = [(DeviceScene, width, height, spp, max_depth), ...] in shared
     wavefronts; returns ([(rgb HxWx3, count HxW), ...], RenderStats)."""
     if tuning is not None:
         wave_paths, rays_per_wave = tuning.wave_paths, tuning.rays_per_wave
         use_graph, lds_ring = bool(tuning.use_graph), tuning.lds_ring
+        queue_segment = tuning.queue_segment
     n = len(jobs)
     descs = _descs(jobs)
     outs = []
@@ -410,7 +413,8 @@ def render_batch(dev, jobs, wave_paths=0, rays_per_wave=0, use_graph=False, prof
         rgb_p[i] = rgb.ctypes.data
         cnt_p[i] = cnt.ctypes.data
     st = RenderStats()
-    tune = Tuning(0, wave_paths, rays_per_wave, int(use_graph), int(profile), lds_ring)
+    tune = Tuning(0, wave_paths, rays_per_wave, int(use_graph), int(profile), lds_ring,
+                  queue_segment)
     _check(lib().orion_render_batch(dev.h, n, descs, rgb_p, cnt_p, C.byref(tune), C.byref(st)))
     return outs, st
 
@@ -420,15 +424,21 @@ class Server:
     per-request times (submitted, admitted, finished; seconds since start)."""
 
     def __init__(self, dev, hbm_budget_bytes, wave_paths=0, use_graph=False, rays_per_wave=0,
-                 lds_ring=0, tuning=None):
+                 lds_ring=0, queue_segment=0, tuning=None):
         if tuning is not None:
             wave_paths, rays_per_wave = tuning.wave_paths, tuning.rays_per_wave
             use_graph, lds_ring = bool(tuning.use_graph), tuning.lds_ring
+            queue_segment = tuning.queue_segment
         self.dev = dev
         self.h = C.c_void_p()
         self._shape = {}
-        tune = Tuning(0, wave_paths, rays_per_wave, int(use_graph), 0, lds_ring)
+        tune = Tuning(0, wave_paths, rays_per_wave, int(use_graph), 0, lds_ring, queue_segment)
         _check(lib().orion_server_create(dev.h, int(hbm_budget_bytes), C.byref(tune), C.byref(self.h)))
+
+    PACK, FAIR = 0, 1
+
+    def set_policy(self, policy):
+        _check(lib().orion_server_set_policy(self.h, policy))
 
     def submit(self, scene, width, height, spp, max_depth):
         jid = C.c_uint64()

@@ -89,6 +89,11 @@ struct WaveRunner {
   Job *h_jobs = nullptr;
   int *h_ctrl = nullptr;
   int max_jobs = 0;
+  // segment -> job index of multi-job waves (pinned staging + device copy)
+  DevBuf segjob_dev;
+  int *h_segjob = nullptr;
+  DevBuf films;  // batched launches: the jobs' films
+  int max_segs = 0;
   hipEvent_t staged = nullptr;
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
 
@@ -96,7 +101,19 @@ struct WaveRunner {
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
     if (h_jobs) (void)hipHostFree(h_jobs);
     if (h_ctrl) (void)hipHostFree(h_ctrl);
+    if (h_segjob) (void)hipHostFree(h_segjob);
     if (staged) (void)hipEventDestroy(staged);
+  }
+  int ensure_segjob(int nseg) {
+    if (nseg <= max_segs) return ORION_OK;
+    int cap = std::max(nseg, 2 * max_segs);
+    if (h_segjob) { HIP_TRY(hipEventSynchronize(staged)); (void)hipHostFree(h_segjob); }
+    HIP_TRY(hipHostMalloc((void **)&h_segjob, sizeof(int) * cap, hipHostMallocDefault));
+    int rc = segjob_dev.ensure(sizeof(int) * cap);
+    if (rc) return rc;
+    max_segs = cap;
+    drop_graphs();
+    return ORION_OK;
   }
   int init(int njobs) {
     if (njobs <= max_jobs && staged) return ORION_OK;
@@ -182,28 +199,39 @@ struct WaveRunner {
   }
 
   // Runs one wave: jobs[0..n) with path0 / pix0 already assigned.
+  // seg: queue segment size (0: chosen from npaths); multi-job waves pass
+  // the size their job table was aligned to
   int run(hipStream_t st, const Job *jobs, int n, int npaths, int npix, int depth, int ring, int stack_bound,
-          bool multi, int chunk, bool use_graph, LaunchTimer *tm) {
+          bool multi, int chunk, bool use_graph, LaunchTimer *tm, int seg = 0) {
     int rc = init(n);
     if (rc) return rc;
     if ((rc = ensure_cap(npaths))) return rc;
-    int seg, nseg;
-    choose_segments(npaths, &seg, &nseg);
+    int nseg;
+    if (seg > 0) nseg = std::max(1, (npaths + seg - 1) / seg);
+    else choose_segments(npaths, &seg, &nseg);
     if ((rc = ensure_spill(stack_bound, ring, chunk, seg, nseg))) return rc;
+    if ((rc = ensure_segjob(nseg))) return rc;
     // stage the job table and the wave sizes (pinned; the previous copy
     // must have completed before the staging area is rewritten)
     HIP_TRY(hipEventSynchronize(staged));
     std::memcpy(h_jobs, jobs, sizeof(Job) * n);
+    // job j owns segments [path0_j / seg, path0_{j+1} / seg)
+    for (int j = 0, sg = 0; j < n; ++j) {
+      const int end = j + 1 < n ? jobs[j + 1].path0 / seg : nseg;
+      for (; sg < end; ++sg) h_segjob[sg] = j;
+    }
     h_ctrl[0] = npaths;
     h_ctrl[1] = npix;
     h_ctrl[2] = n;
     HIP_TRY(hipMemcpyAsync(jobs_dev.ptr, h_jobs, sizeof(Job) * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(segjob_dev.ptr, h_segjob, sizeof(int) * nseg, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync((int *)ctrl.ptr + C_NPATHS, h_ctrl, 3 * sizeof(int), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(staged, st));
     const Job *dj = (const Job *)jobs_dev.ptr;
     Wave w = wave();
     w.seg = seg;
     w.nseg = nseg;
+    w.seg_job = (const int *)segjob_dev.ptr;
     if (!use_graph || tm) {
       enqueue_wave(dj, w, depth, ring, multi, chunk, st, tm);
       HIP_TRY(hipGetLastError());
@@ -235,6 +263,16 @@ void destroy_runner(WaveRunner *r) { delete r; }
 
 static int ring_of(const orion_tuning *t) { return ring_variant(t ? t->lds_ring : 0); }
 static int chunk_of(const orion_tuning *t) { return trace_chunk(t ? t->rays_per_wave : 0); }
+// queue segment size for a wave of `paths` paths
+static int seg_of(const orion_tuning *t, int paths) {
+  int seg, nseg;
+  choose_segments(paths, &seg, &nseg);
+  if (t && t->queue_segment > 0) {
+    seg = kSegMin;
+    while (seg < kSegMax && seg < t->queue_segment) seg *= 2;
+  }
+  return seg;
+}
 
 static Job make_job(orion_scene s, int W, int H, int maxDepth, float *film, int *count) {
   Job J;
@@ -293,7 +331,7 @@ extern "C" int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, co
     J.S = Sk;
     J.k0 = (uint32_t)k;
     if ((rc = R.run(c->stream, &J, 1, Sk * npix, npix, maxDepth, ring, s->d.stack_bound, false, chunk, graph,
-                    tm.on ? &tm : nullptr)))
+                    tm.on ? &tm : nullptr, seg_of(tune, Sk * npix))))
       return rc;
   }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
@@ -314,56 +352,29 @@ namespace orion_amd {
 struct JobState {
   orion_job_desc d;
   int next_k = 0;
-  DevBuf film, count;
+  float *film = nullptr;  // W * H * 3 sums, then W * H counts
+  int *count = nullptr;
+  DevBuf own;             // the film's memory when the job owns it
 };
 
 constexpr int kMaxWavePaths = 1 << 25;
 
-// Picks each active job's next sample slab for one wave.  The wave holds
-// max(target, one sample of every active job) paths, at most kMaxWavePaths
-// (or one job's film if that is larger).  First every job gets one sample,
-// visited from a rotating start (*rr) so that a job left out of a full wave
-// goes first in the next one; the capacity left is then split evenly, in
-// whole samples.  taken[i] = samples given to act[i].
-static std::vector<Job> compose_wave(std::vector<JobState *> &act, int target, int *rr, int *npaths, int *npix,
-                                     int *depth, int *stack, std::vector<int> *taken) {
-  const int n_act = (int)act.size();
-  taken->assign(n_act, 0);
-  int64_t sum_np = 0;
-  int max_np = 1;
-  for (JobState *js : act)
-    if (js->d.spp > js->next_k) {
-      int np = js->d.width * js->d.height;
-      sum_np += np;
-      max_np = std::max(max_np, np);
-    }
-  const int64_t cap = std::max<int64_t>(max_np, std::min<int64_t>(std::max<int64_t>(target, sum_np),
-                                                                   std::max(target, kMaxWavePaths)));
-  // one sample each, rotating start
-  std::vector<int> order;
-  int64_t paths = 0;
-  const int start = n_act ? *rr % n_act : 0;
-  for (int t = 0; t < n_act; ++t) {
-    const int i = (start + t) % n_act;
-    JobState *js = act[i];
-    const int64_t np = (int64_t)js->d.width * js->d.height;
-    if (js->d.spp <= js->next_k || paths + np > cap) continue;
-    (*taken)[i] = 1;
-    paths += np;
-    order.push_back(i);
-  }
-  if (!order.empty()) *rr = order.back() + 1;
-  // the rest of the capacity, evenly
-  if (!order.empty()) {
-    const int64_t share = (cap - paths) / (int64_t)order.size();
-    for (int i : order) {
-      JobState *js = act[i];
-      const int64_t np = (int64_t)js->d.width * js->d.height;
-      const int extra = (int)std::min<int64_t>(js->d.spp - js->next_k - 1, share / np);
-      (*taken)[i] += extra;
-      paths += extra * np;
-    }
-  }
+// Picks each active job's next sample slab for one wave.
+//   kPack: jobs in the given order, each taking as many samples as still
+//     fit in `target` paths (at least one job's full film per wave), so a
+//     wave touches few scenes (scene data stays in the caches) and the
+//     tail of one job's samples shares its wave with the next job's start.
+//   kFair: the wave holds max(target, one sample of every active job)
+//     paths, at most kMaxWavePaths; every job first gets one sample, from a
+//     rotating start (*rr) so a job left out of a full wave goes first in the
+//     next, then the capacity left is split evenly in whole samples.
+// Each job's paths start at a multiple of `seg` (queue segments never mix
+// jobs).  taken[i] = samples given to act[i].
+// The job table of a wave: jobs act[order[k]] with taken[] samples each,
+// paths starting at multiples of seg.
+static std::vector<Job> emit_wave(std::vector<JobState *> &act, const std::vector<int> &order,
+                                  const std::vector<int> &taken, int seg, int *npaths, int *npix, int *depth,
+                                  int *stack) {
   std::vector<Job> jobs;
   int64_t p0 = 0;
   int pix = 0;
@@ -372,13 +383,12 @@ static std::vector<Job> compose_wave(std::vector<JobState *> &act, int target, i
   for (int i : order) {
     JobState *js = act[i];
     const int np = js->d.width * js->d.height;
-    Job J = make_job(js->d.scene, js->d.width, js->d.height, js->d.max_depth, (float *)js->film.ptr,
-                     (int *)js->count.ptr);
-    J.S = (*taken)[i];
+    Job J = make_job(js->d.scene, js->d.width, js->d.height, js->d.max_depth, js->film, js->count);
+    J.S = taken[i];
     J.k0 = (uint32_t)js->next_k;
     J.path0 = (int)p0;
     J.pix0 = pix;
-    p0 += (int64_t)J.S * np;
+    p0 = (p0 + (int64_t)J.S * np + seg - 1) / seg * seg;
     pix += np;
     *depth = std::max(*depth, js->d.max_depth);
     *stack = std::max(*stack, js->d.scene->d.stack_bound);
@@ -387,6 +397,71 @@ static std::vector<Job> compose_wave(std::vector<JobState *> &act, int target, i
   *npaths = (int)p0;
   *npix = pix;
   return jobs;
+}
+
+enum WavePolicy { kPack, kFair };
+static std::vector<Job> compose_wave(std::vector<JobState *> &act, int target, int seg, WavePolicy policy, int *rr,
+                                     int *npaths, int *npix, int *depth, int *stack, std::vector<int> *taken) {
+  const int n_act = (int)act.size();
+  taken->assign(n_act, 0);
+  auto padded = [&](int64_t paths) { return (paths + seg - 1) / seg * seg; };
+  if (policy == kPack) {
+    int max_np = 1;
+    for (JobState *js : act) max_np = std::max(max_np, js->d.width * js->d.height);
+    const int64_t cap = std::max<int64_t>(padded(max_np), target);
+    int64_t paths = 0;
+    std::vector<int> order;
+    for (int i = 0; i < n_act && paths < cap; ++i) {
+      JobState *js = act[i];
+      const int64_t np = (int64_t)js->d.width * js->d.height;
+      int c = (int)std::min<int64_t>(js->d.spp - js->next_k, (cap - paths) / np);
+      while (c > 0 && paths + padded(c * np) > cap) --c;
+      if (c <= 0) continue;
+      (*taken)[i] = c;
+      paths += padded(c * np);
+      order.push_back(i);
+    }
+    return emit_wave(act, order, *taken, seg, npaths, npix, depth, stack);
+  }
+  int64_t sum_np = 0;
+  int max_np = 1;
+  for (JobState *js : act)
+    if (js->d.spp > js->next_k) {
+      int np = js->d.width * js->d.height;
+      sum_np += padded(np);
+      max_np = std::max(max_np, np);
+    }
+  const int64_t cap = std::max<int64_t>(padded(max_np), std::min<int64_t>(std::max<int64_t>(target, sum_np),
+                                                                           std::max(target, kMaxWavePaths)));
+  // one sample each, rotating start
+  std::vector<int> order;
+  int64_t paths = 0;
+  const int start = n_act ? *rr % n_act : 0;
+  for (int t = 0; t < n_act; ++t) {
+    const int i = (start + t) % n_act;
+    JobState *js = act[i];
+    const int64_t np = (int64_t)js->d.width * js->d.height;
+    if (js->d.spp <= js->next_k || paths + padded(np) > cap) continue;
+    (*taken)[i] = 1;
+    paths += padded(np);
+    order.push_back(i);
+  }
+  if (!order.empty()) *rr = order.back() + 1;
+  // the rest of the capacity, evenly
+  if (!order.empty()) {
+    int64_t left = cap - paths;
+    const int64_t share = left / (int64_t)order.size();
+    for (int i : order) {
+      JobState *js = act[i];
+      const int64_t np = (int64_t)js->d.width * js->d.height;
+      int extra = (int)std::min<int64_t>(js->d.spp - js->next_k - 1, share / np);
+      // the padded slab may not outgrow the share
+      while (extra > 0 && padded((1 + extra) * np) - padded(np) > share) --extra;
+      (*taken)[i] += extra;
+      paths += padded((1 + extra) * np) - padded(np);
+    }
+  }
+  return emit_wave(act, order, *taken, seg, npaths, npix, depth, stack);
 }
 
 static int check_desc(const orion_job_desc &d) {
@@ -414,17 +489,23 @@ extern "C" int orion_render_batch(orion_ctx c, int n, const orion_job_desc *desc
   HIP_TRY(hipSetDevice(c->device));
   if (!c->runner) c->runner = new WaveRunner();
   WaveRunner &R = *c->runner;
+  // every job's film (sums then counts) in one region of the runner's film
+  // buffer, kept between calls
   std::vector<JobState> js(n);
-  int max_np = 1;
+  size_t film_bytes = 0;
+  for (int j = 0; j < n; ++j) film_bytes += (size_t)descs[j].width * descs[j].height * 16;
+  {
+    int rc = R.films.ensure(std::max<size_t>(film_bytes, 16));
+    if (rc) return rc;
+  }
+  HIP_TRY(hipMemsetAsync(R.films.ptr, 0, film_bytes, c->stream));
+  size_t off = 0;
   for (int j = 0; j < n; ++j) {
     js[j].d = descs[j];
     size_t np = (size_t)descs[j].width * descs[j].height;
-    max_np = std::max(max_np, (int)np);
-    int rc;
-    if ((rc = js[j].film.ensure(np * 12))) return rc;
-    if ((rc = js[j].count.ensure(np * 4))) return rc;
-    HIP_TRY(hipMemsetAsync(js[j].film.ptr, 0, np * 12, c->stream));
-    HIP_TRY(hipMemsetAsync(js[j].count.ptr, 0, np * 4, c->stream));
+    js[j].film = (float *)((char *)R.films.ptr + off);
+    js[j].count = (int *)((char *)R.films.ptr + off + np * 12);
+    off += np * 16;
   }
   int target = tune && tune->wave_paths > 0 ? tune->wave_paths : kDefaultWavePaths;
   int rc;
@@ -438,14 +519,14 @@ extern "C" int orion_render_batch(orion_ctx c, int n, const orion_job_desc *desc
   std::vector<JobState *> act;
   for (auto &j : js) act.push_back(&j);
   int rr = 0;
+  const int seg = seg_of(tune, target);
   for (;;) {
     int npaths, npix, depth, stack;
     std::vector<int> taken;
-    std::vector<Job> jobs = compose_wave(act, target, &rr, &npaths, &npix, &depth, &stack, &taken);
+    std::vector<Job> jobs = compose_wave(act, target, seg, kPack, &rr, &npaths, &npix, &depth, &stack, &taken);
     if (jobs.empty()) break;
     if ((rc = R.run(c->stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, ring_of(tune), stack,
-                    jobs.size() > 1, chunk_of(tune), graph,
-                    tm.on ? &tm : nullptr)))
+                    jobs.size() > 1, chunk_of(tune), graph, tm.on ? &tm : nullptr, seg)))
       return rc;
     for (size_t i = 0; i < act.size(); ++i) act[i]->next_k += taken[i];
     act.erase(std::remove_if(act.begin(), act.end(), [](JobState *j) { return j->next_k >= j->d.spp; }), act.end());
@@ -453,8 +534,8 @@ extern "C" int orion_render_batch(orion_ctx c, int n, const orion_job_desc *desc
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   for (int j = 0; j < n; ++j) {
     size_t np = (size_t)descs[j].width * descs[j].height;
-    HIP_TRY(hipMemcpyAsync(rgb_out[j], js[j].film.ptr, np * 12, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(count_out[j], js[j].count.ptr, np * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(rgb_out[j], js[j].film, np * 12, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(count_out[j], js[j].count, np * 4, hipMemcpyDeviceToHost, c->stream));
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
   float ms = 0;
@@ -486,6 +567,7 @@ struct orion_server_s {
   int64_t budget = 0, in_use = 0;
   orion_tuning tune{};
   bool has_tune = false;
+  orion_amd::WavePolicy policy = orion_amd::kPack;
   std::mutex mu;
   std::condition_variable cv, cv_done;
   std::deque<orion_amd::ServerJob *> queue;
@@ -511,6 +593,7 @@ void orion_server_s::loop() {
   const orion_tuning *tn = has_tune ? &tune : nullptr;
   int target = tn && tn->wave_paths > 0 ? tn->wave_paths : kDefaultWavePaths;
   int rr = 0;
+  const int seg = seg_of(tn, target);
   for (;;) {
     std::vector<ServerJob *> admitted;
     {
@@ -530,10 +613,12 @@ void orion_server_s::loop() {
     }
     for (ServerJob *j : admitted) {
       size_t np = (size_t)j->st.d.width * j->st.d.height;
-      int rc = j->st.film.ensure(np * 12);
-      if (!rc) rc = j->st.count.ensure(np * 4);
-      if (!rc && (hipMemsetAsync(j->st.film.ptr, 0, np * 12, stream) != hipSuccess ||
-                  hipMemsetAsync(j->st.count.ptr, 0, np * 4, stream) != hipSuccess))
+      int rc = j->st.own.ensure(np * 16);
+      if (!rc) {
+        j->st.film = (float *)j->st.own.ptr;
+        j->st.count = (int *)((char *)j->st.own.ptr + np * 12);
+      }
+      if (!rc && hipMemsetAsync(j->st.own.ptr, 0, np * 16, stream) != hipSuccess)
         rc = ORION_ERR_HIP;
       std::lock_guard<std::mutex> lk(mu);
       j->t_admit = now();
@@ -550,21 +635,39 @@ void orion_server_s::loop() {
     if (!admitted.empty()) cv_done.notify_all();
     std::vector<JobState *> act;
     std::vector<ServerJob *> owners;
+    WavePolicy policy;
     {
       std::lock_guard<std::mutex> lk(mu);
+      policy = this->policy;
       for (ServerJob *j : active)
         if (j->state == 1) { act.push_back(&j->st); owners.push_back(j); }
     }
     if (act.empty()) continue;
     int npaths, npix, depth, stack;
     std::vector<int> taken;
-    std::vector<Job> jobs = compose_wave(act, target, &rr, &npaths, &npix, &depth, &stack, &taken);
+    // packed: smallest remaining work first (shortest-remaining-time order)
+    std::vector<size_t> idx(act.size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    auto remaining = [&](size_t i) {
+      return (int64_t)(act[i]->d.spp - act[i]->next_k) * act[i]->d.width * act[i]->d.height;
+    };
+    if (policy == kPack)
+      std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return remaining(a) < remaining(b); });
+    std::vector<JobState *> sorted_act;
+    std::vector<ServerJob *> sorted_owners;
+    for (size_t i : idx) {
+      sorted_act.push_back(act[i]);
+      sorted_owners.push_back(owners[i]);
+    }
+    act.swap(sorted_act);
+    owners.swap(sorted_owners);
+    std::vector<Job> jobs = compose_wave(act, target, seg, policy, &rr, &npaths, &npix, &depth, &stack, &taken);
     auto w0 = std::chrono::steady_clock::now();
     int rc = runner.init(std::max(1, (int)jobs.size()));
     if (!rc) rc = runner.reset_stats(stream);
     if (!rc && !jobs.empty())
       rc = runner.run(stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, ring_of(tn), stack,
-                      jobs.size() > 1, chunk_of(tn), tn && tn->use_graph, nullptr);
+                      jobs.size() > 1, chunk_of(tn), tn && tn->use_graph, nullptr, seg);
     if (!rc && hipStreamSynchronize(stream) != hipSuccess) rc = ORION_ERR_HIP;
     orion_render_stats ws{};
     if (!rc) rc = runner.read_stats(&ws);
@@ -578,11 +681,12 @@ void orion_server_s::loop() {
       size_t np = (size_t)j->st.d.width * j->st.d.height;
       j->rgb.resize(np * 3);
       j->cnt.resize(np);
-      if (!rc && (hipMemcpy(j->rgb.data(), j->st.film.ptr, np * 12, hipMemcpyDeviceToHost) != hipSuccess ||
-                  hipMemcpy(j->cnt.data(), j->st.count.ptr, np * 4, hipMemcpyDeviceToHost) != hipSuccess))
+      if (!rc && (hipMemcpy(j->rgb.data(), j->st.film, np * 12, hipMemcpyDeviceToHost) != hipSuccess ||
+                  hipMemcpy(j->cnt.data(), j->st.count, np * 4, hipMemcpyDeviceToHost) != hipSuccess))
         rc = ORION_ERR_HIP;
-      j->st.film.release();
-      j->st.count.release();
+      j->st.own.release();
+      j->st.film = nullptr;
+      j->st.count = nullptr;
     }
     {
       std::lock_guard<std::mutex> lk(mu);
@@ -624,6 +728,13 @@ int orion_server_create(orion_ctx c, int64_t budget, const orion_tuning *tune, o
   s->t0 = std::chrono::steady_clock::now();
   s->th = std::thread([s] { s->loop(); });
   *out = s;
+  return ORION_OK;
+}
+
+int orion_server_set_policy(orion_server s, int policy) {
+  if (!s || (policy != ORION_SERVE_PACK && policy != ORION_SERVE_FAIR)) { set_error("bad argument"); return ORION_ERR_ARG; }
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->policy = policy == ORION_SERVE_PACK ? orion_amd::kPack : orion_amd::kFair;
   return ORION_OK;
 }
 
@@ -749,6 +860,11 @@ int orion_autotune(orion_ctx c, int n, const orion_job_desc *jobs, orion_tuning 
     t.lds_ring = ring;
     if ((rc = trial(t))) return rc;
   }
+  for (int seg : {1024, 2048, 4096, 8192, 16384}) {
+    orion_tuning t = b;
+    t.queue_segment = seg;
+    if ((rc = trial(t))) return rc;
+  }
   {
     orion_tuning t = b;
     t.use_graph = 1;
@@ -756,6 +872,7 @@ int orion_autotune(orion_ctx c, int n, const orion_job_desc *jobs, orion_tuning 
   }
   if (b.rays_per_wave == 0) b.rays_per_wave = kDefaultChunk;
   if (b.lds_ring == 0) b.lds_ring = kDefaultRing;
+  if (b.queue_segment == 0) b.queue_segment = seg_of(nullptr, b.wave_paths);
   *best = b;
   return ORION_OK;
 }

@@ -24,35 +24,26 @@
 namespace orion_amd {
 using namespace dev;
 
-__device__ __forceinline__ int find_job(const Job *jobs, int njobs, int p) {
-  int lo = 0, hi = njobs - 1;  // last job with path0 <= p
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].path0 <= p) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// One thread per path slot of the wave's capacity: camera rays for the
-// wave's paths, and every segment's initial queue lengths (all paths in the
-// path and closest queues, none in the shadow queue).
+// One thread per path slot of the wave's segments: camera rays for the
+// wave's paths, and every segment's initial queue lengths (its live paths in
+// the path and closest queues, none in the shadow queue).  Multi-job waves
+// start every job at a segment boundary, so a segment belongs to one job and
+// its live paths are a prefix of it (the job's padding follows).
 template <bool MULTI>
 __global__ void k_raygen(const Job *jobs, Wave w) {
-  const int npaths = w.ctrl[C_NPATHS];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= w.nseg * w.seg) return;
+  const Job &J = jobs[MULTI ? w.seg_job[p / w.seg] : 0];
+  const int live_end = J.path0 + J.S * J.npix;
   if (p % w.seg == 0) {
     const int sg = p / w.seg;
-    const int n = max(0, min(w.seg, npaths - p));
+    const int n = max(0, min(w.seg, live_end - p));
     w.cnt_path[0][sg] = n;
     w.cnt_ray[sg] = n;
     w.cnt_shadow[sg] = 0;
     stat_add(w.stats, S_CLOSEST, (unsigned long long)n);
   }
-  if (p >= npaths) return;
-  int jb = MULTI ? find_job(jobs, w.ctrl[C_NJOBS], p) : 0;
-  const Job &J = jobs[jb];
+  if (p >= live_end) return;
   int q = p - J.path0;
   int s = q / J.npix, pix = q - s * J.npix;
   int j = pix / J.W, i = pix - j * J.W;
@@ -67,7 +58,6 @@ __global__ void k_raygen(const Job *jobs, Wave w) {
   w.dim[p] = 7;
   w.bounce[p] = 0;
   w.flags[p] = F_ALIVE;
-  if (MULTI) w.job[p] = jb;
   const int sg = p / w.seg, j0 = p - sg * w.seg;
   w.qpath[0][p] = p;
   w.qray[2 * sg * w.seg + j0] = p;
@@ -77,22 +67,20 @@ __global__ void k_raygen(const Job *jobs, Wave w) {
 // Wave queues as traversal-engine IO.  Closest rays: entries r < N are
 // continuation rays (slot 0), N + p MIS rays (slot 1); shadow entries are
 // path ids whose ray lives in slot 2.
-template <bool ANY, bool MULTI>
+template <bool ANY>
 struct WaveRays {
-  const Job *jobs;
   const Wave *w;
-  int base;  // first queue entry of the segment
+  int base;                   // first queue entry of the segment
+  const float4 *nodes, *tris;  // the segment's job's scene (wave-uniform)
   __device__ void load(int q, TRay *ray) const {
     const int r = (ANY ? w->qshadow : w->qray)[base + q];
     const int slot = ANY ? 2 * w->N + r : r;
-    const int p = ANY ? r : (r >= w->N ? r - w->N : r);
-    const DScene &sc = jobs[MULTI ? w->job[p] : 0].sc;
     ray->o = mk(w->ox[slot], w->oy[slot], w->oz[slot]);
     ray->d = mk(w->dx[slot], w->dy[slot], w->dz[slot]);
     ray->tmin = 0.0f;
     ray->tmax = w->tmax[slot];
-    ray->nodes = sc.nodes;
-    ray->tris = sc.tris;
+    ray->nodes = nodes;
+    ray->tris = tris;
     ray->tag = r;
   }
   __device__ void store(const TRay &ray, bool hit, const HitRec &h) const {
@@ -128,7 +116,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w, 
   stk.spill = w.spill + blockIdx.x * kTraceBlock + threadIdx.x;
   stk.spill_cap = w.spill_cap;
   uint32_t ovf = 0, nv = 0, nt = 0;
-  WaveRays<ANY, MULTI> io{jobs, &w, sg * seg_entries};
+  const DScene &sc = jobs[MULTI ? w.seg_job[sg] : 0].sc;
+  WaveRays<ANY> io{&w, sg * seg_entries, sc.nodes, sc.tris};
   trace_engine<ANY, RING>(io, begin, min(begin + chunk, count), stk, &nv, &nt, &ovf);
   const int lane = (int)__lane_id();
   // per-wave reduction of traversal counters (for the roofline's n_node / n_tri)
@@ -168,6 +157,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const int seg0 = sg * w.seg;
   const int count = w.cnt_path[cur][sg];
   const int N = w.N;
+  const Job &J = jobs[MULTI ? w.seg_job[sg] : 0];  // the segment's job
+  const DScene &sc = J.sc;
   if (threadIdx.x < 3) lcnt[threadIdx.x] = 0;
   __syncthreads();
   uint32_t n_done = 0, n_mis = 0;
@@ -177,8 +168,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     int p = act ? w.qpath[cur][seg0 + i] : 0;
     bool want_ray0 = false, want_ray1 = false, want_shadow = false, keep = false, done = false;
     if (act) {
-      const Job &J = jobs[MULTI ? w.job[p] : 0];
-      const DScene &sc = J.sc;
       int fl = w.flags[p];
       Sp L = sp3(w.Lr[p], w.Lg[p], w.Lb[p]);
       // 1) resolve the direct-lighting estimate of the previous bounce

@@ -81,7 +81,7 @@ int WaveBuffers::ensure(int N) {
   if (N <= cap && mem.ptr) return ORION_OK;
   // floats/ints per path: rays 7 x 3 slots, hits 4 x 2, vis 1, state 9,
   // pending 15, queues 2 + 2 + 1
-  const size_t per = 21 + 8 + 1 + 9 + 15 + 5 + 1;
+  const size_t per = 21 + 8 + 1 + 9 + 15 + 5;
   const size_t nseg = ((size_t)N + kSegMin - 1) / kSegMin + 1;
   int rc = mem.ensure(per * (size_t)N * 4 + 4 * nseg * 4 + 256);
   if (rc) return rc;
@@ -105,7 +105,6 @@ int WaveBuffers::ensure(int N) {
   w.bpdf = (float *)take(N); w.sel = (float *)take(N);
   w.nbr = (float *)take(N); w.nbg = (float *)take(N); w.nbb = (float *)take(N);
   w.nlight = (int *)take(N);
-  w.job = (int *)take(N);
   w.N = N;  // slot stride
   w.qpath[0] = (int *)take(N); w.qpath[1] = (int *)take(N);
   w.qray = (int *)take(2 * (size_t)N);

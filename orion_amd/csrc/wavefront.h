@@ -57,7 +57,8 @@ inline void choose_segments(int npaths, int *seg, int *nseg) {
 // sample slabs of any number of jobs -- each with its own scene, camera,
 // film, sample range and depth -- into one set of queues; path ids
 // [path0, path0 + S * npix) belong to the job, sample s of pixel q at
-// path0 + s * npix + q.
+// path0 + s * npix + q.  In multi-job waves path0 is a multiple of the
+// wave's segment size (the path ids up to the next job are padding).
 struct Job {
   DScene sc;          // geometry, materials, lights; sc.cam is this job's camera
   int W, H, npix;
@@ -75,7 +76,6 @@ struct Wave {
   int npix;       // (SPPM) pixels of the image
   uint32_t k0;
   int W, H;
-  int *job;       // job index per path (multi-job waves)
   // rays: slot s of path p at s * N + p; slots 0 continuation, 1 MIS, 2 shadow
   float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
   // hits for slots 0 and 1
@@ -107,6 +107,7 @@ struct Wave {
   int seg, nseg;
   int *cnt_path[2];
   int *cnt_ray, *cnt_shadow;
+  const int *seg_job;  // multi-job waves: job index of each segment
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

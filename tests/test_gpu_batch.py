@@ -95,14 +95,16 @@ def test_graph_replay_equals_eager(device, scenes):
             np.testing.assert_array_equal(bits(x), bits(y))
 
 
-@pytest.mark.parametrize("budget", ["wide", "one_job"])
-def test_server_results_equal_alone(device, scenes, budget):
+@pytest.mark.parametrize("budget,policy", [("wide", oa.Server.PACK), ("wide", oa.Server.FAIR),
+                                           ("one_job", oa.Server.PACK)])
+def test_server_results_equal_alone(device, scenes, budget, policy):
     jobs = job_list(scenes)
     alone = [oa.render_batch(device, [j])[0][0] for j in jobs]
     film_bytes = [W * H * 16 for _, W, H, _, _ in jobs]
     # "one_job": no two films fit, so jobs run one after another
     nbytes = 100 * max(film_bytes) if budget == "wide" else min(film_bytes)
     srv = oa.Server(device, nbytes, wave_paths=1)
+    srv.set_policy(policy)
     ids = [srv.submit(*j) for j in jobs]
     srv.drain()
     st = srv.stats()
@@ -124,7 +126,8 @@ def test_server_continuous_batching(device, scenes):
     big = (sc, 512, 512, 64, 5)  # 64 waves of one sample
     small = (scenes["cornell"], 48, 48, 4, 5)
     alone_small = oa.render_batch(device, [small])[0][0]
-    srv = oa.Server(device, 1 << 30, wave_paths=1 << 18)
+    # a wave holds one sample of the big film plus the small job
+    srv = oa.Server(device, 1 << 30, wave_paths=(1 << 18) + (1 << 14))
     a = srv.submit(*big)
     t0 = time.time()
     while srv.stats()["waves"] < 2 and time.time() - t0 < 30:

@@ -31,7 +31,8 @@ NODE_BYTES = 64      # 4-wide node: origin, grid steps, 4 child links, 8-bit chi
 TRI_BYTES = 48       # p0, e1, e2 + primitive id
 RAY_IO_BYTES = 48    # 32 B ray in (o, d, tmax, id) + 16 B hit out (t, prim, u, v)
 SHADOW_IO_BYTES = 36  # 32 B ray in + 4 B visibility out
-TRACE_KERNELS = {"closest": "k_trace<false, 16, false>", "shadow": "k_trace<true, 16, false>"}
+SHADE_BYTES = 292     # per shaded path segment
+TRACE_KERNEL = "k_trace_pair<16, false>"
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_icosphere_L8_1024x1024_32spp.json"
 
 
@@ -70,6 +71,7 @@ def parse():
                     help="offered load of the Poisson stream, as a fraction of the plain path's "
                          "one-at-a-time capacity")
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0)
+    ap.add_argument("--sppm-iters", type=int, default=64, help="SPPM caustics leg (0 = skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (a known-bytes "
                          "kernel to scale the FETCH_SIZE / WRITE_SIZE counters by)")
@@ -176,6 +178,31 @@ def batch_leg(oa, dev, n_jobs, seed):
                    "lds_ring": tune.lds_ring, "queue_segment": tune.queue_segment,
                    "use_graph": tune.use_graph},
         "autotune_s": round(tune_s, 2), "setup_s": round(setup_s, 2),
+    }
+
+
+def sppm_leg(oa, dev, iters):
+    """SPPM on the caustics scene: glass icosphere (81,920 triangles, eta 1.5)
+    on a matte plane under a 30 degree spot light, 512x512, 2^20 photons per
+    iteration, initial radius 0.01, 2^22 hash cells, depth 10."""
+    hs = oa.HostScene.generate("caustics", seed=1, params=[6, 512, 512])
+    ds = oa.DeviceScene(dev, hs)
+    ds.render_sppm(1, 1 << 20, 0.01, 10, hash_size=1 << 22)  # warm
+    ds.film_clear()
+    dev.synchronize()
+    t0 = time.perf_counter()
+    st, _ = ds.render_sppm(iters, 1 << 20, 0.01, 10, hash_size=1 << 22)
+    el = time.perf_counter() - t0
+    return {
+        "workload": f"SPPM caustics, glass icosphere ({hs.n_tris} triangles) on a matte plane, "
+                    f"spot light, 512x512, {iters} iterations x 2^20 photons, r0 0.01, "
+                    f"2^22 hash cells, depth 10",
+        "photons_per_s": round(st.photons / el, 1),
+        "camera_paths_per_s": round(st.paths / el, 1),
+        "ms_per_iteration": round(el * 1e3 / iters, 3),
+        "rays_per_s": round((st.closest_rays + st.shadow_rays) / el, 1),
+        "deposits": int(st.deposits),
+        "seconds": round(el, 3),
     }
 
 
@@ -309,43 +336,48 @@ def main():
     paths = tot["paths"]
     value = paths / el
     rays = tot["closest_rays"] + tot["shadow_rays"]
-    # roofline of the traversal kernels; the headline is the one with the
-    # larger device time (the dominant kernel)
-    kern = {"trace_closest": tot["trace_closest_ms"], "trace_shadow": tot["trace_shadow_ms"],
-            "shade": tot["shade_ms"]}
+    # roofline of the traversal launch (closest-hit and any-hit queues in one
+    # launch per bounce), the dominant kernel by device time
+    kern = {"trace_pair": tot["trace_pair_ms"], "shade": tot["shade_ms"]}
     traffic = load_traffic(a)
 
-    def trace_roofline(kind):
-        rays = tot[f"{kind}_rays"]
-        launches = max(tot[f"trace_{kind}_launches"], 1)
+    def ray_bytes(kind):
         io = RAY_IO_BYTES if kind == "closest" else SHADOW_IO_BYTES
-        nodes, tris = tot[f"node_visits_{kind}"], tot[f"tri_tests_{kind}"]
-        bytes_alg = rays * io + nodes * NODE_BYTES + tris * TRI_BYTES
-        per_launch_s = tot[f"trace_{kind}_ms"] / 1e3 / launches
-        achieved = bytes_alg / launches / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
-        k = TRACE_KERNELS[kind]
-        t = traffic["kernels"].get(k) if traffic else None
-        return {
-            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": t["hbm_bytes_per_dispatch"] if t else None,
-            "kernel": f"{k} ({'closest-hit' if kind == 'closest' else 'any-hit'} traversal, 4-wide "
-                      f"quantized BVH, LDS ring stack)",
-            "algorithmic_bytes_per_launch": round(bytes_alg / launches),
-            "launch_ms": round(per_launch_s * 1e3, 4),
-            "bytes_per_ray": round(bytes_alg / max(rays, 1), 1),
-            "nodes_per_ray": round(nodes / max(rays, 1), 2),
-            "tris_per_ray": round(tris / max(rays, 1), 2),
-            "l2_hit_rate": t["l2_hit_rate"] if t else None,
-        }
+        return (tot[f"{kind}_rays"] * io + tot[f"node_visits_{kind}"] * NODE_BYTES
+                + tot[f"tri_tests_{kind}"] * TRI_BYTES)
 
-    rl = {kind: trace_roofline(kind) for kind in ("closest", "shadow")}
-    dom = "closest" if tot["trace_closest_ms"] >= tot["trace_shadow_ms"] else "shadow"
-    roofline = dict(rl[dom])
-    roofline["kernel_ms"] = {k: round(v, 2) for k, v in kern.items()}
-    roofline["other_trace_kernel"] = rl["shadow" if dom == "closest" else "closest"]
+    def per_ray(kind):
+        rays = max(tot[f"{kind}_rays"], 1)
+        return {"rays": tot[f"{kind}_rays"], "bytes_per_ray": round(ray_bytes(kind) / rays, 1),
+                "nodes_per_ray": round(tot[f"node_visits_{kind}"] / rays, 2),
+                "tris_per_ray": round(tot[f"tri_tests_{kind}"] / rays, 2)}
+
+    launches = max(tot["trace_pair_launches"], 1)
+    bytes_alg = ray_bytes("closest") + ray_bytes("shadow")
+    per_launch_s = tot["trace_pair_ms"] / 1e3 / launches
+    achieved = bytes_alg / launches / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
+    t = traffic["kernels"].get(TRACE_KERNEL) if traffic else None
+    roofline = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": t["hbm_bytes_per_dispatch"] if t else None,
+        "kernel": f"{TRACE_KERNEL} (closest-hit + any-hit traversal, 4-wide quantized BVH, "
+                  f"LDS ring stack)",
+        "algorithmic_bytes_per_launch": round(bytes_alg / launches),
+        "launch_ms": round(per_launch_s * 1e3, 4),
+        "closest": per_ray("closest"), "shadow": per_ray("shadow"),
+        "l2_hit_rate": t["l2_hit_rate"] if t else None,
+        "kernel_ms": {k: round(v, 2) for k, v in kern.items()},
+    }
     if traffic:
         roofline["traffic_source"] = traffic["source"]
+    # whole-path figure: every ray's traversal bytes plus B_shade per shaded
+    # segment (hit 16 + path state 48 + vertices 72 + material 32 + new and
+    # shadow rays 64 + state write 48 + film 12 = 292 B), over the step time
+    path_bytes = bytes_alg + tot["closest_rays"] * SHADE_BYTES
+    roofline["whole_path"] = {"achieved": round(path_bytes / el / 1e9, 1), "unit": "GB/s",
+                              "frac": round(path_bytes / el / 1e9 / HBM_PEAK_GBS, 4),
+                              "bytes_per_path": round(path_bytes / max(tot["paths"], 1), 1)}
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "paths/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(el * 1e3 / a.steps, 3),
@@ -366,6 +398,9 @@ def main():
         "roofline": roofline,
         "scene_gen_s": round(gen_s, 2), "bvh_build_ms": round(sst["build_ms"], 1),
     }
+    if a.sppm_iters > 0:
+        out["sppm"] = sppm_leg(oa, dev, a.sppm_iters)
+        print("# sppm leg done", file=sys.stderr, flush=True)
     if a.batch_jobs > 0:
         out["batched"] = batch_leg(oa, dev, a.batch_jobs, a.seed)
         print(f"# batched leg done: {out['batched']['speedup']}x", file=sys.stderr, flush=True)

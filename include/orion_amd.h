@@ -155,6 +155,10 @@ typedef struct {
   double trace_closest_ms, trace_shadow_ms, shade_ms;
   int64_t trace_closest_launches, trace_shadow_launches, shade_launches;
   int64_t stack_overflows;  /* traversal stack entries dropped (0 for a well-formed BVH) */
+  /* closest-hit + any-hit queues traced by one launch per bounce (the
+   * default; split_trace = 1 launches them separately, timed above) */
+  double trace_pair_ms;
+  int64_t trace_pair_launches;
 } orion_render_stats;
 
 typedef struct {
@@ -165,6 +169,7 @@ typedef struct {
   int profile;          /* time every traversal / shade launch with HIP events */
   int lds_ring;         /* traversal stack entries per lane kept in LDS: 8, 16, 32 (0 = 16) */
   int queue_segment;    /* paths per queue segment, a power of two in [512, 16384] (0 = by wave size) */
+  int split_trace;      /* 1: closest-hit and any-hit queues in separate launches (0: one launch) */
 } orion_tuning;
 
 /* PathTracing::render over samples [k0, k1): accumulates into the scene's film

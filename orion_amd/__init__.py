@@ -45,7 +45,8 @@ class RenderStats(C.Structure):
                 ("tri_tests_shadow", C.c_int64), ("trace_closest_ms", C.c_double),
                 ("trace_shadow_ms", C.c_double), ("shade_ms", C.c_double),
                 ("trace_closest_launches", C.c_int64), ("trace_shadow_launches", C.c_int64),
-                ("shade_launches", C.c_int64), ("stack_overflows", C.c_int64)]
+                ("shade_launches", C.c_int64), ("stack_overflows", C.c_int64),
+                ("trace_pair_ms", C.c_double), ("trace_pair_launches", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -54,7 +55,7 @@ class RenderStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("spp_per_wave", C.c_int), ("wave_paths", C.c_int), ("rays_per_wave", C.c_int),
                 ("use_graph", C.c_int), ("profile", C.c_int), ("lds_ring", C.c_int),
-                ("queue_segment", C.c_int)]
+                ("queue_segment", C.c_int), ("split_trace", C.c_int)]
 
 
 class SppmDebug(C.Structure):
@@ -327,16 +328,19 @@ class DeviceScene:
         return occ
 
     def render_path(self, k0, k1, max_depth, spp_per_wave=0, rays_per_wave=0, profile=False,
-                    use_graph=False, wave_paths=0, lds_ring=0, queue_segment=0, tuning=None):
+                    use_graph=False, wave_paths=0, lds_ring=0, queue_segment=0, split_trace=False,
+                    tuning=None):
         if tuning is not None:
             spp_per_wave, wave_paths = tuning.spp_per_wave, tuning.wave_paths
             rays_per_wave, use_graph = tuning.rays_per_wave, bool(tuning.use_graph)
             lds_ring, queue_segment = tuning.lds_ring, tuning.queue_segment
+            split_trace = bool(tuning.split_trace)
         st = RenderStats()
         tune = None  # NULL: the defaults
-        if spp_per_wave or wave_paths or rays_per_wave or use_graph or profile or lds_ring or queue_segment:
+        if (spp_per_wave or wave_paths or rays_per_wave or use_graph or profile or lds_ring
+                or queue_segment or split_trace):
             tune = C.byref(Tuning(spp_per_wave, wave_paths, rays_per_wave, int(use_graph),
-                                  int(profile), lds_ring, queue_segment))
+                                  int(profile), lds_ring, queue_segment, int(split_trace)))
         _check(lib().orion_render_path(self.h, k0, k1, max_depth, tune, C.byref(st)))
         return st
 
@@ -394,13 +398,13 @@ def autotune(dev, jobs):
 
 
 def render_batch(dev, jobs, wave_paths=0, rays_per_wave=0, use_graph=False, profile=False,
-                 lds_ring=0, queue_segment=0, tuning=None):
+                 lds_ring=0, queue_segment=0, split_trace=False, tuning=None):
     """Renders jobs = [(DeviceScene, width, height, spp, max_depth), ...] in shared
     wavefronts; returns ([(rgb HxWx3, count HxW), ...], RenderStats)."""
     if tuning is not None:
         wave_paths, rays_per_wave = tuning.wave_paths, tuning.rays_per_wave
         use_graph, lds_ring = bool(tuning.use_graph), tuning.lds_ring
-        queue_segment = tuning.queue_segment
+        queue_segment, split_trace = tuning.queue_segment, bool(tuning.split_trace)
     n = len(jobs)
     descs = _descs(jobs)
     outs = []
@@ -414,7 +418,7 @@ def render_batch(dev, jobs, wave_paths=0, rays_per_wave=0, use_graph=False, prof
         cnt_p[i] = cnt.ctypes.data
     st = RenderStats()
     tune = Tuning(0, wave_paths, rays_per_wave, int(use_graph), int(profile), lds_ring,
-                  queue_segment)
+                  queue_segment, int(split_trace))
     _check(lib().orion_render_batch(dev.h, n, descs, rgb_p, cnt_p, C.byref(tune), C.byref(st)))
     return outs, st
 
@@ -424,15 +428,16 @@ class Server:
     per-request times (submitted, admitted, finished; seconds since start)."""
 
     def __init__(self, dev, hbm_budget_bytes, wave_paths=0, use_graph=False, rays_per_wave=0,
-                 lds_ring=0, queue_segment=0, tuning=None):
+                 lds_ring=0, queue_segment=0, split_trace=False, tuning=None):
         if tuning is not None:
             wave_paths, rays_per_wave = tuning.wave_paths, tuning.rays_per_wave
             use_graph, lds_ring = bool(tuning.use_graph), tuning.lds_ring
-            queue_segment = tuning.queue_segment
+            queue_segment, split_trace = tuning.queue_segment, bool(tuning.split_trace)
         self.dev = dev
         self.h = C.c_void_p()
         self._shape = {}
-        tune = Tuning(0, wave_paths, rays_per_wave, int(use_graph), 0, lds_ring, queue_segment)
+        tune = Tuning(0, wave_paths, rays_per_wave, int(use_graph), 0, lds_ring, queue_segment,
+                      int(split_trace))
         _check(lib().orion_server_create(dev.h, int(hbm_budget_bytes), C.byref(tune), C.byref(self.h)))
 
     PACK, FAIR = 0, 1

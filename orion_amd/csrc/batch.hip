@@ -47,7 +47,7 @@ struct LaunchTimer {
   bool on = false;
   hipStream_t st = nullptr;
   std::vector<hipEvent_t> ev;
-  std::vector<int> kind;  // 0 closest, 1 shadow, 2 shade
+  std::vector<int> kind;  // 0 closest, 1 shadow, 2 shade, 3 closest + shadow
   int mark(int k) {
     if (!on) return ORION_OK;
     hipEvent_t e;
@@ -64,6 +64,7 @@ struct LaunchTimer {
       if (!s) continue;
       if (kind[i] == 0) { s->trace_closest_ms += ms; s->trace_closest_launches++; }
       else if (kind[i] == 1) { s->trace_shadow_ms += ms; s->trace_shadow_launches++; }
+      else if (kind[i] == 3) { s->trace_pair_ms += ms; s->trace_pair_launches++; }
       else { s->shade_ms += ms; s->shade_launches++; }
     }
     return ORION_OK;
@@ -74,11 +75,12 @@ struct LaunchTimer {
 };
 
 struct GraphKey {
-  int depth, ring, multi, chunk, cap, spill_cap, seg, nseg;
+  int depth, ring, multi, chunk, cap, spill_cap, seg, nseg, split;
   void *mem, *spill;
   bool operator==(const GraphKey &o) const {
     return depth == o.depth && ring == o.ring && multi == o.multi && chunk == o.chunk && cap == o.cap &&
-           spill_cap == o.spill_cap && seg == o.seg && nseg == o.nseg && mem == o.mem && spill == o.spill;
+           spill_cap == o.spill_cap && seg == o.seg && nseg == o.nseg && split == o.split && mem == o.mem &&
+           spill == o.spill;
   }
 };
 
@@ -142,7 +144,7 @@ struct WaveRunner {
   }
   // traversal stack spill area for a wave of nseg segments of seg paths
   int ensure_spill(int stack_bound, int ring, int chunk, int seg, int nseg) {
-    size_t need = trace_spill_ints(stack_bound, ring, trace_grid(nseg, seg, false, chunk)) * 4 + 4;
+    size_t need = trace_spill_ints(stack_bound, ring, trace_pair_grid(nseg, seg, chunk)) * 4 + 4;
     if (need > spill.bytes) {
       drop_graphs();
       int rc = spill.ensure(need);
@@ -180,16 +182,22 @@ struct WaveRunner {
     return ORION_OK;
   }
 
-  void enqueue_wave(const Job *dj, const Wave &w, int depth, int ring, bool multi, int chunk,
+  void enqueue_wave(const Job *dj, const Wave &w, int depth, int ring, bool multi, int chunk, bool split,
                     hipStream_t st, LaunchTimer *tm) {
     launch_raygen(dj, w, multi, st);
     int cur = 0;
     for (int it = 0; it <= depth; ++it) {
-      if (tm) tm->mark(0);
-      launch_trace(dj, w, false, ring, chunk, multi, st);
-      if (tm) { tm->mark(-1); tm->mark(1); }
-      launch_trace(dj, w, true, ring, chunk, multi, st);
-      if (tm) tm->mark(-1);
+      if (split) {
+        if (tm) tm->mark(0);
+        launch_trace(dj, w, false, ring, chunk, multi, st);
+        if (tm) { tm->mark(-1); tm->mark(1); }
+        launch_trace(dj, w, true, ring, chunk, multi, st);
+        if (tm) tm->mark(-1);
+      } else {
+        if (tm) tm->mark(3);
+        launch_trace_pair(dj, w, ring, chunk, multi, st);
+        if (tm) tm->mark(-1);
+      }
       if (tm) tm->mark(2);
       launch_shade(dj, w, cur, multi, st);
       if (tm) tm->mark(-1);
@@ -202,7 +210,7 @@ struct WaveRunner {
   // seg: queue segment size (0: chosen from npaths); multi-job waves pass
   // the size their job table was aligned to
   int run(hipStream_t st, const Job *jobs, int n, int npaths, int npix, int depth, int ring, int stack_bound,
-          bool multi, int chunk, bool use_graph, LaunchTimer *tm, int seg = 0) {
+          bool multi, int chunk, bool split, bool use_graph, LaunchTimer *tm, int seg = 0) {
     int rc = init(n);
     if (rc) return rc;
     if ((rc = ensure_cap(npaths))) return rc;
@@ -233,18 +241,18 @@ struct WaveRunner {
     w.nseg = nseg;
     w.seg_job = (const int *)segjob_dev.ptr;
     if (!use_graph || tm) {
-      enqueue_wave(dj, w, depth, ring, multi, chunk, st, tm);
+      enqueue_wave(dj, w, depth, ring, multi, chunk, split, st, tm);
       HIP_TRY(hipGetLastError());
       return ORION_OK;
     }
-    GraphKey key{depth, ring, multi ? 1 : 0, chunk, wb.cap, w.spill_cap, seg, nseg, wb.mem.ptr, spill.ptr};
+    GraphKey key{depth, ring, multi ? 1 : 0, chunk, wb.cap, w.spill_cap, seg, nseg, split ? 1 : 0, wb.mem.ptr, spill.ptr};
     hipGraphExec_t exec = nullptr;
     for (auto &g : graphs)
       if (g.first == key) exec = g.second;
     if (!exec) {
       hipGraph_t graph;
       HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      enqueue_wave(dj, w, depth, ring, multi, chunk, st, nullptr);
+      enqueue_wave(dj, w, depth, ring, multi, chunk, split, st, nullptr);
       HIP_TRY(hipStreamEndCapture(st, &graph));
       hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
@@ -263,6 +271,7 @@ void destroy_runner(WaveRunner *r) { delete r; }
 
 static int ring_of(const orion_tuning *t) { return ring_variant(t ? t->lds_ring : 0); }
 static int chunk_of(const orion_tuning *t) { return trace_chunk(t ? t->rays_per_wave : 0); }
+static bool split_of(const orion_tuning *t) { return t && t->split_trace; }
 // queue segment size for a wave of `paths` paths
 static int seg_of(const orion_tuning *t, int paths) {
   int seg, nseg;
@@ -330,8 +339,8 @@ extern "C" int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, co
     Job J = make_job(s, s->W, s->H, maxDepth, (float *)s->film.ptr, (int *)s->count.ptr);
     J.S = Sk;
     J.k0 = (uint32_t)k;
-    if ((rc = R.run(c->stream, &J, 1, Sk * npix, npix, maxDepth, ring, s->d.stack_bound, false, chunk, graph,
-                    tm.on ? &tm : nullptr, seg_of(tune, Sk * npix))))
+    if ((rc = R.run(c->stream, &J, 1, Sk * npix, npix, maxDepth, ring, s->d.stack_bound, false, chunk,
+                    split_of(tune), graph, tm.on ? &tm : nullptr, seg_of(tune, Sk * npix))))
       return rc;
   }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
@@ -526,7 +535,7 @@ extern "C" int orion_render_batch(orion_ctx c, int n, const orion_job_desc *desc
     std::vector<Job> jobs = compose_wave(act, target, seg, kPack, &rr, &npaths, &npix, &depth, &stack, &taken);
     if (jobs.empty()) break;
     if ((rc = R.run(c->stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, ring_of(tune), stack,
-                    jobs.size() > 1, chunk_of(tune), graph, tm.on ? &tm : nullptr, seg)))
+                    jobs.size() > 1, chunk_of(tune), split_of(tune), graph, tm.on ? &tm : nullptr, seg)))
       return rc;
     for (size_t i = 0; i < act.size(); ++i) act[i]->next_k += taken[i];
     act.erase(std::remove_if(act.begin(), act.end(), [](JobState *j) { return j->next_k >= j->d.spp; }), act.end());
@@ -667,7 +676,7 @@ void orion_server_s::loop() {
     if (!rc) rc = runner.reset_stats(stream);
     if (!rc && !jobs.empty())
       rc = runner.run(stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, ring_of(tn), stack,
-                      jobs.size() > 1, chunk_of(tn), tn && tn->use_graph, nullptr, seg);
+                      jobs.size() > 1, chunk_of(tn), split_of(tn), tn && tn->use_graph, nullptr, seg);
     if (!rc && hipStreamSynchronize(stream) != hipSuccess) rc = ORION_ERR_HIP;
     orion_render_stats ws{};
     if (!rc) rc = runner.read_stats(&ws);
@@ -868,6 +877,11 @@ int orion_autotune(orion_ctx c, int n, const orion_job_desc *jobs, orion_tuning 
   {
     orion_tuning t = b;
     t.use_graph = 1;
+    if ((rc = trial(t))) return rc;
+  }
+  {
+    orion_tuning t = b;
+    t.split_trace = 1;
     if ((rc = trial(t))) return rc;
   }
   if (b.rays_per_wave == 0) b.rays_per_wave = kDefaultChunk;

@@ -18,6 +18,9 @@ void launch_raygen(const Job *jobs, const Wave &w, bool multi, hipStream_t st);
 // wave64.  w.spill must hold w.spill_cap entries for each of
 // trace_grid(w.nseg, w.seg, any, chunk) * kTraceBlock threads.
 void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk, bool multi, hipStream_t st);
+// closest-hit and any-hit queues in one launch (trace_pair_grid blocks)
+void launch_trace_pair(const Job *jobs, const Wave &w, int ring, int chunk, bool multi, hipStream_t st);
+int trace_pair_grid(int nseg, int seg, int chunk);
 int ring_variant(int ring);
 int trace_chunk(int chunk);
 // blocks of kTraceBlock threads covering nseg queue segments (closest

@@ -96,15 +96,16 @@ struct WaveRays {
   }
 };
 
-// Wave64 number g traces entries [c * chunk, (c + 1) * chunk) of queue
-// segment g / (waves per segment), c = g mod (waves per segment); the grid
-// covers every segment at full length and waves past a segment's length exit.
+// Wave64 number g of a launch part traces entries [c * chunk, (c + 1) *
+// chunk) of queue segment g / (waves per segment), c = g mod (waves per
+// segment); the part covers every segment at full length and waves past a
+// segment's length exit.  `lb` is the block's logical index within its part
+// (XCD-contiguous), `ring` the block's LDS.
 template <bool ANY, int RING, bool MULTI>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w, int chunk) {
-  __shared__ int ring[RING * kTraceBlock];
+__device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int chunk, int lb, int *ring) {
   const int seg_entries = ANY ? w.seg : 2 * w.seg;
   const int wps = (seg_entries + chunk - 1) / chunk;
-  const int gw = xcd_block() * (kTraceBlock / 64) + (int)(threadIdx.x >> 6);
+  const int gw = lb * (kTraceBlock / 64) + (int)(threadIdx.x >> 6);
   const int sg = gw / wps;
   if (sg >= w.nseg) return;
   const int count = (ANY ? w.cnt_shadow : w.cnt_ray)[sg];
@@ -130,6 +131,25 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w, 
     stat_add(w.stats, ANY ? S_NODES_S : S_NODES_C, nv);
     stat_add(w.stats, ANY ? S_TRIS_S : S_TRIS_C, nt);
   }
+}
+
+template <bool ANY, int RING, bool MULTI>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w, int chunk) {
+  __shared__ int ring[RING * kTraceBlock];
+  trace_block<ANY, RING, MULTI>(jobs, w, chunk, xcd_remap(blockIdx.x, gridDim.x), ring);
+}
+
+// Closest-hit and any-hit queues of a bounce in one launch: blocks
+// [0, n_closest) take the closest-hit part, the rest the any-hit part, so
+// the any-hit blocks fill the GPU while the closest-hit part drains (and
+// the other way round).  n_closest is a multiple of the XCD count, so a
+// block's XCD is the same in its part's numbering.
+template <int RING, bool MULTI>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_pair(const Job *jobs, Wave w, int chunk, int n_closest) {
+  __shared__ int ring[RING * kTraceBlock];
+  const int b = blockIdx.x;
+  if (b < n_closest) trace_block<false, RING, MULTI>(jobs, w, chunk, xcd_remap(b, n_closest), ring);
+  else trace_block<true, RING, MULTI>(jobs, w, chunk, xcd_remap(b - n_closest, gridDim.x - n_closest), ring);
 }
 
 __global__ void k_between(Wave w, int nxt) {
@@ -393,6 +413,20 @@ void launch_raygen(const Job *jobs, const Wave &w, bool multi, hipStream_t st) {
   if (multi) hipLaunchKernelGGL(k_raygen<true>, g, dim3(256), 0, st, jobs, w);
   else hipLaunchKernelGGL(k_raygen<false>, g, dim3(256), 0, st, jobs, w);
 }
+void launch_trace_pair(const Job *jobs, const Wave &w, int ring, int chunk, bool multi, hipStream_t st) {
+  ring = ring_variant(ring);
+  chunk = trace_chunk(chunk);
+  const int nc = (trace_grid(w.nseg, w.seg, false, chunk) + kXcds - 1) / kXcds * kXcds;
+  const dim3 g(nc + trace_grid(w.nseg, w.seg, true, chunk));
+  if (multi) ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace_pair<ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk, nc));
+  else ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace_pair<ST, false>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk, nc));
+}
+
+int trace_pair_grid(int nseg, int seg, int chunk) {
+  chunk = trace_chunk(chunk);
+  return (trace_grid(nseg, seg, false, chunk) + kXcds - 1) / kXcds * kXcds + trace_grid(nseg, seg, true, chunk);
+}
+
 void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk, bool multi, hipStream_t st) {
   ring = ring_variant(ring);
   chunk = trace_chunk(chunk);

@@ -118,12 +118,12 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 // blocks: neighbouring queue segments (neighbouring pixels, coherent rays)
 // then share one L2 instead of being fetched into all eight.
 constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_block() {
-  const int G = gridDim.x, b = blockIdx.x;
+__device__ __forceinline__ int xcd_remap(int b, int G) {
   const int per = G / kXcds, rem = G % kXcds;
   const int x = b % kXcds, k = b / kXcds;
   return x * per + min(x, rem) + k;
 }
+__device__ __forceinline__ int xcd_block() { return xcd_remap(blockIdx.x, gridDim.x); }
 
 __device__ __forceinline__ void stat_add(unsigned long long *stats, int s, unsigned long long v) {
   if (stats && v) atomicAdd(&stats[s * kStatStripes + (blockIdx.x & (kStatStripes - 1))], v);

@@ -177,44 +177,38 @@ __device__ __forceinline__ int node_step(const TRay &ray, float best, int node, 
   return nh > 0 ? ck[0] : st.pop();
 }
 
-// Leaf: triangle tests, keeping the lexicographic minimum (t, prim) and the
-// tie flag.  Any-hit returns true at the first accepted triangle.
+// One triangle of a leaf: the triangle test, keeping the lexicographic
+// minimum (t, prim) and the tie flag.  Any-hit returns true when accepted.
 template <bool ANY>
-__device__ __forceinline__ bool leaf_step(const TRay &ray, float *best, int node, HitRec *h, bool *found,
-                                          uint32_t *tri_tests) {
-  const int code = ~node;
-  const int first = code >> 3, cnt = (code & 7) + 1;
-  for (int k = first; k < first + cnt; ++k) {
-    const float4 *tr = ray.tris + 3 * (size_t)k;
-    float4 a = gld(tr), b = gld(tr + 1), c = gld(tr + 2);
-    ++*tri_tests;
-    float t, u, v;
-    if (!tri_test(a, b, c, ray.o, ray.d, ray.tmin, ANY ? ray.tmax : *best, &t, &u, &v)) continue;
-    int prim = __float_as_int(a.w);
-    if (ANY) {
-      h->t = t; h->u = u; h->v = v; h->prim = prim;
-      *found = true;
-      return true;
-    }
-    if (!*found || t < *best) {
-      *best = t; h->t = t; h->u = u; h->v = v; h->prim = prim; h->tie = 0; *found = true;
-    } else if (t == *best) {
-      h->tie = 1;
-      if (prim < h->prim) { h->u = u; h->v = v; h->prim = prim; }
-    }
+__device__ __forceinline__ bool tri_step(const TRay &ray, int k, float *best, HitRec *h, bool *found) {
+  const float4 *tr = ray.tris + 3 * (size_t)k;
+  float4 a = gld(tr), b = gld(tr + 1), c = gld(tr + 2);
+  float t, u, v;
+  if (!tri_test(a, b, c, ray.o, ray.d, ray.tmin, ANY ? ray.tmax : *best, &t, &u, &v)) return false;
+  int prim = __float_as_int(a.w);
+  if (ANY) {
+    h->t = t; h->u = u; h->v = v; h->prim = prim;
+    *found = true;
+    return true;
+  }
+  if (!*found || t < *best) {
+    *best = t; h->t = t; h->u = u; h->v = v; h->prim = prim; h->tie = 0; *found = true;
+  } else if (t == *best) {
+    h->tie = 1;
+    if (prim < h->prim) { h->u = u; h->v = v; h->prim = prim; }
   }
   return false;
 }
 
-// Traversal engine for one wave over queue entries [begin, end): a lane
-// whose ray finishes takes the next entry once at least kRefillMin lanes are
-// idle, so the wave's time is the sum of its rays' work spread over 64 lanes
-// rather than the longest ray of each group of 64.  Per step every active
-// lane descends through inner nodes to its next leaf, then tests that leaf.
-// Waves own fixed ranges (no shared work counter: device-scope atomics on
-// one address serialise across the XCDs and cost more than they balance).
-// IO: void load(int entry, TRay *ray); void store(const TRay &, bool hit,
-// const HitRec &).
+// Traversal engine for one wave over queue entries [begin, end).  Every
+// iteration each lane does one unit of work -- one inner-node visit or one
+// triangle of its open leaf -- so lanes never wait for the slowest lane to
+// reach a leaf (with a descend-then-intersect loop a wave64 kept ~16% of
+// its lanes busy).  A lane whose ray finishes takes the next entry once at
+// least kRefillMin lanes are idle.  Waves own fixed ranges (no shared work
+// counter: device-scope atomics on one address serialise across the XCDs
+// and cost more than they balance).  IO: void load(int entry, TRay *ray);
+// void store(const TRay &, bool hit, const HitRec &).
 constexpr int kRefillMin = 8;
 
 template <bool ANY, int RING, class IO>
@@ -230,7 +224,8 @@ __device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> 
   h.tie = 0;
   bool found = false;
   float best = 0.f;
-  int node = kLeafEmpty;
+  int node = kLeafEmpty;  // next inner node / leaf to visit
+  int tcur = 0, tend = 0;  // triangles of the open leaf
   uint32_t nv = 0, nt = 0;
   for (;;) {
     const uint64_t idle = __ballot(!active);
@@ -246,17 +241,29 @@ __device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> 
         h.prim = -1;
         h.tie = 0;
         node = 0;
+        tcur = tend = 0;
         st.top = st.base = 0;
       }
       next = min(next + nidle, end);
     }
     if (__ballot(active) == 0) break;
-    while (node >= 0) {
+    if (tcur < tend) {
+      ++nt;
+      if (tri_step<ANY>(ray, tcur++, &best, &h, &found)) {
+        tcur = tend;  // any-hit: occluded, done
+        node = kLeafEmpty;
+      }
+    } else if (node >= 0) {
       ++nv;
       node = node_step<ANY, RING>(ray, best, node, st, overflow);
     }
-    if (node != kLeafEmpty) node = leaf_step<ANY>(ray, &best, node, &h, &found, &nt) ? kLeafEmpty : st.pop();
-    if (active && node == kLeafEmpty) {
+    if (tcur >= tend && node < 0 && node != kLeafEmpty) {  // open the leaf reached
+      const int code = ~node;
+      tcur = code >> 3;
+      tend = tcur + (code & 7) + 1;
+      node = st.pop();
+    }
+    if (active && tcur >= tend && node == kLeafEmpty) {
       io.store(ray, found, h);
       active = false;
     }

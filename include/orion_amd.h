@@ -24,10 +24,15 @@
  *   triangle  18 floats  p0 p1 p2 n0 n1 n2, world space (Mesh de-indexed,
  *                        core/triangle.h:16-25, after createTriangleMesh)
  *   tri_mat   material index per triangle; tri_light area-light index or -1
- *   material  16 floats  [0] type: 0 matte, 1 metal, 2 glass
- *             matte: [1..3] Kd  [4] sigma (must be 0: Lambert)
- *             metal: [1..3] eta  [4..6] k  [7] roughness  [8] alpha
- *             glass: [1..3] Kr  [4..6] Kt  [7] roughness  [8] alpha  [9] eta
+ *   material  ORION_MAT_FLOATS (24) floats  [0] type: 0 matte, 1 metal,
+ *             2 glass, 3 mirror, 4 plastic, 5 uber
+ *             matte:   [1..3] Kd  [4] sigma (0: Lambert, else Oren-Nayar)
+ *             metal:   [1..3] eta  [4..6] k  [7] roughness  [8] alpha
+ *             glass:   [1..3] Kr  [4..6] Kt  [7] roughness  [8] alpha  [9] eta
+ *             mirror:  [1..3] Kr
+ *             plastic: [1..3] Kd  [4..6] Ks  [7] roughness  [8] alpha
+ *             uber:    [1..3] Kd  [4..6] Ks  [7..9] Kr  [10..12] Kt
+ *                      [13..15] opacity  [16] roughness  [17] alpha  [18] eta
  *             (alpha = GGX::roughnessToAlpha(roughness), microfacet.h:44-50,
  *             evaluated once on the host)
  *   light     32 floats  [0] type: 0 area, 1 point, 2 spot  [1] triangle index
@@ -40,6 +45,8 @@
 #ifndef ORION_AMD_H_
 #define ORION_AMD_H_
 #include <stdint.h>
+
+#define ORION_MAT_FLOATS 24
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -73,11 +80,12 @@ typedef struct {
 } orion_scene_info;
 
 /* XML .ori loader (util/parser.cpp): Camera, Model (mesh | meshHandy),
- * Material (matte | metal | glass, named/name), AreaLight, Light (point |
- * spot), Integrator (path | sppm), Sampler (sobol). */
+ * Material (matte | metal | glass | mirror | plastic | uber, named/name),
+ * AreaLight, Light (point | spot), Integrator (path | sppm), Sampler (sobol). */
 int orion_hscene_load_ori(const char *path, orion_hscene *out);
 /* Synthetic scenes: name is one of "cornell", "icosphere", "caustics",
- * "batch_job"; params (may be NULL) override defaults:
+ * "batch_job", "materials"; params (may be NULL) override defaults:
+ *   cornell, materials: [0] width [1] height
  *   icosphere: [0] level [1] width [2] height
  *   caustics:  [0] level [1] width [2] height
  *   batch_job: [0] triangle count [1] width [2] height [3] material mix */

@@ -232,8 +232,9 @@ struct Ray { V3 o, d; float tmin, tmax; };
 
 // ---------------------------------------------------------------- scene
 struct Tri { V3 p0, p1, p2, n0, n1, n2; int mat, light; };
-struct Material { int type; float v[15]; };
-enum { MAT_MATTE = 0, MAT_METAL = 1, MAT_GLASS = 2 };
+constexpr int kMatFloats = 24;  // material record: type + 23 parameters
+struct Material { int type; float v[kMatFloats - 1]; };
+enum { MAT_MATTE = 0, MAT_METAL = 1, MAT_GLASS = 2, MAT_MIRROR = 3, MAT_PLASTIC = 4, MAT_UBER = 5 };
 enum { LIGHT_AREA = 0, LIGHT_POINT = 1, LIGHT_SPOT = 2 };
 struct Light {
   int type, prim;
@@ -545,21 +546,23 @@ inline float roughness_to_alpha(float roughness) {  // microfacet.h:44-50
          0.000640711f * x * x * x * x;
 }
 
-enum { BX_LAMBERT = 0, BX_MF_REFL = 1, BX_MF_TRANS = 2, BX_FRESNEL_SPEC = 3 };
-enum { FR_CONDUCTOR = 0, FR_DIELECTRIC = 1 };
+enum { BX_LAMBERT = 0, BX_MF_REFL = 1, BX_MF_TRANS = 2, BX_FRESNEL_SPEC = 3, BX_OREN_NAYAR = 4,
+       BX_SPEC_REFL = 5, BX_SPEC_TRANS = 6 };
+enum { FR_CONDUCTOR = 0, FR_DIELECTRIC = 1, FR_NOOP = 2 };
 struct BxDF {
   int kind, type;
   S R, T;
   float alpha;
-  int fresnel;        // for BX_MF_REFL
+  int fresnel;        // BX_MF_REFL / BX_SPEC_REFL
   S etaT, k;          // conductor
   float etaA, etaB;   // dielectric / transmission
   int photon;         // TransportMode::Photon
+  float A, B;         // Oren-Nayar
 };
 struct BSDF {
   V3 ng, ns, u, v;
   int n = 0;
-  BxDF b[2];
+  BxDF b[5];          // Uber adds up to five lobes
   V3 to_local(V3 w) const { return mk(dot(w, u), dot(w, v), dot(w, ns)); }  // bsdf.h:117-120
   V3 to_world(V3 w) const {  // bsdf.h:122-127
     return mk(u.x * w.x + v.x * w.y + ns.x * w.z, u.y * w.x + v.y * w.y + ns.y * w.z,
@@ -603,6 +606,7 @@ float fr_dielectric(float cosThetaI, float etaI, float etaT) {
 }
 inline S fresnel_eval(const BxDF &b, float cosI) {  // bsdf.cpp:172-180: |cos| passed
   if (b.fresnel == FR_CONDUCTOR) return fr_conductor(std::abs(cosI), sp(1.0f), b.etaT, b.k);
+  if (b.fresnel == FR_NOOP) return sp(1.0f);  // bsdf.h:96-99
   return sp(fr_dielectric(std::abs(cosI), b.etaA, b.etaB));
 }
 
@@ -653,6 +657,26 @@ inline bool refract(V3 wi, V3 n, float eta, V3 *wt) {  // bsdf.h:105-115
 S bxdf_f(const BxDF &b, V3 wi, V3 wo) {
   switch (b.kind) {
     case BX_LAMBERT: return b.R * kInvPi;  // lambert.cpp:5-8
+    case BX_OREN_NAYAR: {  // orennayar.cpp:12-37
+      float sinThetaI = std::sqrt(sin2t(wi)), sinThetaO = std::sqrt(sin2t(wo));
+      float cosTerm = 0;
+      if (sinThetaI > 1e-4 && sinThetaO > 1e-4) {  // float promoted to double, as in C++
+        // sampling.h:22-29: sinPhi / cosPhi divide by sin(theta) unconditionally
+        float sinPhiI = clampf(wi.y / sinThetaI, -1.0f, 1.0f), cosPhiI = clampf(wi.x / sinThetaI, -1.0f, 1.0f);
+        float sinPhiO = clampf(wo.y / sinThetaO, -1.0f, 1.0f), cosPhiO = clampf(wo.x / sinThetaO, -1.0f, 1.0f);
+        float dCos = cosPhiI * cosPhiO + sinPhiI * sinPhiO;
+        cosTerm = fmax_ref(0.0f, dCos);
+      }
+      float sinAlpha, tanBeta;
+      if (std::abs(wi.z) > std::abs(wo.z)) {
+        sinAlpha = sinThetaO;
+        tanBeta = sinThetaI / std::abs(wi.z);
+      } else {
+        sinAlpha = sinThetaI;
+        tanBeta = sinThetaO / std::abs(wo.z);
+      }
+      return b.R * kInvPi * (b.A + b.B * cosTerm * sinAlpha * tanBeta);
+    }
     case BX_MF_REFL: {  // microfacet.cpp:55-66
       float cosThetaO = std::abs(wo.z), cosThetaI = std::abs(wi.z);
       if (cosThetaI == 0 || cosThetaO == 0) return sp(0);
@@ -682,7 +706,8 @@ S bxdf_f(const BxDF &b, V3 wi, V3 wo) {
 }
 float bxdf_pdf(const BxDF &b, V3 wi, V3 wo) {
   switch (b.kind) {
-    case BX_LAMBERT: return same_hemi(wi, wo) ? std::abs(wi.z) * kInvPi : 0;  // bsdf.cpp:117-121
+    case BX_LAMBERT:
+    case BX_OREN_NAYAR: return same_hemi(wi, wo) ? std::abs(wi.z) * kInvPi : 0;  // bsdf.cpp:117-121
     case BX_MF_REFL: {  // microfacet.cpp:68-74
       if (!same_hemi(wi, wo)) return 0;
       V3 wh = normalize(add(wi, wo));
@@ -703,7 +728,8 @@ float bxdf_pdf(const BxDF &b, V3 wi, V3 wo) {
 // writing it, as in the reference), *stype
 S bxdf_sample(const BxDF &b, V3 wo, float u0, float u1, V3 *wi, float *pdf, int *stype) {
   switch (b.kind) {
-    case BX_LAMBERT: {  // bsdf.cpp:109-115
+    case BX_LAMBERT:
+    case BX_OREN_NAYAR: {  // bsdf.cpp:109-115
       *wi = cosine_hemisphere(u0, u1);
       if (wo.z < 0) wi->z *= -1;
       *pdf = bxdf_pdf(b, *wi, wo);
@@ -723,6 +749,23 @@ S bxdf_sample(const BxDF &b, V3 wo, float u0, float u1, V3 *wi, float *pdf, int 
       float eta = wo.z > 0 ? (b.etaA / b.etaB) : (b.etaB / b.etaA);
       if (!refract(wo, wh, eta, wi)) return sp(0);
       return bxdf_f(b, *wi, wo);
+    }
+    case BX_SPEC_REFL: {  // specular.cpp:19-24
+      *wi = mk(-wo.x, -wo.y, wo.z);
+      *pdf = 1;
+      return fresnel_eval(b, wi->z) * b.R / std::abs(wi->z);
+    }
+    case BX_SPEC_TRANS: {  // specular.cpp:5-17 (FresnelDielectric(etaA, etaB))
+      bool entering = wo.z > 0;
+      float etaI = entering ? b.etaA : b.etaB;
+      float etaT = entering ? b.etaB : b.etaA;
+      V3 n = mk(0, 0, 1);
+      if (dot(n, wo) < 0.f) n = neg(n);  // faceforward
+      if (!refract(wo, n, etaI / etaT, wi)) return sp(0);
+      *pdf = 1.0f;
+      S ft = b.T * (sp(1.0f) - sp(fr_dielectric(std::abs(wi->z), b.etaA, b.etaB)));
+      if (!b.photon) ft *= sp((etaI * etaI) / (etaT * etaT));
+      return ft / std::abs(wi->z);
     }
     case BX_FRESNEL_SPEC: {  // specular.cpp:28-54
       float F = fr_dielectric(wo.z, b.etaA, b.etaB);
@@ -804,14 +847,78 @@ S bsdf_sample(const BSDF &bs, V3 *wi, V3 wo, float u0, float u1, float *pdf, int
   return f;
 }
 
-// material/matte.cpp:9-21, material/metal.cpp:8-17, material/glass.cpp:9-37
+// material/matte.cpp:9-21, material/metal.cpp:8-17, material/glass.cpp:9-37,
+// material/mirror.cpp:7-14, material/plastic.cpp:8-24, material/uber.cpp:9-42
 void make_bsdf(const Scene &sc, int matIdx, V3 ng, V3 ns, V3 u, V3 v, bool photon, BSDF *bs) {
   bs->ng = ng; bs->ns = ns; bs->u = u; bs->v = v; bs->n = 0;
   const Material &m = sc.mats[matIdx];
   if (m.type == MAT_MATTE) {
     S r = sclamp0(sp(m.v[0], m.v[1], m.v[2]));
+    // Spectrum::clamp(0, 90) clamps to [0, max float] (spectrum.h:89-93)
+    float sig = clampf(m.v[3], 0.0f, kInf);
     if (!black(r)) {
-      BxDF b{}; b.kind = BX_LAMBERT; b.type = REFL | DIFF; b.R = r;
+      BxDF b{}; b.type = REFL | DIFF; b.R = r;
+      if (sig == 0) {
+        b.kind = BX_LAMBERT;
+      } else {  // orennayar.cpp:4-10
+        b.kind = BX_OREN_NAYAR;
+        float sigma = sig * 0.01745329251994329576923690768489f;  // radians()
+        float sigma2 = sigma * sigma;
+        b.A = 1.f - (sigma2 / (2.f * (sigma2 + 0.33f)));
+        b.B = 0.45f * sigma2 / (sigma2 + 0.09f);
+      }
+      bs->b[bs->n++] = b;
+    }
+  } else if (m.type == MAT_MIRROR) {
+    S R = sclamp0(sp(m.v[0], m.v[1], m.v[2]));
+    if (!black(R)) {
+      BxDF b{}; b.kind = BX_SPEC_REFL; b.type = REFL | SPEC; b.R = R; b.fresnel = FR_NOOP;
+      bs->b[bs->n++] = b;
+    }
+  } else if (m.type == MAT_PLASTIC) {
+    S kd = sclamp0(sp(m.v[0], m.v[1], m.v[2]));
+    if (!black(kd)) {
+      BxDF b{}; b.kind = BX_LAMBERT; b.type = REFL | DIFF; b.R = kd;
+      bs->b[bs->n++] = b;
+    }
+    S ks = sclamp0(sp(m.v[3], m.v[4], m.v[5]));
+    if (!black(ks)) {
+      BxDF b{}; b.kind = BX_MF_REFL; b.type = REFL | GLOSSY; b.R = ks;
+      b.alpha = roughness_to_alpha(m.v[6]);
+      b.fresnel = FR_DIELECTRIC; b.etaA = 1.5f; b.etaB = 1.0f;
+      bs->b[bs->n++] = b;
+    }
+  } else if (m.type == MAT_UBER) {
+    float e = m.v[17];
+    S op = sp(m.v[12], m.v[13], m.v[14]);
+    S t = sclamp0(sp(1.0f) - op);
+    if (!black(t)) {
+      BxDF b{}; b.kind = BX_SPEC_TRANS; b.type = TRANS | SPEC; b.T = t;
+      b.etaA = 1.0f; b.etaB = 1.0f; b.photon = photon;
+      bs->b[bs->n++] = b;
+    }
+    S kd = op * sclamp0(sp(m.v[0], m.v[1], m.v[2]));
+    if (!black(kd)) {
+      BxDF b{}; b.kind = BX_LAMBERT; b.type = REFL | DIFF; b.R = kd;
+      bs->b[bs->n++] = b;
+    }
+    S ks = op * sclamp0(sp(m.v[3], m.v[4], m.v[5]));
+    if (!black(ks)) {
+      BxDF b{}; b.kind = BX_MF_REFL; b.type = REFL | GLOSSY; b.R = ks;
+      b.alpha = roughness_to_alpha(m.v[15]);
+      b.fresnel = FR_DIELECTRIC; b.etaA = 1.0f; b.etaB = e;
+      bs->b[bs->n++] = b;
+    }
+    S kr = op * sclamp0(sp(m.v[6], m.v[7], m.v[8]));
+    if (!black(kr)) {
+      BxDF b{}; b.kind = BX_SPEC_REFL; b.type = REFL | SPEC; b.R = kr;
+      b.fresnel = FR_DIELECTRIC; b.etaA = 1.0f; b.etaB = e;
+      bs->b[bs->n++] = b;
+    }
+    S kt = op * sclamp0(sp(m.v[9], m.v[10], m.v[11]));
+    if (!black(kt)) {
+      BxDF b{}; b.kind = BX_SPEC_TRANS; b.type = TRANS | SPEC; b.T = kt;
+      b.etaA = 1.0f; b.etaB = e; b.photon = photon;
       bs->b[bs->n++] = b;
     }
   } else if (m.type == MAT_METAL) {
@@ -1122,8 +1229,8 @@ void *oracle_scene_create(int n_tris, const float *pn, const int *tri_mat, const
   }
   sc->mats.resize(n_mats);
   for (int i = 0; i < n_mats; ++i) {
-    sc->mats[i].type = (int)mats[16 * i];
-    for (int k = 0; k < 15; ++k) sc->mats[i].v[k] = mats[16 * i + 1 + k];
+    sc->mats[i].type = (int)mats[kMatFloats * (size_t)i];
+    for (int k = 0; k < kMatFloats - 1; ++k) sc->mats[i].v[k] = mats[kMatFloats * (size_t)i + 1 + k];
   }
   sc->lights.resize(n_lights);
   std::vector<float> pw(n_lights);

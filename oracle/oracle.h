@@ -25,7 +25,7 @@ extern "C" {
 
 /* Scene handle.  tri_pn: n_tris x 18 floats (p0,p1,p2,n0,n1,n2, world space).
  * tri_mat: material index per triangle.  tri_light: area-light index or -1.
- * mats: n_mats x 16 floats.  lights: n_lights x 32 floats.  camera: 16 floats. */
+ * mats: n_mats x 24 floats.  lights: n_lights x 32 floats.  camera: 16 floats. */
 void *oracle_scene_create(int n_tris, const float *tri_pn, const int *tri_mat,
                           const int *tri_light, int n_mats, const float *mats,
                           int n_lights, const float *lights, const float *camera);

@@ -3,7 +3,7 @@
 // TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile from the translation
 // units under /root/reference/src that compile and link without Embree
 // (sampler/, bsdf/, core/bsdf, core/spectrum, core/transform, material/
-// matte|metal|glass, light/point|spot, camera/pinhole, texture/, util/strutil);
+// matte|metal|glass|mirror|plastic|uber, light/point|spot, camera/pinhole, texture/, util/strutil);
 // core/triangle.cpp and light/diffuse.cpp are excluded because they need
 // Scene::intersect -> EmbreeKernel, whose library ships only as a Windows
 // binary.  The harness reads private members through `#define private public`
@@ -41,6 +41,9 @@
 #include <material/matte.h>
 #include <material/metal.h>
 #include <material/glass.h>
+#include <material/mirror.h>
+#include <material/plastic.h>
+#include <material/uber.h>
 #include <texture/constant.h>
 #include <texture/floattexture.h>
 #include <light/point.h>
@@ -76,12 +79,20 @@ Intersection make_isect(const float *frame) {
   return is;
 }
 
-// kind 0 matte(Kd), 1 metal(eta,k,rough) or metal preset by name, 2 glass(Kr,Kt,rough,eta)
+// Parameters are a material record without its type word:
+// kind 0 matte(Kd, sigma), 1 metal(eta, k, rough) or metal preset by name,
+// 2 glass(Kr, Kt, rough, -, eta), 3 mirror(Kr), 4 plastic(Kd, Ks, rough),
+// 5 uber(Kd, Ks, Kr, Kt, opacity, rough, -, eta)
+std::shared_ptr<ConstantTexture> ctex(const float *p) {
+  return std::make_shared<ConstantTexture>(Spectrum(p[0], p[1], p[2]));
+}
 std::shared_ptr<Material> make_material(int kind, const float *p, const char *metal_name) {
-  if (kind == 0) {
-    auto kd = std::make_shared<ConstantTexture>(Spectrum(p[0], p[1], p[2]));
-    return createMatteMaterial(kd, nullptr);
-  }
+  if (kind == 0) return createMatteMaterial(ctex(p), std::make_shared<FloatTexture>(p[3]));
+  if (kind == 3) return createMirrorMaterial(ctex(p));
+  if (kind == 4) return createPlasticMaterial(ctex(p), ctex(p + 3), std::make_shared<FloatTexture>(p[6]));
+  if (kind == 5)
+    return createUberMaterial(ctex(p), ctex(p + 3), ctex(p + 6), ctex(p + 9), ctex(p + 12),
+                              std::make_shared<FloatTexture>(p[15]), std::make_shared<FloatTexture>(p[17]));
   if (kind == 1) {
     auto r = std::make_shared<FloatTexture>(p[6]);
     if (metal_name && metal_name[0]) {

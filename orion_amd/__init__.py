@@ -13,6 +13,7 @@ import numpy as np
 
 _PKG = Path(__file__).resolve().parent
 LIB_PATH = _PKG / "liborion_amd.so"
+MAT_FLOATS = 24  # ORION_MAT_FLOATS: floats per material record
 
 f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -188,7 +189,7 @@ class HostScene:
         camera = np.ascontiguousarray(camera, np.float32)
         h = C.c_void_p()
         _check(lib().orion_hscene_from_records(len(tri_mat), tri_pn, tri_mat, tri_light,
-                                               len(mats) // 16, mats, len(lights) // 32, lights,
+                                               len(mats) // MAT_FLOATS, mats, len(lights) // 32, lights,
                                                camera, C.byref(h)))
         return cls(h)
 
@@ -208,7 +209,7 @@ class HostScene:
         self.tri_pn = view(ptrs[0], 18 * n, np.float32)
         self.tri_mat = view(ptrs[1], n, np.int32)
         self.tri_light = view(ptrs[2], n, np.int32)
-        self.mats = view(ptrs[3], 16 * m, np.float32)
+        self.mats = view(ptrs[3], MAT_FLOATS * m, np.float32)
         self.lights = view(ptrs[4], 32 * l, np.float32)
         self.camera = view(ptrs[5], 16, np.float32)
 

@@ -51,7 +51,7 @@ int orion_hscene_from_records(int n_tris, const float *tri_pn, const int *tri_ma
   s->tri_pn.assign(tri_pn, tri_pn + 18 * (size_t)n_tris);
   s->tri_mat.assign(tri_mat, tri_mat + n_tris);
   s->tri_light.assign(tri_light, tri_light + n_tris);
-  s->mats.assign(mats, mats + 16 * (size_t)n_mats);
+  s->mats.assign(mats, mats + kMatFloats * (size_t)n_mats);
   s->lights.assign(lights, lights + 32 * (size_t)n_lights);
   std::memcpy(s->camera, camera, 16 * sizeof(float));
   s->refresh_info();

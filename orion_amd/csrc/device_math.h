@@ -127,11 +127,9 @@ __device__ __forceinline__ V3 mat3(const float *m, V3 v) {
 }
 
 // ---------------------------------------------------------------- BSDFs
-// A material record resolves to at most two lobes (Matte: Lambert; Metal:
-// GGX reflection with a conductor Fresnel; Glass: FresnelSpecular, or GGX
-// reflection + GGX transmission when rough).
-enum { BX_LAMBERT = 0, BX_MF_REFL = 1, BX_MF_TRANS = 2, BX_FRESNEL_SPEC = 3 };
-enum { FR_CONDUCTOR = 0, FR_DIELECTRIC = 1 };
+enum { BX_LAMBERT = 0, BX_MF_REFL = 1, BX_MF_TRANS = 2, BX_FRESNEL_SPEC = 3, BX_OREN_NAYAR = 4,
+       BX_SPEC_REFL = 5, BX_SPEC_TRANS = 6 };
+enum { FR_CONDUCTOR = 0, FR_DIELECTRIC = 1, FR_NOOP = 2 };
 // Load through a global-address-space pointer: scene arrays reach the
 // kernels inside a job table in memory, where the compiler cannot prove
 // they are global and would emit flat loads (which also hold the LDS
@@ -167,22 +165,13 @@ __device__ __forceinline__ uint2 gld<uint2>(const uint2 *p) {
   return make_uint2(v.x, v.y);
 }
 
+
+// One BxDF with its parameters (bsdf/*.h); A, B are Oren-Nayar's.
 struct Lobe {
   int kind, type, fresnel, photon;
   Sp R, T, etaT, k;
-  float alpha, etaA, etaB;
+  float alpha, etaA, etaB, A, B;
 };
-struct Bsdf {
-  V3 ng, ns, u, v;
-  int n;
-  Lobe b[2];
-  __device__ __forceinline__ V3 to_local(V3 w) const { return mk(dot(w, u), dot(w, v), dot(w, ns)); }
-  __device__ __forceinline__ V3 to_world(V3 w) const {
-    return mk(u.x * w.x + v.x * w.y + ns.x * w.z, u.y * w.x + v.y * w.y + ns.y * w.z,
-              u.z * w.x + v.z * w.y + ns.z * w.z);
-  }
-};
-__device__ __forceinline__ bool match(const Lobe &b, int t) { return (b.type & t) == b.type; }
 
 __device__ inline Sp fr_conductor(float cosThetaI, Sp etai, Sp etat, Sp k) {  // bsdf.cpp:124-147
   cosThetaI = clampf(cosThetaI, -1.0f, 1.0f);
@@ -217,6 +206,7 @@ __device__ inline float fr_dielectric(float cosThetaI, float etaI, float etaT) {
 }
 __device__ __forceinline__ Sp fresnel_eval(const Lobe &b, float cosI) {  // |cos| passed, bsdf.cpp:172-180
   if (b.fresnel == FR_CONDUCTOR) return fr_conductor(fabsf(cosI), sp(1.0f), b.etaT, b.k);
+  if (b.fresnel == FR_NOOP) return sp(1.0f);  // bsdf.h:96-99
   return sp(fr_dielectric(fabsf(cosI), b.etaA, b.etaB));
 }
 
@@ -261,9 +251,32 @@ __device__ __forceinline__ bool refract(V3 wi, V3 n, float eta, V3 *wt) {  // bs
   return true;
 }
 
+// ------------------------------------------------------- lobes
 __device__ inline Sp lobe_f(const Lobe &b, V3 wi, V3 wo) {
   if (b.kind == BX_LAMBERT) return b.R * kInvPi;  // lambert.cpp:5-8
-  if (b.kind == BX_MF_REFL) {                      // microfacet.cpp:55-66 (G(wi, wi) as in the reference)
+  if (b.kind == BX_OREN_NAYAR) {                   // orennayar.cpp:12-37
+    float sinThetaI = sqrtf(sin2t(wi)), sinThetaO = sqrtf(sin2t(wo));
+    float cosTerm = 0;
+    // 1e-4 is a double literal: the comparison is made in double
+    if ((double)sinThetaI > 1e-4 && (double)sinThetaO > 1e-4) {
+      // sampling.h:22-29 (the zero test there compares a function address,
+      // so the division is always taken)
+      float sinPhiI = clampf(wi.y / sinThetaI, -1.0f, 1.0f), cosPhiI = clampf(wi.x / sinThetaI, -1.0f, 1.0f);
+      float sinPhiO = clampf(wo.y / sinThetaO, -1.0f, 1.0f), cosPhiO = clampf(wo.x / sinThetaO, -1.0f, 1.0f);
+      float dCos = cosPhiI * cosPhiO + sinPhiI * sinPhiO;
+      cosTerm = mx(0.0f, dCos);
+    }
+    float sinAlpha, tanBeta;
+    if (fabsf(wi.z) > fabsf(wo.z)) {
+      sinAlpha = sinThetaO;
+      tanBeta = sinThetaI / fabsf(wi.z);
+    } else {
+      sinAlpha = sinThetaI;
+      tanBeta = sinThetaO / fabsf(wo.z);
+    }
+    return b.R * kInvPi * (b.A + b.B * cosTerm * sinAlpha * tanBeta);
+  }
+  if (b.kind == BX_MF_REFL) {  // microfacet.cpp:55-66 (G(wi, wi) as in the reference)
     float cosThetaO = fabsf(wo.z), cosThetaI = fabsf(wi.z);
     if (cosThetaI == 0 || cosThetaO == 0) return sp(0);
     V3 wh = add(wi, wo);
@@ -286,10 +299,11 @@ __device__ inline Sp lobe_f(const Lobe &b, V3 wi, V3 wo) {
            fabsf(ggx_D(b.alpha, wh) * ggx_G(b.alpha, wi, wo) * eta * eta * absdot(wi, wh) * absdot(wo, wh) *
                  factor * factor / (cosThetaI * cosThetaO * sqrtDenom * sqrtDenom));
   }
-  return sp(0);
+  return sp(0);  // specular lobes (specular.h:31-34, 50-53, 72-75)
 }
 __device__ inline float lobe_pdf(const Lobe &b, V3 wi, V3 wo) {
-  if (b.kind == BX_LAMBERT) return same_hemi(wi, wo) ? fabsf(wi.z) * kInvPi : 0;
+  // Lambert and Oren-Nayar use BxDF::pdf (bsdf.cpp:117-121)
+  if (b.kind == BX_LAMBERT || b.kind == BX_OREN_NAYAR) return same_hemi(wi, wo) ? fabsf(wi.z) * kInvPi : 0;
   if (b.kind == BX_MF_REFL) {
     if (!same_hemi(wi, wo)) return 0;
     V3 wh = normalize(add(wi, wo));
@@ -305,14 +319,17 @@ __device__ inline float lobe_pdf(const Lobe &b, V3 wi, V3 wo) {
   }
   return 0;
 }
+// bsdf.cpp:109-115 (cosine-weighted default), microfacet.cpp:42-53 / 76-85,
+// specular.cpp:5-54.  *pdf is left untouched where the reference does not
+// write it.
 __device__ inline Sp lobe_sample(const Lobe &b, V3 wo, float u0, float u1, V3 *wi, float *pdf, int *stype) {
-  if (b.kind == BX_LAMBERT) {  // bsdf.cpp:109-115
+  if (b.kind == BX_LAMBERT || b.kind == BX_OREN_NAYAR) {
     *wi = cosine_hemisphere(u0, u1);
     if (wo.z < 0) wi->z *= -1;
     *pdf = lobe_pdf(b, *wi, wo);
     return lobe_f(b, *wi, wo);
   }
-  if (b.kind == BX_MF_REFL) {  // microfacet.cpp:42-53
+  if (b.kind == BX_MF_REFL) {
     if (wo.z == 0) return sp(0);
     V3 wh = ggx_sample_wh(b.alpha, wo, u0, u1);
     *wi = reflect(wo, wh);
@@ -320,12 +337,29 @@ __device__ inline Sp lobe_sample(const Lobe &b, V3 wo, float u0, float u1, V3 *w
     *pdf = ggx_pdf(b.alpha, wh) / (4 * dot(wo, wh));
     return lobe_f(b, *wi, wo);
   }
-  if (b.kind == BX_MF_TRANS) {  // microfacet.cpp:76-85: *pdf is never written
+  if (b.kind == BX_MF_TRANS) {  // *pdf is never written
     if (wo.z == 0) return sp(0);
     V3 wh = ggx_sample_wh(b.alpha, wo, u0, u1);
     float eta = wo.z > 0 ? (b.etaA / b.etaB) : (b.etaB / b.etaA);
     if (!refract(wo, wh, eta, wi)) return sp(0);
     return lobe_f(b, *wi, wo);
+  }
+  if (b.kind == BX_SPEC_REFL) {  // specular.cpp:19-24
+    *wi = mk(-wo.x, -wo.y, wo.z);
+    *pdf = 1;
+    return fresnel_eval(b, wi->z) * b.R / fabsf(wi->z);
+  }
+  if (b.kind == BX_SPEC_TRANS) {  // specular.cpp:5-17
+    bool entering = wo.z > 0;
+    float etaI = entering ? b.etaA : b.etaB;
+    float etaT = entering ? b.etaB : b.etaA;
+    V3 n = mk(0, 0, 1);
+    if (dot(n, wo) < 0.f) n = neg(n);  // faceforward
+    if (!refract(wo, n, etaI / etaT, wi)) return sp(0);
+    *pdf = 1.0f;
+    Sp ft = b.T * (sp(1.0f) - sp(fr_dielectric(fabsf(wi->z), b.etaA, b.etaB)));
+    if (!b.photon) mulby(ft, sp((etaI * etaI) / (etaT * etaT)));
+    return ft / fabsf(wi->z);
   }
   // FresnelSpecular, specular.cpp:28-54
   float F = fr_dielectric(wo.z, b.etaA, b.etaB);
@@ -348,18 +382,166 @@ __device__ inline Sp lobe_sample(const Lobe &b, V3 wo, float u0, float u1, V3 *w
   return ft / fabsf(wi->z);
 }
 
-__device__ __forceinline__ int bsdf_num(const Bsdf &bs, int type) {
+// ------------------------------------------------------- material -> lobes
+// A BSDF keeps the material record's address and one 4-bit code per lobe (up
+// to five, Uber's count); a lobe's parameters are re-derived from the record
+// where they are used, so a shading lane holds one lobe in registers at a
+// time, whatever the material.
+enum {
+  LC_LAMBERT = 0,    // clamp(rec[1..3])                        matte, plastic Kd
+  LC_OREN_NAYAR,     // clamp(rec[1..3]), sigma rec[4]          matte, sigma != 0
+  LC_METAL,          // GGX, conductor eta rec[1..3] k rec[4..6], alpha rec[8]
+  LC_GLASS_SPEC,     // FresnelSpecular Kr rec[1..3] Kt rec[4..6], 1 | rec[9]
+  LC_GLASS_MF_R,     // GGX reflection Kr, alpha rec[8], dielectric 1 | rec[9]
+  LC_GLASS_MF_T,     // GGX transmission Kt, alpha rec[8], 1 | rec[9]
+  LC_MIRROR,         // SpecularReflection clamp(rec[1..3]), FresnelNoOp
+  LC_PLASTIC_MF,     // GGX clamp(rec[4..6]), alpha rec[8], dielectric 1.5 | 1
+  LC_UBER_OPEN,      // SpecularTransmission clamp(1 - op), 1 | 1
+  LC_UBER_DIFF,      // Lambert op * clamp(Kd)
+  LC_UBER_MF,        // GGX op * clamp(Ks), alpha rec[17], dielectric 1 | e
+  LC_UBER_SPEC_R,    // SpecularReflection op * clamp(Kr), dielectric 1 | e
+  LC_UBER_SPEC_T,    // SpecularTransmission op * clamp(Kt), 1 | e
+};
+__device__ __forceinline__ int lc_type(int code) {
+  switch (code) {
+    case LC_LAMBERT: case LC_OREN_NAYAR: case LC_UBER_DIFF: return REFL | DIFF;
+    case LC_METAL: case LC_GLASS_MF_R: case LC_PLASTIC_MF: case LC_UBER_MF: return REFL | GLOSSY;
+    case LC_GLASS_SPEC: return REFL | TRANS | SPEC;
+    case LC_GLASS_MF_T: return TRANS | GLOSSY;
+    case LC_MIRROR: case LC_UBER_SPEC_R: return REFL | SPEC;
+    default: return TRANS | SPEC;  // LC_UBER_OPEN, LC_UBER_SPEC_T
+  }
+}
+
+struct Bsdf {
+  V3 ng, ns, u, v;
+  const float *m;  // material record
+  uint32_t codes;  // lobe i: bits [4i, 4i + 4)
+  uint32_t types;  // lobe i: BxDF type flags at bits [5i, 5i + 5)
+  int n, photon;
+  __device__ __forceinline__ V3 to_local(V3 w) const { return mk(dot(w, u), dot(w, v), dot(w, ns)); }
+  __device__ __forceinline__ V3 to_world(V3 w) const {
+    return mk(u.x * w.x + v.x * w.y + ns.x * w.z, u.y * w.x + v.y * w.y + ns.y * w.z,
+              u.z * w.x + v.z * w.y + ns.z * w.z);
+  }
+  __device__ __forceinline__ int type_of(int i) const { return (int)(types >> (5 * i)) & 31; }
+  __device__ __forceinline__ bool matches(int i, int t) const { int ty = type_of(i); return (ty & t) == ty; }
+  __device__ __forceinline__ void push(int code) {
+    codes |= (uint32_t)code << (4 * n);
+    types |= (uint32_t)lc_type(code) << (5 * n);
+    ++n;
+  }
+};
+
+__device__ __forceinline__ Sp rec3(const float *m, int k) { return sp3(gld(m + k), gld(m + k + 1), gld(m + k + 2)); }
+// Spectrum::clamp() (spectrum.h:89-93): [0, max float] whatever bounds are passed
+__device__ __forceinline__ Sp clamp0(Sp s) {
+  return sp3(clampf(s.r, 0, kFltMax), clampf(s.g, 0, kFltMax), clampf(s.b, 0, kFltMax));
+}
+__device__ __forceinline__ Sp uber_op(const float *m) { return rec3(m, 13); }
+
+__device__ inline Lobe lobe_at(const Bsdf &bs, int i) {
+  const float *m = bs.m;
+  int code = (int)(bs.codes >> (4 * i)) & 15;
+  Lobe b;
+  b.kind = BX_LAMBERT; b.type = lc_type(code); b.fresnel = FR_DIELECTRIC; b.photon = bs.photon;
+  b.R = sp(0); b.T = sp(0); b.etaT = sp(0); b.k = sp(0);
+  b.alpha = 0; b.etaA = 1.0f; b.etaB = 1.0f; b.A = 0; b.B = 0;
+  switch (code) {
+    case LC_LAMBERT: b.R = clamp0(rec3(m, 1)); break;
+    case LC_OREN_NAYAR: {  // orennayar.cpp:4-10
+      b.kind = BX_OREN_NAYAR;
+      b.R = clamp0(rec3(m, 1));
+      float sigma = clampf(gld(m + 4), 0, kFltMax) * 0.01745329251994329576923690768489f;
+      float sigma2 = sigma * sigma;
+      b.A = 1.f - (sigma2 / (2.f * (sigma2 + 0.33f)));
+      b.B = 0.45f * sigma2 / (sigma2 + 0.09f);
+      break;
+    }
+    case LC_METAL:
+      b.kind = BX_MF_REFL; b.R = sp(1.0f); b.alpha = gld(m + 8); b.fresnel = FR_CONDUCTOR;
+      b.etaT = rec3(m, 1); b.k = rec3(m, 4);
+      break;
+    case LC_GLASS_SPEC:
+      b.kind = BX_FRESNEL_SPEC; b.R = rec3(m, 1); b.T = rec3(m, 4); b.etaB = gld(m + 9);
+      break;
+    case LC_GLASS_MF_R:
+      b.kind = BX_MF_REFL; b.R = rec3(m, 1); b.alpha = gld(m + 8); b.etaB = gld(m + 9);
+      break;
+    case LC_GLASS_MF_T:
+      b.kind = BX_MF_TRANS; b.T = rec3(m, 4); b.alpha = gld(m + 8); b.etaB = gld(m + 9);
+      break;
+    case LC_MIRROR:
+      b.kind = BX_SPEC_REFL; b.R = clamp0(rec3(m, 1)); b.fresnel = FR_NOOP;
+      break;
+    case LC_PLASTIC_MF:
+      b.kind = BX_MF_REFL; b.R = clamp0(rec3(m, 4)); b.alpha = gld(m + 8); b.etaA = 1.5f; b.etaB = 1.0f;
+      break;
+    case LC_UBER_OPEN:
+      b.kind = BX_SPEC_TRANS; b.T = clamp0(sp(1.0f) - uber_op(m));
+      break;
+    case LC_UBER_DIFF: b.R = uber_op(m) * clamp0(rec3(m, 1)); break;
+    case LC_UBER_MF:
+      b.kind = BX_MF_REFL; b.R = uber_op(m) * clamp0(rec3(m, 4)); b.alpha = gld(m + 17); b.etaB = gld(m + 18);
+      break;
+    case LC_UBER_SPEC_R:
+      b.kind = BX_SPEC_REFL; b.R = uber_op(m) * clamp0(rec3(m, 7)); b.etaB = gld(m + 18);
+      break;
+    default:  // LC_UBER_SPEC_T
+      b.kind = BX_SPEC_TRANS; b.T = uber_op(m) * clamp0(rec3(m, 10)); b.etaB = gld(m + 18);
+      break;
+  }
+  return b;
+}
+
+// Material record -> lobe list, in the reference's addBxDF order
+// (matte.cpp:9-21, metal.cpp:8-17, glass.cpp:9-37, mirror.cpp:7-14,
+// plastic.cpp:8-24, uber.cpp:9-42); alpha comes precomputed in the record.
+__device__ inline void make_bsdf(const float *m, V3 ng, V3 ns, V3 u, V3 v, bool photon, Bsdf *bs) {
+  bs->ng = ng; bs->ns = ns; bs->u = u; bs->v = v;
+  bs->m = m; bs->codes = 0; bs->types = 0; bs->n = 0; bs->photon = photon;
+  int type = (int)gld(m + 0);
+  if (type == 0) {
+    if (!black(clamp0(rec3(m, 1)))) bs->push(clampf(gld(m + 4), 0, kFltMax) == 0 ? LC_LAMBERT : LC_OREN_NAYAR);
+  } else if (type == 1) {
+    bs->push(LC_METAL);
+  } else if (type == 2) {
+    Sp R = rec3(m, 1), T = rec3(m, 4);
+    if (black(R) && black(T)) return;
+    if (gld(m + 7) == 0) {
+      bs->push(LC_GLASS_SPEC);
+    } else {
+      if (!black(R)) bs->push(LC_GLASS_MF_R);
+      if (!black(T)) bs->push(LC_GLASS_MF_T);
+    }
+  } else if (type == 3) {
+    if (!black(clamp0(rec3(m, 1)))) bs->push(LC_MIRROR);
+  } else if (type == 4) {
+    if (!black(clamp0(rec3(m, 1)))) bs->push(LC_LAMBERT);
+    if (!black(clamp0(rec3(m, 4)))) bs->push(LC_PLASTIC_MF);
+  } else {
+    Sp op = uber_op(m);
+    if (!black(clamp0(sp(1.0f) - op))) bs->push(LC_UBER_OPEN);
+    if (!black(op * clamp0(rec3(m, 1)))) bs->push(LC_UBER_DIFF);
+    if (!black(op * clamp0(rec3(m, 4)))) bs->push(LC_UBER_MF);
+    if (!black(op * clamp0(rec3(m, 7)))) bs->push(LC_UBER_SPEC_R);
+    if (!black(op * clamp0(rec3(m, 10)))) bs->push(LC_UBER_SPEC_T);
+  }
+}
+
+// ------------------------------------------------------- BSDF (bsdf.cpp)
+__device__ __forceinline__ int bsdf_num(const Bsdf &bs, int type) {  // bsdf.cpp:4-12
   int n = 0;
 #pragma unroll
-  for (int i = 0; i < 2; ++i) n += (i < bs.n && match(bs.b[i], type)) ? 1 : 0;
+  for (int i = 0; i < 5; ++i) n += (i < bs.n && bs.matches(i, type)) ? 1 : 0;
   return n;
 }
 __device__ inline Sp bsdf_f(const Bsdf &bs, V3 wi, V3 wo, int type) {  // bsdf.cpp:17-29
   Sp r = sp(0);
   V3 swi = bs.to_local(wi), swo = bs.to_local(wo);
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-    if (i < bs.n && match(bs.b[i], type)) acc(r, lobe_f(bs.b[i], swi, swo));
+#pragma unroll 1
+  for (int i = 0; i < bs.n; ++i)
+    if (bs.matches(i, type)) acc(r, lobe_f(lobe_at(bs, i), swi, swo));
   return r;
 }
 __device__ inline float bsdf_pdf(const Bsdf &bs, V3 wi, V3 wo, int type) {  // bsdf.cpp:85-103
@@ -368,9 +550,9 @@ __device__ inline float bsdf_pdf(const Bsdf &bs, V3 wi, V3 wo, int type) {  // b
   if (swo.z == 0) return 0;
   float pdf = 0;
   int m = 0;
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-    if (i < bs.n && match(bs.b[i], type)) { ++m; pdf += lobe_pdf(bs.b[i], swi, swo); }
+#pragma unroll 1
+  for (int i = 0; i < bs.n; ++i)
+    if (bs.matches(i, type)) { ++m; pdf += lobe_pdf(lobe_at(bs, i), swi, swo); }
   return m > 0 ? pdf / m : 0.f;
 }
 // bsdf.cpp:31-83; *pdf and *stype stay untouched on the swo.z == 0 exit
@@ -378,89 +560,37 @@ __device__ inline Sp bsdf_sample(const Bsdf &bs, V3 *wi, V3 wo, float u0, float 
   int matching = bsdf_num(bs, type);
   if (matching == 0) { *pdf = 0; *stype = 0; return sp(0); }
   int pos = min((int)floorf(u0 * matching), matching - 1);
-  // the pos-th matching lobe (at most two lobes: constant indices only, so
-  // the lobes stay in registers)
-  const bool m0 = bs.n > 0 && match(bs.b[0], type);
-  const int sel = (m0 && pos == 0) ? 0 : 1;
-  const Lobe bx = sel == 0 ? bs.b[0] : bs.b[1];
+  int count = pos, sel = 0;
+#pragma unroll 1
+  for (int i = 0; i < bs.n; ++i)
+    if (bs.matches(i, type) && count-- == 0) { sel = i; break; }
   float nu0 = mn(u0 * matching - pos, kOneMinusEps);
   V3 swi = mk(0, 0, 0), swo = bs.to_local(wo);
   if (swo.z == 0) return sp(0);
+  const int sel_type = bs.type_of(sel);
   *pdf = 0;
-  *stype = bx.type;
-  Sp f = lobe_sample(bx, swo, nu0, u1, &swi, pdf, stype);
+  *stype = sel_type;
+  Sp f = lobe_sample(lobe_at(bs, sel), swo, nu0, u1, &swi, pdf, stype);
   if (*pdf == 0) { *stype = 0; return sp(0); }
   *wi = bs.to_world(swi);
-  bool hasSpecular = (bx.type & SPEC) != 0;
+  bool hasSpecular = (sel_type & SPEC) != 0;
   if (!hasSpecular && matching > 1) {
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-      if (i < bs.n && i != sel && match(bs.b[i], type)) *pdf += lobe_pdf(bs.b[i], swi, swo);
+#pragma unroll 1
+    for (int i = 0; i < bs.n; ++i)
+      if (i != sel && bs.matches(i, type)) *pdf += lobe_pdf(lobe_at(bs, i), swi, swo);
   }
   if (matching > 1) *pdf /= matching;
   if (!hasSpecular && matching > 1) {
     bool refl = dot(*wi, bs.ng) * dot(wo, bs.ng) > 0;
     f = sp(0);
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      bool hr = (bs.b[i].type & REFL) != 0, ht = (bs.b[i].type & TRANS) != 0;
-      if (i < bs.n && match(bs.b[i], type) && ((refl && hr) || (!refl && ht))) acc(f, lobe_f(bs.b[i], swi, swo));
+#pragma unroll 1
+    for (int i = 0; i < bs.n; ++i) {
+      int ty = bs.type_of(i);
+      bool hr = (ty & REFL) != 0, ht = (ty & TRANS) != 0;
+      if (bs.matches(i, type) && ((refl && hr) || (!refl && ht))) acc(f, lobe_f(lobe_at(bs, i), swi, swo));
     }
   }
   return f;
-}
-
-// Material record -> lobes (matte.cpp:9-21, metal.cpp:8-17, glass.cpp:9-37);
-// alpha comes precomputed in the record.
-__device__ inline void make_bsdf(const float *m, V3 ng, V3 ns, V3 u, V3 v, bool photon, Bsdf *bs) {
-  bs->ng = ng; bs->ns = ns; bs->u = u; bs->v = v; bs->n = 0;
-  // lobes are appended through constant indices so they stay in registers
-  auto add = [&](const Lobe &l) {
-    if (bs->n == 0) bs->b[0] = l;
-    else bs->b[1] = l;
-    bs->n++;
-  };
-  Lobe b{};
-  b.photon = photon;
-  int type = (int)gld(m + 0);
-  if (type == 0) {
-    Sp r = sp3(clampf(gld(m + 1), 0, kFltMax), clampf(gld(m + 2), 0, kFltMax), clampf(gld(m + 3), 0, kFltMax));
-    if (!black(r)) {
-      b.kind = BX_LAMBERT; b.type = REFL | DIFF; b.R = r;
-      b.fresnel = 0; b.alpha = 0; b.etaA = b.etaB = 0;
-      add(b);
-    }
-  } else if (type == 1) {
-    b.kind = BX_MF_REFL; b.type = REFL | GLOSSY; b.R = sp(1.0f);
-    b.alpha = gld(m + 8);
-    b.fresnel = FR_CONDUCTOR;
-    b.etaT = sp3(gld(m + 1), gld(m + 2), gld(m + 3));
-    b.k = sp3(gld(m + 4), gld(m + 5), gld(m + 6));
-    b.etaA = b.etaB = 0;
-    add(b);
-  } else {
-    Sp R = sp3(gld(m + 1), gld(m + 2), gld(m + 3)), T = sp3(gld(m + 4), gld(m + 5), gld(m + 6));
-    float rough = gld(m + 7), alpha = gld(m + 8), e = gld(m + 9);
-    if (black(R) && black(T)) return;
-    if (rough == 0) {
-      b.kind = BX_FRESNEL_SPEC; b.type = REFL | TRANS | SPEC;
-      b.R = R; b.T = T; b.etaA = 1.0f; b.etaB = e; b.fresnel = FR_DIELECTRIC; b.alpha = 0;
-      add(b);
-    } else {
-      if (!black(R)) {
-        Lobe br = b;
-        br.kind = BX_MF_REFL; br.type = REFL | GLOSSY; br.R = R; br.alpha = alpha;
-        br.fresnel = FR_DIELECTRIC; br.etaA = 1.0f; br.etaB = e;
-        add(br);
-      }
-      if (!black(T)) {
-        Lobe bt = b;
-        bt.kind = BX_MF_TRANS; bt.type = TRANS | GLOSSY; bt.T = T; bt.alpha = alpha;
-        bt.etaA = 1.0f; bt.etaB = e; bt.fresnel = FR_DIELECTRIC;
-        add(bt);
-      }
-    }
-  }
 }
 
 }  // namespace dev

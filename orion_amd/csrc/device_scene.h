@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/orion_amd.h"
 #include "bvh.h"
 #include "device_math.h"
 
@@ -20,7 +21,7 @@ struct DScene {
   const float4 *nodes;   // 4 x float4 per node (bvh.h)
   const float4 *tris;    // 3 x float4 per leaf slot
   const float4 *shade;   // 5 x float4 per primitive: p0 p1 p2 n0 n1 n2 (mat, light bits)
-  const float *mats;     // 16 floats per material
+  const float *mats;     // ORION_MAT_FLOATS floats per material
   const float *lights;   // 32 floats per light
   const float *light_cdf;   // n_lights + 1
   const float *light_func;  // n_lights

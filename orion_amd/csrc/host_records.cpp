@@ -47,14 +47,15 @@ float roughness_to_alpha(float roughness) {
          0.000640711f * x * x * x * x;
 }
 
-void matte_record(float r, float g, float b, float rec[16]) {
-  std::memset(rec, 0, 16 * sizeof(float));
+void matte_record(float r, float g, float b, float rec[kMatFloats], float sigma) {
+  std::memset(rec, 0, kMatFloats * sizeof(float));
   rec[0] = MAT_MATTE;
   rec[1] = r; rec[2] = g; rec[3] = b;
+  rec[4] = sigma;
 }
 
-void metal_record(const float eta[3], const float k[3], float roughness, float rec[16]) {
-  std::memset(rec, 0, 16 * sizeof(float));
+void metal_record(const float eta[3], const float k[3], float roughness, float rec[kMatFloats]) {
+  std::memset(rec, 0, kMatFloats * sizeof(float));
   rec[0] = MAT_METAL;
   for (int i = 0; i < 3; ++i) { rec[1 + i] = eta[i]; rec[4 + i] = k[i]; }
   rec[7] = roughness;
@@ -72,13 +73,39 @@ bool metal_preset(const std::string &name, float eta[3], float k[3]) {
   return false;
 }
 
-void glass_record(const float Kr[3], const float Kt[3], float roughness, float eta, float rec[16]) {
-  std::memset(rec, 0, 16 * sizeof(float));
+void glass_record(const float Kr[3], const float Kt[3], float roughness, float eta, float rec[kMatFloats]) {
+  std::memset(rec, 0, kMatFloats * sizeof(float));
   rec[0] = MAT_GLASS;
   for (int i = 0; i < 3; ++i) { rec[1 + i] = Kr[i]; rec[4 + i] = Kt[i]; }
   rec[7] = roughness;
   rec[8] = roughness_to_alpha(roughness);
   rec[9] = eta;
+}
+
+void mirror_record(const float Kr[3], float rec[kMatFloats]) {  // material/mirror.cpp:7-14
+  std::memset(rec, 0, kMatFloats * sizeof(float));
+  rec[0] = MAT_MIRROR;
+  for (int i = 0; i < 3; ++i) rec[1 + i] = Kr[i];
+}
+
+void plastic_record(const float Kd[3], const float Ks[3], float roughness, float rec[kMatFloats]) {
+  std::memset(rec, 0, kMatFloats * sizeof(float));  // material/plastic.cpp:8-24
+  rec[0] = MAT_PLASTIC;
+  for (int i = 0; i < 3; ++i) { rec[1 + i] = Kd[i]; rec[4 + i] = Ks[i]; }
+  rec[7] = roughness;
+  rec[8] = roughness_to_alpha(roughness);
+}
+
+void uber_record(const float Kd[3], const float Ks[3], const float Kr[3], const float Kt[3],
+                 const float opacity[3], float roughness, float eta, float rec[kMatFloats]) {
+  std::memset(rec, 0, kMatFloats * sizeof(float));  // material/uber.cpp:9-42
+  rec[0] = MAT_UBER;
+  for (int i = 0; i < 3; ++i) {
+    rec[1 + i] = Kd[i]; rec[4 + i] = Ks[i]; rec[7 + i] = Kr[i]; rec[10 + i] = Kt[i]; rec[13 + i] = opacity[i];
+  }
+  rec[16] = roughness;
+  rec[17] = roughness_to_alpha(roughness);
+  rec[18] = eta;
 }
 
 // point.cpp:34-40: light-to-world = translate(from); p = that applied to 0.

@@ -9,15 +9,15 @@
 
 namespace orion_amd {
 
-enum { MAT_MATTE = 0, MAT_METAL = 1, MAT_GLASS = 2 };
+enum { MAT_MATTE = 0, MAT_METAL = 1, MAT_GLASS = 2, MAT_MIRROR = 3, MAT_PLASTIC = 4, MAT_UBER = 5 };
 enum { LIGHT_AREA = 0, LIGHT_POINT = 1, LIGHT_SPOT = 2 };
-constexpr int kTriFloats = 18, kMatFloats = 16, kLightFloats = 32, kCamFloats = 16;
+constexpr int kTriFloats = 18, kMatFloats = ORION_MAT_FLOATS, kLightFloats = 32, kCamFloats = 16;
 
 struct HostScene {
   std::vector<float> tri_pn;    // n x 18
   std::vector<int> tri_mat;     // n
   std::vector<int> tri_light;   // n (area light index or -1)
-  std::vector<float> mats;      // m x 16
+  std::vector<float> mats;      // m x kMatFloats
   std::vector<float> lights;    // l x 32
   float camera[kCamFloats] = {0};
   orion_scene_info info{};
@@ -34,10 +34,14 @@ struct HostScene {
 // Record builders shared by the loader and the generators.
 void camera_record(const float lookat[9], float fov_deg, int W, int H, float rec[16]);
 float roughness_to_alpha(float roughness);
-void matte_record(float r, float g, float b, float rec[16]);
-void metal_record(const float eta[3], const float k[3], float roughness, float rec[16]);
+void matte_record(float r, float g, float b, float rec[kMatFloats], float sigma = 0.f);
+void metal_record(const float eta[3], const float k[3], float roughness, float rec[kMatFloats]);
+void mirror_record(const float Kr[3], float rec[kMatFloats]);
+void plastic_record(const float Kd[3], const float Ks[3], float roughness, float rec[kMatFloats]);
+void uber_record(const float Kd[3], const float Ks[3], const float Kr[3], const float Kt[3],
+                 const float opacity[3], float roughness, float eta, float rec[kMatFloats]);
 bool metal_preset(const std::string &name, float eta[3], float k[3]);
-void glass_record(const float Kr[3], const float Kt[3], float roughness, float eta, float rec[16]);
+void glass_record(const float Kr[3], const float Kt[3], float roughness, float eta, float rec[kMatFloats]);
 void point_light_record(const float I[3], const float from[3], float rec[32]);
 void spot_light_record(const float I[3], const float from[3], const float to[3], float angle,
                        float delta, float rec[32]);

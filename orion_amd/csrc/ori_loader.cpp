@@ -350,12 +350,28 @@ bool texture_color(const XNode *mat, const char *name, bool *present, float o[3]
   return true;
 }
 
-// Parser::_makeMaterial (parser.cpp:262-431) for matte / metal / glass
+// A FloatTexture parameter (CREATE_FLOAT_TEXTURE, parser.cpp:37-43): only
+// type "float" survives the dynamic_pointer_cast; anything else is absent.
+bool float_texture(const XNode *mat, const char *name, bool *present, float *v) {
+  const XNode *n = mat->first(name);
+  *present = false;
+  if (!n) return true;
+  ParamSet ps = params_of(n);
+  std::string type = ps.get("type");
+  trim(type);
+  if (type == "float" && !ps.get("value").empty()) {
+    *v = pf(ps.get("value"));
+    *present = true;
+  }
+  return true;
+}
+
+// Parser::_makeMaterial (parser.cpp:262-431)
 bool make_material(const XNode *mat, std::map<std::string, std::vector<float>> &named,
                    std::vector<float> *rec, std::string *err) {
   if (!mat) { *err = "Model without Material"; return false; }
   const char *nm = mat->attr("named");
-  rec->assign(16, 0.f);
+  rec->assign(kMatFloats, 0.f);
   if (nm && *nm) {
     auto it = named.find(nm);
     if (it == named.end()) { *err = std::string("no named material ") + nm; return false; }
@@ -367,9 +383,9 @@ bool make_material(const XNode *mat, std::map<std::string, std::vector<float>> &
       float kd[3] = {0, 0, 0}, sg[3] = {0, 0, 0};
       texture_color(mat, "Kd", &have, kd);
       if (!have) { *err = "matte without Kd"; return false; }
-      texture_color(mat, "sigma", &have, sg);
-      if (have && sg[0] != 0) { *err = "OrenNayar (sigma != 0) is outside this build's scope"; return false; }
-      matte_record(kd[0], kd[1], kd[2], rec->data());
+      // matte.cpp:23-29: a missing (or non-float) sigma is FloatTexture(0)
+      float_texture(mat, "sigma", &have, sg);
+      matte_record(kd[0], kd[1], kd[2], rec->data(), have ? sg[0] : 0.f);
     } else if (type == "metal") {
       float eta[3] = {0, 0, 0}, k[3] = {0, 0, 0}, r[3] = {0.1f, 0.1f, 0.1f};
       bool he, hk, hr;
@@ -395,6 +411,34 @@ bool make_material(const XNode *mat, std::map<std::string, std::vector<float>> &
       texture_color(mat, "roughness", &h, r);
       if (!h) { *err = "glass without roughness"; return false; }
       glass_record(kr, kt, r[0], eta[0], rec->data());
+    } else if (type == "mirror") {  // parser.cpp:297-307
+      float kr[3] = {0, 0, 0};
+      texture_color(mat, "Kr", &have, kr);
+      if (!have) { *err = "mirror without Kr"; return false; }
+      mirror_record(kr, rec->data());
+    } else if (type == "plastic") {  // parser.cpp:339-365
+      float kd[3] = {0, 0, 0}, ks[3] = {0, 0, 0}, r = 0;
+      texture_color(mat, "Kd", &have, kd);
+      if (!have) { *err = "plastic without Kd"; return false; }
+      texture_color(mat, "Ks", &have, ks);
+      if (!have) { *err = "plastic without Ks"; return false; }
+      float_texture(mat, "roughness", &have, &r);
+      if (!have) { *err = "plastic without a float roughness"; return false; }
+      plastic_record(kd, ks, r, rec->data());
+    } else if (type == "uber") {  // parser.cpp:412-421: every texture is evaluated
+      float kd[3], ks[3], kr[3], kt[3], op[3], r = 0, e = 0;
+      const char *names[5] = {"Kd", "Ks", "Kr", "Kt", "opacity"};
+      float *dst[5] = {kd, ks, kr, kt, op};
+      for (int i = 0; i < 5; ++i) {
+        dst[i][0] = dst[i][1] = dst[i][2] = 0;
+        texture_color(mat, names[i], &have, dst[i]);
+        if (!have) { *err = std::string("uber without ") + names[i]; return false; }
+      }
+      float_texture(mat, "roughness", &have, &r);
+      if (!have) { *err = "uber without a float roughness"; return false; }
+      float_texture(mat, "eta", &have, &e);
+      if (!have) { *err = "uber without a float eta"; return false; }
+      uber_record(kd, ks, kr, kt, op, r, e, rec->data());
     } else {
       *err = "material type '" + type + "' is outside this build's scope";
       return false;
@@ -595,6 +639,24 @@ void write_material(std::ostream &o, const float *m) {
   if (type == MAT_MATTE) {
     o << "    <Material type=\"matte\">\n";
     cst("Kd", m + 1);
+    if (m[4] != 0) flt("sigma", m[4]);
+  } else if (type == MAT_MIRROR) {
+    o << "    <Material type=\"mirror\">\n";
+    cst("Kr", m + 1);
+  } else if (type == MAT_PLASTIC) {
+    o << "    <Material type=\"plastic\">\n";
+    cst("Kd", m + 1);
+    cst("Ks", m + 4);
+    flt("roughness", m[7]);
+  } else if (type == MAT_UBER) {
+    o << "    <Material type=\"uber\">\n";
+    cst("Kd", m + 1);
+    cst("Ks", m + 4);
+    cst("Kr", m + 7);
+    cst("Kt", m + 10);
+    cst("opacity", m + 13);
+    flt("roughness", m[16]);
+    flt("eta", m[18]);
   } else if (type == MAT_METAL) {
     o << "    <Material type=\"metal\">\n";
     cst("eta", m + 1);
@@ -667,7 +729,7 @@ int write_ori(const HostScene &s, const std::string &path, int inline_limit, std
         for (int c = 0; c < 9; ++c) o << ((t == a && c == 0) ? "" : ",") << fstr(s.tri_pn[18 * (size_t)t + 9 + c]);
       o << "\"/>\n";
     }
-    write_material(o, &s.mats[16 * (size_t)m]);
+    write_material(o, &s.mats[kMatFloats * (size_t)m]);
     if (lit) o << "    <AreaLight><Param Le=\"" << f3str(Le) << "\"/><Param nLightSamples=\"1\"/></AreaLight>\n";
     o << "  </Model>\n";
     a = b;

@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
           fl &= ~F_ALIVE;
         } else {
           Bsdf bs;
-          make_bsdf(sc.mats + 16 * (size_t)is.mat, is.ng, is.ns, is.u, is.v, false, &bs);
+          make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)is.mat, is.ng, is.ns, is.u, is.v, false, &bs);
           int q = p - J.path0;
           int s = q / J.npix;
           uint32_t index = (J.k0 + (uint32_t)s) * (uint32_t)J.npix + (uint32_t)(q - s * J.npix);

@@ -191,6 +191,13 @@ int orion_scene_upload(orion_ctx c, orion_hscene hs, orion_scene *out) {
     if (s.tri_mat[i] < 0 || s.tri_mat[i] >= s.n_mats()) { set_error("triangle material out of range"); return ORION_ERR_ARG; }
     if (s.tri_light[i] >= s.n_lights()) { set_error("triangle light out of range"); return ORION_ERR_ARG; }
   }
+  for (int m = 0; m < s.n_mats(); ++m) {
+    float t = s.mats[(size_t)kMatFloats * m];
+    if (!(t == MAT_MATTE || t == MAT_METAL || t == MAT_GLASS || t == MAT_MIRROR || t == MAT_PLASTIC || t == MAT_UBER)) {
+      set_error("material " + std::to_string(m) + " has an unknown type");
+      return ORION_ERR_ARG;
+    }
+  }
   HIP_TRY(hipSetDevice(c->device));
   auto sc = new orion_scene_s();
   sc->ctx = c;

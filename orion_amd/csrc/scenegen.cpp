@@ -10,6 +10,10 @@
 //              spot light; SPPM 512^2, 64 x 2^20 photons, r0 0.01, depth 10
 //   batch_job  one job of a batched stream: displaced icosphere of a requested
 //              size with a material from {Lambert, gold GGX, dielectric}
+//   materials  the cornell box with every material kind: Oren-Nayar walls,
+//              plastic and mirror boxes, an uber sphere (all five lobes), a
+//              rough glass sphere and a gold sphere; ceiling light + point
+//              light; 128^2, 16 spp, depth 6
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -75,7 +79,7 @@ struct Builder {
   }
   int material(const float *rec) {
     int m = s->n_mats();
-    s->mats.insert(s->mats.end(), rec, rec + 16);
+    s->mats.insert(s->mats.end(), rec, rec + kMatFloats);
     return m;
   }
   void box(V center, V half, double angle_deg, int mat) {
@@ -110,7 +114,7 @@ void set_camera(HostScene *s, const float la[9], float fov, int W, int H) {
 // ---------------------------------------------------------------- cornell
 void gen_cornell(HostScene *s, int W, int H) {
   Builder b{s};
-  float grey[16], red[16], green[16], black[16];
+  float grey[kMatFloats], red[kMatFloats], green[kMatFloats], black[kMatFloats];
   matte_record(0.75f, 0.75f, 0.75f, grey);
   matte_record(0.63f, 0.06f, 0.05f, red);
   matte_record(0.14f, 0.45f, 0.09f, green);
@@ -132,6 +136,62 @@ void gen_cornell(HostScene *s, int W, int H) {
   s->info.integrator = 0;
   s->info.spp = 16;
   s->info.max_depth = 5;
+}
+
+// ---------------------------------------------------------------- materials
+void add_sphere(Builder &b, int L, V center, double radius, double amp, uint64_t noise_seed,
+                const std::vector<int> &region_mat);
+
+void gen_materials(HostScene *s, int W, int H) {
+  Builder b{s};
+  float rec[kMatFloats];
+  matte_record(0.75f, 0.75f, 0.75f, rec, 20.f);  // Oren-Nayar
+  int mwall = b.material(rec);
+  matte_record(0.63f, 0.06f, 0.05f, rec, 35.f);
+  int mred = b.material(rec);
+  matte_record(0.14f, 0.45f, 0.09f, rec);         // Lambert
+  int mgreen = b.material(rec);
+  matte_record(0.f, 0.f, 0.f, rec);
+  int mblack = b.material(rec);
+  const float pkd[3] = {0.25f, 0.3f, 0.65f}, pks[3] = {0.4f, 0.4f, 0.4f};
+  plastic_record(pkd, pks, 0.12f, rec);
+  int mplastic = b.material(rec);
+  const float mkr[3] = {0.9f, 0.85f, 0.8f};
+  mirror_record(mkr, rec);
+  int mmirror = b.material(rec);
+  const float ukd[3] = {0.5f, -0.1f, 0.3f}, uks[3] = {0.2f, 0.2f, 0.2f}, ukr[3] = {0.25f, 0.25f, 0.25f},
+              ukt[3] = {0.4f, 0.5f, 0.6f}, uop[3] = {0.6f, 0.75f, 1.2f};
+  uber_record(ukd, uks, ukr, ukt, uop, 0.2f, 1.45f, rec);
+  int muber = b.material(rec);
+  const float gkr[3] = {0.95f, 0.95f, 0.95f}, gkt[3] = {0.9f, 0.9f, 0.9f};
+  glass_record(gkr, gkt, 0.18f, 1.5f, rec);
+  int mglass = b.material(rec);
+  float eta[3], k[3];
+  metal_preset("gold", eta, k);
+  metal_record(eta, k, 0.25f, rec);
+  int mgold = b.material(rec);
+  const float Le[3] = {15.f, 15.f, 15.f};
+  b.quad({0.375, 0.999, 0.375}, {0.625, 0.999, 0.375}, {0.625, 0.999, 0.625}, {0.375, 0.999, 0.625},
+         {0, -1, 0}, mblack, Le);
+  b.quad({0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1}, {0, 1, 0}, mwall, nullptr);     // floor
+  b.quad({0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}, {0, -1, 0}, mwall, nullptr);    // ceiling
+  b.quad({0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, mwall, nullptr);     // back
+  b.quad({0, 0, 0}, {0, 1, 0}, {0, 1, 1}, {0, 0, 1}, {1, 0, 0}, mred, nullptr);      // left
+  b.quad({1, 0, 0}, {1, 1, 0}, {1, 1, 1}, {1, 0, 1}, {-1, 0, 0}, mgreen, nullptr);   // right
+  b.box({0.30, 0.12, 0.70}, {0.12, 0.12, 0.12}, -17.0, mplastic);                     // short box
+  b.box({0.70, 0.28, 0.30}, {0.12, 0.28, 0.12}, 20.0, mmirror);                       // tall box
+  add_sphere(b, 3, {0.30, 0.38, 0.68}, 0.13, 0.0, 0, std::vector<int>(1, muber));
+  add_sphere(b, 3, {0.72, 0.12, 0.72}, 0.12, 0.0, 0, std::vector<int>(1, mglass));
+  add_sphere(b, 2, {0.18, 0.10, 0.25}, 0.10, 0.0, 0, std::vector<int>(1, mgold));
+  const float I[3] = {0.6f, 0.6f, 0.5f}, from[3] = {0.5f, 0.8f, 0.9f};
+  float lrec[32];
+  point_light_record(I, from, lrec);
+  s->lights.insert(s->lights.end(), lrec, lrec + 32);
+  const float la[9] = {0.5f, 0.5f, 2.4f, 0.5f, 0.5f, 0.0f, 0.f, 1.f, 0.f};
+  set_camera(s, la, 40.f, W, H);
+  s->info.integrator = 0;
+  s->info.spp = 16;
+  s->info.max_depth = 6;
 }
 
 // ---------------------------------------------------------------- value noise
@@ -249,11 +309,11 @@ void gen_icosphere(HostScene *s, uint64_t seed, int L, int W, int H) {
   metal_preset("gold", eta, k);
   std::vector<int> region(20);
   for (int r = 0; r < 20; ++r) {
-    float rec[16];
+    float rec[kMatFloats];
     metal_record(eta, k, (float)(0.05 + 0.45 * rng.uniform()), rec);
     region[r] = b.material(rec);
   }
-  float ground[16], black[16];
+  float ground[kMatFloats], black[kMatFloats];
   matte_record(0.5f, 0.5f, 0.5f, ground);
   matte_record(0.f, 0.f, 0.f, black);
   int mground = b.material(ground), mblack = b.material(black);
@@ -274,7 +334,7 @@ void gen_icosphere(HostScene *s, uint64_t seed, int L, int W, int H) {
 
 void gen_caustics(HostScene *s, int L, int W, int H) {
   Builder b{s};
-  float glass[16], plane[16];
+  float glass[kMatFloats], plane[kMatFloats];
   const float one[3] = {1.f, 1.f, 1.f};
   glass_record(one, one, 0.f, 1.5f, glass);
   matte_record(0.5f, 0.5f, 0.5f, plane);
@@ -302,7 +362,7 @@ void gen_batch_job(HostScene *s, uint64_t seed, double tris, int W, int H, int m
   int L = 1;
   while (20.0 * std::pow(4.0, L + 1) <= tris && L < 9) ++L;
   if (mix == 3) mix = (int)(rng.uniform() * 3) % 3;
-  float rec[16];
+  float rec[kMatFloats];
   if (mix == 0) {
     matte_record((float)(0.3 + 0.6 * rng.uniform()), (float)(0.3 + 0.6 * rng.uniform()),
                  (float)(0.3 + 0.6 * rng.uniform()), rec);
@@ -315,7 +375,7 @@ void gen_batch_job(HostScene *s, uint64_t seed, double tris, int W, int H, int m
     glass_record(one, one, 0.f, 1.5f, rec);
   }
   int mobj = b.material(rec);
-  float ground[16], black[16];
+  float ground[kMatFloats], black[kMatFloats];
   matte_record(0.5f, 0.5f, 0.5f, ground);
   matte_record(0.f, 0.f, 0.f, black);
   int mground = b.material(ground), mblack = b.material(black);
@@ -350,6 +410,8 @@ int generate_scene(const std::string &name, uint64_t seed, const double *params,
     int L = (int)P(0, 6);
     if (L < 0 || L > 9) { *err = "caustics level must be in [0, 9]"; return ORION_ERR_ARG; }
     gen_caustics(&s, L, (int)P(1, 512), (int)P(2, 512));
+  } else if (name == "materials") {
+    gen_materials(&s, (int)P(0, 128), (int)P(1, 128));
   } else if (name == "batch_job") {
     gen_batch_job(&s, seed, P(0, 20000), (int)P(1, 256), (int)P(2, 256), (int)P(3, 3));
   } else {

@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, in
           Isect is;
           make_isect(sc, hp, w.hu[p], w.hv[p], &is);
           Bsdf bs;
-          make_bsdf(sc.mats + 16 * (size_t)is.mat, is.ng, is.ns, is.u, is.v, false, &bs);
+          make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)is.mat, is.ng, is.ns, is.u, is.v, false, &bs);
           V3 wo = neg(d);
           if ((depth == 0 || (fl & F_SPEC)) && is.light >= 0) acc(L, beta * area_emission(sc, is, wo));
           uint32_t index = (uint32_t)iter * (uint32_t)w.npix + (uint32_t)p;
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px,
               V3 vp = mk(px.vpx[q], px.vpy[q], px.vpz[q]);
               if (lensq(sub(vp, is.p)) > r * r) continue;
               Bsdf vb;
-              make_bsdf(sc.mats + 16 * (size_t)px.vmat[q], ld3(px.vng, q), ld3(px.vns, q), ld3(px.vu, q),
+              make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)px.vmat[q], ld3(px.vng, q), ld3(px.vns, q), ld3(px.vu, q),
                         ld3(px.vv, q), false, &vb);
               Sp Phi = beta * bsdf_f(vb, neg(d), mk(px.vwox[q], px.vwoy[q], px.vwoz[q]), ALL);
               atomicAdd(&px.Phir[q], Phi.r);
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px,
           }
         }
         Bsdf bs;
-        make_bsdf(sc.mats + 16 * (size_t)is.mat, is.ng, is.ns, is.u, is.v, true, &bs);
+        make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)is.mat, is.ng, is.ns, is.u, is.v, true, &bs);
         uint32_t index = index0 + (uint32_t)p;
         int dim = w.dim[p];
         float u1 = sobol(sc.sobol, index, dim++), u0 = sobol(sc.sobol, index, dim++);

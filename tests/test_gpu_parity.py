@@ -122,6 +122,22 @@ def test_path_icosphere_crop(device, oracle):
     assert mean_err <= MEAN_TOL
 
 
+def test_path_all_materials(device, oracle):
+    """Every material kind: Oren-Nayar and Lambert walls, plastic and mirror
+    boxes, an uber sphere with all five lobes, rough glass and gold spheres."""
+    hs = oa.HostScene.generate("materials")
+    ds = oa.DeviceScene(device, hs)
+    ds.film_clear()
+    ds.render_path(0, 16, 6)
+    rgb, cnt = ds.film()
+    assert (cnt == 16).all()
+    ref, rcnt, _ = oracle.scene(hs).render_path(16, 6)
+    frac, mean_err, bits = image_compare(rgb / cnt[..., None], ref / rcnt[..., None])
+    print(f"materials: within 1e-3 {frac:.6f}, mean rel err {mean_err:.2e}, bit-equal {bits:.6f}")
+    assert frac >= PIX_FRAC
+    assert mean_err <= MEAN_TOL
+
+
 def test_path_waves_equal_single(device):
     """Splitting the sample range into several wavefronts must not change a bit."""
     hs = oa.HostScene.generate("cornell", params=[64, 64])

@@ -15,13 +15,14 @@ PIX_FRAC = 0.999
 MEAN_TOL = 1e-4
 
 
-def _compare(device, oracle, params, iters, photons, radius, hash_size, debug_iter):
-    hs = oa.HostScene.generate("caustics", params=params)
+def _compare(device, oracle, params, iters, photons, radius, hash_size, debug_iter, scene="caustics",
+             depth=10):
+    hs = oa.HostScene.generate(scene, params=params)
     ds = oa.DeviceScene(device, hs)
     ds.film_clear()
-    st, gd = ds.render_sppm(iters, photons, radius, 10, hash_size=hash_size, debug_iter=debug_iter)
+    st, gd = ds.render_sppm(iters, photons, radius, depth, hash_size=hash_size, debug_iter=debug_iter)
     g_rgb, g_cnt = ds.film()
-    o_rgb, o_cnt, od = oracle.scene(hs).render_sppm(iters, photons, radius, 10, hash_size=hash_size,
+    o_rgb, o_cnt, od = oracle.scene(hs).render_sppm(iters, photons, radius, depth, hash_size=hash_size,
                                                     debug_iter=debug_iter)
     np.testing.assert_array_equal(gd["grid_info"][:4], od["grid_info"][:4])
     np.testing.assert_array_equal(gd["grid_bounds"], od["grid_bounds"])
@@ -33,7 +34,7 @@ def _compare(device, oracle, params, iters, photons, radius, hash_size, debug_it
     o = o_rgb / o_cnt[..., None]
     ok = (np.abs(g - o) <= REL_TOL * np.maximum(np.abs(o), 1e-6)).all(-1).mean()
     mean_err = abs(g.mean() - o.mean()) / max(abs(o.mean()), 1e-12)
-    print(f"sppm {params} iters={iters} photons={photons} hash={hash_size}: pairs={gd['grid_info'][3]} "
+    print(f"sppm {scene} {params} iters={iters} photons={photons} hash={hash_size}: pairs={gd['grid_info'][3]} "
           f"M={gd['M'].sum()} photon-cells={gd['photon_cell_count'].sum()} within1e-3={ok:.6f} "
           f"mean_err={mean_err:.2e} deposits={st.deposits}")
     assert gd["M"].sum() > 0 and gd["photon_cell_count"].sum() > 0
@@ -50,3 +51,9 @@ def test_sppm_reduced(device, oracle, hash_size):
 def test_sppm_c3_single_iteration(device, oracle):
     """C3's sizes for one iteration: 512^2, 2^20 photons, r0 0.01, 2^22 cells."""
     _compare(device, oracle, [6, 512, 512], 1, 1 << 20, 0.01, 1 << 22, 0)
+
+
+def test_sppm_all_materials(device, oracle):
+    """Every material kind (Oren-Nayar, plastic, mirror, uber, rough glass,
+    gold) under photon transport mode, area + point lights."""
+    _compare(device, oracle, [96, 96], 4, 1 << 16, 0.02, 0, 3, scene="materials", depth=6)

@@ -74,7 +74,8 @@ def test_camera_record_matches_reference(ref):
 
 
 @pytest.mark.parametrize("scene,params,limit", [("cornell", None, 1000), ("icosphere", [3, 64, 48], 100),
-                                                ("caustics", [2, 40, 30], 10 ** 9)])
+                                                ("caustics", [2, 40, 30], 10 ** 9),
+                                                ("materials", [32, 32], 1000)])
 def test_ori_round_trip(tmp_path, scene, params, limit):
     hs = oa.HostScene.generate(scene, params=params)
     path = tmp_path / "scene.ori"
@@ -86,8 +87,8 @@ def test_ori_round_trip(tmp_path, scene, params, limit):
         np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32), err_msg=name)
     # the loader makes one material per Model (parser.cpp:95-96), so indices are
     # renumbered; the resolved material of every triangle is unchanged
-    ma = hs.mats.reshape(-1, 16)[hs.tri_mat]
-    mb = back.mats.reshape(-1, 16)[back.tri_mat]
+    ma = hs.mats.reshape(-1, oa.MAT_FLOATS)[hs.tri_mat]
+    mb = back.mats.reshape(-1, oa.MAT_FLOATS)[back.tri_mat]
     np.testing.assert_array_equal(ma.view(np.int32), mb.view(np.int32))
     assert back.info.integrator == hs.info.integrator and back.info.max_depth == hs.info.max_depth
 
@@ -119,7 +120,7 @@ def test_loader_schema_features(tmp_path):
     assert hs.n_tris == 2 and hs.n_mats == 2 and hs.n_lights == 2
     p = hs.tri_pn.reshape(-1, 18)
     np.testing.assert_array_equal(p[0, :9], [0, 1, 0, 2, 1, 0, 0, 3, 0])
-    m = hs.mats.reshape(-1, 16)
+    m = hs.mats.reshape(-1, oa.MAT_FLOATS)
     np.testing.assert_array_equal(m[0], m[1])
     assert m[0][0] == 1 and m[0][7] == np.float32(0.2)
     L = hs.lights.reshape(-1, 32)
@@ -127,6 +128,45 @@ def test_loader_schema_features(tmp_path):
     assert L[1][0] == 1 and (L[1][2:5] == [1, 2, 3]).all() and (L[1][5:8] == [0, 4, 0]).all()
     assert list(hs.tri_light) == [-1, 0]
     assert hs.info.spp == 4 and hs.info.max_depth == 5 and hs.width == 32 and hs.height == 24
+
+
+def test_loader_materials(tmp_path, ref):
+    """mirror / plastic / uber / Oren-Nayar matte records (parser.cpp:262-421)."""
+    tri = "<Param p='0,0,0, 1,0,0, 0,1,0'/><Param n='0,0,1'/>"
+    def c(n, v):
+        return f"<{n}><Param type='constant'/><Param color='{v}'/></{n}>"
+    def f(n, v):
+        return f"<{n}><Param type='float'/><Param value='{v}'/></{n}>"
+    mats = [
+        "<Material type='mirror'>" + c("Kr", "0.9,0.8,0.7") + "</Material>",
+        "<Material type='plastic'>" + c("Kd", "0.1,0.2,0.3") + f("Ks", "0.4") + f("roughness", "0.12") +
+        "</Material>",
+        "<Material type='uber'>" + c("Kd", "0.5,0.4,0.3") + c("Ks", "0.2,0.2,0.2") + c("Kr", "0.1,0.1,0.1") +
+        c("Kt", "0.3,0.3,0.3") + c("opacity", "0.6,0.7,0.8") + f("roughness", "0.2") + f("eta", "1.45") +
+        "</Material>",
+        "<Material type='matte'>" + c("Kd", "0.5") + f("sigma", "20") + "</Material>",
+        # a constant (non-float) sigma fails the FloatTexture cast: Lambert
+        "<Material type='matte'>" + c("Kd", "0.5") + c("sigma", "20,20,20") + "</Material>",
+    ]
+    body = ("<Scene><Camera type='pinhole'><Param fov='40'/><Param filmsize='4,4'/></Camera>" +
+            "".join(f"<Model type='meshHandy'>{tri}{m}</Model>" for m in mats) +
+            "<Integrator type='path'><Param maxDepth='1'/><Param nSamplesPerPixel='1'/></Integrator></Scene>")
+    (tmp_path / "m.ori").write_text(body)
+    hs = oa.HostScene.load_ori(tmp_path / "m.ori")
+    m = hs.mats.reshape(-1, oa.MAT_FLOATS)
+    f32 = np.float32
+    assert m[0][0] == 3 and (m[0][1:4] == f32([0.9, 0.8, 0.7])).all()
+    assert m[1][0] == 4 and (m[1][1:4] == f32([0.1, 0.2, 0.3])).all() and (m[1][4:7] == f32(0.4)).all()
+    assert m[1][7] == f32(0.12) and m[1][8] == f32(ref.L.ref_roughness_to_alpha(f32(0.12)))
+    assert m[2][0] == 5 and (m[2][13:16] == f32([0.6, 0.7, 0.8])).all() and (m[2][10:13] == f32(0.3)).all()
+    assert m[2][16] == f32(0.2) and m[2][17] == f32(ref.L.ref_roughness_to_alpha(f32(0.2))) and m[2][18] == f32(1.45)
+    assert m[3][0] == 0 and m[3][4] == 20
+    assert m[4][0] == 0 and m[4][4] == 0
+    # uber needs every texture (the reference dereferences each one)
+    (tmp_path / "u.ori").write_text(body.replace(c("Kt", "0.3,0.3,0.3"), ""))
+    with pytest.raises(oa.OrionError) as e:
+        oa.HostScene.load_ori(tmp_path / "u.ori")
+    assert "Kt" in str(e.value)
 
 
 @pytest.mark.parametrize("body,frag", [

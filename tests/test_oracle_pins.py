@@ -153,34 +153,43 @@ def _frames(rng, n):
 
 
 MATS = {
-    # kind, params (as oracle/_ref make_material), metal preset name
-    "matte": (0, [0.63, 0.06, 0.05, 0, 0, 0, 0, 0, 0], b""),
+    # kind, params (a material record without its type word, as oracle/_ref
+    # make_material reads them), metal preset name
+    "matte": (0, [0.63, 0.06, 0.05, 0], b""),
+    "matte_oren_nayar": (0, [0.5, 0.4, 0.3, 20.0], b""),
+    # sigma above 90: Spectrum::clamp(0, 90) keeps it (the upper bound is ignored)
+    "matte_oren_nayar_wide": (0, [0.5, 0.4, 0.3, 120.0], b""),
     "gold": (1, [0, 0, 0, 0, 0, 0, 0.3, 0, 0], b"gold"),
     "metal_explicit": (1, [0.2, 0.9, 1.1, 3.9, 2.4, 2.1, 0.07, 0, 0], b""),
     "glass_smooth": (2, [1, 1, 1, 1, 1, 1, 0.0, 0, 1.5], b""),
     "glass_rough": (2, [0.9, 0.9, 0.9, 0.8, 0.8, 0.8, 0.25, 0, 1.33], b""),
+    "mirror": (3, [0.9, 0.8, -0.2], b""),
+    "plastic": (4, [0.25, 0.3, 0.65, 0.4, 0.4, 0.4, 0.12], b""),
+    "plastic_specular_only": (4, [0, 0, 0, 0.5, 0.6, 0.7, 0.004], b""),
+    # Kd: [0] Ks: [3] Kr: [6] Kt: [9] opacity: [12] roughness: [15] eta: [17]
+    "uber": (5, [0.5, -0.1, 0.3, 0.2, 0.2, 0.2, 0.25, 0.25, 0.25, 0.4, 0.5, 0.6, 0.6, 0.75, 1.2,
+                 0.2, 0, 1.45], b""),
+    "uber_opaque": (5, [0.4, 0.4, 0.4, 0.3, 0.3, 0.3, 0, 0, 0, 0, 0, 0, 1, 1, 1, 0.35, 0, 1.5], b""),
+    "uber_clear": (5, [0.4, 0.4, 0.4, 0, 0, 0, 0, 0, 0, 0.5, 0.5, 0.5, 0, 0, 0, 0.1, 0, 1.5], b""),
 }
 
 
 def _record(name):
     kind, p, preset = MATS[name]
-    if kind == 0:
-        rec = [0, p[0], p[1], p[2]] + [0] * 12
-    elif kind == 1:
-        if preset:
-            eta, k = oa_metal_preset(preset.decode())
-        else:
-            eta, k = p[0:3], p[3:6]
-        rec = [1, *eta, *k, p[6], 0] + [0] * 7
+    rec = np.zeros(oa.MAT_FLOATS, np.float32)
+    rec[0] = kind
+    if kind == 1 and preset:
+        eta, k = oa_metal_preset(preset.decode())
+        rec[1:4], rec[4:7], rec[7] = eta, k, p[6]
     else:
-        rec = [2, *p[0:3], *p[3:6], p[6], 0, p[8]] + [0] * 6
-    return np.array(rec, np.float32)
+        rec[1:1 + len(p)] = p
+    return rec
 
 
 def oa_metal_preset(name):
     hs = oa.HostScene.generate("icosphere", params=[0, 8, 8])  # gold regions
     assert name == "gold"
-    m = hs.mats.reshape(-1, 16)[0]
+    m = hs.mats.reshape(-1, oa.MAT_FLOATS)[0]
     return m[1:4], m[4:7]
 
 
@@ -249,6 +258,47 @@ def test_fresnel_closed_forms(mat_scene):
     assert abs(pdf - 0.8 / np.pi) < 1e-6
 
 
+def test_new_lobes_closed_forms(mat_scene):
+    """Mirror, Oren-Nayar, plastic and uber lobes against what the formulas fix."""
+    frame = np.array([0, 0, 1, 0, 0, 1, 1, 0, 0, 0, 1, 0], np.float32)
+    names = list(MATS)
+    # mirror: wi = reflection of wo, pdf 1, f * |cos| = clamp(Kr) (FresnelNoOp)
+    wo = np.array([0.36, 0.48, 0.8], np.float32)
+    f, wi, pdf, t = mat_scene.bsdf_sample(names.index("mirror"), frame, wo, [0.3, 0.7])
+    assert t == (16 | 1) and pdf == 1
+    np.testing.assert_allclose(wi, [-0.36, -0.48, 0.8], atol=1e-7)
+    np.testing.assert_allclose(f * 0.8, [0.9, 0.8, 0.0], rtol=1e-6)
+    # mirror f and pdf are zero for any pair (delta lobe)
+    f, pdf = mat_scene.bsdf_f_pdf(names.index("mirror"), frame, [-0.36, -0.48, 0.8], wo)
+    assert (f == 0).all() and pdf == 0
+    # Oren-Nayar at normal incidence: sin(theta_o) = 0, f = R / pi * A with
+    # A = 1 - s2 / (2 (s2 + 0.33)), s = sigma in radians
+    s2 = np.radians(20.0) ** 2
+    A = 1 - s2 / (2 * (s2 + 0.33))
+    f, pdf = mat_scene.bsdf_f_pdf(names.index("matte_oren_nayar"), frame, [0, 0.6, 0.8], [0, 0, 1])
+    np.testing.assert_allclose(f, np.array([0.5, 0.4, 0.3]) / np.pi * A, rtol=1e-5)
+    assert abs(pdf - 0.8 / np.pi) < 1e-6
+    # Oren-Nayar with wi, wo in one plane on the same side of the normal:
+    # cos(phi_i - phi_o) = 1, f = R / pi (A + B sin(alpha) tan(beta))
+    B = 0.45 * s2 / (s2 + 0.09)
+    ti, to = 0.5, 0.9
+    wi_ = [np.sin(ti), 0, np.cos(ti)]; wo_ = [np.sin(to), 0, np.cos(to)]
+    f, _ = mat_scene.bsdf_f_pdf(names.index("matte_oren_nayar"), frame, wi_, wo_)
+    np.testing.assert_allclose(f, np.array([0.5, 0.4, 0.3]) / np.pi * (A + B * np.sin(to) * np.tan(ti)),
+                               rtol=1e-5)
+    # plastic: Lambert + GGX with Fresnel of a 1.5 -> 1 interface; the sum of
+    # the two lobe pdfs halves (two matching lobes)
+    f, pdf = mat_scene.bsdf_f_pdf(names.index("plastic"), frame, [0, 0.6, 0.8], [0, -0.6, 0.8])
+    assert (f > np.array([0.25, 0.3, 0.65]) / np.pi).all() and pdf > 0.8 / np.pi / 2
+    # uber with opacity 0: one straight-through specular transmission lobe,
+    # f * |cos| = 1 - F(1 | 1) = 1, direction -wo, pdf 1
+    wo = np.array([0.28, 0.0, 0.96], np.float32)
+    f, wi, pdf, t = mat_scene.bsdf_sample(names.index("uber_clear"), frame, wo, [0.5, 0.5])
+    assert t == (16 | 2) and pdf == 1
+    np.testing.assert_allclose(wi, -wo, atol=1e-6)
+    np.testing.assert_allclose(f * 0.96, [1, 1, 1], rtol=1e-5)
+
+
 def test_ggx_normalisation(mat_scene):
     """integral of D(wh) cos(theta_h) over the hemisphere is 1 (checked through
     the reflection pdf's half-vector density, numerically)."""
@@ -305,7 +355,7 @@ def test_lights_match_reference(oracle, ref, typ):
 
 def test_metal_presets_match_reference(ref):
     hs = oa.HostScene.generate("icosphere", params=[0, 8, 8])
-    m = hs.mats.reshape(-1, 16)[0]
+    m = hs.mats.reshape(-1, oa.MAT_FLOATS)[0]
     eta, k = ref.metal_preset("gold")
     assert bits_equal(m[1:4], eta) and bits_equal(m[4:7], k)
     assert m[8] == ref.L.ref_roughness_to_alpha(float(m[7]))

@@ -35,10 +35,18 @@
  *                      [13..15] opacity  [16] roughness  [17] alpha  [18] eta
  *             (alpha = GGX::roughnessToAlpha(roughness), microfacet.h:44-50,
  *             evaluated once on the host)
- *   light     32 floats  [0] type: 0 area, 1 point, 2 spot  [1] triangle index
- *             (area)  [2..4] Le or I  [5..7] position  [8] spot cone angle
- *             [9] cosTotalWidth  [10] cosFalloffStart  [11..19] world-to-light
- *             3x3  [20..28] light-to-world 3x3 (spot)  [29..31] spot target
+ *   light     32 floats  [0] type: 0 area, 1 point, 2 spot, 3 distant,
+ *             4 envmap
+ *             area/point/spot: [1] triangle index (area)  [2..4] Le or I
+ *             [5..7] position  [8] spot cone angle  [9] cosTotalWidth
+ *             [10] cosFalloffStart  [11..19] world-to-light 3x3
+ *             [20..28] light-to-world 3x3 (spot)  [29..31] spot target
+ *             distant: [2..4] L  [5..7] unit direction towards the light
+ *             (light/distant.cpp:58-65)  [8..10] from  [11..13] to
+ *             envmap:  [1] envmap image index  [2..4] L  [5] width
+ *             [6] height  [7] nLightSamples  [8..16] light-to-world 3x3
+ *             [17..25] world-to-light 3x3  [26..28] worldUp
+ *             Envmaps come after every other light (parser.cpp:45-56).
  *   camera    16 floats  origin(3), camera-to-world 3x3 row-major(9),
  *             tan(vfov/2), aspect, W, H   (camera/pinhole.cpp:8-33)
  */
@@ -84,8 +92,8 @@ typedef struct {
  * AreaLight, Light (point | spot), Integrator (path | sppm), Sampler (sobol). */
 int orion_hscene_load_ori(const char *path, orion_hscene *out);
 /* Synthetic scenes: name is one of "cornell", "icosphere", "caustics",
- * "batch_job", "materials"; params (may be NULL) override defaults:
- *   cornell, materials: [0] width [1] height
+ * "batch_job", "materials", "sky"; params (may be NULL) override defaults:
+ *   cornell, materials, sky: [0] width [1] height
  *   icosphere: [0] level [1] width [2] height
  *   caustics:  [0] level [1] width [2] height
  *   batch_job: [0] triangle count [1] width [2] height [3] material mix */
@@ -116,7 +124,21 @@ int orion_camera_record(const float lookat[9], float fov_deg, int width, int hei
 /* createTransform over (key, value) pairs (transform.cpp:218-252): m and mInv, row-major. */
 int orion_create_transform(int n, const char *const *keys, const char *const *values,
                            float m[16], float m_inv[16]);
-/* Resolved spot/point light record as the loader builds it (light/spot.cpp:46-59). */
+/* Appends an environment light (light/envmap.cpp:9-42, createEnvMap
+ * :108-128).  rgb: width x height RGB texels, row 0 first, as loadImage gives
+ * them (byte / 255; texture/imagetexture.cpp:22-48); NULL with width =
+ * height = 1 is the constant map L.  The map is L x rgb.  light_to_world and
+ * world_to_light are row-major 4x4 (their 3x3 parts orient the map). */
+int orion_hscene_add_envmap(orion_hscene s, int width, int height, const float *rgb,
+                            const float L[3], const float light_to_world[16],
+                            const float world_to_light[16]);
+/* Texels (before L) of envmap image k: borrowed pointer. */
+int orion_hscene_envmap_image(orion_hscene s, int k, int *width, int *height,
+                              const float **rgb);
+/* Resolved point / spot / distant light record as the loader builds it
+ * (light/point.cpp:34-40, light/spot.cpp:46-59, light/distant.cpp:58-65):
+ * type 1 point (I, from), 2 spot (I, from, to, angle, delta), 3 distant
+ * (L, from, to). */
 int orion_light_record(int type, const float I[3], const float from[3], const float to[3],
                        float angle, float delta, float record[32]);
 

@@ -18,6 +18,10 @@
 //   warps                  sampling.cpp:5-61   -> oracle/_ref + closed forms
 //   BSDF f/pdf/sample_f    bsdf.cpp, bsdf/*    -> oracle/_ref Matte/Metal/Glass::getBSDF
 //   point/spot lights      light/point.cpp, light/spot.cpp -> oracle/_ref
+//   distant light, envmap  light/distant.cpp, light/envmap.cpp, Distribution2D
+//                          (sampling.h:127-150) -> oracle/_ref DistantLight /
+//                          Envmap built from the same image; closed forms
+//                          (constant map: Le = L, pdf of a uniform map)
 //   area light, triangle   light/diffuse.cpp, core/triangle.cpp:85-135
 //                          -> uniformSampleTriangle via _ref; rest by closed forms
 //                             (solid-angle pdf of a quad, constant-emission scene)
@@ -30,10 +34,11 @@
 //   SPPM                   integrator/sppm.cpp -> ToGrid/hash integer closed forms,
 //                          brute-force per-cell counts, radius-update closed form
 //
-// sin/cos: the reference calls sinf/cosf (<= 0.56 ulp in glibc).  The oracle
-// evaluates (float)sin((double)x), the correctly rounded value except in ~1e-8
-// of inputs, so that the GPU (which does the same in FP64) follows identical
-// sample paths.  Build with -DORACLE_LIBM_TRIG to use sinf/cosf instead.
+// sin/cos/acos/atan2: the reference calls the float libm routines.  The
+// oracle evaluates them in double and rounds once, the correctly rounded value
+// except in ~1e-8 of inputs, so that the GPU (which does the same in FP64)
+// follows identical sample paths.  Build with -DORACLE_LIBM_TRIG to use the
+// float routines instead.
 #include "oracle.h"
 
 #include <algorithm>
@@ -70,6 +75,20 @@ float osin(float x) {
   return std::sin(x);
 #else
   return (float)std::sin((double)x);
+#endif
+}
+float oacos(float x) {
+#ifdef ORACLE_LIBM_TRIG
+  return std::acos(x);
+#else
+  return (float)std::acos((double)x);
+#endif
+}
+float oatan2(float y, float x) {
+#ifdef ORACLE_LIBM_TRIG
+  return std::atan2(y, x);
+#else
+  return (float)std::atan2((double)y, (double)x);
 #endif
 }
 float ocos(float x) {
@@ -235,14 +254,16 @@ struct Tri { V3 p0, p1, p2, n0, n1, n2; int mat, light; };
 constexpr int kMatFloats = 24;  // material record: type + 23 parameters
 struct Material { int type; float v[kMatFloats - 1]; };
 enum { MAT_MATTE = 0, MAT_METAL = 1, MAT_GLASS = 2, MAT_MIRROR = 3, MAT_PLASTIC = 4, MAT_UBER = 5 };
-enum { LIGHT_AREA = 0, LIGHT_POINT = 1, LIGHT_SPOT = 2 };
+enum { LIGHT_AREA = 0, LIGHT_POINT = 1, LIGHT_SPOT = 2, LIGHT_DISTANT = 3, LIGHT_ENVMAP = 4 };
 struct Light {
   int type, prim;
-  S L;          // Le (area) or I (point/spot)
+  S L;          // Le (area), I (point/spot), L (distant, envmap)
   V3 pos;
   float area;
   float cosTotal, cosFalloffStart;
   float w2l[9], l2w[9];
+  V3 dir;       // distant: unit direction towards the light
+  int env;      // envmap: index into Scene::envs
 };
 
 struct Distribution1D {  // sampling.h:71-125
@@ -261,6 +282,25 @@ struct Distribution1D {  // sampling.h:71-125
     }
   }
   int count() const { return (int)func.size(); }
+  int find_interval(float u) const {  // geometry.h:876-889
+    int size = (int)cdf.size();
+    int first = 0, length = size;
+    while (length > 0) {
+      int half = length >> 1, middle = first + half;
+      if (cdf[middle] <= u) { first = middle + 1; length -= half + 1; }
+      else length = half;
+    }
+    int off = first - 1;
+    return off < 0 ? 0 : (off > size - 2 ? size - 2 : off);
+  }
+  float sample_continuous(float u, float *pdf, int *o) const {  // sampling.h:91-109
+    int offset = find_interval(u);
+    if (o) *o = offset;
+    float du = u - cdf[offset];
+    if ((cdf[offset + 1] - cdf[offset]) > 0) du /= (cdf[offset + 1] - cdf[offset]);
+    if (pdf) *pdf = (funcInt > 0) ? func[offset] / funcInt : 0;
+    return (offset + du) / count();
+  }
   int sample_discrete(float u, float *pdf) const {
     // findInterval, geometry.h:876-889
     int size = (int)cdf.size();
@@ -279,6 +319,37 @@ struct Distribution1D {  // sampling.h:71-125
 
 struct BVHNode { float lo[3], hi[3]; int left_or_first, count; };  // count>0: leaf
 
+// Envmap (light/envmap.cpp:9-42): the map L x image, its average and the
+// Distribution2D over intensity x sin(theta) (sampling.h:127-150,
+// sampling.cpp:63-76)
+struct EnvMap {
+  int w = 1, h = 1;
+  std::vector<S> tex;
+  S ave;
+  std::vector<Distribution1D> cond;
+  Distribution1D marg;
+  S sample(float u, float v) const {  // Texture::sample(float, float), texture.cpp:4-8
+    return texel((int)(u * w), (int)(v * h));
+  }
+  S texel(int x, int y) const {  // ImageTexture::sample(int, int), FILTER_WRAP (texture.cpp:13-24)
+    if (x >= 0) x = x % w; else x = w - (-x) % w;
+    if (y >= 0) y = y % h; else y = h - (-y) % h;
+    return tex[(size_t)y * w + x];
+  }
+  void sample_uv(float u0, float u1, float *su, float *sv, float *pdf) const {
+    float p1, p0;
+    int v;
+    *sv = marg.sample_continuous(u1, &p1, &v);
+    *su = cond[v].sample_continuous(u0, &p0, nullptr);
+    *pdf = p0 * p1;
+  }
+  float uv_pdf(float pu, float pv) const {
+    int iu = std::min(std::max((int)(pu * cond[0].count()), 0), cond[0].count() - 1);
+    int iv = std::min(std::max((int)(pv * marg.count()), 0), marg.count() - 1);
+    return cond[iv].func[iu] / marg.funcInt;
+  }
+};
+
 struct Scene {
   std::vector<Tri> tris;
   std::vector<Material> mats;
@@ -286,6 +357,10 @@ struct Scene {
   float cam[16];
   int W, H;
   Distribution1D power;
+  std::vector<EnvMap> envs;
+  std::vector<int> env_lights;   // Scene::envmaps (scene.cpp:10-14)
+  V3 world_c = {0, 0, 0};
+  float world_r = 0;
   std::vector<BVHNode> nodes;
   std::vector<int> order;
 };
@@ -958,7 +1033,46 @@ float tri_area(const Tri &t) { return 0.5f * std::sqrt(lensq(cross(sub(t.p1, t.p
 S light_power(const Scene &sc, const Light &l) {
   if (l.type == LIGHT_AREA) return l.L * l.area * kPi;  // diffuse.cpp:8-11
   if (l.type == LIGHT_POINT) return l.L * (4 * kPi);    // point.cpp:6-9
+  if (l.type == LIGHT_DISTANT) return l.L * kPi * sc.world_r * sc.world_r;  // distant.cpp:11-14
+  if (l.type == LIGHT_ENVMAP) return sc.envs[l.env].ave * (kPi * sc.world_r * sc.world_r);  // envmap.cpp:51-54
   return l.L * 2 * kPi * (1 - .5f * (l.cosFalloffStart + l.cosTotal));  // spot.cpp:6-9
+}
+
+// sampling.h:60-68
+inline float spherical_theta(V3 v) { return oacos(clampf(v.z, -1.0f, 1.0f)); }
+inline float spherical_phi(V3 v) {
+  float p = oatan2(v.y, v.x);
+  return (p < 0) ? (p + 2 * kPi) : p;
+}
+constexpr float kInv2Pi = 0.15915494309189533577f;  // orion.h:54
+// sampling.cpp:35-54
+inline void concentric_disk(float u0, float u1, float *x, float *y) {
+  float ox = 2.f * u0 - 1.0f, oy = 2.f * u1 - 1.0f;
+  if (ox == 0 && oy == 0) { *x = 0; *y = 0; return; }
+  float theta, r;
+  if (std::abs(ox) > std::abs(oy)) {
+    r = ox;
+    theta = 0.78539816339744830961f * (oy / ox);
+  } else {
+    r = oy;
+    theta = 1.57079632679489661923f - 0.78539816339744830961f * (ox / oy);
+  }
+  *x = ocos(theta) * r;
+  *y = osin(theta) * r;
+}
+// Envmap::Le (envmap.cpp:44-49)
+S env_le(const Scene &sc, const Light &l, V3 dir) {
+  V3 w = normalize(mat3_apply(l.w2l, dir));
+  float u = spherical_phi(w) * kInv2Pi, v = spherical_theta(w) * kInvPi;
+  return sc.envs[l.env].sample(u, v);
+}
+// Envmap::pdf_Li (envmap.cpp:74-82): theta scaled by 1 / (2 pi) and the
+// zero-sine test comparing a function address (never true), as written.
+float env_pdf_li(const Scene &sc, const Light &l, V3 wi) {
+  V3 swi = mat3_apply(l.w2l, wi);
+  float theta = spherical_theta(swi), phi = spherical_phi(swi);
+  float sinTh = osin(theta);
+  return sc.envs[l.env].uv_pdf(phi * kInv2Pi, theta * kInv2Pi) / (2 * kPi * kPi * sinTh);
 }
 float spot_falloff(const Light &l, V3 w) {  // spot.cpp:10-19
   V3 wl = normalize(mat3_apply(l.w2l, w));
@@ -1000,6 +1114,26 @@ S light_sample_li(const Scene &sc, const Light &l, const Isect &ref, float u0, f
     *shadow = spawn_to(ref, p);
     return area_L(l, ng, neg(*wi));
   }
+  if (l.type == LIGHT_DISTANT) {  // distant.cpp:16-27
+    *wi = l.dir;
+    *pdf = 1;
+    *shadow = spawn_to(ref, add(ref.p, mul(l.dir, 2 * sc.world_r)));
+    return l.L;
+  }
+  if (l.type == LIGHT_ENVMAP) {  // envmap.cpp:55-73
+    const EnvMap &em = sc.envs[l.env];
+    float su, sv, mapPdf;
+    em.sample_uv(u0, u1, &su, &sv, &mapPdf);
+    if (mapPdf == 0) { *pdf = 0; return sp(0); }
+    float theta = sv * kPi, phi = su * 2 * kPi;
+    float cosTheta = ocos(theta), sinTheta = osin(theta);
+    float sinPhi = osin(phi), cosPhi = ocos(phi);
+    *wi = mat3_apply(l.l2w, mk(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta));
+    *pdf = mapPdf / (2 * kPi * kPi * sinTheta);
+    if (sinTheta == 0) *pdf = 0;
+    *shadow = spawn_to(ref, add(ref.p, mul(*wi, 2 * sc.world_r)));
+    return em.sample(su, sv);
+  }
   // point.cpp:10-19, spot.cpp:20-29
   *wi = normalize(sub(l.pos, ref.p));
   *pdf = 1.0f;
@@ -1024,6 +1158,36 @@ S light_sample_le(const Scene &sc, const Light &l, float a0, float a1, float b0,
     *ray = spawn_dir(is, w);
     return area_L(l, ng, w);
   }
+  if (l.type == LIGHT_DISTANT || l.type == LIGHT_ENVMAP) {
+    V3 dir;  // towards the light
+    float cx, cy, mapPdf = 1, sinTheta = 1, su = 0, sv = 0;
+    if (l.type == LIGHT_DISTANT) {  // distant.cpp:29-43
+      dir = l.dir;
+      concentric_disk(a0, a1, &cx, &cy);
+    } else {  // envmap.cpp:83-107: d = -dir
+      sc.envs[l.env].sample_uv(a0, a1, &su, &sv, &mapPdf);
+      if (mapPdf == 0) { *pdfPos = 0; *pdfDir = 0; return sp(0); }
+      float theta = sv * kPi, phi = su * 2.f * kPi;
+      float cosTheta = ocos(theta);
+      sinTheta = osin(theta);
+      float sinPhi = osin(phi), cosPhi = ocos(phi);
+      dir = mat3_apply(l.l2w, mk(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta));
+      concentric_disk(b0, b1, &cx, &cy);
+    }
+    V3 v1, v2;
+    coordinate_system(dir, &v1, &v2);
+    float r = sc.world_r;
+    V3 pDisk = add(sc.world_c, mul(add(mul(v1, cx), mul(v2, cy)), r));
+    *ray = Ray{add(pDisk, mul(dir, r)), neg(dir), 0, kInf};
+    *nLight = ray->d;
+    *pdfPos = 1 / (kPi * r * r);
+    if (l.type == LIGHT_DISTANT) {
+      *pdfDir = 1;
+      return l.L;
+    }
+    *pdfDir = sinTheta == 0 ? 0 : mapPdf / (2 * kPi * kPi * sinTheta);
+    return sc.envs[l.env].sample(su, sv);
+  }
   if (l.type == LIGHT_POINT) {  // point.cpp:20-27
     *ray = Ray{l.pos, uniform_sphere(a0, a1), 0, kInf};
     *nLight = ray->d;
@@ -1046,7 +1210,7 @@ S estimate_direct(const Scene &sc, const Ray &ray, const Isect &is, const BSDF &
                   float bu1, int li, float lu0, float lu1, Counters *c) {
   const Light &light = sc.lights[li];
   const int type = ALL & ~SPEC;
-  const bool delta = light.type != LIGHT_AREA;
+  const bool delta = light.type != LIGHT_AREA && light.type != LIGHT_ENVMAP;  // light.h:39-42
   S Ld = sp(0);
   V3 wi;
   V3 wo = normalize(neg(ray.d));
@@ -1072,7 +1236,19 @@ S estimate_direct(const Scene &sc, const Ray &ray, const Isect &is, const BSDF &
     S f = bsdf_sample(bs, &wi, wo, bu0, bu1, &bsdfPdf, type, &stype);
     f *= sp(absdot(wi, is.ns));
     bool sampledSpecular = (stype & SPEC) != 0;
-    if (!black(f) && bsdfPdf > 0) {
+    if (!black(f) && bsdfPdf > 0 && light.type == LIGHT_ENVMAP) {
+      float weight = 1;
+      if (!sampledSpecular) {
+        lightPdf = env_pdf_li(sc, light, wi);
+        if (lightPdf == 0) return Ld;
+        weight = power_heuristic(bsdfPdf, lightPdf);
+      }
+      Ray rr = spawn_dir(is, wi);
+      Isect in;
+      // a hit is never this light (no triangle carries it); a miss sees Le
+      S Lm = scene_intersect(sc, rr, &in, c) ? sp(0) : env_le(sc, light, rr.d);
+      if (!black(Lm)) Ld += f * Lm * weight / bsdfPdf;
+    } else if (!black(f) && bsdfPdf > 0) {
       float weight = 1;
       Ray rr = spawn_dir(is, wi);
       Isect in;
@@ -1142,6 +1318,8 @@ S path_Li(const Scene &sc, Ray ray, Sampler &s, int maxDepth, Counters *c) {
     bool found = scene_intersect(sc, ray, &is, c);
     if (bounces == 0 || specularBounce) {
       if (found) L += beta * isect_Le(sc, is, neg(ray.d));
+      else
+        for (int li : sc.env_lights) L += beta * env_le(sc, sc.lights[li], ray.d);  // path.cpp:65-68
     }
     if (!found || bounces >= maxDepth) break;
     BSDF bs;
@@ -1216,7 +1394,8 @@ extern "C" {
 
 void *oracle_scene_create(int n_tris, const float *pn, const int *tri_mat, const int *tri_light,
                           int n_mats, const float *mats, int n_lights, const float *lights,
-                          const float *camera) {
+                          const float *camera, int n_images, const int *image_wh,
+                          const float *const *image_rgb) {
   Scene *sc = new Scene();
   sc->tris.resize(n_tris);
   for (int i = 0; i < n_tris; ++i) {
@@ -1245,6 +1424,67 @@ void *oracle_scene_create(int n_tris, const float *pn, const int *tri_mat, const
     l.cosFalloffStart = q[10];
     for (int k = 0; k < 9; ++k) { l.w2l[k] = q[11 + k]; l.l2w[k] = q[20 + k]; }
     l.area = l.type == LIGHT_AREA ? tri_area(sc->tris[l.prim]) : 0;
+    l.dir = mk(q[5], q[6], q[7]);
+    l.env = -1;
+    if (l.type == LIGHT_ENVMAP) {
+      for (int k = 0; k < 9; ++k) { l.l2w[k] = q[8 + k]; l.w2l[k] = q[17 + k]; }
+      int img = (int)q[1];
+      if (img < 0 || img >= n_images) { delete sc; return nullptr; }
+      // Envmap::Envmap (envmap.cpp:9-42)
+      EnvMap em;
+      em.w = image_wh[2 * img];
+      em.h = image_wh[2 * img + 1];
+      const float *rgb = image_rgb[img];
+      em.tex.resize((size_t)em.w * em.h);
+      for (size_t t = 0; t < em.tex.size(); ++t) {
+        S px = sp(rgb[3 * t], rgb[3 * t + 1], rgb[3 * t + 2]);
+        px *= l.L;  // data[i] *= L
+        em.tex[t] = px;
+      }
+      std::vector<float> f((size_t)em.w * em.h);
+      S ave = sp(0);
+      for (int j = 0; j < em.h; ++j) {
+        float v = (j + 0.5f) / (float)em.h;
+        float sinTheta = std::sin(kPi * v);
+        for (int x = 0; x < em.w; ++x) {
+          S t = em.texel(x, j);
+          f[x + (size_t)j * em.w] = intensity(t) * sinTheta;
+          ave += t;
+        }
+      }
+      ave = ave / (float)(em.w * em.h);
+      em.ave = ave;
+      em.cond.resize(em.h);
+      std::vector<float> mf(em.h);
+      for (int v = 0; v < em.h; ++v) {
+        em.cond[v].init(&f[(size_t)v * em.w], em.w);
+        mf[v] = em.cond[v].funcInt;
+      }
+      em.marg.init(mf.data(), em.h);
+      l.env = (int)sc->envs.size();
+      sc->envs.push_back(std::move(em));
+      sc->env_lights.push_back(i);
+    }
+  }
+  // Scene::worldbound().boundingSphere (kernel.cpp:44-50, triangle.cpp:60-66,
+  // geometry.h:551-554)
+  {
+    float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
+    for (const Tri &t : sc->tris) {
+      const float a[3] = {t.p0.x, t.p0.y, t.p0.z}, b[3] = {t.p1.x, t.p1.y, t.p1.z}, c[3] = {t.p2.x, t.p2.y, t.p2.z};
+      for (int k = 0; k < 3; ++k) {
+        float tlo = fmin_ref(fmin_ref(a[k], b[k]), c[k]), thi = fmax_ref(fmax_ref(a[k], b[k]), c[k]);
+        lo[k] = fmin_ref(lo[k], tlo);
+        hi[k] = fmax_ref(hi[k], thi);
+      }
+    }
+    float inv = (float)1 / 2;
+    V3 ctr = mk(inv * (lo[0] + hi[0]), inv * (lo[1] + hi[1]), inv * (lo[2] + hi[2]));
+    bool inside = ctr.x >= lo[0] && ctr.x <= hi[0] && ctr.y >= lo[1] && ctr.y <= hi[1] && ctr.z >= lo[2] &&
+                  ctr.z <= hi[2];
+    V3 dd = sub(ctr, mk(hi[0], hi[1], hi[2]));
+    sc->world_c = ctr;
+    sc->world_r = inside ? std::sqrt(dd.x * dd.x + dd.y * dd.y + dd.z * dd.z) : 0;
   }
   for (int i = 0; i < n_lights; ++i) pw[i] = intensity(light_power(*sc, sc->lights[i]));
   if (n_lights > 0) sc->power.init(pw.data(), n_lights);  // path.cpp:39-48
@@ -1338,7 +1578,10 @@ int oracle_render_sppm(void *s, int nIterations, int photonPerIteration, float i
         bool specularBounce = false;
         for (int depth = 0; depth < maxDepth; ++depth) {
           Isect is;
-          if (!scene_intersect(sc, ray, &is, nullptr)) break;
+          if (!scene_intersect(sc, ray, &is, nullptr)) {  // sppm.cpp:95-99: envmaps at any depth
+            for (int li : sc.env_lights) pixel.Ld += beta * env_le(sc, sc.lights[li], ray.d);
+            break;
+          }
           BSDF bs;
           make_bsdf(sc, sc.tris[is.prim].mat, is.ng, is.ns, is.u, is.v, false, &bs);
           V3 wo = neg(ray.d);
@@ -1586,6 +1829,25 @@ void oracle_light_sample_le(void *s, int l, const float *u1, const float *u2, fl
   S Le = light_sample_le(sc, sc.lights[l], u1[0], u1[1], u2[0], u2[1], &r, &n, &pp, &pd);
   float o[] = {Le.r, Le.g, Le.b, r.o.x, r.o.y, r.o.z, r.d.x, r.d.y, r.d.z, n.x, n.y, n.z, pp, pd};
   std::memcpy(out, o, sizeof(o));
+}
+
+void oracle_env_le(void *s, int l, const float *d, float *out) {
+  const Scene &sc = *(const Scene *)s;
+  S r = env_le(sc, sc.lights[l], mk(d[0], d[1], d[2]));
+  out[0] = r.r; out[1] = r.g; out[2] = r.b;
+}
+float oracle_env_pdf_li(void *s, int l, const float *wi) {
+  const Scene &sc = *(const Scene *)s;
+  return env_pdf_li(sc, sc.lights[l], mk(wi[0], wi[1], wi[2]));
+}
+void oracle_light_power(void *s, int l, float *out) {
+  const Scene &sc = *(const Scene *)s;
+  S r = light_power(sc, sc.lights[l]);
+  out[0] = r.r; out[1] = r.g; out[2] = r.b;
+}
+void oracle_world_sphere(void *s, float *out) {
+  const Scene &sc = *(const Scene *)s;
+  out[0] = sc.world_c.x; out[1] = sc.world_c.y; out[2] = sc.world_c.z; out[3] = sc.world_r;
 }
 
 float oracle_sin(float x) { return osin(x); }

@@ -25,10 +25,13 @@ extern "C" {
 
 /* Scene handle.  tri_pn: n_tris x 18 floats (p0,p1,p2,n0,n1,n2, world space).
  * tri_mat: material index per triangle.  tri_light: area-light index or -1.
- * mats: n_mats x 24 floats.  lights: n_lights x 32 floats.  camera: 16 floats. */
+ * mats: n_mats x 24 floats.  lights: n_lights x 32 floats.  camera: 16 floats.
+ * Envmap images: image_wh[2k], image_wh[2k+1] = width, height of image k,
+ * image_rgb[k] its texels (before L).  Returns NULL on a bad image index. */
 void *oracle_scene_create(int n_tris, const float *tri_pn, const int *tri_mat,
                           const int *tri_light, int n_mats, const float *mats,
-                          int n_lights, const float *lights, const float *camera);
+                          int n_lights, const float *lights, const float *camera,
+                          int n_images, const int *image_wh, const float *const *image_rgb);
 void oracle_scene_destroy(void *scene);
 
 /* Closest / any hit over a batch of rays (rays: n x 8 = o, d, tmin, tmax).
@@ -84,6 +87,13 @@ void oracle_light_sample_li(void *scene, int l, const float *p, const float *ng,
 /* Light::sample_Le.  out: Le(3), o(3), d(3), nLight(3), pdfPos, pdfDir. */
 void oracle_light_sample_le(void *scene, int l, const float *u1, const float *u2,
                             float *out);
+/* Envmap light l: Le of a ray direction and pdf_Li of a direction
+ * (light/envmap.cpp:44-82); any light's power (Light::power); the world
+ * bounding sphere (center xyz, radius). */
+void oracle_env_le(void *scene, int l, const float *d, float *out);
+float oracle_env_pdf_li(void *scene, int l, const float *wi);
+void oracle_light_power(void *scene, int l, float *out);
+void oracle_world_sphere(void *scene, float *out);
 /* sin/cos as the renderer uses them. */
 float oracle_sin(float x);
 float oracle_cos(float x);

@@ -3,7 +3,8 @@
 // TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile from the translation
 // units under /root/reference/src that compile and link without Embree
 // (sampler/, bsdf/, core/bsdf, core/spectrum, core/transform, material/
-// matte|metal|glass|mirror|plastic|uber, light/point|spot, camera/pinhole, texture/, util/strutil);
+// matte|metal|glass|mirror|plastic|uber, light/point|spot|distant|envmap,
+// texture/imagetexture + the vendored stb_image_aug.c, camera/pinhole, texture/, util/strutil);
 // core/triangle.cpp and light/diffuse.cpp are excluded because they need
 // Scene::intersect -> EmbreeKernel, whose library ships only as a Windows
 // binary.  The harness reads private members through `#define private public`
@@ -26,6 +27,7 @@
 #include <xmmintrin.h>
 #include <pmmintrin.h>
 #include <unistd.h>
+#include <stb_image_aug.h>
 // Standard headers are included first so the access override below touches
 // only the reference's own classes.
 #define private public
@@ -48,6 +50,9 @@
 #include <texture/floattexture.h>
 #include <light/point.h>
 #include <light/spot.h>
+#include <light/distant.h>
+#include <light/envmap.h>
+#include <texture/imagetexture.h>
 #include <bsdf/microfacet.h>
 #include <util/param.h>
 #include <util/strutil.h>
@@ -307,6 +312,121 @@ void ref_spawn(const float *p, const float *ng, const float *d, float *o3) {
   Intersection is(Point3f(p[0], p[1], p[2]));
   is.ng = Normal3f(ng[0], ng[1], ng[2]);
   put3(o3, is.spawnRay(Vector3f(d[0], d[1], d[2])).o);
+}
+
+
+// ------------------------------------------------------------------ infinite lights
+// The Scene a light's preprocess reads cannot be built (Embree): the world
+// sphere is set directly, and pdf_Li's unused Scene argument is a stand-in
+// that is never touched.
+namespace {
+const Scene &no_scene() {
+  alignas(Scene) static char storage[sizeof(Scene)];
+  return *reinterpret_cast<const Scene *>(storage);
+}
+Transform xform16(const float *m, const float *mi) {
+  Matrix4f a, b;
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) { a.m[r][c] = m[4 * r + c]; b.m[r][c] = mi[4 * r + c]; }
+  return Transform(a, b);
+}
+void put_ray(float *o, const Ray &r) {
+  put3(o, r.o); put3(o + 3, r.d); o[6] = r.tMax;
+}
+// out: Li(3) wi(3) pdf shadow ray o(3) d(3) tmax
+void sample_li(const Light &l, const float *p, const float *ng, const float *u, float *out) {
+  Intersection is(Point3f(p[0], p[1], p[2]));
+  is.ng = Normal3f(ng[0], ng[1], ng[2]);
+  Vector3f wi(0, 0, 0);
+  Float pdf = 0;
+  ShadowTester sdt;
+  Spectrum Li = l.sample_Li(is, Point2f(u[0], u[1]), &wi, &pdf, &sdt);
+  put3(out, Li); put3(out + 3, wi); out[6] = pdf;
+  if (pdf > 0 && !Li.isBlack()) put_ray(out + 7, sdt.p0.spawnRay(sdt.p1));
+}
+// out: Le(3) ray o(3) d(3) nLight(3) pdfPos pdfDir
+void sample_le(const Light &l, const float *u1, const float *u2, float *out) {
+  Ray r(Point3f(0, 0, 0), Vector3f(0, 0, 0));
+  Normal3f n(0, 0, 0);
+  Float pp = 0, pd = 0;
+  Spectrum Le = l.sample_Le(Point2f(u1[0], u1[1]), Point2f(u2[0], u2[1]), &r, &n, &pp, &pd);
+  put3(out, Le); put3(out + 3, r.o); put3(out + 6, r.d); put3(out + 9, n); out[12] = pp; out[13] = pd;
+}
+}  // namespace
+
+// loadImage (texture/imagetexture.cpp:31-49): returns channels, fills rgb (w*h*3)
+int ref_load_image(const char *path, int *w, int *h, float *rgb, int cap) {
+  int ww, hh, ch;
+  unsigned char *d = stbi_load(path, &ww, &hh, &ch, 0);
+  if (!d) return -1;
+  stbi_image_free(d);
+  auto im = loadImage(path);
+  *w = im->width; *h = im->height;
+  for (int i = 0; i < im->width * im->height && 3 * i + 2 < cap; ++i) put3(rgb + 3 * i, im->data[i]);
+  return ch;
+}
+
+// Envmap(light2world, L, 1, path) over an image file (or "" = constant L)
+void *ref_envmap_new(const char *path, const float *L, const float *m, const float *mi, const float *center,
+                     float radius) {
+  auto *e = new Envmap(xform16(m, mi), Spectrum(L[0], L[1], L[2]), 1, path ? path : "");
+  e->worldCenter = Point3f(center[0], center[1], center[2]);
+  e->worldRadius = radius;
+  return e;
+}
+void ref_envmap_free(void *e) { delete (Envmap *)e; }
+void ref_envmap_le(void *e, const float *d, float *out) {
+  put3(out, ((Envmap *)e)->Le(Ray(Point3f(0, 0, 0), v3(d))));
+}
+float ref_envmap_pdf_li(void *e, const float *wi) {
+  Intersection is(Point3f(0, 0, 0));
+  return ((Envmap *)e)->pdf_Li(is, v3(wi), no_scene());
+}
+void ref_envmap_sample_li(void *e, const float *p, const float *ng, const float *u, float *out) {
+  sample_li(*(Envmap *)e, p, ng, u, out);
+}
+void ref_envmap_sample_le(void *e, const float *u1, const float *u2, float *out) {
+  sample_le(*(Envmap *)e, u1, u2, out);
+}
+void ref_envmap_power(void *e, float *out) { put3(out, ((Envmap *)e)->power()); }
+
+// createDistantLight (distant.cpp:58-65) from L / from / to strings
+void *ref_distant_new(const char *L, const char *from, const char *to, const float *center, float radius) {
+  ParamSet ps;
+  ps.setParam("L", L);
+  ps.setParam("from", from);
+  ps.setParam("to", to);
+  auto d = createDistantLight(ps);
+  auto *l = new DistantLight(*d);
+  l->worldCenter = Point3f(center[0], center[1], center[2]);
+  l->worldRadius = radius;
+  return l;
+}
+void ref_distant_free(void *l) { delete (DistantLight *)l; }
+void ref_distant_dir(void *l, float *out) { put3(out, ((DistantLight *)l)->dir); }
+void ref_distant_sample_li(void *l, const float *p, const float *ng, const float *u, float *out) {
+  sample_li(*(DistantLight *)l, p, ng, u, out);
+}
+void ref_distant_sample_le(void *l, const float *u1, const float *u2, float *out) {
+  sample_le(*(DistantLight *)l, u1, u2, out);
+}
+void ref_distant_power(void *l, float *out) { put3(out, ((DistantLight *)l)->power()); }
+
+// createEnvMap's light-to-world / world-to-light (envmap.cpp:108-128) for a
+// <Transform> parameter list and a worldUp string (constant map: no image)
+void ref_envmap_xform(int n, const char **keys, const char **values, const char *world_up, float *out) {
+  ParamVec pv;
+  for (int i = 0; i < n; ++i) pv.addParam(keys[i], values[i]);
+  Transform t = createTransform(pv);
+  ParamSet ps;
+  ps.setParam("worldUp", world_up);
+  ps.setParam("nLightSamples", "1");
+  auto e = createEnvMap(t, ps);
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      out[4 * r + c] = e->local2world.getMatrix().m[r][c];
+      out[16 + 4 * r + c] = e->world2local.getMatrix().m[r][c];
+    }
 }
 
 }  // extern "C"

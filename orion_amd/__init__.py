@@ -96,6 +96,9 @@ _SIGS = {
     "orion_create_transform": (C.c_int, [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p),
                                          f32p, f32p]),
     "orion_light_record": (C.c_int, [C.c_int, f32p, f32p, f32p, C.c_float, C.c_float, f32p]),
+    "orion_hscene_add_envmap": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, f32p, f32p, f32p]),
+    "orion_hscene_envmap_image": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                            C.POINTER(C.c_void_p)]),
     "orion_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "orion_ctx_destroy": (C.c_int, [C.c_void_p]),
     "orion_ctx_synchronize": (C.c_int, [C.c_void_p]),
@@ -212,6 +215,31 @@ class HostScene:
         self.mats = view(ptrs[3], MAT_FLOATS * m, np.float32)
         self.lights = view(ptrs[4], 32 * l, np.float32)
         self.camera = view(ptrs[5], 16, np.float32)
+        # envmap images (light record [1] indexes them), texels before L
+        self.env_images = []
+        k = 0
+        while True:
+            w, h, px = C.c_int(), C.c_int(), C.c_void_p()
+            if lib().orion_hscene_envmap_image(self.h, k, C.byref(w), C.byref(h), C.byref(px)) != 0:
+                break
+            self.env_images.append(view(px, 3 * w.value * h.value, np.float32).reshape(h.value, w.value, 3))
+            k += 1
+
+    def add_envmap(self, L, rgb=None, light_to_world=None, world_to_light=None):
+        """Appends an environment light: rgb (H, W, 3) texels (byte / 255 values)
+        or None for the constant map L; orientation matrices default to identity."""
+        eye = np.eye(4, dtype=np.float32)
+        l2w = np.ascontiguousarray(eye if light_to_world is None else light_to_world, np.float32)
+        w2l = np.ascontiguousarray(eye if world_to_light is None else world_to_light, np.float32)
+        if rgb is None:
+            hgt, wid, px = 1, 1, None
+        else:
+            rgb = np.ascontiguousarray(rgb, np.float32)
+            hgt, wid = rgb.shape[:2]
+            px = rgb
+        _check(lib().orion_hscene_add_envmap(self.h, wid, hgt, _ptr(px), np.asarray(L, np.float32).reshape(3),
+                                             l2w, w2l))
+        self._refresh()
 
     # oracle_lib.OracleScene reads these names
     @property

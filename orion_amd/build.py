@@ -20,7 +20,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "liborion_amd.so"
 
 HIP_SOURCES = ["runtime.hip", "path_kernels.hip", "sppm_kernels.hip", "batch.hip"]
-CPP_SOURCES = ["api_host.cpp", "host_records.cpp", "ori_loader.cpp", "scenegen.cpp", "bvh.cpp"]
+CPP_SOURCES = ["api_host.cpp", "host_records.cpp", "ori_loader.cpp", "scenegen.cpp", "bvh.cpp", "png_io.cpp"]
 
 ARCH = os.environ.get("ORION_AMD_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
@@ -59,7 +59,7 @@ def build_library(verbose=False):
                 print(o)
     objs = [str(BUILD / (s + ".o")) for s in HIP_SOURCES + CPP_SOURCES]
     tmp = str(LIB) + ".tmp"
-    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lpthread"])
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lpthread", "-lz"])
     os.replace(tmp, LIB)
     # the C++ host layer's driver (orion_amd/host), linked against the library
     drv = PKG / "orion_render"

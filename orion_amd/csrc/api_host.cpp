@@ -126,13 +126,46 @@ int orion_light_record(int type, const float I[3], const float from[3], const fl
                        float rec[32]) {
   if (!I || !from || !rec) { set_error("bad argument"); return ORION_ERR_ARG; }
   if (type == LIGHT_POINT) point_light_record(I, from, rec);
-  else if (type == LIGHT_SPOT) {
-    if (!to) { set_error("spot light needs to"); return ORION_ERR_ARG; }
-    spot_light_record(I, from, to, angle, delta, rec);
+  else if (type == LIGHT_SPOT || type == LIGHT_DISTANT) {
+    if (!to) { set_error("spot and distant lights need to"); return ORION_ERR_ARG; }
+    if (type == LIGHT_SPOT) spot_light_record(I, from, to, angle, delta, rec);
+    else distant_light_record(I, from, to, rec);
   } else {
-    set_error("type must be 1 (point) or 2 (spot)");
+    set_error("type must be 1 (point), 2 (spot) or 3 (distant)");
     return ORION_ERR_ARG;
   }
+  return ORION_OK;
+}
+
+int orion_hscene_add_envmap(orion_hscene h, int width, int height, const float *rgb, const float L[3],
+                            const float light_to_world[16], const float world_to_light[16]) {
+  if (!h || !L || !light_to_world || !world_to_light || width <= 0 || height <= 0 ||
+      (!rgb && (width != 1 || height != 1)) || (int64_t)width * height > (1 << 26)) {
+    set_error("bad argument");
+    return ORION_ERR_ARG;
+  }
+  HostScene *s = reinterpret_cast<HostScene *>(h);
+  EnvImage img;
+  img.width = width;
+  img.height = height;
+  if (rgb) img.rgb.assign(rgb, rgb + 3 * (size_t)width * height);
+  else img.rgb.assign(3, 1.f);
+  img.has_xform_params = false;
+  float rec[32];
+  envmap_light_record((int)s->env.size(), L, width, height, 1, light_to_world, world_to_light, nullptr, rec);
+  s->lights.insert(s->lights.end(), rec, rec + 32);
+  s->env.push_back(std::move(img));
+  s->refresh_info();
+  return ORION_OK;
+}
+
+int orion_hscene_envmap_image(orion_hscene h, int k, int *width, int *height, const float **rgb) {
+  if (!h) { set_error("null scene"); return ORION_ERR_ARG; }
+  HostScene *s = reinterpret_cast<HostScene *>(h);
+  if (k < 0 || k >= (int)s->env.size()) { set_error("envmap image index out of range"); return ORION_ERR_ARG; }
+  if (width) *width = s->env[k].width;
+  if (height) *height = s->env[k].height;
+  if (rgb) *rgb = s->env[k].rgb.data();
   return ORION_OK;
 }
 

@@ -29,6 +29,8 @@ __device__ __forceinline__ bool is_inf(float x) { return __builtin_isinf(x); }
 // same with libm's double routines, so sample paths agree).
 __device__ __forceinline__ float dsin(float x) { return (float)sin((double)x); }
 __device__ __forceinline__ float dcos(float x) { return (float)cos((double)x); }
+__device__ __forceinline__ float dacos(float x) { return (float)acos((double)x); }
+__device__ __forceinline__ float datan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 
 struct V3 { float x, y, z; };
 __device__ __forceinline__ V3 mk(float x, float y, float z) { V3 r; r.x = x; r.y = y; r.z = z; return r; }
@@ -83,6 +85,27 @@ __device__ __forceinline__ V3 uniform_cone(float u0, float u1, float cosThetaMax
   float sinTheta = sqrtf(1.0f - cosTheta * cosTheta);
   float phi = u1 * 2 * kPi;
   return mk(dcos(phi) * sinTheta, dsin(phi) * sinTheta, cosTheta);
+}
+// sampling.cpp:35-54 (Shirley-Chiu concentric mapping)
+__device__ __forceinline__ void concentric_disk(float u0, float u1, float *dx, float *dy) {
+  float ox = 2.f * u0 - 1.0f, oy = 2.f * u1 - 1.0f;
+  if (ox == 0 && oy == 0) { *dx = 0; *dy = 0; return; }
+  float theta, r;
+  if (fabsf(ox) > fabsf(oy)) {
+    r = ox;
+    theta = 0.78539816339744830961f * (oy / ox);
+  } else {
+    r = oy;
+    theta = 1.57079632679489661923f - 0.78539816339744830961f * (ox / oy);
+  }
+  *dx = dcos(theta) * r;
+  *dy = dsin(theta) * r;
+}
+// sampling.h:60-68
+__device__ __forceinline__ float spherical_theta(V3 v) { return dacos(clampf(v.z, -1.0f, 1.0f)); }
+__device__ __forceinline__ float spherical_phi(V3 v) {
+  float p = datan2(v.y, v.x);
+  return (p < 0) ? (p + 2 * kPi) : p;
 }
 __device__ __forceinline__ float power_heuristic(float fPdf, float gPdf) {
   float f = 1 * fPdf, g = 1 * gPdf;

@@ -27,6 +27,11 @@ struct DScene {
   const float *light_func;  // n_lights
   const float *light_area;  // n_lights (area lights; 0 otherwise)
   const uint32_t *sobol;    // 1024 dims x 4 bytes x 256: Sobol byte tables
+  const float *env_tex;     // envmap texels x L, RGB
+  const float *env_dist;    // envmap Distribution2D tables
+  const int *env_meta;      // per envmap: light, width, height, texel offset, table offset
+  int n_env;                // envmap lights, in light order
+  float world_c[3], world_r;  // Scene::worldbound().boundingSphere
   float light_func_int;
   int n_lights;
   int n_prims;

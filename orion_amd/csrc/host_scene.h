@@ -4,14 +4,26 @@
 #include <cstdint>
 #include <string>
 #include <vector>
+#include <utility>
 
 #include "../../include/orion_amd.h"
 
 namespace orion_amd {
 
 enum { MAT_MATTE = 0, MAT_METAL = 1, MAT_GLASS = 2, MAT_MIRROR = 3, MAT_PLASTIC = 4, MAT_UBER = 5 };
-enum { LIGHT_AREA = 0, LIGHT_POINT = 1, LIGHT_SPOT = 2 };
+enum { LIGHT_AREA = 0, LIGHT_POINT = 1, LIGHT_SPOT = 2, LIGHT_DISTANT = 3, LIGHT_ENVMAP = 4 };
 constexpr int kTriFloats = 18, kMatFloats = ORION_MAT_FLOATS, kLightFloats = 32, kCamFloats = 16;
+
+// One environment map image: texels as loadImage returns them (byte / 255),
+// row 0 first, before the light's L is applied; plus what the .ori writer
+// needs to reproduce the <Envmap> element.
+struct EnvImage {
+  int width = 1, height = 1;
+  std::vector<float> rgb;                                   // width x height x 3
+  std::string file;                                         // image file name ("" = constant)
+  std::vector<std::pair<std::string, std::string>> xform;  // <Transform> parameters
+  bool has_xform_params = true;                             // false: matrices given directly
+};
 
 struct HostScene {
   std::vector<float> tri_pn;    // n x 18
@@ -19,6 +31,7 @@ struct HostScene {
   std::vector<int> tri_light;   // n (area light index or -1)
   std::vector<float> mats;      // m x kMatFloats
   std::vector<float> lights;    // l x 32
+  std::vector<EnvImage> env;    // images of the envmap lights (light record [1])
   float camera[kCamFloats] = {0};
   orion_scene_info info{};
   // Writer support: keeps the lookat/fov the camera was built from.
@@ -46,6 +59,35 @@ void point_light_record(const float I[3], const float from[3], float rec[32]);
 void spot_light_record(const float I[3], const float from[3], const float to[3], float angle,
                        float delta, float rec[32]);
 void area_light_record(int tri, const float Le[3], float rec[32]);
+void distant_light_record(const float L[3], const float from[3], const float to[3], float rec[32]);
+// Envmap record over image `image` (light/envmap.cpp:108-128): l2w / w2l are
+// the full light-to-world transform (worldUp rotation included).
+void envmap_light_record(int image, const float L[3], int width, int height, int n_samples, const float l2w[16],
+                         const float w2l[16], const float world_up[3], float rec[32]);
+// createEnvMap's orientation: rotate(worldUp -> +z) * transform.
+void envmap_orientation(const float xform_m[16], const float xform_minv[16], const float world_up[3],
+                        float l2w[16], float w2l[16]);
+
+// Device tables of the envmap lights, in light order (light/envmap.cpp:9-42,
+// sampling.h:127-150): texels = L x image; per envmap a marginal and one
+// conditional Distribution1D per row, laid out as
+//   [nv func][nv+1 cdf][funcInt] then per row v: [nu func][nu+1 cdf][funcInt]
+// meta per envmap: light index, width, height, texel offset, table offset.
+struct EnvTables {
+  std::vector<float> tex, dist, ave;  // ave: aveSpectrum, 3 per envmap
+  std::vector<int> meta;
+};
+bool build_env_tables(const HostScene &s, EnvTables *t, std::string *err);
+// Scene::worldbound().boundingSphere (kernel.cpp:44-50, geometry.h:551-554)
+void world_sphere(const HostScene &s, float center[3], float *radius);
+// Light::power() of every light (path.cpp:39-48 weights light choice by it);
+// infinite lights need the world radius and the envmap averages.
+void light_powers(const HostScene &s, const EnvTables &env, float radius, std::vector<float> *rgb);
+
+// 8-bit PNG (png_io.cpp): RGB / RGBA / palette / grey, non-interlaced
+bool png_read(const std::string &path, int *width, int *height, int *channels, std::vector<unsigned char> *px,
+              std::string *err);
+bool png_write_rgb8(const std::string &path, int width, int height, const unsigned char *rgb, std::string *err);
 
 // .ori loading / writing (ori_loader.cpp)
 int load_ori(const std::string &path, HostScene *out, std::string *err);

@@ -2,9 +2,10 @@
 //
 // Reads the XML schema orion's Parser understands (util/parser.cpp:45-471)
 // and resolves it into records: Model (meshHandy inline data or OBJ mesh)
-// with an optional Transform chain, a Material (matte / metal / glass,
-// reusable through name= / named=), an optional AreaLight (one light per
-// triangle), point and spot Lights, the pinhole Camera, the Integrator and
+// with an optional Transform chain, a Material (matte / metal / glass /
+// mirror / plastic / uber, reusable through name= / named=), an optional
+// AreaLight (one light per triangle), point / spot / distant Lights, Envmaps
+// (8-bit PNG images or constant), the pinhole Camera, the Integrator and
 // Sampler.  Values are parsed the way orion parses them (atof then a float
 // cast, comma-separated lists, util/strutil.cpp:26-99), so the resolved
 // floats equal the reference's.  The XML reader is a small one of this
@@ -582,13 +583,55 @@ int load_ori(const std::string &path, HostScene *out, std::string *err) {
         *err = "spot light needs from and to"; return ORION_ERR_PARSE;
       }
       spot_light_record(I, from, to, pf(ps.get("angle")), pf(ps.get("delta")), rec);
+    } else if (type == "distant") {  // distant.cpp:58-65
+      float L[3], from[3], to[3];
+      spectrum(ps.get("L"), L);
+      if (!vec3(ps.get("from"), from) || !vec3(ps.get("to"), to)) {
+        *err = "distant light needs from and to"; return ORION_ERR_PARSE;
+      }
+      distant_light_record(L, from, to, rec);
     } else {
       *err = "light type '" + type + "' is outside this build's scope";
       return ORION_ERR_UNSUPPORTED;
     }
     s.lights.insert(s.lights.end(), rec, rec + 32);
   }
-  if (root->first("Envmap")) { *err = "Envmap is outside this build's scope"; return ORION_ERR_UNSUPPORTED; }
+  // Envmaps (parser.cpp:221-242, createEnvMap envmap.cpp:108-128), after
+  // every other light
+  for (const XNode *en : root->all("Envmap")) {
+    EnvImage img;
+    img.xform = paramvec_of(en->first("Transform"));
+    Xform t;
+    if (!create_transform(img.xform, &t, err)) return ORION_ERR_PARSE;
+    ParamSet ps = params_of(en);
+    float L[3] = {1.f, 1.f, 1.f}, up[3];
+    if (ps.has("L")) spectrum(ps.get("L"), L);
+    // parseVector3f on a missing worldUp reads past an empty vector in the
+    // reference: required here
+    if (!vec3(ps.get("worldUp"), up)) { *err = "Envmap needs worldUp"; return ORION_ERR_PARSE; }
+    if (ps.has("name")) {
+      img.file = ps.get("name");
+      int w, h, ch;
+      std::vector<unsigned char> px;
+      if (!png_read(res + img.file, &w, &h, &ch, &px, err)) return ORION_ERR_IO;
+      if (ch != 3 && ch != 4) { *err = img.file + ": only 3 or 4 channel images are supported"; return ORION_ERR_UNSUPPORTED; }
+      img.width = w;
+      img.height = h;
+      img.rgb.resize((size_t)w * h * 3);
+      const float inv255 = 0.0039215686274509803921568627451f;  // imagetexture.cpp:15-20
+      for (size_t i = 0; i < (size_t)w * h; ++i)
+        for (int c = 0; c < 3; ++c) img.rgb[3 * i + c] = px[ch * i + c] * inv255;
+    } else {
+      img.rgb.assign(3, 1.f);  // ConstantTexture(L): a 1 x 1 map of L
+    }
+    float l2w[16], w2l[16];
+    envmap_orientation(&t.m.m[0][0], &t.mInv.m[0][0], up, l2w, w2l);
+    float rec[32];
+    envmap_light_record((int)s.env.size(), L, img.width, img.height, pi_(ps.get("nLightSamples")), l2w, w2l, up,
+                        rec);
+    s.lights.insert(s.lights.end(), rec, rec + 32);
+    s.env.push_back(std::move(img));
+  }
   // Sampler (parser.cpp:196-212)
   if (const XNode *sm = root->first("Sampler")) {
     std::string type = sm->attr("type") ? sm->attr("type") : "";
@@ -746,7 +789,42 @@ int write_ori(const HostScene &s, const std::string &path, int inline_limit, std
       o << "  <Light type=\"spot\"><Param intensity=\"" << f3str(r + 2) << "\"/><Param from=\""
         << f3str(r + 5) << "\"/><Param to=\"" << f3str(to) << "\"/><Param angle=\"" << fstr(ang)
         << "\"/><Param delta=\"0\"/></Light>\n";
+    } else if ((int)r[0] == LIGHT_DISTANT) {
+      o << "  <Light type=\"distant\"><Param L=\"" << f3str(r + 2) << "\"/><Param from=\"" << f3str(r + 8)
+        << "\"/><Param to=\"" << f3str(r + 11) << "\"/></Light>\n";
     }
+  }
+  for (int l = 0; l < s.n_lights(); ++l) {
+    const float *r = &s.lights[32 * (size_t)l];
+    if ((int)r[0] != LIGHT_ENVMAP) continue;
+    const EnvImage &img = s.env[(size_t)r[1]];
+    if (!img.has_xform_params) { *err = "envmap orientation given as matrices cannot be written as .ori"; return ORION_ERR_UNSUPPORTED; }
+    o << "  <Envmap>\n";
+    if (!img.xform.empty()) {
+      o << "    <Transform>";
+      for (auto &kv : img.xform) o << "<Param " << kv.first << "=\"" << kv.second << "\"/>";
+      o << "</Transform>\n";
+    }
+    o << "    <Param L=\"" << f3str(r + 2) << "\"/>\n";
+    bool constant = img.file.empty() && img.width * img.height == 1 && img.rgb[0] == 1.f && img.rgb[1] == 1.f &&
+                    img.rgb[2] == 1.f;
+    if (!constant) {
+      // texels are byte / 255 values: written back as an 8-bit PNG beside the .ori
+      std::string fn = img.file.empty() ? stem + "_env" + std::to_string(l) + ".png" : img.file;
+      size_t slash = fn.find_last_of('/');
+      std::string flat = slash == std::string::npos ? fn : fn.substr(slash + 1);
+      std::vector<unsigned char> px(img.rgb.size());
+      for (size_t i = 0; i < px.size(); ++i) {
+        float v = std::round(img.rgb[i] * 255.f);
+        px[i] = (unsigned char)(v < 0 ? 0 : (v > 255 ? 255 : v));
+      }
+      std::string dir = base.substr(0, base.size() - stem.size());
+      if (!png_write_rgb8(dir + flat, img.width, img.height, px.data(), err)) return ORION_ERR_IO;
+      o << "    <Param name=\"" << flat << "\"/>\n";
+    }
+    o << "    <Param nLightSamples=\"" << (int)r[7] << "\"/>\n";
+    o << "    <Param worldUp=\"" << f3str(r + 26) << "\"/>\n";
+    o << "  </Envmap>\n";
   }
   const orion_scene_info &in = s.info;
   if (in.integrator == 1) {

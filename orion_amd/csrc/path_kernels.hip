@@ -204,6 +204,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         Isect is;
         if (hp >= 0) make_isect(sc, hp, w.hu[p], w.hv[p], &is);
         if ((b == 0 || (fl & F_SPEC)) && hp >= 0 && is.light >= 0) acc(L, beta * area_emission(sc, is, neg(d)));
+        if ((b == 0 || (fl & F_SPEC)) && hp < 0) add_env_le(sc, d, beta, &L);  // path.cpp:65-68
         if (hp < 0 || b >= J.maxDepth) {
           fl &= ~F_ALIVE;
         } else {

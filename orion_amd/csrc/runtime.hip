@@ -214,35 +214,38 @@ int orion_scene_upload(orion_ctx c, orion_hscene hs, orion_scene *out) {
     std::memcpy(&shade[20 * (size_t)i + 18], &s.tri_mat[i], 4);
     std::memcpy(&shade[20 * (size_t)i + 19], &s.tri_light[i], 4);
   }
-  // lights: power distribution (path.cpp:39-48) and areas (triangle.cpp:77-83)
+  // lights: power distribution (path.cpp:39-48), areas (triangle.cpp:77-83),
+  // the world's bounding sphere and the envmap tables (infinite lights)
   int nl = s.n_lights();
   std::vector<float> area(std::max(nl, 1), 0.f), func(std::max(nl, 1), 0.f), cdf(nl + 1, 0.f);
+  std::vector<float> dev_lights(s.lights);
   for (int l = 0; l < nl; ++l) {
-    const float *L = &s.lights[32 * (size_t)l];
-    int type = (int)L[0];
-    float pr, pg, pb;
+    int type = (int)s.lights[32 * (size_t)l];
+    if (type < LIGHT_AREA || type > LIGHT_ENVMAP) { delete sc; set_error("unknown light type"); return ORION_ERR_ARG; }
     if (type == LIGHT_AREA) {
-      int t = (int)L[1];
+      int t = (int)s.lights[32 * (size_t)l + 1];
       if (t < 0 || t >= n) { delete sc; set_error("area light triangle out of range"); return ORION_ERR_ARG; }
-      const float *q = &s.tri_pn[18 * (size_t)t];
+    }
+  }
+  EnvTables env;
+  std::string env_err;
+  if (!build_env_tables(s, &env, &env_err)) { delete sc; set_error(env_err); return ORION_ERR_ARG; }
+  float wc[3], wr;
+  world_sphere(s, wc, &wr);
+  std::vector<float> power;
+  light_powers(s, env, wr, &power);
+  for (int l = 0, e = 0; l < nl; ++l) {
+    const float *L = &s.lights[32 * (size_t)l];
+    if ((int)L[0] == LIGHT_AREA) {
+      const float *q = &s.tri_pn[18 * (size_t)(int)L[1]];
       float e1x = q[3] - q[0], e1y = q[4] - q[1], e1z = q[5] - q[2];
       float e2x = q[6] - q[0], e2y = q[7] - q[1], e2z = q[8] - q[2];
       float cx = (e1y * e2z) - (e1z * e2y), cy = (e1z * e2x) - (e1x * e2z), cz = (e1x * e2y) - (e1y * e2x);
       area[l] = 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
-      // DiffuseAreaLight::power = Le * area * pi
-      pr = L[2] * area[l] * 3.14159265358979323846f;
-      pg = L[3] * area[l] * 3.14159265358979323846f;
-      pb = L[4] * area[l] * 3.14159265358979323846f;
-    } else if (type == LIGHT_POINT) {
-      const float k = 4 * 3.14159265358979323846f;  // PointLight::power = 4 pi I
-      pr = L[2] * k; pg = L[3] * k; pb = L[4] * k;
-    } else {  // SpotLight::power = I * 2 * pi * (1 - .5 (cosFalloffStart + cosTotal))
-      float f = (1 - .5f * (L[10] + L[9]));
-      pr = L[2] * 2 * 3.14159265358979323846f * f;
-      pg = L[3] * 2 * 3.14159265358979323846f * f;
-      pb = L[4] * 2 * 3.14159265358979323846f * f;
     }
-    func[l] = 0.212671f * pr + 0.715160f * pg + 0.072169f * pb;  // Spectrum::intensity
+    if ((int)L[0] == LIGHT_ENVMAP) dev_lights[32 * (size_t)l + 1] = (float)e++;  // device: envmap ordinal
+    const float *p = &power[3 * (size_t)l];
+    func[l] = 0.212671f * p[0] + 0.715160f * p[1] + 0.072169f * p[2];  // Spectrum::intensity
   }
   float funcInt = 0;
   if (nl > 0) {  // Distribution1D (sampling.h:76-90)
@@ -270,7 +273,10 @@ int orion_scene_upload(orion_ctx c, orion_hscene hs, orion_scene *out) {
   if ((rc = up(sc->tris, sc->bvh.tris.data(), sc->bvh.tris.size() * sizeof(BvhTri)))) return fail(rc);
   if ((rc = up(sc->shade, shade.data(), shade.size() * 4))) return fail(rc);
   if ((rc = up(sc->mats, s.mats.data(), s.mats.size() * 4))) return fail(rc);
-  if ((rc = up(sc->lights, s.lights.data(), s.lights.size() * 4))) return fail(rc);
+  if ((rc = up(sc->lights, dev_lights.data(), dev_lights.size() * 4))) return fail(rc);
+  if ((rc = up(sc->env_tex, env.tex.data(), env.tex.size() * 4))) return fail(rc);
+  if ((rc = up(sc->env_dist, env.dist.data(), env.dist.size() * 4))) return fail(rc);
+  if ((rc = up(sc->env_meta, env.meta.data(), env.meta.size() * 4))) return fail(rc);
   if ((rc = up(sc->light_cdf, cdf.data(), cdf.size() * 4))) return fail(rc);
   if ((rc = up(sc->light_func, func.data(), func.size() * 4))) return fail(rc);
   if ((rc = up(sc->light_area, area.data(), area.size() * 4))) return fail(rc);
@@ -297,6 +303,12 @@ int orion_scene_upload(orion_ctx c, orion_hscene hs, orion_scene *out) {
   d.light_area = (const float *)sc->light_area.ptr;
   d.sobol = (const uint32_t *)sc->sobol.ptr;
   d.light_func_int = funcInt;
+  d.env_tex = (const float *)sc->env_tex.ptr;
+  d.env_dist = (const float *)sc->env_dist.ptr;
+  d.env_meta = (const int *)sc->env_meta.ptr;
+  d.n_env = (int)(env.meta.size() / 5);
+  d.world_c[0] = wc[0]; d.world_c[1] = wc[1]; d.world_c[2] = wc[2];
+  d.world_r = wr;
   d.n_lights = nl;
   d.n_prims = n;
   d.stack_bound = sc->bvh.stack_bound;

@@ -14,6 +14,10 @@
 //              plastic and mirror boxes, an uber sphere (all five lobes), a
 //              rough glass sphere and a gold sphere; ceiling light + point
 //              light; 128^2, 16 spp, depth 6
+//   sky        spheres (mirror, glass, uber, plastic) on an Oren-Nayar ground
+//              under a 64 x 32 sky envmap (sun disc, worldUp +y, turned 30
+//              degrees about its up axis) and a distant sun; 128^2, 16 spp,
+//              depth 6
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -26,6 +30,8 @@
 #include "host_scene.h"
 
 namespace orion_amd {
+// createTransform (ori_loader.cpp)
+bool create_transform(const std::vector<std::pair<std::string, std::string>> &pv, Xform *out, std::string *err);
 namespace {
 
 struct Rng {  // splitmix64
@@ -188,6 +194,72 @@ void gen_materials(HostScene *s, int W, int H) {
   point_light_record(I, from, lrec);
   s->lights.insert(s->lights.end(), lrec, lrec + 32);
   const float la[9] = {0.5f, 0.5f, 2.4f, 0.5f, 0.5f, 0.0f, 0.f, 1.f, 0.f};
+  set_camera(s, la, 40.f, W, H);
+  s->info.integrator = 0;
+  s->info.spp = 16;
+  s->info.max_depth = 6;
+}
+
+// ---------------------------------------------------------------- sky
+void gen_sky(HostScene *s, int W, int H) {
+  Builder b{s};
+  float rec[kMatFloats];
+  matte_record(0.5f, 0.45f, 0.4f, rec, 10.f);
+  int mground = b.material(rec);
+  const float kr[3] = {0.95f, 0.95f, 0.95f};
+  mirror_record(kr, rec);
+  int mmirror = b.material(rec);
+  const float gk[3] = {1.f, 1.f, 1.f};
+  glass_record(gk, gk, 0.f, 1.5f, rec);
+  int mglass = b.material(rec);
+  const float ukd[3] = {0.6f, 0.3f, 0.2f}, uks[3] = {0.3f, 0.3f, 0.3f}, ukr[3] = {0.2f, 0.2f, 0.2f},
+              ukt[3] = {0.3f, 0.3f, 0.3f}, uop[3] = {0.8f, 0.8f, 0.8f};
+  uber_record(ukd, uks, ukr, ukt, uop, 0.15f, 1.4f, rec);
+  int muber = b.material(rec);
+  const float pkd[3] = {0.1f, 0.5f, 0.2f}, pks[3] = {0.5f, 0.5f, 0.5f};
+  plastic_record(pkd, pks, 0.05f, rec);
+  int mplastic = b.material(rec);
+  b.quad({-4, 0, -4}, {4, 0, -4}, {4, 0, 4}, {-4, 0, 4}, {0, 1, 0}, mground, nullptr);
+  add_sphere(b, 3, {-1.1, 0.5, 0.0}, 0.5, 0.0, 0, std::vector<int>(1, mmirror));
+  add_sphere(b, 3, {0.0, 0.5, 0.6}, 0.5, 0.0, 0, std::vector<int>(1, mglass));
+  add_sphere(b, 3, {1.1, 0.5, 0.0}, 0.5, 0.0, 0, std::vector<int>(1, muber));
+  add_sphere(b, 2, {0.4, 0.25, 1.45}, 0.25, 0.0, 0, std::vector<int>(1, mplastic));
+  // distant sun
+  const float sunL[3] = {2.0f, 1.9f, 1.7f}, from[3] = {0.f, 0.f, 0.f}, to[3] = {0.4f, 1.f, 0.3f};
+  float lrec[32];
+  distant_light_record(sunL, from, to, lrec);
+  s->lights.insert(s->lights.end(), lrec, lrec + 32);
+  // envmap: 8-bit texels (byte / 255), theta from the map's +z (= worldUp)
+  EnvImage img;
+  img.width = 64;
+  img.height = 32;
+  img.file = "";
+  img.xform = {{"rotate", "0,0,1,30"}};
+  img.rgb.resize(64 * 32 * 3);
+  const float inv255 = 0.0039215686274509803921568627451f;
+  for (int j = 0; j < 32; ++j)
+    for (int i = 0; i < 64; ++i) {
+      double u = (i + 0.5) / 64, v = (j + 0.5) / 32;
+      double c[3];
+      if (v < 0.5) {
+        c[0] = 0.3 + 0.4 * (1 - 2 * v); c[1] = 0.5 + 0.3 * (1 - 2 * v); c[2] = 0.9;
+      } else {
+        c[0] = 0.25; c[1] = 0.2; c[2] = 0.15;
+      }
+      double du = u - 0.3, dv = v - 0.22;
+      if (du * du + dv * dv < 0.05 * 0.05) c[0] = c[1] = c[2] = 1.0;
+      for (int k = 0; k < 3; ++k) img.rgb[3 * (j * 64 + i) + k] = (float)(int)std::lround(c[k] * 255) * inv255;
+    }
+  Xform t;
+  std::string err;
+  create_transform(img.xform, &t, &err);
+  const float up[3] = {0.f, 1.f, 0.f}, envL[3] = {1.5f, 1.5f, 1.5f};
+  float l2w[16], w2l[16];
+  envmap_orientation(&t.m.m[0][0], &t.mInv.m[0][0], up, l2w, w2l);
+  envmap_light_record((int)s->env.size(), envL, 64, 32, 1, l2w, w2l, up, lrec);
+  s->lights.insert(s->lights.end(), lrec, lrec + 32);
+  s->env.push_back(std::move(img));
+  const float la[9] = {0.f, 1.2f, 4.0f, 0.f, 0.5f, 0.f, 0.f, 1.f, 0.f};
   set_camera(s, la, 40.f, W, H);
   s->info.integrator = 0;
   s->info.spp = 16;
@@ -410,6 +482,8 @@ int generate_scene(const std::string &name, uint64_t seed, const double *params,
     int L = (int)P(0, 6);
     if (L < 0 || L > 9) { *err = "caustics level must be in [0, 9]"; return ORION_ERR_ARG; }
     gen_caustics(&s, L, (int)P(1, 512), (int)P(2, 512));
+  } else if (name == "sky") {
+    gen_sky(&s, (int)P(0, 128), (int)P(1, 128));
   } else if (name == "materials") {
     gen_materials(&s, (int)P(0, 128), (int)P(1, 128));
   } else if (name == "batch_job") {

@@ -64,6 +64,118 @@ __device__ __forceinline__ void tri_sample(const DScene &sc, int prim, float u0,
   *ng = dot(g, ns) < 0.f ? neg(g) : g;
 }
 
+// ---------------------------------------------------------------- infinite lights
+constexpr float kInv2Pi = 0.15915494309189533577f;
+enum { LT_AREA = 0, LT_POINT = 1, LT_SPOT = 2, LT_DISTANT = 3, LT_ENVMAP = 4 };
+
+// Envmap e's image and Distribution2D tables (light/envmap.cpp:9-42,
+// sampling.h:127-150).  Table layout: marginal [nv func][nv+1 cdf][funcInt],
+// then per row v: [nu func][nu+1 cdf][funcInt].
+struct EnvView {
+  int nu, nv;
+  const float *tex;
+  const float *d;
+  __device__ __forceinline__ const float *row(int v) const { return d + 2 * nv + 2 + (size_t)v * (2 * nu + 2); }
+};
+__device__ __forceinline__ EnvView env_view(const DScene &sc, int e) {
+  const int *m = sc.env_meta + 5 * e;
+  EnvView v;
+  v.nu = gld(m + 1);
+  v.nv = gld(m + 2);
+  v.tex = sc.env_tex + gld(m + 3);
+  v.d = sc.env_dist + gld(m + 4);
+  return v;
+}
+// Texture::sample(float, float) then ImageTexture's wrap filter
+// (texture.cpp:4-8, 13-24).  u, v lie in [0, 1] here; a negative multiple of
+// the width wraps to 0 (the reference would index one past the row).
+__device__ __forceinline__ Sp env_lookup(const EnvView &v, float u, float w) {
+  int x = (int)(u * v.nu), y = (int)(w * v.nv);
+  x %= v.nu;
+  if (x < 0) x += v.nu;
+  y %= v.nv;
+  if (y < 0) y += v.nv;
+  const float *t = v.tex + 3 * ((size_t)y * v.nu + x);
+  return sp3(gld(t), gld(t + 1), gld(t + 2));
+}
+// Distribution1D::sampleContinuous (sampling.h:91-109) over [func][cdf][funcInt]
+__device__ inline float dist_sample(const float *d, int n, float u, float *pdf, int *off) {
+  const float *cdf = d + n;
+  int size = n + 1, first = 0, len = size;
+  while (len > 0) {
+    int half = len >> 1, mid = first + half;
+    if (gld(cdf + mid) <= u) { first = mid + 1; len -= half + 1; }
+    else len = half;
+  }
+  int o = first - 1;
+  o = o < 0 ? 0 : (o > size - 2 ? size - 2 : o);
+  *off = o;
+  float c0 = gld(cdf + o), c1 = gld(cdf + o + 1);
+  float du = u - c0;
+  if ((c1 - c0) > 0) du /= (c1 - c0);
+  float fint = gld(d + 2 * n + 1);
+  *pdf = (fint > 0) ? gld(d + o) / fint : 0;
+  return (o + du) / n;
+}
+// Distribution2D::SampleContinuous / Pdf (sampling.h:131-145)
+__device__ inline void env_sample_uv(const EnvView &v, float u0, float u1, float *su, float *sv, float *pdf) {
+  float p1, p0;
+  int row, c_;
+  *sv = dist_sample(v.d, v.nv, u1, &p1, &row);
+  *su = dist_sample(v.row(row), v.nu, u0, &p0, &c_);
+  *pdf = p0 * p1;
+}
+__device__ inline float env_uv_pdf(const EnvView &v, float pu, float pv) {
+  int iu = (int)(pu * v.nu), iv = (int)(pv * v.nv);
+  iu = iu < 0 ? 0 : (iu > v.nu - 1 ? v.nu - 1 : iu);
+  iv = iv < 0 ? 0 : (iv > v.nv - 1 ? v.nv - 1 : iv);
+  return gld(v.row(iv) + iu) / gld(v.d + 2 * v.nv + 1);
+}
+// The envmap routines are kept out of line: their FP64 acos / atan2 / sin /
+// cos would otherwise be inlined at every call site of the shading kernels
+// and push the kernels' register allocation into scratch.
+// Envmap::Le (envmap.cpp:44-49); L = the light record (world-to-light at [17])
+__device__ __noinline__ Sp env_le(const DScene &sc, const float *L, V3 dir) {
+  EnvView v = env_view(sc, (int)gld(L + 1));
+  V3 w = normalize(mat3(L + 17, dir));
+  float u = spherical_phi(w) * kInv2Pi, t = spherical_theta(w) * kInvPi;
+  return env_lookup(v, u, t);
+}
+// Envmap::pdf_Li (envmap.cpp:74-82): phi and theta both scaled by 1 / (2 pi)
+// as in the reference, and its zero-sine test compares a function address
+// (never true), so the division is always taken.
+__device__ __noinline__ float env_pdf_li(const DScene &sc, const float *L, V3 wi) {
+  EnvView v = env_view(sc, (int)gld(L + 1));
+  V3 swi = mat3(L + 17, wi);
+  float theta = spherical_theta(swi), phi = spherical_phi(swi);
+  float sinTh = dsin(theta);
+  return env_uv_pdf(v, phi * kInv2Pi, theta * kInv2Pi) / (2 * kPi * kPi * sinTh);
+}
+// Sum of Le over the scene's envmaps in light order (path.cpp:65-68,
+// sppm.cpp:95-99), added to L one envmap at a time.
+__device__ inline void add_env_le(const DScene &sc, V3 dir, Sp beta, Sp *L) {
+  for (int e = 0; e < sc.n_env; ++e) {
+    const float *LR = sc.lights + 32 * (size_t)gld(sc.env_meta + 5 * e);
+    acc(*L, beta * env_le(sc, LR, dir));
+  }
+}
+
+// Envmap::sample_Li's direction, pdf and radiance (envmap.cpp:55-73); *pdf = 0
+// where the map pdf is 0
+__device__ __noinline__ Sp env_sample_li(const DScene &sc, const float *L, float u0, float u1, V3 *wi, float *pdf) {
+  EnvView v = env_view(sc, (int)gld(L + 1));
+  float su, sv, mapPdf;
+  env_sample_uv(v, u0, u1, &su, &sv, &mapPdf);
+  if (mapPdf == 0) { *pdf = 0; return sp(0); }
+  float theta = sv * kPi, phi = su * 2 * kPi;
+  float cosTheta = dcos(theta), sinTheta = dsin(theta);
+  float sinPhi = dsin(phi), cosPhi = dcos(phi);
+  *wi = mat3(L + 8, mk(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta));
+  *pdf = mapPdf / (2 * kPi * kPi * sinTheta);
+  if (sinTheta == 0) *pdf = 0;
+  return env_lookup(v, su, sv);
+}
+
 // Light::sample_Li; *so / *sd = the ShadowTester ray (intersection.h:40-44)
 __device__ inline Sp light_sample_li(const DScene &sc, int l, V3 refp, V3 refng, float u0, float u1, V3 *wi,
                                      float *pdf, V3 *so, V3 *sd) {
@@ -87,6 +199,14 @@ __device__ inline Sp light_sample_li(const DScene &sc, int l, V3 refp, V3 refng,
     *wi = normalize(sub(p, refp));
     target = p;
     Le = dot(ng, neg(*wi)) > 0 ? Le : sp(0);
+  } else if (type == LT_DISTANT) {  // distant.cpp:16-27
+    *wi = mk(gld(L + 5), gld(L + 6), gld(L + 7));
+    *pdf = 1;
+    target = add(refp, mul(*wi, 2 * sc.world_r));
+  } else if (type == LT_ENVMAP) {
+    Le = env_sample_li(sc, L, u0, u1, wi, pdf);
+    if (*pdf == 0) return sp(0);
+    target = add(refp, mul(*wi, 2 * sc.world_r));
   } else {  // point.cpp:10-19, spot.cpp:20-29
     V3 pos = mk(gld(L + 5), gld(L + 6), gld(L + 7));
     *wi = normalize(sub(pos, refp));
@@ -120,6 +240,39 @@ __device__ inline Sp light_sample_le(const DScene &sc, int l, float a0, float a1
     *rd = w;
     return dot(ng, w) > 0 ? Le : sp(0);
   }
+  if (type == LT_DISTANT || type == LT_ENVMAP) {
+    V3 dir;
+    float cx, cy, mapPdf = 1, sinTheta = 1, su = 0, sv = 0;
+    EnvView v;
+    if (type == LT_DISTANT) {  // distant.cpp:29-43
+      dir = mk(gld(L + 5), gld(L + 6), gld(L + 7));
+      concentric_disk(a0, a1, &cx, &cy);
+    } else {  // envmap.cpp:83-107
+      v = env_view(sc, (int)gld(L + 1));
+      env_sample_uv(v, a0, a1, &su, &sv, &mapPdf);
+      if (mapPdf == 0) { *pdfPos = 0; *pdfDir = 0; return sp(0); }
+      float theta = sv * kPi, phi = su * 2.f * kPi;
+      float cosTheta = dcos(theta);
+      sinTheta = dsin(theta);
+      float sinPhi = dsin(phi), cosPhi = dcos(phi);
+      dir = mat3(L + 8, mk(sinTheta * cosPhi, sinTheta * sinPhi, cosTheta));  // -d
+      concentric_disk(b0, b1, &cx, &cy);
+    }
+    V3 v1, v2;
+    coordinate_system(dir, &v1, &v2);
+    const float r = sc.world_r;
+    V3 pDisk = add(mk(sc.world_c[0], sc.world_c[1], sc.world_c[2]), mul(add(mul(v1, cx), mul(v2, cy)), r));
+    *ro = add(pDisk, mul(dir, r));
+    *rd = neg(dir);
+    *nLight = *rd;
+    *pdfPos = 1 / (kPi * r * r);
+    if (type == LT_DISTANT) {
+      *pdfDir = 1;
+      return Le;
+    }
+    *pdfDir = sinTheta == 0 ? 0 : mapPdf / (2 * kPi * kPi * sinTheta);
+    return env_lookup(v, su, sv);
+  }
   V3 pos = mk(gld(L + 5), gld(L + 6), gld(L + 7));
   *ro = pos;
   *pdfPos = 1;
@@ -138,7 +291,8 @@ __device__ inline Sp light_sample_le(const DScene &sc, int l, float a0, float a1
 
 // Second half of estimateDirect: adds the pending light-sampled term (when
 // its shadow ray was unoccluded) and the BSDF-sampled term (when its MIS ray
-// hit the sampled light's triangle) to L, as L += beta * (Ld / lightPdf).
+// hit the sampled light's triangle, or for an envmap escaped the scene) to L,
+// as L += beta * (Ld / lightPdf).
 __device__ inline void resolve_direct(const DScene &sc, const Wave &w, int p, int fl, Sp *L) {
   const int N = w.N;
   Sp Ld = sp(0);
@@ -147,7 +301,17 @@ __device__ inline void resolve_direct(const DScene &sc, const Wave &w, int p, in
     int hp = w.hprim[N + p];
     int l = w.nlight[p];
     const float *LR = sc.lights + 32 * (size_t)l;
-    if (hp >= 0 && hp == (int)gld(LR + 1)) {
+    if ((int)gld(LR + 0) == LT_ENVMAP) {
+      // integrator.cpp:66-88: pdf_Li is analytic; a miss sees Envmap::Le, a
+      // hit on any triangle is not this light
+      if (hp < 0) {
+        V3 wi = mk(w.dx[N + p], w.dy[N + p], w.dz[N + p]);
+        float bp = w.bpdf[p];
+        float weight = power_heuristic(bp, env_pdf_li(sc, LR, wi));
+        Sp Lm = env_le(sc, LR, wi);
+        if (!black(Lm)) acc(Ld, sp3(w.fr[p], w.fg[p], w.fb[p]) * Lm * weight / bp);
+      }
+    } else if (hp >= 0 && hp == (int)gld(LR + 1)) {
       // Triangle::pdf (triangle.cpp:123-135) on the MIS ray's hit
       Isect in;
       make_isect(sc, hp, w.hu[N + p], w.hv[N + p], &in);
@@ -190,7 +354,8 @@ __device__ inline int setup_direct(const DScene &sc, const Wave &w, int p, const
   V3 wi = mk(0, 0, 0), so, sdir;
   float lpdf = 0;
   Sp Li = light_sample_li(sc, l, is.p, is.ng, lu0, lu1, &wi, &lpdf, &so, &sdir);
-  bool delta = (int)gld(sc.lights + 32 * (size_t)l) != 0;
+  const int ltype = (int)gld(sc.lights + 32 * (size_t)l);
+  bool delta = ltype != LT_AREA && ltype != LT_ENVMAP;  // isDeltaLight (light.h:39-42)
   bool hs = false, hm = false;
   if (lpdf > 0 && !black(Li)) {
     Sp f = bsdf_f(bs, wi, wo, nonspec) * absdot(wi, is.ns);
@@ -213,7 +378,11 @@ __device__ inline int setup_direct(const DScene &sc, const Wave &w, int p, const
     V3 wi2 = wi;
     Sp f2 = bsdf_sample(bs, &wi2, wo, bu0, bu1, &bp2, nonspec, &st);
     mulby(f2, sp(absdot(wi2, is.ns)));
-    if (!black(f2) && bp2 > 0) {
+    // an envmap's pdf_Li needs no ray: a zero ends the estimate before it
+    bool traced = !black(f2) && bp2 > 0;
+    if (traced && ltype == LT_ENVMAP && !(st & SPEC) && env_pdf_li(sc, sc.lights + 32 * (size_t)l, wi2) == 0)
+      traced = false;
+    if (traced) {
       V3 o2 = offset_ray_origin(is.p, is.ng, wi2);
       int s1 = N + p;
       w.ox[s1] = o2.x; w.oy[s1] = o2.y; w.oz[s1] = o2.z;

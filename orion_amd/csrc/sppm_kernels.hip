@@ -202,6 +202,8 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, in
           }
           w.br[p] = beta.r; w.bg[p] = beta.g; w.bb[p] = beta.b;
           w.dim[p] = dim;
+        } else {  // escaped: every envmap's Le at any depth (sppm.cpp:95-99)
+          add_env_le(sc, mk(w.dx[p], w.dy[p], w.dz[p]), sp3(w.br[p], w.bg[p], w.bb[p]), &L);
         }
       }
       w.Lr[p] = L.r; w.Lg[p] = L.g; w.Lb[p] = L.b;

@@ -26,7 +26,7 @@ class Oracle:
         L = C.CDLL(str(ORACLE_SO))
         L.oracle_scene_create.restype = C.c_void_p
         L.oracle_scene_create.argtypes = [C.c_int, f32p, i32p, i32p, C.c_int, f32p,
-                                          C.c_int, f32p, f32p]
+                                          C.c_int, f32p, f32p, C.c_int, C.c_void_p, C.c_void_p]
         L.oracle_scene_destroy.argtypes = [C.c_void_p]
         L.oracle_intersect.argtypes = [C.c_void_p, C.c_int, f32p, C.c_int, C.c_int,
                                        f32p, i32p, f32p, f32p, i32p]
@@ -46,6 +46,11 @@ class Oracle:
                                          C.c_int, f32p, f32p, f32p, i32p]
         L.oracle_light_sample_li.argtypes = [C.c_void_p, C.c_int, f32p, f32p, f32p, f32p]
         L.oracle_light_sample_le.argtypes = [C.c_void_p, C.c_int, f32p, f32p, f32p]
+        L.oracle_env_le.argtypes = [C.c_void_p, C.c_int, f32p, f32p]
+        L.oracle_env_pdf_li.restype = C.c_float
+        L.oracle_env_pdf_li.argtypes = [C.c_void_p, C.c_int, f32p]
+        L.oracle_light_power.argtypes = [C.c_void_p, C.c_int, f32p]
+        L.oracle_world_sphere.argtypes = [C.c_void_p, f32p]
         L.oracle_sin.restype = C.c_float
         L.oracle_sin.argtypes = [C.c_float]
         L.oracle_cos.restype = C.c_float
@@ -62,9 +67,15 @@ class OracleScene:
     def __init__(self, orc, rec):
         self.o = orc
         self.rec = rec
+        imgs = [np.ascontiguousarray(im, np.float32) for im in getattr(rec, "env_images", [])]
+        self._imgs = imgs  # kept alive for the oracle's copy
+        wh = np.array([[im.shape[1], im.shape[0]] for im in imgs] or [[0, 0]], np.int32)
+        ptrs = (C.c_void_p * max(len(imgs), 1))(*[im.ctypes.data for im in imgs])
         self.h = orc.L.oracle_scene_create(
             rec.n_tris, rec.tri_pn, rec.tri_mat, rec.tri_light, rec.n_mats, rec.mats,
-            rec.n_lights, rec.lights, rec.camera)
+            rec.n_lights, rec.lights, rec.camera, len(imgs), wh.ctypes.data, C.cast(ptrs, C.c_void_p))
+        if not self.h:
+            raise ValueError("oracle scene: bad envmap image index")
 
     def __del__(self):
         try:
@@ -149,6 +160,24 @@ class OracleScene:
                                         np.asarray(u2, np.float32), out)
         return out
 
+    def env_le(self, l, d):
+        out = np.zeros(3, np.float32)
+        self.o.L.oracle_env_le(self.h, l, np.asarray(d, np.float32), out)
+        return out
+
+    def env_pdf_li(self, l, wi):
+        return self.o.L.oracle_env_pdf_li(self.h, l, np.asarray(wi, np.float32))
+
+    def light_power(self, l):
+        out = np.zeros(3, np.float32)
+        self.o.L.oracle_light_power(self.h, l, out)
+        return out
+
+    def world_sphere(self):
+        out = np.zeros(4, np.float32)
+        self.o.L.oracle_world_sphere(self.h, out)
+        return out[:3], out[3]
+
 
 class Ref:
     """The reference's own compiled sources (oracle/_ref)."""
@@ -182,7 +211,32 @@ class Ref:
         L.ref_light_sample_le.argtypes = [C.c_int, f32p, f32p, f32p, C.c_float, C.c_float,
                                           f32p, f32p, f32p]
         L.ref_spawn.argtypes = [f32p, f32p, f32p, f32p]
+        L.ref_load_image.restype = C.c_int
+        L.ref_load_image.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), f32p, C.c_int]
+        L.ref_envmap_new.restype = C.c_void_p
+        L.ref_envmap_new.argtypes = [C.c_char_p, f32p, f32p, f32p, f32p, C.c_float]
+        L.ref_envmap_free.argtypes = [C.c_void_p]
+        L.ref_envmap_le.argtypes = [C.c_void_p, f32p, f32p]
+        L.ref_envmap_pdf_li.restype = C.c_float
+        L.ref_envmap_pdf_li.argtypes = [C.c_void_p, f32p]
+        L.ref_envmap_sample_li.argtypes = [C.c_void_p, f32p, f32p, f32p, f32p]
+        L.ref_envmap_sample_le.argtypes = [C.c_void_p, f32p, f32p, f32p]
+        L.ref_envmap_power.argtypes = [C.c_void_p, f32p]
+        L.ref_distant_new.restype = C.c_void_p
+        L.ref_distant_new.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, f32p, C.c_float]
+        L.ref_distant_free.argtypes = [C.c_void_p]
+        L.ref_distant_dir.argtypes = [C.c_void_p, f32p]
+        L.ref_distant_sample_li.argtypes = [C.c_void_p, f32p, f32p, f32p, f32p]
+        L.ref_distant_sample_le.argtypes = [C.c_void_p, f32p, f32p, f32p]
+        L.ref_distant_power.argtypes = [C.c_void_p, f32p]
+        L.ref_envmap_xform.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), C.c_char_p, f32p]
         self.L = L
+
+    def load_image(self, path, cap=1 << 22):
+        w, h = C.c_int(), C.c_int()
+        rgb = np.zeros(cap, np.float32)
+        ch = self.L.ref_load_image(str(path).encode(), C.byref(w), C.byref(h), rgb, cap)
+        return ch, rgb[:3 * w.value * h.value].reshape(h.value, w.value, 3)
 
     @staticmethod
     def _f(a, n=None):

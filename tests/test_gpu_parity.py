@@ -138,6 +138,36 @@ def test_path_all_materials(device, oracle):
     assert mean_err <= MEAN_TOL
 
 
+def test_path_sky(device, oracle):
+    """Envmap (image, rotated, worldUp +y) + distant light: escaped camera and
+    specular rays see the map, MIS over the map, the sun as a delta light."""
+    hs = oa.HostScene.generate("sky")
+    ds = oa.DeviceScene(device, hs)
+    ds.film_clear()
+    ds.render_path(0, 16, 6)
+    rgb, cnt = ds.film()
+    ref, rcnt, _ = oracle.scene(hs).render_path(16, 6)
+    frac, mean_err, bits = image_compare(rgb / cnt[..., None], ref / rcnt[..., None])
+    print(f"sky: within 1e-3 {frac:.6f}, mean rel err {mean_err:.2e}, bit-equal {bits:.6f}")
+    assert frac >= PIX_FRAC
+    assert mean_err <= MEAN_TOL
+
+
+def test_path_constant_envmap(device, oracle):
+    """The cornell box opened to a constant envmap (uniform map pdf)."""
+    hs = oa.HostScene.generate("cornell", params=[64, 64])
+    hs.add_envmap([0.3, 0.4, 0.5])
+    ds = oa.DeviceScene(device, hs)
+    ds.film_clear()
+    ds.render_path(0, 8, 5)
+    rgb, cnt = ds.film()
+    ref, rcnt, _ = oracle.scene(hs).render_path(8, 5)
+    frac, mean_err, bits = image_compare(rgb / cnt[..., None], ref / rcnt[..., None])
+    print(f"cornell+env: within 1e-3 {frac:.6f}, mean rel err {mean_err:.2e}, bit-equal {bits:.6f}")
+    assert frac >= PIX_FRAC
+    assert mean_err <= MEAN_TOL
+
+
 def test_path_waves_equal_single(device):
     """Splitting the sample range into several wavefronts must not change a bit."""
     hs = oa.HostScene.generate("cornell", params=[64, 64])

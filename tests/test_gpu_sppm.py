@@ -57,3 +57,9 @@ def test_sppm_all_materials(device, oracle):
     """Every material kind (Oren-Nayar, plastic, mirror, uber, rough glass,
     gold) under photon transport mode, area + point lights."""
     _compare(device, oracle, [96, 96], 4, 1 << 16, 0.02, 0, 3, scene="materials", depth=6)
+
+
+def test_sppm_sky(device, oracle):
+    """Photons from an envmap and a distant light (world-disk emission), the
+    camera pass adding the map on escape at any depth."""
+    _compare(device, oracle, [64, 64], 3, 1 << 16, 0.05, 0, 2, scene="sky", depth=6)

@@ -75,7 +75,7 @@ def test_camera_record_matches_reference(ref):
 
 @pytest.mark.parametrize("scene,params,limit", [("cornell", None, 1000), ("icosphere", [3, 64, 48], 100),
                                                 ("caustics", [2, 40, 30], 10 ** 9),
-                                                ("materials", [32, 32], 1000)])
+                                                ("materials", [32, 32], 1000), ("sky", [32, 32], 1000)])
 def test_ori_round_trip(tmp_path, scene, params, limit):
     hs = oa.HostScene.generate(scene, params=params)
     path = tmp_path / "scene.ori"
@@ -91,6 +91,9 @@ def test_ori_round_trip(tmp_path, scene, params, limit):
     mb = back.mats.reshape(-1, oa.MAT_FLOATS)[back.tri_mat]
     np.testing.assert_array_equal(ma.view(np.int32), mb.view(np.int32))
     assert back.info.integrator == hs.info.integrator and back.info.max_depth == hs.info.max_depth
+    assert len(back.env_images) == len(hs.env_images)
+    for a, b in zip(hs.env_images, back.env_images):
+        np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32))
 
 
 def test_loader_schema_features(tmp_path):

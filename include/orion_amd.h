@@ -17,6 +17,9 @@
  *   orion_render_sppm         replaces SPPM::render (integrator/sppm.cpp:58-322)
  *   orion_film_read           replaces RenderTarget::getAveraged's inputs
  *                             (texture/rendertarget.cpp:22-37): sum + count
+ *   orion_film_read_averaged  replaces RenderTarget::getAveraged
+ *   orion_film_write          replaces saveAsBmp (texture/texture.cpp:77-137),
+ *                             plus FP32 PFM / EXR
  *   orion_batch_* / orion_server_*  no counterpart in orion (batched launch and
  *                             continuous-batching serving of many render jobs)
  *
@@ -209,6 +212,22 @@ int orion_render_path(orion_scene s, int k0, int k1, int max_depth, const orion_
 int orion_film_clear(orion_scene s);
 /* rgb_sum: W*H*3 floats, count: W*H ints (host pointers). */
 int orion_film_read(orion_scene s, float *rgb_sum, int *count);
+/* RenderTarget::getAveraged (texture/rendertarget.cpp:28-37), computed on the
+ * device: rgb = sum / count per pixel (W*H*3 floats, host pointer; a pixel
+ * with count 0 divides by zero, as in orion). */
+int orion_film_read_averaged(orion_scene s, float *rgb);
+/* getAveraged of a host (sum, count) film, e.g. a batched or served job's. */
+int orion_film_average(int width, int height, const float *rgb_sum, const int *count,
+                       float *rgb);
+/* Writes an averaged film (W*H*3 floats, row 0 = top):
+ *   ORION_IMAGE_BMP  saveAsBmp (texture/texture.cpp:77-137): 32-bit BGRA,
+ *                    channels (char)(clamp(v, 0, 1) * 255), 54-byte header
+ *   ORION_IMAGE_PFM  FP32 RGB portable float map
+ *   ORION_IMAGE_EXR  OpenEXR scanline, uncompressed FLOAT R, G, B */
+#define ORION_IMAGE_BMP 0
+#define ORION_IMAGE_PFM 1
+#define ORION_IMAGE_EXR 2
+int orion_film_write(const char *path, int format, int width, int height, const float *rgb);
 
 /* SPPM::render.  hash_size <= 0 -> nPixels (the reference's choice).
  * Optional bit-exact debug outputs (host pointers, may be NULL) for iteration

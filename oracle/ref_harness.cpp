@@ -53,6 +53,7 @@
 #include <light/distant.h>
 #include <light/envmap.h>
 #include <texture/imagetexture.h>
+#include <texture/rendertarget.h>
 #include <bsdf/microfacet.h>
 #include <util/param.h>
 #include <util/strutil.h>
@@ -411,6 +412,19 @@ void ref_distant_sample_le(void *l, const float *u1, const float *u2, float *out
   sample_le(*(DistantLight *)l, u1, u2, out);
 }
 void ref_distant_power(void *l, float *out) { put3(out, ((DistantLight *)l)->power()); }
+
+// RenderTarget with the given sums and counts -> getAveraged (avg, W*H*3)
+// -> saveAsBmp(path) (texture/rendertarget.cpp:28-37, core/texture.cpp:77-137)
+int ref_film_average_bmp(int W, int H, const float *sum, const int *count, float *avg, const char *path) {
+  RenderTarget rt(W, H);
+  for (int i = 0; i < W * H; ++i) {
+    rt.data[i] = Spectrum(sum[3 * i], sum[3 * i + 1], sum[3 * i + 2]);
+    rt.counter[i] = count[i];
+  }
+  auto a = rt.getAveraged();
+  for (int i = 0; i < W * H; ++i) put3(avg + 3 * i, a->data[i]);
+  return path ? (saveAsBmp(a.get(), path) ? 0 : -1) : 0;
+}
 
 // createEnvMap's light-to-world / world-to-light (envmap.cpp:108-128) for a
 // <Transform> parameter list and a worldUp string (constant map: no image)

@@ -112,6 +112,9 @@ _SIGS = {
                                     C.POINTER(RenderStats)]),
     "orion_film_clear": (C.c_int, [C.c_void_p]),
     "orion_film_read": (C.c_int, [C.c_void_p, f32p, i32p]),
+    "orion_film_read_averaged": (C.c_int, [C.c_void_p, f32p]),
+    "orion_film_average": (C.c_int, [C.c_int, C.c_int, f32p, i32p, f32p]),
+    "orion_film_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, f32p]),
     "orion_render_sppm": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
                                     C.POINTER(SppmDebug), C.POINTER(RenderStats)]),
     "orion_render_batch": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(JobDesc),
@@ -280,6 +283,28 @@ class HostScene:
             pass
 
 
+IMAGE_FORMATS = {"bmp": 0, "pfm": 1, "exr": 2}
+
+
+def film_average(rgb_sum, count):
+    """RenderTarget::getAveraged of a host (sum, count) film."""
+    rgb_sum = np.ascontiguousarray(rgb_sum, np.float32)
+    count = np.ascontiguousarray(count, np.int32)
+    h, w = count.shape
+    out = np.zeros((h, w, 3), np.float32)
+    _check(lib().orion_film_average(w, h, rgb_sum, count, out))
+    return out
+
+
+def write_image(path, rgb, fmt=None):
+    """Writes an averaged film (H, W, 3) as BMP (saveAsBmp), PFM or EXR; the
+    format defaults to the file extension."""
+    fmt = fmt or str(path).rsplit(".", 1)[-1].lower()
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    h, w = rgb.shape[:2]
+    _check(lib().orion_film_write(str(path).encode(), IMAGE_FORMATS[fmt], w, h, rgb))
+
+
 def camera_record(lookat, fov, width, height):
     out = np.zeros(16, np.float32)
     _check(lib().orion_camera_record(np.asarray(lookat, np.float32), fov, width, height, out))
@@ -381,6 +406,12 @@ class DeviceScene:
         cnt = np.zeros(self.W * self.H, np.int32)
         _check(lib().orion_film_read(self.h, rgb, cnt))
         return rgb.reshape(self.H, self.W, 3), cnt.reshape(self.H, self.W)
+
+    def film_averaged(self):
+        """RenderTarget::getAveraged, computed on the device."""
+        rgb = np.zeros(self.W * self.H * 3, np.float32)
+        _check(lib().orion_film_read_averaged(self.h, rgb))
+        return rgb.reshape(self.H, self.W, 3)
 
     def render_sppm(self, iters, photons, radius, max_depth, hash_size=0, debug_iter=-1):
         W, H = self.W, self.H

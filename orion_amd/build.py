@@ -20,7 +20,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "liborion_amd.so"
 
 HIP_SOURCES = ["runtime.hip", "path_kernels.hip", "sppm_kernels.hip", "batch.hip"]
-CPP_SOURCES = ["api_host.cpp", "host_records.cpp", "ori_loader.cpp", "scenegen.cpp", "bvh.cpp", "png_io.cpp"]
+CPP_SOURCES = ["api_host.cpp", "host_records.cpp", "ori_loader.cpp", "scenegen.cpp", "bvh.cpp", "png_io.cpp", "film_io.cpp"]
 
 ARCH = os.environ.get("ORION_AMD_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",

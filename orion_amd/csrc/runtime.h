@@ -55,6 +55,7 @@ struct orion_scene_s {
   orion_amd::Bvh bvh;
   orion_amd::DevBuf nodes, tris, shade, mats, lights, light_cdf, light_func, light_area, sobol;
   orion_amd::DevBuf env_tex, env_dist, env_meta;
+  orion_amd::DevBuf film_avg;  // getAveraged output
   orion_amd::DevBuf film, count, ctrl, stats;
   orion_amd::DevBuf spill;  // traversal stack spill area of the intersection API
   orion_amd::DScene d{};

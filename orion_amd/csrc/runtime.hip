@@ -418,6 +418,37 @@ int orion_film_clear(orion_scene s) {
   return ORION_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// RenderTarget::getAveraged: Spectrum / int, a true division per channel
+__global__ void k_film_average(const float *sum, const int *count, float *avg, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float c = (float)count[i];
+    avg[3 * (size_t)i] = sum[3 * (size_t)i] / c;
+    avg[3 * (size_t)i + 1] = sum[3 * (size_t)i + 1] / c;
+    avg[3 * (size_t)i + 2] = sum[3 * (size_t)i + 2] / c;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int orion_film_read_averaged(orion_scene s, float *rgb) {
+  if (!s || !rgb) { set_error("null argument"); return ORION_ERR_ARG; }
+  HIP_TRY(hipSetDevice(s->ctx->device));
+  const int n = s->W * s->H;
+  int rc = s->film_avg.ensure((size_t)n * 3 * 4);
+  if (rc) return rc;
+  int blocks = std::min((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_film_average, dim3(blocks), dim3(256), 0, s->ctx->stream, (const float *)s->film.ptr,
+                     (const int *)s->count.ptr, (float *)s->film_avg.ptr, n);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(rgb, s->film_avg.ptr, (size_t)n * 3 * 4, hipMemcpyDeviceToHost, s->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+  return ORION_OK;
+}
+
 int orion_film_read(orion_scene s, float *rgb, int *count) {
   if (!s || !rgb || !count) { set_error("null argument"); return ORION_ERR_ARG; }
   HIP_TRY(hipSetDevice(s->ctx->device));

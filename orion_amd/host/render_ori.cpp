@@ -1,17 +1,19 @@
 // Renders an .ori scene through the C++ host layer, as orion's main.cpp does
-// (parse -> integrator->render(*scene) -> getFilm()->getAveraged()), and
-// writes the averaged FP32 image as a PFM.
+// (parse -> integrator->render(*scene) -> getFilm()->getAveraged() ->
+// saveAsBmp), writing the averaged image as BMP, or as FP32 PFM / EXR, by the
+// output file's extension.
 //
-//   orion_render scene.ori out.pfm [device]
+//   orion_render scene.ori out.{bmp,pfm,exr} [device]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #include "orion_gpu.h"
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s scene.ori out.pfm [device]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s scene.ori out.{bmp,pfm,exr} [device]\n", argv[0]);
     return 2;
   }
   orion_ctx ctx = nullptr;
@@ -29,17 +31,12 @@ int main(int argc, char **argv) {
       auto t2 = std::chrono::steady_clock::now();
       auto film = opt->camera->getFilm();
       std::vector<float> img = film->getAveraged();
-      FILE *f = std::fopen(argv[2], "wb");
-      if (!f) {
-        std::fprintf(stderr, "cannot write %s\n", argv[2]);
+      std::string out = argv[2];
+      std::string ext = out.size() > 4 ? out.substr(out.size() - 4) : "";
+      int fmt = ext == ".pfm" ? ORION_IMAGE_PFM : (ext == ".exr" ? ORION_IMAGE_EXR : ORION_IMAGE_BMP);
+      if (!orion_gpu::check(orion_film_write(out.c_str(), fmt, film->getWidth(), film->getHeight(), img.data()),
+                            "write image"))
         rc = 1;
-      } else {
-        // PFM rows run bottom to top
-        std::fprintf(f, "PF\n%d %d\n-1.0\n", film->getWidth(), film->getHeight());
-        for (int y = film->getHeight() - 1; y >= 0; --y)
-          std::fwrite(&img[3 * (size_t)y * film->getWidth()], sizeof(float), 3 * (size_t)film->getWidth(), f);
-        std::fclose(f);
-      }
       const auto &st = opt->integrator->stats;
       std::printf("{\"load_ms\": %.1f, \"render_ms\": %.2f, \"device_ms\": %.2f, \"paths\": %lld, "
                   "\"closest_rays\": %lld, \"shadow_rays\": %lld, \"photons\": %lld}\n",

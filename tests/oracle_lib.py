@@ -229,6 +229,8 @@ class Ref:
         L.ref_distant_sample_li.argtypes = [C.c_void_p, f32p, f32p, f32p, f32p]
         L.ref_distant_sample_le.argtypes = [C.c_void_p, f32p, f32p, f32p]
         L.ref_distant_power.argtypes = [C.c_void_p, f32p]
+        L.ref_film_average_bmp.restype = C.c_int
+        L.ref_film_average_bmp.argtypes = [C.c_int, C.c_int, f32p, i32p, f32p, C.c_char_p]
         L.ref_envmap_xform.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), C.c_char_p, f32p]
         self.L = L
 

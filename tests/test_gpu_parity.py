@@ -168,6 +168,16 @@ def test_path_constant_envmap(device, oracle):
     assert mean_err <= MEAN_TOL
 
 
+def test_film_averaged_on_device(device):
+    """getAveraged on the device equals the host division of the read film."""
+    hs = oa.HostScene.generate("cornell", params=[40, 24])
+    ds = oa.DeviceScene(device, hs)
+    ds.film_clear()
+    ds.render_path(0, 3, 5)
+    rgb, cnt = ds.film()
+    np.testing.assert_array_equal(ds.film_averaged().view(np.int32), oa.film_average(rgb, cnt).view(np.int32))
+
+
 def test_path_waves_equal_single(device):
     """Splitting the sample range into several wavefronts must not change a bit."""
     hs = oa.HostScene.generate("cornell", params=[64, 64])

@@ -283,6 +283,12 @@ int orion_server_submit(orion_server s, const orion_job_desc *job, uint64_t *job
  * and the request's times in seconds since the server started:
  * times[0] submitted, [1] admitted, [2] finished. */
 int orion_server_poll(orion_server s, uint64_t job_id, int *done, float *rgb, int *count, double times[3]);
+/* Progressive preview of a job: its film (sum + count) after the samples
+ * rendered so far; *samples = how many (every pixel's count).  A running job's
+ * copy is taken at the next wave boundary (the call waits for it); a queued
+ * job returns zeros, a finished one its final film.  The preview equals the
+ * film of the same job rendered with spp = *samples. */
+int orion_server_peek(orion_server s, uint64_t job_id, float *rgb, int *count, int *samples);
 /* blocks until every submitted job has finished */
 int orion_server_drain(orion_server s);
 typedef struct {

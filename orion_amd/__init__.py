@@ -126,6 +126,7 @@ _SIGS = {
     "orion_server_submit": (C.c_int, [C.c_void_p, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),
     "orion_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int), C.c_void_p,
                                     C.c_void_p, C.POINTER(C.c_double)]),
+    "orion_server_peek": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
     "orion_server_drain": (C.c_int, [C.c_void_p]),
     "orion_server_get_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "orion_server_destroy": (C.c_int, [C.c_void_p]),
@@ -523,6 +524,15 @@ class Server:
         if not done.value:
             return None
         return rgb, cnt, tuple(times)
+
+    def peek(self, jid):
+        """Progressive preview: (rgb sum, count, samples rendered so far)."""
+        H, W = self._shape[jid]
+        rgb = np.zeros((H, W, 3), np.float32)
+        cnt = np.zeros((H, W), np.int32)
+        k = C.c_int()
+        _check(lib().orion_server_peek(self.h, jid, _ptr(rgb), _ptr(cnt), C.byref(k)))
+        return rgb, cnt, k.value
 
     def drain(self):
         _check(lib().orion_server_drain(self.h))

@@ -567,6 +567,12 @@ struct ServerJob {
   int64_t bytes = 0;
   std::vector<float> rgb;
   std::vector<int> cnt;
+  // progressive preview: a peek asks for a copy of the film between waves
+  bool peek_req = false;
+  uint64_t peek_seq = 0;
+  int peek_samples = 0;
+  std::vector<float> prgb;
+  std::vector<int> pcnt;
 };
 }  // namespace orion_amd
 
@@ -686,6 +692,23 @@ void orion_server_s::loop() {
       act[i]->next_k += taken[i];
       if (act[i]->next_k >= act[i]->d.spp) finished.push_back(owners[i]);
     }
+    // previews of running jobs asked for since the last wave (the stream is
+    // idle here, so the films hold whole sample passes)
+    std::vector<ServerJob *> peeks;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (ServerJob *j : active)
+        if (j->peek_req && j->state == 1 && std::find(finished.begin(), finished.end(), j) == finished.end())
+          peeks.push_back(j);
+    }
+    for (ServerJob *j : peeks) {
+      size_t np = (size_t)j->st.d.width * j->st.d.height;
+      j->prgb.resize(np * 3);
+      j->pcnt.resize(np);
+      if (!rc && (hipMemcpy(j->prgb.data(), j->st.film, np * 12, hipMemcpyDeviceToHost) != hipSuccess ||
+                  hipMemcpy(j->pcnt.data(), j->st.count, np * 4, hipMemcpyDeviceToHost) != hipSuccess))
+        rc = ORION_ERR_HIP;
+    }
     for (ServerJob *j : finished) {
       size_t np = (size_t)j->st.d.width * j->st.d.height;
       j->rgb.resize(np * 3);
@@ -707,6 +730,11 @@ void orion_server_s::loop() {
       closest += ws.closest_rays;
       shadow += ws.shadow_rays;
       if (rc) { error = rc; error_msg = orion_last_error(); }
+      for (ServerJob *j : peeks) {
+        j->peek_samples = j->st.next_k;
+        j->peek_req = false;
+        j->peek_seq++;
+      }
       for (ServerJob *j : finished) {
         j->state = rc ? 3 : 2;
         j->t_done = now();
@@ -779,6 +807,35 @@ int orion_server_poll(orion_server s, uint64_t id, int *done, float *rgb, int *c
   if (*done) {
     if (rgb) std::memcpy(rgb, j->rgb.data(), j->rgb.size() * 4);
     if (count) std::memcpy(count, j->cnt.data(), j->cnt.size() * 4);
+  }
+  return ORION_OK;
+}
+
+int orion_server_peek(orion_server s, uint64_t id, float *rgb, int *count, int *samples) {
+  if (!s || !samples) { set_error("bad argument"); return ORION_ERR_ARG; }
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->all.find(id);
+  if (it == s->all.end()) { set_error("unknown job id"); return ORION_ERR_ARG; }
+  orion_amd::ServerJob *j = it->second;
+  const size_t np = (size_t)j->st.d.width * j->st.d.height;
+  if (j->state == 1) {  // running: wait for the next wave boundary's copy
+    uint64_t seq = j->peek_seq;
+    j->peek_req = true;
+    s->cv_done.wait(lk, [&] { return j->peek_seq != seq || j->state != 1; });
+  }
+  if (j->state == 3) { set_error("job failed: " + s->error_msg); return s->error ? s->error : ORION_ERR_HIP; }
+  if (j->state == 0) {  // queued: nothing rendered yet
+    if (rgb) std::memset(rgb, 0, np * 12);
+    if (count) std::memset(count, 0, np * 4);
+    *samples = 0;
+  } else if (j->state == 2) {
+    if (rgb) std::memcpy(rgb, j->rgb.data(), np * 12);
+    if (count) std::memcpy(count, j->cnt.data(), np * 4);
+    *samples = j->st.d.spp;
+  } else {
+    if (rgb) std::memcpy(rgb, j->prgb.data(), np * 12);
+    if (count) std::memcpy(count, j->pcnt.data(), np * 4);
+    *samples = j->peek_samples;
   }
   return ORION_OK;
 }

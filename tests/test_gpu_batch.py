@@ -143,6 +143,31 @@ def test_server_continuous_batching(device, scenes):
     srv.close()
 
 
+def test_server_progressive_peek(device, scenes):
+    """previews of a running job are whole sample passes: each equals the job
+    rendered alone with that many samples, and the last one is the final film"""
+    sc = scenes["gold"]
+    job = (sc, 96, 80, 12, 5)
+    srv = oa.Server(device, 1 << 30, wave_paths=96 * 80)   # one sample pass per wave
+    q = srv.submit(sc, 96, 80, 12, 5)
+    seen = []
+    while True:
+        rgb, cnt, k = srv.peek(q)
+        assert (cnt == k).all()
+        if k and (not seen or seen[-1][2] != k):
+            seen.append((rgb, cnt, k))
+        if k == 12:
+            break
+    srv.drain()
+    final, fcnt, _ = srv.poll(q)
+    assert seen[-1][2] == 12
+    np.testing.assert_array_equal(bits(seen[-1][0]), bits(final))
+    for rgb, cnt, k in seen[:4]:
+        alone = oa.render_batch(device, [(sc, 96, 80, k, 5)])[0][0]
+        np.testing.assert_array_equal(bits(alone[0]), bits(rgb))
+    srv.close()
+
+
 def test_autotune(device, scenes):
     jobs = job_list(scenes)
     best = oa.autotune(device, jobs)

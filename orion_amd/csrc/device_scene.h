@@ -126,7 +126,6 @@ __device__ __forceinline__ float q2f(uint32_t word, int k) { return (float)((wor
 struct TRay {
   V3 o, d, inv;
   float tmin, tmax;
-  const float4 *nodes, *tris;
   int tag;  // caller's id (queue entry)
 };
 
@@ -142,13 +141,11 @@ __device__ __forceinline__ V3 safe_inv(V3 d) {
 // child and push the others far-to-near (any-hit: in slot order).  Returns
 // the next node (a leaf code or kLeafEmpty when the stack is exhausted).
 template <bool ANY, int RING>
-__device__ __forceinline__ int node_step(const TRay &ray, float best, int node, TravStack<RING> &st,
-                                         uint32_t *overflow) {
-  const float4 *nd = ray.nodes + 4 * (size_t)node;
-  const float4 A = gld(nd);
-  const int4 C = gld(reinterpret_cast<const int4 *>(nd + 1));
-  const uint4 Q = gld(reinterpret_cast<const uint4 *>(nd + 2));
-  const uint2 R = gld(reinterpret_cast<const uint2 *>(nd + 3));
+__device__ __forceinline__ int node_step(const TRay &ray, float best, float4 A, float4 f1, float4 f2, float4 f3,
+                                         TravStack<RING> &st, uint32_t *overflow) {
+  const int4 C = make_int4(__float_as_int(f1.x), __float_as_int(f1.y), __float_as_int(f1.z), __float_as_int(f1.w));
+  const uint4 Q = make_uint4(__float_as_uint(f2.x), __float_as_uint(f2.y), __float_as_uint(f2.z), __float_as_uint(f2.w));
+  const uint2 R = make_uint2(__float_as_uint(f3.x), __float_as_uint(f3.y));
   const uint32_t eb = __float_as_uint(A.w);
   const float sx = __uint_as_float((eb & 0xffu) << 23);
   const float sy = __uint_as_float(((eb >> 8) & 0xffu) << 23);
@@ -183,12 +180,12 @@ __device__ __forceinline__ int node_step(const TRay &ray, float best, int node, 
   return nh > 0 ? ck[0] : st.pop();
 }
 
-// One triangle of a leaf: the triangle test, keeping the lexicographic
-// minimum (t, prim) and the tie flag.  Any-hit returns true when accepted.
+// One triangle of a leaf (its three float4s): the triangle test, keeping the
+// lexicographic minimum (t, prim) and the tie flag.  Any-hit returns true
+// when accepted.
 template <bool ANY>
-__device__ __forceinline__ bool tri_step(const TRay &ray, int k, float *best, HitRec *h, bool *found) {
-  const float4 *tr = ray.tris + 3 * (size_t)k;
-  float4 a = gld(tr), b = gld(tr + 1), c = gld(tr + 2);
+__device__ __forceinline__ bool tri_step(const TRay &ray, float4 a, float4 b, float4 c, float *best, HitRec *h,
+                                         bool *found) {
   float t, u, v;
   if (!tri_test(a, b, c, ray.o, ray.d, ray.tmin, ANY ? ray.tmax : *best, &t, &u, &v)) return false;
   int prim = __float_as_int(a.w);
@@ -214,7 +211,8 @@ __device__ __forceinline__ bool tri_step(const TRay &ray, int k, float *best, Hi
 // least kRefillMin lanes are idle.  Waves own fixed ranges (no shared work
 // counter: device-scope atomics on one address serialise across the XCDs
 // and cost more than they balance).  IO: void load(int entry, TRay *ray);
-// void store(const TRay &, bool hit, const HitRec &).
+// void store(const TRay &, bool hit, const HitRec &); nodes, tris: the
+// scene, the same for every ray of the call (kept in scalar registers).
 constexpr int kRefillMin = 8;
 
 template <bool ANY, int RING, class IO>
@@ -222,6 +220,7 @@ __device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> 
                                     uint32_t *tri_tests, uint32_t *overflow) {
   const int lane = (int)__lane_id();
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const float4 *const nodes = io.nodes, *const tris = io.tris;
   int next = begin;  // wave-uniform: next unassigned entry
   bool active = false;
   TRay ray;
@@ -253,15 +252,26 @@ __device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> 
       next = min(next + nidle, end);
     }
     if (__ballot(active) == 0) break;
-    if (tcur < tend) {
-      ++nt;
-      if (tri_step<ANY>(ray, tcur++, &best, &h, &found)) {
+    // one fetch for the whole wave whatever each lane works on: a node is 4
+    // float4s, a triangle 3 (the fourth is the next record's, or the buffer's
+    // padding), so lanes on nodes and lanes on triangles wait on one latency
+    const bool do_tri = tcur < tend, do_node = !do_tri && node >= 0;
+    float4 f0, f1, f2, f3;
+    if (do_tri || do_node) {
+      const float4 *src = do_tri ? tris + 3 * (size_t)tcur : nodes + 4 * (size_t)node;
+      f0 = gld(src); f1 = gld(src + 1); f2 = gld(src + 2); f3 = gld(src + 3);
+    }
+    // visit counters are kept per wave (scalar registers)
+    nt += (uint32_t)__popcll(__ballot(do_tri));
+    nv += (uint32_t)__popcll(__ballot(do_node));
+    if (do_tri) {
+      ++tcur;
+      if (tri_step<ANY>(ray, f0, f1, f2, &best, &h, &found)) {
         tcur = tend;  // any-hit: occluded, done
         node = kLeafEmpty;
       }
-    } else if (node >= 0) {
-      ++nv;
-      node = node_step<ANY, RING>(ray, best, node, st, overflow);
+    } else if (do_node) {
+      node = node_step<ANY, RING>(ray, best, f0, f1, f2, f3, st, overflow);
     }
     if (tcur >= tend && node < 0 && node != kLeafEmpty) {  // open the leaf reached
       const int code = ~node;
@@ -274,7 +284,7 @@ __device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> 
       active = false;
     }
   }
-  *node_visits += nv;
+  *node_visits += nv;  // the wave's totals, the same in every lane
   *tri_tests += nt;
 }
 

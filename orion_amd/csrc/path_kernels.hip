@@ -79,8 +79,6 @@ struct WaveRays {
     ray->d = mk(w->dx[slot], w->dy[slot], w->dz[slot]);
     ray->tmin = 0.0f;
     ray->tmax = w->tmax[slot];
-    ray->nodes = nodes;
-    ray->tris = tris;
     ray->tag = r;
   }
   __device__ void store(const TRay &ray, bool hit, const HitRec &h) const {
@@ -105,7 +103,9 @@ template <bool ANY, int RING, bool MULTI>
 __device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int chunk, int lb, int *ring) {
   const int seg_entries = ANY ? w.seg : 2 * w.seg;
   const int wps = (seg_entries + chunk - 1) / chunk;
-  const int gw = lb * (kTraceBlock / 64) + (int)(threadIdx.x >> 6);
+  // the wave's index is uniform: read from lane 0 so that everything derived
+  // from it (segment, job, scene pointers) stays in scalar registers
+  const int gw = lb * (kTraceBlock / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int sg = gw / wps;
   if (sg >= w.nseg) return;
   const int count = (ANY ? w.cnt_shadow : w.cnt_ray)[sg];
@@ -121,11 +121,7 @@ __device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int 
   WaveRays<ANY> io{&w, sg * seg_entries, sc.nodes, sc.tris};
   trace_engine<ANY, RING>(io, begin, min(begin + chunk, count), stk, &nv, &nt, &ovf);
   const int lane = (int)__lane_id();
-  // per-wave reduction of traversal counters (for the roofline's n_node / n_tri)
-  for (int off = 32; off > 0; off >>= 1) {
-    nv += __shfl_down(nv, off);
-    nt += __shfl_down(nt, off);
-  }
+  // nv / nt are the wave's totals (for the roofline's n_node / n_tri)
   if (ovf) stat_add(w.stats, S_OVERFLOW, ovf);
   if (lane == 0) {
     stat_add(w.stats, ANY ? S_NODES_S : S_NODES_C, nv);
@@ -326,7 +322,7 @@ __global__ void k_film(const Job *jobs, Wave w) {
 
 // ---------------------------------------------------------------- batch intersection API
 struct SoaRays {
-  const DScene *sc;
+  const float4 *nodes, *tris;
   int n;
   const float *rays;
   float *t, *u, *v;
@@ -336,8 +332,6 @@ struct SoaRays {
     ray->d = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
     ray->tmin = rays[6 * n + i];
     ray->tmax = rays[7 * n + i];
-    ray->nodes = sc->nodes;
-    ray->tris = sc->tris;
     ray->tag = i;
   }
   __device__ void store(const TRay &ray, bool hit, const HitRec &h) const {
@@ -368,7 +362,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect_batch(DScene sc, int 
   stk.spill = spill + blockIdx.x * kTraceBlock + threadIdx.x;
   stk.spill_cap = spill_cap;
   uint32_t ovf = 0, nv = 0, nt = 0;
-  SoaRays io{&sc, n, rays, t, u, v, prim, tie, ANY ? occluded : nullptr};
+  SoaRays io{sc.nodes, sc.tris, n, rays, t, u, v, prim, tie, ANY ? occluded : nullptr};
   trace_engine<ANY, RING>(io, begin, min(begin + chunk, n), stk, &nv, &nt, &ovf);
   if (ovf) stat_add(stats, S_OVERFLOW, ovf);
 }

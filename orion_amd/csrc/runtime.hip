@@ -270,6 +270,9 @@ int orion_scene_upload(orion_ctx c, orion_hscene hs, orion_scene *out) {
   };
   int rc;
   if ((rc = up(sc->nodes, sc->bvh.nodes.data(), sc->bvh.nodes.size() * sizeof(BvhNode)))) return fail(rc);
+  // 16 bytes past the last triangle: the traversal fetches four float4s for a
+  // triangle's three (one fetch shape for nodes and triangles)
+  if ((rc = sc->tris.ensure(sc->bvh.tris.size() * sizeof(BvhTri) + 16))) return fail(rc);
   if ((rc = up(sc->tris, sc->bvh.tris.data(), sc->bvh.tris.size() * sizeof(BvhTri)))) return fail(rc);
   if ((rc = up(sc->shade, shade.data(), shade.size() * 4))) return fail(rc);
   if ((rc = up(sc->mats, s.mats.data(), s.mats.size() * 4))) return fail(rc);

@@ -367,6 +367,8 @@ def main():
         "launch_ms": round(per_launch_s * 1e3, 4),
         "closest": per_ray("closest"), "shadow": per_ray("shadow"),
         "l2_hit_rate": t["l2_hit_rate"] if t else None,
+        # VALU busy from the same rocprofv3 runs: the traversal is issue bound
+        "valu_busy": t.get("valu_busy") if t else None,
         "kernel_ms": {k: round(v, 2) for k, v in kern.items()},
     }
     if traffic:

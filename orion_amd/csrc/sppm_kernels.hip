@@ -762,7 +762,9 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   HIP_TRY(hipEventRecord(c->ev0, st));
   const int TB = 256;
   hipLaunchKernelGGL(k_pix_init, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, b.px, npix, r0);
-  const int chunk = kDefaultChunk;
+  // SPPM's launches trace at most 2^20 rays: 64-entry chunks give such a launch
+  // enough waves to fill the GPU (measured 10% faster per iteration than 256)
+  const int chunk = 64;
   {
     int blocks = trace_grid(1, npix, false, chunk);
     if (photons > 0) blocks = std::max(blocks, trace_grid(1, photons, false, chunk));

@@ -234,7 +234,7 @@ __global__ void k_grid_init(Grid g) {
 
 // bounds = union of expand(vp.p, r) over visible points (sppm.cpp:149-157)
 __global__ __launch_bounds__(256) void k_grid_bounds(Pix px, Grid g, int npix) {
-  __shared__ int red[8][7];
+  __shared__ int red[4][8];  // per wave of the 256-thread block
   int lo[3], hi[3], mr = 0, any = 0;
   for (int a = 0; a < 3; ++a) { lo[a] = 0x7f7fffff; hi[a] = f2key(-3.40282347e+38f); }
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
@@ -254,13 +254,20 @@ __global__ __launch_bounds__(256) void k_grid_bounds(Pix px, Grid g, int npix) {
       int o = __shfl_down(vals[k], off);
       vals[k] = k < 3 ? min(vals[k], o) : max(vals[k], o);
     }
-  if (vals[7] == 0 && lane_id() == 0) return;
-  if (lane_id() == 0) {
-    for (int a = 0; a < 3; ++a) { atomicMin(&g.ictl[4 + a], vals[a]); atomicMax(&g.ictl[7 + a], vals[3 + a]); }
-    atomicMax(&g.ictl[10], vals[6]);
-    atomicMax(&g.ictl[3], 1);
-  }
-  (void)red;
+  // block reduction through LDS, then one set of device atomics per block:
+  // one set per wave (a 1024-block grid) serialised on these 8 words and
+  // took ~0.3 ms per iteration
+  const int wv = threadIdx.x >> 6;
+  if (lane_id() == 0)
+    for (int k = 0; k < 8; ++k) red[wv][k] = vals[k];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int v = 1; v < (int)(blockDim.x >> 6); ++v)
+    for (int k = 0; k < 8; ++k) vals[k] = k < 3 ? min(vals[k], red[v][k]) : max(vals[k], red[v][k]);
+  if (vals[7] == 0) return;
+  for (int a = 0; a < 3; ++a) { atomicMin(&g.ictl[4 + a], vals[a]); atomicMax(&g.ictl[7 + a], vals[3 + a]); }
+  atomicMax(&g.ictl[10], vals[6]);
+  atomicMax(&g.ictl[3], 1);
 }
 
 // grid resolution (sppm.cpp:159-163)
@@ -796,7 +803,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     }
     // --- grid
     hipLaunchKernelGGL(k_grid_init, dim3(1), dim3(64), 0, st, g);
-    hipLaunchKernelGGL(k_grid_bounds, dim3(std::min((npix + 255) / 256, 1024)), dim3(256), 0, st, b.px, g, npix);
+    hipLaunchKernelGGL(k_grid_bounds, dim3(std::min((npix + 255) / 256, 256)), dim3(256), 0, st, b.px, g, npix);
     hipLaunchKernelGGL(k_grid_res, dim3(1), dim3(64), 0, st, g);
     hipLaunchKernelGGL(k_grid_count, dim3((npix + 1 + TB - 1) / TB), dim3(TB), 0, st, b.px, g, npix);
     device_scan(g.cnt, g.cnt, npix + 1, g.scan_tmp, st);

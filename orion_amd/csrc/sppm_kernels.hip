@@ -458,7 +458,10 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(const uint32_t *kin
 
 __global__ void k_cell_clear(Grid g) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < g.hash_size) { g.cell_start[i] = 0; g.cell_end[i] = 0; g.photon_cell[i] = 0; }
+  if (i < g.hash_size) {
+    g.cell_start[i] = 0; g.cell_end[i] = 0;
+    if (g.photon_cell) g.photon_cell[i] = 0;
+  }
 }
 __global__ void k_cell_ranges(Grid g, const uint32_t *keys, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -473,13 +476,9 @@ __global__ void k_cell_counts(Grid g, int *out) {
 }
 
 // ---------------------------------------------------------------- photon pass
-__global__ void k_photon_init(DScene sc, Wave w, uint32_t index0, int P) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) {
-    w.ctrl[C_WORK_RAY] = 0;
-    w.ctrl[C_SHADOW] = 0;
-  }
-  if (p >= P) return;
+// Emits photon p (sppm.cpp:203-222) into the wave's ray slots; true when it
+// carries power.
+__device__ __forceinline__ bool photon_emit(DScene sc, Wave w, uint32_t index0, int p) {
   uint32_t index = index0 + (uint32_t)p;  // sppm.cpp:203
   float lightPdf;
   int l = sample_light(sc, sobol(sc.sobol, index, 5), &lightPdf);
@@ -499,7 +498,19 @@ __global__ void k_photon_init(DScene sc, Wave w, uint32_t index0, int P) {
   w.tmax[p] = kFltMax;
   w.br[p] = beta.r; w.bg[p] = beta.g; w.bb[p] = beta.b;
   w.dim[p] = 10;
-  int a = wave_append(&w.ctrl[C_RAY], alive);
+  return alive;
+}
+
+__global__ __launch_bounds__(256) void k_photon_init(DScene sc, Wave w, uint32_t index0, int P) {
+  __shared__ int lds[5];
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) {
+    w.ctrl[C_WORK_RAY] = 0;
+    w.ctrl[C_SHADOW] = 0;
+  }
+  bool alive = false;
+  if (p < P) alive = photon_emit(sc, w, index0, p);
+  int a = block_append(&w.ctrl[C_RAY], alive, lds);
   if (alive) w.qray[a] = p;
 }
 
@@ -512,73 +523,120 @@ __global__ void k_photon_between(Wave w, unsigned long long *stats) {
   }
 }
 
+// Photon deposits of one wave (sppm.cpp:230-257), balanced over its lanes.
+// Each lane brings one photon and the range [e0, e0 + n) of visible points
+// in its cell; the wave walks the concatenation of all 64 ranges, 64 entries
+// per step.  Cells near the horizon hold thousands of visible points, so a
+// lane-per-photon loop would leave 63 lanes idle behind one such cell.
+// ph holds the wave's photons (9 floats each: p, -d, beta), inc the wave's
+// inclusive prefix of n.  Returns the deposits this lane made.
+__device__ __forceinline__ unsigned long long photon_deposits(const DScene &sc, const Pix &px, const Grid &g,
+                                                               int e0, int n, const float *ph, int *ebase,
+                                                               int *inc) {
+  const int lane = lane_id();
+  int x = n;
+  for (int off = 1; off < 64; off <<= 1) {
+    int y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  const int total = __shfl(x, 63);
+  if (total == 0) return 0;
+  inc[lane] = x;
+  ebase[lane] = e0 - (x - n);  // entry k of the concatenation owned by lane j is ebase[j] + k
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  unsigned long long nd = 0;
+  for (int k = lane; k < total; k += 64) {
+    int lo = 0, hi = 63;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (inc[mid] > k) hi = mid; else lo = mid + 1;
+    }
+    const float *f = ph + 9 * lo;
+    V3 hp = mk(f[0], f[1], f[2]);
+    int q = g.vals[0][ebase[lo] + k];
+    float r = px.radius[q];
+    V3 vp = mk(px.vpx[q], px.vpy[q], px.vpz[q]);
+    if (lensq(sub(vp, hp)) > r * r) continue;
+    Bsdf vb;
+    make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)px.vmat[q], ld3(px.vng, q), ld3(px.vns, q), ld3(px.vu, q),
+              ld3(px.vv, q), false, &vb);
+    Sp Phi = sp3(f[6], f[7], f[8]) * bsdf_f(vb, mk(f[3], f[4], f[5]), mk(px.vwox[q], px.vwoy[q], px.vwoz[q]), ALL);
+    atomicAdd(&px.Phir[q], Phi.r);
+    atomicAdd(&px.Phig[q], Phi.g);
+    atomicAdd(&px.Phib[q], Phi.b);
+    atomicAdd(&px.M[q], 1);
+    ++nd;
+  }
+  __builtin_amdgcn_wave_barrier();  // ph / inc are rewritten by the next call
+  return nd;
+}
+
 // One photon bounce (sppm.cpp:225-271).  Reads the traced queue (qray,
 // count in C_PATH0), appends survivors to qpath[0] (count in C_SHADOW),
 // then the host swaps them back for the next trace.
 __global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px, Grid g, uint32_t index0,
                                                       int depth, unsigned long long *stats) {
+  __shared__ int lds[5];
+  __shared__ float phot[256 * 9];
+  __shared__ int ebase[256], inc[256];
   const int count = w.ctrl[C_PATH0];
+  const int wv = threadIdx.x >> 6;
   unsigned long long ndep = 0;
   for (int base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
     int i = base + threadIdx.x;
-    bool keep = false;
-    int p = 0;
+    bool keep = false, hit = false;
+    int p = 0, e0 = 0, n = 0;
+    Isect is;
+    V3 d = mk(0, 0, 0);
+    Sp beta = sp(0);
     if (i < count) {
       p = w.qray[i];
       int hp = w.hprim[p];
       if (hp >= 0) {
-        Isect is;
+        hit = true;
         make_isect(sc, hp, w.hu[p], w.hv[p], &is);
-        V3 d = mk(w.dx[p], w.dy[p], w.dz[p]);
-        Sp beta = sp3(w.br[p], w.bg[p], w.bb[p]);
-        if (depth > 0) {
-          int c[3];
-          if (to_grid(is.p, g.fctl, g.fctl + 3, g.ictl, c)) {
-            uint32_t h = grid_hash(c[0], c[1], c[2], g.hash_size);
-            atomicAdd(&g.photon_cell[h], 1);
-            int e0 = g.cell_start[h], e1 = g.cell_end[h];
-            for (int e = e0; e < e1; ++e) {
-              int q = g.vals[0][e];
-              float r = px.radius[q];
-              V3 vp = mk(px.vpx[q], px.vpy[q], px.vpz[q]);
-              if (lensq(sub(vp, is.p)) > r * r) continue;
-              Bsdf vb;
-              make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)px.vmat[q], ld3(px.vng, q), ld3(px.vns, q), ld3(px.vu, q),
-                        ld3(px.vv, q), false, &vb);
-              Sp Phi = beta * bsdf_f(vb, neg(d), mk(px.vwox[q], px.vwoy[q], px.vwoz[q]), ALL);
-              atomicAdd(&px.Phir[q], Phi.r);
-              atomicAdd(&px.Phig[q], Phi.g);
-              atomicAdd(&px.Phib[q], Phi.b);
-              atomicAdd(&px.M[q], 1);
-              ++ndep;
-            }
-          }
+        d = mk(w.dx[p], w.dy[p], w.dz[p]);
+        beta = sp3(w.br[p], w.bg[p], w.bb[p]);
+        int c[3];
+        if (depth > 0 && to_grid(is.p, g.fctl, g.fctl + 3, g.ictl, c)) {
+          uint32_t h = grid_hash(c[0], c[1], c[2], g.hash_size);
+          if (g.photon_cell) atomicAdd(&g.photon_cell[h], 1);  // debug output only
+          e0 = g.cell_start[h];
+          n = g.cell_end[h] - e0;
+          float *f = phot + 9 * threadIdx.x;
+          f[0] = is.p.x; f[1] = is.p.y; f[2] = is.p.z;
+          f[3] = -d.x; f[4] = -d.y; f[5] = -d.z;
+          f[6] = beta.r; f[7] = beta.g; f[8] = beta.b;
         }
-        Bsdf bs;
-        make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)is.mat, is.ng, is.ns, is.u, is.v, true, &bs);
-        uint32_t index = index0 + (uint32_t)p;
-        int dim = w.dim[p];
-        float u1 = sobol(sc.sobol, index, dim++), u0 = sobol(sc.sobol, index, dim++);
-        V3 wi = mk(0, 0, 0);
-        float pdf = 0;
-        int flags = 0;
-        Sp fr = bsdf_sample(bs, &wi, neg(d), u0, u1, &pdf, ALL, &flags);
-        if (!(black(fr) || pdf == 0)) {
-          Sp bnew = beta * fr * absdot(wi, is.ns) / pdf;
-          float q = mx(0.0f, 1 - intensity(bnew) / intensity(beta));
-          if (!(sobol(sc.sobol, index, dim++) < q)) {
-            beta = bnew / (1 - q);
-            V3 o = offset_ray_origin(is.p, is.ng, wi);
-            w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
-            w.dx[p] = wi.x; w.dy[p] = wi.y; w.dz[p] = wi.z;
-            w.br[p] = beta.r; w.bg[p] = beta.g; w.bb[p] = beta.b;
-            keep = true;
-          }
-        }
-        w.dim[p] = dim;
       }
     }
-    int a = wave_append(&w.ctrl[C_SHADOW], keep);
+    if (depth > 0) ndep += photon_deposits(sc, px, g, e0, n, phot + 9 * 64 * wv, ebase + 64 * wv, inc + 64 * wv);
+    if (hit) {
+      Bsdf bs;
+      make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)is.mat, is.ng, is.ns, is.u, is.v, true, &bs);
+      uint32_t index = index0 + (uint32_t)p;
+      int dim = w.dim[p];
+      float u1 = sobol(sc.sobol, index, dim++), u0 = sobol(sc.sobol, index, dim++);
+      V3 wi = mk(0, 0, 0);
+      float pdf = 0;
+      int flags = 0;
+      Sp fr = bsdf_sample(bs, &wi, neg(d), u0, u1, &pdf, ALL, &flags);
+      if (!(black(fr) || pdf == 0)) {
+        Sp bnew = beta * fr * absdot(wi, is.ns) / pdf;
+        float q = mx(0.0f, 1 - intensity(bnew) / intensity(beta));
+        if (!(sobol(sc.sobol, index, dim++) < q)) {
+          beta = bnew / (1 - q);
+          V3 o = offset_ray_origin(is.p, is.ng, wi);
+          w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
+          w.dx[p] = wi.x; w.dy[p] = wi.y; w.dz[p] = wi.z;
+          w.br[p] = beta.r; w.bg[p] = beta.g; w.bb[p] = beta.b;
+          keep = true;
+        }
+      }
+      w.dim[p] = dim;
+    }
+    int a = block_append(&w.ctrl[C_SHADOW], keep, lds);
     if (keep) w.qpath[0][a] = p;
   }
   for (int off = 32; off > 0; off >>= 1) ndep += __shfl_down(ndep, off);
@@ -831,7 +889,12 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
       HIP_TRY(hipMemcpyAsync(g.keys[0], g.keys[1], (size_t)n_pairs * 4, hipMemcpyDeviceToDevice, st));
       HIP_TRY(hipMemcpyAsync(g.vals[0], g.vals[1], (size_t)n_pairs * 4, hipMemcpyDeviceToDevice, st));
     }
-    hipLaunchKernelGGL(k_cell_clear, dim3((hash_size + TB - 1) / TB), dim3(TB), 0, st, g);
+    // per-cell photon counts are kept only for the iteration the debug output
+    // asks for: caustic photons land in few cells, and these atomics then
+    // serialise
+    Grid gi = g;
+    if (!(want_dbg && iter == dbg->debug_iter && dbg->photon_cell_count)) gi.photon_cell = nullptr;
+    hipLaunchKernelGGL(k_cell_clear, dim3((hash_size + TB - 1) / TB), dim3(TB), 0, st, gi);
     if (n_pairs > 0)
       hipLaunchKernelGGL(k_cell_ranges, dim3((n_pairs + TB - 1) / TB), dim3(TB), 0, st, g, g.keys[0], n_pairs);
     if (want_dbg && iter == dbg->debug_iter) {
@@ -869,7 +932,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
         launch_trace(dj, pw, false, ring, chunk, false, st);
         hipLaunchKernelGGL(k_photon_between, dim3(1), dim3(64), 0, st, pw, dstats);
         int blocks = std::min((photons + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_photon_shade, dim3(blocks), dim3(256), 0, st, s->d, pw, b.px, g, index0, depth, dstats);
+        hipLaunchKernelGGL(k_photon_shade, dim3(blocks), dim3(256), 0, st, s->d, pw, b.px, gi, index0, depth, dstats);
         hipLaunchKernelGGL(k_photon_swap, dim3(1), dim3(64), 0, st, pw);
         hipLaunchKernelGGL(k_copy_queue, dim3((photons + TB - 1) / TB), dim3(TB), 0, st, pw);
       }

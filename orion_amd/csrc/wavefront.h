@@ -161,4 +161,26 @@ __device__ __forceinline__ int wave_append(int *counter, bool pred) {
   return base + __popcll(below);
 }
 
+// Block-aggregated append: one device atomic per block, slots in thread
+// order.  Returning atomics on one address serialise at L2 (~10 ns each), so
+// a 2^20-entry queue filled one atomic per wave costs ~160 us; per block of
+// 256 threads it is a quarter of that.  Every thread of the block must call
+// it (uniform control flow); lds holds blockDim.x / 64 + 1 ints.
+__device__ __forceinline__ int block_append(int *counter, bool pred, int *lds) {
+  unsigned long long mask = __ballot(pred);
+  const int lane = lane_id(), wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) lds[wv] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int v = 0; v < nw; ++v) { int c = lds[v]; lds[v] = tot; tot += c; }
+    lds[nw] = tot ? atomicAdd(counter, tot) : 0;
+  }
+  __syncthreads();
+  unsigned long long below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+  int r = lds[nw] + lds[wv] + __popcll(below);
+  __syncthreads();  // lds is free for the next call
+  return r;
+}
+
 }  // namespace orion_amd

@@ -501,26 +501,16 @@ __device__ __forceinline__ bool photon_emit(DScene sc, Wave w, uint32_t index0, 
   return alive;
 }
 
+// Emits photons [0, P) into queue 0 (counter C_PQ + 0, zeroed by the host
+// beforehand) and zeroes counter C_PQ + 1 for the first bounce.
 __global__ __launch_bounds__(256) void k_photon_init(DScene sc, Wave w, uint32_t index0, int P) {
   __shared__ int lds[5];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) {
-    w.ctrl[C_WORK_RAY] = 0;
-    w.ctrl[C_SHADOW] = 0;
-  }
+  if (p == 0) w.ctrl[C_PQ + 1] = 0;
   bool alive = false;
   if (p < P) alive = photon_emit(sc, w, index0, p);
-  int a = block_append(&w.ctrl[C_RAY], alive, lds);
+  int a = block_append(&w.ctrl[C_PQ], alive, lds);
   if (alive) w.qray[a] = p;
-}
-
-__global__ void k_photon_between(Wave w, unsigned long long *stats) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    stat_add(stats, S_CLOSEST, (unsigned long long)w.ctrl[C_RAY]);
-    w.ctrl[C_PATH0] = w.ctrl[C_RAY];  // photons to shade = photons just traced
-    w.ctrl[C_SHADOW] = 0;             // next queue length
-    w.ctrl[C_WORK_RAY] = 0;
-  }
 }
 
 // Photon deposits of one wave (sppm.cpp:230-257), balanced over its lanes.
@@ -572,15 +562,23 @@ __device__ __forceinline__ unsigned long long photon_deposits(const DScene &sc, 
   return nd;
 }
 
-// One photon bounce (sppm.cpp:225-271).  Reads the traced queue (qray,
-// count in C_PATH0), appends survivors to qpath[0] (count in C_SHADOW),
-// then the host swaps them back for the next trace.
+// One photon bounce (sppm.cpp:225-271).  Bounce d reads the queue just
+// traced (qcur, count in C_PQ + d % 3) and appends its survivors to qnext
+// (count in C_PQ + (d + 1) % 3, zeroed by the bounce before); it zeroes
+// C_PQ + (d + 2) % 3 for the bounce after.  The host alternates the two
+// queue buffers, so no launch sits between a bounce and the next trace.
 __global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px, Grid g, uint32_t index0,
-                                                      int depth, unsigned long long *stats) {
+                                                      int depth, const int *qcur, int *qnext,
+                                                      unsigned long long *stats) {
   __shared__ int lds[5];
   __shared__ float phot[256 * 9];
   __shared__ int ebase[256], inc[256];
-  const int count = w.ctrl[C_PATH0];
+  int *const cnt = w.ctrl + C_PQ;
+  const int count = cnt[depth % 3];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    cnt[(depth + 2) % 3] = 0;
+    stat_add(stats, S_CLOSEST, (unsigned long long)count);
+  }
   const int wv = threadIdx.x >> 6;
   unsigned long long ndep = 0;
   for (int base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
@@ -591,7 +589,7 @@ __global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px,
     V3 d = mk(0, 0, 0);
     Sp beta = sp(0);
     if (i < count) {
-      p = w.qray[i];
+      p = qcur[i];
       int hp = w.hprim[p];
       if (hp >= 0) {
         hit = true;
@@ -636,23 +634,11 @@ __global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px,
       }
       w.dim[p] = dim;
     }
-    int a = block_append(&w.ctrl[C_SHADOW], keep, lds);
-    if (keep) w.qpath[0][a] = p;
+    int a = block_append(&cnt[(depth + 1) % 3], keep, lds);
+    if (keep) qnext[a] = p;
   }
   for (int off = 32; off > 0; off >>= 1) ndep += __shfl_down(ndep, off);
   if (lane_id() == 0) stat_add(stats, S_DEPOSITS, ndep);
-}
-
-__global__ void k_photon_swap(Wave w) {
-  // survivors (qpath[0], count C_SHADOW) become the next trace queue
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    w.ctrl[C_RAY] = w.ctrl[C_SHADOW];
-    w.ctrl[C_WORK_RAY] = 0;
-  }
-}
-__global__ void k_copy_queue(Wave w) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < w.ctrl[C_RAY]) w.qray[i] = w.qpath[0][i];
 }
 
 // ---------------------------------------------------------------- update + film
@@ -831,7 +817,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   // enough waves to fill the GPU (measured 10% faster per iteration than 256)
   const int chunk = 64;
   {
-    int blocks = trace_grid(1, npix, false, chunk);
+    int blocks = trace_pair_grid(1, npix, chunk);  // the camera pass traces both queues in one launch
     if (photons > 0) blocks = std::max(blocks, trace_grid(1, photons, false, chunk));
     if ((rc = b.spill.ensure(trace_spill_ints(s->d.stack_bound, ring, blocks) * 4 + 4))) return rc;
   }
@@ -852,8 +838,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     hipLaunchKernelGGL(k_cam_raygen, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, s->d, cw, b.px, iter);
     int cur = 0;
     for (int it = 0; it <= maxDepth; ++it) {
-      launch_trace(dj, cw, false, ring, chunk, false, st);
-      launch_trace(dj, cw, true, ring, chunk, false, st);
+      launch_trace_pair(dj, cw, ring, chunk, false, st);  // closest-hit + shadow queues in one launch
       launch_between(cw, cur ^ 1, st);
       int blocks = std::min((npix + 255) / 256, 8192);
       hipLaunchKernelGGL(k_cam_shade, dim3(blocks), dim3(256), 0, st, s->d, cw, b.px, cur, maxDepth, iter);
@@ -923,18 +908,18 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
       pw.N = photons;
       pw.seg = photons;
       pw.nseg = 1;
-      pw.cnt_ray = w.ctrl + C_RAY;
-      pw.cnt_shadow = w.ctrl + C_SHADOW;
       uint32_t index0 = (uint32_t)(n_iters * npix + iter * photons);
-      HIP_TRY(hipMemsetAsync(w.ctrl + C_RAY, 0, 4, st));
+      HIP_TRY(hipMemsetAsync(w.ctrl + C_PQ, 0, 4, st));
       hipLaunchKernelGGL(k_photon_init, dim3((photons + TB - 1) / TB), dim3(TB), 0, st, s->d, pw, index0, photons);
+      int *qbuf[2] = {w.qray, w.qpath[0]};  // bounce d traces qbuf[d & 1]
+      const int blocks = std::min((photons + 255) / 256, 8192);
       for (int depth = 0; depth < maxDepth; ++depth) {
-        launch_trace(dj, pw, false, ring, chunk, false, st);
-        hipLaunchKernelGGL(k_photon_between, dim3(1), dim3(64), 0, st, pw, dstats);
-        int blocks = std::min((photons + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_photon_shade, dim3(blocks), dim3(256), 0, st, s->d, pw, b.px, gi, index0, depth, dstats);
-        hipLaunchKernelGGL(k_photon_swap, dim3(1), dim3(64), 0, st, pw);
-        hipLaunchKernelGGL(k_copy_queue, dim3((photons + TB - 1) / TB), dim3(TB), 0, st, pw);
+        Wave tw = pw;
+        tw.qray = qbuf[depth & 1];
+        tw.cnt_ray = w.ctrl + C_PQ + depth % 3;
+        launch_trace(dj, tw, false, ring, chunk, false, st);
+        hipLaunchKernelGGL(k_photon_shade, dim3(blocks), dim3(256), 0, st, s->d, tw, b.px, gi, index0, depth,
+                           qbuf[depth & 1], qbuf[(depth + 1) & 1], dstats);
       }
     }
     if (want_dbg && iter == dbg->debug_iter && dbg->photon_cell_count)

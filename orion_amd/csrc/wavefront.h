@@ -26,6 +26,7 @@ enum : int {
   C_NPATHS = 6,               // camera paths in the current wave (all jobs)
   C_NPIX = 7,                 // film pixels of all jobs in the wave
   C_NJOBS = 8,                // jobs in the current wave
+  C_PQ = 9,                   // (SPPM) photon queue lengths, 3 rotating counters
   C_INTS = 16,
 };
 // 64-bit statistics, each striped over kStatStripes counters (a block adds

@@ -185,26 +185,36 @@ struct WaveRunner {
   void enqueue_wave(const Job *dj, const Wave &w, int depth, int ring, bool multi, int chunk, bool split,
                     hipStream_t st, LaunchTimer *tm) {
     launch_raygen(dj, w, multi, st);
-    int cur = 0;
     for (int it = 0; it <= depth; ++it) {
+      Wave wi = w;  // bounce it traces the ray lengths of counter set it % 3
+      wi.cnt_ray = w.cnt + (3 * (it % 3) + 1) * w.cnt_stride;
+      wi.cnt_shadow = w.cnt + (3 * (it % 3) + 2) * w.cnt_stride;
       if (split) {
         if (tm) tm->mark(0);
-        launch_trace(dj, w, false, ring, chunk, multi, st);
+        launch_trace(dj, wi, false, ring, chunk, multi, st);
         if (tm) { tm->mark(-1); tm->mark(1); }
-        launch_trace(dj, w, true, ring, chunk, multi, st);
+        launch_trace(dj, wi, true, ring, chunk, multi, st);
         if (tm) tm->mark(-1);
       } else {
         if (tm) tm->mark(3);
-        launch_trace_pair(dj, w, ring, chunk, multi, st);
+        launch_trace_pair(dj, wi, ring, chunk, multi, st);
         if (tm) tm->mark(-1);
       }
       if (tm) tm->mark(2);
-      launch_shade(dj, w, cur, multi, st);
+      launch_shade(dj, wi, it, multi, st);
       if (tm) tm->mark(-1);
-      cur ^= 1;
     }
     launch_film(dj, w, std::max(wb.cap, 1 << 20), multi, st);
   }
+
+  // Shading blocks per segment: enough that a launch has ~kShadeBlocks
+  // blocks to balance over the CUs, each part at least 1024 paths.
+  static int shade_parts(int seg, int nseg) {
+    int k = 1;
+    while (2 * k * 1024 <= seg && (int64_t)nseg * k < kShadeBlocks) k *= 2;
+    return k;
+  }
+  static constexpr int kShadeBlocks = 8192;
 
   // Runs one wave: jobs[0..n) with path0 / pix0 already assigned.
   // seg: queue segment size (0: chosen from npaths); multi-job waves pass
@@ -239,6 +249,7 @@ struct WaveRunner {
     Wave w = wave();
     w.seg = seg;
     w.nseg = nseg;
+    w.shade_parts = shade_parts(seg, nseg);
     w.seg_job = (const int *)segjob_dev.ptr;
     if (!use_graph || tm) {
       enqueue_wave(dj, w, depth, ring, multi, chunk, split, st, tm);

@@ -36,11 +36,15 @@ __global__ void k_raygen(const Job *jobs, Wave w) {
   const Job &J = jobs[MULTI ? w.seg_job[p / w.seg] : 0];
   const int live_end = J.path0 + J.S * J.npix;
   if (p % w.seg == 0) {
-    const int sg = p / w.seg;
+    // set 0: this bounce's lengths; set 1: empty, bounce 0 appends to it
+    const int sg = p / w.seg, cs = w.cnt_stride;
     const int n = max(0, min(w.seg, live_end - p));
-    w.cnt_path[0][sg] = n;
-    w.cnt_ray[sg] = n;
-    w.cnt_shadow[sg] = 0;
+    w.cnt[sg] = n;
+    w.cnt[cs + sg] = n;
+    w.cnt[2 * cs + sg] = 0;
+    w.cnt[3 * cs + sg] = 0;
+    w.cnt[4 * cs + sg] = 0;
+    w.cnt[5 * cs + sg] = 0;
     stat_add(w.stats, S_CLOSEST, (unsigned long long)n);
   }
   if (p >= live_end) return;
@@ -163,24 +167,36 @@ __global__ void k_between(Wave w, int nxt) {
 }
 
 // ---------------------------------------------------------------- shade
+// Bounce b: the w.shade_parts blocks of segment sg shade consecutive parts
+// of its live paths (set b % 3 lengths) and append to the segment's next
+// path, closest-ray and shadow-ray queues (set (b + 1) % 3 lengths) with one
+// device atomic per queue per 256 paths.  Several blocks per segment keep
+// segments long -- the traversal's rays come from neighbouring pixels --
+// while the shading launch still has enough blocks to balance the GPU.
 template <bool MULTI>
-// Block sg shades the live paths of segment sg and writes the segment's
-// next path, closest-ray and shadow-ray queues.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_shade(const Job *jobs, Wave w, int cur) {
-  __shared__ int lcnt[3];  // next path queue, closest rays, shadow rays
-  const int nxt = cur ^ 1;
-  const int sg = xcd_block();
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_shade(const Job *jobs, Wave w, int b) {
+  __shared__ int lw[4][3];  // per wave64: its appends, then their offsets in the block's range
+  __shared__ int lbase[3];  // the block's ranges in the three queues
+  const int cur = b & 1, nxt = cur ^ 1;
+  const int K = w.shade_parts, cs = w.cnt_stride;
+  const int lb = xcd_block();
+  const int sg = lb / K, part = lb - sg * K;
   const int seg0 = sg * w.seg;
-  const int count = w.cnt_path[cur][sg];
+  int *const cnt_cur = w.cnt + 3 * cs * (b % 3);
+  int *const cnt_nxt = w.cnt + 3 * cs * ((b + 1) % 3);
+  if (part == 0 && threadIdx.x < 3) w.cnt[3 * cs * ((b + 2) % 3) + threadIdx.x * cs + sg] = 0;
+  const int count = cnt_cur[sg];
+  const int per = ((count + K - 1) / K + 255) / 256 * 256;
+  const int p_begin = min(count, part * per), p_end = min(count, p_begin + per);
   const int N = w.N;
   const Job &J = jobs[MULTI ? w.seg_job[sg] : 0];  // the segment's job
   const DScene &sc = J.sc;
-  if (threadIdx.x < 3) lcnt[threadIdx.x] = 0;
-  __syncthreads();
-  uint32_t n_done = 0, n_mis = 0;
-  for (int base = 0; base < count; base += blockDim.x) {
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint32_t n_done = 0, n_mis = 0, n_ray = 0, n_shadow = 0;
+  for (int base = p_begin; base < p_end; base += blockDim.x) {
     int i = base + threadIdx.x;
-    bool act = i < count;
+    bool act = i < p_end;
     int p = act ? w.qpath[cur][seg0 + i] : 0;
     bool want_ray0 = false, want_ray1 = false, want_shadow = false, keep = false, done = false;
     if (act) {
@@ -253,16 +269,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       keep = (fl & (F_ALIVE | F_PEND)) != 0;
       done = !keep;
     }
-    int a = lds_append(&lcnt[0], keep);
-    if (keep) w.qpath[nxt][seg0 + a] = p;
-    a = lds_append(&lcnt[1], want_ray0);
-    if (want_ray0) w.qray[2 * seg0 + a] = p;
-    a = lds_append(&lcnt[1], want_ray1);
-    if (want_ray1) w.qray[2 * seg0 + a] = N + p;
-    a = lds_append(&lcnt[2], want_shadow);
-    if (want_shadow) w.qshadow[seg0 + a] = p;
+    // appends: per wave64 ballots, per block one atomic per queue
+    const uint64_t mk = __ballot(keep), m0 = __ballot(want_ray0), m1 = __ballot(want_ray1),
+                   ms = __ballot(want_shadow);
+    if (lane == 0) {
+      lw[wv][0] = __popcll(mk);
+      lw[wv][1] = __popcll(m0) + __popcll(m1);
+      lw[wv][2] = __popcll(ms);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      const int q = threadIdx.x;
+      int tot = 0;
+      for (int v = 0; v < (int)(blockDim.x >> 6); ++v) {
+        const int c = lw[v][q];
+        lw[v][q] = tot;
+        tot += c;
+      }
+      lbase[q] = tot ? atomicAdd(&cnt_nxt[q * cs + sg], tot) : 0;
+      if (q == 1) n_ray += tot;
+      if (q == 2) n_shadow += tot;
+    }
+    __syncthreads();
+    if (keep) w.qpath[nxt][seg0 + lbase[0] + lw[wv][0] + __popcll(mk & below)] = p;
+    const int r0 = 2 * seg0 + lbase[1] + lw[wv][1];
+    if (want_ray0) w.qray[r0 + __popcll(m0 & below)] = p;
+    if (want_ray1) w.qray[r0 + __popcll(m0) + __popcll(m1 & below)] = N + p;
+    if (want_shadow) w.qshadow[seg0 + lbase[2] + lw[wv][2] + __popcll(ms & below)] = p;
     n_done += done ? 1 : 0;
     n_mis += want_ray1 ? 1 : 0;
+    __syncthreads();  // lw / lbase are rewritten by the next step
   }
   // block totals: queue lengths of the segment and striped statistics
   for (int off = 32; off > 0; off >>= 1) {
@@ -277,12 +313,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     atomicAdd(&ltot[1], n_mis);
   }
   __syncthreads();
+  if (threadIdx.x == 1) stat_add(w.stats, S_CLOSEST, (unsigned long long)n_ray);
+  if (threadIdx.x == 2) stat_add(w.stats, S_SHADOW, (unsigned long long)n_shadow);
   if (threadIdx.x == 0) {
-    w.cnt_path[nxt][sg] = lcnt[0];
-    w.cnt_ray[sg] = lcnt[1];
-    w.cnt_shadow[sg] = lcnt[2];
-    stat_add(w.stats, S_CLOSEST, (unsigned long long)lcnt[1]);
-    stat_add(w.stats, S_SHADOW, (unsigned long long)lcnt[2]);
     stat_add(w.stats, S_PATHS, ltot[0]);
     stat_add(w.stats, S_REFEQ, ltot[1]);
   }
@@ -437,10 +470,10 @@ void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk,
 void launch_between(const Wave &w, int nxt, hipStream_t st) {
   hipLaunchKernelGGL(k_between, dim3(1), dim3(64), 0, st, w, nxt);
 }
-void launch_shade(const Job *jobs, const Wave &w, int cur, bool multi, hipStream_t st) {
-  const int blocks = w.nseg;  // one block per segment
-  if (multi) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, st, jobs, w, cur);
-  else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, st, jobs, w, cur);
+void launch_shade(const Job *jobs, const Wave &w, int bounce, bool multi, hipStream_t st) {
+  const int blocks = w.nseg * w.shade_parts;
+  if (multi) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, st, jobs, w, bounce);
+  else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, st, jobs, w, bounce);
 }
 void launch_film(const Job *jobs, const Wave &w, int max_pix, bool multi, hipStream_t st) {
   int blocks = (max_pix + 255) / 256;

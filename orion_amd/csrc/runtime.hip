@@ -83,7 +83,7 @@ int WaveBuffers::ensure(int N) {
   // pending 15, queues 2 + 2 + 1
   const size_t per = 21 + 8 + 1 + 9 + 15 + 5;
   const size_t nseg = ((size_t)N + kSegMin - 1) / kSegMin + 1;
-  int rc = mem.ensure(per * (size_t)N * 4 + 4 * nseg * 4 + 256);
+  int rc = mem.ensure(per * (size_t)N * 4 + 9 * nseg * 4 + 256);
   if (rc) return rc;
   cap = N;
   char *b = (char *)mem.ptr;
@@ -110,8 +110,11 @@ int WaveBuffers::ensure(int N) {
   w.qray = (int *)take(2 * (size_t)N);
   w.qshadow = (int *)take(N);
   choose_segments(N, &w.seg, &w.nseg);
-  w.cnt_path[0] = (int *)take(nseg); w.cnt_path[1] = (int *)take(nseg);
-  w.cnt_ray = (int *)take(nseg); w.cnt_shadow = (int *)take(nseg);
+  w.cnt = (int *)take(9 * nseg);
+  w.cnt_stride = (int)nseg;
+  w.cnt_ray = w.cnt + nseg;
+  w.cnt_shadow = w.cnt + 2 * nseg;
+  w.shade_parts = 1;
   return ORION_OK;
 }
 

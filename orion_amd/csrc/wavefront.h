@@ -106,8 +106,16 @@ struct Wave {
   // queue segments: `seg` paths per segment (closest queues hold 2 seg
   // entries per segment), nseg segments in use, lengths per segment
   int seg, nseg;
-  int *cnt_path[2];
-  int *cnt_ray, *cnt_shadow;
+  int *cnt_ray, *cnt_shadow;  // the lengths a traversal launch reads
+  // Path pipeline queue lengths, three rotating sets: bounce b traces the
+  // ray lengths of set b % 3, shades the path lengths of set b % 3, appends
+  // to set (b + 1) % 3 and zeroes set (b + 2) % 3 (nothing reads it during
+  // bounce b).  Set t: path lengths at cnt + 3 t cnt_stride, closest-ray
+  // lengths cnt_stride after them, shadow lengths 2 cnt_stride after.
+  int *cnt;
+  int cnt_stride;
+  int shade_parts;  // shading blocks per segment
+
   const int *seg_job;  // multi-job waves: job index of each segment
 };
 

@@ -110,8 +110,9 @@ struct WaveRunner {
     if (nseg <= max_segs) return ORION_OK;
     int cap = std::max(nseg, 2 * max_segs);
     if (h_segjob) { HIP_TRY(hipEventSynchronize(staged)); (void)hipHostFree(h_segjob); }
-    HIP_TRY(hipHostMalloc((void **)&h_segjob, sizeof(int) * cap, hipHostMallocDefault));
-    int rc = segjob_dev.ensure(sizeof(int) * cap);
+    // job of each segment, then the segment order (seg_perm)
+    HIP_TRY(hipHostMalloc((void **)&h_segjob, 2 * sizeof(int) * cap, hipHostMallocDefault));
+    int rc = segjob_dev.ensure(2 * sizeof(int) * cap);
     if (rc) return rc;
     max_segs = cap;
     drop_graphs();
@@ -208,10 +209,10 @@ struct WaveRunner {
   }
 
   // Shading blocks per segment: enough that a launch has ~kShadeBlocks
-  // blocks to balance over the CUs, each part at least 1024 paths.
+  // blocks to balance over the CUs, each part at least 256 paths.
   static int shade_parts(int seg, int nseg) {
     int k = 1;
-    while (2 * k * 1024 <= seg && (int64_t)nseg * k < kShadeBlocks) k *= 2;
+    while (2 * k * 256 <= seg && (int64_t)nseg * k < kShadeBlocks) k *= 2;
     return k;
   }
   static constexpr int kShadeBlocks = 8192;
@@ -238,11 +239,21 @@ struct WaveRunner {
       const int end = j + 1 < n ? jobs[j + 1].path0 / seg : nseg;
       for (; sg < end; ++sg) h_segjob[sg] = j;
     }
+    // segment order: for each XCD x, the x-th eighth of every job's segments
+    int *perm = h_segjob + nseg, np = 0;
+    for (int x = 0; x < kXcds; ++x)
+      for (int j = 0; j < n; ++j) {
+        const int a = jobs[j].path0 / seg, e = j + 1 < n ? jobs[j + 1].path0 / seg : nseg;
+        const int len = e - a;
+        for (int sg = a + (int)((int64_t)x * len / kXcds); sg < a + (int)((int64_t)(x + 1) * len / kXcds); ++sg)
+          perm[np++] = sg;
+      }
+    if (np != nseg) { set_error("segment order does not cover the wave"); return ORION_ERR_ARG; }
     h_ctrl[0] = npaths;
     h_ctrl[1] = npix;
     h_ctrl[2] = n;
     HIP_TRY(hipMemcpyAsync(jobs_dev.ptr, h_jobs, sizeof(Job) * n, hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(segjob_dev.ptr, h_segjob, sizeof(int) * nseg, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(segjob_dev.ptr, h_segjob, 2 * sizeof(int) * nseg, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync((int *)ctrl.ptr + C_NPATHS, h_ctrl, 3 * sizeof(int), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(staged, st));
     const Job *dj = (const Job *)jobs_dev.ptr;
@@ -251,6 +262,7 @@ struct WaveRunner {
     w.nseg = nseg;
     w.shade_parts = shade_parts(seg, nseg);
     w.seg_job = (const int *)segjob_dev.ptr;
+    w.seg_perm = (const int *)segjob_dev.ptr + nseg;
     if (!use_graph || tm) {
       enqueue_wave(dj, w, depth, ring, multi, chunk, split, st, tm);
       HIP_TRY(hipGetLastError());

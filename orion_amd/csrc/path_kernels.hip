@@ -110,10 +110,11 @@ __device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int 
   // the wave's index is uniform: read from lane 0 so that everything derived
   // from it (segment, job, scene pointers) stays in scalar registers
   const int gw = lb * (kTraceBlock / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int sg = gw / wps;
-  if (sg >= w.nseg) return;
+  const int ls = gw / wps;
+  if (ls >= w.nseg) return;
+  const int sg = MULTI ? w.seg_perm[ls] : ls;
   const int count = (ANY ? w.cnt_shadow : w.cnt_ray)[sg];
-  const int begin = (gw - sg * wps) * chunk;
+  const int begin = (gw - ls * wps) * chunk;
   if (begin >= count) return;
   TravStack<RING> stk;
   stk.ring = ring + threadIdx.x;
@@ -180,7 +181,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   const int cur = b & 1, nxt = cur ^ 1;
   const int K = w.shade_parts, cs = w.cnt_stride;
   const int lb = xcd_block();
-  const int sg = lb / K, part = lb - sg * K;
+  const int ls = lb / K, part = lb - ls * K;
+  const int sg = MULTI ? w.seg_perm[ls] : ls;
   const int seg0 = sg * w.seg;
   int *const cnt_cur = w.cnt + 3 * cs * (b % 3);
   int *const cnt_nxt = w.cnt + 3 * cs * ((b + 1) % 3);

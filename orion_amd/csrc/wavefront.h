@@ -44,12 +44,13 @@ constexpr size_t kStatBytes = (size_t)S_COUNT * kStatStripes * 8;
 // queues with their own lengths, filled by the one block that shades the
 // segment through LDS counters, so appends never contend on a shared device
 // counter.  seg is a power of two in [kSegMin, kSegMax] chosen per wave:
-// large enough that a traversal wave's rays come from neighbouring pixels,
-// small enough that a wave still has about 2048 shading blocks.
+// as large as leaves every XCD a segment (long segments keep a traversal
+// wave's rays from neighbouring pixels, in one XCD's L2); shading balances
+// by giving a segment several blocks (Wave::shade_parts).
 constexpr int kSegMin = 512, kSegMax = 16384;
 inline void choose_segments(int npaths, int *seg, int *nseg) {
   int s = kSegMin;
-  while (s < kSegMax && npaths / (2 * s) >= 2048) s *= 2;
+  while (s < kSegMax && npaths / (2 * s) >= 8) s *= 2;
   *seg = s;
   *nseg = npaths > 0 ? (npaths + s - 1) / s : 1;
 }
@@ -117,6 +118,12 @@ struct Wave {
   int shade_parts;  // shading blocks per segment
 
   const int *seg_job;  // multi-job waves: job index of each segment
+  // multi-job waves: the segment behind each logical segment index.  Blocks
+  // map to logical indices XCD-contiguously (xcd_remap); the order deals
+  // every job's segments to the eight XCDs in eight contiguous parts, so
+  // each XCD gets a share of every job (balanced) and neighbouring segments
+  // (one L2).  One-job waves use the identity.
+  const int *seg_perm;
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

@@ -7,7 +7,8 @@
 // previous bounce (so L accumulates in the reference's order: emission of
 // bounce b, direct light of bounce b, emission of bounce b + 1, ...), then
 // shades the new hit: emission, light sampling, BSDF sampling, Russian
-// roulette.  Queues are compacted with wave64 ballots (one atomic per wave).
+// roulette.  Queues are compacted with wave64 ballots, one atomic per block
+// step on the queue segment's own counters.
 //
 // A wave may pack several jobs (device job table, wavefront.h): every path
 // carries its job index and reads that job's scene, camera, depth and film.

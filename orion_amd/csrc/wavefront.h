@@ -41,16 +41,17 @@ constexpr size_t kStatBytes = (size_t)S_COUNT * kStatStripes * 8;
 
 // Queue segments: path ids [s * seg, (s + 1) * seg) form segment s.  Each
 // segment has its own path, closest-ray (2 seg entries) and shadow-ray
-// queues with their own lengths, filled by the one block that shades the
-// segment through LDS counters, so appends never contend on a shared device
-// counter.  seg is a power of two in [kSegMin, kSegMax] chosen per wave:
-// as large as leaves every XCD a segment (long segments keep a traversal
-// wave's rays from neighbouring pixels, in one XCD's L2); shading balances
-// by giving a segment several blocks (Wave::shade_parts).
+// queues with their own lengths, filled by the blocks that shade the
+// segment (one device atomic per queue per 256 paths, on the segment's own
+// counters: appends never contend on one shared counter).  seg is kSegMin or kSegMax, chosen per wave: waves of 2^21
+// paths or more take long segments (a traversal wave's rays then come from
+// neighbouring pixels, in one XCD's L2: 4M paths per wave ran 25% faster
+// at 16384 than at 512-1024), smaller waves short ones (at 2^18 paths 16
+// long segments ran 25% slower than 512 short ones).  Shading balances by
+// giving a segment several blocks (Wave::shade_parts).
 constexpr int kSegMin = 512, kSegMax = 16384;
 inline void choose_segments(int npaths, int *seg, int *nseg) {
-  int s = kSegMin;
-  while (s < kSegMax && npaths / (2 * s) >= 8) s *= 2;
+  const int s = npaths < (1 << 21) ? kSegMin : kSegMax;
   *seg = s;
   *nseg = npaths > 0 ? (npaths + s - 1) / s : 1;
 }

@@ -496,6 +496,26 @@ static std::vector<Job> compose_wave(std::vector<JobState *> &act, int target, i
   return emit_wave(act, order, *taken, seg, npaths, npix, depth, stack);
 }
 
+// compose_wave with the queue segment the wave's size calls for
+// (choose_segments): composed at kSegMax first, and again at kSegMin when
+// the wave came out under 2^21 paths.  A tuning that fixes the segment
+// (fixed = true) keeps `seg`.
+static std::vector<Job> compose_sized(std::vector<JobState *> &act, int target, int seg, bool fixed,
+                                      WavePolicy policy, int *rr, int *npaths, int *npix, int *depth, int *stack,
+                                      std::vector<int> *taken, int *seg_out) {
+  const int rr0 = *rr;
+  *seg_out = fixed ? seg : kSegMax;
+  std::vector<Job> jobs = compose_wave(act, target, *seg_out, policy, rr, npaths, npix, depth, stack, taken);
+  int s2, n2;
+  choose_segments(*npaths, &s2, &n2);
+  if (!fixed && !jobs.empty() && s2 != *seg_out) {
+    *rr = rr0;
+    *seg_out = s2;
+    jobs = compose_wave(act, target, s2, policy, rr, npaths, npix, depth, stack, taken);
+  }
+  return jobs;
+}
+
 static int check_desc(const orion_job_desc &d) {
   if (!d.scene || d.width <= 0 || d.height <= 0 || d.spp < 0 || d.max_depth < 0 || d.max_depth > 60) {
     set_error("bad job description");
@@ -555,10 +575,12 @@ extern "C" int orion_render_batch(orion_ctx c, int n, const orion_job_desc *desc
   for (;;) {
     int npaths, npix, depth, stack;
     std::vector<int> taken;
-    std::vector<Job> jobs = compose_wave(act, target, seg, kPack, &rr, &npaths, &npix, &depth, &stack, &taken);
+    int wseg;
+    std::vector<Job> jobs = compose_sized(act, target, seg, tune && tune->queue_segment > 0, kPack, &rr, &npaths,
+                                          &npix, &depth, &stack, &taken, &wseg);
     if (jobs.empty()) break;
     if ((rc = R.run(c->stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, ring_of(tune), stack,
-                    jobs.size() > 1, chunk_of(tune), split_of(tune), graph, tm.on ? &tm : nullptr, seg)))
+                    jobs.size() > 1, chunk_of(tune), split_of(tune), graph, tm.on ? &tm : nullptr, wseg)))
       return rc;
     for (size_t i = 0; i < act.size(); ++i) act[i]->next_k += taken[i];
     act.erase(std::remove_if(act.begin(), act.end(), [](JobState *j) { return j->next_k >= j->d.spp; }), act.end());
@@ -699,13 +721,15 @@ void orion_server_s::loop() {
     }
     act.swap(sorted_act);
     owners.swap(sorted_owners);
-    std::vector<Job> jobs = compose_wave(act, target, seg, policy, &rr, &npaths, &npix, &depth, &stack, &taken);
+    int wseg;
+    std::vector<Job> jobs = compose_sized(act, target, seg, tn && tn->queue_segment > 0, policy, &rr, &npaths, &npix,
+                                          &depth, &stack, &taken, &wseg);
     auto w0 = std::chrono::steady_clock::now();
     int rc = runner.init(std::max(1, (int)jobs.size()));
     if (!rc) rc = runner.reset_stats(stream);
     if (!rc && !jobs.empty())
       rc = runner.run(stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, ring_of(tn), stack,
-                      jobs.size() > 1, chunk_of(tn), split_of(tn), tn && tn->use_graph, nullptr, seg);
+                      jobs.size() > 1, chunk_of(tn), split_of(tn), tn && tn->use_graph, nullptr, wseg);
     if (!rc && hipStreamSynchronize(stream) != hipSuccess) rc = ORION_ERR_HIP;
     orion_render_stats ws{};
     if (!rc) rc = runner.read_stats(&ws);

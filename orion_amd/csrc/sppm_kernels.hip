@@ -65,12 +65,27 @@ struct Grid {
   int hash_size;
 };
 
+// Iteration i's camera pass runs on its own stream beside iteration i - 1's
+// photon pass (it depends on nothing the photon pass, the update or the
+// film touch), so it has its own wave buffers and traversal spill area;
+// visible points alternate between two sets (px[i % 2]) and the film reads
+// a snapshot of Ld taken when the camera pass ends (ld_snap[i % 2]).
 struct SppmBuffers {
-  DevBuf pix_mem, grid_mem, pair_mem, hist_mem, scan_mem, job, spill;
-  Pix px{};
+  DevBuf pix_mem, grid_mem, pair_mem, hist_mem, scan_mem, job, spill, spill_cam;
+  WaveBuffers camw;
+  Pix px[2]{};
+  float *ld_snap[2] = {nullptr, nullptr};  // 3 x npix each (r, g, b planes)
   Grid g{};
   int npix = 0, hash_size = 0;
   size_t pair_cap = 0, hist_cap = 0, scan_cap = 0;
+  hipStream_t cam_stream = nullptr;
+  hipEvent_t cam_done[2] = {nullptr, nullptr}, iter_done = nullptr;
+  ~SppmBuffers() {
+    if (cam_stream) { (void)hipStreamSynchronize(cam_stream); (void)hipStreamDestroy(cam_stream); }
+    for (hipEvent_t e : cam_done)
+      if (e) (void)hipEventDestroy(e);
+    if (iter_done) (void)hipEventDestroy(iter_done);
+  }
 };
 
 void destroy_sppm(SppmBuffers *b) { delete b; }
@@ -692,25 +707,37 @@ static int ensure_buffers(orion_scene s, int hash_size) {
   SppmBuffers &b = *s->sppm;
   int npix = s->W * s->H;
   if (b.npix != npix) {
-    size_t per = 12 + 6 + 12 + 1 + 3;  // pixel state, vp p/wo, frame, mat, beta
+    // pixel state, two visible-point sets (p/wo, frame, mat, beta), two Ld snapshots
+    size_t per = 12 + 2 * (6 + 12 + 1 + 3) + 2 * 3;
     int rc = b.pix_mem.ensure(per * (size_t)npix * 4 + 256);
     if (rc) return rc;
     char *m = (char *)b.pix_mem.ptr;
     size_t off = 0;
     auto take = [&](size_t n) { void *q = m + off; off += n * 4; return q; };
-    Pix &p = b.px;
+    Pix &p = b.px[0];
     p.radius = (float *)take(npix); p.N = (float *)take(npix);
     p.Ldr = (float *)take(npix); p.Ldg = (float *)take(npix); p.Ldb = (float *)take(npix);
     p.taur = (float *)take(npix); p.taug = (float *)take(npix); p.taub = (float *)take(npix);
     p.Phir = (float *)take(npix); p.Phig = (float *)take(npix); p.Phib = (float *)take(npix);
     p.M = (int *)take(npix);
-    p.vpx = (float *)take(npix); p.vpy = (float *)take(npix); p.vpz = (float *)take(npix);
-    p.vwox = (float *)take(npix); p.vwoy = (float *)take(npix); p.vwoz = (float *)take(npix);
-    p.vng = (float *)take(3 * (size_t)npix); p.vns = (float *)take(3 * (size_t)npix);
-    p.vu = (float *)take(3 * (size_t)npix); p.vv = (float *)take(3 * (size_t)npix);
-    p.vmat = (int *)take(npix);
-    p.vbr = (float *)take(npix); p.vbg = (float *)take(npix); p.vbb = (float *)take(npix);
+    b.px[1] = p;
+    for (Pix &q : b.px) {
+      q.vpx = (float *)take(npix); q.vpy = (float *)take(npix); q.vpz = (float *)take(npix);
+      q.vwox = (float *)take(npix); q.vwoy = (float *)take(npix); q.vwoz = (float *)take(npix);
+      q.vng = (float *)take(3 * (size_t)npix); q.vns = (float *)take(3 * (size_t)npix);
+      q.vu = (float *)take(3 * (size_t)npix); q.vv = (float *)take(3 * (size_t)npix);
+      q.vmat = (int *)take(npix);
+      q.vbr = (float *)take(npix); q.vbg = (float *)take(npix); q.vbb = (float *)take(npix);
+    }
+    b.ld_snap[0] = (float *)take(3 * (size_t)npix);
+    b.ld_snap[1] = (float *)take(3 * (size_t)npix);
+    if ((rc = b.camw.ensure(npix))) return rc;
     b.npix = npix;
+  }
+  if (!b.cam_stream) {
+    HIP_TRY(hipStreamCreateWithFlags(&b.cam_stream, hipStreamNonBlocking));
+    for (hipEvent_t &e : b.cam_done) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&b.iter_done, hipEventDisableTiming));
   }
   if (b.hash_size != hash_size || !b.grid_mem.ptr) {
     size_t ints = 64 + (size_t)(npix + 1) + 3 * (size_t)hash_size;
@@ -812,50 +839,76 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   HIP_TRY(hipMemsetAsync(s->stats.ptr, 0, kStatBytes, st));
   HIP_TRY(hipEventRecord(c->ev0, st));
   const int TB = 256;
-  hipLaunchKernelGGL(k_pix_init, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, b.px, npix, r0);
+  for (const Pix &q : b.px) hipLaunchKernelGGL(k_pix_init, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, q, npix, r0);
   // SPPM's launches trace at most 2^20 rays: 64-entry chunks give such a launch
   // enough waves to fill the GPU (measured 10% faster per iteration than 256)
   const int chunk = 64;
-  {
-    int blocks = trace_pair_grid(1, npix, chunk);  // the camera pass traces both queues in one launch
-    if (photons > 0) blocks = std::max(blocks, trace_grid(1, photons, false, chunk));
-    if ((rc = b.spill.ensure(trace_spill_ints(s->d.stack_bound, ring, blocks) * 4 + 4))) return rc;
-  }
+  if ((rc = b.spill.ensure(trace_spill_ints(s->d.stack_bound, ring, trace_grid(1, std::max(photons, 1), false, chunk)) *
+                               4 + 4)))
+    return rc;
+  // the camera pass traces both its queues in one launch
+  if ((rc = b.spill_cam.ensure(trace_spill_ints(s->d.stack_bound, ring, trace_pair_grid(1, npix, chunk)) * 4 + 4)))
+    return rc;
   w.spill = (int *)b.spill.ptr;
   w.spill_cap = std::max(0, s->d.stack_bound - ring);
   const float nIterF = (float)n_iters;
   const float NpF = (float)((uint64_t)n_iters * (uint64_t)photons);
   const bool want_dbg = dbg && dbg->debug_iter >= 0 && dbg->debug_iter < n_iters;
-  for (int iter = 0; iter < n_iters; ++iter) {
-    // --- camera pass: the wave's radiance arrays are the pixels' Ld
-    Wave cw = w;
-    cw.N = npix;
-    cw.seg = npix;  // one queue segment; its lengths live in the control block
-    cw.nseg = 1;
-    cw.cnt_ray = w.ctrl + C_RAY;
-    cw.cnt_shadow = w.ctrl + C_SHADOW;
-    cw.Lr = b.px.Ldr; cw.Lg = b.px.Ldg; cw.Lb = b.px.Ldb;
-    hipLaunchKernelGGL(k_cam_raygen, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, s->d, cw, b.px, iter);
+  // camera pass of iteration `ci` (sppm.cpp:84-140) on the camera stream: the
+  // wave's radiance arrays are the pixels' Ld; it waits for iteration ci - 2
+  // to have finished with visible-point set ci % 2 and Ld snapshot ci % 2
+  Wave cw = b.camw.wave;
+  cw.ctrl = w.ctrl;
+  cw.stats = w.stats;
+  cw.npix = npix;
+  cw.W = s->W;
+  cw.H = s->H;
+  cw.k0 = 0;
+  cw.spill = (int *)b.spill_cam.ptr;
+  cw.spill_cap = w.spill_cap;
+  cw.N = npix;
+  cw.seg = npix;  // one queue segment; its lengths live in the control block
+  cw.nseg = 1;
+  cw.cnt_ray = w.ctrl + C_RAY;
+  cw.cnt_shadow = w.ctrl + C_SHADOW;
+  cw.Lr = b.px[0].Ldr; cw.Lg = b.px[0].Ldg; cw.Lb = b.px[0].Ldb;
+  hipStream_t cst = b.cam_stream;
+  HIP_TRY(hipEventRecord(b.iter_done, st));
+  auto camera_pass = [&](int ci) -> int {
+    const Pix pc = b.px[ci & 1];
+    HIP_TRY(hipStreamWaitEvent(cst, b.iter_done, 0));
+    hipLaunchKernelGGL(k_cam_raygen, dim3((npix + TB - 1) / TB), dim3(TB), 0, cst, s->d, cw, pc, ci);
     int cur = 0;
     for (int it = 0; it <= maxDepth; ++it) {
-      launch_trace_pair(dj, cw, ring, chunk, false, st);  // closest-hit + shadow queues in one launch
-      launch_between(cw, cur ^ 1, st);
+      launch_trace_pair(dj, cw, ring, chunk, false, cst);  // closest-hit + shadow queues in one launch
+      launch_between(cw, cur ^ 1, cst);
       int blocks = std::min((npix + 255) / 256, 8192);
-      hipLaunchKernelGGL(k_cam_shade, dim3(blocks), dim3(256), 0, st, s->d, cw, b.px, cur, maxDepth, iter);
+      hipLaunchKernelGGL(k_cam_shade, dim3(blocks), dim3(256), 0, cst, s->d, cw, pc, cur, maxDepth, ci);
       cur ^= 1;
     }
+    // Ld r, g, b planes are contiguous
+    HIP_TRY(hipMemcpyAsync(b.ld_snap[ci & 1], b.px[0].Ldr, (size_t)3 * npix * 4, hipMemcpyDeviceToDevice, cst));
+    HIP_TRY(hipEventRecord(b.cam_done[ci & 1], cst));
+    return ORION_OK;
+  };
+  if ((rc = camera_pass(0))) return rc;
+  for (int iter = 0; iter < n_iters; ++iter) {
+    // the next iteration's camera pass overlaps this one's grid and photon pass
+    if (iter + 1 < n_iters && (rc = camera_pass(iter + 1))) return rc;
+    HIP_TRY(hipStreamWaitEvent(st, b.cam_done[iter & 1], 0));
+    const Pix &px = b.px[iter & 1];
     // --- grid
     hipLaunchKernelGGL(k_grid_init, dim3(1), dim3(64), 0, st, g);
-    hipLaunchKernelGGL(k_grid_bounds, dim3(std::min((npix + 255) / 256, 256)), dim3(256), 0, st, b.px, g, npix);
+    hipLaunchKernelGGL(k_grid_bounds, dim3(std::min((npix + 255) / 256, 256)), dim3(256), 0, st, px, g, npix);
     hipLaunchKernelGGL(k_grid_res, dim3(1), dim3(64), 0, st, g);
-    hipLaunchKernelGGL(k_grid_count, dim3((npix + 1 + TB - 1) / TB), dim3(TB), 0, st, b.px, g, npix);
+    hipLaunchKernelGGL(k_grid_count, dim3((npix + 1 + TB - 1) / TB), dim3(TB), 0, st, px, g, npix);
     device_scan(g.cnt, g.cnt, npix + 1, g.scan_tmp, st);
     int n_pairs = 0;
     HIP_TRY(hipMemcpyAsync(&n_pairs, g.cnt + npix, 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (n_pairs < 0) { set_error("visible-point cell pairs overflow"); return ORION_ERR_UNSUPPORTED; }
     if ((rc = ensure_pairs(b, (size_t)n_pairs + 1))) return rc;
-    hipLaunchKernelGGL(k_grid_emit, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, b.px, g, npix);
+    hipLaunchKernelGGL(k_grid_emit, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, px, g, npix);
     int bits = 1;
     while ((1 << bits) < hash_size) ++bits;
     int ntiles = (n_pairs + kSortTile - 1) / kSortTile;
@@ -918,7 +971,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
         tw.qray = qbuf[depth & 1];
         tw.cnt_ray = w.ctrl + C_PQ + depth % 3;
         launch_trace(dj, tw, false, ring, chunk, false, st);
-        hipLaunchKernelGGL(k_photon_shade, dim3(blocks), dim3(256), 0, st, s->d, tw, b.px, gi, index0, depth,
+        hipLaunchKernelGGL(k_photon_shade, dim3(blocks), dim3(256), 0, st, s->d, tw, px, gi, index0, depth,
                            qbuf[depth & 1], qbuf[(depth + 1) & 1], dstats);
       }
     }
@@ -931,13 +984,18 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
       if ((rc = mdbg.ensure((size_t)npix * 4))) return rc;
       M_out = (int *)mdbg.ptr;
     }
-    hipLaunchKernelGGL(k_pixel_update, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, b.px, npix, M_out);
-    hipLaunchKernelGGL(k_sppm_film, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, b.px, npix, nIterF, NpF,
+    hipLaunchKernelGGL(k_pixel_update, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, px, npix, M_out);
+    Pix pf = px;  // Ld as the camera pass left it
+    pf.Ldr = b.ld_snap[iter & 1];
+    pf.Ldg = pf.Ldr + npix;
+    pf.Ldb = pf.Ldg + npix;
+    hipLaunchKernelGGL(k_sppm_film, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, pf, npix, nIterF, NpF,
                        (float *)s->film.ptr, (int *)s->count.ptr);
     if (M_out) {
       HIP_TRY(hipMemcpyAsync(dbg->M, M_out, (size_t)npix * 4, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
     }
+    HIP_TRY(hipEventRecord(b.iter_done, st));
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipEventRecord(c->ev1, st));

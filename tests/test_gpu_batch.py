@@ -48,6 +48,31 @@ def test_batch_alone_equals_mix(device, scenes):
             np.testing.assert_array_equal(bits(argb), bits(rgb))
 
 
+def test_long_segment_waves_equal_short(device, scenes):
+    """Waves of 2^21 paths or more use 16384-path queue segments shaded by
+    several blocks each, and a multi-job wave deals every job's segments to
+    all XCDs; smaller waves use 512-path segments and one block per segment.
+    Films must not depend on which."""
+    jobs = [(scenes["gold"], 768, 512, 6, 5), (scenes["glass"], 640, 640, 5, 8),
+            (scenes["lambert"], 512, 384, 8, 6)]
+    short = [oa.render_batch(device, [j], wave_paths=1 << 18)[0][0] for j in jobs]
+    for wp in (0, 1 << 22):  # one wave of all three jobs; waves of ~4M paths mixing two
+        mixed, st = oa.render_batch(device, jobs, wave_paths=wp)
+        assert st.paths == sum(W * H * spp for _, W, H, spp, _ in jobs)
+        for (rgb, cnt), (srgb, scnt), (_, W, H, spp, _) in zip(mixed, short, jobs):
+            assert (cnt == spp).all()
+            np.testing.assert_array_equal(scnt, cnt)
+            np.testing.assert_array_equal(bits(srgb), bits(rgb))
+    # one job: 300 samples of a 96x80 film in one 2.3M-path wave vs 30 per wave
+    sc = scenes["gold"]
+    sc.film_clear()
+    sc.render_path(0, 300, 5, spp_per_wave=30)
+    a = sc.film()[0].copy()
+    sc.film_clear()
+    sc.render_path(0, 300, 5, spp_per_wave=300)
+    np.testing.assert_array_equal(bits(a), bits(sc.film()[0]))
+
+
 def test_batch_job_equals_render_path(device, scenes):
     """a batched job whose film is its scene's own equals orion_render_path"""
     sc = scenes["cornell"]

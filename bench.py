@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=64, help="batched-launch leg (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=24, help="serving leg (0 = skip)")
+    ap.add_argument("--serve-policy", choices=["pack", "arrival", "fair"], default="pack",
+                    help="the server's wave order: least remaining work first, submission order, or fair share")
     ap.add_argument("--serve-load", type=float, default=0.9,
                     help="offered load of the Poisson stream, as a fraction of the plain path's "
                          "one-at-a-time capacity")
@@ -206,7 +208,7 @@ def sppm_leg(oa, dev, iters):
     }
 
 
-def serve_leg(oa, dev, n_req, load, budget_gib, seed):
+def serve_leg(oa, dev, n_req, load, budget_gib, seed, policy="pack", also=("arrival",)):
     """Poisson stream of render requests against 4 resident scenes (~8.2M
     triangles: displaced icospheres of 5.2M, 1.3M, 1.3M and 0.33M triangles,
     material drawn per mesh); each request: film sides uniform in [256, 1024],
@@ -254,27 +256,40 @@ def serve_leg(oa, dev, n_req, load, budget_gib, seed):
     lat_plain1, plain1_span = fifo(service1)
     lat_plain, plain_span = fifo(service)
     tune = oa.autotune(dev, [(ds, W, H, min(spp, 2), d) for ds, W, H, spp, d in reqs[:8]])
-    srv = oa.Server(dev, int(budget_gib * (1 << 30)), tuning=tune)
-    t0 = time.perf_counter()
-    ids = []
-    for a, r in zip(arrive, reqs):
-        dt = t0 + a - time.perf_counter()
-        if dt > 0:
-            time.sleep(dt)
-        ids.append(srv.submit(*r))
-    srv.drain()
-    lat, done = [], []
-    for jid in ids:
-        _, _, (ts, ta, tf) = srv.poll(jid, want_film=False)
-        lat.append(tf - ts)
-        done.append(tf)
-    sst = srv.stats()
-    srv.close()
-    span = max(done) - 0.0
+
+    def run_server(pol):  # the same requests on the same arrival schedule
+        srv = oa.Server(dev, int(budget_gib * (1 << 30)), tuning=tune)
+        srv.set_policy({"pack": oa.Server.PACK, "arrival": oa.Server.ARRIVAL, "fair": oa.Server.FAIR}[pol])
+        t0 = time.perf_counter()
+        ids = []
+        for a, r in zip(arrive, reqs):
+            dt = t0 + a - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(*r))
+        srv.drain()
+        lat, done = [], []
+        for jid in ids:
+            _, _, (ts, ta, tf) = srv.poll(jid, want_film=False)
+            lat.append(tf - ts)
+            done.append(tf)
+        sst = srv.stats()
+        srv.close()
+        return lat, sst, max(done)
+
+    others = {}
+    for pol in also:
+        if pol != policy:
+            l2, s2, _ = run_server(pol)
+            others[pol + "_order"] = {
+                "p50_s": round(float(np.percentile(l2, 50)), 3), "p99_s": round(float(np.percentile(l2, 99)), 3),
+                "speedup_busy_vs_plain_multisample_wave": round(sum(service) / (s2["busy_ms"] / 1e3), 3),
+                "waves": s2["waves"]}
+    lat, sst, span = run_server(policy)
     return {
         "workload": f"{n_req} requests, Poisson at {qps:.2f}/s ({load:g} x multi-sample plain "
                     f"capacity), 4 resident scenes ({tris_tot} tris), films 256..1024 per side, "
-                    f"16-256 spp, depth 5-8, HBM budget {budget_gib:g} GiB",
+                    f"16-256 spp, depth 5-8, HBM budget {budget_gib:g} GiB, wave order {policy}",
         "paths": int(paths),
         "p50_s": round(float(np.percentile(lat, 50)), 3),
         "p99_s": round(float(np.percentile(lat, 99)), 3),
@@ -292,6 +307,9 @@ def serve_leg(oa, dev, n_req, load, budget_gib, seed):
         "plain_multisample_wave_span_s": round(plain_span, 3),
         "waves": sst["waves"], "mixed_waves": sst["mixed_waves"],
         "max_jobs_in_wave": sst["max_jobs_in_wave"],
+        "tuning": {"wave_paths": tune.wave_paths, "rays_per_wave": tune.rays_per_wave, "lds_ring": tune.lds_ring,
+                   "queue_segment": tune.queue_segment, "use_graph": tune.use_graph},
+        **others,
     }
 
 
@@ -407,7 +425,8 @@ def main():
         out["batched"] = batch_leg(oa, dev, a.batch_jobs, a.seed)
         print(f"# batched leg done: {out['batched']['speedup']}x", file=sys.stderr, flush=True)
     if a.serve_requests > 0:
-        out["serving"] = serve_leg(oa, dev, a.serve_requests, a.serve_load, a.hbm_budget_gib, a.seed)
+        out["serving"] = serve_leg(oa, dev, a.serve_requests, a.serve_load, a.hbm_budget_gib, a.seed,
+                                   a.serve_policy)
         print("# serving leg done", file=sys.stderr, flush=True)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(hs, a.depth, a.cpu_seconds)

@@ -271,11 +271,15 @@ int orion_render_batch(orion_ctx c, int n_jobs, const orion_job_desc *jobs, floa
 typedef struct orion_server_s *orion_server;
 int orion_server_create(orion_ctx c, int64_t hbm_budget_bytes, const orion_tuning *tune, orion_server *out);
 /* Wave composition: ORION_SERVE_PACK (default) fills each wave with the jobs
- * of least remaining work first, as many samples each as fit; ORION_SERVE_FAIR
+ * of least remaining work first, as many samples each as fit;
+ * ORION_SERVE_ARRIVAL fills it the same way in submission order (the oldest
+ * job first: no job waits behind later, smaller ones); ORION_SERVE_FAIR
  * gives every admitted job at least one sample per wave (all images refine
- * together).  Either way a job's film is bit-identical to rendering it alone. */
+ * together).  Whatever the policy a job's film is bit-identical to rendering
+ * it alone. */
 #define ORION_SERVE_PACK 0
 #define ORION_SERVE_FAIR 1
+#define ORION_SERVE_ARRIVAL 2
 int orion_server_set_policy(orion_server s, int policy);
 /* thread-safe; *job_id identifies the request */
 int orion_server_submit(orion_server s, const orion_job_desc *job, uint64_t *job_id);

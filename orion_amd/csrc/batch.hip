@@ -628,6 +628,7 @@ struct orion_server_s {
   orion_tuning tune{};
   bool has_tune = false;
   orion_amd::WavePolicy policy = orion_amd::kPack;
+  bool shortest_first = true;  // kPack order: least remaining work first, else submission order
   std::mutex mu;
   std::condition_variable cv, cv_done;
   std::deque<orion_amd::ServerJob *> queue;
@@ -696,9 +697,11 @@ void orion_server_s::loop() {
     std::vector<JobState *> act;
     std::vector<ServerJob *> owners;
     WavePolicy policy;
+    bool srw;
     {
       std::lock_guard<std::mutex> lk(mu);
       policy = this->policy;
+      srw = shortest_first;
       for (ServerJob *j : active)
         if (j->state == 1) { act.push_back(&j->st); owners.push_back(j); }
     }
@@ -711,7 +714,7 @@ void orion_server_s::loop() {
     auto remaining = [&](size_t i) {
       return (int64_t)(act[i]->d.spp - act[i]->next_k) * act[i]->d.width * act[i]->d.height;
     };
-    if (policy == kPack)
+    if (policy == kPack && srw)
       std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return remaining(a) < remaining(b); });
     std::vector<JobState *> sorted_act;
     std::vector<ServerJob *> sorted_owners;
@@ -816,9 +819,13 @@ int orion_server_create(orion_ctx c, int64_t budget, const orion_tuning *tune, o
 }
 
 int orion_server_set_policy(orion_server s, int policy) {
-  if (!s || (policy != ORION_SERVE_PACK && policy != ORION_SERVE_FAIR)) { set_error("bad argument"); return ORION_ERR_ARG; }
+  if (!s || (policy != ORION_SERVE_PACK && policy != ORION_SERVE_FAIR && policy != ORION_SERVE_ARRIVAL)) {
+    set_error("bad argument");
+    return ORION_ERR_ARG;
+  }
   std::lock_guard<std::mutex> lk(s->mu);
-  s->policy = policy == ORION_SERVE_PACK ? orion_amd::kPack : orion_amd::kFair;
+  s->policy = policy == ORION_SERVE_FAIR ? orion_amd::kFair : orion_amd::kPack;
+  s->shortest_first = policy == ORION_SERVE_PACK;
   return ORION_OK;
 }
 
@@ -945,20 +952,26 @@ int orion_autotune(orion_ctx c, int n, const orion_job_desc *jobs, orion_tuning 
     rp[j] = rgb[j].data();
     cp[j] = cnt[j].data();
   }
-  // coordinate descent: wave size, then traversal chunk, LDS ring, graph
+  // coordinate descent from the defaults: wave size, then traversal chunk,
+  // LDS ring, queue segment, graph, split launches.  A candidate replaces the
+  // current choice only when it is at least 2% faster: a cut-down mix often
+  // cannot tell two settings apart (every wave size above its path count
+  // gives the same single wave), and noise must not pick for the full run.
   double best_ms = 1e30;
   orion_tuning b{};
+  b.wave_paths = kDefaultWavePaths;
   auto trial = [&](const orion_tuning &t) -> int {
     orion_render_stats st{};
     // the first call captures graphs and sizes buffers; time the second
     int rc = orion_render_batch(c, n, trial_jobs.data(), rp.data(), cp.data(), &t, &st);
     if (!rc) rc = orion_render_batch(c, n, trial_jobs.data(), rp.data(), cp.data(), &t, &st);
     if (rc) return rc;
-    if (st.ms < best_ms) { best_ms = st.ms; b = t; }
+    if (st.ms < 0.98 * best_ms) { best_ms = st.ms; b = t; }
     return ORION_OK;
   };
   int rc;
-  for (int wp : {1 << 22, 1 << 23, 1 << 24, 1 << 25}) {
+  if ((rc = trial(b))) return rc;
+  for (int wp : {1 << 24, 1 << 23, 1 << 22}) {
     orion_tuning t{};
     t.wave_paths = wp;
     if ((rc = trial(t))) return rc;
@@ -990,8 +1003,7 @@ int orion_autotune(orion_ctx c, int n, const orion_job_desc *jobs, orion_tuning 
   }
   if (b.rays_per_wave == 0) b.rays_per_wave = kDefaultChunk;
   if (b.lds_ring == 0) b.lds_ring = kDefaultRing;
-  if (b.queue_segment == 0) b.queue_segment = seg_of(nullptr, b.wave_paths);
-  *best = b;
+  *best = b;  // queue_segment 0: chosen per wave from its size
   return ORION_OK;
 }
 

@@ -815,6 +815,9 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   orion_ctx c = s->ctx;
   hipStream_t st = c->stream;
   HIP_TRY(hipSetDevice(c->device));
+  // a call that returned an error mid-way may have left camera-pass work on
+  // the camera stream: let it finish before any buffer is resized
+  if (s->sppm && s->sppm->cam_stream) HIP_TRY(hipStreamSynchronize(s->sppm->cam_stream));
   int rc = ensure_buffers(s, hash_size);
   if (rc) return rc;
   SppmBuffers &b = *s->sppm;

@@ -189,3 +189,22 @@ def test_path_waves_equal_single(device):
     ds.render_path(0, 8, 5, spp_per_wave=3)
     b, _ = ds.film()
     np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32))
+
+
+def test_c2_full_size_waves_equal(device):
+    """The 1.31M-triangle scene at its full 1024x1024 film: 4 samples in one
+    4M-path wave (16384-path queue segments, several shading blocks per
+    segment) and one sample per wave (512-path segments) give the same film,
+    bit for bit, and every pixel counts its 4 samples."""
+    hs = oa.HostScene.generate("icosphere", seed=1, params=[8, 1024, 1024])
+    ds = oa.DeviceScene(device, hs)
+    ds.film_clear()
+    st = ds.render_path(0, 4, 8, spp_per_wave=4)
+    a, ca = ds.film()
+    assert st.paths == 4 * 1024 * 1024 and st.stack_overflows == 0
+    ds.film_clear()
+    ds.render_path(0, 4, 8, spp_per_wave=1)
+    b, cb = ds.film()
+    assert (ca == 4).all() and (cb == 4).all()
+    assert np.isfinite(a).all() and (a >= 0).all()
+    np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32))

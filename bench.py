@@ -318,7 +318,7 @@ def main():
     if a.gpus != 1:
         print(f"bench.py measures one GPU: every box in this setup has a single MI355X, so "
               f"--gpus {a.gpus} is not run (the multi-GPU design shards samples / pixels with one "
-              f"film reduction; see DESIGN.md)", file=sys.stderr)
+              f"film reduction per job)", file=sys.stderr)
         sys.exit(2)
     import orion_amd as oa
     if a.copy_calibration_mib > 0:

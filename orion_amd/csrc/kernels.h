@@ -7,7 +7,7 @@
 
 namespace orion_amd {
 
-constexpr int kTraceBlock = dev::kRingStride;  // threads per traversal block (2 wave64)
+constexpr int kTraceBlock = dev::kRingStride;  // threads per traversal block (one wave64)
 constexpr int kDefaultRing = 16;          // LDS ring entries per lane (8, 16 or 32)
 constexpr int kDefaultChunk = 256;        // queue entries per traversal wave64
 

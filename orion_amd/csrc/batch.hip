@@ -209,13 +209,14 @@ struct WaveRunner {
   }
 
   // Shading blocks per segment: enough that a launch has ~kShadeBlocks
-  // blocks to balance over the CUs, each part at least 256 paths.
+  // blocks to balance over the CUs, each part at least one block's worth of
+  // paths.
   static int shade_parts(int seg, int nseg) {
     int k = 1;
-    while (2 * k * 256 <= seg && (int64_t)nseg * k < kShadeBlocks) k *= 2;
+    while (2 * k * kShadeThreads <= seg && (int64_t)nseg * k < kShadeBlocks) k *= 2;
     return k;
   }
-  static constexpr int kShadeBlocks = 8192;
+  static constexpr int kShadeBlocks = 8192 * 256 / kShadeThreads;
 
   // Runs one wave: jobs[0..n) with path0 / pix0 already assigned.
   // seg: queue segment size (0: chosen from npaths); multi-job waves pass

@@ -10,6 +10,7 @@ namespace orion_amd {
 constexpr int kTraceBlock = dev::kRingStride;  // threads per traversal block (one wave64)
 constexpr int kDefaultRing = 16;          // LDS ring entries per lane (8, 16 or 32)
 constexpr int kDefaultChunk = 256;        // queue entries per traversal wave64
+constexpr int kShadeThreads = 256;        // threads per path-shading block
 
 // Wave launchers: `jobs` is the device job table, `multi` selects the
 // per-path job lookup (false: every path belongs to job 0).

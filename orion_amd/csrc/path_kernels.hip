@@ -176,8 +176,8 @@ __global__ void k_between(Wave w, int nxt) {
 // segments long -- the traversal's rays come from neighbouring pixels --
 // while the shading launch still has enough blocks to balance the GPU.
 template <bool MULTI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_shade(const Job *jobs, Wave w, int b) {
-  __shared__ int lw[4][3];  // per wave64: its appends, then their offsets in the block's range
+__global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_shade(const Job *jobs, Wave w, int b) {
+  __shared__ int lw[kShadeThreads / 64][3];  // per wave64: its appends, then their offsets in the block's range
   __shared__ int lbase[3];  // the block's ranges in the three queues
   const int cur = b & 1, nxt = cur ^ 1;
   const int K = w.shade_parts, cs = w.cnt_stride;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
   int *const cnt_nxt = w.cnt + 3 * cs * ((b + 1) % 3);
   if (part == 0 && threadIdx.x < 3) w.cnt[3 * cs * ((b + 2) % 3) + threadIdx.x * cs + sg] = 0;
   const int count = cnt_cur[sg];
-  const int per = ((count + K - 1) / K + 255) / 256 * 256;
+  const int per = ((count + K - 1) / K + kShadeThreads - 1) / kShadeThreads * kShadeThreads;
   const int p_begin = min(count, part * per), p_end = min(count, p_begin + per);
   const int N = w.N;
   const Job &J = jobs[MULTI ? w.seg_job[sg] : 0];  // the segment's job
@@ -475,8 +475,8 @@ void launch_between(const Wave &w, int nxt, hipStream_t st) {
 }
 void launch_shade(const Job *jobs, const Wave &w, int bounce, bool multi, hipStream_t st) {
   const int blocks = w.nseg * w.shade_parts;
-  if (multi) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, st, jobs, w, bounce);
-  else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, st, jobs, w, bounce);
+  if (multi) hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(kShadeThreads), 0, st, jobs, w, bounce);
+  else hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(kShadeThreads), 0, st, jobs, w, bounce);
 }
 void launch_film(const Job *jobs, const Wave &w, int max_pix, bool multi, hipStream_t st) {
   int blocks = (max_pix + 255) / 256;

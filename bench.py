@@ -67,8 +67,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=64, help="batched-launch leg (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=24, help="serving leg (0 = skip)")
-    ap.add_argument("--serve-policy", choices=["pack", "arrival", "fair"], default="pack",
-                    help="the server's wave order: least remaining work first, submission order, or fair share")
+    ap.add_argument("--serve-policy", choices=["pack", "aged", "arrival", "fair"], default="pack",
+                    help="the server's wave order: least remaining work first (optionally credited for "
+                         "waiting time), submission order, or fair share")
     ap.add_argument("--serve-load", type=float, default=0.9,
                     help="offered load of the Poisson stream, as a fraction of the plain path's "
                          "one-at-a-time capacity")
@@ -208,7 +209,7 @@ def sppm_leg(oa, dev, iters):
     }
 
 
-def serve_leg(oa, dev, n_req, load, budget_gib, seed, policy="pack", also=("arrival",)):
+def serve_leg(oa, dev, n_req, load, budget_gib, seed, policy="pack", also=("aged", "arrival")):
     """Poisson stream of render requests against 4 resident scenes (~8.2M
     triangles: displaced icospheres of 5.2M, 1.3M, 1.3M and 0.33M triangles,
     material drawn per mesh); each request: film sides uniform in [256, 1024],
@@ -259,7 +260,8 @@ def serve_leg(oa, dev, n_req, load, budget_gib, seed, policy="pack", also=("arri
 
     def run_server(pol):  # the same requests on the same arrival schedule
         srv = oa.Server(dev, int(budget_gib * (1 << 30)), tuning=tune)
-        srv.set_policy({"pack": oa.Server.PACK, "arrival": oa.Server.ARRIVAL, "fair": oa.Server.FAIR}[pol])
+        srv.set_policy({"pack": oa.Server.PACK, "aged": oa.Server.AGED, "arrival": oa.Server.ARRIVAL,
+                        "fair": oa.Server.FAIR}[pol])
         t0 = time.perf_counter()
         ids = []
         for a, r in zip(arrive, reqs):

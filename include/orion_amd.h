@@ -280,6 +280,10 @@ int orion_server_create(orion_ctx c, int64_t hbm_budget_bytes, const orion_tunin
 #define ORION_SERVE_PACK 0
 #define ORION_SERVE_FAIR 1
 #define ORION_SERVE_ARRIVAL 2
+/* ORION_SERVE_AGED: like PACK, but a job's remaining work is credited with
+ * the work the device did while it waited (its waiting time x the measured
+ * paths/s), so a large job cannot be overtaken indefinitely. */
+#define ORION_SERVE_AGED 3
 int orion_server_set_policy(orion_server s, int policy);
 /* thread-safe; *job_id identifies the request */
 int orion_server_submit(orion_server s, const orion_job_desc *job, uint64_t *job_id);

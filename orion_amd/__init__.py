@@ -501,7 +501,7 @@ class Server:
                       int(split_trace))
         _check(lib().orion_server_create(dev.h, int(hbm_budget_bytes), C.byref(tune), C.byref(self.h)))
 
-    PACK, FAIR, ARRIVAL = 0, 1, 2
+    PACK, FAIR, ARRIVAL, AGED = 0, 1, 2, 3
 
     def set_policy(self, policy):
         _check(lib().orion_server_set_policy(self.h, policy))

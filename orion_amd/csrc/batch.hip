@@ -630,6 +630,7 @@ struct orion_server_s {
   bool has_tune = false;
   orion_amd::WavePolicy policy = orion_amd::kPack;
   bool shortest_first = true;  // kPack order: least remaining work first, else submission order
+  bool aged = false;           // ... with remaining work credited for the time waited
   std::mutex mu;
   std::condition_variable cv, cv_done;
   std::deque<orion_amd::ServerJob *> queue;
@@ -698,11 +699,14 @@ void orion_server_s::loop() {
     std::vector<JobState *> act;
     std::vector<ServerJob *> owners;
     WavePolicy policy;
-    bool srw;
+    bool srw, age;
+    double rate;  // paths/s over the busy time so far
     {
       std::lock_guard<std::mutex> lk(mu);
       policy = this->policy;
       srw = shortest_first;
+      age = aged;
+      rate = busy_ms > 0 ? (double)paths / (busy_ms * 1e-3) : 0.0;
       for (ServerJob *j : active)
         if (j->state == 1) { act.push_back(&j->st); owners.push_back(j); }
     }
@@ -712,8 +716,11 @@ void orion_server_s::loop() {
     // packed: smallest remaining work first (shortest-remaining-time order)
     std::vector<size_t> idx(act.size());
     for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+    const double tnow = now();
     auto remaining = [&](size_t i) {
-      return (int64_t)(act[i]->d.spp - act[i]->next_k) * act[i]->d.width * act[i]->d.height;
+      double r = (double)(act[i]->d.spp - act[i]->next_k) * act[i]->d.width * act[i]->d.height;
+      if (age) r -= rate * (tnow - owners[i]->t_submit);
+      return r;
     };
     if (policy == kPack && srw)
       std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return remaining(a) < remaining(b); });
@@ -820,13 +827,14 @@ int orion_server_create(orion_ctx c, int64_t budget, const orion_tuning *tune, o
 }
 
 int orion_server_set_policy(orion_server s, int policy) {
-  if (!s || (policy != ORION_SERVE_PACK && policy != ORION_SERVE_FAIR && policy != ORION_SERVE_ARRIVAL)) {
+  if (!s || policy < ORION_SERVE_PACK || policy > ORION_SERVE_AGED) {
     set_error("bad argument");
     return ORION_ERR_ARG;
   }
   std::lock_guard<std::mutex> lk(s->mu);
   s->policy = policy == ORION_SERVE_FAIR ? orion_amd::kFair : orion_amd::kPack;
-  s->shortest_first = policy == ORION_SERVE_PACK;
+  s->shortest_first = policy == ORION_SERVE_PACK || policy == ORION_SERVE_AGED;
+  s->aged = policy == ORION_SERVE_AGED;
   return ORION_OK;
 }
 

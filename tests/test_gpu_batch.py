@@ -121,7 +121,8 @@ def test_graph_replay_equals_eager(device, scenes):
 
 
 @pytest.mark.parametrize("budget,policy", [("wide", oa.Server.PACK), ("wide", oa.Server.FAIR),
-                                           ("wide", oa.Server.ARRIVAL), ("one_job", oa.Server.PACK)])
+                                           ("wide", oa.Server.ARRIVAL), ("wide", oa.Server.AGED),
+                                           ("one_job", oa.Server.PACK)])
 def test_server_results_equal_alone(device, scenes, budget, policy):
     jobs = job_list(scenes)
     alone = [oa.render_batch(device, [j])[0][0] for j in jobs]

@@ -32,7 +32,9 @@ TRI_BYTES = 48       # p0, e1, e2 + primitive id
 RAY_IO_BYTES = 48    # 32 B ray in (o, d, tmax, id) + 16 B hit out (t, prim, u, v)
 SHADOW_IO_BYTES = 36  # 32 B ray in + 4 B visibility out
 SHADE_BYTES = 292     # per shaded path segment
-TRACE_KERNEL = "k_trace_pair<16, false>"
+# a bounce's traversal: the closest-hit and any-hit kernels, launched side by
+# side on two streams (one fork / join per bounce)
+TRACE_KERNELS = ("k_trace<false, 16, false>", "k_trace<true, 16, false>")
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_icosphere_L8_1024x1024_32spp.json"
 
 
@@ -356,8 +358,9 @@ def main():
     paths = tot["paths"]
     value = paths / el
     rays = tot["closest_rays"] + tot["shadow_rays"]
-    # roofline of the traversal launch (closest-hit and any-hit queues in one
-    # launch per bounce), the dominant kernel by device time
+    # roofline of a bounce's traversal (the closest-hit and any-hit kernels,
+    # concurrent on two streams, timed from the fork to the join on the wave's
+    # stream), the dominant work by device time
     kern = {"trace_pair": tot["trace_pair_ms"], "shade": tot["shade_ms"]}
     traffic = load_traffic(a)
 
@@ -376,19 +379,26 @@ def main():
     bytes_alg = ray_bytes("closest") + ray_bytes("shadow")
     per_launch_s = tot["trace_pair_ms"] / 1e3 / launches
     achieved = bytes_alg / launches / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
-    t = traffic["kernels"].get(TRACE_KERNEL) if traffic else None
+    ts = [traffic["kernels"].get(k) for k in TRACE_KERNELS] if traffic else [None]
+    t = None
+    if all(ts):
+        # per bounce: one dispatch of each kernel; fetch-weighted L2 hit rate
+        fetch = [x["fetch_kib"] for x in ts]
+        t = {"hbm_bytes_per_dispatch": sum(x["hbm_bytes_per_dispatch"] for x in ts),
+             "l2_hit_rate": round(sum(x["l2_hit_rate"] * f for x, f in zip(ts, fetch)) / max(sum(fetch), 1), 4),
+             "valu_busy": {k: x.get("valu_busy") for k, x in zip(TRACE_KERNELS, ts)}}
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": t["hbm_bytes_per_dispatch"] if t else None,
-        "kernel": f"{TRACE_KERNEL} (closest-hit + any-hit traversal, 4-wide quantized BVH, "
-                  f"LDS ring stack)",
+        "kernel": f"{' + '.join(TRACE_KERNELS)}: a bounce's closest-hit and any-hit traversal (4-wide "
+                  f"quantized BVH, LDS ring stack), concurrent on two streams; one launch = one bounce's "
+                  f"fork-to-join interval",
         "algorithmic_bytes_per_launch": round(bytes_alg / launches),
         "launch_ms": round(per_launch_s * 1e3, 4),
         "closest": per_ray("closest"), "shadow": per_ray("shadow"),
         "l2_hit_rate": t["l2_hit_rate"] if t else None,
-        # VALU busy from the same rocprofv3 runs: the traversal is issue bound
-        "valu_busy": t.get("valu_busy") if t else None,
+        "valu_busy": t["valu_busy"] if t else None,
         "kernel_ms": {k: round(v, 2) for k, v in kern.items()},
     }
     if traffic:

@@ -97,6 +97,10 @@ struct WaveRunner {
   DevBuf films;  // batched launches: the jobs' films
   int max_segs = 0;
   hipEvent_t staged = nullptr;
+  // the any-hit traversal's stream: a bounce's closest-hit and any-hit
+  // launches run side by side (fork / join through two events)
+  hipStream_t st_any = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<std::pair<GraphKey, hipGraphExec_t>> graphs;
 
   ~WaveRunner() {
@@ -105,6 +109,9 @@ struct WaveRunner {
     if (h_ctrl) (void)hipHostFree(h_ctrl);
     if (h_segjob) (void)hipHostFree(h_segjob);
     if (staged) (void)hipEventDestroy(staged);
+    if (st_any) (void)hipStreamDestroy(st_any);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
   }
   int ensure_segjob(int nseg) {
     if (nseg <= max_segs) return ORION_OK;
@@ -130,6 +137,11 @@ struct WaveRunner {
     if ((rc = stats.ensure(kStatBytes))) return rc;
     HIP_TRY(hipMemset(ctrl.ptr, 0, 64 * 4));
     if (!staged) HIP_TRY(hipEventCreateWithFlags(&staged, hipEventDisableTiming));
+    if (!st_any) {
+      HIP_TRY(hipStreamCreateWithFlags(&st_any, hipStreamNonBlocking));
+      HIP_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
     max_jobs = cap;
     drop_graphs();
     return ORION_OK;
@@ -183,9 +195,18 @@ struct WaveRunner {
     return ORION_OK;
   }
 
+  // A bounce traces its closest-hit queue on the wave's stream and its
+  // any-hit queue on st_any at the same time: separate kernels hold 68 and
+  // 65 VGPRs (7 waves/SIMD) where one launch of both parts held 74 (6), and
+  // the any-hit blocks still fill the GPU while the closest-hit part drains
+  // (measured 5% faster traversal than the single paired launch, 8% than two
+  // launches in a row).  The any-hit launch takes the spill area's second
+  // part.  `split` runs the two launches in a row on one stream.
   void enqueue_wave(const Job *dj, const Wave &w, int depth, int ring, bool multi, int chunk, bool split,
                     hipStream_t st, LaunchTimer *tm) {
     launch_raygen(dj, w, multi, st);
+    Wave wa_base = w;
+    wa_base.spill = w.spill + (size_t)w.spill_cap * trace_grid(w.nseg, w.seg, false, chunk) * kTraceBlock;
     for (int it = 0; it <= depth; ++it) {
       Wave wi = w;  // bounce it traces the ray lengths of counter set it % 3
       wi.cnt_ray = w.cnt + (3 * (it % 3) + 1) * w.cnt_stride;
@@ -198,7 +219,14 @@ struct WaveRunner {
         if (tm) tm->mark(-1);
       } else {
         if (tm) tm->mark(3);
-        launch_trace_pair(dj, wi, ring, chunk, multi, st);
+        Wave wa = wi;
+        wa.spill = wa_base.spill;
+        (void)hipEventRecord(ev_fork, st);
+        (void)hipStreamWaitEvent(st_any, ev_fork, 0);
+        launch_trace(dj, wi, false, ring, chunk, multi, st);
+        launch_trace(dj, wa, true, ring, chunk, multi, st_any);
+        (void)hipEventRecord(ev_join, st_any);
+        (void)hipStreamWaitEvent(st, ev_join, 0);
         if (tm) tm->mark(-1);
       }
       if (tm) tm->mark(2);

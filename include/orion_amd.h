@@ -159,7 +159,7 @@ typedef struct {
   int bvh_depth, node_format;
   double build_ms, upload_ms;
   int stack_bound;          /* traversal stack entries that can be live at once */
-  int trace_waves_per_simd; /* occupancy of the paired closest-hit + any-hit traversal (default LDS ring) */
+  int trace_waves_per_simd; /* occupancy of the closest-hit traversal kernel (default LDS ring) */
 } orion_scene_stats;
 int orion_scene_get_stats(orion_scene s, orion_scene_stats *st);
 

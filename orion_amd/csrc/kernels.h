@@ -30,7 +30,7 @@ int trace_grid(int nseg, int seg, bool any, int chunk);
 // spill area (ints) for `blocks` traversal blocks; the per-thread capacity
 // is max(0, stack_bound - ring)
 size_t trace_spill_ints(int stack_bound, int ring, int blocks);
-int trace_waves_per_simd(int ring);  // occupancy of the paired closest-hit + any-hit traversal kernel
+int trace_waves_per_simd(int ring);  // occupancy of the closest-hit traversal kernel
 // single-queue waves (SPPM): account the traced rays, reset the queues
 void launch_between(const Wave &w, int nxt, hipStream_t st);
 void launch_shade(const Job *jobs, const Wave &w, int cur, bool multi, hipStream_t st);

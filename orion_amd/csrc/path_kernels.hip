@@ -421,7 +421,7 @@ int trace_waves_per_simd(int ring) {
   ring = ring_variant(ring);
   int nb = 0;
   hipError_t e = hipSuccess;
-  ORION_RING_DISPATCH(ring, e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_pair<ST, false>, kTraceBlock, 0));
+  ORION_RING_DISPATCH(ring, e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace<false, ST, false>, kTraceBlock, 0));
   return e == hipSuccess ? nb * (kTraceBlock / 64) / 4 : 0;
 }
 

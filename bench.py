@@ -54,6 +54,19 @@ def load_traffic(a):
     return d
 
 
+def traversal_traffic(traffic):
+    """A bounce's traversal traffic from the rocprofv3 summary: HBM bytes of
+    one dispatch of each traversal kernel summed, their fetch-weighted L2 hit
+    rate and each kernel's VALU busy; None without both kernels' figures."""
+    ts = [traffic["kernels"].get(k) for k in TRACE_KERNELS] if traffic else [None]
+    if not all(ts):
+        return None
+    fetch = [x["fetch_kib"] for x in ts]
+    return {"hbm_bytes_per_dispatch": sum(x["hbm_bytes_per_dispatch"] for x in ts),
+            "l2_hit_rate": round(sum(x["l2_hit_rate"] * f for x, f in zip(ts, fetch)) / max(sum(fetch), 1), 4),
+            "valu_busy": {k: x.get("valu_busy") for k, x in zip(TRACE_KERNELS, ts)}}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -379,14 +392,7 @@ def main():
     bytes_alg = ray_bytes("closest") + ray_bytes("shadow")
     per_launch_s = tot["trace_pair_ms"] / 1e3 / launches
     achieved = bytes_alg / launches / per_launch_s / 1e9 if per_launch_s > 0 else 0.0
-    ts = [traffic["kernels"].get(k) for k in TRACE_KERNELS] if traffic else [None]
-    t = None
-    if all(ts):
-        # per bounce: one dispatch of each kernel; fetch-weighted L2 hit rate
-        fetch = [x["fetch_kib"] for x in ts]
-        t = {"hbm_bytes_per_dispatch": sum(x["hbm_bytes_per_dispatch"] for x in ts),
-             "l2_hit_rate": round(sum(x["l2_hit_rate"] * f for x, f in zip(ts, fetch)) / max(sum(fetch), 1), 4),
-             "valu_busy": {k: x.get("valu_busy") for k, x in zip(TRACE_KERNELS, ts)}}
+    t = traversal_traffic(traffic)
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -4,9 +4,11 @@
 Workload (one GPU): level-8 icosphere (1,310,720 triangles + ground + light),
 gold GGX per region, quad area light + point light, 1024 x 1024, depth 8, RR
 after bounce 3, Sobol.  A step is one wavefront pass of `--spp-per-step`
-samples per pixel over the whole image (32 spp = 33,554,432 camera paths in
-flight); the default 2 steps render the configuration's full 64 spp.  The scene (BVH,
-triangles, shading data, Sobol table) is resident in HBM before timing.
+samples per pixel over the whole image: the default 64 spp (the
+configuration's full sample count) puts 67,108,864 camera paths in flight,
+about 16 GB of wave state, and the default 2 steps render the configuration
+twice.  The scene (BVH, triangles, shading data, Sobol table) is resident in
+HBM before timing.
 
 Prints one JSON line: paths/s as `value`, rays/s beside it, the roofline of
 the dominant kernel (closest-hit traversal, HBM bound) from HIP events on its
@@ -35,14 +37,14 @@ SHADE_BYTES = 292     # per shaded path segment
 # a bounce's traversal: the closest-hit and any-hit kernels, launched side by
 # side on two streams (one fork / join per bounce)
 TRACE_KERNELS = ("k_trace<false, 16, false>", "k_trace<true, 16, false>")
-TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_icosphere_L8_1024x1024_32spp.json"
+TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_icosphere_L8_1024x1024_64spp.json"
 
 
 def load_traffic(a):
     """HBM bytes per dispatch measured with rocprofv3 --pmc (FETCH_SIZE /
     WRITE_SIZE, scaled by a 256 MiB copy) for this exact configuration, or
     None when the configuration differs or the measurement is absent."""
-    if not (a.level == 8 and a.res == 1024 and a.depth == 8 and a.spp_per_step == 32):
+    if not (a.level == 8 and a.res == 1024 and a.depth == 8 and a.spp_per_step == 64):
         return None
     try:
         d = json.loads(TRAFFIC_FILE.read_text())
@@ -72,7 +74,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spp-per-step", type=int, default=32)
+    ap.add_argument("--spp-per-step", type=int, default=64)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--depth", type=int, default=8)

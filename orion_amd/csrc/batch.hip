@@ -38,9 +38,13 @@ namespace orion_amd {
 
 // Paths per wave: the traversal launches are latency-bound (a launch lasts
 // at least as long as its slowest rays), so the wave is made large enough
-// that each bounce's launch is dominated by throughput (32 samples per pixel
-// of a 1024x1024 film; ~240 B of wave state per path = 8 GB).
-constexpr int kDefaultWavePaths = 1 << 25;
+// that each bounce's launch is dominated by throughput (64 samples per pixel
+// of a 1024x1024 film; ~240 B of wave state per path = 16 GB of MI355X's 288).
+// The server caps its waves at half that: a request that arrives waits for
+// the running wave, so a smaller wave bounds its wait (measured: 2^26-path
+// served waves raised the median latency from 0.10 to 0.15 s).
+constexpr int kDefaultWavePaths = 1 << 26;
+constexpr int kServeMaxWavePaths = 1 << 25;
 
 // Per-launch HIP events recorded on the launch stream (kernel timing).
 struct LaunchTimer {
@@ -682,7 +686,7 @@ void orion_server_s::loop() {
   using namespace orion_amd;
   (void)hipSetDevice(ctx->device);
   const orion_tuning *tn = has_tune ? &tune : nullptr;
-  int target = tn && tn->wave_paths > 0 ? tn->wave_paths : kDefaultWavePaths;
+  int target = std::min(tn && tn->wave_paths > 0 ? tn->wave_paths : kDefaultWavePaths, kServeMaxWavePaths);
   int rr = 0;
   const int seg = seg_of(tn, target);
   for (;;) {
@@ -1008,7 +1012,7 @@ int orion_autotune(orion_ctx c, int n, const orion_job_desc *jobs, orion_tuning 
   };
   int rc;
   if ((rc = trial(b))) return rc;
-  for (int wp : {1 << 24, 1 << 23, 1 << 22}) {
+  for (int wp : {1 << 25, 1 << 24, 1 << 23}) {
     orion_tuning t{};
     t.wave_paths = wp;
     if ((rc = trial(t))) return rc;

@@ -192,6 +192,7 @@ typedef struct {
    * default; split_trace = 1 launches them separately, timed above) */
   double trace_pair_ms;
   int64_t trace_pair_launches;
+  int64_t shaded_segments;  /* path segments shaded (one per live path per bounce) */
 } orion_render_stats;
 
 typedef struct {
@@ -247,9 +248,14 @@ int orion_render_sppm(orion_scene s, int n_iters, int photons_per_iter, float in
 
 /* ---------------------------------------------------------------- batched launch
  * No counterpart in orion: many render jobs -- each its own resident scene,
- * film size, spp and depth -- packed into shared wavefronts that one
- * persistent traversal kernel drains.  A job's camera is its scene's view
- * with the job's film size (aspect = width / height). */
+ * film size, spp and depth -- packed into shared wavefronts: a wave's queue
+ * segments each belong to one job, and every bounce's traversal and shading
+ * launches serve all of the wave's jobs at once.  A job's camera is its
+ * scene's view with the job's film size (aspect = width / height).
+ * HBM: a wave of N paths holds 236 B of state per path plus the traversal
+ * spill area (60 B per path for the 1.3M-triangle scene: 20 spilled stack
+ * entries per traversal lane), so the default 2^26-path wave needs about
+ * 20 GB; a wave that does not fit is halved until it does. */
 typedef struct {
   orion_scene scene;   /* resident scene (BVH, materials, lights, view) */
   int width, height;   /* film of this job */
@@ -312,10 +318,11 @@ int orion_server_destroy(orion_server s);
 
 /* ---------------------------------------------------------------- autotuner
  * No counterpart in orion.  Times short batched renders of `jobs` (each
- * cut to its first two samples per pixel) over candidate wave sizes,
- * persistent traversal blocks per CU and graph capture, and returns the
- * fastest setting for orion_render_batch / orion_server_create; with one
- * job whose film is its scene's, also for orion_render_path. */
+ * cut to its first two samples per pixel) by coordinate descent over the
+ * wave size, traversal chunk (queue entries per wave64), LDS stack ring,
+ * queue segment size, graph capture and split traversal launches, and
+ * returns the fastest setting for orion_render_batch / orion_server_create;
+ * with one job whose film is its scene's, also for orion_render_path. */
 int orion_autotune(orion_ctx c, int n_jobs, const orion_job_desc *jobs, orion_tuning *best);
 
 #ifdef __cplusplus

@@ -47,7 +47,8 @@ class RenderStats(C.Structure):
                 ("trace_shadow_ms", C.c_double), ("shade_ms", C.c_double),
                 ("trace_closest_launches", C.c_int64), ("trace_shadow_launches", C.c_int64),
                 ("shade_launches", C.c_int64), ("stack_overflows", C.c_int64),
-                ("trace_pair_ms", C.c_double), ("trace_pair_launches", C.c_int64)]
+                ("trace_pair_ms", C.c_double), ("trace_pair_launches", C.c_int64),
+                ("shaded_segments", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

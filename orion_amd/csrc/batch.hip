@@ -170,6 +170,14 @@ struct WaveRunner {
     spill_cap = std::max(0, stack_bound - ring_variant(ring));
     return ORION_OK;
   }
+  // Wave state and traversal spill area for waves of up to `paths` paths in
+  // segments of `seg`; ORION_ERR_OOM leaves the runner usable (a caller
+  // retries with a smaller wave).
+  int reserve(int paths, int seg, int stack_bound, int ring, int chunk) {
+    int rc = ensure_cap(paths);
+    if (rc) return rc;
+    return ensure_spill(stack_bound, ring, chunk, seg, std::max(1, (paths + seg - 1) / seg));
+  }
   Wave wave() {
     Wave w = wb.wave;
     w.ctrl = (int *)ctrl.ptr;
@@ -186,6 +194,7 @@ struct WaveRunner {
     unsigned long long h[S_COUNT];
     int rc = read_striped_stats(stats.ptr, h);
     if (rc) return rc;
+    if ((rc = check_overflow(h[S_OVERFLOW]))) return rc;
     if (!o) return ORION_OK;
     o->paths += (int64_t)h[S_PATHS];
     o->closest_rays += (int64_t)h[S_CLOSEST];
@@ -196,6 +205,7 @@ struct WaveRunner {
     o->node_visits_shadow += (int64_t)h[S_NODES_S];
     o->tri_tests_shadow += (int64_t)h[S_TRIS_S];
     o->stack_overflows += (int64_t)h[S_OVERFLOW];
+    o->shaded_segments += (int64_t)h[S_SHADED];
     return ORION_OK;
   }
 
@@ -325,6 +335,21 @@ struct WaveRunner {
 
 void destroy_runner(WaveRunner *r) { delete r; }
 
+// A tuning is checked before anything else: ray slot s of path p is the
+// 32-bit index s * N + p (s < 3), so no wave may exceed kMaxWavePathsLimit.
+static int check_tuning(const orion_tuning *t) {
+  if (!t) return ORION_OK;
+  if (t->wave_paths < 0 || t->spp_per_wave < 0 || t->rays_per_wave < 0 || t->lds_ring < 0 || t->queue_segment < 0) {
+    set_error("negative tuning value");
+    return ORION_ERR_ARG;
+  }
+  if (t->wave_paths > kMaxWavePathsLimit || t->spp_per_wave > kMaxWavePathsLimit) {
+    set_error("wave of more than 2^28 paths: ray-slot indices would overflow 32 bits");
+    return ORION_ERR_UNSUPPORTED;
+  }
+  return ORION_OK;
+}
+
 static int ring_of(const orion_tuning *t) { return ring_variant(t ? t->lds_ring : 0); }
 static int chunk_of(const orion_tuning *t) { return trace_chunk(t ? t->rays_per_wave : 0); }
 static bool split_of(const orion_tuning *t) { return t && t->split_trace; }
@@ -364,6 +389,7 @@ using namespace orion_amd;
 // ======================================================================= plain renderer
 extern "C" int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, const orion_tuning *tune,
                                  orion_render_stats *stats) {
+  if (int rc = check_tuning(tune)) return rc;
   if (!s) { set_error("null scene"); return ORION_ERR_ARG; }
   if (k0 < 0 || k1 < k0 || maxDepth < 0) { set_error("bad sample range / depth"); return ORION_ERR_ARG; }
   const int npix = s->W * s->H;
@@ -371,6 +397,7 @@ extern "C" int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, co
     set_error("sample index k * W * H must stay below 2^32");
     return ORION_ERR_UNSUPPORTED;
   }
+  if (npix > kMaxWavePathsLimit) { set_error("film of more than 2^28 pixels"); return ORION_ERR_UNSUPPORTED; }
   if (maxDepth > 60) { set_error("maxDepth above 60 exceeds the Sobol dimension budget"); return ORION_ERR_UNSUPPORTED; }
   orion_ctx c = s->ctx;
   HIP_TRY(hipSetDevice(c->device));
@@ -378,13 +405,20 @@ extern "C" int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, co
   WaveRunner &R = *c->runner;
   const int wave_paths = tune && tune->wave_paths > 0 ? tune->wave_paths : kDefaultWavePaths;
   int S = tune && tune->spp_per_wave > 0 ? tune->spp_per_wave : std::max(1, wave_paths / npix);
-  S = std::min(S, std::max(1, k1 - k0));
+  S = std::min({S, std::max(1, k1 - k0), kMaxWavePathsLimit / npix});
   int rc;
   if ((rc = R.init(1))) return rc;
-  if ((rc = R.ensure_cap(S * npix))) return rc;
-  if ((rc = R.reset_stats(c->stream))) return rc;
   const int ring = ring_of(tune);
   const int chunk = chunk_of(tune);
+  // a wave that does not fit in HBM is halved until it does (down to one
+  // sample pass over the film)
+  for (;;) {
+    rc = R.reserve(S * npix, seg_of(tune, S * npix), s->d.stack_bound, ring, chunk);
+    if (rc != ORION_ERR_OOM || S == 1) break;
+    S = (S + 1) / 2;
+  }
+  if (rc) return rc;
+  if ((rc = R.reset_stats(c->stream))) return rc;
   const bool graph = tune && tune->use_graph;
   LaunchTimer tm;
   tm.on = tune && tune->profile;
@@ -558,6 +592,11 @@ static int check_desc(const orion_job_desc &d) {
     set_error("sample index k * W * H must stay below 2^32");
     return ORION_ERR_UNSUPPORTED;
   }
+  if ((int64_t)d.width * d.height > kMaxWavePathsLimit / 2) {
+    // one sample pass of the film, padded to a queue segment, must fit a wave
+    set_error("film of more than 2^27 pixels");
+    return ORION_ERR_UNSUPPORTED;
+  }
   return ORION_OK;
 }
 
@@ -565,6 +604,7 @@ static int check_desc(const orion_job_desc &d) {
 
 extern "C" int orion_render_batch(orion_ctx c, int n, const orion_job_desc *descs, float *const *rgb_out,
                                   int *const *count_out, const orion_tuning *tune, orion_render_stats *stats) {
+  if (int rc = check_tuning(tune)) return rc;
   if (!c || n < 0 || (n && (!descs || !rgb_out || !count_out))) { set_error("bad argument"); return ORION_ERR_ARG; }
   for (int j = 0; j < n; ++j) {
     int rc = check_desc(descs[j]);
@@ -595,6 +635,23 @@ extern "C" int orion_render_batch(orion_ctx c, int n, const orion_job_desc *desc
   int target = tune && tune->wave_paths > 0 ? tune->wave_paths : kDefaultWavePaths;
   int rc;
   if ((rc = R.init(std::max(n, 1)))) return rc;
+  {
+    // the largest wave: the target, or one sample pass of the largest film
+    // padded to a segment; halved while it does not fit in HBM
+    int max_np = 1, max_stack = 1;
+    for (int j = 0; j < n; ++j) {
+      max_np = std::max(max_np, descs[j].width * descs[j].height);
+      max_stack = std::max(max_stack, descs[j].scene->d.stack_bound);
+    }
+    const int floor_paths = (max_np + kSegMax - 1) / kSegMax * kSegMax;
+    for (;;) {
+      const int paths = std::max(target, floor_paths);
+      rc = R.reserve(paths, seg_of(tune, paths), max_stack, ring_of(tune), chunk_of(tune));
+      if (rc != ORION_ERR_OOM || target <= floor_paths) break;
+      target /= 2;
+    }
+    if (rc) return rc;
+  }
   if ((rc = R.reset_stats(c->stream))) return rc;
   const bool graph = tune && tune->use_graph;
   LaunchTimer tm;
@@ -840,6 +897,7 @@ void orion_server_s::loop() {
 extern "C" {
 
 int orion_server_create(orion_ctx c, int64_t budget, const orion_tuning *tune, orion_server *out) {
+  if (int rc = orion_amd::check_tuning(tune)) return rc;
   if (!c || !out || budget <= 0) { set_error("bad argument"); return ORION_ERR_ARG; }
   HIP_TRY(hipSetDevice(c->device));
   auto *s = new orion_server_s();

@@ -86,9 +86,11 @@ struct HitRec {
 // (`ring` = the lane's first slot, slots kRingStride ints apart, one LDS
 // column per lane so a wave's accesses are conflict-free); older entries
 // move to the lane's spill area in HBM (`spill`, entries `spill_stride` ints
-// apart, `spill_cap` entries).  A DFS stack of a 4-wide tree holds at most
-// three siblings per level, so stack_bound entries never overflow; an
-// overflow (a malformed tree) is counted in *overflow and the entry dropped.
+// apart, `spill_cap` entries; the offset stride * entry is formed in 64 bits:
+// a 2^26-path wave's launch has up to 2^27 threads).  A DFS stack of a
+// 4-wide tree holds at most three siblings per level, so stack_bound entries
+// never overflow; an overflow (a malformed tree) is counted in *overflow,
+// the entry dropped, and the host fails the call with ORION_ERR_UNSUPPORTED.
 constexpr int kRingStride = 64;  // threads per traversal block: one wave64 (2 and 4 measured 3% and 20% slower)
 
 template <int RING>
@@ -99,7 +101,7 @@ struct TravStack {
   int top = 0, base = 0;  // ring holds [base, top); spill holds [0, base)
   __device__ __forceinline__ void push(int x, uint32_t *overflow) {
     if (top - base == RING) {
-      if (base < spill_cap) spill[spill_stride * base] = ring[kRingStride * (base & (RING - 1))];
+      if (base < spill_cap) spill[(size_t)spill_stride * base] = ring[kRingStride * (base & (RING - 1))];
       else ++*overflow;
       ++base;
     }
@@ -114,7 +116,7 @@ struct TravStack {
     if (base > 0) {
       --base;
       --top;
-      return base < spill_cap ? spill[spill_stride * base] : kLeafEmpty;
+      return base < spill_cap ? spill[(size_t)spill_stride * base] : kLeafEmpty;
     }
     return kLeafEmpty;
   }

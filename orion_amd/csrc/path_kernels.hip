@@ -321,6 +321,7 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
   if (threadIdx.x == 0) {
     stat_add(w.stats, S_PATHS, ltot[0]);
     stat_add(w.stats, S_REFEQ, ltot[1]);
+    stat_add(w.stats, S_SHADED, (unsigned long long)(p_end - p_begin));
   }
 }
 

@@ -37,6 +37,12 @@ struct WaveRunner;
 void destroy_runner(WaveRunner *r);
 // sums the striped device statistics (kStatBytes at dev) into out[S_COUNT]
 int read_striped_stats(const void *dev, unsigned long long *out);
+// ORION_ERR_UNSUPPORTED when a traversal dropped stack entries (a tree
+// deeper than its stack bound: the hits of that call are not trustworthy)
+int check_overflow(unsigned long long dropped);
+// Largest wave (paths in flight): ray slot s of path p lives at s * N + p
+// with s < 3, so 3 N must stay below 2^31.
+constexpr int kMaxWavePathsLimit = 1 << 28;
 
 }  // namespace orion_amd
 

@@ -57,6 +57,7 @@ int dev_alloc(void **p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
     *p = nullptr;
+    (void)hipGetLastError();  // a caller may retry smaller: leave no error for the next launch check
     set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
     return e == hipErrorOutOfMemory ? ORION_ERR_OOM : ORION_ERR_HIP;
   }
@@ -133,6 +134,13 @@ int read_striped_stats(const void *dev, unsigned long long *out) {
     out[s] = v;
   }
   return ORION_OK;
+}
+
+int check_overflow(unsigned long long dropped) {
+  if (!dropped) return ORION_OK;
+  set_error("BVH traversal stack overflow: " + std::to_string(dropped) +
+            " stack entries dropped (the tree is deeper than its stack bound); results are invalid");
+  return ORION_ERR_UNSUPPORTED;
 }
 
 }  // namespace orion_amd
@@ -359,10 +367,22 @@ static int intersect_launch(orion_scene s, int n, const float *rays, float *t, i
   int rc = s->spill.ensure(
       trace_spill_ints(s->d.stack_bound, kDefaultRing, trace_grid(1, n, true, kDefaultChunk)) * 4 + 4);
   if (rc) return rc;
+  // the scene's statistics area counts dropped stack entries of this call
+  HIP_TRY(hipMemsetAsync(s->stats.ptr, 0, kStatBytes, c->stream));
   launch_intersect_batch(s->d, n, rays, t, prim, u, v, tie, occ, any, (int *)s->spill.ptr,
-                         std::max(0, s->d.stack_bound - kDefaultRing), nullptr, c->stream);
+                         std::max(0, s->d.stack_bound - kDefaultRing), (unsigned long long *)s->stats.ptr,
+                         c->stream);
   HIP_TRY(hipGetLastError());
   return ORION_OK;
+}
+
+// After an intersection launch has completed: fails the call when the
+// traversal dropped stack entries.
+static int intersect_check(orion_scene s) {
+  unsigned long long h[S_COUNT];
+  int rc = read_striped_stats(s->stats.ptr, h);
+  if (rc) return rc;
+  return check_overflow(h[S_OVERFLOW]);
 }
 
 int orion_intersect_closest(orion_scene s, int n, const float *rays, float *t, int *prim, float *u, float *v,
@@ -375,7 +395,7 @@ int orion_intersect_closest(orion_scene s, int n, const float *rays, float *t, i
     int rc = intersect_launch(s, n, rays, t, prim, u, v, tie, nullptr, false);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    return ORION_OK;
+    return intersect_check(s);
   }
   DevBuf buf;
   int rc = buf.ensure((size_t)n * (8 + 5) * 4);
@@ -391,7 +411,7 @@ int orion_intersect_closest(orion_scene s, int n, const float *rays, float *t, i
   if (v) HIP_TRY(hipMemcpyAsync(v, dv, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   if (tie) HIP_TRY(hipMemcpyAsync(tie, dtie, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  return ORION_OK;
+  return intersect_check(s);
 }
 
 int orion_intersect_any(orion_scene s, int n, const float *rays, int *occluded, int on_device) {
@@ -403,7 +423,7 @@ int orion_intersect_any(orion_scene s, int n, const float *rays, int *occluded, 
     int rc = intersect_launch(s, n, rays, nullptr, nullptr, nullptr, nullptr, nullptr, occluded, true);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    return ORION_OK;
+    return intersect_check(s);
   }
   DevBuf buf;
   int rc = buf.ensure((size_t)n * 9 * 4);
@@ -414,7 +434,7 @@ int orion_intersect_any(orion_scene s, int n, const float *rays, int *occluded, 
   if ((rc = intersect_launch(s, n, dr, nullptr, nullptr, nullptr, nullptr, nullptr, docc, true))) return rc;
   HIP_TRY(hipMemcpyAsync(occluded, docc, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  return ORION_OK;
+  return intersect_check(s);
 }
 
 int orion_film_clear(orion_scene s) {

@@ -1007,6 +1007,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   unsigned long long h[S_COUNT];
   if ((rc = read_striped_stats(s->stats.ptr, h))) return rc;
+  if ((rc = check_overflow(h[S_OVERFLOW]))) return rc;
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->closest_rays = (int64_t)h[S_CLOSEST];

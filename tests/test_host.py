@@ -219,3 +219,24 @@ def test_bvh_tiny_scenes():
                                        np.zeros(0, np.float32), cam)
         nn, depth, bound = hs.bvh_check()
         assert nn >= 1
+
+
+def test_wave_size_bound_rejected_before_any_device_call():
+    """Ray slot s of path p is the 32-bit index s * N + p (s < 3): tunings that
+    ask for waves above 2^28 paths are refused with ORION_ERR_UNSUPPORTED, and
+    negative values with ORION_ERR_ARG, before a scene or context is touched
+    (so this runs without a GPU)."""
+    import ctypes
+    L = oa.lib()
+    big = oa.Tuning(wave_paths=(1 << 28) + 1)
+    assert L.orion_render_path(None, 0, 1, 5, ctypes.byref(big), None) == -6
+    assert b"2^28" in L.orion_last_error()
+    assert L.orion_render_batch(None, 0, None, None, None, ctypes.byref(big), None) == -6
+    srv = ctypes.c_void_p()
+    assert L.orion_server_create(None, 1 << 30, ctypes.byref(big), ctypes.byref(srv)) == -6
+    assert L.orion_render_path(None, 0, 1, 5, ctypes.byref(oa.Tuning(spp_per_wave=(1 << 28) + 1)), None) == -6
+    assert L.orion_render_path(None, 0, 1, 5, ctypes.byref(oa.Tuning(wave_paths=-1)), None) == -1
+    # the bound itself is accepted by the check (the null scene is then the error)
+    ok = oa.Tuning(wave_paths=1 << 28)
+    assert L.orion_render_path(None, 0, 1, 5, ctypes.byref(ok), None) == -1
+    assert b"null scene" in L.orion_last_error()

@@ -11,28 +11,46 @@
 // denormals preserved (the reference turns FTZ/DAZ on at Embree build time,
 // core/kernel.cpp:65-66; both oracle and GPU keep IEEE denormals instead).
 //
-// Pin table (function -> how it is pinned; see tests/):
-//   sobol()                sobol.h:28-43       -> oracle/_ref sobol::sample; dim 0 = radical inverse
+// Pin table (function -> how it is pinned; the tests named are in tests/):
+//   sobol()                sobol.h:28-43       -> oracle/_ref sobol::sample; dim 0 = radical
+//                                                 inverse (test_oracle_pins)
 //   Sampler::next2 order   sampler.h:31        -> oracle/_ref SobolSampler (g++ evaluation order)
 //   camera_ray()           pinhole.cpp:8-33    -> oracle/_ref Pinhole::generateRay
-//   warps                  sampling.cpp:5-61   -> oracle/_ref + closed forms
-//   BSDF f/pdf/sample_f    bsdf.cpp, bsdf/*    -> oracle/_ref Matte/Metal/Glass::getBSDF
+//   warps                  sampling.cpp:5-61   -> oracle/_ref uniformSampleTriangle / Disk /
+//                                                 cosineSampleHemisphere / Sphere / Cone
+//                                                 (test_warps_match_reference)
+//   offset_ray_origin      geometry.h:850-863  -> oracle/_ref Intersection::spawnRay, bit for bit
+//                                                 (test_spawn_matches_reference)
+//   Distribution1D         sampling.h:71-125   -> oracle/_ref sampleDiscrete / sampleContinuous,
+//                                                 bit for bit, + closed forms
+//                                                 (test_distribution1d_matches_reference)
+//   BSDF f/pdf/sample_f    bsdf.cpp, bsdf/*    -> oracle/_ref Matte/Metal/Glass/...::getBSDF
 //   point/spot lights      light/point.cpp, light/spot.cpp -> oracle/_ref
 //   distant light, envmap  light/distant.cpp, light/envmap.cpp, Distribution2D
 //                          (sampling.h:127-150) -> oracle/_ref DistantLight /
 //                          Envmap built from the same image; closed forms
 //                          (constant map: Le = L, pdf of a uniform map)
-//   area light, triangle   light/diffuse.cpp, core/triangle.cpp:85-135
-//                          -> uniformSampleTriangle via _ref; rest by closed forms
-//                             (solid-angle pdf of a quad, constant-emission scene)
+//   area light, triangle   light/diffuse.cpp, core/triangle.cpp:85-135 (the reference
+//                          TUs need Embree to link) -> closed forms: point formula,
+//                          uniform area (chi-square), solid-angle pdf d^2 / (|cos| A)
+//                          (test_area_light_sample_closed_forms)
+//   estimateDirect, uniformSampleOneLight  core/integrator.cpp:8-90 -> Lambert plane
+//                          under a tilted quad light vs the irradiance integral
+//                          (test_direct_light_on_plane_matches_irradiance_integral)
 //   intersect()            core/kernel.cpp:111-174 (Embree semantics)
 //                          -> brute force over all triangles (Embree itself is
 //                             binary-only in the reference: parity of Embree's own
 //                             arithmetic is unpinned; hit semantics are defined here)
-//   Li / render            integrator/path.cpp, core/integrator.cpp
-//                          -> white-furnace closed form + per-stage pins above
-//   SPPM                   integrator/sppm.cpp -> ToGrid/hash integer closed forms,
-//                          brute-force per-cell counts, radius-update closed form
+//   Li / render            integrator/path.cpp -> Lambert furnaces (Le sum a^k over
+//                          0..maxDepth) and mirror furnaces (deterministic below
+//                          bounce 4; roulette beyond) (test_path_furnace_*)
+//   SPPM                   integrator/sppm.cpp -> camera-pass direct light on the plane
+//                          and a furnace, both times orion's snapshot-mean factor
+//                          (nIter + 1) / (2 nIter); radius update r' = r sqrt((N + 2/3 M)
+//                          / (N + M)) (test_sppm_*); ToGrid / hash by an independent
+//                          numpy restatement; per-cell counts by brute force
+// Every closed form above rejects a planted mistake (ORACLE_MUTANT below,
+// tests/test_oracle_mutants.py).
 //
 // sin/cos/acos/atan2: the reference calls the float libm routines.  The
 // oracle evaluates them in double and rounds once, the correctly rounded value
@@ -54,6 +72,15 @@
 
 #define ORION_SOBOL_STORAGE static const
 #include "../orion_amd/csrc/sobol_matrices.inc"
+
+// Planted mistakes: the pin tests (tests/test_oracle_mutants.py) build this
+// file with -DORACLE_MUTANT=k, k > 0, and check that a closed form or a
+// reference comparison fails for every k.  The default build (0) is the
+// oracle; no other value is ever loaded outside that test.
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+#define MUTANT(k) (ORACLE_MUTANT == (k))
 
 namespace {
 
@@ -229,6 +256,7 @@ V3 offset_ray_origin(V3 p, V3 n, V3 w) {
   if (dot(w, n) < 0) offset = neg(offset);
   V3 po = add(p, offset);
   for (int i = 0; i < 3; ++i) {
+    if (MUTANT(12)) continue;
     if (comp(offset, i) > 0) setc(po, i, next_up(comp(po, i)));
     else if (comp(offset, i) < 0) setc(po, i, next_down(comp(po, i)));
   }
@@ -312,7 +340,7 @@ struct Distribution1D {  // sampling.h:71-125
     }
     int off = first - 1;
     off = off < 0 ? 0 : (off > size - 2 ? size - 2 : off);
-    if (pdf) *pdf = (funcInt > 0) ? func[off] / (funcInt * count()) : 0;
+    if (pdf) *pdf = (funcInt > 0) ? func[off] / (funcInt * (MUTANT(13) ? 1 : count())) : 0;
     return off;
   }
 };
@@ -1088,12 +1116,12 @@ inline S area_L(const Light &l, V3 ng, V3 w) { return dot(ng, w) > 0 ? l.L : sp(
 // triangle.cpp:85-106: point on the triangle, ng face-forwarded to ns
 void tri_sample(const Tri &tr, float u0, float u1, V3 *p, V3 *ng, V3 *ns, float *pdf) {
   float b0, b1;
-  uniform_triangle(u0, u1, &b0, &b1);
+  uniform_triangle(u0, u1, MUTANT(6) ? &b1 : &b0, MUTANT(6) ? &b0 : &b1);
   *p = add(add(mul(tr.p0, b0), mul(tr.p1, b1)), mul(tr.p2, 1 - b0 - b1));
   V3 g = normalize(cross(sub(tr.p1, tr.p0), sub(tr.p2, tr.p0)));
   *ns = add(add(mul(tr.n0, b0), mul(tr.n1, b1)), mul(tr.n2, 1.0f - b0 - b1));
   *ng = dot(g, *ns) < 0.f ? neg(g) : g;
-  *pdf = 1.0f / tri_area(tr);
+  *pdf = (MUTANT(5) ? 0.5f : 1.0f) / tri_area(tr);
 }
 
 // Light::sample_Li.  Returns Li; *shadow is the ShadowTester ray.
@@ -1218,7 +1246,7 @@ S estimate_direct(const Scene &sc, const Ray &ray, const Isect &is, const BSDF &
   Ray shadow;
   S Li = light_sample_li(sc, light, is, lu0, lu1, &wi, &lightPdf, &shadow);
   if (lightPdf > 0 && !black(Li)) {
-    S f = bsdf_f(bs, wi, wo, type) * absdot(wi, is.ns);
+    S f = bsdf_f(bs, wi, wo, type) * (MUTANT(1) ? 1.f : absdot(wi, is.ns));
     bsdfPdf = bsdf_pdf(bs, wi, wo, type);
     if (!black(f)) {
       if (scene_occluded(sc, shadow, c)) Li = sp(0);
@@ -1260,7 +1288,7 @@ S estimate_direct(const Scene &sc, const Ray &ray, const Isect &is, const BSDF &
         if (std::isinf(pdf)) pdf = 0;
         lightPdf = pdf;
         if (lightPdf == 0) return Ld;
-        weight = power_heuristic(bsdfPdf, lightPdf);
+        weight = MUTANT(2) ? 1.f : power_heuristic(bsdfPdf, lightPdf);
       }
       S Lm = sp(0);
       if (hit) {
@@ -1288,7 +1316,7 @@ S sample_one_light(const Scene &sc, const Ray &ray, const Isect &is, const BSDF 
   float lu0, lu1, bu0, bu1;
   s.next2(&lu0, &lu1);
   s.next2(&bu0, &bu1);
-  return estimate_direct(sc, ray, is, bs, bu0, bu1, lightNum, lu0, lu1, c) / lightPdf;
+  return estimate_direct(sc, ray, is, bs, bu0, bu1, lightNum, lu0, lu1, c) / (MUTANT(3) ? 1.f : lightPdf);
 }
 
 inline S isect_Le(const Scene &sc, const Isect &is, V3 w) {  // intersection.cpp:4-8
@@ -1316,12 +1344,12 @@ S path_Li(const Scene &sc, Ray ray, Sampler &s, int maxDepth, Counters *c) {
   for (int bounces = 0;; ++bounces) {
     Isect is;
     bool found = scene_intersect(sc, ray, &is, c);
-    if (bounces == 0 || specularBounce) {
+    if (bounces == 0 || (specularBounce && !MUTANT(7))) {
       if (found) L += beta * isect_Le(sc, is, neg(ray.d));
       else
         for (int li : sc.env_lights) L += beta * env_le(sc, sc.lights[li], ray.d);  // path.cpp:65-68
     }
-    if (!found || bounces >= maxDepth) break;
+    if (!found || bounces >= maxDepth - (MUTANT(4) ? 1 : 0)) break;
     BSDF bs;
     make_bsdf(sc, sc.tris[is.prim].mat, is.ng, is.ns, is.u, is.v, false, &bs);
     if (bsdf_num(bs, ALL & ~SPEC) > 0) {
@@ -1689,12 +1717,12 @@ int oracle_render_sppm(void *s, int nIterations, int photonPerIteration, float i
       float pdfPos, pdfDir;
       S Le = light_sample_le(sc, light, a0, a1, c0, c1, &photonRay, &nLight, &pdfPos, &pdfDir);
       if (pdfPos == 0 || pdfDir == 0 || black(Le)) continue;
-      S beta = Le * absdot(nLight, photonRay.d) / (lightPdf * pdfPos * pdfDir);
+      S beta = Le * absdot(nLight, photonRay.d) / ((MUTANT(8) ? 1.f : lightPdf) * pdfPos * pdfDir);
       if (black(beta)) continue;
       Isect is;
       for (int depth = 0; depth < maxDepth; ++depth) {
         if (!scene_intersect(sc, photonRay, &is, nullptr)) break;
-        if (depth > 0) {
+        if (depth > 0 || MUTANT(11)) {
           int cell[3];
           if (to_grid(is.p, blo, bhi, gridRes, cell)) {
             unsigned h = grid_hash(cell, hashSize);
@@ -1742,7 +1770,7 @@ int oracle_render_sppm(void *s, int nIterations, int photonPerIteration, float i
     for (int i = 0; i < nPixels; ++i) {
       Pixel &p = pixels[i];
       if (p.M > 0) {
-        float gamma = (float)2 / (float)3;
+        float gamma = MUTANT(9) ? (float)3 / (float)4 : (float)2 / (float)3;
         float Nnew = p.N + gamma * p.M;
         float Rnew = p.radius * std::sqrt(Nnew / (p.N + p.M));
         S Phi = sp(p.Phi[0], p.Phi[1], p.Phi[2]);
@@ -1758,7 +1786,7 @@ int oracle_render_sppm(void *s, int nIterations, int photonPerIteration, float i
     uint64_t Np = (uint64_t)nIterations * (uint64_t)photonPerIteration;
     for (int i = 0; i < nPixels; ++i) {
       const Pixel &px = pixels[i];
-      S L = px.Ld / (float)nIterations;
+      S L = px.Ld / (MUTANT(10) ? (float)(iter + 1) : (float)nIterations);
       L += px.tau / (Np * kPi * px.radius * px.radius);
       rgb[3 * i] += L.r; rgb[3 * i + 1] += L.g; rgb[3 * i + 2] += L.b;
       count[i] += 1;
@@ -1852,5 +1880,34 @@ void oracle_world_sphere(void *s, float *out) {
 
 float oracle_sin(float x) { return osin(x); }
 float oracle_cos(float x) { return ocos(x); }
+
+void oracle_warps(const float *u, float cos_theta_max, float *out) {
+  uniform_triangle(u[0], u[1], out, out + 1);
+  uniform_disk(u[0], u[1], out + 2, out + 3);
+  V3 h = cosine_hemisphere(u[0], u[1]);
+  V3 s = uniform_sphere(u[0], u[1]);
+  V3 c = uniform_cone(u[0], u[1], cos_theta_max);
+  const V3 v[3] = {h, s, c};
+  for (int k = 0; k < 3; ++k) {
+    out[4 + 3 * k] = v[k].x; out[5 + 3 * k] = v[k].y; out[6 + 3 * k] = v[k].z;
+  }
+}
+
+void oracle_spawn(const float *p, const float *ng, const float *d, float *o3) {
+  V3 o = offset_ray_origin(mk(p[0], p[1], p[2]), mk(ng[0], ng[1], ng[2]), mk(d[0], d[1], d[2]));
+  o3[0] = o.x; o3[1] = o.y; o3[2] = o.z;
+}
+
+void oracle_distribution1d(int n, const float *func, float u, float *out) {
+  Distribution1D d;
+  d.init(func, n);
+  float pdf = 0;
+  out[0] = (float)d.sample_discrete(u, &pdf);
+  out[1] = pdf;
+  int off = 0;
+  out[2] = d.sample_continuous(u, &pdf, &off);
+  out[3] = pdf;
+  out[4] = (float)off;
+}
 
 }  // extern "C"

@@ -97,6 +97,18 @@ void oracle_world_sphere(void *scene, float *out);
 /* sin/cos as the renderer uses them. */
 float oracle_sin(float x);
 float oracle_cos(float x);
+/* Warps of sampler/sampling.cpp for u = (u[0], u[1]); out (13 floats, the
+ * layout of the _ref harness's ref_warps): uniformSampleTriangle b0 b1,
+ * uniformSampleDisk x y, cosineSampleHemisphere xyz, uniformSampleSphere xyz,
+ * uniformSampleCone(cos_theta_max) xyz. */
+void oracle_warps(const float *u, float cos_theta_max, float *out);
+/* offsetRayOrigin(p, ng, d) (core/geometry.h:850-863): the origin of
+ * Intersection::spawnRay (core/intersection.h:40-49). */
+void oracle_spawn(const float *p, const float *ng, const float *d, float *o3);
+/* Distribution1D over func[0..n) (sampler/sampling.h:71-125) at u; out:
+ * [0] sampleDiscrete offset, [1] its pdf, [2] sampleContinuous value,
+ * [3] its pdf, [4] its offset. */
+void oracle_distribution1d(int n, const float *func, float u, float *out);
 
 #ifdef __cplusplus
 }

@@ -308,6 +308,19 @@ void ref_light_sample_le(int type, const float *I, const float *from, const floa
   l->~Light();
 }
 
+// Distribution1D (sampling.h:71-125) over func[0..n) at u: [0] sampleDiscrete
+// offset, [1] its pdf, [2] sampleContinuous value, [3] its pdf, [4] its offset
+void ref_distribution1d(int n, const float *func, float u, float *out) {
+  Distribution1D d(func, n);
+  Float pdf = 0;
+  out[0] = (float)d.sampleDiscrete(u, &pdf);
+  out[1] = pdf;
+  int off = 0;
+  out[2] = d.sampleContinuous(u, &pdf, &off);
+  out[3] = pdf;
+  out[4] = (float)off;
+}
+
 // offsetRayOrigin + spawnRay(d) (geometry.h:850-863, intersection.h:46-49)
 void ref_spawn(const float *p, const float *ng, const float *d, float *o3) {
   Intersection is(Point3f(p[0], p[1], p[2]));

@@ -20,10 +20,11 @@ def _opt(p):
 
 
 class Oracle:
-    def __init__(self):
-        if not ORACLE_SO.exists():
-            raise FileNotFoundError(f"{ORACLE_SO} not built (make -C oracle)")
-        L = C.CDLL(str(ORACLE_SO))
+    def __init__(self, path=None):
+        path = Path(path) if path else ORACLE_SO
+        if not path.exists():
+            raise FileNotFoundError(f"{path} not built (make -C oracle)")
+        L = C.CDLL(str(path))
         L.oracle_scene_create.restype = C.c_void_p
         L.oracle_scene_create.argtypes = [C.c_int, f32p, i32p, i32p, C.c_int, f32p,
                                           C.c_int, f32p, f32p, C.c_int, C.c_void_p, C.c_void_p]
@@ -55,7 +56,26 @@ class Oracle:
         L.oracle_sin.argtypes = [C.c_float]
         L.oracle_cos.restype = C.c_float
         L.oracle_cos.argtypes = [C.c_float]
+        L.oracle_warps.argtypes = [f32p, C.c_float, f32p]
+        L.oracle_spawn.argtypes = [f32p, f32p, f32p, f32p]
+        L.oracle_distribution1d.argtypes = [C.c_int, f32p, C.c_float, f32p]
         self.L = L
+
+    def warps(self, u, cos_max):
+        out = np.zeros(13, np.float32)
+        self.L.oracle_warps(np.ascontiguousarray(u, np.float32), cos_max, out)
+        return out
+
+    def spawn(self, p, ng, d):
+        out = np.zeros(3, np.float32)
+        self.L.oracle_spawn(*(np.ascontiguousarray(x, np.float32) for x in (p, ng, d)), out)
+        return out
+
+    def distribution1d(self, func, u):
+        out = np.zeros(5, np.float32)
+        func = np.ascontiguousarray(func, np.float32)
+        self.L.oracle_distribution1d(len(func), func, u, out)
+        return out
 
     def scene(self, rec):
         return OracleScene(self, rec)
@@ -211,6 +231,7 @@ class Ref:
         L.ref_light_sample_le.argtypes = [C.c_int, f32p, f32p, f32p, C.c_float, C.c_float,
                                           f32p, f32p, f32p]
         L.ref_spawn.argtypes = [f32p, f32p, f32p, f32p]
+        L.ref_distribution1d.argtypes = [C.c_int, f32p, C.c_float, f32p]
         L.ref_load_image.restype = C.c_int
         L.ref_load_image.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), f32p, C.c_int]
         L.ref_envmap_new.restype = C.c_void_p
@@ -312,6 +333,12 @@ class Ref:
     def spawn(self, p, ng, d):
         out = np.zeros(3, np.float32)
         self.L.ref_spawn(self._f(p), self._f(ng), self._f(d), out)
+        return out
+
+    def distribution1d(self, func, u):
+        out = np.zeros(5, np.float32)
+        func = self._f(func)
+        self.L.ref_distribution1d(len(func), func, u, out)
         return out
 
 

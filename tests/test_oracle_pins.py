@@ -359,3 +359,87 @@ def test_metal_presets_match_reference(ref):
     eta, k = ref.metal_preset("gold")
     assert bits_equal(m[1:4], eta) and bits_equal(m[4:7], k)
     assert m[8] == ref.L.ref_roughness_to_alpha(float(m[7]))
+
+
+# ------------------------------------------------------------------ warps, spawn, Distribution1D
+def _unit_pairs(rng, n):
+    """Random pairs plus the edges a Sobol draw can take (0, 1.0f exactly)."""
+    u = rng.random((n, 2)).astype(np.float32)
+    edge = np.array([[0, 0], [0, 1], [1, 0], [1, 1], [0.5, 0.5], [np.float32(1 - 2 ** -24), 0.25],
+                     [2 ** -20, 0.75]], np.float32)
+    return np.concatenate([edge, u])
+
+
+def test_warps_match_reference(oracle, ref):
+    """uniformSampleTriangle bit for bit; uniformSampleDisk,
+    cosineSampleHemisphere, uniformSampleSphere and uniformSampleCone
+    (sampling.cpp:5-61) bit for bit where no sine / cosine is involved (the
+    sphere's and cone's z) and within 4e-6 elsewhere (sinf / cosf in _ref,
+    the FP64 routines rounded once in the oracle); most outputs agree exactly."""
+    rng = np.random.default_rng(11)
+    exact = total = 0
+    for cos_max in (0.0, 0.5, np.cos(np.radians(15.0)), 0.999):
+        for u in _unit_pairs(rng, 400):
+            a = oracle.warps(u, cos_max)
+            b = ref.warps(u, cos_max)
+            assert bits_equal(a[:2], b[:2]), (u, a[:2], b[:2])          # triangle
+            assert bits_equal(a[9], b[9]) and bits_equal(a[12], b[12])   # sphere z, cone z
+            assert np.allclose(a, b, rtol=0, atol=4e-6), (u, cos_max, a - b)
+            exact += int(np.sum(a.view(np.int32) == b.view(np.int32)))
+            total += a.size
+    assert exact / total > 0.9, exact / total
+
+
+def test_spawn_matches_reference(oracle, ref):
+    """offsetRayOrigin (geometry.h:850-863) through Intersection::spawnRay,
+    bit for bit, including zero components of p and n, d perpendicular to n
+    (offset along +n) and large magnitudes (the 1-ulp nudges)."""
+    rng = np.random.default_rng(5)
+    cases = []
+    for _ in range(3000):
+        p = (rng.standard_normal(3) * 10.0 ** rng.integers(-3, 4)).astype(np.float32)
+        n = rng.standard_normal(3)
+        n = (n / np.linalg.norm(n)).astype(np.float32)
+        d = rng.standard_normal(3).astype(np.float32)
+        cases.append((p, n, d))
+    axis = np.eye(3, dtype=np.float32)
+    for k in range(3):
+        cases.append((np.zeros(3, np.float32), axis[k], axis[(k + 1) % 3]))          # p = 0
+        cases.append((np.array([1e3, 0, -2], np.float32), axis[k], -axis[k]))        # d against n
+        cases.append((np.array([0.5, -0.25, 7], np.float32), -axis[k], axis[(k + 1) % 3]))  # d . n = 0
+        cases.append((np.array([1e-30, 3, 0], np.float32), axis[k], axis[k]))
+    for p, n, d in cases:
+        a, b = oracle.spawn(p, n, d), ref.spawn(p, n, d)
+        assert bits_equal(a, b), (p, n, d, a, b)
+    # the origin moves off the surface along the side d points to
+    p, n = np.array([1.0, 2.0, 3.0], np.float32), np.array([0, 0, 1], np.float32)
+    assert oracle.spawn(p, n, [0, 0, 1])[2] > 3 and oracle.spawn(p, n, [0, 0, -1])[2] < 3
+
+
+def test_distribution1d_matches_reference(oracle, ref):
+    """Distribution1D::sampleDiscrete / sampleContinuous (sampling.h:71-125)
+    bit for bit against _ref, on random, sparse, all-zero and one-entry
+    functions and on u at 0, exactly 1, and exactly on CDF steps; plus the
+    closed forms: the discrete pdf is func[i] / sum(func), the chosen entry's
+    CDF interval holds u, and the continuous value lies in that entry."""
+    rng = np.random.default_rng(9)
+    funcs = [np.array([1.0], np.float32), np.zeros(4, np.float32), np.array([0, 3, 0, 1], np.float32),
+             np.array([2.5, 2.5, 2.5], np.float32)]
+    funcs += [(rng.random(n) * (rng.random(n) > 0.3)).astype(np.float32) for n in (2, 5, 9, 17, 33)]
+    for f in funcs:
+        n = len(f)
+        tot = float(np.sum(f, dtype=np.float64))
+        cdf = np.concatenate([[0.0], np.cumsum(f.astype(np.float64))]) / tot if tot > 0 else np.arange(n + 1) / n
+        us = list(rng.random(50).astype(np.float32)) + [np.float32(0), np.float32(1)]
+        us += [np.float32(c) for c in cdf[1:-1]]
+        for u in us:
+            a, b = oracle.distribution1d(f, u), ref.distribution1d(f, u)
+            assert bits_equal(a, b), (f, u, a, b)
+            i = int(a[0])
+            assert 0 <= i < n and int(a[4]) == i
+            if tot > 0:
+                assert a[1] == pytest.approx(f[i] / tot, rel=1e-5)
+                assert a[3] == pytest.approx(n * f[i] / tot, rel=1e-5)
+                if 0 < u < 1:
+                    assert cdf[i] - 1e-6 <= u <= cdf[i + 1] + 1e-6
+            assert i / n - 1e-6 <= a[2] <= (i + 1) / n + 1e-6

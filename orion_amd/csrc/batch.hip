@@ -225,7 +225,16 @@ struct WaveRunner {
       Wave wi = w;  // bounce it traces the ray lengths of counter set it % 3
       wi.cnt_ray = w.cnt + (3 * (it % 3) + 1) * w.cnt_stride;
       wi.cnt_shadow = w.cnt + (3 * (it % 3) + 2) * w.cnt_stride;
-      if (split) {
+      if (it == 0) {
+        // camera rays only: no shadow rays exist before the first shading,
+        // and at most seg closest-hit entries per segment (an empty any-hit
+        // launch of the full grid had cost 1 ms beside this one at C2)
+        Wave w0 = wi;
+        w0.first_bounce = 1;
+        if (tm) tm->mark(split ? 0 : 3);
+        launch_trace(dj, w0, false, ring, chunk, multi, st);
+        if (tm) tm->mark(-1);
+      } else if (split) {
         if (tm) tm->mark(0);
         launch_trace(dj, wi, false, ring, chunk, multi, st);
         if (tm) { tm->mark(-1); tm->mark(1); }

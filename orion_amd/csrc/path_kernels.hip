@@ -106,8 +106,9 @@ struct WaveRays {
 // (XCD-contiguous), `ring` the block's LDS.
 template <bool ANY, int RING, bool MULTI>
 __device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int chunk, int lb, int *ring) {
-  const int seg_entries = ANY ? w.seg : 2 * w.seg;
-  const int wps = (seg_entries + chunk - 1) / chunk;
+  const int seg_entries = ANY ? w.seg : 2 * w.seg;  // the queue's stride per segment
+  const int cover = w.first_bounce ? w.seg : seg_entries;  // entries per segment this launch covers
+  const int wps = (cover + chunk - 1) / chunk;
   // the wave's index is uniform: read from lane 0 so that everything derived
   // from it (segment, job, scene pointers) stays in scalar registers
   const int gw = lb * (kTraceBlock / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -462,7 +463,9 @@ int trace_pair_grid(int nseg, int seg, int chunk) {
 void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk, bool multi, hipStream_t st) {
   ring = ring_variant(ring);
   chunk = trace_chunk(chunk);
-  const dim3 g(trace_grid(w.nseg, w.seg, any, chunk));
+  // a first bounce's closest-hit launch has the any-hit grid's shape (seg
+  // entries per segment)
+  const dim3 g(trace_grid(w.nseg, w.seg, any || w.first_bounce, chunk));
   if (multi) {
     if (any) ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<true, ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
     else ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<false, ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));

@@ -116,6 +116,7 @@ int WaveBuffers::ensure(int N) {
   w.cnt_ray = w.cnt + nseg;
   w.cnt_shadow = w.cnt + 2 * nseg;
   w.shade_parts = 1;
+  w.first_bounce = 0;
   w.seg_job = nullptr;
   w.seg_perm = nullptr;
   return ORION_OK;

@@ -330,23 +330,28 @@ __device__ inline void resolve_direct(const DScene &sc, const Wave &w, int p, in
   acc(*L, sp3(w.nbr[p], w.nbg[p], w.nbb[p]) * (Ld / w.sel[p]));
 }
 
-// First half: light choice (power CDF, or uniform when `uniform`), light and
-// BSDF samples, the light-sampled term and the two rays.  Draws exactly the
-// reference's dimensions.  Returns the F_PEND | F_SHADOW | F_MIS bits.
-__device__ inline int setup_direct(const DScene &sc, const Wave &w, int p, const Bsdf &bs, const Isect &is, V3 d,
-                                   Sp beta, uint32_t index, int *dim, bool uniform) {
-  if (sc.n_lights == 0) return 0;
-  const int N = w.N;
-  float sel;
-  int l;
+// uniformSampleOneLight's light choice (core/integrator.cpp:8-16): by the
+// power CDF, or uniform when `uniform`; one Sobol draw.  Returns the light,
+// or -1 when there is nothing to sample: no lights (no draw) or a zero
+// selection pdf (the reference returns after the draw).
+__device__ inline int nee_pick(const DScene &sc, uint32_t index, int *dim, bool uniform, float *sel) {
+  if (sc.n_lights == 0) return -1;
   if (!uniform) {
-    l = sample_light(sc, sobol(sc.sobol, index, (*dim)++), &sel);
-    if (sel == 0) return 0;
-  } else {
-    int nl = sc.n_lights;
-    l = min((int)(sobol(sc.sobol, index, (*dim)++) * nl), nl - 1);
-    sel = 1.0f / nl;
+    const int l = sample_light(sc, sobol(sc.sobol, index, (*dim)++), sel);
+    return *sel == 0 ? -1 : l;
   }
+  const int nl = sc.n_lights;
+  *sel = 1.0f / nl;
+  return min((int)(sobol(sc.sobol, index, (*dim)++) * nl), nl - 1);
+}
+
+// estimateDirect's sampling half (core/integrator.cpp:29-90) for light l,
+// chosen with probability sel: light and BSDF samples (four draws), the
+// light-sampled term and the shadow and MIS rays.  Returns the F_PEND |
+// F_SHADOW | F_MIS bits.
+__device__ inline int nee_sample(const DScene &sc, const Wave &w, int p, int l, float sel, const Bsdf &bs,
+                                 const Isect &is, V3 d, Sp beta, uint32_t index, int *dim) {
+  const int N = w.N;
   float lu1 = sobol(sc.sobol, index, (*dim)++), lu0 = sobol(sc.sobol, index, (*dim)++);
   float bu1 = sobol(sc.sobol, index, (*dim)++), bu0 = sobol(sc.sobol, index, (*dim)++);
   const int nonspec = ALL & ~SPEC;
@@ -399,6 +404,16 @@ __device__ inline int setup_direct(const DScene &sc, const Wave &w, int p, const
   w.sel[p] = sel;
   w.nbr[p] = beta.r; w.nbg[p] = beta.g; w.nbb[p] = beta.b;
   return F_PEND | (hs ? F_SHADOW : 0) | (hm ? F_MIS : 0);
+}
+
+// Both halves in one place (the SPPM camera pass).  Draws exactly the
+// reference's dimensions.
+__device__ inline int setup_direct(const DScene &sc, const Wave &w, int p, const Bsdf &bs, const Isect &is, V3 d,
+                                   Sp beta, uint32_t index, int *dim, bool uniform) {
+  float sel;
+  const int l = nee_pick(sc, index, dim, uniform, &sel);
+  if (l < 0) return 0;
+  return nee_sample(sc, w, p, l, sel, bs, is, d, beta, index, dim);
 }
 
 __device__ __forceinline__ Sp area_emission(const DScene &sc, const Isect &is, V3 w) {  // diffuse.cpp:53-56

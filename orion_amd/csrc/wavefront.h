@@ -109,6 +109,10 @@ struct Wave {
   // entries per segment), nseg segments in use, lengths per segment
   int seg, nseg;
   int *cnt_ray, *cnt_shadow;  // the lengths a traversal launch reads
+  // 1 for a wave's first bounce: its closest-hit queues hold at most seg
+  // camera rays per segment (later bounces up to 2 seg: continuation and
+  // MIS rays), so its launch covers seg entries per segment
+  int first_bounce;
   // Path pipeline queue lengths, three rotating sets: bounce b traces the
   // ray lengths of set b % 3, shades the path lengths of set b % 3, appends
   // to set (b + 1) % 3 and zeroes set (b + 2) % 3 (nothing reads it during

@@ -129,21 +129,49 @@ def cpu_baseline(hs, depth, target_s):
     }
 
 
+def c4_job_specs(n_jobs, seed):
+    """The batched-launch mix: per job (triangles, width, height, spp, depth,
+    scene seed): displaced icosphere of 1k-500k triangles, log-uniform; film
+    sides uniform in [128, 1024]; spp uniform in [4, 64]; depth 5-8."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_jobs):
+        tris = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+        W, H = (int(x) for x in rng.integers(128, 1025, 2))
+        spp, depth = int(rng.integers(4, 65)), int(rng.integers(5, 9))
+        out.append((tris, W, H, spp, depth, seed * 1000 + i))
+    return out
+
+
+def c5_scene_specs(seed):
+    """The serving leg's four resident scenes: (triangles, scene seed),
+    displaced icospheres of 5.3M, 1.4M, 1.4M and 0.34M target triangles."""
+    return [(tris, seed * 100 + i) for i, tris in enumerate([5.3e6, 1.4e6, 1.4e6, 3.4e5])]
+
+
+def c5_request_specs(n_req, seed):
+    """Serving requests: (scene index, width, height, spp, depth), film sides
+    uniform in [256, 1024], spp uniform in [16, 256], depth 5-8; the rng
+    also draws the arrival gaps afterwards (returned)."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n_req):
+        k = int(rng.integers(0, 4))
+        W, H = (int(x) for x in rng.integers(256, 1025, 2))
+        reqs.append((k, W, H, int(rng.integers(16, 257)), int(rng.integers(5, 9))))
+    return reqs, rng
+
+
 def batch_leg(oa, dev, n_jobs, seed):
     """Batched launch of n_jobs seeded procedural jobs (displaced icosphere of
     1k-500k triangles, log-uniform; film sides uniform in [128, 1024]; spp
     uniform in [4, 64]; depth 5-8; material drawn per mesh) against the plain
     path rendering the same jobs one at a time.  The tuning comes from the
     autotuner on the same mix."""
-    rng = np.random.default_rng(seed)
     jobs, scenes = [], []
     t0 = time.perf_counter()
-    for i in range(n_jobs):
-        tris = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
-        W, H = (int(x) for x in rng.integers(128, 1025, 2))
-        spp, depth = int(rng.integers(4, 65)), int(rng.integers(5, 9))
-        ds = oa.DeviceScene(dev, oa.HostScene.generate("batch_job", seed=seed * 1000 + i,
-                                                       params=[tris, W, H, 3]))
+    for tris, W, H, spp, depth, sseed in c4_job_specs(n_jobs, seed):
+        ds = oa.DeviceScene(dev, oa.HostScene.generate("batch_job", seed=sseed, params=[tris, W, H, 3]))
         scenes.append(ds)
         jobs.append((ds, W, H, spp, depth))
     setup_s = time.perf_counter() - t0
@@ -240,17 +268,11 @@ def serve_leg(oa, dev, n_req, load, budget_gib, seed, policy="pack", also=("aged
     times on the same arrival schedule.  The arrival rate is `load` x the
     multi-sample plain path's capacity (the one-sample-per-wave plain path
     cannot keep up with it, so its queue grows over the stream)."""
-    rng = np.random.default_rng(seed)
-    scenes = []
-    for i, tris in enumerate([5.3e6, 1.4e6, 1.4e6, 3.4e5]):
-        scenes.append(oa.DeviceScene(dev, oa.HostScene.generate(
-            "batch_job", seed=seed * 100 + i, params=[tris, 512, 512, 3])))
+    scenes = [oa.DeviceScene(dev, oa.HostScene.generate("batch_job", seed=sseed, params=[tris, 512, 512, 3]))
+              for tris, sseed in c5_scene_specs(seed)]
     tris_tot = sum(ds.stats()["n_tris"] for ds in scenes)
-    reqs = []
-    for i in range(n_req):
-        ds = scenes[int(rng.integers(0, 4))]
-        W, H = (int(x) for x in rng.integers(256, 1025, 2))
-        reqs.append((ds, W, H, int(rng.integers(16, 257)), int(rng.integers(5, 9))))
+    specs, rng = c5_request_specs(n_req, seed)
+    reqs = [(scenes[k], W, H, spp, depth) for k, W, H, spp, depth in specs]
     paths = sum(W * H * spp for _, W, H, spp, _ in reqs)
     oa.render_batch(dev, [(scenes[0], 256, 256, 1, 5)])  # warm
     service, service1 = [], []

@@ -241,6 +241,15 @@ typedef struct {
   int *M;
   int *grid_info;
   float *grid_bounds;
+  /* The radix-sorted (hash key, pixel) pairs, pairs[2 i] = key, pairs[2 i
+   * + 1] = pixel, keys ascending and each key's pixels in the order the
+   * reference's per-cell lists receive them (sppm.cpp:170-193); written
+   * when there are at most max_pairs (ORION_ERR_ARG otherwise).  cell_range
+   * [2 hash_size]: each cell's [start, end) in that order (0, 0 when
+   * empty).  Either pointer may be NULL. */
+  int *pairs;
+  int64_t max_pairs;
+  int *cell_range;
 } orion_sppm_debug;
 int orion_render_sppm(orion_scene s, int n_iters, int photons_per_iter, float init_radius,
                       int max_depth, int hash_size, const orion_sppm_debug *dbg,
@@ -312,6 +321,7 @@ typedef struct {
   double busy_ms;
   int64_t mixed_waves;  /* waves that carried more than one job */
   int max_jobs_in_wave;
+  int64_t hbm_peak;      /* largest film memory reserved at once (admission) */
 } orion_server_stats;
 int orion_server_get_stats(orion_server s, orion_server_stats *st);
 int orion_server_destroy(orion_server s);

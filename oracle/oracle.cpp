@@ -1585,7 +1585,8 @@ int oracle_path_samples(void *s, int n, const int *pixel, const int *kk, int max
 int oracle_render_sppm(void *s, int nIterations, int photonPerIteration, float initialSearchRadius,
                        int maxDepth, int hashSizeArg, float *rgb, int *count, int threads,
                        int debug_iter, int *vp_cell_count, int *photon_cell_count, int *M_debug,
-                       int *grid_info, float *grid_bounds, float *vp_out) {
+                       int *grid_info, float *grid_bounds, float *vp_out, int *pairs_out,
+                       int64_t max_pairs) {
   const Scene &sc = *(const Scene *)s;
   int W = sc.W, H = sc.H;
   int nPixels = W * H;
@@ -1691,6 +1692,11 @@ int oracle_render_sppm(void *s, int nIterations, int photonPerIteration, float i
       }
       if (grid_bounds)
         for (int a = 0; a < 3; ++a) { grid_bounds[a] = blo[a]; grid_bounds[3 + a] = bhi[a]; }
+      if (pairs_out && any && n_pairs <= max_pairs) {  // the lists by key, each in insertion order
+        int64_t k = 0;
+        for (int h = 0; h < hashSize; ++h)
+          for (int pi : grid[h]) { pairs_out[2 * k] = h; pairs_out[2 * k + 1] = pi; ++k; }
+      }
       if (vp_out)
         for (int i = 0; i < nPixels; ++i) {
           const Pixel &px = pixels[i];

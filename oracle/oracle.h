@@ -59,12 +59,17 @@ int oracle_path_samples(void *scene, int n, const int *pixel, const int *k,
  * hash cell), photon_cell_count[hash_size] (in-bounds photon deposits per
  * cell), M_debug[nPixels] (photon count per pixel before the update),
  * grid_info[8] = gridRes xyz, n_pairs; grid_bounds[6] = bounds lo, hi;
- * vp_out[nPixels * 5] = visible point x, y, z, radius, valid. */
+ * vp_out[nPixels * 5] = visible point x, y, z, radius, valid;
+ * pairs_out[2 * n_pairs] = the per-cell lists (sppm.cpp:170-193) as (hash
+ * key, pixel) pairs, keys ascending, each cell's pixels in insertion order
+ * (pixel order: the lists are built serially), written when n_pairs <=
+ * max_pairs. */
 int oracle_render_sppm(void *scene, int n_iters, int photons_per_iter,
                        float init_radius, int max_depth, int hash_size,
                        float *rgb_sum, int *count, int threads, int debug_iter,
                        int *vp_cell_count, int *photon_cell_count, int *M_debug,
-                       int *grid_info, float *grid_bounds, float *vp_out);
+                       int *grid_info, float *grid_bounds, float *vp_out,
+                       int *pairs_out, int64_t max_pairs);
 
 /* Building blocks exposed for the unit pins. */
 float oracle_sobol(uint64_t index, int dim);

@@ -63,7 +63,8 @@ class Tuning(C.Structure):
 class SppmDebug(C.Structure):
     _fields_ = [("debug_iter", C.c_int), ("vp_cell_count", C.c_void_p),
                 ("photon_cell_count", C.c_void_p), ("M", C.c_void_p), ("grid_info", C.c_void_p),
-                ("grid_bounds", C.c_void_p)]
+                ("grid_bounds", C.c_void_p), ("pairs", C.c_void_p), ("max_pairs", C.c_int64),
+                ("cell_range", C.c_void_p)]
 
 
 class JobDesc(C.Structure):
@@ -75,7 +76,7 @@ class ServerStats(C.Structure):
     _fields_ = [("jobs_done", C.c_int64), ("waves", C.c_int64), ("paths", C.c_int64),
                 ("closest_rays", C.c_int64), ("shadow_rays", C.c_int64), ("active_jobs", C.c_int),
                 ("queued_jobs", C.c_int), ("hbm_in_use", C.c_int64), ("busy_ms", C.c_double),
-                ("mixed_waves", C.c_int64), ("max_jobs_in_wave", C.c_int)]
+                ("mixed_waves", C.c_int64), ("max_jobs_in_wave", C.c_int), ("hbm_peak", C.c_int64)]
 
 
 # name -> (restype, argtypes); the header's full export list
@@ -415,7 +416,9 @@ class DeviceScene:
         _check(lib().orion_film_read_averaged(self.h, rgb))
         return rgb.reshape(self.H, self.W, 3)
 
-    def render_sppm(self, iters, photons, radius, max_depth, hash_size=0, debug_iter=-1):
+    def render_sppm(self, iters, photons, radius, max_depth, hash_size=0, debug_iter=-1, pairs=False):
+        """pairs: also return the debug iteration's radix-sorted (key, pixel)
+        pairs as an (n_pairs, 2) array and each cell's [start, end) range."""
         W, H = self.W, self.H
         hs = hash_size if hash_size > 0 else W * H
         dbg = {}
@@ -424,13 +427,23 @@ class DeviceScene:
             dbg = dict(vp_cell_count=np.zeros(hs, np.int32), photon_cell_count=np.zeros(hs, np.int32),
                        M=np.zeros(W * H, np.int32), grid_info=np.zeros(8, np.int32),
                        grid_bounds=np.zeros(6, np.float32))
+            cap = 27 * W * H if pairs else 0  # a visible point's box spans at most 3 cells per axis
+            if pairs:
+                dbg["pairs"] = np.zeros(2 * cap, np.int32)
+                dbg["cell_range"] = np.zeros(2 * hs, np.int32)
             d = SppmDebug(debug_iter, dbg["vp_cell_count"].ctypes.data,
                           dbg["photon_cell_count"].ctypes.data, dbg["M"].ctypes.data,
-                          dbg["grid_info"].ctypes.data, dbg["grid_bounds"].ctypes.data)
+                          dbg["grid_info"].ctypes.data, dbg["grid_bounds"].ctypes.data,
+                          dbg["pairs"].ctypes.data if pairs else None, cap,
+                          dbg["cell_range"].ctypes.data if pairs else None)
             dptr = C.byref(d)
         st = RenderStats()
         _check(lib().orion_render_sppm(self.h, iters, photons, radius, max_depth, hash_size, dptr,
                                        C.byref(st)))
+        if pairs:
+            n = int(dbg["grid_info"][3])
+            dbg["pairs"] = dbg["pairs"][:2 * n].reshape(n, 2)
+            dbg["cell_range"] = dbg["cell_range"].reshape(hs, 2)
         return st, dbg
 
     def close(self):

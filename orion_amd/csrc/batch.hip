@@ -743,6 +743,7 @@ struct orion_server_s {
   std::chrono::steady_clock::time_point t0;
   int64_t jobs_done = 0, waves = 0, paths = 0, closest = 0, shadow = 0, mixed_waves = 0;
   int max_jobs_in_wave = 0;
+  int64_t hbm_peak = 0;
   double busy_ms = 0;
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
   void loop();
@@ -768,6 +769,7 @@ void orion_server_s::loop() {
         if (!active.empty() && in_use + j->bytes > budget) break;
         queue.pop_front();
         in_use += j->bytes;
+        hbm_peak = std::max(hbm_peak, in_use);
         admitted.push_back(j);
         active.push_back(j);
       }
@@ -1024,6 +1026,7 @@ int orion_server_get_stats(orion_server s, orion_server_stats *st) {
   st->busy_ms = s->busy_ms;
   st->mixed_waves = s->mixed_waves;
   st->max_jobs_in_wave = s->max_jobs_in_wave;
+  st->hbm_peak = s->hbm_peak;
   return ORION_OK;
 }
 

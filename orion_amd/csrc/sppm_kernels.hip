@@ -957,6 +957,27 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
       }
       if (dbg->grid_bounds)
         for (int a = 0; a < 6; ++a) dbg->grid_bounds[a] = fctl[a];
+      if (dbg->pairs) {
+        if (n_pairs > dbg->max_pairs) {
+          set_error("debug pair buffer holds " + std::to_string(dbg->max_pairs) + " pairs, the grid has " +
+                    std::to_string(n_pairs));
+          return ORION_ERR_ARG;
+        }
+        std::vector<int> k((size_t)n_pairs), v((size_t)n_pairs);
+        if (n_pairs > 0) {
+          HIP_TRY(hipMemcpyAsync(k.data(), g.keys[0], (size_t)n_pairs * 4, hipMemcpyDeviceToHost, st));
+          HIP_TRY(hipMemcpyAsync(v.data(), g.vals[0], (size_t)n_pairs * 4, hipMemcpyDeviceToHost, st));
+          HIP_TRY(hipStreamSynchronize(st));
+        }
+        for (int i = 0; i < n_pairs; ++i) { dbg->pairs[2 * i] = k[i]; dbg->pairs[2 * i + 1] = v[i]; }
+      }
+      if (dbg->cell_range) {
+        std::vector<int> cs((size_t)hash_size), ce((size_t)hash_size);
+        HIP_TRY(hipMemcpyAsync(cs.data(), g.cell_start, (size_t)hash_size * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(ce.data(), g.cell_end, (size_t)hash_size * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int h = 0; h < hash_size; ++h) { dbg->cell_range[2 * h] = cs[h]; dbg->cell_range[2 * h + 1] = ce[h]; }
+      }
     }
     // --- photon pass (sppm.cpp:199-273)
     if (photons > 0) {

@@ -37,7 +37,7 @@ class Oracle:
         L.oracle_render_sppm.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_int,
                                          C.c_int, f32p, i32p, C.c_int, C.c_int,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                         C.c_void_p, C.c_void_p]
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         L.oracle_sobol.restype = C.c_float
         L.oracle_sobol.argtypes = [C.c_uint64, C.c_int]
         L.oracle_camera_ray.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint64, f32p]
@@ -128,23 +128,33 @@ class OracleScene:
         return out.reshape(-1, 3)
 
     def render_sppm(self, iters, photons, radius, max_depth, hash_size=0, debug_iter=-1,
-                    threads=0):
+                    threads=0, pairs=False):
+        """pairs: also return the debug iteration's per-cell lists as an
+        (n_pairs, 2) array of (hash key, pixel), keys ascending."""
         W, H = self.rec.width, self.rec.height
         hs = hash_size if hash_size > 0 else W * H
         rgb = np.zeros(W * H * 3, np.float32)
         cnt = np.zeros(W * H, np.int32)
         dbg = {}
+        cap = 0
         if debug_iter >= 0:
             dbg = dict(vp_cell_count=np.zeros(hs, np.int32),
                        photon_cell_count=np.zeros(hs, np.int32),
                        M=np.zeros(W * H, np.int32), grid_info=np.zeros(8, np.int32),
                        grid_bounds=np.zeros(6, np.float32), vp=np.zeros(W * H * 5, np.float32))
+            if pairs:  # a visible point's box spans at most 3 cells per axis
+                cap = 27 * W * H
+                dbg["pairs"] = np.zeros(2 * cap, np.int32)
         self.o.L.oracle_render_sppm(self.h, iters, photons, radius, max_depth, hash_size, rgb,
                                     cnt, threads, debug_iter,
                                     _opt(dbg.get("vp_cell_count")),
                                     _opt(dbg.get("photon_cell_count")), _opt(dbg.get("M")),
                                     _opt(dbg.get("grid_info")), _opt(dbg.get("grid_bounds")),
-                                    _opt(dbg.get("vp")))
+                                    _opt(dbg.get("vp")), _opt(dbg.get("pairs")), cap)
+        if "pairs" in dbg:
+            n = int(dbg["grid_info"][3])
+            assert n <= cap, (n, cap)
+            dbg["pairs"] = dbg["pairs"][:2 * n].reshape(n, 2)
         return rgb.reshape(H, W, 3), cnt.reshape(H, W), dbg
 
     def camera_ray(self, i, j, index):

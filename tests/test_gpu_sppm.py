@@ -63,3 +63,30 @@ def test_sppm_sky(device, oracle):
     """Photons from an envmap and a distant light (world-disk emission), the
     camera pass adding the map on escape at any depth."""
     _compare(device, oracle, [64, 64], 3, 1 << 16, 0.05, 0, 2, scene="sky", depth=6)
+
+
+def test_sppm_c3_three_iterations(device, oracle):
+    """C3's sizes over three iterations: iteration 2's grid comes from radii
+    the first two updates shrank, and the camera passes of iterations 1 and 2
+    ran on the second stream beside the previous iteration's photon pass
+    (sppm.cpp:79-321)."""
+    _compare(device, oracle, [6, 512, 512], 3, 1 << 20, 0.01, 1 << 22, 2)
+
+
+def test_sppm_c3_sorted_pairs_and_cell_ranges(device, oracle):
+    """The radix-sorted (hash key, pixel) pairs and every cell's [start, end)
+    equal the reference's per-cell lists (sppm.cpp:170-193) concatenated by
+    key, bit for bit, hash collisions' repeated pixels included (C3 sizes,
+    iteration 1, after one radius update)."""
+    hs = oa.HostScene.generate("caustics", params=[6, 512, 512])
+    ds = oa.DeviceScene(device, hs)
+    _, gd = ds.render_sppm(2, 1 << 20, 0.01, 10, hash_size=1 << 22, debug_iter=1, pairs=True)
+    _, _, od = oracle.scene(hs).render_sppm(2, 1 << 20, 0.01, 10, hash_size=1 << 22, debug_iter=1, pairs=True)
+    gp, op = gd["pairs"], od["pairs"]
+    print(f"pairs: {len(gp)} (oracle {len(op)}), distinct keys {len(np.unique(gp[:, 0]))}")
+    assert len(gp) > 100000
+    np.testing.assert_array_equal(gp, op)
+    counts = od["vp_cell_count"].astype(np.int64)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    want = np.stack([np.where(counts > 0, starts, 0), np.where(counts > 0, starts + counts, 0)], 1)
+    np.testing.assert_array_equal(gd["cell_range"], want)

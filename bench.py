@@ -34,10 +34,20 @@ TRI_BYTES = 48       # p0, e1, e2 + primitive id
 RAY_IO_BYTES = 48    # 32 B ray in (o, d, tmax, id) + 16 B hit out (t, prim, u, v)
 SHADOW_IO_BYTES = 36  # 32 B ray in + 4 B visibility out
 SHADE_BYTES = 292     # per shaded path segment
+# SPPM photon shading: per photon hit (queue entry 4, hit 12, shading record
+# 80, direction 12, beta 12, dimension 4 read; ray 24, beta 12, dimension 4,
+# queue 4 written), per in-bounds cell lookup (start, end), per visible point
+# scanned (pixel 4, radius 4, position 12), per deposit (material 4, frame 48,
+# wo 12 read; Phi 12 + M 4 atomically added)
+PHOTON_SEG_BYTES = 168
+CELL_RANGE_BYTES = 8
+VP_SCAN_BYTES = 20
+DEPOSIT_BYTES = 80
 # a bounce's traversal: the closest-hit and any-hit kernels, launched side by
 # side on two streams (one fork / join per bounce)
 TRACE_KERNELS = ("k_trace<false, 16, false>", "k_trace<true, 16, false>")
 TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_icosphere_L8_1024x1024_64spp.json"
+SPPM_TRAFFIC_FILE = ROOT / "profiles" / "pmc_traffic_sppm_caustics_512x512_2e20_photons.json"
 
 
 def load_traffic(a):
@@ -232,6 +242,15 @@ def batch_leg(oa, dev, n_jobs, seed):
     }
 
 
+def sppm_traffic(kernel):
+    """HBM bytes per dispatch of an SPPM kernel from the committed rocprofv3
+    --pmc summary of this configuration, or None when absent."""
+    try:
+        return json.loads(SPPM_TRAFFIC_FILE.read_text())["kernels"][kernel]["hbm_bytes_per_dispatch"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def sppm_leg(oa, dev, iters):
     """SPPM on the caustics scene: glass icosphere (81,920 triangles, eta 1.5)
     on a matte plane under a 30 degree spot light, 512x512, 2^20 photons per
@@ -242,8 +261,25 @@ def sppm_leg(oa, dev, iters):
     ds.film_clear()
     dev.synchronize()
     t0 = time.perf_counter()
-    st, _ = ds.render_sppm(iters, 1 << 20, 0.01, 10, hash_size=1 << 22)
+    st, _ = ds.render_sppm(iters, 1 << 20, 0.01, 10, hash_size=1 << 22, profile=True)
     el = time.perf_counter() - t0
+    d = st.as_dict()
+    # roofline of the dominant kernel: the photon bounces' closest-hit
+    # traversal (k_trace_photons), HIP events around each launch on its stream
+    tl = max(d["photon_trace_launches"], 1)
+    trace_bytes = (d["photon_rays"] * RAY_IO_BYTES + d["node_visits_photon"] * NODE_BYTES
+                   + d["tri_tests_photon"] * TRI_BYTES)
+    t_launch = d["photon_trace_ms"] / 1e3 / tl
+    achieved = trace_bytes / tl / t_launch / 1e9 if t_launch > 0 else 0.0
+    rays = max(d["photon_rays"], 1)
+    # photon shading (k_photon_shade): per photon hit its segment's state and
+    # shading record, per in-bounds lookup the cell range, per scanned entry
+    # the visible point's pixel, radius and position, per deposit its frame,
+    # direction and material index read and Phi, M added
+    sl = max(d["photon_shade_launches"], 1)
+    shade_bytes = (d["photon_rays"] * PHOTON_SEG_BYTES + d["photon_lookups"] * CELL_RANGE_BYTES
+                   + d["photon_vp_scanned"] * VP_SCAN_BYTES + d["deposits"] * DEPOSIT_BYTES)
+    s_launch = d["photon_shade_ms"] / 1e3 / sl
     return {
         "workload": f"SPPM caustics, glass icosphere ({hs.n_tris} triangles) on a matte plane, "
                     f"spot light, 512x512, {iters} iterations x 2^20 photons, r0 0.01, "
@@ -254,6 +290,26 @@ def sppm_leg(oa, dev, iters):
         "rays_per_s": round((st.closest_rays + st.shadow_rays) / el, 1),
         "deposits": int(st.deposits),
         "seconds": round(el, 3),
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": sppm_traffic("k_trace_photons<16>"),
+            "traffic_source": f"{SPPM_TRAFFIC_FILE.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE x 2.0, "
+                              f"WRITE_SIZE x 1.0, per dispatch)",
+            "kernel": "k_trace_photons<16>: one photon bounce's closest-hit traversal (the photon "
+                      "pass of every iteration, 10 bounces each)",
+            "algorithmic_bytes_per_launch": round(trace_bytes / tl),
+            "launch_ms": round(t_launch * 1e3, 4), "launches": int(tl),
+            "photon_rays": int(d["photon_rays"]),
+            "nodes_per_ray": round(d["node_visits_photon"] / rays, 2),
+            "tris_per_ray": round(d["tri_tests_photon"] / rays, 2),
+            "photon_shade": {
+                "achieved": round(shade_bytes / sl / s_launch / 1e9, 1) if s_launch > 0 else 0.0,
+                "unit": "GB/s", "launch_ms": round(s_launch * 1e3, 4),
+                "algorithmic_bytes_per_launch": round(shade_bytes / sl),
+                "lookups": int(d["photon_lookups"]), "vp_scanned": int(d["photon_vp_scanned"]),
+                "bytes": f"{PHOTON_SEG_BYTES} B per photon hit, {CELL_RANGE_BYTES} B per cell lookup, "
+                         f"{VP_SCAN_BYTES} B per visible point scanned, {DEPOSIT_BYTES} B per deposit"},
+        },
     }
 
 

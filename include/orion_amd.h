@@ -193,6 +193,14 @@ typedef struct {
   double trace_pair_ms;
   int64_t trace_pair_launches;
   int64_t shaded_segments;  /* path segments shaded (one per live path per bounce) */
+  /* SPPM photon pass: photon rays traced (all bounces), their traversal work,
+   * in-bounds grid lookups of photon hits after the first, visible-point
+   * entries scanned by those lookups; launch times when
+   * orion_sppm_debug.profile != 0 (HIP events on the photon pass's stream) */
+  int64_t photon_rays, node_visits_photon, tri_tests_photon;
+  int64_t photon_lookups, photon_vp_scanned;
+  double photon_trace_ms, photon_shade_ms;
+  int64_t photon_trace_launches, photon_shade_launches;
 } orion_render_stats;
 
 typedef struct {
@@ -250,6 +258,9 @@ typedef struct {
   int *pairs;
   int64_t max_pairs;
   int *cell_range;
+  /* != 0: time every photon-pass traversal and shading launch into the
+   * render statistics (applies whatever debug_iter is) */
+  int profile;
 } orion_sppm_debug;
 int orion_render_sppm(orion_scene s, int n_iters, int photons_per_iter, float init_radius,
                       int max_depth, int hash_size, const orion_sppm_debug *dbg,

@@ -48,7 +48,11 @@ class RenderStats(C.Structure):
                 ("trace_closest_launches", C.c_int64), ("trace_shadow_launches", C.c_int64),
                 ("shade_launches", C.c_int64), ("stack_overflows", C.c_int64),
                 ("trace_pair_ms", C.c_double), ("trace_pair_launches", C.c_int64),
-                ("shaded_segments", C.c_int64)]
+                ("shaded_segments", C.c_int64), ("photon_rays", C.c_int64),
+                ("node_visits_photon", C.c_int64), ("tri_tests_photon", C.c_int64),
+                ("photon_lookups", C.c_int64), ("photon_vp_scanned", C.c_int64),
+                ("photon_trace_ms", C.c_double), ("photon_shade_ms", C.c_double),
+                ("photon_trace_launches", C.c_int64), ("photon_shade_launches", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -64,7 +68,7 @@ class SppmDebug(C.Structure):
     _fields_ = [("debug_iter", C.c_int), ("vp_cell_count", C.c_void_p),
                 ("photon_cell_count", C.c_void_p), ("M", C.c_void_p), ("grid_info", C.c_void_p),
                 ("grid_bounds", C.c_void_p), ("pairs", C.c_void_p), ("max_pairs", C.c_int64),
-                ("cell_range", C.c_void_p)]
+                ("cell_range", C.c_void_p), ("profile", C.c_int)]
 
 
 class JobDesc(C.Structure):
@@ -416,13 +420,19 @@ class DeviceScene:
         _check(lib().orion_film_read_averaged(self.h, rgb))
         return rgb.reshape(self.H, self.W, 3)
 
-    def render_sppm(self, iters, photons, radius, max_depth, hash_size=0, debug_iter=-1, pairs=False):
+    def render_sppm(self, iters, photons, radius, max_depth, hash_size=0, debug_iter=-1, pairs=False,
+                    profile=False):
         """pairs: also return the debug iteration's radix-sorted (key, pixel)
-        pairs as an (n_pairs, 2) array and each cell's [start, end) range."""
+        pairs as an (n_pairs, 2) array and each cell's [start, end) range.
+        profile: time the photon pass's launches into the statistics."""
         W, H = self.W, self.H
         hs = hash_size if hash_size > 0 else W * H
         dbg = {}
         dptr = None
+        if profile and debug_iter < 0:
+            d = SppmDebug(-1)
+            d.profile = 1
+            dptr = C.byref(d)
         if debug_iter >= 0:
             dbg = dict(vp_cell_count=np.zeros(hs, np.int32), photon_cell_count=np.zeros(hs, np.int32),
                        M=np.zeros(W * H, np.int32), grid_info=np.zeros(8, np.int32),
@@ -435,7 +445,7 @@ class DeviceScene:
                           dbg["photon_cell_count"].ctypes.data, dbg["M"].ctypes.data,
                           dbg["grid_info"].ctypes.data, dbg["grid_bounds"].ctypes.data,
                           dbg["pairs"].ctypes.data if pairs else None, cap,
-                          dbg["cell_range"].ctypes.data if pairs else None)
+                          dbg["cell_range"].ctypes.data if pairs else None, int(profile))
             dptr = C.byref(d)
         st = RenderStats()
         _check(lib().orion_render_sppm(self.h, iters, photons, radius, max_depth, hash_size, dptr,

@@ -19,6 +19,9 @@ void launch_raygen(const Job *jobs, const Wave &w, bool multi, hipStream_t st);
 // wave64.  w.spill must hold w.spill_cap entries for each of
 // trace_grid(w.nseg, w.seg, any, chunk) * kTraceBlock threads.
 void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk, bool multi, hipStream_t st);
+// an SPPM photon bounce (closest hit, trace_grid(w.nseg, w.seg, false,
+// chunk) blocks; counted apart from camera and path rays)
+void launch_trace_photons(const Job *jobs, const Wave &w, int ring, int chunk, hipStream_t st);
 // closest-hit and any-hit queues in one launch (trace_pair_grid blocks)
 void launch_trace_pair(const Job *jobs, const Wave &w, int ring, int chunk, bool multi, hipStream_t st);
 int trace_pair_grid(int nseg, int seg, int chunk);

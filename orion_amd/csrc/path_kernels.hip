@@ -103,8 +103,9 @@ struct WaveRays {
 // chunk) of queue segment g / (waves per segment), c = g mod (waves per
 // segment); the part covers every segment at full length and waves past a
 // segment's length exit.  `lb` is the block's logical index within its part
-// (XCD-contiguous), `ring` the block's LDS.
-template <bool ANY, int RING, bool MULTI>
+// (XCD-contiguous), `ring` the block's LDS.  PHOTON: an SPPM photon bounce
+// (closest hit; its traversal work has its own counters).
+template <bool ANY, int RING, bool MULTI, bool PHOTON = false>
 __device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int chunk, int lb, int *ring) {
   const int seg_entries = ANY ? w.seg : 2 * w.seg;  // the queue's stride per segment
   const int cover = w.first_bounce ? w.seg : seg_entries;  // entries per segment this launch covers
@@ -131,8 +132,8 @@ __device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int 
   // nv / nt are the wave's totals (for the roofline's n_node / n_tri)
   if (ovf) stat_add(w.stats, S_OVERFLOW, ovf);
   if (lane == 0) {
-    stat_add(w.stats, ANY ? S_NODES_S : S_NODES_C, nv);
-    stat_add(w.stats, ANY ? S_TRIS_S : S_TRIS_C, nt);
+    stat_add(w.stats, ANY ? S_NODES_S : PHOTON ? S_NODES_P : S_NODES_C, nv);
+    stat_add(w.stats, ANY ? S_TRIS_S : PHOTON ? S_TRIS_P : S_TRIS_C, nt);
   }
 }
 
@@ -140,6 +141,14 @@ template <bool ANY, int RING, bool MULTI>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w, int chunk) {
   __shared__ int ring[RING * kTraceBlock];
   trace_block<ANY, RING, MULTI>(jobs, w, chunk, xcd_remap(blockIdx.x, gridDim.x), ring);
+}
+
+// An SPPM photon bounce's closest-hit traversal: the same engine, its own
+// counters and kernel (profiles attribute it apart from the path tracer's).
+template <int RING>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_photons(const Job *jobs, Wave w, int chunk) {
+  __shared__ int ring[RING * kTraceBlock];
+  trace_block<false, RING, false, true>(jobs, w, chunk, xcd_remap(blockIdx.x, gridDim.x), ring);
 }
 
 // Closest-hit and any-hit queues of a bounce in one launch: blocks
@@ -473,6 +482,12 @@ void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk,
     if (any) ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<true, ST, false>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
     else ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<false, ST, false>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
   }
+}
+void launch_trace_photons(const Job *jobs, const Wave &w, int ring, int chunk, hipStream_t st) {
+  ring = ring_variant(ring);
+  chunk = trace_chunk(chunk);
+  const dim3 g(trace_grid(w.nseg, w.seg, false, chunk));
+  ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace_photons<ST>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
 }
 void launch_between(const Wave &w, int nxt, hipStream_t st) {
   hipLaunchKernelGGL(k_between, dim3(1), dim3(64), 0, st, w, nxt);

@@ -593,9 +593,10 @@ __global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px,
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     cnt[(depth + 2) % 3] = 0;
     stat_add(stats, S_CLOSEST, (unsigned long long)count);
+    stat_add(stats, S_PHOTON_RAYS, (unsigned long long)count);
   }
   const int wv = threadIdx.x >> 6;
-  unsigned long long ndep = 0;
+  unsigned long long ndep = 0, nlook = 0, nscan = 0;
   for (int base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
     int i = base + threadIdx.x;
     bool keep = false, hit = false;
@@ -617,6 +618,8 @@ __global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px,
           if (g.photon_cell) atomicAdd(&g.photon_cell[h], 1);  // debug output only
           e0 = g.cell_start[h];
           n = g.cell_end[h] - e0;
+          ++nlook;
+          nscan += (unsigned long long)n;
           float *f = phot + 9 * threadIdx.x;
           f[0] = is.p.x; f[1] = is.p.y; f[2] = is.p.z;
           f[3] = -d.x; f[4] = -d.y; f[5] = -d.z;
@@ -652,8 +655,16 @@ __global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px,
     int a = block_append(&cnt[(depth + 1) % 3], keep, lds);
     if (keep) qnext[a] = p;
   }
-  for (int off = 32; off > 0; off >>= 1) ndep += __shfl_down(ndep, off);
-  if (lane_id() == 0) stat_add(stats, S_DEPOSITS, ndep);
+  for (int off = 32; off > 0; off >>= 1) {
+    ndep += __shfl_down(ndep, off);
+    nlook += __shfl_down(nlook, off);
+    nscan += __shfl_down(nscan, off);
+  }
+  if (lane_id() == 0) {
+    stat_add(stats, S_DEPOSITS, ndep);
+    stat_add(stats, S_LOOKUPS, nlook);
+    stat_add(stats, S_SCANNED, nscan);
+  }
 }
 
 // ---------------------------------------------------------------- update + film
@@ -794,6 +805,37 @@ static int ensure_pairs(SppmBuffers &b, size_t n) {
   return ORION_OK;
 }
 
+// HIP events around the photon pass's traversal (kind 0) and shading (kind
+// 1) launches on the stream they run on, when the caller asks for timing.
+struct PhotonTimer {
+  bool on = false;
+  hipStream_t st = nullptr;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> kind;
+  int mark(int k) {
+    if (!on) return ORION_OK;
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    ev.push_back(e);
+    kind.push_back(k);
+    HIP_TRY(hipEventRecord(e, st));
+    return ORION_OK;
+  }
+  // after the stream has finished: per kind, summed milliseconds and launches
+  int collect(double ms[2], int64_t n[2]) {
+    for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+      float t = 0;
+      HIP_TRY(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+      ms[kind[i]] += t;
+      n[kind[i]]++;
+    }
+    return ORION_OK;
+  }
+  ~PhotonTimer() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
+
 }  // namespace orion_amd
 
 using namespace orion_amd;
@@ -857,6 +899,9 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   const float nIterF = (float)n_iters;
   const float NpF = (float)((uint64_t)n_iters * (uint64_t)photons);
   const bool want_dbg = dbg && dbg->debug_iter >= 0 && dbg->debug_iter < n_iters;
+  PhotonTimer ptimer;
+  ptimer.on = dbg && dbg->profile;
+  ptimer.st = st;
   // camera pass of iteration `ci` (sppm.cpp:84-140) on the camera stream: the
   // wave's radiance arrays are the pixels' Ld; it waits for iteration ci - 2
   // to have finished with visible-point set ci % 2 and Ld snapshot ci % 2
@@ -994,9 +1039,12 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
         Wave tw = pw;
         tw.qray = qbuf[depth & 1];
         tw.cnt_ray = w.ctrl + C_PQ + depth % 3;
-        launch_trace(dj, tw, false, ring, chunk, false, st);
+        if ((rc = ptimer.mark(0))) return rc;
+        launch_trace_photons(dj, tw, ring, chunk, st);
+        if ((rc = ptimer.mark(0)) || (rc = ptimer.mark(1))) return rc;
         hipLaunchKernelGGL(k_photon_shade, dim3(blocks), dim3(256), 0, st, s->d, tw, px, gi, index0, depth,
                            qbuf[depth & 1], qbuf[(depth + 1) & 1], dstats);
+        if ((rc = ptimer.mark(1))) return rc;
       }
     }
     if (want_dbg && iter == dbg->debug_iter && dbg->photon_cell_count)
@@ -1042,6 +1090,18 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     stats->tri_tests_closest = (int64_t)h[S_TRIS_C];
     stats->node_visits_shadow = (int64_t)h[S_NODES_S];
     stats->tri_tests_shadow = (int64_t)h[S_TRIS_S];
+    stats->photon_rays = (int64_t)h[S_PHOTON_RAYS];
+    stats->node_visits_photon = (int64_t)h[S_NODES_P];
+    stats->tri_tests_photon = (int64_t)h[S_TRIS_P];
+    stats->photon_lookups = (int64_t)h[S_LOOKUPS];
+    stats->photon_vp_scanned = (int64_t)h[S_SCANNED];
+    double pms[2] = {0, 0};
+    int64_t pn[2] = {0, 0};
+    if ((rc = ptimer.collect(pms, pn))) return rc;
+    stats->photon_trace_ms = pms[0];
+    stats->photon_trace_launches = pn[0];
+    stats->photon_shade_ms = pms[1];
+    stats->photon_shade_launches = pn[1];
   }
   return ORION_OK;
 }

@@ -170,6 +170,20 @@ int orion_scene_get_stats(orion_scene s, orion_scene_stats *st);
 int orion_intersect_closest(orion_scene s, int n, const float *rays, float *t, int *prim,
                             float *u, float *v, int *tie, int on_device);
 int orion_intersect_any(orion_scene s, int n, const float *rays, int *occluded, int on_device);
+/* No counterpart in orion: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  orion_intersect_closest with on_device = 1 -- the same
+ * results bit for bit, all pointers device pointers, u, v and tie optional
+ * -- but it allocates nothing: its traversal runs on a fixed grid whose
+ * stack spill area and overflow counter live in one static device array per
+ * GPU (32 MiB, held until the process ends).  Limit: scenes whose traversal
+ * stack bound (orion_scene_stats.stack_bound, 3 x (BVH depth + 1)) is at
+ * most 48, i.e. BVH depth at most 15; larger fail with
+ * ORION_ERR_UNSUPPORTED.  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (several contexts' streams) race.  Returns after
+ * the hits are written (it synchronises the context's stream). */
+int orion_intersect_closest_noalloc(orion_scene s, int n, const float *rays, float *t, int *prim, float *u,
+                                    float *v, int *tie);
 
 /* Render statistics (device counters). */
 typedef struct {

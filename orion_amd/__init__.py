@@ -114,6 +114,8 @@ _SIGS = {
     "orion_intersect_closest": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "orion_intersect_any": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "orion_intersect_closest_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                  C.c_void_p, C.c_void_p, C.c_void_p]),
     "orion_render_path": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Tuning),
                                     C.POINTER(RenderStats)]),
     "orion_film_clear": (C.c_int, [C.c_void_p]),

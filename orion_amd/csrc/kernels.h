@@ -38,6 +38,13 @@ int trace_waves_per_simd(int ring);  // occupancy of the closest-hit traversal k
 void launch_between(const Wave &w, int nxt, hipStream_t st);
 void launch_shade(const Job *jobs, const Wave &w, int cur, bool multi, hipStream_t st);
 void launch_film(const Job *jobs, const Wave &w, int max_pix, bool multi, hipStream_t st);
+// closest hits on `blocks` one-wave blocks striding over the rays' chunks of
+// kDefaultChunk; spill holds spill_cap entries for each of blocks x
+// kTraceBlock threads (with the default LDS ring); *overflow counts dropped
+// stack entries
+void launch_intersect_fixed(const DScene &sc, int n, const float *rays, float *t, int *prim, float *u, float *v,
+                            int *tie, int *spill, int spill_cap, int blocks, unsigned int *overflow,
+                            hipStream_t st);
 void launch_intersect_batch(const DScene &sc, int n, const float *rays, float *t, int *prim, float *u, float *v,
                             int *tie, int *occluded, bool any, int *spill, int spill_cap,
                             unsigned long long *stats, hipStream_t st);

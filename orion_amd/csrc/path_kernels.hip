@@ -414,6 +414,32 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect_batch(DScene sc, int 
   if (ovf) stat_add(stats, S_OVERFLOW, ovf);
 }
 
+// Closest hits of n rays on a fixed grid: the wave64 of block b traces the
+// ray chunks b, b + G, b + 2G, ... (G = gridDim.x), so the spill area is G x
+// 64 lanes x spill_cap entries whatever n is.  Dropped stack entries are
+// added to *overflow.
+static_assert(kTraceBlock == 64, "one wave64 per traversal block");
+template <int RING>
+__global__ __launch_bounds__(kTraceBlock) void k_intersect_fixed(DScene sc, int n, const float *rays, float *t,
+                                                                 int *prim, float *u, float *v, int *tie,
+                                                                 int *spill, int spill_cap, int chunk,
+                                                                 unsigned int *overflow) {
+  __shared__ int ring[RING * kTraceBlock];
+  TravStack<RING> stk;
+  stk.ring = ring + threadIdx.x;
+  stk.spill_stride = gridDim.x * kTraceBlock;
+  stk.spill = spill + blockIdx.x * kTraceBlock + threadIdx.x;
+  stk.spill_cap = spill_cap;
+  uint32_t ovf = 0, nv = 0, nt = 0;
+  SoaRays io{sc.nodes, sc.tris, n, rays, t, u, v, prim, tie, nullptr};
+  const int64_t nchunks = ((int64_t)n + chunk - 1) / chunk;
+  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const int begin = (int)(c * chunk);
+    trace_engine<false, RING>(io, begin, (int)min((int64_t)begin + chunk, (int64_t)n), stk, &nv, &nt, &ovf);
+  }
+  if (ovf) atomicAdd(overflow, ovf);
+}
+
 // ---------------------------------------------------------------- launchers
 // LDS ring variants: RING x 128 lanes x 4 B of LDS per block.
 #define ORION_RING_DISPATCH(S, CALL)                     \
@@ -515,6 +541,13 @@ void launch_intersect_batch(const DScene &sc, int n, const float *rays, float *t
   else
     hipLaunchKernelGGL((k_intersect_batch<false, ST>), g, dim3(kTraceBlock), 0, st, sc, n, rays, t, prim, u, v, tie,
                        occluded, spill, spill_cap, chunk, stats);
+}
+void launch_intersect_fixed(const DScene &sc, int n, const float *rays, float *t, int *prim, float *u, float *v,
+                            int *tie, int *spill, int spill_cap, int blocks, unsigned int *overflow,
+                            hipStream_t st) {
+  constexpr int ST = kDefaultRing;
+  hipLaunchKernelGGL((k_intersect_fixed<ST>), dim3(blocks), dim3(kTraceBlock), 0, st, sc, n, rays, t, prim, u, v,
+                     tie, spill, spill_cap, kDefaultChunk, overflow);
 }
 
 }  // namespace orion_amd

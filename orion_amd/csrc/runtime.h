@@ -53,6 +53,9 @@ struct orion_ctx_s {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   orion_amd::WaveRunner *runner = nullptr;  // wave buffers of every render on this stream
+  // this device's copy of the allocation-free intersection's scratch array
+  // (its device address, from hipGetSymbolAddress at context creation)
+  int *scratch = nullptr;
 };
 
 struct orion_scene_s {

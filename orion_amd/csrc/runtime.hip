@@ -137,6 +137,24 @@ int read_striped_stats(const void *dev, unsigned long long *out) {
   return ORION_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free intersection
+// Scratch of orion_intersect_closest_noalloc: the traversal spill area of its
+// fixed grid (kNoAllocBlocks one-wave blocks x 64 lanes x kNoAllocSpill
+// stack entries beyond the LDS ring: scenes whose stack bound is at most
+// kNoAllocSpill + kDefaultRing) and, in its own 256-byte line, the count of
+// dropped stack entries.  32 MiB + 256 B per GPU, held until the process
+// ends.  Host code never uses the array's name: orion_ctx_create stores this
+// device's address of it (hipGetSymbolAddress) in orion_ctx::scratch.
+constexpr int kNoAllocBlocks = 4096;
+constexpr int kNoAllocSpill = 32;
+constexpr size_t kNoAllocSpillInts = (size_t)kNoAllocBlocks * kTraceBlock * kNoAllocSpill;
+static __device__ __align__(256) int scratch[kNoAllocSpillInts + 64];
+
+__global__ void k_zero_ints(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 int check_overflow(unsigned long long dropped) {
   if (!dropped) return ORION_OK;
   set_error("BVH traversal stack overflow: " + std::to_string(dropped) +
@@ -167,6 +185,13 @@ int orion_ctx_create(int device, orion_ctx *out) {
     delete c;
     set_error("orion_amd is built for gfx950 only; device reports " + std::string(prop.gcnArchName));
     return ORION_ERR_UNSUPPORTED;
+  }
+  // this device's copy of the allocation-free entry point's scratch array
+  hipError_t e = hipGetSymbolAddress((void **)&c->scratch, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !c->scratch) {
+    delete c;
+    set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    return ORION_ERR_HIP;
   }
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&c->ev0));
@@ -436,6 +461,31 @@ int orion_intersect_any(orion_scene s, int n, const float *rays, int *occluded, 
   HIP_TRY(hipMemcpyAsync(occluded, docc, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   return intersect_check(s);
+}
+
+int orion_intersect_closest_noalloc(orion_scene s, int n, const float *rays, float *t, int *prim, float *u,
+                                    float *v, int *tie) {
+  if (!s || n < 0 || !rays || !t || !prim) { set_error("bad argument"); return ORION_ERR_ARG; }
+  if (n == 0) return ORION_OK;
+  orion_ctx c = s->ctx;
+  const int cap = std::max(0, s->d.stack_bound - kDefaultRing);
+  if (cap > kNoAllocSpill) {
+    set_error("scene's traversal stack bound " + std::to_string(s->d.stack_bound) +
+              " exceeds the allocation-free entry point's maximum " + std::to_string(kNoAllocSpill + kDefaultRing));
+    return ORION_ERR_UNSUPPORTED;
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  int *dropped = c->scratch + kNoAllocSpillInts;
+  // the counter starts at zero (a kernel, not a memset: callers may capture)
+  hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, c->stream, dropped, 64);
+  const int blocks = (int)std::min<int64_t>(kNoAllocBlocks, ((int64_t)n + kDefaultChunk - 1) / kDefaultChunk);
+  launch_intersect_fixed(s->d, n, rays, t, prim, u, v, tie, c->scratch, cap, blocks, (unsigned int *)dropped,
+                         c->stream);
+  HIP_TRY(hipGetLastError());
+  unsigned int ndrop = 0;
+  HIP_TRY(hipMemcpyAsync(&ndrop, dropped, 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return check_overflow(ndrop);
 }
 
 int orion_film_clear(orion_scene s) {

@@ -39,7 +39,7 @@ namespace orion_amd {
 // Paths per wave: the traversal launches are latency-bound (a launch lasts
 // at least as long as its slowest rays), so the wave is made large enough
 // that each bounce's launch is dominated by throughput (64 samples per pixel
-// of a 1024x1024 film; ~240 B of wave state per path = 16 GB of MI355X's 288).
+// of a 1024x1024 film; 208 B of wave state per path = 14 GB of MI355X's 288).
 // The server caps its waves at half that: a request that arrives waits for
 // the running wave, so a smaller wave bounds its wait (measured: 2^26-path
 // served waves raised the median latency from 0.10 to 0.15 s).
@@ -226,13 +226,11 @@ struct WaveRunner {
       wi.cnt_ray = w.cnt + (3 * (it % 3) + 1) * w.cnt_stride;
       wi.cnt_shadow = w.cnt + (3 * (it % 3) + 2) * w.cnt_stride;
       if (it == 0) {
-        // camera rays only: no shadow rays exist before the first shading,
-        // and at most seg closest-hit entries per segment (an empty any-hit
-        // launch of the full grid had cost 1 ms beside this one at C2)
-        Wave w0 = wi;
-        w0.first_bounce = 1;
+        // camera rays only: no any-hit rays exist before the first shading
+        // (an empty any-hit launch of the full grid had cost 1 ms beside
+        // this one at C2)
         if (tm) tm->mark(split ? 0 : 3);
-        launch_trace(dj, w0, false, ring, chunk, multi, st);
+        launch_trace(dj, wi, false, ring, chunk, multi, st);
         if (tm) tm->mark(-1);
       } else if (split) {
         if (tm) tm->mark(0);

@@ -124,11 +124,17 @@ struct TravStack {
 
 __device__ __forceinline__ float q2f(uint32_t word, int k) { return (float)((word >> (8 * k)) & 0xffu); }
 
-// One ray in flight in the traversal engine.
+// One ray in flight in the traversal engine.  Any-hit rays: a triangle at
+// t < tmax occludes; one at exactly tmax only when its primitive id is below
+// `tie` (kTieAll: every one), so a visibility query on (0, t_L] towards
+// primitive L with tie = L answers "is L the closest hit" in the closest-hit
+// order, lexicographic in (t, primitive id).
+constexpr int kTieAll = 0x7fffffff;
 struct TRay {
   V3 o, d, inv;
   float tmin, tmax;
   int tag;  // caller's id (queue entry)
+  int tie;
 };
 
 // |d| components below 1e-35 are clamped so (lo - o) * inv is never 0 * inf
@@ -193,6 +199,7 @@ __device__ __forceinline__ bool tri_step(const TRay &ray, float4 a, float4 b, fl
   if (!tri_test(a, b, c, ray.o, ray.d, ray.tmin, ANY ? ray.tmax : *best, &t, &u, &v)) return false;
   int prim = __float_as_int(a.w);
   if (ANY) {
+    if (!(t < ray.tmax || prim < ray.tie)) return false;
     h->t = t; h->u = u; h->v = v; h->prim = prim;
     *found = true;
     return true;

@@ -65,13 +65,13 @@ __global__ void k_raygen(const Job *jobs, Wave w) {
   w.flags[p] = F_ALIVE;
   const int sg = p / w.seg, j0 = p - sg * w.seg;
   w.qpath[0][p] = p;
-  w.qray[2 * sg * w.seg + j0] = p;
+  w.qray[sg * w.seg + j0] = p;
 }
 
 // ---------------------------------------------------------------- traversal
-// Wave queues as traversal-engine IO.  Closest rays: entries r < N are
-// continuation rays (slot 0), N + p MIS rays (slot 1); shadow entries are
-// path ids whose ray lives in slot 2.
+// Wave queues as traversal-engine IO.  Closest rays: path ids, ray in
+// slot 0.  Any-hit entries r < N: shadow ray of path r (slot 2); N + p: MIS
+// visibility ray of path p (slot 1, its tie primitive in hprim[N + p]).
 template <bool ANY>
 struct WaveRays {
   const Wave *w;
@@ -79,12 +79,13 @@ struct WaveRays {
   const float4 *nodes, *tris;  // the segment's job's scene (wave-uniform)
   __device__ void load(int q, TRay *ray) const {
     const int r = (ANY ? w->qshadow : w->qray)[base + q];
-    const int slot = ANY ? 2 * w->N + r : r;
+    const int slot = ANY && r < w->N ? 2 * w->N + r : r;
     ray->o = mk(w->ox[slot], w->oy[slot], w->oz[slot]);
     ray->d = mk(w->dx[slot], w->dy[slot], w->dz[slot]);
     ray->tmin = 0.0f;
     ray->tmax = w->tmax[slot];
     ray->tag = r;
+    ray->tie = ANY && r >= w->N ? w->hprim[r] : kTieAll;
   }
   __device__ void store(const TRay &ray, bool hit, const HitRec &h) const {
     const int r = ray.tag;
@@ -107,9 +108,8 @@ struct WaveRays {
 // (closest hit; its traversal work has its own counters).
 template <bool ANY, int RING, bool MULTI, bool PHOTON = false>
 __device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int chunk, int lb, int *ring) {
-  const int seg_entries = ANY ? w.seg : 2 * w.seg;  // the queue's stride per segment
-  const int cover = w.first_bounce ? w.seg : seg_entries;  // entries per segment this launch covers
-  const int wps = (cover + chunk - 1) / chunk;
+  const int seg_entries = ANY ? 2 * w.seg : w.seg;  // the queue's stride per segment
+  const int wps = (seg_entries + chunk - 1) / chunk;
   // the wave's index is uniform: read from lane 0 so that everything derived
   // from it (segment, job, scene pointers) stays in scalar registers
   const int gw = lb * (kTraceBlock / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -212,12 +212,13 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
     bool act = i < p_end;
     int p = act ? w.qpath[cur][seg0 + i] : 0;
     bool want_ray0 = false, want_ray1 = false, want_shadow = false, keep = false, done = false;
+    int ref_rays = 0;  // closest-hit casts orion makes for the MIS sample
     if (act) {
       int fl = w.flags[p];
       Sp L = sp3(w.Lr[p], w.Lg[p], w.Lb[p]);
       // 1) resolve the direct-lighting estimate of the previous bounce
       if (fl & F_PEND) {
-        resolve_direct(sc, w, p, fl, &L);
+        resolve_direct(w, p, fl, &L);
         fl &= ~(F_PEND | F_SHADOW | F_MIS);
       }
       // 2) the hit of this bounce
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
           uint32_t index = (J.k0 + (uint32_t)s) * (uint32_t)J.npix + (uint32_t)(q - s * J.npix);
           int dim = w.dim[p];
           if (bsdf_num(bs, ALL & ~SPEC) > 0) {
-            int nf = setup_direct(sc, w, p, bs, is, d, beta, index, &dim, false);
+            int nf = setup_direct(sc, w, p, bs, is, d, beta, index, &dim, false, &ref_rays);
             fl |= nf;
             want_shadow = (nf & F_SHADOW) != 0;
             want_ray1 = (nf & F_MIS) != 0;
@@ -287,8 +288,8 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
                    ms = __ballot(want_shadow);
     if (lane == 0) {
       lw[wv][0] = __popcll(mk);
-      lw[wv][1] = __popcll(m0) + __popcll(m1);
-      lw[wv][2] = __popcll(ms);
+      lw[wv][1] = __popcll(m0);
+      lw[wv][2] = __popcll(ms) + __popcll(m1);
     }
     __syncthreads();
     if (threadIdx.x < 3) {
@@ -305,12 +306,14 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
     }
     __syncthreads();
     if (keep) w.qpath[nxt][seg0 + lbase[0] + lw[wv][0] + __popcll(mk & below)] = p;
-    const int r0 = 2 * seg0 + lbase[1] + lw[wv][1];
-    if (want_ray0) w.qray[r0 + __popcll(m0 & below)] = p;
-    if (want_ray1) w.qray[r0 + __popcll(m0) + __popcll(m1 & below)] = N + p;
-    if (want_shadow) w.qshadow[seg0 + lbase[2] + lw[wv][2] + __popcll(ms & below)] = p;
+    if (want_ray0) w.qray[seg0 + lbase[1] + lw[wv][1] + __popcll(m0 & below)] = p;
+    const int a0 = 2 * seg0 + lbase[2] + lw[wv][2];
+    if (want_shadow) w.qshadow[a0 + __popcll(ms & below)] = p;
+    if (want_ray1) w.qshadow[a0 + __popcll(ms) + __popcll(m1 & below)] = N + p;
     n_done += done ? 1 : 0;
-    n_mis += want_ray1 ? 1 : 0;
+    // orion casts the MIS sample's ray and counts it twice; the visibility
+    // rays traced here are counted as any-hit rays
+    n_mis += 2 * ref_rays - (want_ray1 ? 1 : 0);
     __syncthreads();  // lw / lbase are rewritten by the next step
   }
   // block totals: queue lengths of the segment and striped statistics
@@ -380,6 +383,7 @@ struct SoaRays {
     ray->tmin = rays[6 * n + i];
     ray->tmax = rays[7 * n + i];
     ray->tag = i;
+    ray->tie = kTieAll;
   }
   __device__ void store(const TRay &ray, bool hit, const HitRec &h) const {
     const int i = ray.tag;
@@ -471,7 +475,7 @@ int trace_chunk(int chunk) { return chunk > 0 ? std::min(std::max(chunk, 64), 40
 
 int trace_grid(int nseg, int seg, bool any, int chunk) {
   chunk = trace_chunk(chunk);
-  const int64_t seg_entries = any ? seg : 2 * (int64_t)seg;
+  const int64_t seg_entries = any ? 2 * (int64_t)seg : seg;
   const int64_t waves = nseg * ((seg_entries + chunk - 1) / chunk);
   return (int)std::max<int64_t>(1, (waves + kTraceBlock / 64 - 1) / (kTraceBlock / 64));
 }
@@ -498,9 +502,7 @@ int trace_pair_grid(int nseg, int seg, int chunk) {
 void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk, bool multi, hipStream_t st) {
   ring = ring_variant(ring);
   chunk = trace_chunk(chunk);
-  // a first bounce's closest-hit launch has the any-hit grid's shape (seg
-  // entries per segment)
-  const dim3 g(trace_grid(w.nseg, w.seg, any || w.first_bounce, chunk));
+  const dim3 g(trace_grid(w.nseg, w.seg, any, chunk));
   if (multi) {
     if (any) ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<true, ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
     else ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<false, ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
@@ -534,7 +536,7 @@ void launch_intersect_batch(const DScene &sc, int n, const float *rays, float *t
                             unsigned long long *stats, hipStream_t st) {
   constexpr int ST = kDefaultRing;
   const int chunk = kDefaultChunk;
-  const dim3 g(trace_grid(1, n, true, chunk));
+  const dim3 g(trace_grid(1, n, false, chunk));
   if (any)
     hipLaunchKernelGGL((k_intersect_batch<true, ST>), g, dim3(kTraceBlock), 0, st, sc, n, rays, t, prim, u, v, tie,
                        occluded, spill, spill_cap, chunk, stats);

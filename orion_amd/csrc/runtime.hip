@@ -80,9 +80,9 @@ int DevBuf::ensure(size_t b) {
 // ---------------------------------------------------------------- wave buffers
 int WaveBuffers::ensure(int N) {
   if (N <= cap && mem.ptr) return ORION_OK;
-  // floats/ints per path: rays 7 x 3 slots, hits 4 x 2, vis 1, state 9,
-  // pending 15, queues 2 + 2 + 1
-  const size_t per = 21 + 8 + 1 + 9 + 15 + 5;
+  // floats/ints per path: rays 7 x 3 slots, hits 3 + 2 (hprim: slot 0 hits
+  // and MIS ties), vis 2, state 9, pending 10, queues 2 + 1 + 2
+  const size_t per = 21 + 5 + 2 + 9 + 10 + 5;
   const size_t nseg = ((size_t)N + kSegMin - 1) / kSegMin + 1;
   int rc = mem.ensure(per * (size_t)N * 4 + 9 * nseg * 4 + 256);
   if (rc) return rc;
@@ -94,29 +94,26 @@ int WaveBuffers::ensure(int N) {
   w.ox = (float *)take(3 * (size_t)N); w.oy = (float *)take(3 * (size_t)N); w.oz = (float *)take(3 * (size_t)N);
   w.dx = (float *)take(3 * (size_t)N); w.dy = (float *)take(3 * (size_t)N); w.dz = (float *)take(3 * (size_t)N);
   w.tmax = (float *)take(3 * (size_t)N);
-  w.ht = (float *)take(2 * (size_t)N); w.hu = (float *)take(2 * (size_t)N); w.hv = (float *)take(2 * (size_t)N);
+  w.ht = (float *)take(N); w.hu = (float *)take(N); w.hv = (float *)take(N);
   w.hprim = (int *)take(2 * (size_t)N);
-  w.vis = (int *)take(N);
+  w.vis = (int *)take(2 * (size_t)N);
   w.br = (float *)take(N); w.bg = (float *)take(N); w.bb = (float *)take(N);
   w.Lr = (float *)take(N); w.Lg = (float *)take(N); w.Lb = (float *)take(N);
   w.dim = (int *)take(N); w.bounce = (int *)take(N); w.flags = (int *)take(N);
   w.Ar = (float *)take(N); w.Ag = (float *)take(N); w.Ab = (float *)take(N);
-  w.fr = (float *)take(N); w.fg = (float *)take(N); w.fb = (float *)take(N);
-  w.px = (float *)take(N); w.py = (float *)take(N); w.pz = (float *)take(N);
-  w.bpdf = (float *)take(N); w.sel = (float *)take(N);
+  w.Mr = (float *)take(N); w.Mg = (float *)take(N); w.Mb = (float *)take(N);
+  w.sel = (float *)take(N);
   w.nbr = (float *)take(N); w.nbg = (float *)take(N); w.nbb = (float *)take(N);
-  w.nlight = (int *)take(N);
   w.N = N;  // slot stride
   w.qpath[0] = (int *)take(N); w.qpath[1] = (int *)take(N);
-  w.qray = (int *)take(2 * (size_t)N);
-  w.qshadow = (int *)take(N);
+  w.qray = (int *)take(N);
+  w.qshadow = (int *)take(2 * (size_t)N);
   choose_segments(N, &w.seg, &w.nseg);
   w.cnt = (int *)take(9 * nseg);
   w.cnt_stride = (int)nseg;
   w.cnt_ray = w.cnt + nseg;
   w.cnt_shadow = w.cnt + 2 * nseg;
   w.shade_parts = 1;
-  w.first_bounce = 0;
   w.seg_job = nullptr;
   w.seg_perm = nullptr;
   return ORION_OK;
@@ -391,7 +388,7 @@ static int intersect_launch(orion_scene s, int n, const float *rays, float *t, i
                             int *tie, int *occ, bool any) {
   orion_ctx c = s->ctx;
   int rc = s->spill.ensure(
-      trace_spill_ints(s->d.stack_bound, kDefaultRing, trace_grid(1, n, true, kDefaultChunk)) * 4 + 4);
+      trace_spill_ints(s->d.stack_bound, kDefaultRing, trace_grid(1, n, false, kDefaultChunk)) * 4 + 4);
   if (rc) return rc;
   // the scene's statistics area counts dropped stack entries of this call
   HIP_TRY(hipMemsetAsync(s->stats.ptr, 0, kStatBytes, c->stream));

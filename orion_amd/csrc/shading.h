@@ -290,43 +290,12 @@ __device__ inline Sp light_sample_le(const DScene &sc, int l, float a0, float a1
 }
 
 // Second half of estimateDirect: adds the pending light-sampled term (when
-// its shadow ray was unoccluded) and the BSDF-sampled term (when its MIS ray
-// hit the sampled light's triangle, or for an envmap escaped the scene) to L,
-// as L += beta * (Ld / lightPdf).
-__device__ inline void resolve_direct(const DScene &sc, const Wave &w, int p, int fl, Sp *L) {
-  const int N = w.N;
+// its shadow ray was unoccluded) and the BSDF-sampled term (when its
+// visibility ray was) to L, as L += beta * (Ld / lightPdf).
+__device__ inline void resolve_direct(const Wave &w, int p, int fl, Sp *L) {
   Sp Ld = sp(0);
   if ((fl & F_SHADOW) && w.vis[p]) acc(Ld, sp3(w.Ar[p], w.Ag[p], w.Ab[p]));
-  if (fl & F_MIS) {
-    int hp = w.hprim[N + p];
-    int l = w.nlight[p];
-    const float *LR = sc.lights + 32 * (size_t)l;
-    if ((int)gld(LR + 0) == LT_ENVMAP) {
-      // integrator.cpp:66-88: pdf_Li is analytic; a miss sees Envmap::Le, a
-      // hit on any triangle is not this light
-      if (hp < 0) {
-        V3 wi = mk(w.dx[N + p], w.dy[N + p], w.dz[N + p]);
-        float bp = w.bpdf[p];
-        float weight = power_heuristic(bp, env_pdf_li(sc, LR, wi));
-        Sp Lm = env_le(sc, LR, wi);
-        if (!black(Lm)) acc(Ld, sp3(w.fr[p], w.fg[p], w.fb[p]) * Lm * weight / bp);
-      }
-    } else if (hp >= 0 && hp == (int)gld(LR + 1)) {
-      // Triangle::pdf (triangle.cpp:123-135) on the MIS ray's hit
-      Isect in;
-      make_isect(sc, hp, w.hu[N + p], w.hv[N + p], &in);
-      V3 wi = mk(w.dx[N + p], w.dy[N + p], w.dz[N + p]);
-      V3 pp = mk(w.px[p], w.py[p], w.pz[p]);
-      float lpdf = lensq(sub(pp, in.p)) / (absdot(in.ng, neg(wi)) * gld(sc.light_area + l));
-      if (is_inf(lpdf)) lpdf = 0;
-      if (lpdf != 0) {
-        float bp = w.bpdf[p];
-        float weight = power_heuristic(bp, lpdf);
-        Sp Lm = dot(in.ng, neg(wi)) > 0 ? sp3(gld(LR + 2), gld(LR + 3), gld(LR + 4)) : sp(0);
-        if (!black(Lm)) acc(Ld, sp3(w.fr[p], w.fg[p], w.fb[p]) * Lm * weight / bp);
-      }
-    }
-  }
+  if ((fl & F_MIS) && w.vis[w.N + p]) acc(Ld, sp3(w.Mr[p], w.Mg[p], w.Mb[p]));
   acc(*L, sp3(w.nbr[p], w.nbg[p], w.nbb[p]) * (Ld / w.sel[p]));
 }
 
@@ -345,12 +314,34 @@ __device__ inline int nee_pick(const DScene &sc, uint32_t index, int *dim, bool 
   return min((int)(sobol(sc.sobol, index, (*dim)++) * nl), nl - 1);
 }
 
+// The traversal's triangle test of primitive `prim` (tri_test on the leaf
+// record p0, p1 - p0, p2 - p0, the differences formed in float as the BVH
+// builder forms them) for the ray (o, d) on (0, FLT_MAX]: the t, u, v a
+// closest-hit traversal reports when `prim` is the hit.
+__device__ inline bool prim_hit(const DScene &sc, int prim, V3 o, V3 d, float *t, float *u, float *v) {
+  const float4 *s = sc.shade + 5 * (size_t)prim;
+  const float4 a = gld(s), b = gld(s + 1), c = gld(s + 2);
+  const float4 p0 = make_float4(a.x, a.y, a.z, 0.f);
+  const float4 e1 = make_float4(a.w - a.x, b.x - a.y, b.y - a.z, 0.f);
+  const float4 e2 = make_float4(b.z - a.x, b.w - a.y, c.x - a.z, 0.f);
+  return tri_test(p0, e1, e2, o, d, 0.0f, kFltMax, t, u, v);
+}
+
 // estimateDirect's sampling half (core/integrator.cpp:29-90) for light l,
 // chosen with probability sel: light and BSDF samples (four draws), the
-// light-sampled term and the shadow and MIS rays.  Returns the F_PEND |
-// F_SHADOW | F_MIS bits.
+// light-sampled term A with its shadow ray (slot 2), and the BSDF-sampled
+// term M with its visibility ray (slot 1).  orion counts M when the
+// BSDF-sampled ray's closest hit is the light (integrator.cpp:66-88).  For
+// an area light that is the light's triangle: it is tested here with the
+// traversal's own test and, when hit at t_L, M is evaluated at that hit and
+// the ray becomes a visibility query on (0, t_L] with tie = the triangle
+// (a triangle at exactly t_L occludes only with a lower primitive id, the
+// closest-hit order); a direction that misses the triangle has no M and no
+// ray.  For an envmap M counts when the ray escapes: a visibility query on
+// (0, FLT_MAX].  Returns the F_PEND | F_SHADOW | F_MIS bits; *ref_rays gets
+// the closest-hit casts orion makes for the BSDF sample (0 or 1).
 __device__ inline int nee_sample(const DScene &sc, const Wave &w, int p, int l, float sel, const Bsdf &bs,
-                                 const Isect &is, V3 d, Sp beta, uint32_t index, int *dim) {
+                                 const Isect &is, V3 d, Sp beta, uint32_t index, int *dim, int *ref_rays) {
   const int N = w.N;
   float lu1 = sobol(sc.sobol, index, (*dim)++), lu0 = sobol(sc.sobol, index, (*dim)++);
   float bu1 = sobol(sc.sobol, index, (*dim)++), bu0 = sobol(sc.sobol, index, (*dim)++);
@@ -359,7 +350,8 @@ __device__ inline int nee_sample(const DScene &sc, const Wave &w, int p, int l, 
   V3 wi = mk(0, 0, 0), so, sdir;
   float lpdf = 0;
   Sp Li = light_sample_li(sc, l, is.p, is.ng, lu0, lu1, &wi, &lpdf, &so, &sdir);
-  const int ltype = (int)gld(sc.lights + 32 * (size_t)l);
+  const float *LR = sc.lights + 32 * (size_t)l;
+  const int ltype = (int)gld(LR);
   bool delta = ltype != LT_AREA && ltype != LT_ENVMAP;  // isDeltaLight (light.h:39-42)
   bool hs = false, hm = false;
   if (lpdf > 0 && !black(Li)) {
@@ -385,19 +377,44 @@ __device__ inline int nee_sample(const DScene &sc, const Wave &w, int p, int l, 
     mulby(f2, sp(absdot(wi2, is.ns)));
     // an envmap's pdf_Li needs no ray: a zero ends the estimate before it
     bool traced = !black(f2) && bp2 > 0;
-    if (traced && ltype == LT_ENVMAP && !(st & SPEC) && env_pdf_li(sc, sc.lights + 32 * (size_t)l, wi2) == 0)
-      traced = false;
+    if (traced && ltype == LT_ENVMAP && !(st & SPEC) && env_pdf_li(sc, LR, wi2) == 0) traced = false;
     if (traced) {
+      ++*ref_rays;
       V3 o2 = offset_ray_origin(is.p, is.ng, wi2);
-      int s1 = N + p;
-      w.ox[s1] = o2.x; w.oy[s1] = o2.y; w.oz[s1] = o2.z;
-      w.dx[s1] = wi2.x; w.dy[s1] = wi2.y; w.dz[s1] = wi2.z;
-      w.tmax[s1] = kFltMax;
-      w.fr[p] = f2.r; w.fg[p] = f2.g; w.fb[p] = f2.b;
-      w.bpdf[p] = bp2;
-      w.px[p] = is.p.x; w.py[p] = is.p.y; w.pz[p] = is.p.z;
-      w.nlight[p] = l;
-      hm = true;
+      Sp M = sp(0);
+      float tvis = kFltMax;
+      int tie = kTieAll;
+      if (ltype == LT_ENVMAP) {
+        float weight = power_heuristic(bp2, env_pdf_li(sc, LR, wi2));
+        Sp Lm = env_le(sc, LR, wi2);
+        if (!black(Lm)) M = f2 * Lm * weight / bp2;
+      } else {
+        // Triangle::pdf (triangle.cpp:123-135) at the light's hit
+        const int pl = (int)gld(LR + 1);
+        float tl, ul, vl;
+        if (prim_hit(sc, pl, o2, wi2, &tl, &ul, &vl)) {
+          Isect in;
+          make_isect(sc, pl, ul, vl, &in);
+          float hpdf = lensq(sub(is.p, in.p)) / (absdot(in.ng, neg(wi2)) * gld(sc.light_area + l));
+          if (is_inf(hpdf)) hpdf = 0;
+          if (hpdf != 0) {
+            float weight = power_heuristic(bp2, hpdf);
+            Sp Lm = dot(in.ng, neg(wi2)) > 0 ? sp3(gld(LR + 2), gld(LR + 3), gld(LR + 4)) : sp(0);
+            if (!black(Lm)) M = f2 * Lm * weight / bp2;
+          }
+          tvis = tl;
+          tie = pl;
+        }
+      }
+      if (!black(M)) {
+        int s1 = N + p;
+        w.ox[s1] = o2.x; w.oy[s1] = o2.y; w.oz[s1] = o2.z;
+        w.dx[s1] = wi2.x; w.dy[s1] = wi2.y; w.dz[s1] = wi2.z;
+        w.tmax[s1] = tvis;
+        w.hprim[s1] = tie;
+        w.Mr[p] = M.r; w.Mg[p] = M.g; w.Mb[p] = M.b;
+        hm = true;
+      }
     }
   }
   if (!(hs || hm)) return 0;
@@ -409,11 +426,11 @@ __device__ inline int nee_sample(const DScene &sc, const Wave &w, int p, int l, 
 // Both halves in one place (the SPPM camera pass).  Draws exactly the
 // reference's dimensions.
 __device__ inline int setup_direct(const DScene &sc, const Wave &w, int p, const Bsdf &bs, const Isect &is, V3 d,
-                                   Sp beta, uint32_t index, int *dim, bool uniform) {
+                                   Sp beta, uint32_t index, int *dim, bool uniform, int *ref_rays) {
   float sel;
   const int l = nee_pick(sc, index, dim, uniform, &sel);
   if (l < 0) return 0;
-  return nee_sample(sc, w, p, l, sel, bs, is, d, beta, index, dim);
+  return nee_sample(sc, w, p, l, sel, bs, is, d, beta, index, dim, ref_rays);
 }
 
 __device__ __forceinline__ Sp area_emission(const DScene &sc, const Isect &is, V3 w) {  // diffuse.cpp:53-56

@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, in
       int fl = w.flags[p];
       Sp L = sp3(w.Lr[p], w.Lg[p], w.Lb[p]);
       if (fl & F_PEND) {
-        resolve_direct(sc, w, p, fl, &L);
+        resolve_direct(w, p, fl, &L);
         fl &= ~(F_PEND | F_SHADOW | F_MIS);
       }
       if (fl & F_ALIVE) {
@@ -177,7 +177,8 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, in
           if ((depth == 0 || (fl & F_SPEC)) && is.light >= 0) acc(L, beta * area_emission(sc, is, wo));
           uint32_t index = (uint32_t)iter * (uint32_t)w.npix + (uint32_t)p;
           int dim = w.dim[p];
-          int nf = setup_direct(sc, w, p, bs, is, d, beta, index, &dim, true);
+          int ref_rays = 0;
+          int nf = setup_direct(sc, w, p, bs, is, d, beta, index, &dim, true, &ref_rays);
           fl |= nf;
           want_shadow = (nf & F_SHADOW) != 0;
           want_ray1 = (nf & F_MIS) != 0;
@@ -229,10 +230,10 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, in
     if (keep) w.qpath[nxt][a] = p;
     a = wave_append(&w.ctrl[C_RAY], want_ray0);
     if (want_ray0) w.qray[a] = p;
-    a = wave_append(&w.ctrl[C_RAY], want_ray1);
-    if (want_ray1) w.qray[a] = N + p;
     a = wave_append(&w.ctrl[C_SHADOW], want_shadow);
     if (want_shadow) w.qshadow[a] = p;
+    a = wave_append(&w.ctrl[C_SHADOW], want_ray1);
+    if (want_ray1) w.qshadow[a] = N + p;
   }
 }
 

@@ -14,7 +14,7 @@ enum : int {
   F_SPEC = 2,      // specularBounce
   F_PEND = 4,      // direct-lighting estimate waiting for its rays
   F_SHADOW = 8,    // ... with a shadow ray in slot 2
-  F_MIS = 16,      // ... with a BSDF-sampled (MIS) ray in slot 1
+  F_MIS = 16,      // ... with the visibility ray of its BSDF-sampled (MIS) term in slot 1
 };
 
 // control block (ints, device memory)
@@ -42,8 +42,8 @@ constexpr int kStatStripes = 64;
 constexpr size_t kStatBytes = (size_t)S_COUNT * kStatStripes * 8;
 
 // Queue segments: path ids [s * seg, (s + 1) * seg) form segment s.  Each
-// segment has its own path, closest-ray (2 seg entries) and shadow-ray
-// queues with their own lengths, filled by the blocks that shade the
+// segment has its own path, closest-ray (seg entries) and any-hit (2 seg
+// entries: shadow and MIS visibility rays) queues with their own lengths, filled by the blocks that shade the
 // segment (one device atomic per queue per 256 paths, on the segment's own
 // counters: appends never contend on one shared counter).  seg is kSegMin or kSegMax, chosen per wave: waves of 2^21
 // paths or more take long segments (a traversal wave's rays then come from
@@ -81,40 +81,36 @@ struct Wave {
   int npix;       // (SPPM) pixels of the image
   uint32_t k0;
   int W, H;
-  // rays: slot s of path p at s * N + p; slots 0 continuation, 1 MIS, 2 shadow
+  // rays: slot s of path p at s * N + p; slots 0 continuation, 1 MIS
+  // visibility, 2 shadow
   float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
-  // hits for slots 0 and 1
+  // closest hits of slot 0 (N entries); hprim[N + p]: the tie primitive of
+  // path p's MIS visibility ray
   float *ht, *hu, *hv;
   int *hprim;
-  int *vis;       // shadow visibility per path
+  int *vis;       // any-hit results: vis[p] shadow ray, vis[N + p] MIS ray (1 = unoccluded)
   // path state
   float *br, *bg, *bb;   // beta
   float *Lr, *Lg, *Lb;   // radiance
   int *dim, *bounce, *flags;
   // pending direct lighting
-  float *Ar, *Ag, *Ab;   // light-sampled term (counts when unoccluded)
-  float *fr, *fg, *fb;   // BSDF-sampled f * |cos| for the MIS ray
-  float *px, *py, *pz;   // shading point
-  float *bpdf, *sel;     // bsdf pdf of the MIS direction, light selection pdf
+  float *Ar, *Ag, *Ab;   // light-sampled term (counts when its shadow ray is unoccluded)
+  float *Mr, *Mg, *Mb;   // BSDF-sampled (MIS) term (counts when its visibility ray is)
+  float *sel;            // light selection pdf
   float *nbr, *nbg, *nbb;  // beta at the shading point
-  int *nlight;
   // queues
   int *qpath[2];
-  int *qray;      // entries s * N + p, s in {0, 1}
-  int *qshadow;   // path ids
+  int *qray;      // path ids (slot 0 rays)
+  int *qshadow;   // p: shadow ray (slot 2); N + p: MIS visibility ray (slot 1)
   int *ctrl;
   unsigned long long *stats;
   // traversal stack spill area: spill_cap entries per traversal thread
   int *spill;
   int spill_cap;
-  // queue segments: `seg` paths per segment (closest queues hold 2 seg
+  // queue segments: `seg` paths per segment (any-hit queues hold 2 seg
   // entries per segment), nseg segments in use, lengths per segment
   int seg, nseg;
   int *cnt_ray, *cnt_shadow;  // the lengths a traversal launch reads
-  // 1 for a wave's first bounce: its closest-hit queues hold at most seg
-  // camera rays per segment (later bounces up to 2 seg: continuation and
-  // MIS rays), so its launch covers seg entries per segment
-  int first_bounce;
   // Path pipeline queue lengths, three rotating sets: bounce b traces the
   // ray lengths of set b % 3, shades the path lengths of set b % 3, appends
   // to set (b + 1) % 3 and zeroes set (b + 2) % 3 (nothing reads it during

@@ -7,12 +7,15 @@ fallback: every render / intersection call runs the HIP kernels and raises
 if the library or a GPU is unavailable.
 """
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
 _PKG = Path(__file__).resolve().parent
-LIB_PATH = _PKG / "liborion_amd.so"
+# ORION_AMD_LIB names another build of the library (the sanitizer build in
+# orion_amd/_asan/, for instance); the default is the in-tree one
+LIB_PATH = Path(os.environ["ORION_AMD_LIB"]).resolve() if os.environ.get("ORION_AMD_LIB") else _PKG / "liborion_amd.so"
 MAT_FLOATS = 24  # ORION_MAT_FLOATS: floats per material record
 
 f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")

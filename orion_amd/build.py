@@ -1,6 +1,7 @@
 """Builds liborion_amd.so (HIP for gfx950 + host C++) in-tree, and the oracle.
 
     python -m orion_amd.build            # library + oracle (+ oracle/_ref if the reference is present)
+    python -m orion_amd.build --sanitize # host C++ under ASan + UBSan, into orion_amd/_asan/
 
 Compiler rules: every HIP and host translation unit is compiled with
 -ffp-contract=off (no FMA contraction: the float operation sequence must equal
@@ -69,6 +70,33 @@ def build_library(verbose=False):
     return LIB
 
 
+def build_sanitized():
+    """A copy of the library whose host C++ (scene loader, PNG reader, BVH
+    builder, scene generators, film writers, record checks) is built with
+    AddressSanitizer and UndefinedBehaviorSanitizer, into orion_amd/_asan/.
+    The HIP translation units are built as usual (no GPU code is
+    instrumented).  Load it with ORION_AMD_LIB=orion_amd/_asan/liborion_amd.so
+    and gcc's sanitizer runtimes preloaded:
+
+        LD_PRELOAD="$(gcc -print-file-name=libasan.so) $(gcc -print-file-name=libubsan.so)" \
+        ASAN_OPTIONS=detect_leaks=0 ORION_AMD_LIB=orion_amd/_asan/liborion_amd.so \
+        python -m pytest tests -m "not gpu"
+    """
+    hipcc = _hipcc()
+    out = PKG / "_asan"
+    objd = out / "obj"
+    objd.mkdir(parents=True, exist_ok=True)
+    san = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"]
+    jobs = [[hipcc, *HIP_FLAGS, "-c", str(CSRC / s), "-o", str(objd / (s + ".o"))] for s in HIP_SOURCES]
+    jobs += [["g++", *COMMON, *san, "-c", str(CSRC / s), "-o", str(objd / (s + ".o"))] for s in CPP_SOURCES]
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
+        list(ex.map(_run, jobs))
+    objs = [str(objd / (s + ".o")) for s in HIP_SOURCES + CPP_SOURCES]
+    lib = out / "liborion_amd.so"
+    _run([hipcc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", str(lib), "-lpthread", "-lz"])
+    return lib
+
+
 def build_oracle():
     """Compiles the oracle (test infrastructure) and, where the reference tree
     exists, oracle/_ref from the reference's own sources."""
@@ -78,6 +106,9 @@ def build_oracle():
 
 
 def main():
+    if "--sanitize" in sys.argv:
+        print(f"built {build_sanitized()}")
+        return
     build_library(verbose="-v" in sys.argv)
     build_oracle()
     print(f"built {LIB}")

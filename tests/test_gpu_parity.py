@@ -208,3 +208,69 @@ def test_c2_full_size_waves_equal(device):
     assert (ca == 4).all() and (cb == 4).all()
     assert np.isfinite(a).all() and (a >= 0).all()
     np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32))
+
+
+def _tie_scene(with_lower_duplicate=True, res=48):
+    """Lambert plane under a 2 x 2 two-triangle quad light 0.3 above it
+    (lights 0 and 1, one sided, facing down), plus a black copy of each light triangle at exactly
+    the same vertices: the copy of triangle A comes before it in primitive
+    order (lower id), the copy of B after it.  A BSDF-sampled ray towards A
+    therefore hits the copy first in the closest-hit order (t, primitive id)
+    and carries no MIS term; one towards B hits B itself and does."""
+    from test_oracle_integrators import area, matte, tri_record
+    up = np.array([0.0, 1.0, 0.0])
+    P = [np.array(p, float) for p in ([-3, 0, -3], [-3, 0, 3], [3, 0, 3], [3, 0, -3])]
+    c = np.array([0.0, 0.3, 0.0])
+    Q = [c + np.array(q) for q in ([-1.0, 0, -1.0], [1.0, 0, -1.0], [1.0, 0, 1.0], [-1.0, 0, 1.0])]
+    ln = np.array([0.0, -1.0, 0.0])
+    light_tris = []
+    for a, b, d in ((Q[0], Q[1], Q[2]), (Q[0], Q[2], Q[3])):
+        if np.dot(np.cross(b - a, d - a), ln) < 0:
+            b, d = d, b
+        light_tris.append(tri_record(a, b, d, ln))
+    plane = [tri_record(P[0], P[1], P[2], up), tri_record(P[0], P[2], P[3], up)]
+    if with_lower_duplicate:
+        # 0: copy of A, 1-2: plane, 3: A, 4: B, 5: copy of B
+        pn = [light_tris[0]] + plane + light_tris + [light_tris[1]]
+        mat = [1, 0, 0, 1, 1, 1]
+        tl = [-1, -1, -1, 0, 1, -1]
+        lights = [area(3, (5.0, 4.0, 3.0)), area(4, (5.0, 4.0, 3.0))]
+    else:
+        # 0-1: plane, 2: A, 3: B, 4: copy of B
+        pn = plane + light_tris + [light_tris[1]]
+        mat = [0, 0, 1, 1, 1]
+        tl = [-1, -1, 0, 1, -1]
+        lights = [area(2, (5.0, 4.0, 3.0)), area(3, (5.0, 4.0, 3.0))]
+    mats = np.concatenate([matte(0.6, 0.5, 0.7), matte(0.0)])
+    cam = oa.camera_record([0.0, 0.2, 1.2, 0, 0, 0, 0, 1, 0], 60.0, res, res)
+    return oa.HostScene.from_records(np.array(pn, np.float32), np.array(mat, np.int32), np.array(tl, np.int32),
+                                     mats, np.array(lights, np.float32), cam)
+
+
+def test_path_mis_light_ties(device, oracle):
+    """MIS visibility queries under exact ties: a light triangle and a black
+    copy at the same vertices.  The GPU film equals the oracle's (whose MIS
+    term needs the closest hit to be the light's own triangle), and removing
+    the lower-id copy changes the image, so the tie decides the result."""
+    hs = _tie_scene(True)
+    ds = oa.DeviceScene(device, hs)
+    ds.film_clear()
+    ds.render_path(0, 32, 3)
+    rgb, cnt = ds.film()
+    ref, rcnt, _ = oracle.scene(hs).render_path(32, 3)
+    frac, mean_err, bits = image_compare(rgb / cnt[..., None], ref / rcnt[..., None])
+    print(f"mis ties: within 1e-3 {frac:.6f}, mean rel err {mean_err:.2e}, bit-equal {bits:.6f}")
+    assert frac >= PIX_FRAC
+    assert mean_err <= MEAN_TOL
+    hs2 = _tie_scene(False)
+    ds2 = oa.DeviceScene(device, hs2)
+    ds2.film_clear()
+    ds2.render_path(0, 32, 3)
+    rgb2, cnt2 = ds2.film()
+    gap = abs((rgb2 / cnt2[..., None]).mean() - (rgb / cnt[..., None]).mean()) / (rgb2 / cnt2[..., None]).mean()
+    print(f"mis ties: image mean without the lower-id copy differs by {gap:.3%}")
+    assert gap > 0.01
+    ref2, rcnt2, _ = oracle.scene(hs2).render_path(32, 3)
+    frac2, mean_err2, _ = image_compare(rgb2 / cnt2[..., None], ref2 / rcnt2[..., None])
+    assert frac2 >= PIX_FRAC
+    assert mean_err2 <= MEAN_TOL

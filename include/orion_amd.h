@@ -188,8 +188,8 @@ int orion_intersect_closest_noalloc(orion_scene s, int n, const float *rays, flo
 /* Render statistics (device counters). */
 typedef struct {
   int64_t paths;          /* camera samples completed */
-  int64_t closest_rays;   /* closest-hit casts (camera, bounce, MIS) */
-  int64_t shadow_rays;    /* any-hit casts */
+  int64_t closest_rays;   /* closest-hit casts (camera, bounce, photon) */
+  int64_t shadow_rays;    /* any-hit casts (shadow rays, MIS visibility queries) */
   int64_t ref_equiv_rays; /* casts orion itself would make (MIS ray counted twice) */
   int64_t photons;
   int64_t deposits;       /* SPPM photon deposit lookups */
@@ -210,11 +210,14 @@ typedef struct {
   /* SPPM photon pass: photon rays traced (all bounces), their traversal work,
    * in-bounds grid lookups of photon hits after the first, visible-point
    * entries scanned by those lookups; launch times when
-   * orion_sppm_debug.profile != 0 (HIP events on the photon pass's stream) */
+   * orion_sppm_debug.profile != 0 (HIP events on the photon pass's stream:
+   * traversal, and shading = photon bounces + deposits) */
   int64_t photon_rays, node_visits_photon, tri_tests_photon;
   int64_t photon_lookups, photon_vp_scanned;
   double photon_trace_ms, photon_shade_ms;
   int64_t photon_trace_launches, photon_shade_launches;
+  int64_t photon_hits;       /* photon hits after the first bounce, kept for deposits */
+  int64_t photon_batch;      /* iterations whose photons are traced together */
 } orion_render_stats;
 
 typedef struct {

@@ -55,7 +55,8 @@ class RenderStats(C.Structure):
                 ("node_visits_photon", C.c_int64), ("tri_tests_photon", C.c_int64),
                 ("photon_lookups", C.c_int64), ("photon_vp_scanned", C.c_int64),
                 ("photon_trace_ms", C.c_double), ("photon_shade_ms", C.c_double),
-                ("photon_trace_launches", C.c_int64), ("photon_shade_launches", C.c_int64)]
+                ("photon_trace_launches", C.c_int64), ("photon_shade_launches", C.c_int64),
+                ("photon_hits", C.c_int64), ("photon_batch", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -72,6 +72,7 @@ struct Grid {
 // a snapshot of Ld taken when the camera pass ends (ld_snap[i % 2]).
 struct SppmBuffers {
   DevBuf pix_mem, grid_mem, pair_mem, hist_mem, scan_mem, job, spill, spill_cam;
+  DevBuf hits;  // photon hits of a batch of iterations, waiting for their deposits
   WaveBuffers camw;
   Pix px[2]{};
   float *ld_snap[2] = {nullptr, nullptr};  // 3 x npix each (r, g, b planes)
@@ -578,17 +579,35 @@ __device__ __forceinline__ unsigned long long photon_deposits(const DScene &sc, 
   return nd;
 }
 
-// One photon bounce (sppm.cpp:225-271).  Bounce d reads the queue just
-// traced (qcur, count in C_PQ + d % 3) and appends its survivors to qnext
-// (count in C_PQ + (d + 1) % 3, zeroed by the bounce before); it zeroes
-// C_PQ + (d + 2) % 3 for the bounce after.  The host alternates the two
-// queue buffers, so no launch sits between a bounce and the next trace.
-__global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px, Grid g, uint32_t index0,
-                                                      int depth, const int *qcur, int *qnext,
-                                                      unsigned long long *stats) {
+// Photon hits waiting for their iteration's grid.  The reference deposits a
+// photon at each bounce after the first as it traces (sppm.cpp:230-257); a
+// photon's path depends only on its sample index, not on the visible points,
+// so the photons of a batch of iterations are traced together (more rays per
+// traversal launch) and every hit is kept until its iteration's grid is
+// sorted.  SoA: nine planes (point, -direction, beta) of `total` floats;
+// iteration j of the batch owns entries [j * cap, j * cap + cnt[j]).
+constexpr int kPhotonBatchRays = 1 << 22;        // photons per batch (up to this many)
+constexpr int kPhotonBatchMax = 16;               // iterations per batch
+constexpr size_t kPhotonHitBytes = (size_t)2 << 30;  // hit records of a batch
+struct PhotonHits {
+  float *f;
+  int *cnt;
+  int cap;      // entries per iteration: photons x (maxDepth - 1), each photon's bounces after the first
+  int photons;  // photons per iteration
+  size_t total;
+};
+
+// One photon bounce (sppm.cpp:225-271) of a batch: the hit is recorded for
+// its iteration's deposits (bounces after the first), then the BSDF is
+// sampled and the photon continues or ends by Russian roulette.  Bounce d
+// reads the queue just traced (qcur, count in C_PQ + d % 3) and appends its
+// survivors to qnext (count in C_PQ + (d + 1) % 3, zeroed by the bounce
+// before); it zeroes C_PQ + (d + 2) % 3 for the bounce after.  The host
+// alternates the two queue buffers, so no launch sits between a bounce and
+// the next trace.
+__global__ __launch_bounds__(256) void k_photon_bounce(DScene sc, Wave w, PhotonHits ph, uint32_t index0, int depth,
+                                                       const int *qcur, int *qnext, unsigned long long *stats) {
   __shared__ int lds[5];
-  __shared__ float phot[256 * 9];
-  __shared__ int ebase[256], inc[256];
   int *const cnt = w.ctrl + C_PQ;
   const int count = cnt[depth % 3];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -596,65 +615,115 @@ __global__ __launch_bounds__(256) void k_photon_shade(DScene sc, Wave w, Pix px,
     stat_add(stats, S_CLOSEST, (unsigned long long)count);
     stat_add(stats, S_PHOTON_RAYS, (unsigned long long)count);
   }
-  const int wv = threadIdx.x >> 6;
-  unsigned long long ndep = 0, nlook = 0, nscan = 0;
+  const int lane = lane_id();
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  unsigned long long nrec = 0, ovf = 0;
   for (int base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
     int i = base + threadIdx.x;
-    bool keep = false, hit = false;
-    int p = 0, e0 = 0, n = 0;
-    Isect is;
-    V3 d = mk(0, 0, 0);
+    bool keep = false, rec = false;
+    int p = 0, jb = 0;
+    V3 hpos = mk(0, 0, 0), d = mk(0, 0, 0);
     Sp beta = sp(0);
     if (i < count) {
       p = qcur[i];
       int hp = w.hprim[p];
       if (hp >= 0) {
-        hit = true;
+        Isect is;
         make_isect(sc, hp, w.hu[p], w.hv[p], &is);
         d = mk(w.dx[p], w.dy[p], w.dz[p]);
         beta = sp3(w.br[p], w.bg[p], w.bb[p]);
-        int c[3];
-        if (depth > 0 && to_grid(is.p, g.fctl, g.fctl + 3, g.ictl, c)) {
-          uint32_t h = grid_hash(c[0], c[1], c[2], g.hash_size);
-          if (g.photon_cell) atomicAdd(&g.photon_cell[h], 1);  // debug output only
-          e0 = g.cell_start[h];
-          n = g.cell_end[h] - e0;
-          ++nlook;
-          nscan += (unsigned long long)n;
-          float *f = phot + 9 * threadIdx.x;
-          f[0] = is.p.x; f[1] = is.p.y; f[2] = is.p.z;
-          f[3] = -d.x; f[4] = -d.y; f[5] = -d.z;
-          f[6] = beta.r; f[7] = beta.g; f[8] = beta.b;
+        hpos = is.p;
+        rec = depth > 0;
+        jb = p / ph.photons;
+        Bsdf bs;
+        make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)is.mat, is.ng, is.ns, is.u, is.v, true, &bs);
+        uint32_t index = index0 + (uint32_t)p;
+        int dim = w.dim[p];
+        float u1 = sobol(sc.sobol, index, dim++), u0 = sobol(sc.sobol, index, dim++);
+        V3 wi = mk(0, 0, 0);
+        float pdf = 0;
+        int flags = 0;
+        Sp fr = bsdf_sample(bs, &wi, neg(d), u0, u1, &pdf, ALL, &flags);
+        if (!(black(fr) || pdf == 0)) {
+          Sp bnew = beta * fr * absdot(wi, is.ns) / pdf;
+          float q = mx(0.0f, 1 - intensity(bnew) / intensity(beta));
+          if (!(sobol(sc.sobol, index, dim++) < q)) {
+            Sp bkeep = bnew / (1 - q);
+            V3 o = offset_ray_origin(is.p, is.ng, wi);
+            w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
+            w.dx[p] = wi.x; w.dy[p] = wi.y; w.dz[p] = wi.z;
+            w.br[p] = bkeep.r; w.bg[p] = bkeep.g; w.bb[p] = bkeep.b;
+            keep = true;
+          }
         }
+        w.dim[p] = dim;
       }
     }
-    if (depth > 0) ndep += photon_deposits(sc, px, g, e0, n, phot + 9 * 64 * wv, ebase + 64 * wv, inc + 64 * wv);
-    if (hit) {
-      Bsdf bs;
-      make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)is.mat, is.ng, is.ns, is.u, is.v, true, &bs);
-      uint32_t index = index0 + (uint32_t)p;
-      int dim = w.dim[p];
-      float u1 = sobol(sc.sobol, index, dim++), u0 = sobol(sc.sobol, index, dim++);
-      V3 wi = mk(0, 0, 0);
-      float pdf = 0;
-      int flags = 0;
-      Sp fr = bsdf_sample(bs, &wi, neg(d), u0, u1, &pdf, ALL, &flags);
-      if (!(black(fr) || pdf == 0)) {
-        Sp bnew = beta * fr * absdot(wi, is.ns) / pdf;
-        float q = mx(0.0f, 1 - intensity(bnew) / intensity(beta));
-        if (!(sobol(sc.sobol, index, dim++) < q)) {
-          beta = bnew / (1 - q);
-          V3 o = offset_ray_origin(is.p, is.ng, wi);
-          w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
-          w.dx[p] = wi.x; w.dy[p] = wi.y; w.dz[p] = wi.z;
-          w.br[p] = beta.r; w.bg[p] = beta.g; w.bb[p] = beta.b;
-          keep = true;
+    // record the hits, one atomic per iteration present in the wave
+    uint64_t left = __ballot(rec);
+    while (left) {
+      const int leader = __ffsll((long long)left) - 1;
+      const int j = __shfl(jb, leader);
+      const uint64_t mj = __ballot(rec && jb == j);
+      int at = 0;
+      if (lane == leader) at = atomicAdd(&ph.cnt[j], __popcll(mj));
+      at = __shfl(at, leader) + __popcll(mj & below);
+      if (rec && jb == j) {
+        if (at < ph.cap) {
+          const size_t k = (size_t)j * ph.cap + at;
+          float *f = ph.f + k;
+          f[0] = hpos.x; f[ph.total] = hpos.y; f[2 * ph.total] = hpos.z;
+          f[3 * ph.total] = -d.x; f[4 * ph.total] = -d.y; f[5 * ph.total] = -d.z;
+          f[6 * ph.total] = beta.r; f[7 * ph.total] = beta.g; f[8 * ph.total] = beta.b;
+          ++nrec;
+        } else {
+          ++ovf;
         }
       }
-      w.dim[p] = dim;
+      left &= ~mj;
     }
     int a = block_append(&cnt[(depth + 1) % 3], keep, lds);
     if (keep) qnext[a] = p;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    nrec += __shfl_down(nrec, off);
+    ovf += __shfl_down(ovf, off);
+  }
+  if (lane == 0) {
+    stat_add(stats, S_PHOTON_HITS, nrec);
+    if (ovf) stat_add(stats, S_OVERFLOW, ovf);
+  }
+}
+
+// The deposits of iteration jb's photon hits into its visible points
+// (sppm.cpp:230-257): per hit its grid cell, then the cell's visible points
+// walked by photon_deposits.
+__global__ __launch_bounds__(256) void k_photon_deposit(DScene sc, Pix px, Grid g, PhotonHits ph, int jb,
+                                                        unsigned long long *stats) {
+  __shared__ float phot[256 * 9];
+  __shared__ int ebase[256], inc[256];
+  const int count = min(ph.cnt[jb], ph.cap);
+  const int wv = threadIdx.x >> 6;
+  unsigned long long ndep = 0, nlook = 0, nscan = 0;
+  for (int base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+    const int i = base + threadIdx.x;
+    int e0 = 0, n = 0;
+    if (i < count) {
+      const float *f = ph.f + (size_t)jb * ph.cap + i;
+      const V3 hp = mk(f[0], f[ph.total], f[2 * ph.total]);
+      int c[3];
+      if (to_grid(hp, g.fctl, g.fctl + 3, g.ictl, c)) {
+        uint32_t h = grid_hash(c[0], c[1], c[2], g.hash_size);
+        if (g.photon_cell) atomicAdd(&g.photon_cell[h], 1);  // debug output only
+        e0 = g.cell_start[h];
+        n = g.cell_end[h] - e0;
+        ++nlook;
+        nscan += (unsigned long long)n;
+        float *q = phot + 9 * threadIdx.x;
+        for (int k = 0; k < 9; ++k) q[k] = f[k * ph.total];
+      }
+    }
+    ndep += photon_deposits(sc, px, g, e0, n, phot + 9 * 64 * wv, ebase + 64 * wv, inc + 64 * wv);
   }
   for (int off = 32; off > 0; off >>= 1) {
     ndep += __shfl_down(ndep, off);
@@ -865,7 +934,28 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   if (rc) return rc;
   SppmBuffers &b = *s->sppm;
   Grid &g = b.g;
-  if ((rc = s->wave.ensure(std::max(npix, photons)))) return rc;
+  // Photon batches: the photons of B iterations are traced together (on the
+  // caustics scene 2^20-ray traversal launches ran at 0.81 ns per ray, 2^22
+  // at 0.46, 2^23 at 0.42); their hits wait in b.hits until each
+  // iteration's grid is sorted.
+  int B = 1;
+  PhotonHits ph{};
+  if (photons > 0) {
+    const int64_t cap = (int64_t)photons * std::max(1, maxDepth - 1);
+    if (cap >= ((int64_t)1 << 31)) {
+      set_error("SPPM photons x (maxDepth - 1) exceeds 2^31 photon hits per iteration");
+      return ORION_ERR_UNSUPPORTED;
+    }
+    B = std::max(1, std::min({n_iters, kPhotonBatchRays / photons, kPhotonBatchMax}));
+    while (B > 1 && (size_t)B * cap * 36 > kPhotonHitBytes) --B;
+    if ((rc = b.hits.ensure((size_t)B * cap * 36 + (size_t)B * 4 + 256))) return rc;
+    ph.total = (size_t)B * cap;
+    ph.f = (float *)b.hits.ptr;
+    ph.cnt = (int *)(ph.f + 9 * ph.total);
+    ph.cap = (int)cap;
+    ph.photons = photons;
+  }
+  if ((rc = s->wave.ensure(std::max(npix, B * photons)))) return rc;
   Wave w = s->wave.wave;
   w.ctrl = (int *)s->ctrl.ptr;
   w.stats = (unsigned long long *)s->stats.ptr;
@@ -889,7 +979,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   // SPPM's launches trace at most 2^20 rays: 128-entry chunks (two rays per
   // lane) measured 3% faster per iteration than 64 and 10% faster than 256
   const int chunk = 128;
-  if ((rc = b.spill.ensure(trace_spill_ints(s->d.stack_bound, ring, trace_grid(1, std::max(photons, 1), false, chunk)) *
+  if ((rc = b.spill.ensure(trace_spill_ints(s->d.stack_bound, ring, trace_grid(1, std::max(B * photons, 1), false, chunk)) *
                                4 + 4)))
     return rc;
   // the camera pass traces both its queues in one launch
@@ -940,10 +1030,46 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     HIP_TRY(hipEventRecord(b.cam_done[ci & 1], cst));
     return ORION_OK;
   };
+  // photon paths of iterations [i0, i0 + nb) (sppm.cpp:199-273 without the
+  // deposits): photon p of the batch is photon p % photons of iteration
+  // i0 + p / photons, sample index nIterations * nPixels + i0 * photons + p
+  // (sppm.cpp:203)
+  auto photon_batch = [&](int i0, int nb) -> int {
+    const int P = nb * photons;
+    Wave pw = w;
+    pw.N = P;
+    pw.seg = P;
+    pw.nseg = 1;
+    const uint32_t index0 = (uint32_t)(n_iters * npix + i0 * photons);
+    HIP_TRY(hipMemsetAsync(ph.cnt, 0, (size_t)nb * 4, st));
+    HIP_TRY(hipMemsetAsync(w.ctrl + C_PQ, 0, 4, st));
+    hipLaunchKernelGGL(k_photon_init, dim3((P + TB - 1) / TB), dim3(TB), 0, st, s->d, pw, index0, P);
+    int *qbuf[2] = {w.qray, w.qpath[0]};  // bounce d traces qbuf[d & 1]
+    const int blocks = std::min((P + 255) / 256, 8192);
+    for (int depth = 0; depth < maxDepth; ++depth) {
+      Wave tw = pw;
+      tw.qray = qbuf[depth & 1];
+      tw.cnt_ray = w.ctrl + C_PQ + depth % 3;
+      int rc2;
+      if ((rc2 = ptimer.mark(0))) return rc2;
+      launch_trace_photons(dj, tw, ring, chunk, st);
+      if ((rc2 = ptimer.mark(0)) || (rc2 = ptimer.mark(1))) return rc2;
+      hipLaunchKernelGGL(k_photon_bounce, dim3(blocks), dim3(256), 0, st, s->d, tw, ph, index0, depth,
+                         qbuf[depth & 1], qbuf[(depth + 1) & 1], dstats);
+      if ((rc2 = ptimer.mark(1))) return rc2;
+    }
+    return ORION_OK;
+  };
   if ((rc = camera_pass(0))) return rc;
+  int batch0 = 0, nbatch = 0;
   for (int iter = 0; iter < n_iters; ++iter) {
     // the next iteration's camera pass overlaps this one's grid and photon pass
     if (iter + 1 < n_iters && (rc = camera_pass(iter + 1))) return rc;
+    if (photons > 0 && iter >= batch0 + nbatch) {
+      batch0 = iter;
+      nbatch = std::min(B, n_iters - iter);
+      if ((rc = photon_batch(batch0, nbatch))) return rc;
+    }
     HIP_TRY(hipStreamWaitEvent(st, b.cam_done[iter & 1], 0));
     const Pix &px = b.px[iter & 1];
     // --- grid
@@ -1025,28 +1151,12 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
         for (int h = 0; h < hash_size; ++h) { dbg->cell_range[2 * h] = cs[h]; dbg->cell_range[2 * h + 1] = ce[h]; }
       }
     }
-    // --- photon pass (sppm.cpp:199-273)
+    // --- deposits of this iteration's photon hits (sppm.cpp:230-257)
     if (photons > 0) {
-      Wave pw = w;
-      pw.N = photons;
-      pw.seg = photons;
-      pw.nseg = 1;
-      uint32_t index0 = (uint32_t)(n_iters * npix + iter * photons);
-      HIP_TRY(hipMemsetAsync(w.ctrl + C_PQ, 0, 4, st));
-      hipLaunchKernelGGL(k_photon_init, dim3((photons + TB - 1) / TB), dim3(TB), 0, st, s->d, pw, index0, photons);
-      int *qbuf[2] = {w.qray, w.qpath[0]};  // bounce d traces qbuf[d & 1]
-      const int blocks = std::min((photons + 255) / 256, 8192);
-      for (int depth = 0; depth < maxDepth; ++depth) {
-        Wave tw = pw;
-        tw.qray = qbuf[depth & 1];
-        tw.cnt_ray = w.ctrl + C_PQ + depth % 3;
-        if ((rc = ptimer.mark(0))) return rc;
-        launch_trace_photons(dj, tw, ring, chunk, st);
-        if ((rc = ptimer.mark(0)) || (rc = ptimer.mark(1))) return rc;
-        hipLaunchKernelGGL(k_photon_shade, dim3(blocks), dim3(256), 0, st, s->d, tw, px, gi, index0, depth,
-                           qbuf[depth & 1], qbuf[(depth + 1) & 1], dstats);
-        if ((rc = ptimer.mark(1))) return rc;
-      }
+      if ((rc = ptimer.mark(1))) return rc;
+      const int dblocks = std::max(1, std::min((ph.cap + 255) / 256, 8192));
+      hipLaunchKernelGGL(k_photon_deposit, dim3(dblocks), dim3(256), 0, st, s->d, px, gi, ph, iter - batch0, dstats);
+      if ((rc = ptimer.mark(1))) return rc;
     }
     if (want_dbg && iter == dbg->debug_iter && dbg->photon_cell_count)
       HIP_TRY(hipMemcpyAsync(dbg->photon_cell_count, g.photon_cell, (size_t)hash_size * 4, hipMemcpyDeviceToHost, st));
@@ -1103,6 +1213,8 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     stats->photon_trace_launches = pn[0];
     stats->photon_shade_ms = pms[1];
     stats->photon_shade_launches = pn[1];
+    stats->photon_hits = (int64_t)h[S_PHOTON_HITS];
+    stats->photon_batch = photons > 0 ? B : 0;
   }
   return ORION_OK;
 }

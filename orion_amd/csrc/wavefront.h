@@ -35,8 +35,10 @@ enum : int {
 enum : int {
   S_CLOSEST = 0, S_SHADOW = 1, S_REFEQ = 2, S_PATHS = 3, S_NODES_C = 4, S_TRIS_C = 5,
   S_PHOTONS = 6, S_DEPOSITS = 7, S_NODES_S = 8, S_TRIS_S = 9, S_OVERFLOW = 10, S_SHADED = 11,
-  // SPPM photon pass: traversal work, rays, grid lookups, visible points scanned
-  S_NODES_P = 12, S_TRIS_P = 13, S_PHOTON_RAYS = 14, S_LOOKUPS = 15, S_SCANNED = 16, S_COUNT = 17,
+  // SPPM photon pass: traversal work, rays, grid lookups, visible points
+  // scanned, hits recorded for deposits
+  S_NODES_P = 12, S_TRIS_P = 13, S_PHOTON_RAYS = 14, S_LOOKUPS = 15, S_SCANNED = 16, S_PHOTON_HITS = 17,
+  S_COUNT = 18,
 };
 constexpr int kStatStripes = 64;
 constexpr size_t kStatBytes = (size_t)S_COUNT * kStatStripes * 8;

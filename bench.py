@@ -93,6 +93,8 @@ def parse():
                     help="target CPU-baseline work (bounded sample)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-jobs", type=int, default=64, help="batched-launch leg (0 = skip)")
+    ap.add_argument("--batch-ceiling", action="store_true",
+                    help="batched leg: also time every job in full default waves (the rate batching can reach)")
     ap.add_argument("--serve-requests", type=int, default=24, help="serving leg (0 = skip)")
     ap.add_argument("--serve-policy", choices=["pack", "aged", "arrival", "fair"], default="pack",
                     help="the server's wave order: least remaining work first (optionally credited for "
@@ -172,7 +174,7 @@ def c5_request_specs(n_req, seed):
     return reqs, rng
 
 
-def batch_leg(oa, dev, n_jobs, seed):
+def batch_leg(oa, dev, n_jobs, seed, ceiling=False):
     """Batched launch of n_jobs seeded procedural jobs (displaced icosphere of
     1k-500k triangles, log-uniform; film sides uniform in [128, 1024]; spp
     uniform in [4, 64]; depth 5-8; material drawn per mesh) against the plain
@@ -217,7 +219,27 @@ def batch_leg(oa, dev, n_jobs, seed):
     t_batch = time.perf_counter() - t0
     equal = sum(int(np.array_equal(a.view(np.int32), b.view(np.int32)))
                 for a, (b, _) in zip(plain_films, outs))
-    return {
+    ceil = None
+    if ceiling:
+        # each job's own scene, film and depth rendered with enough samples to
+        # fill a default wave: the job's paths at that full-wave rate are the
+        # time a batcher that packs every job into full waves would need
+        t_ideal = 0.0
+        for ds, W, H, spp, depth in jobs:
+            s_full = max(spp, (1 << 26) // (W * H))
+            ds.film_clear()
+            dev.synchronize()
+            t0 = time.perf_counter()
+            ds.render_path(0, s_full, depth)
+            dev.synchronize()
+            dt = time.perf_counter() - t0
+            t_ideal += dt * spp / s_full
+        ceil = {"method": "per job: the same scene, film and depth with max(spp, 2^26 / pixels) samples "
+                          "(at least one full default wave), its time scaled to the job's spp; summed",
+                "full_wave_paths_per_s": round(paths / t_ideal, 1),
+                "speedup_ceiling_vs_plain": round(t_plain / t_ideal, 3),
+                "batched_fraction_of_ceiling": round(t_ideal / t_batch, 3)}
+    out = {
         "workload": f"{n_jobs} jobs, displaced icosphere 1k-500k tris log-uniform "
                     f"({tris_tot} tris total), films 128..1024 per side, 4-64 spp, depth 5-8, "
                     f"Lambert/GGX/dielectric per mesh",
@@ -240,6 +262,9 @@ def batch_leg(oa, dev, n_jobs, seed):
                    "use_graph": tune.use_graph},
         "autotune_s": round(tune_s, 2), "setup_s": round(setup_s, 2),
     }
+    if ceil:
+        out["ceiling"] = ceil
+    return out
 
 
 def sppm_traffic(kernel):
@@ -530,7 +555,7 @@ def main():
         out["sppm"] = sppm_leg(oa, dev, a.sppm_iters)
         print("# sppm leg done", file=sys.stderr, flush=True)
     if a.batch_jobs > 0:
-        out["batched"] = batch_leg(oa, dev, a.batch_jobs, a.seed)
+        out["batched"] = batch_leg(oa, dev, a.batch_jobs, a.seed, a.batch_ceiling)
         print(f"# batched leg done: {out['batched']['speedup']}x", file=sys.stderr, flush=True)
     if a.serve_requests > 0:
         out["serving"] = serve_leg(oa, dev, a.serve_requests, a.serve_load, a.hbm_budget_gib, a.seed,

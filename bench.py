@@ -43,6 +43,7 @@ PHOTON_SEG_BYTES = 168
 CELL_RANGE_BYTES = 8
 VP_SCAN_BYTES = 20
 DEPOSIT_BYTES = 80
+HIT_RECORD_BYTES = 72  # a photon hit kept for its iteration's deposits: 36 B written, 36 B read
 # a bounce's traversal: the closest-hit and any-hit kernels, launched side by
 # side on two streams (one fork / join per bounce)
 TRACE_KERNELS = ("k_trace<false, 16, false>", "k_trace<true, 16, false>")
@@ -297,13 +298,15 @@ def sppm_leg(oa, dev, iters):
     t_launch = d["photon_trace_ms"] / 1e3 / tl
     achieved = trace_bytes / tl / t_launch / 1e9 if t_launch > 0 else 0.0
     rays = max(d["photon_rays"], 1)
-    # photon shading (k_photon_shade): per photon hit its segment's state and
-    # shading record, per in-bounds lookup the cell range, per scanned entry
-    # the visible point's pixel, radius and position, per deposit its frame,
-    # direction and material index read and Phi, M added
+    # photon shading (k_photon_bounce + k_photon_deposit): per photon segment
+    # its state and shading record, per hit kept for the deposits its record
+    # written and read back, per in-bounds lookup the cell range, per scanned
+    # entry the visible point's pixel, radius and position, per deposit its
+    # frame, direction and material index read and Phi, M added
     sl = max(d["photon_shade_launches"], 1)
     shade_bytes = (d["photon_rays"] * PHOTON_SEG_BYTES + d["photon_lookups"] * CELL_RANGE_BYTES
-                   + d["photon_vp_scanned"] * VP_SCAN_BYTES + d["deposits"] * DEPOSIT_BYTES)
+                   + d["photon_vp_scanned"] * VP_SCAN_BYTES + d["deposits"] * DEPOSIT_BYTES
+                   + d["photon_hits"] * HIT_RECORD_BYTES)
     s_launch = d["photon_shade_ms"] / 1e3 / sl
     return {
         "workload": f"SPPM caustics, glass icosphere ({hs.n_tris} triangles) on a matte plane, "
@@ -320,8 +323,8 @@ def sppm_leg(oa, dev, iters):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": sppm_traffic("k_trace_photons<16>"),
             "traffic_source": f"{SPPM_TRAFFIC_FILE.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE x 2.0, "
                               f"WRITE_SIZE x 1.0, per dispatch)",
-            "kernel": "k_trace_photons<16>: one photon bounce's closest-hit traversal (the photon "
-                      "pass of every iteration, 10 bounces each)",
+            "kernel": "k_trace_photons<16>: one photon bounce's closest-hit traversal over a batch of "
+                      "iterations' photons (10 bounces per batch)",
             "algorithmic_bytes_per_launch": round(trace_bytes / tl),
             "launch_ms": round(t_launch * 1e3, 4), "launches": int(tl),
             "photon_rays": int(d["photon_rays"]),
@@ -332,8 +335,10 @@ def sppm_leg(oa, dev, iters):
                 "unit": "GB/s", "launch_ms": round(s_launch * 1e3, 4),
                 "algorithmic_bytes_per_launch": round(shade_bytes / sl),
                 "lookups": int(d["photon_lookups"]), "vp_scanned": int(d["photon_vp_scanned"]),
-                "bytes": f"{PHOTON_SEG_BYTES} B per photon hit, {CELL_RANGE_BYTES} B per cell lookup, "
-                         f"{VP_SCAN_BYTES} B per visible point scanned, {DEPOSIT_BYTES} B per deposit"},
+                "bytes": f"{PHOTON_SEG_BYTES} B per photon segment, {HIT_RECORD_BYTES} B per hit kept for "
+                         f"deposits, {CELL_RANGE_BYTES} B per cell lookup, {VP_SCAN_BYTES} B per visible point "
+                         f"scanned, {DEPOSIT_BYTES} B per deposit",
+                "photon_batch": int(d["photon_batch"])},
         },
     }
 

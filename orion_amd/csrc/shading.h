@@ -289,15 +289,16 @@ __device__ inline Sp light_sample_le(const DScene &sc, int l, float a0, float a1
   return Le * spot_falloff(L, *rd);
 }
 
-// Second half of estimateDirect: adds the pending light-sampled term (when
-// its shadow ray was unoccluded) and the BSDF-sampled term (when its
-// visibility ray was) to L, as L += beta * (Ld / lightPdf).
-__device__ inline void resolve_direct(const Wave &w, int p, int fl, Sp *L) {
+// Second half of estimateDirect: the pending light-sampled term (when its
+// shadow ray was unoccluded) plus the BSDF-sampled term (when its
+// visibility ray was), as the radiance term beta * (Ld / lightPdf).
+__device__ inline Sp direct_term(const Wave &w, int p, int fl) {
   Sp Ld = sp(0);
   if ((fl & F_SHADOW) && w.vis[p]) acc(Ld, sp3(w.Ar[p], w.Ag[p], w.Ab[p]));
   if ((fl & F_MIS) && w.vis[w.N + p]) acc(Ld, sp3(w.Mr[p], w.Mg[p], w.Mb[p]));
-  acc(*L, sp3(w.nbr[p], w.nbg[p], w.nbb[p]) * (Ld / w.sel[p]));
+  return sp3(w.nbr[p], w.nbg[p], w.nbb[p]) * (Ld / w.sel[p]);
 }
+__device__ inline void resolve_direct(const Wave &w, int p, int fl, Sp *L) { acc(*L, direct_term(w, p, fl)); }
 
 // uniformSampleOneLight's light choice (core/integrator.cpp:8-16): by the
 // power CDF, or uniform when `uniform`; one Sobol draw.  Returns the light,

@@ -65,29 +65,41 @@ struct Grid {
   int hash_size;
 };
 
-// Iteration i's camera pass runs on its own stream beside iteration i - 1's
-// photon pass (it depends on nothing the photon pass, the update or the
-// film touch), so it has its own wave buffers and traversal spill area;
-// visible points alternate between two sets (px[i % 2]) and the film reads
-// a snapshot of Ld taken when the camera pass ends (ld_snap[i % 2]).
+// The camera passes of a batch of iterations run on their own stream beside
+// the previous batch's grids, deposits and photons (they depend on nothing
+// those touch), so they have their own wave buffers and traversal spill
+// area.  Iteration i writes visible-point set i % nsets (px[i % nsets]: the
+// pixel state is shared, each set has its own visible-point fields, set s
+// at s x npix in field-major arrays), and the film reads the snapshot of Ld
+// the replay takes after iteration i (set i % nsets of ld_snap).  nsets is
+// two batches' worth, so one batch's camera passes fill the sets the
+// previous batch is not using.
 struct SppmBuffers {
   DevBuf pix_mem, grid_mem, pair_mem, hist_mem, scan_mem, job, spill, spill_cam;
-  DevBuf hits;  // photon hits of a batch of iterations, waiting for their deposits
+  DevBuf hits;       // photon hits of a batch of iterations, waiting for their deposits
+  DevBuf cam_terms;  // a camera batch's recorded radiance terms
   WaveBuffers camw;
-  Pix px[2]{};
-  float *ld_snap[2] = {nullptr, nullptr};  // 3 x npix each (r, g, b planes)
+  std::vector<Pix> px;
+  float *ld_snap = nullptr;  // nsets x 3 planes (r, g, b) x npix
+  int nsets = 0, cam_batch = 1;
   Grid g{};
   int npix = 0, hash_size = 0;
   size_t pair_cap = 0, hist_cap = 0, scan_cap = 0;
   hipStream_t cam_stream = nullptr;
   hipEvent_t cam_done[2] = {nullptr, nullptr}, iter_done = nullptr;
+  std::vector<hipEvent_t> iter_end;  // end of iteration i: iter_end[i % nsets]
   ~SppmBuffers() {
     if (cam_stream) { (void)hipStreamSynchronize(cam_stream); (void)hipStreamDestroy(cam_stream); }
     for (hipEvent_t e : cam_done)
       if (e) (void)hipEventDestroy(e);
     if (iter_done) (void)hipEventDestroy(iter_done);
+    for (hipEvent_t e : iter_end)
+      if (e) (void)hipEventDestroy(e);
   }
 };
+// camera paths per batch: up to 2^20 (iterations of 2^18-pixel films at 4 per
+// batch), at most 8 iterations
+constexpr int kCamBatchPaths = 1 << 20, kCamBatchMax = 8;
 
 void destroy_sppm(SppmBuffers *b) { delete b; }
 
@@ -121,7 +133,26 @@ __device__ __forceinline__ V3 ld3(const float *a, int i) { return mk(a[3 * i], a
 __device__ __forceinline__ void st3(float *a, int i, V3 v) { a[3 * i] = v.x; a[3 * i + 1] = v.y; a[3 * i + 2] = v.z; }
 
 // ---------------------------------------------------------------- camera pass
-__global__ void k_cam_raygen(DScene sc, Wave w, Pix px, int iter) {
+// The camera passes of a batch of nb iterations [c0, c0 + nb) run as one
+// wavefront of nb x npix paths: path p is pixel q = p % npix of iteration
+// c0 + p / npix.  Iteration ci writes visible-point set ci % nsets.  A
+// path's radiance terms (emission, direct light, envmap on escape) are
+// recorded in order instead of being added to the pixel's Ld at once: Ld
+// accumulates across iterations (sppm.cpp:93-99 adds each term to the
+// running sum), so k_cam_replay adds them afterwards, iteration by
+// iteration, term by term, in the reference's order.
+struct CamTerms {
+  float *t;  // 3 planes (r, g, b) of T x N floats: term k of path p at k * N + p
+  int *n;    // terms recorded per path
+  int T, N;
+};
+
+__device__ __forceinline__ int vp_index(int p, int npix, int c0, int nsets) {
+  const int j = p / npix;
+  return ((c0 + j) % nsets) * npix + (p - j * npix);
+}
+
+__global__ void k_cam_raygen(DScene sc, Wave w, CamTerms ct, int c0) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p == 0) {
     w.ctrl[C_PATH0] = w.N;
@@ -131,8 +162,9 @@ __global__ void k_cam_raygen(DScene sc, Wave w, Pix px, int iter) {
     w.ctrl[C_WORK_SHADOW] = 0;
   }
   if (p >= w.N) return;
-  int j = p / w.W, i = p - j * w.W;
-  uint32_t index = (uint32_t)iter * (uint32_t)w.npix + (uint32_t)p;  // sppm.cpp:86
+  const int jb = p / w.npix, q = p - jb * w.npix;
+  int j = q / w.W, i = q - j * w.W;
+  uint32_t index = (uint32_t)(c0 + jb) * (uint32_t)w.npix + (uint32_t)q;  // sppm.cpp:86
   V3 o, d;
   camera_ray(sc, i, j, index, &o, &d);
   w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
@@ -144,13 +176,16 @@ __global__ void k_cam_raygen(DScene sc, Wave w, Pix px, int iter) {
   w.flags[p] = F_ALIVE;
   w.qpath[0][p] = p;
   w.qray[p] = p;
+  ct.n[p] = 0;
 }
 
-// One camera-pass step (sppm.cpp:93-138).  w.L* alias the pixels' Ld.
-__global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, int cur, int maxDepth, int iter) {
+// One camera-pass step (sppm.cpp:93-138) of a batch; px is visible-point set 0.
+__global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, CamTerms ct, int cur, int maxDepth,
+                                                   int c0, int nsets, unsigned long long *stats) {
   const int nxt = cur ^ 1;
   const int count = w.ctrl[C_PATH0 + cur];
   const int N = w.N;
+  unsigned long long lost = 0;
   for (int base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
     int i = base + threadIdx.x;
     bool act = i < count;
@@ -158,15 +193,28 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, in
     bool want_ray0 = false, want_ray1 = false, want_shadow = false, keep = false;
     if (act) {
       int fl = w.flags[p];
-      Sp L = sp3(w.Lr[p], w.Lg[p], w.Lb[p]);
+      int nt = ct.n[p];
+      auto put = [&](Sp x) {
+        if (nt < ct.T) {
+          const size_t k = (size_t)nt * ct.N + p;
+          ct.t[k] = x.r;
+          ct.t[(size_t)ct.T * ct.N + k] = x.g;
+          ct.t[2 * (size_t)ct.T * ct.N + k] = x.b;
+        } else {
+          ++lost;
+        }
+        ++nt;
+      };
       if (fl & F_PEND) {
-        resolve_direct(w, p, fl, &L);
+        put(direct_term(w, p, fl));
         fl &= ~(F_PEND | F_SHADOW | F_MIS);
       }
       if (fl & F_ALIVE) {
         fl &= ~F_ALIVE;
         int depth = w.bounce[p];
         int hp = w.hprim[p];
+        const uint32_t jb = (uint32_t)(p / w.npix);
+        const uint32_t index = ((uint32_t)c0 + jb) * (uint32_t)w.npix + (uint32_t)(p - (int)jb * w.npix);
         if (hp >= 0) {
           Sp beta = sp3(w.br[p], w.bg[p], w.bb[p]);
           V3 d = mk(w.dx[p], w.dy[p], w.dz[p]);
@@ -175,8 +223,7 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, in
           Bsdf bs;
           make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)is.mat, is.ng, is.ns, is.u, is.v, false, &bs);
           V3 wo = neg(d);
-          if ((depth == 0 || (fl & F_SPEC)) && is.light >= 0) acc(L, beta * area_emission(sc, is, wo));
-          uint32_t index = (uint32_t)iter * (uint32_t)w.npix + (uint32_t)p;
+          if ((depth == 0 || (fl & F_SPEC)) && is.light >= 0) put(beta * area_emission(sc, is, wo));
           int dim = w.dim[p];
           int ref_rays = 0;
           int nf = setup_direct(sc, w, p, bs, is, d, beta, index, &dim, true, &ref_rays);
@@ -186,11 +233,12 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, in
           bool isDiffuse = bsdf_num(bs, DIFF | REFL | TRANS) > 0;
           bool isGlossy = bsdf_num(bs, GLOSSY | REFL | TRANS) > 0;
           if (isDiffuse || (isGlossy && depth == maxDepth - 1)) {
-            px.vpx[p] = is.p.x; px.vpy[p] = is.p.y; px.vpz[p] = is.p.z;
-            px.vwox[p] = wo.x; px.vwoy[p] = wo.y; px.vwoz[p] = wo.z;
-            st3(px.vng, p, is.ng); st3(px.vns, p, is.ns); st3(px.vu, p, is.u); st3(px.vv, p, is.v);
-            px.vmat[p] = is.mat;
-            px.vbr[p] = beta.r; px.vbg[p] = beta.g; px.vbb[p] = beta.b;
+            const int v = vp_index(p, w.npix, c0, nsets);
+            px.vpx[v] = is.p.x; px.vpy[v] = is.p.y; px.vpz[v] = is.p.z;
+            px.vwox[v] = wo.x; px.vwoy[v] = wo.y; px.vwoz[v] = wo.z;
+            st3(px.vng, v, is.ng); st3(px.vns, v, is.ns); st3(px.vu, v, is.u); st3(px.vv, v, is.v);
+            px.vmat[v] = is.mat;
+            px.vbr[v] = beta.r; px.vbg[v] = beta.g; px.vbb[v] = beta.b;
           } else if (depth < maxDepth - 1) {
             float u1 = sobol(sc.sobol, index, dim++), u0 = sobol(sc.sobol, index, dim++);
             V3 wi = mk(0, 0, 0);
@@ -220,10 +268,15 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, in
           w.br[p] = beta.r; w.bg[p] = beta.g; w.bb[p] = beta.b;
           w.dim[p] = dim;
         } else {  // escaped: every envmap's Le at any depth (sppm.cpp:95-99)
-          add_env_le(sc, mk(w.dx[p], w.dy[p], w.dz[p]), sp3(w.br[p], w.bg[p], w.bb[p]), &L);
+          const V3 d = mk(w.dx[p], w.dy[p], w.dz[p]);
+          const Sp beta = sp3(w.br[p], w.bg[p], w.bb[p]);
+          for (int e = 0; e < sc.n_env; ++e) {
+            const float *LR = sc.lights + 32 * (size_t)gld(sc.env_meta + 5 * e);
+            put(beta * env_le(sc, LR, d));
+          }
         }
       }
-      w.Lr[p] = L.r; w.Lg[p] = L.g; w.Lb[p] = L.b;
+      ct.n[p] = nt;
       w.flags[p] = fl;
       keep = (fl & (F_ALIVE | F_PEND)) != 0;
     }
@@ -236,6 +289,28 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, in
     a = wave_append(&w.ctrl[C_SHADOW], want_ray1);
     if (want_ray1) w.qshadow[a] = N + p;
   }
+  if (lost) stat_add(stats, S_OVERFLOW, lost);
+}
+
+// Adds a batch's recorded terms to each pixel's Ld in the reference's order
+// (iteration by iteration, term by term) and keeps Ld after each iteration
+// (the snapshot its film reads) in set (c0 + j) % nsets of snap (3 planes of
+// npix floats per set).
+__global__ void k_cam_replay(Pix px, CamTerms ct, float *snap, int npix, int nb, int c0, int nsets) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npix) return;
+  Sp Ld = sp3(px.Ldr[q], px.Ldg[q], px.Ldb[q]);
+  for (int j = 0; j < nb; ++j) {
+    const int p = j * npix + q;
+    const int n = min(ct.n[p], ct.T);
+    for (int k = 0; k < n; ++k) {
+      const size_t o = (size_t)k * ct.N + p;
+      acc(Ld, sp3(ct.t[o], ct.t[(size_t)ct.T * ct.N + o], ct.t[2 * (size_t)ct.T * ct.N + o]));
+    }
+    float *sn = snap + (size_t)((c0 + j) % nsets) * 3 * npix;
+    sn[q] = Ld.r; sn[npix + q] = Ld.g; sn[2 * npix + q] = Ld.b;
+  }
+  px.Ldr[q] = Ld.r; px.Ldg[q] = Ld.g; px.Ldb[q] = Ld.b;
 }
 
 // ---------------------------------------------------------------- grid
@@ -788,31 +863,43 @@ static int ensure_buffers(orion_scene s, int hash_size) {
   SppmBuffers &b = *s->sppm;
   int npix = s->W * s->H;
   if (b.npix != npix) {
-    // pixel state, two visible-point sets (p/wo, frame, mat, beta), two Ld snapshots
-    size_t per = 12 + 2 * (6 + 12 + 1 + 3) + 2 * 3;
+    const int Bc = std::max(1, std::min(kCamBatchPaths / std::max(npix, 1), kCamBatchMax));
+    const int nsets = 2 * Bc;
+    // pixel state, nsets visible-point sets (p/wo, frame, mat, beta), nsets Ld snapshots
+    size_t per = 12 + (size_t)nsets * (6 + 12 + 1 + 3) + (size_t)nsets * 3;
     int rc = b.pix_mem.ensure(per * (size_t)npix * 4 + 256);
     if (rc) return rc;
     char *m = (char *)b.pix_mem.ptr;
     size_t off = 0;
     auto take = [&](size_t n) { void *q = m + off; off += n * 4; return q; };
-    Pix &p = b.px[0];
+    Pix p;
     p.radius = (float *)take(npix); p.N = (float *)take(npix);
     p.Ldr = (float *)take(npix); p.Ldg = (float *)take(npix); p.Ldb = (float *)take(npix);
     p.taur = (float *)take(npix); p.taug = (float *)take(npix); p.taub = (float *)take(npix);
     p.Phir = (float *)take(npix); p.Phig = (float *)take(npix); p.Phib = (float *)take(npix);
     p.M = (int *)take(npix);
-    b.px[1] = p;
-    for (Pix &q : b.px) {
-      q.vpx = (float *)take(npix); q.vpy = (float *)take(npix); q.vpz = (float *)take(npix);
-      q.vwox = (float *)take(npix); q.vwoy = (float *)take(npix); q.vwoz = (float *)take(npix);
-      q.vng = (float *)take(3 * (size_t)npix); q.vns = (float *)take(3 * (size_t)npix);
-      q.vu = (float *)take(3 * (size_t)npix); q.vv = (float *)take(3 * (size_t)npix);
-      q.vmat = (int *)take(npix);
-      q.vbr = (float *)take(npix); q.vbg = (float *)take(npix); q.vbb = (float *)take(npix);
+    const size_t sn = (size_t)nsets * npix;
+    p.vpx = (float *)take(sn); p.vpy = (float *)take(sn); p.vpz = (float *)take(sn);
+    p.vwox = (float *)take(sn); p.vwoy = (float *)take(sn); p.vwoz = (float *)take(sn);
+    p.vng = (float *)take(3 * sn); p.vns = (float *)take(3 * sn);
+    p.vu = (float *)take(3 * sn); p.vv = (float *)take(3 * sn);
+    p.vmat = (int *)take(sn);
+    p.vbr = (float *)take(sn); p.vbg = (float *)take(sn); p.vbb = (float *)take(sn);
+    b.px.assign(nsets, p);
+    for (int k = 0; k < nsets; ++k) {
+      Pix &q = b.px[k];
+      const size_t o = (size_t)k * npix;
+      q.vpx += o; q.vpy += o; q.vpz += o; q.vwox += o; q.vwoy += o; q.vwoz += o;
+      q.vng += 3 * o; q.vns += 3 * o; q.vu += 3 * o; q.vv += 3 * o;
+      q.vmat += o; q.vbr += o; q.vbg += o; q.vbb += o;
     }
-    b.ld_snap[0] = (float *)take(3 * (size_t)npix);
-    b.ld_snap[1] = (float *)take(3 * (size_t)npix);
-    if ((rc = b.camw.ensure(npix))) return rc;
+    b.ld_snap = (float *)take(3 * sn);
+    if ((rc = b.camw.ensure(Bc * npix))) return rc;
+    for (hipEvent_t e : b.iter_end) (void)hipEventDestroy(e);
+    b.iter_end.assign(nsets, nullptr);
+    for (hipEvent_t &e : b.iter_end) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b.nsets = nsets;
+    b.cam_batch = Bc;
     b.npix = npix;
   }
   if (!b.cam_stream) {
@@ -983,7 +1070,8 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
                                4 + 4)))
     return rc;
   // the camera pass traces both its queues in one launch
-  if ((rc = b.spill_cam.ensure(trace_spill_ints(s->d.stack_bound, ring, trace_pair_grid(1, npix, chunk)) * 4 + 4)))
+  if ((rc = b.spill_cam.ensure(
+           trace_spill_ints(s->d.stack_bound, ring, trace_pair_grid(1, b.cam_batch * npix, chunk)) * 4 + 4)))
     return rc;
   w.spill = (int *)b.spill.ptr;
   w.spill_cap = std::max(0, s->d.stack_bound - ring);
@@ -1005,29 +1093,41 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   cw.k0 = 0;
   cw.spill = (int *)b.spill_cam.ptr;
   cw.spill_cap = w.spill_cap;
-  cw.N = npix;
-  cw.seg = npix;  // one queue segment; its lengths live in the control block
-  cw.nseg = 1;
+  cw.nseg = 1;  // one queue segment; its lengths live in the control block
   cw.cnt_ray = w.ctrl + C_RAY;
   cw.cnt_shadow = w.ctrl + C_SHADOW;
-  cw.Lr = b.px[0].Ldr; cw.Lg = b.px[0].Ldg; cw.Lb = b.px[0].Ldb;
   hipStream_t cst = b.cam_stream;
+  const int Bc = std::min(b.cam_batch, n_iters), nsets = b.nsets;
+  // a camera path's radiance terms: per bounce the pending direct light and
+  // either the hit's emission or every envmap's Le on escape
+  CamTerms ct;
+  ct.N = b.cam_batch * npix;
+  ct.T = (maxDepth + 1) * (1 + std::max(1, s->d.n_env));
+  if ((rc = b.cam_terms.ensure((size_t)ct.N * (3 * (size_t)ct.T + 1) * 4))) return rc;
+  ct.t = (float *)b.cam_terms.ptr;
+  ct.n = (int *)(ct.t + 3 * (size_t)ct.T * ct.N);
   HIP_TRY(hipEventRecord(b.iter_done, st));
-  auto camera_pass = [&](int ci) -> int {
-    const Pix pc = b.px[ci & 1];
-    HIP_TRY(hipStreamWaitEvent(cst, b.iter_done, 0));
-    hipLaunchKernelGGL(k_cam_raygen, dim3((npix + TB - 1) / TB), dim3(TB), 0, cst, s->d, cw, pc, ci);
+  // camera passes of iterations [c0, c0 + nb): they overwrite the sets of
+  // iterations c0 - nsets .. c0 + nb - 1 - nsets, so wait for the last of those
+  auto camera_batch = [&](int c0, int nb) -> int {
+    const int e = c0 + nb - 1 - nsets;
+    HIP_TRY(hipStreamWaitEvent(cst, e >= 0 ? b.iter_end[e % nsets] : b.iter_done, 0));
+    Wave cb = cw;
+    cb.N = nb * npix;
+    cb.seg = nb * npix;
+    hipLaunchKernelGGL(k_cam_raygen, dim3((cb.N + TB - 1) / TB), dim3(TB), 0, cst, s->d, cb, ct, c0);
     int cur = 0;
+    const int blocks = std::min((cb.N + 255) / 256, 8192);
     for (int it = 0; it <= maxDepth; ++it) {
-      launch_trace_pair(dj, cw, ring, chunk, false, cst);  // closest-hit + shadow queues in one launch
-      launch_between(cw, cur ^ 1, cst);
-      int blocks = std::min((npix + 255) / 256, 8192);
-      hipLaunchKernelGGL(k_cam_shade, dim3(blocks), dim3(256), 0, cst, s->d, cw, pc, cur, maxDepth, ci);
+      launch_trace_pair(dj, cb, ring, chunk, false, cst);  // closest-hit + any-hit queues in one launch
+      launch_between(cb, cur ^ 1, cst);
+      hipLaunchKernelGGL(k_cam_shade, dim3(blocks), dim3(256), 0, cst, s->d, cb, b.px[0], ct, cur, maxDepth, c0,
+                         nsets, dstats);
       cur ^= 1;
     }
-    // Ld r, g, b planes are contiguous
-    HIP_TRY(hipMemcpyAsync(b.ld_snap[ci & 1], b.px[0].Ldr, (size_t)3 * npix * 4, hipMemcpyDeviceToDevice, cst));
-    HIP_TRY(hipEventRecord(b.cam_done[ci & 1], cst));
+    hipLaunchKernelGGL(k_cam_replay, dim3((npix + TB - 1) / TB), dim3(TB), 0, cst, b.px[0], ct, b.ld_snap, npix, nb,
+                       c0, nsets);
+    HIP_TRY(hipEventRecord(b.cam_done[(c0 / Bc) & 1], cst));
     return ORION_OK;
   };
   // photon paths of iterations [i0, i0 + nb) (sppm.cpp:199-273 without the
@@ -1060,18 +1160,19 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     }
     return ORION_OK;
   };
-  if ((rc = camera_pass(0))) return rc;
+  if ((rc = camera_batch(0, Bc))) return rc;
   int batch0 = 0, nbatch = 0;
   for (int iter = 0; iter < n_iters; ++iter) {
-    // the next iteration's camera pass overlaps this one's grid and photon pass
-    if (iter + 1 < n_iters && (rc = camera_pass(iter + 1))) return rc;
+    // the next camera batch overlaps this batch's grids, deposits and photons
+    if (iter % Bc == 0 && iter + Bc < n_iters && (rc = camera_batch(iter + Bc, std::min(Bc, n_iters - iter - Bc))))
+      return rc;
     if (photons > 0 && iter >= batch0 + nbatch) {
       batch0 = iter;
       nbatch = std::min(B, n_iters - iter);
       if ((rc = photon_batch(batch0, nbatch))) return rc;
     }
-    HIP_TRY(hipStreamWaitEvent(st, b.cam_done[iter & 1], 0));
-    const Pix &px = b.px[iter & 1];
+    HIP_TRY(hipStreamWaitEvent(st, b.cam_done[(iter / Bc) & 1], 0));
+    const Pix &px = b.px[iter % nsets];
     // --- grid
     hipLaunchKernelGGL(k_grid_init, dim3(1), dim3(64), 0, st, g);
     hipLaunchKernelGGL(k_grid_bounds, dim3(std::min((npix + 255) / 256, 256)), dim3(256), 0, st, px, g, npix);
@@ -1169,7 +1270,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     }
     hipLaunchKernelGGL(k_pixel_update, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, px, npix, M_out);
     Pix pf = px;  // Ld as the camera pass left it
-    pf.Ldr = b.ld_snap[iter & 1];
+    pf.Ldr = b.ld_snap + (size_t)(iter % nsets) * 3 * npix;
     pf.Ldg = pf.Ldr + npix;
     pf.Ldb = pf.Ldg + npix;
     hipLaunchKernelGGL(k_sppm_film, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, pf, npix, nIterF, NpF,
@@ -1178,7 +1279,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
       HIP_TRY(hipMemcpyAsync(dbg->M, M_out, (size_t)npix * 4, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
     }
-    HIP_TRY(hipEventRecord(b.iter_done, st));
+    HIP_TRY(hipEventRecord(b.iter_end[iter % nsets], st));
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipEventRecord(c->ev1, st));

@@ -210,6 +210,26 @@ def test_c2_full_size_waves_equal(device):
     np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32))
 
 
+def test_c2_default_wave_equals_smaller_waves(device):
+    """C2 exactly (1.31M triangles, 1024x1024, 64 spp, depth 8): one default
+    wave of 2^26 paths -- the default configuration's largest traversal
+    grids (786,432 one-wave blocks per bounce) and spill area -- and four
+    16-spp waves give the same film, bit for bit, with no dropped stack
+    entries."""
+    hs = oa.HostScene.generate("icosphere", seed=1, params=[8, 1024, 1024])
+    ds = oa.DeviceScene(device, hs)
+    ds.film_clear()
+    st = ds.render_path(0, 64, 8)
+    a, ca = ds.film()
+    assert st.paths == 64 * 1024 * 1024 and st.stack_overflows == 0
+    ds.film_clear()
+    st2 = ds.render_path(0, 64, 8, spp_per_wave=16)
+    b, cb = ds.film()
+    assert st2.stack_overflows == 0
+    assert (ca == 64).all() and (cb == 64).all()
+    np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32))
+
+
 def _tie_scene(with_lower_duplicate=True, res=48):
     """Lambert plane under a 2 x 2 two-triangle quad light 0.3 above it
     (lights 0 and 1, one sided, facing down), plus a black copy of each light triangle at exactly

@@ -73,6 +73,15 @@ def test_sppm_c3_three_iterations(device, oracle):
     _compare(device, oracle, [6, 512, 512], 3, 1 << 20, 0.01, 1 << 22, 2)
 
 
+def test_sppm_batches_reuse_buffers(device, oracle):
+    """Twenty iterations at 128^2: the camera passes run in batches of 8
+    iterations (16 visible-point sets and Ld snapshots, so iteration 17
+    reuses iteration 1's set) and the photons in batches of 16 iterations
+    (two hit buffers); iteration 17's grid, deposits and M equal the
+    oracle's bit for bit and the film matches."""
+    _compare(device, oracle, [6, 128, 128], 20, 1 << 14, 0.02, 0, 17)
+
+
 def test_sppm_c3_sorted_pairs_and_cell_ranges(device, oracle):
     """The radix-sorted (hash key, pixel) pairs and every cell's [start, end)
     equal the reference's per-cell lists (sppm.cpp:170-193) concatenated by

@@ -27,8 +27,8 @@ void launch_trace_pair(const Job *jobs, const Wave &w, int ring, int chunk, bool
 int trace_pair_grid(int nseg, int seg, int chunk);
 int ring_variant(int ring);
 int trace_chunk(int chunk);
-// blocks of kTraceBlock threads covering nseg queue segments (closest
-// queues: 2 seg entries per segment; any-hit: seg)
+// blocks of kTraceBlock threads covering nseg queue segments (closest-hit
+// queues: seg entries per segment; any-hit: 2 seg)
 int trace_grid(int nseg, int seg, bool any, int chunk);
 // spill area (ints) for `blocks` traversal blocks; the per-thread capacity
 // is max(0, stack_bound - ring)

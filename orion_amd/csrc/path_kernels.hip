@@ -31,7 +31,7 @@ using namespace dev;
 // start every job at a segment boundary, so a segment belongs to one job and
 // its live paths are a prefix of it (the job's padding follows).
 template <bool MULTI>
-__global__ void k_raygen(const Job *jobs, Wave w) {
+__global__ void k_raygen(const Job *__restrict__ jobs, Wave w) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= w.nseg * w.seg) return;
   const Job &J = jobs[MULTI ? w.seg_job[p / w.seg] : 0];
@@ -185,8 +185,12 @@ __global__ void k_between(Wave w, int nxt) {
 // device atomic per queue per 256 paths.  Several blocks per segment keep
 // segments long -- the traversal's rays come from neighbouring pixels --
 // while the shading launch still has enough blocks to balance the GPU.
+// `jobs` is __restrict__ (nothing writes the job table during a launch): the
+// scene's pointers are then read through the scalar cache, where otherwise
+// every store to the wave's arrays forced a vector reload of each pointer
+// before its next dependent load (measured 8% faster shading).
 template <bool MULTI>
-__global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_shade(const Job *jobs, Wave w, int b) {
+__global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_shade(const Job *__restrict__ jobs, Wave w, int b) {
   __shared__ int lw[kShadeThreads / 64][3];  // per wave64: its appends, then their offsets in the block's range
   __shared__ int lbase[3];  // the block's ranges in the three queues
   const int cur = b & 1, nxt = cur ^ 1;
@@ -340,7 +344,7 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
 
 // RenderTarget::acumulate in sample order per pixel (rendertarget.cpp:22-27)
 template <bool MULTI>
-__global__ void k_film(const Job *jobs, Wave w) {
+__global__ void k_film(const Job *__restrict__ jobs, Wave w) {
   const int total = w.ctrl[C_NPIX];
   const int njobs = w.ctrl[C_NJOBS];
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {

@@ -229,6 +229,10 @@ typedef struct {
   int lds_ring;         /* traversal stack entries per lane kept in LDS: 8, 16, 32 (0 = 16) */
   int queue_segment;    /* paths per queue segment, a power of two in [512, 16384] (0 = by wave size) */
   int split_trace;      /* 1: closest-hit and any-hit queues in separate launches (0: one launch) */
+  int trace_plan;       /* plain renders' traversal grids: 0 = each bounce's grid sized from the longest
+                         * queue segments of the previous call on this scene with the same depth and
+                         * queue segment size (a longer segment is still traced in full), 1 = full grids,
+                         * 2 = one wave per queue segment at every launch (for tests) */
 } orion_tuning;
 
 /* PathTracing::render over samples [k0, k1): accumulates into the scene's film

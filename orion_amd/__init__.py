@@ -65,7 +65,7 @@ class RenderStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("spp_per_wave", C.c_int), ("wave_paths", C.c_int), ("rays_per_wave", C.c_int),
                 ("use_graph", C.c_int), ("profile", C.c_int), ("lds_ring", C.c_int),
-                ("queue_segment", C.c_int), ("split_trace", C.c_int)]
+                ("queue_segment", C.c_int), ("split_trace", C.c_int), ("trace_plan", C.c_int)]
 
 
 class SppmDebug(C.Structure):
@@ -396,18 +396,18 @@ class DeviceScene:
 
     def render_path(self, k0, k1, max_depth, spp_per_wave=0, rays_per_wave=0, profile=False,
                     use_graph=False, wave_paths=0, lds_ring=0, queue_segment=0, split_trace=False,
-                    tuning=None):
+                    tuning=None, trace_plan=0):
         if tuning is not None:
             spp_per_wave, wave_paths = tuning.spp_per_wave, tuning.wave_paths
             rays_per_wave, use_graph = tuning.rays_per_wave, bool(tuning.use_graph)
             lds_ring, queue_segment = tuning.lds_ring, tuning.queue_segment
-            split_trace = bool(tuning.split_trace)
+            split_trace, trace_plan = bool(tuning.split_trace), tuning.trace_plan
         st = RenderStats()
         tune = None  # NULL: the defaults
         if (spp_per_wave or wave_paths or rays_per_wave or use_graph or profile or lds_ring
-                or queue_segment or split_trace):
+                or queue_segment or split_trace or trace_plan):
             tune = C.byref(Tuning(spp_per_wave, wave_paths, rays_per_wave, int(use_graph),
-                                  int(profile), lds_ring, queue_segment, int(split_trace)))
+                                  int(profile), lds_ring, queue_segment, int(split_trace), trace_plan))
         _check(lib().orion_render_path(self.h, k0, k1, max_depth, tune, C.byref(st)))
         return st
 

@@ -88,9 +88,35 @@ struct GraphKey {
   }
 };
 
+// Traversal grids of the one-job path, planned from the previous call with
+// the same segment size, depth and chunk: the longest segment of each
+// bounce's closest-hit and any-hit queues there (the traversal records
+// them).  A bounce's launch then covers only the chunk positions such
+// segments need, with a margin; its waves take every cps-th chunk of their
+// segment, so a longer segment than planned is still traced in full.  Deep
+// bounces hold a few thousand paths in thousands of segments: a full grid
+// dispatches ~786k one-wave blocks for them.
+struct GridPlan {
+  const void *scene = nullptr;
+  int seg = 0, depth = -1, chunk = 0;
+  std::vector<int> longest[2];  // per bounce: closest-hit, any-hit
+  // chunk positions per segment for bounce b's queue q (0: every chunk of a
+  // full segment)
+  int cps(int b, int q, int seg_entries, int chunk) const {
+    if (b >= (int)longest[q].size()) return 0;
+    const int64_t need = (int64_t)longest[q][b] + longest[q][b] / 4 + 32;
+    const int64_t full = (seg_entries + chunk - 1) / chunk;
+    const int64_t c = (need + chunk - 1) / chunk;
+    return c >= full ? 0 : (int)std::max<int64_t>(1, c);
+  }
+};
+
 struct WaveRunner {
   WaveBuffers wb;
-  DevBuf jobs_dev, ctrl, stats, spill;
+  DevBuf jobs_dev, ctrl, stats, spill, segmax;
+  GridPlan plan;
+  bool planned = false;   // the current call's one-job waves use `plan`
+  bool one_wave = false;  // ... or one wave per queue segment (orion_tuning.trace_plan 2)
   int spill_cap = 0;
   Job *h_jobs = nullptr;
   int *h_ctrl = nullptr;
@@ -139,6 +165,7 @@ struct WaveRunner {
     if (rc) return rc;
     if ((rc = ctrl.ensure(64 * 4))) return rc;
     if ((rc = stats.ensure(kStatBytes))) return rc;
+    if ((rc = segmax.ensure(kSegMaxBytes))) return rc;
     HIP_TRY(hipMemset(ctrl.ptr, 0, 64 * 4));
     if (!staged) HIP_TRY(hipEventCreateWithFlags(&staged, hipEventDisableTiming));
     if (!st_any) {
@@ -182,12 +209,34 @@ struct WaveRunner {
     Wave w = wb.wave;
     w.ctrl = (int *)ctrl.ptr;
     w.stats = (unsigned long long *)stats.ptr;
+    w.segmax = (unsigned long long *)segmax.ptr;
     w.spill = (int *)spill.ptr;
     w.spill_cap = spill_cap;
     return w;
   }
   int reset_stats(hipStream_t st) {
     HIP_TRY(hipMemsetAsync(stats.ptr, 0, kStatBytes, st));
+    HIP_TRY(hipMemsetAsync(segmax.ptr, 0, kSegMaxBytes, st));
+    return ORION_OK;
+  }
+  // After a one-job call (stream synchronised): the longest segments its
+  // waves recorded become the plan for the next call of that shape.
+  int record_plan(const void *scene, int seg, int depth, int chunk) {
+    std::vector<unsigned long long> h((size_t)kSegMaxBounces * 2 * kStatStripes);
+    HIP_TRY(hipMemcpy(h.data(), segmax.ptr, kSegMaxBytes, hipMemcpyDeviceToHost));
+    const int nb = std::min(depth + 1, kSegMaxBounces);
+    for (int q = 0; q < 2; ++q) {
+      plan.longest[q].assign(nb, 0);
+      for (int b = 0; b < nb; ++b) {
+        unsigned long long m = 0;
+        for (int k = 0; k < kStatStripes; ++k) m = std::max(m, h[((size_t)b * 2 + q) * kStatStripes + k]);
+        plan.longest[q][b] = (int)std::min<unsigned long long>(m, 1u << 30);
+      }
+    }
+    plan.scene = scene;
+    plan.seg = seg;
+    plan.depth = depth;
+    plan.chunk = chunk;
     return ORION_OK;
   }
   int read_stats(orion_render_stats *o) {
@@ -217,26 +266,33 @@ struct WaveRunner {
   // launches in a row).  The any-hit launch takes the spill area's second
   // part.  `split` runs the two launches in a row on one stream.
   void enqueue_wave(const Job *dj, const Wave &w, int depth, int ring, bool multi, int chunk, bool split,
-                    hipStream_t st, LaunchTimer *tm) {
+                    hipStream_t st, LaunchTimer *tm, bool use_plan) {
     launch_raygen(dj, w, multi, st);
     Wave wa_base = w;
     wa_base.spill = w.spill + (size_t)w.spill_cap * trace_grid(w.nseg, w.seg, false, chunk) * kTraceBlock;
+    use_plan = use_plan && !multi && plan.seg == w.seg && plan.depth == depth && plan.chunk == chunk;
     for (int it = 0; it <= depth; ++it) {
       Wave wi = w;  // bounce it traces the ray lengths of counter set it % 3
       wi.cnt_ray = w.cnt + (3 * (it % 3) + 1) * w.cnt_stride;
       wi.cnt_shadow = w.cnt + (3 * (it % 3) + 2) * w.cnt_stride;
+      // one-job waves record each bounce's longest segments for later plans
+      const int rec_c = !multi && it < kSegMaxBounces ? 2 * it : -1;
+      const int rec_a = rec_c >= 0 ? rec_c + 1 : -1;
+      int cps_c = use_plan ? plan.cps(it, 0, w.seg, trace_chunk(chunk)) : 0;
+      int cps_a = use_plan ? plan.cps(it, 1, 2 * w.seg, trace_chunk(chunk)) : 0;
+      if (one_wave && !multi) cps_c = cps_a = 1;
       if (it == 0) {
         // camera rays only: no any-hit rays exist before the first shading
         // (an empty any-hit launch of the full grid had cost 1 ms beside
         // this one at C2)
         if (tm) tm->mark(split ? 0 : 3);
-        launch_trace(dj, wi, false, ring, chunk, multi, st);
+        launch_trace(dj, wi, false, ring, chunk, multi, st, cps_c, rec_c);
         if (tm) tm->mark(-1);
       } else if (split) {
         if (tm) tm->mark(0);
-        launch_trace(dj, wi, false, ring, chunk, multi, st);
+        launch_trace(dj, wi, false, ring, chunk, multi, st, cps_c, rec_c);
         if (tm) { tm->mark(-1); tm->mark(1); }
-        launch_trace(dj, wi, true, ring, chunk, multi, st);
+        launch_trace(dj, wi, true, ring, chunk, multi, st, cps_a, rec_a);
         if (tm) tm->mark(-1);
       } else {
         if (tm) tm->mark(3);
@@ -244,8 +300,8 @@ struct WaveRunner {
         wa.spill = wa_base.spill;
         (void)hipEventRecord(ev_fork, st);
         (void)hipStreamWaitEvent(st_any, ev_fork, 0);
-        launch_trace(dj, wi, false, ring, chunk, multi, st);
-        launch_trace(dj, wa, true, ring, chunk, multi, st_any);
+        launch_trace(dj, wi, false, ring, chunk, multi, st, cps_c, rec_c);
+        launch_trace(dj, wa, true, ring, chunk, multi, st_any, cps_a, rec_a);
         (void)hipEventRecord(ev_join, st_any);
         (void)hipStreamWaitEvent(st, ev_join, 0);
         if (tm) tm->mark(-1);
@@ -314,7 +370,7 @@ struct WaveRunner {
     w.seg_job = (const int *)segjob_dev.ptr;
     w.seg_perm = (const int *)segjob_dev.ptr + nseg;
     if (!use_graph || tm) {
-      enqueue_wave(dj, w, depth, ring, multi, chunk, split, st, tm);
+      enqueue_wave(dj, w, depth, ring, multi, chunk, split, st, tm, planned);
       HIP_TRY(hipGetLastError());
       return ORION_OK;
     }
@@ -325,7 +381,7 @@ struct WaveRunner {
     if (!exec) {
       hipGraph_t graph;
       HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      enqueue_wave(dj, w, depth, ring, multi, chunk, split, st, nullptr);
+      enqueue_wave(dj, w, depth, ring, multi, chunk, split, st, nullptr, false);
       HIP_TRY(hipStreamEndCapture(st, &graph));
       hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
@@ -341,6 +397,9 @@ struct WaveRunner {
 };
 
 void destroy_runner(WaveRunner *r) { delete r; }
+void forget_scene(WaveRunner *r, const void *scene) {
+  if (r->plan.scene == scene) r->plan = GridPlan();
+}
 
 // A tuning is checked before anything else: ray slot s of path p is the
 // 32-bit index s * N + p (s < 3), so no wave may exceed kMaxWavePathsLimit.
@@ -348,6 +407,10 @@ static int check_tuning(const orion_tuning *t) {
   if (!t) return ORION_OK;
   if (t->wave_paths < 0 || t->spp_per_wave < 0 || t->rays_per_wave < 0 || t->lds_ring < 0 || t->queue_segment < 0) {
     set_error("negative tuning value");
+    return ORION_ERR_ARG;
+  }
+  if (t->trace_plan < 0 || t->trace_plan > 2) {
+    set_error("trace_plan must be 0, 1 or 2");
     return ORION_ERR_ARG;
   }
   if (t->wave_paths > kMaxWavePathsLimit || t->spp_per_wave > kMaxWavePathsLimit) {
@@ -427,6 +490,9 @@ extern "C" int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, co
   if (rc) return rc;
   if ((rc = R.reset_stats(c->stream))) return rc;
   const bool graph = tune && tune->use_graph;
+  const int mode = tune ? tune->trace_plan : 0;
+  R.planned = mode == 0 && R.plan.scene == (const void *)s;
+  R.one_wave = mode == 2;
   LaunchTimer tm;
   tm.on = tune && tune->profile;
   tm.st = c->stream;
@@ -436,12 +502,14 @@ extern "C" int orion_render_path(orion_scene s, int k0, int k1, int maxDepth, co
     Job J = make_job(s, s->W, s->H, maxDepth, (float *)s->film.ptr, (int *)s->count.ptr);
     J.S = Sk;
     J.k0 = (uint32_t)k;
-    if ((rc = R.run(c->stream, &J, 1, Sk * npix, npix, maxDepth, ring, s->d.stack_bound, false, chunk,
-                    split_of(tune), graph, tm.on ? &tm : nullptr, seg_of(tune, Sk * npix))))
-      return rc;
+    rc = R.run(c->stream, &J, 1, Sk * npix, npix, maxDepth, ring, s->d.stack_bound, false, chunk, split_of(tune),
+               graph, tm.on ? &tm : nullptr, seg_of(tune, Sk * npix));
+    if (rc) { R.planned = R.one_wave = false; return rc; }
   }
+  R.planned = R.one_wave = false;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  if ((rc = R.record_plan(s, seg_of(tune, S * npix), maxDepth, chunk))) return rc;
   float ms = 0;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   if (stats) {

@@ -16,9 +16,14 @@ constexpr int kShadeThreads = 256;        // threads per path-shading block
 // per-path job lookup (false: every path belongs to job 0).
 void launch_raygen(const Job *jobs, const Wave &w, bool multi, hipStream_t st);
 // ring: LDS ring entries per lane (8, 16 or 32); chunk: queue entries per
-// wave64.  w.spill must hold w.spill_cap entries for each of
-// trace_grid(w.nseg, w.seg, any, chunk) * kTraceBlock threads.
-void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk, bool multi, hipStream_t st);
+// wave64.  w.spill must hold w.spill_cap entries for each of the launch's
+// w.nseg * cps * kTraceBlock threads (at most trace_grid(w.nseg, w.seg,
+// any, chunk) blocks).  cps: chunk positions per segment, each wave taking
+// every cps-th chunk of its segment (<= 0: all of a full segment's chunks,
+// one wave per chunk).  rec >= 0: record the bounce's longest segment in
+// w.segmax slot rec.
+void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk, bool multi, hipStream_t st,
+                  int cps = 0, int rec = -1);
 // an SPPM photon bounce (closest hit, trace_grid(w.nseg, w.seg, false,
 // chunk) blocks; counted apart from camera and path rays)
 void launch_trace_photons(const Job *jobs, const Wave &w, int ring, int chunk, hipStream_t st);

@@ -72,13 +72,18 @@ __global__ void k_raygen(const Job *__restrict__ jobs, Wave w) {
 // Wave queues as traversal-engine IO.  Closest rays: path ids, ray in
 // slot 0.  Any-hit entries r < N: shadow ray of path r (slot 2); N + p: MIS
 // visibility ray of path p (slot 1, its tie primitive in hprim[N + p]).
+// The engine's entry q is the segment's entry (c + (q / chunk) cps) chunk +
+// q mod chunk: the wave takes the segment's chunks c, c + cps, c + 2 cps ...
 template <bool ANY>
 struct WaveRays {
   const Wave *w;
   int base;                   // first queue entry of the segment
+  int c, cps, chunk;          // the wave's first chunk, chunk stride, chunk size
   const float4 *nodes, *tris;  // the segment's job's scene (wave-uniform)
   __device__ void load(int q, TRay *ray) const {
-    const int r = (ANY ? w->qshadow : w->qray)[base + q];
+    const int k = q / chunk;
+    const int e = (c + k * cps) * chunk + (q - k * chunk);
+    const int r = (ANY ? w->qshadow : w->qray)[base + e];
     const int slot = ANY && r < w->N ? 2 * w->N + r : r;
     ray->o = mk(w->ox[slot], w->oy[slot], w->oz[slot]);
     ray->d = mk(w->dx[slot], w->dy[slot], w->dz[slot]);
@@ -100,25 +105,36 @@ struct WaveRays {
   }
 };
 
-// Wave64 number g of a launch part traces entries [c * chunk, (c + 1) *
-// chunk) of queue segment g / (waves per segment), c = g mod (waves per
-// segment); the part covers every segment at full length and waves past a
-// segment's length exit.  `lb` is the block's logical index within its part
-// (XCD-contiguous), `ring` the block's LDS.  PHOTON: an SPPM photon bounce
-// (closest hit; its traversal work has its own counters).
+// A launch part covers `cps` chunk positions of every queue segment: wave64
+// number g (its block's logical index within the part, XCD-contiguous)
+// takes segment g / cps, chunks c, c + cps, c + 2 cps ... (c = g mod cps) of
+// `chunk` entries, up to the segment's length.  cps = the chunks of a full
+// segment gives each wave at most one chunk; a smaller one (a wave runner
+// that knows a bounce's segments are short) saves dispatching waves that
+// would find nothing, and still traces every entry.  rec >= 0: the wave of
+// chunk position 0 records the segment's length as the bounce's longest
+// (w.segmax slot rec).  `ring` is the block's LDS.  PHOTON: an SPPM photon
+// bounce (closest hit; its traversal work has its own counters).
 template <bool ANY, int RING, bool MULTI, bool PHOTON = false>
-__device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int chunk, int lb, int *ring) {
+__device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int chunk, int cps, int rec, int lb,
+                                            int *ring) {
+  static_assert(kTraceBlock == 64, "one wave64 per traversal block");
   const int seg_entries = ANY ? 2 * w.seg : w.seg;  // the queue's stride per segment
-  const int wps = (seg_entries + chunk - 1) / chunk;
-  // the wave's index is uniform: read from lane 0 so that everything derived
+  if (cps <= 0) cps = (seg_entries + chunk - 1) / chunk;
+  // the wave's index is uniform (one wave per block), so everything derived
   // from it (segment, job, scene pointers) stays in scalar registers
-  const int gw = lb * (kTraceBlock / 64) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int ls = gw / wps;
+  const int ls = lb / cps;
   if (ls >= w.nseg) return;
+  const int c = lb - ls * cps;
   const int sg = MULTI ? w.seg_perm[ls] : ls;
   const int count = (ANY ? w.cnt_shadow : w.cnt_ray)[sg];
-  const int begin = (gw - ls * wps) * chunk;
-  if (begin >= count) return;
+  if (rec >= 0 && c == 0 && count > 0 && w.segmax && __lane_id() == 0)
+    atomicMax(&w.segmax[(size_t)rec * kStatStripes + (blockIdx.x & (kStatStripes - 1))], (unsigned long long)count);
+  const int nch = (count + chunk - 1) / chunk;  // the segment's chunks
+  if (c >= nch) return;
+  const int m = (nch - 1 - c) / cps + 1;        // the wave's chunks
+  const int last = c + (m - 1) * cps;
+  const int n = (m - 1) * chunk + (last == nch - 1 ? count - last * chunk : chunk);
   TravStack<RING> stk;
   stk.ring = ring + threadIdx.x;
   stk.spill_stride = gridDim.x * kTraceBlock;
@@ -126,8 +142,8 @@ __device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int 
   stk.spill_cap = w.spill_cap;
   uint32_t ovf = 0, nv = 0, nt = 0;
   const DScene &sc = jobs[MULTI ? w.seg_job[sg] : 0].sc;
-  WaveRays<ANY> io{&w, sg * seg_entries, sc.nodes, sc.tris};
-  trace_engine<ANY, RING>(io, begin, min(begin + chunk, count), stk, &nv, &nt, &ovf);
+  WaveRays<ANY> io{&w, sg * seg_entries, c, cps, chunk, sc.nodes, sc.tris};
+  trace_engine<ANY, RING>(io, 0, n, stk, &nv, &nt, &ovf);
   const int lane = (int)__lane_id();
   // nv / nt are the wave's totals (for the roofline's n_node / n_tri)
   if (ovf) stat_add(w.stats, S_OVERFLOW, ovf);
@@ -138,9 +154,9 @@ __device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int 
 }
 
 template <bool ANY, int RING, bool MULTI>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w, int chunk) {
+__global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w, int chunk, int cps, int rec) {
   __shared__ int ring[RING * kTraceBlock];
-  trace_block<ANY, RING, MULTI>(jobs, w, chunk, xcd_remap(blockIdx.x, gridDim.x), ring);
+  trace_block<ANY, RING, MULTI>(jobs, w, chunk, cps, rec, xcd_remap(blockIdx.x, gridDim.x), ring);
 }
 
 // An SPPM photon bounce's closest-hit traversal: the same engine, its own
@@ -148,7 +164,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w, 
 template <int RING>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_photons(const Job *jobs, Wave w, int chunk) {
   __shared__ int ring[RING * kTraceBlock];
-  trace_block<false, RING, false, true>(jobs, w, chunk, xcd_remap(blockIdx.x, gridDim.x), ring);
+  trace_block<false, RING, false, true>(jobs, w, chunk, 0, -1, xcd_remap(blockIdx.x, gridDim.x), ring);
 }
 
 // Closest-hit and any-hit queues of a bounce in one launch: blocks
@@ -160,8 +176,8 @@ template <int RING, bool MULTI>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_pair(const Job *jobs, Wave w, int chunk, int n_closest) {
   __shared__ int ring[RING * kTraceBlock];
   const int b = blockIdx.x;
-  if (b < n_closest) trace_block<false, RING, MULTI>(jobs, w, chunk, xcd_remap(b, n_closest), ring);
-  else trace_block<true, RING, MULTI>(jobs, w, chunk, xcd_remap(b - n_closest, gridDim.x - n_closest), ring);
+  if (b < n_closest) trace_block<false, RING, MULTI>(jobs, w, chunk, 0, -1, xcd_remap(b, n_closest), ring);
+  else trace_block<true, RING, MULTI>(jobs, w, chunk, 0, -1, xcd_remap(b - n_closest, gridDim.x - n_closest), ring);
 }
 
 __global__ void k_between(Wave w, int nxt) {
@@ -503,16 +519,19 @@ int trace_pair_grid(int nseg, int seg, int chunk) {
   return (trace_grid(nseg, seg, false, chunk) + kXcds - 1) / kXcds * kXcds + trace_grid(nseg, seg, true, chunk);
 }
 
-void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk, bool multi, hipStream_t st) {
+void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk, bool multi, hipStream_t st, int cps,
+                  int rec) {
   ring = ring_variant(ring);
   chunk = trace_chunk(chunk);
-  const dim3 g(trace_grid(w.nseg, w.seg, any, chunk));
+  const int full = (int)(((any ? 2 * (int64_t)w.seg : w.seg) + chunk - 1) / chunk);
+  if (cps <= 0 || cps > full) cps = full;
+  const dim3 g((unsigned)std::max(1, w.nseg * cps));
   if (multi) {
-    if (any) ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<true, ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
-    else ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<false, ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
+    if (any) ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<true, ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk, cps, rec));
+    else ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<false, ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk, cps, rec));
   } else {
-    if (any) ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<true, ST, false>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
-    else ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<false, ST, false>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
+    if (any) ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<true, ST, false>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk, cps, rec));
+    else ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<false, ST, false>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk, cps, rec));
   }
 }
 void launch_trace_photons(const Job *jobs, const Wave &w, int ring, int chunk, hipStream_t st) {

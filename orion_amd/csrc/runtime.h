@@ -35,6 +35,8 @@ struct SppmBuffers;
 void destroy_sppm(SppmBuffers *b);
 struct WaveRunner;
 void destroy_runner(WaveRunner *r);
+// drops the runner's traversal-grid plan when it was made for `scene`
+void forget_scene(WaveRunner *r, const void *scene);
 // sums the striped device statistics (kStatBytes at dev) into out[S_COUNT]
 int read_striped_stats(const void *dev, unsigned long long *out);
 // ORION_ERR_UNSUPPORTED when a traversal dropped stack entries (a tree

@@ -362,6 +362,7 @@ int orion_scene_destroy(orion_scene s) {
   if (!s) return ORION_OK;
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
+  if (s->ctx->runner) forget_scene(s->ctx->runner, s);
   delete s;
   return ORION_OK;
 }

@@ -42,6 +42,8 @@ enum : int {
 };
 constexpr int kStatStripes = 64;
 constexpr size_t kStatBytes = (size_t)S_COUNT * kStatStripes * 8;
+constexpr int kSegMaxBounces = 64;  // bounces whose longest segment is recorded (maxDepth is at most 60)
+constexpr size_t kSegMaxBytes = (size_t)kSegMaxBounces * 2 * kStatStripes * 8;
 
 // Queue segments: path ids [s * seg, (s + 1) * seg) form segment s.  Each
 // segment has its own path, closest-ray (seg entries) and any-hit (2 seg
@@ -106,6 +108,10 @@ struct Wave {
   int *qshadow;   // p: shadow ray (slot 2); N + p: MIS visibility ray (slot 1)
   int *ctrl;
   unsigned long long *stats;
+  // Longest queue segment of each bounce (kSegMaxBounces x {closest-hit,
+  // any-hit}, striped like the statistics), recorded by the traversal when
+  // non-null: the wave runner sizes later waves' traversal grids from it.
+  unsigned long long *segmax;
   // traversal stack spill area: spill_cap entries per traversal thread
   int *spill;
   int spill_cap;

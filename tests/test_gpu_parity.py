@@ -230,6 +230,28 @@ def test_c2_default_wave_equals_smaller_waves(device):
     np.testing.assert_array_equal(a.view(np.int32), b.view(np.int32))
 
 
+def test_trace_plan_modes_equal(device):
+    """Traversal grid plans change which wave traces which queue chunk, never
+    a result: full grids (trace_plan 1), grids sized from the previous call's
+    longest segments (0, the default: the second call on the scene uses the
+    first one's plan), and one wave per queue segment taking all of its
+    segment's chunks (2) give the same film bit for bit and the same ray
+    counts, over 2^20-path waves in 2048-path segments."""
+    hs = oa.HostScene.generate("icosphere", seed=3, params=[6, 256, 256])
+    ds = oa.DeviceScene(device, hs)
+    films, rays = [], []
+    for mode in (1, 0, 0, 2):
+        ds.film_clear()
+        st = ds.render_path(0, 32, 8, spp_per_wave=16, queue_segment=2048, trace_plan=mode)
+        assert st.stack_overflows == 0 and st.paths == 32 * 256 * 256
+        films.append(ds.film())
+        rays.append((st.closest_rays, st.shadow_rays, st.node_visits_closest, st.node_visits_shadow))
+    for (f, c), r in zip(films[1:], rays[1:]):
+        np.testing.assert_array_equal(f.view(np.int32), films[0][0].view(np.int32))
+        np.testing.assert_array_equal(c, films[0][1])
+        assert r == rays[0]
+
+
 def _tie_scene(with_lower_duplicate=True, res=48):
     """Lambert plane under a 2 x 2 two-triangle quad light 0.3 above it
     (lights 0 and 1, one sided, facing down), plus a black copy of each light triangle at exactly

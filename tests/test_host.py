@@ -236,6 +236,8 @@ def test_wave_size_bound_rejected_before_any_device_call():
     assert L.orion_server_create(None, 1 << 30, ctypes.byref(big), ctypes.byref(srv)) == -6
     assert L.orion_render_path(None, 0, 1, 5, ctypes.byref(oa.Tuning(spp_per_wave=(1 << 28) + 1)), None) == -6
     assert L.orion_render_path(None, 0, 1, 5, ctypes.byref(oa.Tuning(wave_paths=-1)), None) == -1
+    assert L.orion_render_path(None, 0, 1, 5, ctypes.byref(oa.Tuning(trace_plan=3)), None) == -1
+    assert b"trace_plan" in L.orion_last_error()
     # the bound itself is accepted by the check (the null scene is then the error)
     ok = oa.Tuning(wave_paths=1 << 28)
     assert L.orion_render_path(None, 0, 1, 5, ctypes.byref(ok), None) == -1

@@ -293,10 +293,10 @@ int orion_render_sppm(orion_scene s, int n_iters, int photons_per_iter, float in
  * segments each belong to one job, and every bounce's traversal and shading
  * launches serve all of the wave's jobs at once.  A job's camera is its
  * scene's view with the job's film size (aspect = width / height).
- * HBM: a wave of N paths holds 208 B of state per path plus the traversal
+ * HBM: a wave of N paths holds 220 B of state per path plus the traversal
  * spill area (60 B per path for the 1.3M-triangle scene: 20 spilled stack
  * entries per traversal lane), so the default 2^26-path wave needs about
- * 18 GB; a wave that does not fit is halved until it does. */
+ * 19 GB; a wave that does not fit is halved until it does. */
 typedef struct {
   orion_scene scene;   /* resident scene (BVH, materials, lights, view) */
   int width, height;   /* film of this job */

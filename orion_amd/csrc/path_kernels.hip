@@ -55,14 +55,9 @@ __global__ void k_raygen(const Job *__restrict__ jobs, Wave w) {
   uint32_t index = (J.k0 + (uint32_t)s) * (uint32_t)J.npix + (uint32_t)pix;  // path.cpp:26
   V3 o, d;
   camera_ray(J.sc, i, j, index, &o, &d);
-  w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
-  w.dx[p] = d.x; w.dy[p] = d.y; w.dz[p] = d.z;
-  w.tmax[p] = kFltMax;
-  w.br[p] = 1.f; w.bg[p] = 1.f; w.bb[p] = 1.f;
-  w.Lr[p] = 0.f; w.Lg[p] = 0.f; w.Lb[p] = 0.f;
-  w.dim[p] = 7;
-  w.bounce[p] = 0;
-  w.flags[p] = F_ALIVE;
+  put_ray(w, p, o, d, kFltMax);
+  w.st[p] = make_float4(1.f, 1.f, 1.f, path_bits(F_ALIVE, 0, 7));
+  w.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
   const int sg = p / w.seg, j0 = p - sg * w.seg;
   w.qpath[0][p] = p;
   w.qray[sg * w.seg + j0] = p;
@@ -85,23 +80,18 @@ struct WaveRays {
     const int e = (c + k * cps) * chunk + (q - k * chunk);
     const int r = (ANY ? w->qshadow : w->qray)[base + e];
     const int slot = ANY && r < w->N ? 2 * w->N + r : r;
-    ray->o = mk(w->ox[slot], w->oy[slot], w->oz[slot]);
-    ray->d = mk(w->dx[slot], w->dy[slot], w->dz[slot]);
+    const float4 a = w->ro[slot], b = w->rd[slot];
+    ray->o = mk(a.x, a.y, a.z);
+    ray->d = mk(b.x, b.y, b.z);
     ray->tmin = 0.0f;
-    ray->tmax = w->tmax[slot];
+    ray->tmax = a.w;
     ray->tag = r;
-    ray->tie = ANY && r >= w->N ? w->hprim[r] : kTieAll;
+    ray->tie = ANY ? __float_as_int(b.w) : kTieAll;
   }
   __device__ void store(const TRay &ray, bool hit, const HitRec &h) const {
     const int r = ray.tag;
-    if (ANY) {
-      w->vis[r] = hit ? 0 : 1;
-    } else {
-      w->hprim[r] = hit ? h.prim : -1;
-      w->ht[r] = hit ? h.t : 0.f;
-      w->hu[r] = hit ? h.u : 0.f;
-      w->hv[r] = hit ? h.v : 0.f;
-    }
+    if (ANY) w->vis[r] = hit ? 0 : 1;
+    else w->hit[r] = hit ? make_float4(h.t, h.u, h.v, __int_as_float(h.prim)) : make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
   }
 };
 
@@ -234,8 +224,11 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
     bool want_ray0 = false, want_ray1 = false, want_shadow = false, keep = false, done = false;
     int ref_rays = 0;  // closest-hit casts orion makes for the MIS sample
     if (act) {
-      int fl = w.flags[p];
-      Sp L = sp3(w.Lr[p], w.Lg[p], w.Lb[p]);
+      const float4 S = w.st[p];
+      int fl = bits_flags(S), b = bits_bounce(S), dim = bits_dim(S);
+      Sp beta = sp3(S.x, S.y, S.z);
+      const float4 R = w.rad[p];
+      Sp L = sp3(R.x, R.y, R.z);
       // 1) resolve the direct-lighting estimate of the previous bounce
       if (fl & F_PEND) {
         resolve_direct(w, p, fl, &L);
@@ -243,12 +236,12 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
       }
       // 2) the hit of this bounce
       if (fl & F_ALIVE) {
-        int b = w.bounce[p];
-        int hp = w.hprim[p];
-        Sp beta = sp3(w.br[p], w.bg[p], w.bb[p]);
-        V3 d = mk(w.dx[p], w.dy[p], w.dz[p]);
+        const float4 H = w.hit[p];
+        const int hp = __float_as_int(H.w);
+        const float4 D = w.rd[p];
+        V3 d = mk(D.x, D.y, D.z);
         Isect is;
-        if (hp >= 0) make_isect(sc, hp, w.hu[p], w.hv[p], &is);
+        if (hp >= 0) make_isect(sc, hp, H.y, H.z, &is);
         if ((b == 0 || (fl & F_SPEC)) && hp >= 0 && is.light >= 0) acc(L, beta * area_emission(sc, is, neg(d)));
         if ((b == 0 || (fl & F_SPEC)) && hp < 0) add_env_le(sc, d, beta, &L);  // path.cpp:65-68
         if (hp < 0 || b >= J.maxDepth) {
@@ -259,7 +252,6 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
           int q = p - J.path0;
           int s = q / J.npix;
           uint32_t index = (J.k0 + (uint32_t)s) * (uint32_t)J.npix + (uint32_t)(q - s * J.npix);
-          int dim = w.dim[p];
           if (bsdf_num(bs, ALL & ~SPEC) > 0) {
             int nf = setup_direct(sc, w, p, bs, is, d, beta, index, &dim, false, &ref_rays);
             fl |= nf;
@@ -285,21 +277,17 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
               else beta = beta / (1 - qq);
             }
             if (alive) {
-              w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
-              w.dx[p] = wi.x; w.dy[p] = wi.y; w.dz[p] = wi.z;
-              w.tmax[p] = kFltMax;
-              w.bounce[p] = b + 1;
+              put_ray(w, p, o, wi, kFltMax);
+              ++b;
               want_ray0 = true;
             } else {
               fl &= ~F_ALIVE;
             }
           }
-          w.br[p] = beta.r; w.bg[p] = beta.g; w.bb[p] = beta.b;
-          w.dim[p] = dim;
         }
       }
-      w.Lr[p] = L.r; w.Lg[p] = L.g; w.Lb[p] = L.b;
-      w.flags[p] = fl;
+      w.rad[p] = make_float4(L.r, L.g, L.b, 0.f);
+      w.st[p] = make_float4(beta.r, beta.g, beta.b, path_bits(fl, b, dim));
       keep = (fl & (F_ALIVE | F_PEND)) != 0;
       done = !keep;
     }
@@ -379,9 +367,10 @@ __global__ void k_film(const Job *__restrict__ jobs, Wave w) {
     float r = J.film[3 * pix], gg = J.film[3 * pix + 1], b = J.film[3 * pix + 2];
     for (int s = 0; s < J.S; ++s) {
       int p = J.path0 + s * J.npix + pix;
-      r += w.Lr[p];
-      gg += w.Lg[p];
-      b += w.Lb[p];
+      const float4 R = w.rad[p];
+      r += R.x;
+      gg += R.y;
+      b += R.z;
     }
     J.film[3 * pix] = r;
     J.film[3 * pix + 1] = gg;

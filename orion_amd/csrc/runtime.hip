@@ -80,36 +80,35 @@ int DevBuf::ensure(size_t b) {
 // ---------------------------------------------------------------- wave buffers
 int WaveBuffers::ensure(int N) {
   if (N <= cap && mem.ptr) return ORION_OK;
-  // floats/ints per path: rays 7 x 3 slots, hits 3 + 2 (hprim: slot 0 hits
-  // and MIS ties), vis 2, state 9, pending 10, queues 2 + 1 + 2
-  const size_t per = 21 + 5 + 2 + 9 + 10 + 5;
+  // bytes per path: rays 2 x 16 x 3 slots, hit 16, vis 2 x 4, state 16 + 16,
+  // pending 3 x 16, queues 4 x (2 + 1 + 2); the float4 arrays first (16-byte
+  // aligned), then the int arrays
+  const size_t per = 96 + 16 + 8 + 32 + 48 + 20;
   const size_t nseg = ((size_t)N + kSegMin - 1) / kSegMin + 1;
-  int rc = mem.ensure(per * (size_t)N * 4 + 9 * nseg * 4 + 256);
+  int rc = mem.ensure(per * (size_t)N + 9 * nseg * 4 + 256);
   if (rc) return rc;
   cap = N;
   char *b = (char *)mem.ptr;
   size_t off = 0;
-  auto take = [&](size_t n) { void *q = b + off; off += n * 4; return q; };
+  auto take = [&](size_t bytes) { void *q = b + off; off += bytes; return q; };
+  auto take4 = [&](size_t n) { return (float4 *)take(n * sizeof(float4)); };
   Wave &w = wave;
-  w.ox = (float *)take(3 * (size_t)N); w.oy = (float *)take(3 * (size_t)N); w.oz = (float *)take(3 * (size_t)N);
-  w.dx = (float *)take(3 * (size_t)N); w.dy = (float *)take(3 * (size_t)N); w.dz = (float *)take(3 * (size_t)N);
-  w.tmax = (float *)take(3 * (size_t)N);
-  w.ht = (float *)take(N); w.hu = (float *)take(N); w.hv = (float *)take(N);
-  w.hprim = (int *)take(2 * (size_t)N);
-  w.vis = (int *)take(2 * (size_t)N);
-  w.br = (float *)take(N); w.bg = (float *)take(N); w.bb = (float *)take(N);
-  w.Lr = (float *)take(N); w.Lg = (float *)take(N); w.Lb = (float *)take(N);
-  w.dim = (int *)take(N); w.bounce = (int *)take(N); w.flags = (int *)take(N);
-  w.Ar = (float *)take(N); w.Ag = (float *)take(N); w.Ab = (float *)take(N);
-  w.Mr = (float *)take(N); w.Mg = (float *)take(N); w.Mb = (float *)take(N);
-  w.sel = (float *)take(N);
-  w.nbr = (float *)take(N); w.nbg = (float *)take(N); w.nbb = (float *)take(N);
+  w.ro = take4(3 * (size_t)N);
+  w.rd = take4(3 * (size_t)N);
+  w.hit = take4(N);
+  w.st = take4(N);
+  w.rad = take4(N);
+  w.pa = take4(N);
+  w.pm = take4(N);
+  w.pn = take4(N);
+  auto takei = [&](size_t n) { return (int *)take(n * 4); };
+  w.vis = takei(2 * (size_t)N);
   w.N = N;  // slot stride
-  w.qpath[0] = (int *)take(N); w.qpath[1] = (int *)take(N);
-  w.qray = (int *)take(N);
-  w.qshadow = (int *)take(2 * (size_t)N);
+  w.qpath[0] = takei(N); w.qpath[1] = takei(N);
+  w.qray = takei(N);
+  w.qshadow = takei(2 * (size_t)N);
   choose_segments(N, &w.seg, &w.nseg);
-  w.cnt = (int *)take(9 * nseg);
+  w.cnt = takei(9 * nseg);
   w.cnt_stride = (int)nseg;
   w.cnt_ray = w.cnt + nseg;
   w.cnt_shadow = w.cnt + 2 * nseg;

@@ -294,9 +294,14 @@ __device__ inline Sp light_sample_le(const DScene &sc, int l, float a0, float a1
 // visibility ray was), as the radiance term beta * (Ld / lightPdf).
 __device__ inline Sp direct_term(const Wave &w, int p, int fl) {
   Sp Ld = sp(0);
-  if ((fl & F_SHADOW) && w.vis[p]) acc(Ld, sp3(w.Ar[p], w.Ag[p], w.Ab[p]));
-  if ((fl & F_MIS) && w.vis[w.N + p]) acc(Ld, sp3(w.Mr[p], w.Mg[p], w.Mb[p]));
-  return sp3(w.nbr[p], w.nbg[p], w.nbb[p]) * (Ld / w.sel[p]);
+  const float4 A = w.pa[p];
+  if ((fl & F_SHADOW) && w.vis[p]) acc(Ld, sp3(A.x, A.y, A.z));
+  if ((fl & F_MIS) && w.vis[w.N + p]) {
+    const float4 M = w.pm[p];
+    acc(Ld, sp3(M.x, M.y, M.z));
+  }
+  const float4 nb = w.pn[p];
+  return sp3(nb.x, nb.y, nb.z) * (Ld / A.w);
 }
 __device__ inline void resolve_direct(const Wave &w, int p, int fl, Sp *L) { acc(*L, direct_term(w, p, fl)); }
 
@@ -355,18 +360,14 @@ __device__ inline int nee_sample(const DScene &sc, const Wave &w, int p, int l, 
   const int ltype = (int)gld(LR);
   bool delta = ltype != LT_AREA && ltype != LT_ENVMAP;  // isDeltaLight (light.h:39-42)
   bool hs = false, hm = false;
+  Sp A = sp(0), M = sp(0);
   if (lpdf > 0 && !black(Li)) {
     Sp f = bsdf_f(bs, wi, wo, nonspec) * absdot(wi, is.ns);
     float bpdf = bsdf_pdf(bs, wi, wo, nonspec);
     if (!black(f)) {
-      Sp A;
       if (delta) A = f * Li / lpdf;
       else A = f * Li * power_heuristic(lpdf, bpdf) / lpdf;
-      w.Ar[p] = A.r; w.Ag[p] = A.g; w.Ab[p] = A.b;
-      int s2 = 2 * N + p;
-      w.ox[s2] = so.x; w.oy[s2] = so.y; w.oz[s2] = so.z;
-      w.dx[s2] = sdir.x; w.dy[s2] = sdir.y; w.dz[s2] = sdir.z;
-      w.tmax[s2] = 1 - kShadowEps;
+      put_ray(w, 2 * (size_t)N + p, so, sdir, 1 - kShadowEps);
       hs = true;
     }
   }
@@ -382,7 +383,6 @@ __device__ inline int nee_sample(const DScene &sc, const Wave &w, int p, int l, 
     if (traced) {
       ++*ref_rays;
       V3 o2 = offset_ray_origin(is.p, is.ng, wi2);
-      Sp M = sp(0);
       float tvis = kFltMax;
       int tie = kTieAll;
       if (ltype == LT_ENVMAP) {
@@ -408,19 +408,15 @@ __device__ inline int nee_sample(const DScene &sc, const Wave &w, int p, int l, 
         }
       }
       if (!black(M)) {
-        int s1 = N + p;
-        w.ox[s1] = o2.x; w.oy[s1] = o2.y; w.oz[s1] = o2.z;
-        w.dx[s1] = wi2.x; w.dy[s1] = wi2.y; w.dz[s1] = wi2.z;
-        w.tmax[s1] = tvis;
-        w.hprim[s1] = tie;
-        w.Mr[p] = M.r; w.Mg[p] = M.g; w.Mb[p] = M.b;
+        put_ray(w, (size_t)N + p, o2, wi2, tvis, tie);
         hm = true;
       }
     }
   }
   if (!(hs || hm)) return 0;
-  w.sel[p] = sel;
-  w.nbr[p] = beta.r; w.nbg[p] = beta.g; w.nbb[p] = beta.b;
+  w.pa[p] = make_float4(A.r, A.g, A.b, sel);
+  w.pm[p] = make_float4(M.r, M.g, M.b, 0.f);
+  w.pn[p] = make_float4(beta.r, beta.g, beta.b, 0.f);
   return F_PEND | (hs ? F_SHADOW : 0) | (hm ? F_MIS : 0);
 }
 

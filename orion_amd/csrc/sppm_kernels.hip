@@ -167,13 +167,8 @@ __global__ void k_cam_raygen(DScene sc, Wave w, CamTerms ct, int c0) {
   uint32_t index = (uint32_t)(c0 + jb) * (uint32_t)w.npix + (uint32_t)q;  // sppm.cpp:86
   V3 o, d;
   camera_ray(sc, i, j, index, &o, &d);
-  w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
-  w.dx[p] = d.x; w.dy[p] = d.y; w.dz[p] = d.z;
-  w.tmax[p] = kFltMax;
-  w.br[p] = 1.f; w.bg[p] = 1.f; w.bb[p] = 1.f;
-  w.dim[p] = 7;
-  w.bounce[p] = 0;
-  w.flags[p] = F_ALIVE;
+  put_ray(w, p, o, d, kFltMax);
+  w.st[p] = make_float4(1.f, 1.f, 1.f, path_bits(F_ALIVE, 0, 7));
   w.qpath[0][p] = p;
   w.qray[p] = p;
   ct.n[p] = 0;
@@ -192,7 +187,9 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, Ca
     int p = act ? w.qpath[cur][i] : 0;
     bool want_ray0 = false, want_ray1 = false, want_shadow = false, keep = false;
     if (act) {
-      int fl = w.flags[p];
+      const float4 S = w.st[p];
+      int fl = bits_flags(S), depth = bits_bounce(S), dim = bits_dim(S);
+      Sp beta = sp3(S.x, S.y, S.z);
       int nt = ct.n[p];
       auto put = [&](Sp x) {
         if (nt < ct.T) {
@@ -211,20 +208,19 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, Ca
       }
       if (fl & F_ALIVE) {
         fl &= ~F_ALIVE;
-        int depth = w.bounce[p];
-        int hp = w.hprim[p];
+        const float4 H = w.hit[p];
+        const int hp = __float_as_int(H.w);
         const uint32_t jb = (uint32_t)(p / w.npix);
         const uint32_t index = ((uint32_t)c0 + jb) * (uint32_t)w.npix + (uint32_t)(p - (int)jb * w.npix);
+        const float4 D = w.rd[p];
+        const V3 d = mk(D.x, D.y, D.z);
         if (hp >= 0) {
-          Sp beta = sp3(w.br[p], w.bg[p], w.bb[p]);
-          V3 d = mk(w.dx[p], w.dy[p], w.dz[p]);
           Isect is;
-          make_isect(sc, hp, w.hu[p], w.hv[p], &is);
+          make_isect(sc, hp, H.y, H.z, &is);
           Bsdf bs;
           make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)is.mat, is.ng, is.ns, is.u, is.v, false, &bs);
           V3 wo = neg(d);
           if ((depth == 0 || (fl & F_SPEC)) && is.light >= 0) put(beta * area_emission(sc, is, wo));
-          int dim = w.dim[p];
           int ref_rays = 0;
           int nf = setup_direct(sc, w, p, bs, is, d, beta, index, &dim, true, &ref_rays);
           fl |= nf;
@@ -256,20 +252,14 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, Ca
               }
               if (alive) {
                 V3 o = offset_ray_origin(is.p, is.ng, wi);
-                w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
-                w.dx[p] = wi.x; w.dy[p] = wi.y; w.dz[p] = wi.z;
-                w.tmax[p] = kFltMax;
-                w.bounce[p] = depth + 1;
+                put_ray(w, p, o, wi, kFltMax);
+                ++depth;
                 fl |= F_ALIVE;
                 want_ray0 = true;
               }
             }
           }
-          w.br[p] = beta.r; w.bg[p] = beta.g; w.bb[p] = beta.b;
-          w.dim[p] = dim;
         } else {  // escaped: every envmap's Le at any depth (sppm.cpp:95-99)
-          const V3 d = mk(w.dx[p], w.dy[p], w.dz[p]);
-          const Sp beta = sp3(w.br[p], w.bg[p], w.bb[p]);
           for (int e = 0; e < sc.n_env; ++e) {
             const float *LR = sc.lights + 32 * (size_t)gld(sc.env_meta + 5 * e);
             put(beta * env_le(sc, LR, d));
@@ -277,7 +267,7 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, Ca
         }
       }
       ct.n[p] = nt;
-      w.flags[p] = fl;
+      w.st[p] = make_float4(beta.r, beta.g, beta.b, path_bits(fl, depth, dim));
       keep = (fl & (F_ALIVE | F_PEND)) != 0;
     }
     int a = wave_append(&w.ctrl[C_PATH0 + nxt], keep);
@@ -585,11 +575,8 @@ __device__ __forceinline__ bool photon_emit(DScene sc, Wave w, uint32_t index0, 
     beta = Le * absdot(nl, d) / (lightPdf * pdfPos * pdfDir);
     alive = !black(beta);
   }
-  w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
-  w.dx[p] = d.x; w.dy[p] = d.y; w.dz[p] = d.z;
-  w.tmax[p] = kFltMax;
-  w.br[p] = beta.r; w.bg[p] = beta.g; w.bb[p] = beta.b;
-  w.dim[p] = 10;
+  put_ray(w, p, o, d, kFltMax);
+  w.st[p] = make_float4(beta.r, beta.g, beta.b, path_bits(0, 0, 10));
   return alive;
 }
 
@@ -701,19 +688,21 @@ __global__ __launch_bounds__(256) void k_photon_bounce(DScene sc, Wave w, Photon
     Sp beta = sp(0);
     if (i < count) {
       p = qcur[i];
-      int hp = w.hprim[p];
+      const float4 H = w.hit[p];
+      const int hp = __float_as_int(H.w);
       if (hp >= 0) {
         Isect is;
-        make_isect(sc, hp, w.hu[p], w.hv[p], &is);
-        d = mk(w.dx[p], w.dy[p], w.dz[p]);
-        beta = sp3(w.br[p], w.bg[p], w.bb[p]);
+        make_isect(sc, hp, H.y, H.z, &is);
+        const float4 D = w.rd[p], S = w.st[p];
+        d = mk(D.x, D.y, D.z);
+        beta = sp3(S.x, S.y, S.z);
         hpos = is.p;
         rec = depth > 0;
         jb = p / ph.photons;
         Bsdf bs;
         make_bsdf(sc.mats + ORION_MAT_FLOATS * (size_t)is.mat, is.ng, is.ns, is.u, is.v, true, &bs);
         uint32_t index = index0 + (uint32_t)p;
-        int dim = w.dim[p];
+        int dim = bits_dim(S);
         float u1 = sobol(sc.sobol, index, dim++), u0 = sobol(sc.sobol, index, dim++);
         V3 wi = mk(0, 0, 0);
         float pdf = 0;
@@ -725,13 +714,11 @@ __global__ __launch_bounds__(256) void k_photon_bounce(DScene sc, Wave w, Photon
           if (!(sobol(sc.sobol, index, dim++) < q)) {
             Sp bkeep = bnew / (1 - q);
             V3 o = offset_ray_origin(is.p, is.ng, wi);
-            w.ox[p] = o.x; w.oy[p] = o.y; w.oz[p] = o.z;
-            w.dx[p] = wi.x; w.dy[p] = wi.y; w.dz[p] = wi.z;
-            w.br[p] = bkeep.r; w.bg[p] = bkeep.g; w.bb[p] = bkeep.b;
+            put_ray(w, p, o, wi, kFltMax);
+            w.st[p] = make_float4(bkeep.r, bkeep.g, bkeep.b, path_bits(0, 0, dim));
             keep = true;
           }
         }
-        w.dim[p] = dim;
       }
     }
     // record the hits, one atomic per iteration present in the wave

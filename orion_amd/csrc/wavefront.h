@@ -1,5 +1,5 @@
-// Wavefront path-tracing state in HBM (structure of arrays, one entry per
-// path of the current wave) and the wave64 queue helpers.
+// Wavefront path-tracing state in HBM (one 16-byte record per path in each
+// of a few arrays) and the wave64 queue helpers.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -85,23 +85,23 @@ struct Wave {
   int npix;       // (SPPM) pixels of the image
   uint32_t k0;
   int W, H;
-  // rays: slot s of path p at s * N + p; slots 0 continuation, 1 MIS
-  // visibility, 2 shadow
-  float *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
-  // closest hits of slot 0 (N entries); hprim[N + p]: the tie primitive of
-  // path p's MIS visibility ray
-  float *ht, *hu, *hv;
-  int *hprim;
+  // Per-path state as float4 records, so that a path's fields are a few
+  // 16-byte loads and stores, each in one cache line: shading and traversal
+  // reach paths through compacted queues, and a scalar array per field put
+  // every field of a path in a line of its own (shading is bound by the
+  // lines its L1 can have in flight).
+  //   rays, slot s of path p at s * N + p (slots: 0 continuation, 1 MIS
+  //   visibility, 2 shadow): ro = (origin, tmax), rd = (direction, tie as
+  //   int bits: the MIS visibility query's tie primitive, kTieAll otherwise)
+  float4 *ro, *rd;
+  float4 *hit;    // closest hit of slot 0: (t, u, v, primitive as int bits; -1 = miss)
   int *vis;       // any-hit results: vis[p] shadow ray, vis[N + p] MIS ray (1 = unoccluded)
-  // path state
-  float *br, *bg, *bb;   // beta
-  float *Lr, *Lg, *Lb;   // radiance
-  int *dim, *bounce, *flags;
-  // pending direct lighting
-  float *Ar, *Ag, *Ab;   // light-sampled term (counts when its shadow ray is unoccluded)
-  float *Mr, *Mg, *Mb;   // BSDF-sampled (MIS) term (counts when its visibility ray is)
-  float *sel;            // light selection pdf
-  float *nbr, *nbg, *nbb;  // beta at the shading point
+  float4 *st;     // (beta, path_bits(flags, bounce, Sobol dimension))
+  float4 *rad;    // (L, unused)
+  // pending direct lighting: (A, light selection pdf), (M, unused), (beta at
+  // the shading point, unused); A counts when the shadow ray is unoccluded,
+  // M (the BSDF-sampled MIS term) when its visibility ray is
+  float4 *pa, *pm, *pn;
   // queues
   int *qpath[2];
   int *qray;      // path ids (slot 0 rays)
@@ -138,6 +138,20 @@ struct Wave {
 };
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// st.w: flags (bits 0-7), bounce (8-15), next Sobol dimension (16-31)
+__device__ __forceinline__ float path_bits(int flags, int bounce, int dim) {
+  return __int_as_float(flags | (bounce << 8) | (dim << 16));
+}
+__device__ __forceinline__ int bits_flags(float4 s) { return __float_as_int(s.w) & 0xff; }
+__device__ __forceinline__ int bits_bounce(float4 s) { return (__float_as_int(s.w) >> 8) & 0xff; }
+__device__ __forceinline__ int bits_dim(float4 s) { return (int)((uint32_t)__float_as_int(s.w) >> 16); }
+// ray slot `slot` (s * N + p)
+__device__ __forceinline__ void put_ray(const Wave &w, size_t slot, dev::V3 o, dev::V3 d, float tmax,
+                                        int tie = dev::kTieAll) {
+  w.ro[slot] = make_float4(o.x, o.y, o.z, tmax);
+  w.rd[slot] = make_float4(d.x, d.y, d.z, __int_as_float(tie));
+}
 
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD
 // b % 8) and each XCD has its own L2.  xcd_block maps the hardware block id

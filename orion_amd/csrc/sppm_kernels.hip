@@ -988,6 +988,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
                                  const orion_sppm_debug *dbg, orion_render_stats *stats) {
   if (!s) { set_error("null scene"); return ORION_ERR_ARG; }
   if (n_iters <= 0 || maxDepth <= 0 || !(r0 > 0)) { set_error("bad SPPM parameters"); return ORION_ERR_ARG; }
+  if (maxDepth > 60) { set_error("maxDepth above 60 exceeds the Sobol dimension budget"); return ORION_ERR_UNSUPPORTED; }
   const int npix = s->W * s->H;
   if (photons < 0) photons = npix;  // sppm.cpp:329-330
   if (hash_size <= 0) hash_size = npix;  // sppm.cpp:147

@@ -55,9 +55,7 @@ __global__ void k_raygen(const Job *__restrict__ jobs, Wave w) {
   uint32_t index = (J.k0 + (uint32_t)s) * (uint32_t)J.npix + (uint32_t)pix;  // path.cpp:26
   V3 o, d;
   camera_ray(J.sc, i, j, index, &o, &d);
-  put_ray(w, p, o, d, kFltMax);
-  w.st[p] = make_float4(1.f, 1.f, 1.f, path_bits(F_ALIVE, 0, 7));
-  w.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+  put_ray(w, p, o, d, kFltMax);  // bounce 0's shading starts the path state
   const int sg = p / w.seg, j0 = p - sg * w.seg;
   w.qpath[0][p] = p;
   w.qray[sg * w.seg + j0] = p;
@@ -196,18 +194,18 @@ __global__ void k_between(Wave w, int nxt) {
 // every store to the wave's arrays forced a vector reload of each pointer
 // before its next dependent load (measured 8% faster shading).
 template <bool MULTI>
-__global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_shade(const Job *__restrict__ jobs, Wave w, int b) {
+__global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_shade(const Job *__restrict__ jobs, Wave w, int it) {
   __shared__ int lw[kShadeThreads / 64][3];  // per wave64: its appends, then their offsets in the block's range
   __shared__ int lbase[3];  // the block's ranges in the three queues
-  const int cur = b & 1, nxt = cur ^ 1;
+  const int cur = it & 1, nxt = cur ^ 1;
   const int K = w.shade_parts, cs = w.cnt_stride;
   const int lb = xcd_block();
   const int ls = lb / K, part = lb - ls * K;
   const int sg = MULTI ? w.seg_perm[ls] : ls;
   const int seg0 = sg * w.seg;
-  int *const cnt_cur = w.cnt + 3 * cs * (b % 3);
-  int *const cnt_nxt = w.cnt + 3 * cs * ((b + 1) % 3);
-  if (part == 0 && threadIdx.x < 3) w.cnt[3 * cs * ((b + 2) % 3) + threadIdx.x * cs + sg] = 0;
+  int *const cnt_cur = w.cnt + 3 * cs * (it % 3);
+  int *const cnt_nxt = w.cnt + 3 * cs * ((it + 1) % 3);
+  if (part == 0 && threadIdx.x < 3) w.cnt[3 * cs * ((it + 2) % 3) + threadIdx.x * cs + sg] = 0;
   const int count = cnt_cur[sg];
   const int per = ((count + K - 1) / K + kShadeThreads - 1) / kShadeThreads * kShadeThreads;
   const int p_begin = min(count, part * per), p_end = min(count, p_begin + per);
@@ -224,10 +222,12 @@ __global__ __launch_bounds__(kShadeThreads) __attribute__((amdgpu_waves_per_eu(3
     bool want_ray0 = false, want_ray1 = false, want_shadow = false, keep = false, done = false;
     int ref_rays = 0;  // closest-hit casts orion makes for the MIS sample
     if (act) {
-      const float4 S = w.st[p];
+      // bounce 0 shades the camera paths: their state is the initial one
+      // (raygen does not write it)
+      const float4 S = it == 0 ? make_float4(1.f, 1.f, 1.f, path_bits(F_ALIVE, 0, 7)) : w.st[p];
       int fl = bits_flags(S), b = bits_bounce(S), dim = bits_dim(S);
       Sp beta = sp3(S.x, S.y, S.z);
-      const float4 R = w.rad[p];
+      const float4 R = it == 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : w.rad[p];
       Sp L = sp3(R.x, R.y, R.z);
       // 1) resolve the direct-lighting estimate of the previous bounce
       if (fl & F_PEND) {

@@ -149,6 +149,13 @@ int orion_light_record(int type, const float I[3], const float from[3], const fl
 int orion_ctx_create(int device, orion_ctx *out);
 int orion_ctx_destroy(orion_ctx c);
 int orion_ctx_synchronize(orion_ctx c);
+/* Self-check of device arithmetic the traversal relies on (no counterpart in
+ * orion): the reciprocal of the triangle and slab tests (the hardware
+ * estimate refined by one Newton-Raphson step) against IEEE division, over
+ * every float whose magnitude lies in [2^-125, 2^125), on the context's GPU.
+ * *mismatches = how many differ in any bit (0 on gfx950; magnitudes outside
+ * the range use the division itself). */
+int orion_selfcheck_rcp(orion_ctx c, uint64_t *mismatches);
 
 /* Builds the SAH BVH on the host and uploads scene + Sobol tables to HBM.
  * Allocates the film (W*H sum + count) on the device. */

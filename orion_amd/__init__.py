@@ -112,6 +112,7 @@ _SIGS = {
     "orion_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "orion_ctx_destroy": (C.c_int, [C.c_void_p]),
     "orion_ctx_synchronize": (C.c_int, [C.c_void_p]),
+    "orion_selfcheck_rcp": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "orion_scene_upload": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "orion_scene_destroy": (C.c_int, [C.c_void_p]),
     "orion_scene_get_stats": (C.c_int, [C.c_void_p, C.POINTER(SceneStats)]),
@@ -350,6 +351,13 @@ class Device:
 
     def synchronize(self):
         _check(lib().orion_ctx_synchronize(self.h))
+
+    def selfcheck_rcp(self):
+        """Floats in [2^-125, 2^125) whose device reciprocal differs from
+        IEEE division (0 expected)."""
+        n = C.c_uint64()
+        _check(lib().orion_selfcheck_rcp(self.h, C.byref(n)))
+        return n.value
 
     def close(self):
         if self.h:

@@ -32,6 +32,21 @@ __device__ __forceinline__ float dcos(float x) { return (float)cos((double)x); }
 __device__ __forceinline__ float dacos(float x) { return (float)acos((double)x); }
 __device__ __forceinline__ float datan2(float y, float x) { return (float)atan2((double)y, (double)x); }
 
+// 1 / x rounded to nearest, as IEEE division rounds it: the hardware
+// reciprocal estimate refined by one Newton-Raphson step with fused
+// multiply-adds (Markstein's correction r + r (1 - x r)).  For |x| in
+// [2^-125, 2^125) the result equals the IEEE quotient bit for bit on gfx950
+// -- every float in that range is compared by orion_selfcheck_rcp -- and
+// other magnitudes (zero, infinities, NaN, the extremes) take the division.
+__device__ __forceinline__ float rcp_rn(float x) {
+  const float ax = fabsf(x);
+  if (__builtin_expect(ax >= 0x1p-125f && ax < 0x1p125f, 1)) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(fmaf(-x, r, 1.0f), r, r);
+  }
+  return 1.0f / x;
+}
+
 struct V3 { float x, y, z; };
 __device__ __forceinline__ V3 mk(float x, float y, float z) { V3 r; r.x = x; r.y = y; r.z = z; return r; }
 __device__ __forceinline__ V3 add(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
@@ -186,6 +201,13 @@ template <>
 __device__ __forceinline__ uint2 gld<uint2>(const uint2 *p) {
   gu2 v = *(const __attribute__((address_space(1))) gu2 *)p;
   return make_uint2(v.x, v.y);
+}
+// three consecutive floats (one 12-byte load: the fourth word of a 16-byte
+// record is not read)
+__device__ __forceinline__ V3 gld3(const float *p) {
+  typedef const __attribute__((address_space(1))) float gF;
+  const gF *q = (const gF *)p;
+  return mk(q[0], q[1], q[2]);
 }
 
 

@@ -41,29 +41,34 @@ struct DScene {
 
 namespace dev {
 
-// Robust slab test (Ize 2013 / pbrt-v3): the exit distance is scaled by
-// 1 + 2 gamma(3) so rounding never culls a box the exact test would enter.
-// The scale is applied once to min(t_far) -- identical to scaling each slab
-// first, as multiplication by a positive constant is monotonic.
-__device__ __forceinline__ bool box_hit(float lox, float hix, float loy, float hiy, float loz, float hiz,
-                                        V3 o, V3 inv, float tmin, float tmax, float *tnear) {
-  float ax = (lox - o.x) * inv.x, bx = (hix - o.x) * inv.x;
-  float ay = (loy - o.y) * inv.y, by = (hiy - o.y) * inv.y;
-  float az = (loz - o.z) * inv.z, bz = (hiz - o.z) * inv.z;
-  float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
-  float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz)) * 1.00000035762786865234375f;
+// Robust slab test (Ize 2013 / pbrt-v3) on a box given by its near and far
+// planes along the ray (per axis: the lower plane when the direction's
+// component is positive, the upper one otherwise).  The exit distance is
+// scaled by 1 + 2 gamma(3) so rounding never culls a box the exact test would
+// enter; the scale is applied once to min(t_far) -- identical to scaling each
+// slab first, as multiplication by a positive constant is monotonic.  Taking
+// the planes by the direction's sign gives the same entry and exit distances
+// as min / max over both planes of each axis: (plane - o) * inv is monotonic
+// in the plane (rounded subtraction and multiplication preserve order), and
+// a box's lower plane never exceeds its upper one.
+__device__ __forceinline__ bool box_hit(float nx, float fx, float ny, float fy, float nz, float fz, V3 o, V3 inv,
+                                        float tmin, float tmax, float *tnear) {
+  const float anx = (nx - o.x) * inv.x, afx = (fx - o.x) * inv.x;
+  const float any = (ny - o.y) * inv.y, afy = (fy - o.y) * inv.y;
+  const float anz = (nz - o.z) * inv.z, afz = (fz - o.z) * inv.z;
+  const float t0 = fmaxf(fmaxf(anx, any), fmaxf(anz, tmin));
+  const float t1 = fminf(fminf(afx, afy), afz) * 1.00000035762786865234375f;
   *tnear = t0;
   return t0 <= fminf(t1, tmax);
 }
 
 // Triangle test: identical operation sequence to the CPU restatement.
-__device__ __forceinline__ bool tri_test(float4 a, float4 b, float4 c, V3 o, V3 d, float tmin, float tmax,
-                                         float *t, float *u, float *v) {
-  V3 p0 = mk(a.x, a.y, a.z), e1 = mk(b.x, b.y, b.z), e2 = mk(c.x, c.y, c.z);
+__device__ __forceinline__ bool tri_test(V3 p0, V3 e1, V3 e2, V3 o, V3 d, float tmin, float tmax, float *t,
+                                         float *u, float *v) {
   V3 pv = cross(d, e2);
   float det = dot(e1, pv);
   if (det == 0.0f || det != det) return false;
-  float inv = 1.0f / det;
+  float inv = rcp_rn(det);
   V3 tv = sub(o, p0);
   float uu = dot(tv, pv) * inv;
   if (!(uu >= 0.0f && uu <= 1.0f)) return false;
@@ -85,23 +90,38 @@ struct HitRec {
 // Traversal stack: the newest RING entries of each lane live in an LDS ring
 // (`ring` = the lane's first slot, slots kRingStride ints apart, one LDS
 // column per lane so a wave's accesses are conflict-free); older entries
-// move to the lane's spill area in HBM (`spill`, entries `spill_stride` ints
-// apart, `spill_cap` entries; the offset stride * entry is formed in 64 bits:
-// a 2^26-path wave's launch has up to 2^27 threads).  A DFS stack of a
-// 4-wide tree holds at most three siblings per level, so stack_bound entries
-// never overflow; an overflow (a malformed tree) is counted in *overflow,
-// the entry dropped, and the host fails the call with ORION_ERR_UNSUPPORTED.
+// move to the wave's spill area in HBM (`spill`, wave-uniform: lane l's entry
+// e is spill[e * spill_stride + l], `spill_cap` entries; the offset is formed
+// in 64 bits: a 2^26-path wave's launch has up to 2^27 threads).  A DFS stack
+// of a 4-wide tree holds at most three siblings per level, so stack_bound
+// entries never overflow; an overflow (a malformed tree) is counted in
+// *overflow, the entry dropped, and the host fails the call with
+// ORION_ERR_UNSUPPORTED.
 constexpr int kRingStride = 64;  // threads per traversal block: one wave64 (2 and 4 measured 3% and 20% slower)
+// Per-lane LDS words after the ring: values the engine needs only when a
+// triangle is accepted or a ray is stored (the closest hit's u and v, the
+// ray's queue tag) live there instead of in registers.
+constexpr int kLaneStash = 3;
+enum { STASH_U = 0, STASH_V = 1, STASH_TAG = 2 };
+// LDS ints per traversal block for a ring of RING entries per lane
+template <int RING>
+constexpr int trace_lds_ints() { return (RING + kLaneStash) * kRingStride; }
 
 template <int RING>
 struct TravStack {
   int *ring;
-  int *spill;
+  int *spill;  // the wave's spill area (scalar)
   int spill_stride, spill_cap;
-  int top = 0, base = 0;  // ring holds [base, top); spill holds [0, base)
+  int top = 0, base = 0;  // the ring holds [base, top), the spill area [0, base)
+  __device__ __forceinline__ void reset() { top = base = 0; }
+  __device__ __forceinline__ void stash(int j, int x) { ring[kRingStride * (RING + j)] = x; }
+  __device__ __forceinline__ int unstash(int j) const { return ring[kRingStride * (RING + j)]; }
+  __device__ __forceinline__ int *spill_slot(int e) const {
+    return spill + ((size_t)spill_stride * e + __lane_id());
+  }
   __device__ __forceinline__ void push(int x, uint32_t *overflow) {
     if (top - base == RING) {
-      if (base < spill_cap) spill[(size_t)spill_stride * base] = ring[kRingStride * (base & (RING - 1))];
+      if (base < spill_cap) *spill_slot(base) = ring[kRingStride * (base & (RING - 1))];
       else ++*overflow;
       ++base;
     }
@@ -116,7 +136,7 @@ struct TravStack {
     if (base > 0) {
       --base;
       --top;
-      return base < spill_cap ? spill[(size_t)spill_stride * base] : kLeafEmpty;
+      return base < spill_cap ? *spill_slot(base) : kLeafEmpty;
     }
     return kLeafEmpty;
   }
@@ -141,7 +161,7 @@ struct TRay {
 // (a NaN would drop a slab); the padded boxes absorb the change.
 __device__ __forceinline__ V3 safe_inv(V3 d) {
   auto safe = [](float x) { return fabsf(x) > 1e-35f ? x : copysignf(1e-35f, x); };
-  return mk(1.0f / safe(d.x), 1.0f / safe(d.y), 1.0f / safe(d.z));
+  return mk(rcp_rn(safe(d.x)), rcp_rn(safe(d.y)), rcp_rn(safe(d.z)));
 }
 
 // Inner node: decode the four child boxes (fmaf(q, step, origin), exactly as
@@ -149,33 +169,41 @@ __device__ __forceinline__ V3 safe_inv(V3 d) {
 // child and push the others far-to-near.  Returns the next node (a leaf code
 // or kLeafEmpty when the stack is exhausted).
 template <bool ANY, int RING>
-__device__ __forceinline__ int node_step(const TRay &ray, float best, float4 A, float4 f1, float4 f2, float4 f3,
+__device__ __forceinline__ int node_step(const TRay &ray, float best, float4 A, float4 f1, float4 f2, uint2 f3,
                                          TravStack<RING> &st, uint32_t *overflow) {
   const int4 C = make_int4(__float_as_int(f1.x), __float_as_int(f1.y), __float_as_int(f1.z), __float_as_int(f1.w));
   const uint4 Q = make_uint4(__float_as_uint(f2.x), __float_as_uint(f2.y), __float_as_uint(f2.z), __float_as_uint(f2.w));
-  const uint2 R = make_uint2(__float_as_uint(f3.x), __float_as_uint(f3.y));
+  const uint2 R = f3;
   const uint32_t eb = __float_as_uint(A.w);
   const float sx = __uint_as_float((eb & 0xffu) << 23);
   const float sy = __uint_as_float(((eb >> 8) & 0xffu) << 23);
   const float sz = __uint_as_float(((eb >> 16) & 0xffu) << 23);
+  // the four children's near and far plane bytes per axis, chosen once per
+  // node by the signs of the ray's inverse direction
+  const bool negx = ray.inv.x < 0.0f, negy = ray.inv.y < 0.0f, negz = ray.inv.z < 0.0f;
+  const uint32_t nxw = negx ? Q.y : Q.x, fxw = negx ? Q.x : Q.y;
+  const uint32_t nyw = negy ? Q.w : Q.z, fyw = negy ? Q.z : Q.w;
+  const uint32_t nzw = negz ? R.y : R.x, fzw = negz ? R.x : R.y;
   const int cc[4] = {C.x, C.y, C.z, C.w};
   float tk[4];
   int ck[4];
   int nh = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    float lox = fmaf(q2f(Q.x, k), sx, A.x), hix = fmaf(q2f(Q.y, k), sx, A.x);
-    float loy = fmaf(q2f(Q.z, k), sy, A.y), hiy = fmaf(q2f(Q.w, k), sy, A.y);
-    float loz = fmaf(q2f(R.x, k), sz, A.z), hiz = fmaf(q2f(R.y, k), sz, A.z);
+    float nx = fmaf(q2f(nxw, k), sx, A.x), fx = fmaf(q2f(fxw, k), sx, A.x);
+    float ny = fmaf(q2f(nyw, k), sy, A.y), fy = fmaf(q2f(fyw, k), sy, A.y);
+    float nz = fmaf(q2f(nzw, k), sz, A.z), fz = fmaf(q2f(fzw, k), sz, A.z);
     float t;
-    bool hit = box_hit(lox, hix, loy, hiy, loz, hiz, ray.o, ray.inv, ray.tmin, best, &t) & (cc[k] != kLeafEmpty);
+    bool hit = box_hit(nx, fx, ny, fy, nz, fz, ray.o, ray.inv, ray.tmin, best, &t) & (cc[k] != kLeafEmpty);
     tk[k] = hit ? t : __builtin_inff();
     ck[k] = cc[k];
     nh += hit ? 1 : 0;
   }
   auto cs = [&](int a, int b) {
     // ascending entry distance (missed children last), for any-hit rays too:
-    // measured 2% faster than visiting hit children in slot order
+    // measured 2% faster than visiting hit children in slot order (a minimum
+    // tree for the nearest child with the others pushed unsorted: 0.5%
+    // slower for any-hit rays)
     bool sw = tk[b] < tk[a];
     float ta = tk[a], tb = tk[b];
     int ca = ck[a], cb = ck[b];
@@ -192,11 +220,11 @@ __device__ __forceinline__ int node_step(const TRay &ray, float best, float4 A, 
 // One triangle of a leaf (its three float4s): the triangle test, keeping the
 // lexicographic minimum (t, prim) and the tie flag.  Any-hit returns true
 // when accepted.
-template <bool ANY>
-__device__ __forceinline__ bool tri_step(const TRay &ray, float4 a, float4 b, float4 c, float *best, HitRec *h,
-                                         bool *found) {
+template <bool ANY, class ST>
+__device__ __forceinline__ bool tri_step(const TRay &ray, float4 a, V3 e1, V3 e2, float *best, HitRec *h,
+                                         bool *found, ST &st) {
   float t, u, v;
-  if (!tri_test(a, b, c, ray.o, ray.d, ray.tmin, ANY ? ray.tmax : *best, &t, &u, &v)) return false;
+  if (!tri_test(mk(a.x, a.y, a.z), e1, e2, ray.o, ray.d, ray.tmin, ANY ? ray.tmax : *best, &t, &u, &v)) return false;
   int prim = __float_as_int(a.w);
   if (ANY) {
     if (!(t < ray.tmax || prim < ray.tie)) return false;
@@ -204,11 +232,19 @@ __device__ __forceinline__ bool tri_step(const TRay &ray, float4 a, float4 b, fl
     *found = true;
     return true;
   }
+  // closest hit: its t is *best, its u and v wait in the lane's LDS words
+  // (h->t, h->u, h->v are filled in when the ray is stored)
   if (!*found || t < *best) {
-    *best = t; h->t = t; h->u = u; h->v = v; h->prim = prim; h->tie = 0; *found = true;
+    *best = t; h->prim = prim; h->tie = 0; *found = true;
+    st.stash(STASH_U, __float_as_int(u));
+    st.stash(STASH_V, __float_as_int(v));
   } else if (t == *best) {
     h->tie = 1;
-    if (prim < h->prim) { h->u = u; h->v = v; h->prim = prim; }
+    if (prim < h->prim) {
+      h->prim = prim;
+      st.stash(STASH_U, __float_as_int(u));
+      st.stash(STASH_V, __float_as_int(v));
+    }
   }
   return false;
 }
@@ -232,8 +268,6 @@ constexpr int kRefillMin = 8;
 template <bool ANY, int RING, class IO>
 __device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> &st, uint32_t *node_visits,
                                     uint32_t *tri_tests, uint32_t *overflow) {
-  const int lane = (int)__lane_id();
-  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const float4 *const nodes = io.nodes, *const tris = io.tris;
   int next = begin;  // wave-uniform: next unassigned entry
   bool active = false;
@@ -250,9 +284,12 @@ __device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> 
     const uint64_t idle = __ballot(!active);
     const int nidle = __popcll(idle);
     if (next < end && (nidle >= kRefillMin || nidle == 64)) {
-      const int entry = next + __popcll(idle & below);
+      // the lane's rank among the idle lanes below it
+      const int entry = next + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
       if (!active && entry < end) {
         io.load(entry, &ray);
+        st.stash(STASH_TAG, ray.tag);
         ray.inv = safe_inv(ray.d);
         active = true;
         found = false;
@@ -261,7 +298,7 @@ __device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> 
         h.tie = 0;
         node = 0;
         tcur = tend = 0;
-        st.top = st.base = 0;
+        st.reset();
       }
       next = min(next + nidle, end);
     }
@@ -269,33 +306,43 @@ __device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> 
     // a lane with an open leaf and a popped inner node does one of each per
     // iteration: both fetches are in flight together
     const bool do_tri = tcur < tend, do_node = node >= 0;
-    float4 f0, f1, f2, f3, g0, g1, g2;
+    float4 f0, f1, f2, g0;
+    uint2 f3;
+    V3 e1, e2;
     if (do_tri) {
       const float4 *src = tris + 3 * (size_t)tcur;
-      g0 = gld(src); g1 = gld(src + 1); g2 = gld(src + 2);
+      g0 = gld(src);
+      e1 = gld3((const float *)(src + 1));
+      e2 = gld3((const float *)(src + 2));
     }
     if (do_node) {
       const float4 *src = nodes + 4 * (size_t)node;
-      f0 = gld(src); f1 = gld(src + 1); f2 = gld(src + 2); f3 = gld(src + 3);
+      f0 = gld(src); f1 = gld(src + 1); f2 = gld(src + 2); f3 = gld((const uint2 *)(src + 3));
     }
     // visit counters are kept per wave (scalar registers)
     nt += (uint32_t)__popcll(__ballot(do_tri));
     nv += (uint32_t)__popcll(__ballot(do_node));
     if (do_tri) {
       ++tcur;
-      if (tri_step<ANY>(ray, g0, g1, g2, &best, &h, &found)) {
+      if (tri_step<ANY>(ray, g0, e1, e2, &best, &h, &found, st)) {
         tcur = tend;  // any-hit: occluded, done
         node = kLeafEmpty;
       }
     }
     if (do_node && node >= 0) node = node_step<ANY, RING>(ray, best, f0, f1, f2, f3, st, overflow);
     if (tcur >= tend && node < 0 && node != kLeafEmpty) {  // open the leaf reached
-      const int code = ~node;
+      const int code = ~node;  // first triangle << 3 | count - 1
       tcur = code >> 3;
       tend = tcur + (code & 7) + 1;
       node = st.pop();
     }
     if (active && tcur >= tend && node == kLeafEmpty) {
+      ray.tag = st.unstash(STASH_TAG);
+      if (!ANY && found) {
+        h.t = best;
+        h.u = __int_as_float(st.unstash(STASH_U));
+        h.v = __int_as_float(st.unstash(STASH_V));
+      }
       io.store(ray, found, h);
       active = false;
     }

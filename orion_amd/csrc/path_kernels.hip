@@ -126,7 +126,7 @@ __device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int 
   TravStack<RING> stk;
   stk.ring = ring + threadIdx.x;
   stk.spill_stride = gridDim.x * kTraceBlock;
-  stk.spill = w.spill + blockIdx.x * kTraceBlock + threadIdx.x;
+  stk.spill = w.spill + (size_t)blockIdx.x * kTraceBlock;
   stk.spill_cap = w.spill_cap;
   uint32_t ovf = 0, nv = 0, nt = 0;
   const DScene &sc = jobs[MULTI ? w.seg_job[sg] : 0].sc;
@@ -143,7 +143,7 @@ __device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int 
 
 template <bool ANY, int RING, bool MULTI>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w, int chunk, int cps, int rec) {
-  __shared__ int ring[RING * kTraceBlock];
+  __shared__ int ring[trace_lds_ints<RING>()];
   trace_block<ANY, RING, MULTI>(jobs, w, chunk, cps, rec, xcd_remap(blockIdx.x, gridDim.x), ring);
 }
 
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w, 
 // counters and kernel (profiles attribute it apart from the path tracer's).
 template <int RING>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_photons(const Job *jobs, Wave w, int chunk) {
-  __shared__ int ring[RING * kTraceBlock];
+  __shared__ int ring[trace_lds_ints<RING>()];
   trace_block<false, RING, false, true>(jobs, w, chunk, 0, -1, xcd_remap(blockIdx.x, gridDim.x), ring);
 }
 
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_photons(const Job *jobs, 
 // block's XCD is the same in its part's numbering.
 template <int RING, bool MULTI>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_pair(const Job *jobs, Wave w, int chunk, int n_closest) {
-  __shared__ int ring[RING * kTraceBlock];
+  __shared__ int ring[trace_lds_ints<RING>()];
   const int b = blockIdx.x;
   if (b < n_closest) trace_block<false, RING, MULTI>(jobs, w, chunk, 0, -1, xcd_remap(b, n_closest), ring);
   else trace_block<true, RING, MULTI>(jobs, w, chunk, 0, -1, xcd_remap(b - n_closest, gridDim.x - n_closest), ring);
@@ -413,13 +413,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect_batch(DScene sc, int 
                                                                  int *prim, float *u, float *v, int *tie,
                                                                  int *occluded, int *spill, int spill_cap,
                                                                  int chunk, unsigned long long *stats) {
-  __shared__ int ring[RING * kTraceBlock];
+  __shared__ int ring[trace_lds_ints<RING>()];
   const int begin = (blockIdx.x * (kTraceBlock / 64) + (int)(threadIdx.x >> 6)) * chunk;
   if (begin >= n) return;
   TravStack<RING> stk;
   stk.ring = ring + threadIdx.x;
   stk.spill_stride = gridDim.x * kTraceBlock;
-  stk.spill = spill + blockIdx.x * kTraceBlock + threadIdx.x;
+  stk.spill = spill + (size_t)blockIdx.x * kTraceBlock;
   stk.spill_cap = spill_cap;
   uint32_t ovf = 0, nv = 0, nt = 0;
   SoaRays io{sc.nodes, sc.tris, n, rays, t, u, v, prim, tie, ANY ? occluded : nullptr};
@@ -437,11 +437,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_intersect_fixed(DScene sc, int 
                                                                  int *prim, float *u, float *v, int *tie,
                                                                  int *spill, int spill_cap, int chunk,
                                                                  unsigned int *overflow) {
-  __shared__ int ring[RING * kTraceBlock];
+  __shared__ int ring[trace_lds_ints<RING>()];
   TravStack<RING> stk;
   stk.ring = ring + threadIdx.x;
   stk.spill_stride = gridDim.x * kTraceBlock;
-  stk.spill = spill + blockIdx.x * kTraceBlock + threadIdx.x;
+  stk.spill = spill + (size_t)blockIdx.x * kTraceBlock;
   stk.spill_cap = spill_cap;
   uint32_t ovf = 0, nv = 0, nt = 0;
   SoaRays io{sc.nodes, sc.tris, n, rays, t, u, v, prim, tie, nullptr};

@@ -215,6 +215,41 @@ int orion_ctx_synchronize(orion_ctx c) {
   return ORION_OK;
 }
 
+namespace {
+// Every float bit pattern with |x| in [2^-125, 2^125): rcp_rn(x) against
+// the IEEE quotient 1 / x; mismatches summed per thread, one atomic each.
+__global__ void k_selfcheck_rcp(unsigned long long *mismatches) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  unsigned long long bad = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (1ull << 32); i += stride) {
+    const float x = __uint_as_float((uint32_t)i);
+    const float ax = fabsf(x);
+    if (!(ax >= 0x1p-125f && ax < 0x1p125f)) continue;
+    bad += __float_as_uint(orion_amd::dev::rcp_rn(x)) != __float_as_uint(1.0f / x) ? 1 : 0;
+  }
+  if (bad) atomicAdd(mismatches, bad);
+}
+}  // namespace
+
+int orion_selfcheck_rcp(orion_ctx c, uint64_t *mismatches) {
+  if (!c || !mismatches) { set_error("null argument"); return ORION_ERR_ARG; }
+  HIP_TRY(hipSetDevice(c->device));
+  unsigned long long *d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(unsigned long long)));
+  unsigned long long h = 0;
+  hipError_t e = hipMemcpyAsync(d, &h, sizeof h, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_selfcheck_rcp, dim3(16384), dim3(256), 0, c->stream, d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  HIP_TRY(e);
+  *mismatches = h;
+  return ORION_OK;
+}
+
 int orion_scene_upload(orion_ctx c, orion_hscene hs, orion_scene *out) {
   if (!c || !hs || !out) { set_error("null argument"); return ORION_ERR_ARG; }
   const HostScene &s = *reinterpret_cast<const HostScene *>(hs);

@@ -327,9 +327,9 @@ __device__ inline int nee_pick(const DScene &sc, uint32_t index, int *dim, bool 
 __device__ inline bool prim_hit(const DScene &sc, int prim, V3 o, V3 d, float *t, float *u, float *v) {
   const float4 *s = sc.shade + 5 * (size_t)prim;
   const float4 a = gld(s), b = gld(s + 1), c = gld(s + 2);
-  const float4 p0 = make_float4(a.x, a.y, a.z, 0.f);
-  const float4 e1 = make_float4(a.w - a.x, b.x - a.y, b.y - a.z, 0.f);
-  const float4 e2 = make_float4(b.z - a.x, b.w - a.y, c.x - a.z, 0.f);
+  const V3 p0 = mk(a.x, a.y, a.z);
+  const V3 e1 = mk(a.w - a.x, b.x - a.y, b.y - a.z);
+  const V3 e2 = mk(b.z - a.x, b.w - a.y, c.x - a.z);
   return tri_test(p0, e1, e2, o, d, 0.0f, kFltMax, t, u, v);
 }
 

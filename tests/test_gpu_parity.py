@@ -43,6 +43,14 @@ def image_compare(gpu, ref):
     return ok.mean(), mean_err, bits.mean()
 
 
+def test_reciprocal_equals_ieee_division(device):
+    """The triangle and slab tests' reciprocal (hardware estimate + one
+    Newton-Raphson step) equals IEEE 1 / x for every float with magnitude in
+    [2^-125, 2^125): the hit t, u, v and the box tests stay those of a true
+    division."""
+    assert device.selfcheck_rcp() == 0
+
+
 @pytest.mark.parametrize("scene,params", [("cornell", None), ("icosphere", [5, 64, 64])])
 def test_intersect_closest_bitexact(device, oracle, scene, params):
     hs = oa.HostScene.generate(scene, seed=1, params=params)

@@ -1153,7 +1153,7 @@ int orion_autotune(orion_ctx c, int n, const orion_job_desc *jobs, orion_tuning 
     t.wave_paths = wp;
     if ((rc = trial(t))) return rc;
   }
-  for (int chunk : {128, 512}) {
+  for (int chunk : {256, 1024}) {
     orion_tuning t = b;
     t.rays_per_wave = chunk;
     if ((rc = trial(t))) return rc;

@@ -260,9 +260,18 @@ __device__ __forceinline__ bool tri_step(const TRay &ray, float4 a, V3 e1, V3 e2
 // open one is done.  A lane whose ray finishes takes the next entry once at
 // least kRefillMin lanes are idle.  Waves own fixed ranges (no shared work
 // counter: device-scope atomics on one address serialise across the XCDs
-// and cost more than they balance).  IO: void load(int entry, TRay *ray);
-// void store(const TRay &, bool hit, const HitRec &); nodes, tris: the
-// scene, the same for every ray of the call (kept in scalar registers).
+// and cost more than they balance).
+//
+// A refill costs no more memory latency than any other iteration: the queue
+// values of the next refill's entries are fetched ahead (lane l holds entry
+// next + l's and hands it to the idle lane of rank l), and a new ray's
+// fields are first used after the iteration's node fetches are issued, so
+// its loads and its root node's are in flight together.
+//
+// IO: int fetch(int entry) -> the entry's queue value; void load(int value,
+// TRay *ray) (ray->tag = value); void store(const TRay &, bool hit, const
+// HitRec &); nodes, tris: the scene, the same for every ray of the call
+// (kept in scalar registers).
 constexpr int kRefillMin = 8;
 
 template <bool ANY, int RING, class IO>
@@ -280,20 +289,26 @@ __device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> 
   int node = kLeafEmpty;  // next inner node / leaf to visit
   int tcur = 0, tend = 0;  // triangles of the open leaf
   uint32_t nv = 0, nt = 0;
+  bool fresh = false;  // the lane took a new ray this iteration (its fields are still in flight)
+  // queue value of entry next + lane: what the idle lane of rank `lane`
+  // takes at the next refill
+  const int lane = (int)__lane_id();
+  int pre = next + lane < end ? io.fetch(next + lane) : 0;
   for (;;) {
     const uint64_t idle = __ballot(!active);
     const int nidle = __popcll(idle);
     if (next < end && (nidle >= kRefillMin || nidle == 64)) {
-      // the lane's rank among the idle lanes below it
-      const int entry = next + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-      if (!active && entry < end) {
-        io.load(entry, &ray);
-        st.stash(STASH_TAG, ray.tag);
-        ray.inv = safe_inv(ray.d);
+      // the lane's rank among the idle lanes below it, and that entry's
+      // queue value from the lane that fetched it
+      const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+      const int val = __shfl(pre, rank);
+      if (!active && next + rank < end) {
+        io.load(val, &ray);
+        st.stash(STASH_TAG, val);
+        fresh = true;
         active = true;
         found = false;
-        best = ray.tmax;
         h.prim = -1;
         h.tie = 0;
         node = 0;
@@ -301,6 +316,7 @@ __device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> 
         st.reset();
       }
       next = min(next + nidle, end);
+      pre = next + lane < end ? io.fetch(next + lane) : 0;
     }
     if (__ballot(active) == 0) break;
     // a lane with an open leaf and a popped inner node does one of each per
@@ -318,6 +334,11 @@ __device__ inline void trace_engine(IO &io, int begin, int end, TravStack<RING> 
     if (do_node) {
       const float4 *src = nodes + 4 * (size_t)node;
       f0 = gld(src); f1 = gld(src + 1); f2 = gld(src + 2); f3 = gld((const uint2 *)(src + 3));
+    }
+    if (fresh) {  // the new ray's fields, once its root node's fetch is issued
+      ray.inv = safe_inv(ray.d);
+      best = ray.tmax;
+      fresh = false;
     }
     // visit counters are kept per wave (scalar registers)
     nt += (uint32_t)__popcll(__ballot(do_tri));

@@ -73,10 +73,12 @@ struct WaveRays {
   int base;                   // first queue entry of the segment
   int c, cps, chunk;          // the wave's first chunk, chunk stride, chunk size
   const float4 *nodes, *tris;  // the segment's job's scene (wave-uniform)
-  __device__ void load(int q, TRay *ray) const {
+  __device__ int fetch(int q) const {
     const int k = q / chunk;
     const int e = (c + k * cps) * chunk + (q - k * chunk);
-    const int r = (ANY ? w->qshadow : w->qray)[base + e];
+    return (ANY ? w->qshadow : w->qray)[base + e];
+  }
+  __device__ void load(int r, TRay *ray) const {
     const int slot = ANY && r < w->N ? 2 * w->N + r : r;
     const float4 a = w->ro[slot], b = w->rd[slot];
     ray->o = mk(a.x, a.y, a.z);
@@ -142,7 +144,7 @@ __device__ __forceinline__ void trace_block(const Job *jobs, const Wave &w, int 
 }
 
 template <bool ANY, int RING, bool MULTI>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(const Job *jobs, Wave w, int chunk, int cps, int rec) {
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace(const Job *jobs, Wave w, int chunk, int cps, int rec) {
   __shared__ int ring[trace_lds_ints<RING>()];
   trace_block<ANY, RING, MULTI>(jobs, w, chunk, cps, rec, xcd_remap(blockIdx.x, gridDim.x), ring);
 }
@@ -386,6 +388,7 @@ struct SoaRays {
   const float *rays;
   float *t, *u, *v;
   int *prim, *tie, *occluded;
+  __device__ int fetch(int i) const { return i; }
   __device__ void load(int i, TRay *ray) const {
     ray->o = mk(rays[i], rays[n + i], rays[2 * n + i]);
     ray->d = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);

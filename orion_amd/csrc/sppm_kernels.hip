@@ -1051,8 +1051,9 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   HIP_TRY(hipEventRecord(c->ev0, st));
   const int TB = 256;
   for (const Pix &q : b.px) hipLaunchKernelGGL(k_pix_init, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, q, npix, r0);
-  // SPPM's launches trace at most 2^20 rays: 128-entry chunks (two rays per
-  // lane) measured 3% faster per iteration than 64 and 10% faster than 256
+  // SPPM's launches trace at most 2^22 rays: 128-entry chunks (two rays per
+  // lane) measured 3% faster per iteration than 64, as fast as 256 and 6%
+  // faster than 512
   const int chunk = 128;
   if ((rc = b.spill.ensure(trace_spill_ints(s->d.stack_bound, ring, trace_grid(1, std::max(B * photons, 1), false, chunk)) *
                                4 + 4)))

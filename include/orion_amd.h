@@ -230,7 +230,7 @@ typedef struct {
 typedef struct {
   int spp_per_wave;     /* plain renders: samples per pixel per wavefront (0 = from wave_paths) */
   int wave_paths;       /* paths per wavefront (0 = 2^26); served waves are capped at 2^25 */
-  int rays_per_wave;    /* traversal queue entries per wave64 (0 = 512) */
+  int rays_per_wave;    /* traversal queue entries per wave64 (0: 512 for one-job waves of 2^25 paths or more, else 256) */
   int use_graph;        /* capture the bounce loop into a hipGraph */
   int profile;          /* time every traversal / shade launch with HIP events */
   int lds_ring;         /* traversal stack entries per lane kept in LDS: 8, 16, 32 (0 = 16) */

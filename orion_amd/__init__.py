@@ -6,8 +6,10 @@ module only marshals numpy arrays across the boundary.  There is no CPU
 fallback: every render / intersection call runs the HIP kernels and raises
 if the library or a GPU is unavailable.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -147,6 +149,24 @@ _SIGS = {
 }
 
 _lib = None
+
+# Live servers, device scenes and devices.  At interpreter exit they are
+# closed here, servers first, then scenes, then their devices, while the
+# library and the HIP runtime are intact (module teardown would otherwise
+# leave the order and the library's availability to the interpreter).
+_LIVE = {"server": weakref.WeakSet(), "scene": weakref.WeakSet(), "device": weakref.WeakSet()}
+
+
+def _close_all():
+    for kind in ("server", "scene", "device"):
+        for obj in list(_LIVE[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+atexit.register(_close_all)
 
 
 def lib():
@@ -348,6 +368,7 @@ class Device:
     def __init__(self, device=0):
         self.h = C.c_void_p()
         _check(lib().orion_ctx_create(device, C.byref(self.h)))
+        _LIVE["device"].add(self)
 
     def synchronize(self):
         _check(lib().orion_ctx_synchronize(self.h))
@@ -360,8 +381,9 @@ class Device:
         return n.value
 
     def close(self):
-        if self.h:
-            lib().orion_ctx_destroy(self.h)
+        # the library refuses (ORION_ERR_BUSY) while scenes or servers of this
+        # context are alive; the handle is then kept for a later close
+        if self.h and lib().orion_ctx_destroy(self.h) == 0:
             self.h = C.c_void_p()
 
     def __del__(self):
@@ -380,6 +402,7 @@ class DeviceScene:
         self.W, self.H = hscene.width, hscene.height
         self.h = C.c_void_p()
         _check(lib().orion_scene_upload(dev.h, hscene.h, C.byref(self.h)))
+        _LIVE["scene"].add(self)
 
     def stats(self):
         st = SceneStats()
@@ -538,6 +561,7 @@ class Server:
         tune = Tuning(0, wave_paths, rays_per_wave, int(use_graph), 0, lds_ring, queue_segment,
                       int(split_trace))
         _check(lib().orion_server_create(dev.h, int(hbm_budget_bytes), C.byref(tune), C.byref(self.h)))
+        _LIVE["server"].add(self)
 
     PACK, FAIR, ARRIVAL, AGED = 0, 1, 2, 3
 

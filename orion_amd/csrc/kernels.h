@@ -9,9 +9,14 @@ namespace orion_amd {
 
 constexpr int kTraceBlock = dev::kRingStride;  // threads per traversal block (one wave64)
 constexpr int kDefaultRing = 16;          // LDS ring entries per lane (8, 16 or 32)
-// queue entries per traversal wave64: 512 (with refills prefetched, 256 and
-// 1024 measured 2% and 4% slower at C2; 128 9% slower)
-constexpr int kDefaultChunk = 512;
+// Queue entries per traversal wave64.  One-job waves of at least
+// kLargeWavePaths paths take kLargeWaveChunk (a 2^26-path wave of the 1.3M-
+// triangle scene: 256 and 1024 measured 2% and 4% slower, 128 9%); smaller
+// and multi-job waves kDefaultChunk (jobs of 65k-67M paths on 1k-500k
+// triangles rendered one at a time: 512 for all of them 9% slower).
+constexpr int kDefaultChunk = 256;
+constexpr int kLargeWaveChunk = 512;
+constexpr int kLargeWavePaths = 1 << 25;
 constexpr int kShadeThreads = 256;        // threads per path-shading block
 
 // Wave launchers: `jobs` is the device job table, `multi` selects the

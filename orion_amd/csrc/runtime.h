@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <string>
 
 #include "../../include/orion_amd.h"
@@ -58,6 +60,9 @@ struct orion_ctx_s {
   // this device's copy of the allocation-free intersection's scratch array
   // (its device address, from hipGetSymbolAddress at context creation)
   int *scratch = nullptr;
+  // scenes and servers created on this context and not yet destroyed: the
+  // context refuses to go before them (they use its stream and runner)
+  std::atomic<int> children{0};
 };
 
 struct orion_scene_s {

@@ -189,15 +189,26 @@ int orion_ctx_create(int device, orion_ctx *out) {
     set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
     return ORION_ERR_HIP;
   }
-  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIP_TRY(hipEventCreate(&c->ev0));
-  HIP_TRY(hipEventCreate(&c->ev1));
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) {
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    set_error(std::string("context stream / events: ") + hipGetErrorString(e));
+    return ORION_ERR_HIP;
+  }
   *out = c;
   return ORION_OK;
 }
 
 int orion_ctx_destroy(orion_ctx c) {
   if (!c) return ORION_OK;
+  if (c->children.load() > 0) {
+    set_error("context still has scenes or servers: destroy them first");
+    return ORION_ERR_BUSY;
+  }
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   orion_amd::destroy_runner(c->runner);
@@ -388,6 +399,7 @@ int orion_scene_upload(orion_ctx c, orion_hscene hs, orion_scene *out) {
   std::memcpy(d.cam, s.camera, sizeof(d.cam));
   sc->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   sc->upload_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  c->children.fetch_add(1);
   *out = sc;
   return ORION_OK;
 }
@@ -397,6 +409,7 @@ int orion_scene_destroy(orion_scene s) {
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
   if (s->ctx->runner) forget_scene(s->ctx->runner, s);
+  s->ctx->children.fetch_sub(1);
   delete s;
   return ORION_OK;
 }

@@ -209,3 +209,22 @@ def test_autotune(device, scenes):
     sc.film_clear()
     sc.render_path(0, 4, 8, tuning=best)
     np.testing.assert_array_equal(bits(plain), bits(sc.film()[0]))
+
+
+def test_context_outlives_its_scenes_and_servers():
+    """A context refuses to go (ORION_ERR_BUSY) while a scene or a server made
+    on it is alive -- they use its stream and wave runner -- and goes once
+    they are destroyed; the scene keeps rendering meanwhile."""
+    dev = oa.Device(0)
+    ds = oa.DeviceScene(dev, oa.HostScene.generate("cornell", params=[32, 32]))
+    srv = oa.Server(dev, 1 << 30)
+    L = oa.lib()
+    assert L.orion_ctx_destroy(dev.h) == -7  # ORION_ERR_BUSY
+    srv.close()
+    assert L.orion_ctx_destroy(dev.h) == -7
+    ds.film_clear()
+    st = ds.render_path(0, 2, 3)
+    assert st.paths == 2 * 32 * 32
+    ds.close()
+    dev.close()
+    assert not dev.h

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_photons(const Job *jobs, 
 // the other way round).  n_closest is a multiple of the XCD count, so a
 // block's XCD is the same in its part's numbering.
 template <int RING, bool MULTI>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_pair(const Job *jobs, Wave w, int chunk, int n_closest) {
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace_pair(const Job *jobs, Wave w, int chunk, int n_closest) {
   __shared__ int ring[trace_lds_ints<RING>()];
   const int b = blockIdx.x;
   if (b < n_closest) trace_block<false, RING, MULTI>(jobs, w, chunk, 0, -1, xcd_remap(b, n_closest), ring);

@@ -511,21 +511,27 @@ void build_bvh(Scene &sc) {
 
 // ---------------------------------------------------------------- triangle test
 // The closest-hit contract this build defines for rtcIntersect
-// (core/kernel.cpp:111-125): Moller-Trumbore in float without contraction,
-// accept tmin < t <= tmax, no culling, closest = lexicographic min (t, prim).
+// (core/kernel.cpp:111-125): Moller-Trumbore in float, its cross and dot
+// products with explicit fused multiply-adds (fmaf rounds once) and nothing
+// else contracted; accept tmin < t <= tmax, no culling, closest =
+// lexicographic min (t, prim).
+inline V3 cross_fma(V3 a, V3 b) {
+  return mk(std::fmaf(a.y, b.z, -(a.z * b.y)), std::fmaf(a.z, b.x, -(a.x * b.z)), std::fmaf(a.x, b.y, -(a.y * b.x)));
+}
+inline float dot_fma(V3 a, V3 b) { return std::fmaf(a.x, b.x, std::fmaf(a.y, b.y, a.z * b.z)); }
 inline bool tri_test(const Tri &tr, const Ray &r, float *t, float *u, float *v) {
   V3 e1 = sub(tr.p1, tr.p0), e2 = sub(tr.p2, tr.p0);
-  V3 pv = cross(r.d, e2);
-  float det = dot(e1, pv);
+  V3 pv = cross_fma(r.d, e2);
+  float det = dot_fma(e1, pv);
   if (det == 0.0f || det != det) return false;
   float inv = 1.0f / det;
   V3 tv = sub(r.o, tr.p0);
-  float uu = dot(tv, pv) * inv;
+  float uu = dot_fma(tv, pv) * inv;
   if (!(uu >= 0.0f && uu <= 1.0f)) return false;
-  V3 qv = cross(tv, e1);
-  float vv = dot(r.d, qv) * inv;
+  V3 qv = cross_fma(tv, e1);
+  float vv = dot_fma(r.d, qv) * inv;
   if (!(vv >= 0.0f && uu + vv <= 1.0f)) return false;
-  float tt = dot(e2, qv) * inv;
+  float tt = dot_fma(e2, qv) * inv;
   if (!(tt > r.tmin && tt <= r.tmax)) return false;
   *t = tt; *u = uu; *v = vv;
   return true;

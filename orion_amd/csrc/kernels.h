@@ -32,8 +32,10 @@ void launch_raygen(const Job *jobs, const Wave &w, bool multi, hipStream_t st);
 void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk, bool multi, hipStream_t st,
                   int cps = 0, int rec = -1);
 // an SPPM photon bounce (closest hit, trace_grid(w.nseg, w.seg, false,
-// chunk) blocks; counted apart from camera and path rays)
-void launch_trace_photons(const Job *jobs, const Wave &w, int ring, int chunk, hipStream_t st);
+// chunk) blocks, or w.nseg * cps with cps > 0; counted apart from camera and
+// path rays); rec >= 0: record the queue's length in w.segmax slot rec
+void launch_trace_photons(const Job *jobs, const Wave &w, int ring, int chunk, hipStream_t st, int cps = 0,
+                          int rec = -1);
 // closest-hit and any-hit queues in one launch (trace_pair_grid blocks)
 void launch_trace_pair(const Job *jobs, const Wave &w, int ring, int chunk, bool multi, hipStream_t st);
 int trace_pair_grid(int nseg, int seg, int chunk);

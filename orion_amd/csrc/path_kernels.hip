@@ -152,9 +152,9 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))
 // An SPPM photon bounce's closest-hit traversal: the same engine, its own
 // counters and kernel (profiles attribute it apart from the path tracer's).
 template <int RING>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_photons(const Job *jobs, Wave w, int chunk) {
+__global__ __launch_bounds__(kTraceBlock) void k_trace_photons(const Job *jobs, Wave w, int chunk, int cps, int rec) {
   __shared__ int ring[trace_lds_ints<RING>()];
-  trace_block<false, RING, false, true>(jobs, w, chunk, 0, -1, xcd_remap(blockIdx.x, gridDim.x), ring);
+  trace_block<false, RING, false, true>(jobs, w, chunk, cps, rec, xcd_remap(blockIdx.x, gridDim.x), ring);
 }
 
 // Closest-hit and any-hit queues of a bounce in one launch: blocks
@@ -526,11 +526,13 @@ void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk,
     else ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace<false, ST, false>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk, cps, rec));
   }
 }
-void launch_trace_photons(const Job *jobs, const Wave &w, int ring, int chunk, hipStream_t st) {
+void launch_trace_photons(const Job *jobs, const Wave &w, int ring, int chunk, hipStream_t st, int cps, int rec) {
   ring = ring_variant(ring);
   chunk = trace_chunk(chunk);
-  const dim3 g(trace_grid(w.nseg, w.seg, false, chunk));
-  ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace_photons<ST>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk));
+  const int full = (w.seg + chunk - 1) / chunk;
+  if (cps <= 0 || cps > full) cps = full;
+  const dim3 g((unsigned)std::max(1, w.nseg * cps));
+  ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace_photons<ST>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk, cps, rec));
 }
 void launch_between(const Wave &w, int nxt, hipStream_t st) {
   hipLaunchKernelGGL(k_between, dim3(1), dim3(64), 0, st, w, nxt);

@@ -88,8 +88,20 @@ struct SppmBuffers {
   hipStream_t cam_stream = nullptr;
   hipEvent_t cam_done[2] = {nullptr, nullptr}, iter_done = nullptr;
   std::vector<hipEvent_t> iter_end;  // end of iteration i: iter_end[i % nsets]
+  // Photon traversal grids: each bounce's queue length is recorded on the
+  // device (striped maxima), copied to pinned memory after the batch, and
+  // sizes that bounce's grid in the next batch of the same photon count
+  // (waves stride over the queue's chunks, so any grid traces every photon).
+  DevBuf segmax;
+  unsigned long long *h_segmax = nullptr;
+  hipEvent_t segmax_copied = nullptr;
+  int copied_P = 0;          // photons of the batch whose counts are in flight to h_segmax (0: none)
+  int plan_P = 0;            // photons of the batch the plan comes from
+  std::vector<int> plan;     // per bounce: photons queued
   ~SppmBuffers() {
     if (cam_stream) { (void)hipStreamSynchronize(cam_stream); (void)hipStreamDestroy(cam_stream); }
+    if (segmax_copied) { (void)hipEventSynchronize(segmax_copied); (void)hipEventDestroy(segmax_copied); }
+    if (h_segmax) (void)hipHostFree(h_segmax);
     for (hipEvent_t e : cam_done)
       if (e) (void)hipEventDestroy(e);
     if (iter_done) (void)hipEventDestroy(iter_done);
@@ -1123,12 +1135,34 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   // deposits): photon p of the batch is photon p % photons of iteration
   // i0 + p / photons, sample index nIterations * nPixels + i0 * photons + p
   // (sppm.cpp:203)
+  if ((rc = b.segmax.ensure(kSegMaxBytes))) return rc;
+  if (!b.h_segmax) HIP_TRY(hipHostMalloc((void **)&b.h_segmax, kSegMaxBytes, hipHostMallocDefault));
+  if (!b.segmax_copied) HIP_TRY(hipEventCreateWithFlags(&b.segmax_copied, hipEventDisableTiming));
+  const int nrec = std::min(maxDepth, kSegMaxBounces);
   auto photon_batch = [&](int i0, int nb) -> int {
     const int P = nb * photons;
     Wave pw = w;
     pw.N = P;
     pw.seg = P;
     pw.nseg = 1;
+    pw.segmax = (unsigned long long *)b.segmax.ptr;
+    // the previous batch's per-bounce photon counts (copied a batch ago, so
+    // the event has long completed) become this size's plan
+    if (b.copied_P > 0) {
+      HIP_TRY(hipEventSynchronize(b.segmax_copied));
+      b.plan.assign(nrec, 0);
+      for (int d = 0; d < nrec; ++d)
+        for (int k = 0; k < kStatStripes; ++k)
+          b.plan[d] = std::max(b.plan[d], (int)std::min<unsigned long long>(b.h_segmax[(size_t)d * kStatStripes + k], 1u << 30));
+      b.plan_P = b.copied_P;
+      b.copied_P = 0;
+    }
+    auto plan_cps = [&](int d) -> int {  // chunk positions for bounce d: its count + a quarter + 256
+      if (b.plan_P != P || d >= (int)b.plan.size()) return 0;
+      const int64_t need = (int64_t)b.plan[d] + b.plan[d] / 4 + 256;
+      return (int)std::max<int64_t>(1, (need + chunk - 1) / chunk);
+    };
+    HIP_TRY(hipMemsetAsync(b.segmax.ptr, 0, (size_t)nrec * kStatStripes * 8, st));
     const uint32_t index0 = (uint32_t)(n_iters * npix + i0 * photons);
     HIP_TRY(hipMemsetAsync(ph.cnt, 0, (size_t)nb * 4, st));
     HIP_TRY(hipMemsetAsync(w.ctrl + C_PQ, 0, 4, st));
@@ -1141,12 +1175,15 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
       tw.cnt_ray = w.ctrl + C_PQ + depth % 3;
       int rc2;
       if ((rc2 = ptimer.mark(0))) return rc2;
-      launch_trace_photons(dj, tw, ring, chunk, st);
+      launch_trace_photons(dj, tw, ring, chunk, st, plan_cps(depth), depth < nrec ? depth : -1);
       if ((rc2 = ptimer.mark(0)) || (rc2 = ptimer.mark(1))) return rc2;
       hipLaunchKernelGGL(k_photon_bounce, dim3(blocks), dim3(256), 0, st, s->d, tw, ph, index0, depth,
                          qbuf[depth & 1], qbuf[(depth + 1) & 1], dstats);
       if ((rc2 = ptimer.mark(1))) return rc2;
     }
+    HIP_TRY(hipMemcpyAsync(b.h_segmax, b.segmax.ptr, (size_t)nrec * kStatStripes * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(b.segmax_copied, st));
+    b.copied_P = P;
     return ORION_OK;
   };
   if ((rc = camera_batch(0, Bc))) return rc;

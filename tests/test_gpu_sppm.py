@@ -99,3 +99,12 @@ def test_sppm_c3_sorted_pairs_and_cell_ranges(device, oracle):
     starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
     want = np.stack([np.where(counts > 0, starts, 0), np.where(counts > 0, starts + counts, 0)], 1)
     np.testing.assert_array_equal(gd["cell_range"], want)
+
+
+def test_sppm_planned_photon_grids(device, oracle):
+    """Sixteen iterations of 2^19 photons: two photon batches of 8
+    iterations, the second tracing each bounce on a grid sized from the
+    first batch's photon counts (its waves stride over the queue's chunks);
+    iteration 15's grid, deposits and M equal the oracle's bit for bit."""
+    st = _compare(device, oracle, [5, 96, 96], 16, 1 << 19, 0.02, 0, 15)
+    assert st.photon_batch == 8

@@ -36,8 +36,12 @@ void launch_trace(const Job *jobs, const Wave &w, bool any, int ring, int chunk,
 // path rays); rec >= 0: record the queue's length in w.segmax slot rec
 void launch_trace_photons(const Job *jobs, const Wave &w, int ring, int chunk, hipStream_t st, int cps = 0,
                           int rec = -1);
-// closest-hit and any-hit queues in one launch (trace_pair_grid blocks)
-void launch_trace_pair(const Job *jobs, const Wave &w, int ring, int chunk, bool multi, hipStream_t st);
+// closest-hit and any-hit queues in one launch (trace_pair_grid blocks, or
+// fewer with cps_c / cps_a > 0 chunk positions per segment for the two
+// queues); rec >= 0: record the two queues' lengths in w.segmax slots 2 rec
+// and 2 rec + 1
+void launch_trace_pair(const Job *jobs, const Wave &w, int ring, int chunk, bool multi, hipStream_t st,
+                       int cps_c = 0, int cps_a = 0, int rec = -1);
 int trace_pair_grid(int nseg, int seg, int chunk);
 int ring_variant(int ring);
 int trace_chunk(int chunk);

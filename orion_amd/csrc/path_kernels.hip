@@ -163,11 +163,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_photons(const Job *jobs, 
 // the other way round).  n_closest is a multiple of the XCD count, so a
 // block's XCD is the same in its part's numbering.
 template <int RING, bool MULTI>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace_pair(const Job *jobs, Wave w, int chunk, int n_closest) {
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_trace_pair(
+    const Job *jobs, Wave w, int chunk, int n_closest, int cps_c, int cps_a, int rec) {
   __shared__ int ring[trace_lds_ints<RING>()];
   const int b = blockIdx.x;
-  if (b < n_closest) trace_block<false, RING, MULTI>(jobs, w, chunk, 0, -1, xcd_remap(b, n_closest), ring);
-  else trace_block<true, RING, MULTI>(jobs, w, chunk, 0, -1, xcd_remap(b - n_closest, gridDim.x - n_closest), ring);
+  if (b < n_closest)
+    trace_block<false, RING, MULTI>(jobs, w, chunk, cps_c, rec >= 0 ? 2 * rec : -1, xcd_remap(b, n_closest), ring);
+  else
+    trace_block<true, RING, MULTI>(jobs, w, chunk, cps_a, rec >= 0 ? 2 * rec + 1 : -1,
+                                   xcd_remap(b - n_closest, gridDim.x - n_closest), ring);
 }
 
 __global__ void k_between(Wave w, int nxt) {
@@ -497,13 +501,21 @@ void launch_raygen(const Job *jobs, const Wave &w, bool multi, hipStream_t st) {
   if (multi) hipLaunchKernelGGL(k_raygen<true>, g, dim3(256), 0, st, jobs, w);
   else hipLaunchKernelGGL(k_raygen<false>, g, dim3(256), 0, st, jobs, w);
 }
-void launch_trace_pair(const Job *jobs, const Wave &w, int ring, int chunk, bool multi, hipStream_t st) {
+void launch_trace_pair(const Job *jobs, const Wave &w, int ring, int chunk, bool multi, hipStream_t st, int cps_c,
+                       int cps_a, int rec) {
   ring = ring_variant(ring);
   chunk = trace_chunk(chunk);
-  const int nc = (trace_grid(w.nseg, w.seg, false, chunk) + kXcds - 1) / kXcds * kXcds;
-  const dim3 g(nc + trace_grid(w.nseg, w.seg, true, chunk));
-  if (multi) ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace_pair<ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk, nc));
-  else ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace_pair<ST, false>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk, nc));
+  const int full_c = (w.seg + chunk - 1) / chunk, full_a = (int)((2 * (int64_t)w.seg + chunk - 1) / chunk);
+  if (cps_c <= 0 || cps_c > full_c) cps_c = full_c;
+  if (cps_a <= 0 || cps_a > full_a) cps_a = full_a;
+  const int nc = (std::max(1, w.nseg * cps_c) + kXcds - 1) / kXcds * kXcds;
+  const dim3 g(nc + std::max(1, w.nseg * cps_a));
+  if (multi)
+    ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace_pair<ST, true>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk,
+                                                 nc, cps_c, cps_a, rec));
+  else
+    ORION_RING_DISPATCH(ring, hipLaunchKernelGGL((k_trace_pair<ST, false>), g, dim3(kTraceBlock), 0, st, jobs, w, chunk,
+                                                 nc, cps_c, cps_a, rec));
 }
 
 int trace_pair_grid(int nseg, int seg, int chunk) {

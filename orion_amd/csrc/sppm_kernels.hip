@@ -98,10 +98,23 @@ struct SppmBuffers {
   int copied_P = 0;          // photons of the batch whose counts are in flight to h_segmax (0: none)
   int plan_P = 0;            // photons of the batch the plan comes from
   std::vector<int> plan;     // per bounce: photons queued
+  // the same for the camera passes' paired traversal, one set per camera
+  // batch in flight (two alternate); per bounce: closest-hit and any-hit
+  // queue lengths
+  DevBuf segmax_cam[2];
+  unsigned long long *h_segmax_cam[2] = {nullptr, nullptr};
+  hipEvent_t cam_copied[2] = {nullptr, nullptr};
+  int cam_copied_N[2] = {0, 0};
+  int cam_plan_N = 0;
+  std::vector<int> cam_plan;
   ~SppmBuffers() {
     if (cam_stream) { (void)hipStreamSynchronize(cam_stream); (void)hipStreamDestroy(cam_stream); }
     if (segmax_copied) { (void)hipEventSynchronize(segmax_copied); (void)hipEventDestroy(segmax_copied); }
     if (h_segmax) (void)hipHostFree(h_segmax);
+    for (int k = 0; k < 2; ++k) {
+      if (cam_copied[k]) { (void)hipEventSynchronize(cam_copied[k]); (void)hipEventDestroy(cam_copied[k]); }
+      if (h_segmax_cam[k]) (void)hipHostFree(h_segmax_cam[k]);
+    }
     for (hipEvent_t e : cam_done)
       if (e) (void)hipEventDestroy(e);
     if (iter_done) (void)hipEventDestroy(iter_done);
@@ -1110,17 +1123,45 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   HIP_TRY(hipEventRecord(b.iter_done, st));
   // camera passes of iterations [c0, c0 + nb): they overwrite the sets of
   // iterations c0 - nsets .. c0 + nb - 1 - nsets, so wait for the last of those
+  for (int k = 0; k < 2; ++k) {
+    if ((rc = b.segmax_cam[k].ensure(kSegMaxBytes))) return rc;
+    if (!b.h_segmax_cam[k]) HIP_TRY(hipHostMalloc((void **)&b.h_segmax_cam[k], kSegMaxBytes, hipHostMallocDefault));
+    if (!b.cam_copied[k]) HIP_TRY(hipEventCreateWithFlags(&b.cam_copied[k], hipEventDisableTiming));
+  }
+  const int ncam = std::min(maxDepth + 1, kSegMaxBounces);  // camera bounces whose queue lengths are recorded
   auto camera_batch = [&](int c0, int nb) -> int {
     const int e = c0 + nb - 1 - nsets;
     HIP_TRY(hipStreamWaitEvent(cst, e >= 0 ? b.iter_end[e % nsets] : b.iter_done, 0));
     Wave cb = cw;
     cb.N = nb * npix;
     cb.seg = nb * npix;
+    // traversal grids sized from the queue lengths of the camera batch two
+    // back (the one that used this set of counters; the main stream has
+    // waited for it before the host got here, so its copy is complete)
+    const int slot = (c0 / Bc) & 1;
+    if (b.cam_copied_N[slot] > 0) {
+      HIP_TRY(hipEventSynchronize(b.cam_copied[slot]));
+      b.cam_plan.assign(2 * ncam, 0);
+      for (int q = 0; q < 2 * ncam; ++q)
+        for (int k = 0; k < kStatStripes; ++k)
+          b.cam_plan[q] = std::max(b.cam_plan[q], (int)std::min<unsigned long long>(
+                                                      b.h_segmax_cam[slot][(size_t)q * kStatStripes + k], 1u << 30));
+      b.cam_plan_N = b.cam_copied_N[slot];
+      b.cam_copied_N[slot] = 0;
+    }
+    auto cam_cps = [&](int q) -> int {  // chunk positions for queue q: its length + a quarter + 256
+      if (b.cam_plan_N != cb.N || q >= (int)b.cam_plan.size()) return 0;
+      const int64_t need = (int64_t)b.cam_plan[q] + b.cam_plan[q] / 4 + 256;
+      return (int)std::max<int64_t>(1, (need + chunk - 1) / chunk);
+    };
+    cb.segmax = (unsigned long long *)b.segmax_cam[slot].ptr;
+    HIP_TRY(hipMemsetAsync(b.segmax_cam[slot].ptr, 0, (size_t)2 * ncam * kStatStripes * 8, cst));
     hipLaunchKernelGGL(k_cam_raygen, dim3((cb.N + TB - 1) / TB), dim3(TB), 0, cst, s->d, cb, ct, c0);
     int cur = 0;
     const int blocks = std::min((cb.N + 255) / 256, 8192);
     for (int it = 0; it <= maxDepth; ++it) {
-      launch_trace_pair(dj, cb, ring, chunk, false, cst);  // closest-hit + any-hit queues in one launch
+      // closest-hit + any-hit queues in one launch
+      launch_trace_pair(dj, cb, ring, chunk, false, cst, cam_cps(2 * it), cam_cps(2 * it + 1), it < ncam ? it : -1);
       launch_between(cb, cur ^ 1, cst);
       hipLaunchKernelGGL(k_cam_shade, dim3(blocks), dim3(256), 0, cst, s->d, cb, b.px[0], ct, cur, maxDepth, c0,
                          nsets, dstats);
@@ -1128,6 +1169,10 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     }
     hipLaunchKernelGGL(k_cam_replay, dim3((npix + TB - 1) / TB), dim3(TB), 0, cst, b.px[0], ct, b.ld_snap, npix, nb,
                        c0, nsets);
+    HIP_TRY(hipMemcpyAsync(b.h_segmax_cam[slot], b.segmax_cam[slot].ptr, (size_t)2 * ncam * kStatStripes * 8,
+                           hipMemcpyDeviceToHost, cst));
+    HIP_TRY(hipEventRecord(b.cam_copied[slot], cst));
+    b.cam_copied_N[slot] = cb.N;
     HIP_TRY(hipEventRecord(b.cam_done[(c0 / Bc) & 1], cst));
     return ORION_OK;
   };

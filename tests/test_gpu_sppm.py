@@ -101,10 +101,13 @@ def test_sppm_c3_sorted_pairs_and_cell_ranges(device, oracle):
     np.testing.assert_array_equal(gd["cell_range"], want)
 
 
-def test_sppm_planned_photon_grids(device, oracle):
-    """Sixteen iterations of 2^19 photons: two photon batches of 8
-    iterations, the second tracing each bounce on a grid sized from the
-    first batch's photon counts (its waves stride over the queue's chunks);
-    iteration 15's grid, deposits and M equal the oracle's bit for bit."""
-    st = _compare(device, oracle, [5, 96, 96], 16, 1 << 19, 0.02, 0, 15)
+def test_sppm_planned_traversal_grids(device, oracle):
+    """Twenty-four iterations of 2^19 photons at 96^2: three photon batches
+    and three camera batches of 8 iterations each.  The later photon batches
+    trace each bounce on a grid sized from the previous batch's photon
+    counts, and the third camera batch on grids sized from the first one's
+    queue lengths (waves stride over their queue's chunks, so any grid traces
+    every ray); iteration 20's grid, deposits and M equal the oracle's bit
+    for bit."""
+    st = _compare(device, oracle, [5, 96, 96], 24, 1 << 19, 0.02, 0, 20)
     assert st.photon_batch == 8

@@ -116,6 +116,10 @@ struct WaveRunner {
   DevBuf jobs_dev, ctrl, stats, spill, segmax;
   GridPlan plan;
   bool planned = false;   // the current call's one-job waves use `plan`
+  // a plan the caller made for the next wave (the server: from the previous
+  // wave when it held the same jobs); used when its segment size, depth and
+  // chunk match the wave's
+  const GridPlan *wave_plan = nullptr;
   bool one_wave = false;  // ... or one wave per queue segment (orion_tuning.trace_plan 2)
   int spill_cap = 0;
   Job *h_jobs = nullptr;
@@ -231,23 +235,29 @@ struct WaveRunner {
   }
   // After a one-job call (stream synchronised): the longest segments its
   // waves recorded become the plan for the next call of that shape.
-  int record_plan(const void *scene, int seg, int depth, int chunk) {
+  // The longest queue segments the last waves recorded (stream synchronised)
+  // as a plan for waves of this segment size, depth and chunk.
+  int read_plan(GridPlan *gp, int seg, int depth, int chunk) {
     std::vector<unsigned long long> h((size_t)kSegMaxBounces * 2 * kStatStripes);
     HIP_TRY(hipMemcpy(h.data(), segmax.ptr, kSegMaxBytes, hipMemcpyDeviceToHost));
     const int nb = std::min(depth + 1, kSegMaxBounces);
     for (int q = 0; q < 2; ++q) {
-      plan.longest[q].assign(nb, 0);
+      gp->longest[q].assign(nb, 0);
       for (int b = 0; b < nb; ++b) {
         unsigned long long m = 0;
         for (int k = 0; k < kStatStripes; ++k) m = std::max(m, h[((size_t)b * 2 + q) * kStatStripes + k]);
-        plan.longest[q][b] = (int)std::min<unsigned long long>(m, 1u << 30);
+        gp->longest[q][b] = (int)std::min<unsigned long long>(m, 1u << 30);
       }
     }
-    plan.scene = scene;
-    plan.seg = seg;
-    plan.depth = depth;
-    plan.chunk = chunk;
+    gp->seg = seg;
+    gp->depth = depth;
+    gp->chunk = chunk;
     return ORION_OK;
+  }
+  int record_plan(const void *scene, int seg, int depth, int chunk) {
+    int rc = read_plan(&plan, seg, depth, chunk);
+    plan.scene = scene;
+    return rc;
   }
   int read_stats(orion_render_stats *o) {
     unsigned long long h[S_COUNT];
@@ -280,16 +290,21 @@ struct WaveRunner {
     launch_raygen(dj, w, multi, st);
     Wave wa_base = w;
     wa_base.spill = w.spill + (size_t)w.spill_cap * trace_grid(w.nseg, w.seg, false, chunk) * kTraceBlock;
-    use_plan = use_plan && !multi && plan.seg == w.seg && plan.depth == depth && plan.chunk == chunk;
+    // the grid plan: the caller's for this wave, else (plain renders) the
+    // previous call's on the same scene
+    auto fits = [&](const GridPlan &g) { return g.seg == w.seg && g.depth == depth && g.chunk == chunk; };
+    const GridPlan *gp = nullptr;
+    if (use_plan && wave_plan && fits(*wave_plan)) gp = wave_plan;
+    else if (use_plan && planned && !multi && fits(plan)) gp = &plan;
     for (int it = 0; it <= depth; ++it) {
       Wave wi = w;  // bounce it traces the ray lengths of counter set it % 3
       wi.cnt_ray = w.cnt + (3 * (it % 3) + 1) * w.cnt_stride;
       wi.cnt_shadow = w.cnt + (3 * (it % 3) + 2) * w.cnt_stride;
-      // one-job waves record each bounce's longest segments for later plans
-      const int rec_c = !multi && it < kSegMaxBounces ? 2 * it : -1;
+      // every wave records each bounce's longest segments for later plans
+      const int rec_c = it < kSegMaxBounces ? 2 * it : -1;
       const int rec_a = rec_c >= 0 ? rec_c + 1 : -1;
-      int cps_c = use_plan ? plan.cps(it, 0, w.seg, trace_chunk(chunk)) : 0;
-      int cps_a = use_plan ? plan.cps(it, 1, 2 * w.seg, trace_chunk(chunk)) : 0;
+      int cps_c = gp ? gp->cps(it, 0, w.seg, trace_chunk(chunk)) : 0;
+      int cps_a = gp ? gp->cps(it, 1, 2 * w.seg, trace_chunk(chunk)) : 0;
       if (one_wave && !multi) cps_c = cps_a = 1;
       if (it == 0) {
         // camera rays only: no any-hit rays exist before the first shading
@@ -380,7 +395,7 @@ struct WaveRunner {
     w.seg_job = (const int *)segjob_dev.ptr;
     w.seg_perm = (const int *)segjob_dev.ptr + nseg;
     if (!use_graph || tm) {
-      enqueue_wave(dj, w, depth, ring, multi, chunk, split, st, tm, planned);
+      enqueue_wave(dj, w, depth, ring, multi, chunk, split, st, tm, true);
       HIP_TRY(hipGetLastError());
       return ORION_OK;
     }
@@ -838,6 +853,8 @@ void orion_server_s::loop() {
   int target = std::min(tn && tn->wave_paths > 0 ? tn->wave_paths : kDefaultWavePaths, kServeMaxWavePaths);
   int rr = 0;
   const int seg = seg_of(tn, target);
+  GridPlan wplan;                // per-bounce longest queue segments of the last wave
+  std::vector<int64_t> plan_sig;  // that wave's composition (empty: no plan)
   for (;;) {
     std::vector<ServerJob *> admitted;
     {
@@ -917,15 +934,33 @@ void orion_server_s::loop() {
     int wseg;
     std::vector<Job> jobs = compose_sized(act, target, seg, tn && tn->queue_segment > 0, policy, &rr, &npaths, &npix,
                                           &depth, &stack, &taken, &wseg);
+    // a wave with the same jobs, sample counts and layout as the previous
+    // one traces each bounce on grids sized from that wave's queues (a
+    // request's waves repeat until its last one)
+    std::vector<int64_t> sig;
+    for (const Job &J : jobs) {
+      sig.push_back((int64_t)(intptr_t)J.sc.nodes);
+      sig.push_back(((int64_t)J.W << 32) | (uint32_t)J.H);
+      sig.push_back(((int64_t)J.S << 32) | (uint32_t)J.maxDepth);
+      sig.push_back(J.path0);
+    }
+    sig.push_back(npaths);
+    sig.push_back(wseg);
+    const bool graph = tn && tn->use_graph;
+    runner.wave_plan = !graph && sig == plan_sig ? &wplan : nullptr;
     auto w0 = std::chrono::steady_clock::now();
     int rc = runner.init(std::max(1, (int)jobs.size()));
     if (!rc) rc = runner.reset_stats(stream);
     if (!rc && !jobs.empty())
       rc = runner.run(stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, ring_of(tn), stack,
-                      jobs.size() > 1, chunk_of(tn), split_of(tn), tn && tn->use_graph, nullptr, wseg);
+                      jobs.size() > 1, chunk_of(tn), split_of(tn), graph, nullptr, wseg);
+    runner.wave_plan = nullptr;
     if (!rc && hipStreamSynchronize(stream) != hipSuccess) rc = ORION_ERR_HIP;
     orion_render_stats ws{};
     if (!rc) rc = runner.read_stats(&ws);
+    plan_sig.clear();
+    if (!rc && !jobs.empty() && !graph && runner.read_plan(&wplan, wseg, depth, chunk_of(tn)) == ORION_OK)
+      plan_sig = sig;
     double wms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     std::vector<ServerJob *> finished;
     for (size_t i = 0; i < act.size(); ++i) {

@@ -1163,7 +1163,14 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
       // closest-hit + any-hit queues in one launch
       launch_trace_pair(dj, cb, ring, chunk, false, cst, cam_cps(2 * it), cam_cps(2 * it + 1), it < ncam ? it : -1);
       launch_between(cb, cur ^ 1, cst);
-      hipLaunchKernelGGL(k_cam_shade, dim3(blocks), dim3(256), 0, cst, s->d, cb, b.px[0], ct, cur, maxDepth, c0,
+      // the shading grid: the bounce's live paths are at most its closest-hit
+      // plus any-hit queue lengths (a grid-stride loop shades any count)
+      int cblocks = blocks;
+      if (b.cam_plan_N == cb.N && 2 * it + 1 < (int)b.cam_plan.size()) {
+        const int64_t np = (int64_t)b.cam_plan[2 * it] + b.cam_plan[2 * it + 1];
+        cblocks = (int)std::min<int64_t>(blocks, (np + np / 4 + 256 + 255) / 256);
+      }
+      hipLaunchKernelGGL(k_cam_shade, dim3(cblocks), dim3(256), 0, cst, s->d, cb, b.px[0], ct, cur, maxDepth, c0,
                          nsets, dstats);
       cur ^= 1;
     }
@@ -1222,7 +1229,12 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
       if ((rc2 = ptimer.mark(0))) return rc2;
       launch_trace_photons(dj, tw, ring, chunk, st, plan_cps(depth), depth < nrec ? depth : -1);
       if ((rc2 = ptimer.mark(0)) || (rc2 = ptimer.mark(1))) return rc2;
-      hipLaunchKernelGGL(k_photon_bounce, dim3(blocks), dim3(256), 0, st, s->d, tw, ph, index0, depth,
+      // the bounce's shading grid from the same plan (it loops over its queue
+      // with a grid stride, so any grid shades every photon)
+      const int pblocks = b.plan_P == P && depth < (int)b.plan.size()
+                              ? std::min(blocks, (b.plan[depth] + b.plan[depth] / 4 + 256 + 255) / 256)
+                              : blocks;
+      hipLaunchKernelGGL(k_photon_bounce, dim3(pblocks), dim3(256), 0, st, s->d, tw, ph, index0, depth,
                          qbuf[depth & 1], qbuf[(depth + 1) & 1], dstats);
       if ((rc2 = ptimer.mark(1))) return rc2;
     }

@@ -1160,8 +1160,15 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     int cur = 0;
     const int blocks = std::min((cb.N + 255) / 256, 8192);
     for (int it = 0; it <= maxDepth; ++it) {
-      // closest-hit + any-hit queues in one launch
-      launch_trace_pair(dj, cb, ring, chunk, false, cst, cam_cps(2 * it), cam_cps(2 * it + 1), it < ncam ? it : -1);
+      // closest-hit + any-hit queues in one launch; small planned queues with
+      // one ray per lane (64-entry chunks, grids within the full ones)
+      int ch = chunk, cc = cam_cps(2 * it), ca = cam_cps(2 * it + 1);
+      if (cc > 0 && ca > 0 && (int64_t)cc * chunk <= cb.seg / 2 && (int64_t)ca * chunk <= cb.seg) {
+        ch = 64;
+        cc = (int)((((int64_t)b.cam_plan[2 * it] + b.cam_plan[2 * it] / 4 + 256) + ch - 1) / ch);
+        ca = (int)((((int64_t)b.cam_plan[2 * it + 1] + b.cam_plan[2 * it + 1] / 4 + 256) + ch - 1) / ch);
+      }
+      launch_trace_pair(dj, cb, ring, ch, false, cst, cc, ca, it < ncam ? it : -1);
       launch_between(cb, cur ^ 1, cst);
       // the shading grid: the bounce's live paths are at most its closest-hit
       // plus any-hit queue lengths (a grid-stride loop shades any count)
@@ -1227,7 +1234,15 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
       tw.cnt_ray = w.ctrl + C_PQ + depth % 3;
       int rc2;
       if ((rc2 = ptimer.mark(0))) return rc2;
-      launch_trace_photons(dj, tw, ring, chunk, st, plan_cps(depth), depth < nrec ? depth : -1);
+      // a planned bounce of at most P / 2 photons traces one photon per lane
+      // (64-entry chunks: its grid stays within the 128-entry full grid the
+      // spill area holds); its latency, not its work, sets the launch's time
+      int ch = chunk, cps = plan_cps(depth);
+      if (cps > 0 && (int64_t)cps * chunk <= P / 2) {
+        ch = 64;
+        cps = (int)((((int64_t)b.plan[depth] + b.plan[depth] / 4 + 256) + ch - 1) / ch);
+      }
+      launch_trace_photons(dj, tw, ring, ch, st, cps, depth < nrec ? depth : -1);
       if ((rc2 = ptimer.mark(0)) || (rc2 = ptimer.mark(1))) return rc2;
       // the bounce's shading grid from the same plan (it loops over its queue
       // with a grid stride, so any grid shades every photon)

@@ -283,7 +283,10 @@ def sppm_leg(oa, dev, iters):
     iteration, initial radius 0.01, 2^22 hash cells, depth 10."""
     hs = oa.HostScene.generate("caustics", seed=1, params=[6, 512, 512])
     ds = oa.DeviceScene(dev, hs)
-    ds.render_sppm(1, 1 << 20, 0.01, 10, hash_size=1 << 22)  # warm
+    # warm-up: 8 iterations, two photon batches and two camera batches of the
+    # timed run's size, so its traversal grids are planned from its first
+    # batch on (as the path tracer's warm-up steps plan its grids)
+    ds.render_sppm(8, 1 << 20, 0.01, 10, hash_size=1 << 22)
     ds.film_clear()
     dev.synchronize()
     t0 = time.perf_counter()

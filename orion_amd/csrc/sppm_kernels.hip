@@ -77,6 +77,7 @@ struct Grid {
 struct SppmBuffers {
   DevBuf pix_mem, grid_mem, pair_mem, hist_mem, scan_mem, job, spill, spill_cam;
   DevBuf hits;       // photon hits of a batch of iterations, waiting for their deposits
+  DevBuf pq;         // photon queue lengths: 3 rotating sets of one counter per queue segment
   DevBuf cam_terms;  // a camera batch's recorded radiance terms
   WaveBuffers camw;
   std::vector<Pix> px;
@@ -202,6 +203,7 @@ __global__ void k_cam_raygen(DScene sc, Wave w, CamTerms ct, int c0) {
 // One camera-pass step (sppm.cpp:93-138) of a batch; px is visible-point set 0.
 __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, CamTerms ct, int cur, int maxDepth,
                                                    int c0, int nsets, unsigned long long *stats) {
+  __shared__ int lds[4 * 5];
   const int nxt = cur ^ 1;
   const int count = w.ctrl[C_PATH0 + cur];
   const int N = w.N;
@@ -295,14 +297,13 @@ __global__ __launch_bounds__(256) void k_cam_shade(DScene sc, Wave w, Pix px, Ca
       w.st[p] = make_float4(beta.r, beta.g, beta.b, path_bits(fl, depth, dim));
       keep = (fl & (F_ALIVE | F_PEND)) != 0;
     }
-    int a = wave_append(&w.ctrl[C_PATH0 + nxt], keep);
-    if (keep) w.qpath[nxt][a] = p;
-    a = wave_append(&w.ctrl[C_RAY], want_ray0);
-    if (want_ray0) w.qray[a] = p;
-    a = wave_append(&w.ctrl[C_SHADOW], want_shadow);
-    if (want_shadow) w.qshadow[a] = p;
-    a = wave_append(&w.ctrl[C_SHADOW], want_ray1);
-    if (want_ray1) w.qshadow[a] = N + p;
+    int a[4];
+    block_append4(&w.ctrl[C_PATH0 + nxt], &w.ctrl[C_RAY], &w.ctrl[C_SHADOW], &w.ctrl[C_SHADOW], keep, want_ray0,
+                  want_shadow, want_ray1, lds, a);
+    if (keep) w.qpath[nxt][a[0]] = p;
+    if (want_ray0) w.qray[a[1]] = p;
+    if (want_shadow) w.qshadow[a[2]] = p;
+    if (want_ray1) w.qshadow[a[3]] = N + p;
   }
   if (lost) stat_add(stats, S_OVERFLOW, lost);
 }
@@ -605,16 +606,24 @@ __device__ __forceinline__ bool photon_emit(DScene sc, Wave w, uint32_t index0, 
   return alive;
 }
 
-// Emits photons [0, P) into queue 0 (counter C_PQ + 0, zeroed by the host
-// beforehand) and zeroes counter C_PQ + 1 for the first bounce.
-__global__ __launch_bounds__(256) void k_photon_init(DScene sc, Wave w, uint32_t index0, int P) {
+// Photon queues are cut into segments of kPhotonSeg entries, each with its
+// own length counter: photon p lives in segment p / kPhotonSeg at every
+// bounce, so a block appends to its segment's counter and no counter is
+// shared by more than kPhotonSeg / 256 blocks (one counter for the whole
+// queue had every block of a launch queue its atomic on one address).
+// Set k of the rotating counters is pq + k * nseg.
+constexpr int kPhotonSeg = 16384;
+
+// Emits photons [0, P) into queue 0 (counter set 0, zeroed by the host
+// beforehand with the other two sets).
+__global__ __launch_bounds__(256) void k_photon_init(DScene sc, Wave w, uint32_t index0, int P, int *pq) {
   __shared__ int lds[5];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) w.ctrl[C_PQ + 1] = 0;
+  const int sg = (blockIdx.x * blockDim.x) / kPhotonSeg;  // blocks do not straddle segments
   bool alive = false;
   if (p < P) alive = photon_emit(sc, w, index0, p);
-  int a = block_append(&w.ctrl[C_PQ], alive, lds);
-  if (alive) w.qray[a] = p;
+  int a = block_append(&pq[sg], alive, lds);
+  if (alive) w.qray[(size_t)sg * kPhotonSeg + a] = p;
 }
 
 // Photon deposits of one wave (sppm.cpp:230-257), balanced over its lanes.
@@ -687,32 +696,37 @@ struct PhotonHits {
 // One photon bounce (sppm.cpp:225-271) of a batch: the hit is recorded for
 // its iteration's deposits (bounces after the first), then the BSDF is
 // sampled and the photon continues or ends by Russian roulette.  Bounce d
-// reads the queue just traced (qcur, count in C_PQ + d % 3) and appends its
-// survivors to qnext (count in C_PQ + (d + 1) % 3, zeroed by the bounce
-// before); it zeroes C_PQ + (d + 2) % 3 for the bounce after.  The host
-// alternates the two queue buffers, so no launch sits between a bounce and
-// the next trace.
+// reads the queue just traced (qcur, lengths in counter set d % 3) and
+// appends its survivors to qnext (set (d + 1) % 3, zeroed before); it
+// zeroes set (d + 2) % 3 for the bounce after.  The bps blocks of segment
+// sg shade its entries with a stride of bps * 256.  The host alternates the
+// two queue buffers, so no launch sits between a bounce and the next trace.
 __global__ __launch_bounds__(256) void k_photon_bounce(DScene sc, Wave w, PhotonHits ph, uint32_t index0, int depth,
-                                                       const int *qcur, int *qnext, unsigned long long *stats) {
+                                                       const int *qcur, int *qnext, int *pq, int nseg, int bps,
+                                                       unsigned long long *stats) {
   __shared__ int lds[5];
-  int *const cnt = w.ctrl + C_PQ;
-  const int count = cnt[depth % 3];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    cnt[(depth + 2) % 3] = 0;
+  __shared__ int rslot[kPhotonBatchMax];  // per iteration: the block's record count, then its first slot
+  const int sg = blockIdx.x / bps, part = blockIdx.x - sg * bps;
+  if (sg >= nseg) return;  // the whole block
+  const int count = pq[(depth % 3) * nseg + sg];
+  if (part == 0 && threadIdx.x == 0) {
+    pq[((depth + 2) % 3) * nseg + sg] = 0;
     stat_add(stats, S_CLOSEST, (unsigned long long)count);
     stat_add(stats, S_PHOTON_RAYS, (unsigned long long)count);
   }
+  int *const nxt = &pq[((depth + 1) % 3) * nseg + sg];
+  const size_t q0 = (size_t)sg * kPhotonSeg;
   const int lane = lane_id();
   const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
   unsigned long long nrec = 0, ovf = 0;
-  for (int base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x) {
+  for (int base = part * blockDim.x; base < count; base += bps * blockDim.x) {
     int i = base + threadIdx.x;
     bool keep = false, rec = false;
     int p = 0, jb = 0;
     V3 hpos = mk(0, 0, 0), d = mk(0, 0, 0);
     Sp beta = sp(0);
     if (i < count) {
-      p = qcur[i];
+      p = qcur[q0 + i];
       const float4 H = w.hit[p];
       const int hp = __float_as_int(H.w);
       if (hp >= 0) {
@@ -746,31 +760,45 @@ __global__ __launch_bounds__(256) void k_photon_bounce(DScene sc, Wave w, Photon
         }
       }
     }
-    // record the hits, one atomic per iteration present in the wave
+    // record the hits: slots counted per iteration in LDS, then one global
+    // atomic per iteration present in the block (the four to sixteen
+    // per-iteration counters are the same few addresses for every block, so
+    // atomics per wave would queue behind each other in L2)
+    if (threadIdx.x < kPhotonBatchMax) rslot[threadIdx.x] = 0;
+    __syncthreads();
+    int at = 0;
     uint64_t left = __ballot(rec);
     while (left) {
       const int leader = __ffsll((long long)left) - 1;
       const int j = __shfl(jb, leader);
       const uint64_t mj = __ballot(rec && jb == j);
-      int at = 0;
-      if (lane == leader) at = atomicAdd(&ph.cnt[j], __popcll(mj));
-      at = __shfl(at, leader) + __popcll(mj & below);
-      if (rec && jb == j) {
-        if (at < ph.cap) {
-          const size_t k = (size_t)j * ph.cap + at;
-          float *f = ph.f + k;
-          f[0] = hpos.x; f[ph.total] = hpos.y; f[2 * ph.total] = hpos.z;
-          f[3 * ph.total] = -d.x; f[4 * ph.total] = -d.y; f[5 * ph.total] = -d.z;
-          f[6 * ph.total] = beta.r; f[7 * ph.total] = beta.g; f[8 * ph.total] = beta.b;
-          ++nrec;
-        } else {
-          ++ovf;
-        }
-      }
+      int woff = 0;
+      if (lane == leader) woff = atomicAdd(&rslot[j], __popcll(mj));
+      woff = __shfl(woff, leader);
+      if (rec && jb == j) at = woff + __popcll(mj & below);
       left &= ~mj;
     }
-    int a = block_append(&cnt[(depth + 1) % 3], keep, lds);
-    if (keep) qnext[a] = p;
+    __syncthreads();
+    if (threadIdx.x < kPhotonBatchMax) {  // jb < iterations in the batch <= kPhotonBatchMax
+      const int c = rslot[threadIdx.x];
+      if (c) rslot[threadIdx.x] = atomicAdd(&ph.cnt[threadIdx.x], c);
+    }
+    __syncthreads();
+    if (rec) {
+      at += rslot[jb];
+      if (at < ph.cap) {
+        const size_t k = (size_t)jb * ph.cap + at;
+        float *f = ph.f + k;
+        f[0] = hpos.x; f[ph.total] = hpos.y; f[2 * ph.total] = hpos.z;
+        f[3 * ph.total] = -d.x; f[4 * ph.total] = -d.y; f[5 * ph.total] = -d.z;
+        f[6 * ph.total] = beta.r; f[7 * ph.total] = beta.g; f[8 * ph.total] = beta.b;
+        ++nrec;
+      } else {
+        ++ovf;
+      }
+    }
+    int a = block_append(nxt, keep, lds);
+    if (keep) qnext[q0 + a] = p;
   }
   for (int off = 32; off > 0; off >>= 1) {
     nrec += __shfl_down(nrec, off);
@@ -1055,7 +1083,11 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     ph.cap = (int)cap;
     ph.photons = photons;
   }
-  if ((rc = s->wave.ensure(std::max(npix, B * photons)))) return rc;
+  // photon queue segments of the largest batch (queue entries of segment k
+  // at k * kPhotonSeg, so the queues hold nseg whole segments)
+  const int pseg_max = std::max(1, (B * photons + kPhotonSeg - 1) / kPhotonSeg);
+  if ((rc = b.pq.ensure((size_t)3 * pseg_max * 4))) return rc;
+  if ((rc = s->wave.ensure(std::max(npix, pseg_max * kPhotonSeg)))) return rc;
   Wave w = s->wave.wave;
   w.ctrl = (int *)s->ctrl.ptr;
   w.stats = (unsigned long long *)s->stats.ptr;
@@ -1080,7 +1112,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   // lane) measured 3% faster per iteration than 64, as fast as 256 and 6%
   // faster than 512
   const int chunk = 128;
-  if ((rc = b.spill.ensure(trace_spill_ints(s->d.stack_bound, ring, trace_grid(1, std::max(B * photons, 1), false, chunk)) *
+  if ((rc = b.spill.ensure(trace_spill_ints(s->d.stack_bound, ring, trace_grid(pseg_max, kPhotonSeg, false, chunk)) *
                                4 + 4)))
     return rc;
   // the camera pass traces both its queues in one launch
@@ -1200,10 +1232,12 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   const int nrec = std::min(maxDepth, kSegMaxBounces);
   auto photon_batch = [&](int i0, int nb) -> int {
     const int P = nb * photons;
+    const int nseg = (P + kPhotonSeg - 1) / kPhotonSeg;
+    int *const pq = (int *)b.pq.ptr;
     Wave pw = w;
     pw.N = P;
-    pw.seg = P;
-    pw.nseg = 1;
+    pw.seg = kPhotonSeg;
+    pw.nseg = nseg;
     pw.segmax = (unsigned long long *)b.segmax.ptr;
     // the previous batch's per-bounce photon counts (copied a batch ago, so
     // the event has long completed) become this size's plan
@@ -1216,7 +1250,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
       b.plan_P = b.copied_P;
       b.copied_P = 0;
     }
-    auto plan_cps = [&](int d) -> int {  // chunk positions for bounce d: its count + a quarter + 256
+    auto plan_cps = [&](int d) -> int {  // chunk positions per segment for bounce d: its longest segment + a quarter + 256
       if (b.plan_P != P || d >= (int)b.plan.size()) return 0;
       const int64_t need = (int64_t)b.plan[d] + b.plan[d] / 4 + 256;
       return (int)std::max<int64_t>(1, (need + chunk - 1) / chunk);
@@ -1224,33 +1258,34 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     HIP_TRY(hipMemsetAsync(b.segmax.ptr, 0, (size_t)nrec * kStatStripes * 8, st));
     const uint32_t index0 = (uint32_t)(n_iters * npix + i0 * photons);
     HIP_TRY(hipMemsetAsync(ph.cnt, 0, (size_t)nb * 4, st));
-    HIP_TRY(hipMemsetAsync(w.ctrl + C_PQ, 0, 4, st));
-    hipLaunchKernelGGL(k_photon_init, dim3((P + TB - 1) / TB), dim3(TB), 0, st, s->d, pw, index0, P);
+    HIP_TRY(hipMemsetAsync(pq, 0, (size_t)3 * nseg * 4, st));
+    hipLaunchKernelGGL(k_photon_init, dim3((P + TB - 1) / TB), dim3(TB), 0, st, s->d, pw, index0, P, pq);
     int *qbuf[2] = {w.qray, w.qpath[0]};  // bounce d traces qbuf[d & 1]
-    const int blocks = std::min((P + 255) / 256, 8192);
+    const int full_bps = (std::min(kPhotonSeg, P) + 255) / 256;  // shading blocks per segment of a full grid
     for (int depth = 0; depth < maxDepth; ++depth) {
       Wave tw = pw;
       tw.qray = qbuf[depth & 1];
-      tw.cnt_ray = w.ctrl + C_PQ + depth % 3;
+      tw.cnt_ray = pq + (depth % 3) * nseg;
       int rc2;
       if ((rc2 = ptimer.mark(0))) return rc2;
-      // a planned bounce of at most P / 2 photons traces one photon per lane
-      // (64-entry chunks: its grid stays within the 128-entry full grid the
-      // spill area holds); its latency, not its work, sets the launch's time
+      // a planned bounce whose segments hold at most half a segment traces
+      // one photon per lane (64-entry chunks: its grid stays within the
+      // 128-entry full grid the spill area holds); its latency, not its
+      // work, sets the launch's time
       int ch = chunk, cps = plan_cps(depth);
-      if (cps > 0 && (int64_t)cps * chunk <= P / 2) {
+      if (cps > 0 && (int64_t)cps * chunk <= kPhotonSeg / 2) {
         ch = 64;
         cps = (int)((((int64_t)b.plan[depth] + b.plan[depth] / 4 + 256) + ch - 1) / ch);
       }
       launch_trace_photons(dj, tw, ring, ch, st, cps, depth < nrec ? depth : -1);
       if ((rc2 = ptimer.mark(0)) || (rc2 = ptimer.mark(1))) return rc2;
-      // the bounce's shading grid from the same plan (it loops over its queue
-      // with a grid stride, so any grid shades every photon)
-      const int pblocks = b.plan_P == P && depth < (int)b.plan.size()
-                              ? std::min(blocks, (b.plan[depth] + b.plan[depth] / 4 + 256 + 255) / 256)
-                              : blocks;
-      hipLaunchKernelGGL(k_photon_bounce, dim3(pblocks), dim3(256), 0, st, s->d, tw, ph, index0, depth,
-                         qbuf[depth & 1], qbuf[(depth + 1) & 1], dstats);
+      // the bounce's shading blocks per segment from the same plan (they
+      // stride over their segment, so any count shades every photon)
+      const int bps = b.plan_P == P && depth < (int)b.plan.size()
+                          ? std::max(1, std::min(full_bps, (b.plan[depth] + b.plan[depth] / 4 + 256 + 255) / 256))
+                          : full_bps;
+      hipLaunchKernelGGL(k_photon_bounce, dim3(nseg * bps), dim3(256), 0, st, s->d, tw, ph, index0, depth,
+                         qbuf[depth & 1], qbuf[(depth + 1) & 1], pq, nseg, bps, dstats);
       if ((rc2 = ptimer.mark(1))) return rc2;
     }
     HIP_TRY(hipMemcpyAsync(b.h_segmax, b.segmax.ptr, (size_t)nrec * kStatStripes * 8, hipMemcpyDeviceToHost, st));

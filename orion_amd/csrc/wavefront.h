@@ -224,4 +224,29 @@ __device__ __forceinline__ int block_append(int *counter, bool pred, int *lds) {
   return r;
 }
 
+// Block-aggregated appends to four queues at once: one device atomic per
+// queue and block (counters may repeat: each append gets its own range),
+// three barriers.  slot[k] is the thread's slot in queue k where pk holds.
+// lds: 4 * (waves per block + 1) ints; every thread of the block must call it.
+__device__ __forceinline__ void block_append4(int *c0, int *c1, int *c2, int *c3, bool p0, bool p1, bool p2, bool p3,
+                                              int *lds, int slot[4]) {
+  const int lane = lane_id(), wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const unsigned long long m[4] = {__ballot(p0), __ballot(p1), __ballot(p2), __ballot(p3)};
+  if (lane == 0)
+    for (int k = 0; k < 4; ++k) lds[k * (nw + 1) + wv] = __popcll(m[k]);
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = threadIdx.x;
+    int *const c = k == 0 ? c0 : k == 1 ? c1 : k == 2 ? c2 : c3;
+    int *const l = lds + k * (nw + 1);
+    int tot = 0;
+    for (int v = 0; v < nw; ++v) { const int n = l[v]; l[v] = tot; tot += n; }
+    l[nw] = tot ? atomicAdd(c, tot) : 0;
+  }
+  __syncthreads();
+  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (int k = 0; k < 4; ++k) slot[k] = lds[k * (nw + 1) + nw] + lds[k * (nw + 1) + wv] + __popcll(m[k] & below);
+  __syncthreads();  // lds is free for the next call
+}
+
 }  // namespace orion_amd

@@ -839,6 +839,7 @@ struct orion_server_s {
   orion_amd::WaveRunner runner;
   std::chrono::steady_clock::time_point t0;
   int64_t jobs_done = 0, waves = 0, paths = 0, closest = 0, shadow = 0, mixed_waves = 0;
+  int retiring = 0;  // finished jobs whose films wait in pinned staging for the host copy
   int max_jobs_in_wave = 0;
   int64_t hbm_peak = 0;
   double busy_ms = 0;
@@ -855,12 +856,58 @@ void orion_server_s::loop() {
   const int seg = seg_of(tn, target);
   GridPlan wplan;                // per-bounce longest queue segments of the last wave
   std::vector<int64_t> plan_sig;  // that wave's composition (empty: no plan)
+  // Finished films go to pinned staging on the stream right after their
+  // wave's kernels; the host copies them out (and reports the jobs done)
+  // once the next wave is enqueued, so that copy overlaps GPU work.  Two
+  // staging buffers alternate by wave: wave k's copies land in one while the
+  // host reads wave k - 1's from the other.
+  char *stage[2] = {nullptr, nullptr};
+  size_t stage_cap[2] = {0, 0};
+  int wave_no = 0;
+  struct Retire {
+    ServerJob *j;
+    const char *src;
+  };
+  std::vector<Retire> pending;
+  auto finish_pending = [&] {
+    if (pending.empty()) return;
+    for (const Retire &r : pending) {
+      const size_t np = (size_t)r.j->st.d.width * r.j->st.d.height;
+      r.j->rgb.resize(np * 3);
+      r.j->cnt.resize(np);
+      std::memcpy(r.j->rgb.data(), r.src, np * 12);
+      std::memcpy(r.j->cnt.data(), r.src + np * 12, np * 4);
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (const Retire &r : pending) {
+        r.j->state = 2;
+        r.j->t_done = now();
+        jobs_done++;
+      }
+      retiring -= (int)pending.size();
+    }
+    pending.clear();
+    cv_done.notify_all();
+  };
   for (;;) {
     std::vector<ServerJob *> admitted;
     {
       std::unique_lock<std::mutex> lk(mu);
-      cv.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
-      if (stop && active.empty() && queue.empty()) return;
+      auto ready = [&] { return stop || !queue.empty() || !active.empty(); };
+      if (!ready() && !pending.empty()) {  // idle next: report the last wave's jobs first
+        lk.unlock();
+        finish_pending();
+        lk.lock();
+      }
+      cv.wait(lk, ready);
+      if (stop && active.empty() && queue.empty()) {
+        lk.unlock();
+        finish_pending();
+        for (char *b : stage)
+          if (b) (void)hipHostFree(b);
+        return;
+      }
       // admission against the HBM budget (FIFO; one job is always admitted
       // when nothing runs, so an oversized job cannot stall the queue)
       while (!queue.empty()) {
@@ -948,13 +995,41 @@ void orion_server_s::loop() {
     sig.push_back(wseg);
     const bool graph = tn && tn->use_graph;
     runner.wave_plan = !graph && sig == plan_sig ? &wplan : nullptr;
+    // the jobs this wave finishes, and their films' place in this wave's staging
+    const int sb = wave_no++ & 1;
+    std::vector<size_t> stage_off(act.size(), SIZE_MAX);
+    size_t stage_need = 0;
+    for (size_t i = 0; i < act.size(); ++i)
+      if (act[i]->next_k + taken[i] >= act[i]->d.spp) {
+        stage_off[i] = stage_need;
+        stage_need += (size_t)act[i]->d.width * act[i]->d.height * 16;
+      }
+    int rc = ORION_OK;
+    if (stage_need > stage_cap[sb]) {  // the host finished reading this buffer a wave ago
+      if (stage[sb]) (void)hipHostFree(stage[sb]);
+      stage[sb] = nullptr;
+      stage_cap[sb] = 0;
+      if (hipHostMalloc((void **)&stage[sb], stage_need, hipHostMallocDefault) != hipSuccess) {
+        stage[sb] = nullptr;
+        set_error("pinned staging allocation failed");
+        rc = ORION_ERR_OOM;
+      } else {
+        stage_cap[sb] = stage_need;
+      }
+    }
     auto w0 = std::chrono::steady_clock::now();
-    int rc = runner.init(std::max(1, (int)jobs.size()));
+    if (!rc) rc = runner.init(std::max(1, (int)jobs.size()));
     if (!rc) rc = runner.reset_stats(stream);
     if (!rc && !jobs.empty())
       rc = runner.run(stream, jobs.data(), (int)jobs.size(), npaths, npix, depth, ring_of(tn), stack,
                       jobs.size() > 1, chunk_of(tn), split_of(tn), graph, nullptr, wseg);
     runner.wave_plan = nullptr;
+    for (size_t i = 0; i < act.size() && !rc; ++i)  // film sums and counts are contiguous
+      if (stage_off[i] != SIZE_MAX &&
+          hipMemcpyAsync(stage[sb] + stage_off[i], act[i]->film, (size_t)act[i]->d.width * act[i]->d.height * 16,
+                         hipMemcpyDeviceToHost, stream) != hipSuccess)
+        rc = ORION_ERR_HIP;
+    finish_pending();  // the previous wave's films, while this wave runs
     if (!rc && hipStreamSynchronize(stream) != hipSuccess) rc = ORION_ERR_HIP;
     orion_render_stats ws{};
     if (!rc) rc = runner.read_stats(&ws);
@@ -963,9 +1038,13 @@ void orion_server_s::loop() {
       plan_sig = sig;
     double wms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     std::vector<ServerJob *> finished;
+    std::vector<const char *> finished_src;
     for (size_t i = 0; i < act.size(); ++i) {
       act[i]->next_k += taken[i];
-      if (act[i]->next_k >= act[i]->d.spp) finished.push_back(owners[i]);
+      if (act[i]->next_k >= act[i]->d.spp) {
+        finished.push_back(owners[i]);
+        finished_src.push_back(stage_off[i] != SIZE_MAX ? stage[sb] + stage_off[i] : nullptr);
+      }
     }
     // previews of running jobs asked for since the last wave (the stream is
     // idle here, so the films hold whole sample passes)
@@ -984,13 +1063,7 @@ void orion_server_s::loop() {
                   hipMemcpy(j->pcnt.data(), j->st.count, np * 4, hipMemcpyDeviceToHost) != hipSuccess))
         rc = ORION_ERR_HIP;
     }
-    for (ServerJob *j : finished) {
-      size_t np = (size_t)j->st.d.width * j->st.d.height;
-      j->rgb.resize(np * 3);
-      j->cnt.resize(np);
-      if (!rc && (hipMemcpy(j->rgb.data(), j->st.film, np * 12, hipMemcpyDeviceToHost) != hipSuccess ||
-                  hipMemcpy(j->cnt.data(), j->st.count, np * 4, hipMemcpyDeviceToHost) != hipSuccess))
-        rc = ORION_ERR_HIP;
+    for (ServerJob *j : finished) {  // their films are in staging (the wave has completed)
       j->st.own.release();
       j->st.film = nullptr;
       j->st.count = nullptr;
@@ -1010,12 +1083,18 @@ void orion_server_s::loop() {
         j->peek_req = false;
         j->peek_seq++;
       }
-      for (ServerJob *j : finished) {
-        j->state = rc ? 3 : 2;
-        j->t_done = now();
+      for (size_t k = 0; k < finished.size(); ++k) {
+        ServerJob *j = finished[k];
         in_use -= j->bytes;
-        jobs_done++;
         active.erase(std::find(active.begin(), active.end(), j));
+        if (rc) {
+          j->state = 3;
+          j->t_done = now();
+          jobs_done++;
+        } else {  // reported once the host has copied the film out
+          pending.push_back(Retire{j, finished_src[k]});
+          retiring++;
+        }
       }
     }
     cv_done.notify_all();
@@ -1125,7 +1204,7 @@ int orion_server_peek(orion_server s, uint64_t id, float *rgb, int *count, int *
 int orion_server_drain(orion_server s) {
   if (!s) { set_error("bad argument"); return ORION_ERR_ARG; }
   std::unique_lock<std::mutex> lk(s->mu);
-  s->cv_done.wait(lk, [&] { return s->queue.empty() && s->active.empty(); });
+  s->cv_done.wait(lk, [&] { return s->queue.empty() && s->active.empty() && s->retiring == 0; });
   if (s->error) { set_error(s->error_msg); return s->error; }
   return ORION_OK;
 }

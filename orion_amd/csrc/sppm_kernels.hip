@@ -1296,14 +1296,17 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   if ((rc = camera_batch(0, Bc))) return rc;
   int batch0 = 0, nbatch = 0;
   for (int iter = 0; iter < n_iters; ++iter) {
-    // the next camera batch overlaps this batch's grids, deposits and photons
-    if (iter % Bc == 0 && iter + Bc < n_iters && (rc = camera_batch(iter + Bc, std::min(Bc, n_iters - iter - Bc))))
-      return rc;
+    // a new photon batch first: the main stream is idle here (the host
+    // waited for the last iteration's pair count), and photons do not
+    // depend on any camera pass
     if (photons > 0 && iter >= batch0 + nbatch) {
       batch0 = iter;
       nbatch = std::min(B, n_iters - iter);
       if ((rc = photon_batch(batch0, nbatch))) return rc;
     }
+    // the next camera batch overlaps this batch's grids, deposits and photons
+    if (iter % Bc == 0 && iter + Bc < n_iters && (rc = camera_batch(iter + Bc, std::min(Bc, n_iters - iter - Bc))))
+      return rc;
     HIP_TRY(hipStreamWaitEvent(st, b.cam_done[(iter / Bc) & 1], 0));
     const Pix &px = b.px[iter % nsets];
     // --- grid

@@ -101,6 +101,15 @@ def test_sppm_c3_sorted_pairs_and_cell_ranges(device, oracle):
     np.testing.assert_array_equal(gd["cell_range"], want)
 
 
+def test_sppm_ragged_photon_segments(device, oracle):
+    """Photon counts that are multiples of neither the photon queue segment
+    (16,384 entries) nor a block: 3 iterations of 50,000 photons (one batch
+    of 150,000, its last segment partly filled) and 4 of 3,001 (a batch
+    smaller than one segment, its last block partly filled)."""
+    _compare(device, oracle, [5, 96, 96], 3, 50000, 0.02, 0, 2)
+    _compare(device, oracle, [5, 96, 96], 4, 3001, 0.02, 0, 3)
+
+
 def test_sppm_planned_traversal_grids(device, oracle):
     """Twenty-four iterations of 2^19 photons at 96^2: three photon batches
     and three camera batches of 8 iterations each.  The later photon batches

@@ -343,7 +343,10 @@ int orion_server_set_policy(orion_server s, int policy);
 int orion_server_submit(orion_server s, const orion_job_desc *job, uint64_t *job_id);
 /* done = 1 once finished; then copies the film (either pointer may be NULL)
  * and the request's times in seconds since the server started:
- * times[0] submitted, [1] admitted, [2] finished. */
+ * times[0] submitted, [1] admitted, [2] finished (its film in host memory:
+ * the film of a job that a wave finishes is staged in pinned memory by that
+ * wave and copied out once the next wave is enqueued, or at once when no
+ * work is left). */
 int orion_server_poll(orion_server s, uint64_t job_id, int *done, float *rgb, int *count, double times[3]);
 /* Progressive preview of a job: its film (sum + count) after the samples
  * rendered so far; *samples = how many (every pixel's count).  A running job's

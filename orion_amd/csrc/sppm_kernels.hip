@@ -63,6 +63,7 @@ struct Grid {
   int *hist;        // radix histogram (256 x tiles) -> offsets
   int *scan_tmp;
   int hash_size;
+  int64_t pair_cap = 0;  // entries of keys / vals
 };
 
 // The camera passes of a batch of iterations run on their own stream beside
@@ -95,6 +96,7 @@ struct SppmBuffers {
   // (waves stride over the queue's chunks, so any grid traces every photon).
   DevBuf segmax;
   unsigned long long *h_segmax = nullptr;
+  int *h_npairs = nullptr;  // pinned: the iteration's pair count, copied while emission is queued
   hipEvent_t segmax_copied = nullptr;
   int copied_P = 0;          // photons of the batch whose counts are in flight to h_segmax (0: none)
   int plan_P = 0;            // photons of the batch the plan comes from
@@ -112,6 +114,7 @@ struct SppmBuffers {
     if (cam_stream) { (void)hipStreamSynchronize(cam_stream); (void)hipStreamDestroy(cam_stream); }
     if (segmax_copied) { (void)hipEventSynchronize(segmax_copied); (void)hipEventDestroy(segmax_copied); }
     if (h_segmax) (void)hipHostFree(h_segmax);
+    if (h_npairs) (void)hipHostFree(h_npairs);
     for (int k = 0; k < 2; ++k) {
       if (cam_copied[k]) { (void)hipEventSynchronize(cam_copied[k]); (void)hipEventDestroy(cam_copied[k]); }
       if (h_segmax_cam[k]) (void)hipHostFree(h_segmax_cam[k]);
@@ -417,9 +420,12 @@ __global__ void k_grid_count(Pix px, Grid g, int npix) {
 }
 
 // (key, pixel) pairs in the reference's z, y, x cell order (sppm.cpp:176-193)
+// Skips itself when the iteration's pairs (and the one spare entry the
+// host keeps) exceed the capacity: the host then grows it and launches the
+// emission again.
 __global__ void k_grid_emit(Pix px, Grid g, int npix) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npix) return;
+  if (p >= npix || (int64_t)g.cnt[npix] + 1 > g.pair_cap) return;
   int off = g.cnt[p], end = g.cnt[p + 1];
   if (off == end) return;
   int a[3], b[3];
@@ -983,6 +989,7 @@ static int ensure_pairs(SppmBuffers &b, size_t n) {
     b.g.vals[0] = (int *)(m + 2 * cap);
     b.g.vals[1] = (int *)(m + 3 * cap);
     b.pair_cap = cap;
+    b.g.pair_cap = (int64_t)cap;
   }
   size_t tiles = (n + kSortTile - 1) / kSortTile + 1;
   size_t hn = 256 * tiles;
@@ -1228,6 +1235,7 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
   // (sppm.cpp:203)
   if ((rc = b.segmax.ensure(kSegMaxBytes))) return rc;
   if (!b.h_segmax) HIP_TRY(hipHostMalloc((void **)&b.h_segmax, kSegMaxBytes, hipHostMallocDefault));
+  if (!b.h_npairs) HIP_TRY(hipHostMalloc((void **)&b.h_npairs, sizeof(int), hipHostMallocDefault));
   if (!b.segmax_copied) HIP_TRY(hipEventCreateWithFlags(&b.segmax_copied, hipEventDisableTiming));
   const int nrec = std::min(maxDepth, kSegMaxBounces);
   auto photon_batch = [&](int i0, int nb) -> int {
@@ -1315,12 +1323,30 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     hipLaunchKernelGGL(k_grid_res, dim3(1), dim3(64), 0, st, g);
     hipLaunchKernelGGL(k_grid_count, dim3((npix + 1 + TB - 1) / TB), dim3(TB), 0, st, px, g, npix);
     device_scan(g.cnt, g.cnt, npix + 1, g.scan_tmp, st);
-    int n_pairs = 0;
-    HIP_TRY(hipMemcpyAsync(&n_pairs, g.cnt + npix, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(b.h_npairs, g.cnt + npix, 4, hipMemcpyDeviceToHost, st));
+    // the pair emission (into the current capacity) and the cell clear need
+    // no pair count on the host: queued before the host waits for it
+    const size_t cap0 = b.pair_cap;
+    hipLaunchKernelGGL(k_grid_emit, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, px, g, npix);
+    // per-cell photon counts are kept only for the iteration the debug output
+    // asks for: caustic photons land in few cells, and these atomics then
+    // serialise
+    const bool cell_counts = want_dbg && iter == dbg->debug_iter && dbg->photon_cell_count;
+    {
+      Grid gc = g;
+      if (!cell_counts) gc.photon_cell = nullptr;
+      hipLaunchKernelGGL(k_cell_clear, dim3((hash_size + TB - 1) / TB), dim3(TB), 0, st, gc);
+    }
     HIP_TRY(hipStreamSynchronize(st));
+    const int n_pairs = *b.h_npairs;
     if (n_pairs < 0) { set_error("visible-point cell pairs overflow"); return ORION_ERR_UNSUPPORTED; }
     if ((rc = ensure_pairs(b, (size_t)n_pairs + 1))) return rc;
-    hipLaunchKernelGGL(k_grid_emit, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, px, g, npix);
+    if ((size_t)n_pairs + 1 > cap0)  // the queued emission skipped itself
+      hipLaunchKernelGGL(k_grid_emit, dim3((npix + TB - 1) / TB), dim3(TB), 0, st, px, g, npix);
+    // the deposits' grid: taken after ensure_pairs, which may have moved the
+    // pairs and histograms
+    Grid gi = g;
+    if (!cell_counts) gi.photon_cell = nullptr;
     int bits = 1;
     while ((1 << bits) < hash_size) ++bits;
     int ntiles = (n_pairs + kSortTile - 1) / kSortTile;
@@ -1339,12 +1365,6 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
       HIP_TRY(hipMemcpyAsync(g.keys[0], g.keys[1], (size_t)n_pairs * 4, hipMemcpyDeviceToDevice, st));
       HIP_TRY(hipMemcpyAsync(g.vals[0], g.vals[1], (size_t)n_pairs * 4, hipMemcpyDeviceToDevice, st));
     }
-    // per-cell photon counts are kept only for the iteration the debug output
-    // asks for: caustic photons land in few cells, and these atomics then
-    // serialise
-    Grid gi = g;
-    if (!(want_dbg && iter == dbg->debug_iter && dbg->photon_cell_count)) gi.photon_cell = nullptr;
-    hipLaunchKernelGGL(k_cell_clear, dim3((hash_size + TB - 1) / TB), dim3(TB), 0, st, gi);
     if (n_pairs > 0)
       hipLaunchKernelGGL(k_cell_ranges, dim3((n_pairs + TB - 1) / TB), dim3(TB), 0, st, g, g.keys[0], n_pairs);
     if (want_dbg && iter == dbg->debug_iter) {

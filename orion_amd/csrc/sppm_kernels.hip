@@ -60,6 +60,7 @@ struct Grid {
   uint32_t *keys[2];
   int *vals[2];
   int *cell_start, *cell_end, *photon_cell;
+  int *part;        // the bounds kernel's per-block partials (8 ints per block, kBoundsBlocks blocks)
   int *hist;        // radix histogram (256 x tiles) -> offsets
   int *scan_tmp;
   int hash_size;
@@ -344,6 +345,7 @@ __global__ void k_grid_init(Grid g) {
 }
 
 // bounds = union of expand(vp.p, r) over visible points (sppm.cpp:149-157)
+constexpr int kBoundsBlocks = 256;
 __global__ __launch_bounds__(256) void k_grid_bounds(Pix px, Grid g, int npix) {
   __shared__ int red[4][8];  // per wave of the 256-thread block
   int lo[3], hi[3], mr = 0, any = 0;
@@ -365,9 +367,9 @@ __global__ __launch_bounds__(256) void k_grid_bounds(Pix px, Grid g, int npix) {
       int o = __shfl_down(vals[k], off);
       vals[k] = k < 3 ? min(vals[k], o) : max(vals[k], o);
     }
-  // block reduction through LDS, then one set of device atomics per block:
-  // one set per wave (a 1024-block grid) serialised on these 8 words and
-  // took ~0.3 ms per iteration
+  // block reduction through LDS, then the block's partial to g.part, which
+  // k_grid_res reduces: device atomics on these 8 words serialise (one set
+  // per block of a 256-block grid took ~18 us, one per wave ~0.3 ms)
   const int wv = threadIdx.x >> 6;
   if (lane_id() == 0)
     for (int k = 0; k < 8; ++k) red[wv][k] = vals[k];
@@ -375,15 +377,29 @@ __global__ __launch_bounds__(256) void k_grid_bounds(Pix px, Grid g, int npix) {
   if (threadIdx.x != 0) return;
   for (int v = 1; v < (int)(blockDim.x >> 6); ++v)
     for (int k = 0; k < 8; ++k) vals[k] = k < 3 ? min(vals[k], red[v][k]) : max(vals[k], red[v][k]);
-  if (vals[7] == 0) return;
-  for (int a = 0; a < 3; ++a) { atomicMin(&g.ictl[4 + a], vals[a]); atomicMax(&g.ictl[7 + a], vals[3 + a]); }
-  atomicMax(&g.ictl[10], vals[6]);
-  atomicMax(&g.ictl[3], 1);
+  for (int k = 0; k < 8; ++k) g.part[blockIdx.x * 8 + k] = vals[k];
 }
 
-// grid resolution (sppm.cpp:159-163)
-__global__ void k_grid_res(Grid g) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// grid resolution (sppm.cpp:159-163), after the bounds of the nb partials
+// of k_grid_bounds; one wave64
+__global__ void k_grid_res(Grid g, int nb) {
+  const int lane = threadIdx.x;
+  int v[8] = {0x7f7fffff, 0x7f7fffff, 0x7f7fffff, f2key(-3.40282347e+38f), f2key(-3.40282347e+38f),
+              f2key(-3.40282347e+38f), 0, 0};
+  for (int b = lane; b < nb; b += 64)
+    for (int k = 0; k < 8; ++k) {
+      const int o = g.part[b * 8 + k];
+      v[k] = k < 3 ? min(v[k], o) : max(v[k], o);
+    }
+  for (int off = 32; off > 0; off >>= 1)
+    for (int k = 0; k < 8; ++k) {
+      const int o = __shfl_down(v[k], off);
+      v[k] = k < 3 ? min(v[k], o) : max(v[k], o);
+    }
+  if (lane != 0) return;
+  for (int a = 0; a < 3; ++a) { g.ictl[4 + a] = v[a]; g.ictl[7 + a] = v[3 + a]; }
+  g.ictl[10] = v[6];
+  g.ictl[3] = v[7];
   if (!g.ictl[3]) { g.ictl[0] = g.ictl[1] = g.ictl[2] = 1; return; }
   float lo[3], hi[3];
   for (int a = 0; a < 3; ++a) { lo[a] = key2f(g.ictl[4 + a]); hi[a] = key2f(g.ictl[7 + a]); }
@@ -954,7 +970,7 @@ static int ensure_buffers(orion_scene s, int hash_size) {
     HIP_TRY(hipEventCreateWithFlags(&b.iter_done, hipEventDisableTiming));
   }
   if (b.hash_size != hash_size || !b.grid_mem.ptr) {
-    size_t ints = 64 + (size_t)(npix + 1) + 3 * (size_t)hash_size;
+    size_t ints = 64 + (size_t)(npix + 1) + 3 * (size_t)hash_size + 8 * (size_t)kBoundsBlocks;
     int rc = b.grid_mem.ensure(ints * 4 + 256);
     if (rc) return rc;
     int *m = (int *)b.grid_mem.ptr;
@@ -965,6 +981,7 @@ static int ensure_buffers(orion_scene s, int hash_size) {
     g.cell_start = g.cnt + npix + 1;
     g.cell_end = g.cell_start + hash_size;
     g.photon_cell = g.cell_end + hash_size;
+    g.part = g.photon_cell + hash_size;
     g.hash_size = hash_size;
     b.hash_size = hash_size;
   }
@@ -1319,8 +1336,9 @@ extern "C" int orion_render_sppm(orion_scene s, int n_iters, int photons, float 
     const Pix &px = b.px[iter % nsets];
     // --- grid
     hipLaunchKernelGGL(k_grid_init, dim3(1), dim3(64), 0, st, g);
-    hipLaunchKernelGGL(k_grid_bounds, dim3(std::min((npix + 255) / 256, 256)), dim3(256), 0, st, px, g, npix);
-    hipLaunchKernelGGL(k_grid_res, dim3(1), dim3(64), 0, st, g);
+    const int bblocks = std::max(1, std::min((npix + 255) / 256, kBoundsBlocks));
+    hipLaunchKernelGGL(k_grid_bounds, dim3(bblocks), dim3(256), 0, st, px, g, npix);
+    hipLaunchKernelGGL(k_grid_res, dim3(1), dim3(64), 0, st, g, bblocks);
     hipLaunchKernelGGL(k_grid_count, dim3((npix + 1 + TB - 1) / TB), dim3(TB), 0, st, px, g, npix);
     device_scan(g.cnt, g.cnt, npix + 1, g.scan_tmp, st);
     HIP_TRY(hipMemcpyAsync(b.h_npairs, g.cnt + npix, 4, hipMemcpyDeviceToHost, st));
